@@ -1,0 +1,275 @@
+/*
+ * ymrt.h -- C ABI of the MI355X (gfx950) wavefront path tracer.
+ *
+ * Plain C: POD structs, opaque handles, int status codes, device pointers and
+ * a hipStream_t passed as void*.  No C++ or GPU-library types cross this
+ * boundary, and no exception leaves it.  On failure every entry point returns a
+ * non-zero YMRT_ERR_* code and ymrt_last_error() holds a message for the
+ * calling thread.
+ *
+ * Reference interfaces replaced (paths relative to the YMRT source tree):
+ *   - Scene upload: SceneModule::GetHostSceneDataPointer
+ *     (src/core/SceneModule.cpp:1434-1463) and the host steps of
+ *     GuiModule::UpdateScene (src/core/GuiModule.cpp:116-271): top BVH build,
+ *     shape-light CDF (SceneModule::LoadShapeLight, SceneModule.cpp:1203-1304),
+ *     Halton permutation table (SceneModule.cpp:1412-1432), camera medium
+ *     (Camera::InitCameraRayTransfer, src/Camera.cpp:15-78), UPLOAD_TO_GPU.
+ *   - Ray queries: TraceRay (src/ShadingUtilities.cuh:310-431) and
+ *     BVHTraverseShadow (src/ShadingUtilities.cuh:563-635).
+ *   - Frame: the extern "C" stage launchers RayGenerationPathTracing
+ *     (src/Shading.cu:1727), RayTracePathTracing (1903), RayShadingPathTracing
+ *     (2304) driven by RenderModule::PathTracingExecuteTask
+ *     (src/core/RenderModule.cpp:494-653); AccumulateImagePathTracing (2375);
+ *     PostProcessingPathTracing (2502).
+ */
+#ifndef YMRT_H
+#define YMRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YMRT_ABI_VERSION 1
+
+/* status codes */
+#define YMRT_OK 0
+#define YMRT_ERR_INVALID_ARGUMENT 1
+#define YMRT_ERR_DEVICE 2
+#define YMRT_ERR_OUT_OF_MEMORY 3
+#define YMRT_ERR_UNSUPPORTED 4
+#define YMRT_ERR_INTERNAL 5
+
+#define YMRT_INVALID_INDEX 0xFFFFFFFFu
+
+/* ---------------------------------------------------------------- scene data
+ * These descriptors carry the same data as YMRT's host arrays: a mesh is the
+ * TriangleMesh blob of src/GeometryDefines.h:197-273 (per-corner position,
+ * normal and texcoord indices), an instance is PrimitiveInstance
+ * (src/PrimtiveInstance.h:9-25) with its transform pair (src/SceneDefines.h:84-86),
+ * a material is Material/DefaultMtl/LightMtl (src/Material.h:21-359), a volume
+ * is Volume (src/Volume.h:9-18).  Matrices are column-major 4x4 like glm::mat4.
+ */
+
+typedef struct ymrt_mesh_desc {
+    uint32_t triangle_count;
+    uint32_t position_count;         /* number of float3 positions           */
+    const uint32_t *position_indices; /* 3 * triangle_count                   */
+    const float *positions;          /* 3 * position_count                    */
+    uint32_t normal_count;
+    const uint32_t *normal_indices;  /* 3 * triangle_count, or NULL with normals NULL */
+    const float *normals;            /* 3 * normal_count                      */
+    uint32_t texcoord_count;
+    const uint32_t *texcoord_indices; /* 3 * triangle_count, or NULL          */
+    const float *texcoords;          /* 2 * texcoord_count                    */
+} ymrt_mesh_desc;
+
+typedef struct ymrt_instance_desc {
+    uint32_t mesh;                   /* index into meshes                     */
+    uint32_t material;               /* index into materials                  */
+    int32_t inner_volume;            /* index into volumes or -1              */
+    int32_t treat_as_boundary;       /* 1: medium boundary, skipped by shading */
+    float object_to_world[16];
+    float world_to_object[16];
+} ymrt_instance_desc;
+
+#define YMRT_MATERIAL_DEFAULT 0u
+#define YMRT_MATERIAL_LIGHT 1u
+
+typedef struct ymrt_material_desc {
+    uint32_t type;
+    /* DefaultMtl (roughness values are squared into GGX alpha at shading) */
+    float diffuse_albedo[3];   uint32_t diffuse_albedo_tex;
+    float specular_albedo[3];  uint32_t specular_albedo_tex;
+    float roughness_x;         uint32_t roughness_x_tex;
+    float roughness_y;         uint32_t roughness_y_tex;
+    float specular_weight;     uint32_t specular_weight_tex;
+    float metalness;           uint32_t metalness_tex;
+    float transmission_weight; uint32_t transmission_weight_tex;
+    float ior_n;               uint32_t ior_priority;
+    float coat_albedo[3];      uint32_t coat_albedo_tex;
+    float coat_weight;         uint32_t coat_weight_tex;
+    float coat_thickness;      uint32_t coat_thickness_tex;
+    float coat_roughness_x;    uint32_t coat_roughness_x_tex;
+    float coat_roughness_y;    uint32_t coat_roughness_y_tex;
+    float coat_ior;
+    /* LightMtl */
+    float light_color[3];
+    float intensity;
+} ymrt_material_desc;
+
+#define YMRT_WRAP_REPEAT 0
+#define YMRT_WRAP_CLAMP 1
+#define YMRT_COLOR_SRGB 0
+#define YMRT_COLOR_LINEAR 1
+
+/* Image texture: 8-bit texels, decoded like LoadPng8 (src/FileIO.cpp:17-45):
+ * value/255 then sRGB->linear when color_space == YMRT_COLOR_SRGB.  Row 0 is
+ * the bottom row (the loader flips vertically).  channels is 1, 3 or 4. */
+typedef struct ymrt_texture_desc {
+    uint32_t width, height, channels;
+    const uint8_t *texels;           /* width * height * channels             */
+    int32_t wrap_mode;
+    int32_t color_space;
+    float scale_u, scale_v, offset_u, offset_v;
+} ymrt_texture_desc;
+
+/* Density grid for heterogeneous media (no counterpart in YMRT, whose
+ * heterogeneous density comes only from procedural textures).  Voxel-constant
+ * density over the unit cube [0,1]^3 of the volume's object space. */
+typedef struct ymrt_grid_desc {
+    uint32_t nx, ny, nz;
+    const float *density;            /* nx*ny*nz, x fastest                   */
+} ymrt_grid_desc;
+
+typedef struct ymrt_volume_desc {
+    float sigma_t[3];
+    float albedo[3];
+    float g;                         /* Henyey-Greenstein asymmetry           */
+    int32_t density_grid;            /* -1: homogeneous                       */
+    float world_to_object[16];       /* maps world to the grid's unit cube    */
+} ymrt_volume_desc;
+
+typedef struct ymrt_camera_desc {
+    float position[3];
+    float w[3];                      /* backward axis, Camera::SetDir         */
+    float fov;                       /* vertical field of view, radians       */
+    float aspect_ratio;
+} ymrt_camera_desc;
+
+typedef struct ymrt_scene_desc {
+    uint32_t mesh_count;       const ymrt_mesh_desc *meshes;
+    uint32_t instance_count;   const ymrt_instance_desc *instances;
+    uint32_t material_count;   const ymrt_material_desc *materials;
+    uint32_t texture_count;    const ymrt_texture_desc *textures;
+    uint32_t grid_count;       const ymrt_grid_desc *grids;
+    uint32_t volume_count;     const ymrt_volume_desc *volumes;
+    ymrt_camera_desc camera;
+    uint32_t halton_seed;      /* seed of the Halton digit permutations     */
+} ymrt_scene_desc;
+
+/* ---------------------------------------------------------------- ray queries */
+
+typedef struct ymrt_ray {          /* 32 B */
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+} ymrt_ray;
+
+typedef struct ymrt_hit {          /* 16 B */
+    float t;                         /* hit distance along d (d as given)     */
+    uint32_t triangle;               /* index in the mesh's triangle list, or YMRT_INVALID_INDEX */
+    uint32_t instance;               /* index in the instance list            */
+    uint32_t back;                   /* 1 when d points along the face normal */
+} ymrt_hit;
+
+/* ---------------------------------------------------------------- rendering */
+
+#define YMRT_SAMPLER_PCG 0
+#define YMRT_SAMPLER_HALTON 1
+
+/* Mirrors RenderSetting (src/SceneDefines.h:17-40). */
+typedef struct ymrt_render_settings {
+    uint32_t width, height;
+    int32_t spp;
+    int32_t ray_depth;
+    int32_t sampler_type;
+    int32_t enable_shape_light;
+    int32_t enable_volume_scattering;
+    int32_t enable_env_light;
+    int32_t enable_russian_roulette;
+    int32_t enable_indirect_clamp;
+    float indirect_limit;
+    float gamma;
+    float exposure;
+    uint32_t path_capacity;          /* paths in flight; 0 = library default  */
+} ymrt_render_settings;
+
+typedef struct ymrt_frame_stats {
+    uint64_t camera_rays;            /* extension segments with depth 1       */
+    uint64_t extension_rays;         /* all closest-hit segments              */
+    uint64_t shadow_rays;            /* any-hit segments                      */
+    uint64_t transmittance_rays;     /* boundary segments of shadow transmittance */
+    uint64_t paths;
+    uint32_t passes;                 /* bounce passes launched                */
+    uint32_t chunks;
+    /* Per-stage GPU time measured with HIP events on the render stream (ms,
+     * summed over launches) and launch counts; filled when timing != 0. */
+    float ms_raygen, ms_extend, ms_shade, ms_shadow, ms_resolve;
+    uint32_t launches_extend, launches_shade, launches_shadow;
+    float ms_total;
+} ymrt_frame_stats;
+
+/* ---------------------------------------------------------------- entry points */
+
+typedef struct ymrt_context ymrt_context;
+typedef struct ymrt_scene ymrt_scene;
+
+const char *ymrt_last_error(void);
+int ymrt_abi_version(void);
+
+int ymrt_context_create(int device, ymrt_context **out);
+int ymrt_context_destroy(ymrt_context *ctx);
+
+/* Builds the two-level wide BVH on the host and uploads every array. */
+int ymrt_scene_create(ymrt_context *ctx, const ymrt_scene_desc *desc, ymrt_scene **out);
+int ymrt_scene_destroy(ymrt_scene *scene);
+
+/* Scene statistics: bvh node count, max stack depth, bytes resident in HBM. */
+typedef struct ymrt_scene_info {
+    uint64_t triangle_count;
+    uint64_t node_count;             /* wide nodes, bottom + top              */
+    uint64_t device_bytes;
+    uint32_t max_stack;              /* stack entries a traversal may need    */
+    uint32_t shape_light_count;
+    float build_ms;
+    uint32_t camera_medium_records;
+} ymrt_scene_info;
+int ymrt_scene_get_info(const ymrt_scene *scene, ymrt_scene_info *info);
+
+/* Closest hit (TraceRay) for n device-resident rays. */
+int ymrt_trace_closest(ymrt_scene *scene, const ymrt_ray *rays, uint32_t n,
+                       ymrt_hit *hits, void *stream);
+/* Any hit (BVHTraverseShadow): occluded[i] = 1 when some non-boundary
+ * triangle lies in [tmin, tmax]. */
+int ymrt_trace_any(ymrt_scene *scene, const ymrt_ray *rays, uint32_t n,
+                   uint32_t *occluded, void *stream);
+/* Same queries, also counting visited wide nodes and tested triangles per ray
+ * (visits[2*i], visits[2*i+1]); used to count the bytes a traversal touches. */
+int ymrt_trace_closest_counted(ymrt_scene *scene, const ymrt_ray *rays, uint32_t n,
+                               ymrt_hit *hits, uint32_t *visits, void *stream);
+
+/* One progressive frame: every pixel gets spp samples of path tracing;
+ * noise_rgba (width*height float4, device) receives the per-frame estimate
+ * (radiance in rgb, 1 in alpha), as the reference's noise image does.
+ * frame_index starts at 1.  stats may be NULL; timing != 0 records per-stage
+ * HIP events (adds a synchronisation at the end of the frame). */
+int ymrt_render_frame(ymrt_scene *scene, const ymrt_render_settings *settings,
+                      int frame_index, float *noise_rgba, void *stream,
+                      int timing, ymrt_frame_stats *stats);
+
+/* Running mean over frames (Accumulate_image_path_tracing). */
+int ymrt_accumulate(const float *noise_rgba, float *accum_rgba, uint32_t width,
+                    uint32_t height, int frame_index, int max_frames, void *stream);
+/* Exposure, ACES fit and gamma in place (Gamma_correction_common). */
+int ymrt_tonemap(float *rgba, uint32_t width, uint32_t height, float gamma,
+                 float exposure, void *stream);
+
+/* Halton sample values for testing the sampler bit for bit:
+ * out[i*dims + d] = value of dimension (dim0 + d) of the sample with global
+ * Halton index indices[i] (dims 0 and 1 are the unpermuted base-2/base-3
+ * pixel-offset radical inverses).  Device pointers. */
+int ymrt_halton_values(ymrt_scene *scene, const uint64_t *indices, uint32_t n,
+                       uint32_t dim0, uint32_t dims, float *out, void *stream);
+/* Global Halton index of (px, py, sample) for an image of width x height
+ * (HaltonEnumerator::GetIndex), computed on the device. */
+int ymrt_halton_enumerate(uint32_t width, uint32_t height, const uint32_t *px,
+                          const uint32_t *py, const uint32_t *sample, uint32_t n,
+                          uint64_t *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* YMRT_H */

@@ -1,0 +1,233 @@
+"""Python access to the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
+and by bench.py's CPU-baseline leg, always as the checker.  The product
+(ymrt_amd) never imports this package.
+"""
+import ctypes as C
+import os
+import subprocess
+import threading
+
+import numpy as np
+
+from ymrt_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+_lock = threading.Lock()
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(_LIB_PATH):
+                build()
+            L = C.CDLL(_LIB_PATH)
+            vp = C.c_void_p
+            L.or_scene_create.restype = vp
+            L.or_scene_create.argtypes = [C.POINTER(abi.SceneDesc)]
+            L.or_scene_destroy.argtypes = [vp]
+            L.or_trace_closest_batch.argtypes = [vp, vp, C.c_uint32, vp]
+            L.or_trace_any_batch.argtypes = [vp, vp, C.c_uint32, vp]
+            L.or_brute_closest_batch.argtypes = [vp, vp, C.c_uint32, vp]
+            L.or_render_rows.argtypes = [vp, C.POINTER(abi.RenderSettings), C.c_int, C.c_uint32,
+                                         C.c_uint32, vp, vp]
+            L.or_halton_table_size.restype = C.c_uint32
+            L.or_halton_table.argtypes = [C.c_uint32, vp]
+            L.or_radical_inverse.restype = C.c_float
+            L.or_radical_inverse.argtypes = [C.c_uint64, C.c_uint32, vp]
+            L.or_halton_value.restype = C.c_float
+            L.or_halton_value.argtypes = [C.c_uint64, C.c_uint32, vp]
+            L.or_radical_inverse2.restype = C.c_float
+            L.or_radical_inverse2.argtypes = [C.c_uint64]
+            L.or_radical_inverse3.restype = C.c_float
+            L.or_radical_inverse3.argtypes = [C.c_uint64]
+            L.or_prime.restype = C.c_uint32
+            L.or_prime.argtypes = [C.c_uint32]
+            L.or_prime_sum.restype = C.c_uint32
+            L.or_prime_sum.argtypes = [C.c_uint32]
+            L.or_mt_seed.argtypes = [vp, C.c_uint32]
+            L.or_mt_next.restype = C.c_uint32
+            L.or_mt_next.argtypes = [vp]
+            L.or_enum_init.argtypes = [vp, C.c_int, C.c_int]
+            L.or_enum_index.restype = C.c_uint64
+            L.or_enum_index.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+            L.or_pcg4d.argtypes = [vp]
+            L.or_bsdf_eval.restype = C.c_int
+            L.or_bsdf_eval.argtypes = [C.POINTER(abi.MaterialDesc), V3, V3, C.c_float, C.c_float,
+                                       C.c_int, vp, vp]
+            L.or_bsdf_sample.restype = C.c_int
+            L.or_bsdf_sample.argtypes = [C.POINTER(abi.MaterialDesc), V3, C.c_float, C.c_float, C.c_int,
+                                         C.c_float, C.c_float, C.c_float, vp, vp, vp]
+            L.or_hg_eval.restype = C.c_float
+            L.or_hg_eval.argtypes = [C.c_float, V3, V3, vp]
+            L.or_hg_sample.restype = C.c_int
+            L.or_hg_sample.argtypes = [C.c_float, C.c_float, C.c_float, V3, vp, vp]
+            L.or_srgb_to_linear.restype = C.c_float
+            L.or_srgb_to_linear.argtypes = [C.c_float]
+            L.or_texture_eval.restype = V3
+            L.or_texture_eval.argtypes = [vp, C.c_uint32, C.c_float, C.c_float]
+            L.or_accumulate.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_int]
+            L.or_tonemap.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_float, C.c_float]
+            _lib = L
+    return _lib
+
+
+class V3(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
+def v3(a):
+    return V3(float(a[0]), float(a[1]), float(a[2]))
+
+
+class MT(C.Structure):
+    _fields_ = [("s", C.c_uint32 * 624), ("i", C.c_int)]
+
+
+class Enumerator(C.Structure):
+    _fields_ = [("x_scale", C.c_float), ("y_scale", C.c_float), ("m_x", C.c_int), ("m_y", C.c_int),
+                ("exp2", C.c_int), ("exp3", C.c_int), ("increment", C.c_int)]
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Scene:
+    """Oracle scene built from the same SceneArrays the product consumes."""
+
+    def __init__(self, arrays):
+        self.arrays = arrays
+        self._desc = arrays.desc()
+        self.h = lib().or_scene_create(C.byref(self._desc))
+
+    def close(self):
+        if self.h:
+            lib().or_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def trace_closest(self, rays, brute=False):
+        rays = np.ascontiguousarray(rays)
+        hits = np.zeros(len(rays), abi.HIT_DTYPE)
+        fn = lib().or_brute_closest_batch if brute else lib().or_trace_closest_batch
+        fn(self.h, _p(rays), len(rays), _p(hits))
+        return hits
+
+    def trace_any(self, rays):
+        rays = np.ascontiguousarray(rays)
+        occ = np.zeros(len(rays), np.uint32)
+        lib().or_trace_any_batch(self.h, _p(rays), len(rays), _p(occ))
+        return occ
+
+    def render(self, settings, frame=1, rows=None, threads=1):
+        """Noise image (H, W, 4) float32 of one frame; ``rows`` limits the rows
+        rendered (others stay zero).  ``threads`` > 1 splits rows over Python
+        threads (ctypes releases the GIL)."""
+        h, w = settings.height, settings.width
+        out = np.zeros((h, w, 4), np.float32)
+        r0, r1 = (0, h) if rows is None else rows
+        counts = np.zeros((max(1, threads), 8), np.uint64)
+
+        def work(k, a, b):
+            lib().or_render_rows(self.h, C.byref(settings), frame, a, b, _p(out), _p(counts[k]))
+
+        if threads <= 1:
+            work(0, r0, r1)
+        else:
+            bounds = np.linspace(r0, r1, threads + 1).astype(int)
+            ts = [threading.Thread(target=work, args=(k, bounds[k], bounds[k + 1])) for k in range(threads)]
+            for t in ts:
+                t.start()
+            for t in ts:
+                t.join()
+        self.last_counts = counts.sum(axis=0)
+        return out
+
+    def texture_eval(self, tex, u, v):
+        r = lib().or_texture_eval(self.h, tex, u, v)
+        return np.array([r.x, r.y, r.z], np.float32)
+
+
+def halton_table(seed):
+    n = lib().or_halton_table_size()
+    t = np.zeros(n, np.uint32)
+    lib().or_halton_table(seed, _p(t))
+    return t
+
+
+def halton_value(index, dim, table):
+    return lib().or_halton_value(index, dim, _p(table))
+
+
+def enum_index(width, height, px, py, sample):
+    e = Enumerator()
+    lib().or_enum_init(C.byref(e), width, height)
+    return lib().or_enum_index(C.byref(e), px, py, sample), e
+
+
+def mt19937(seed, n):
+    m = MT()
+    lib().or_mt_seed(C.byref(m), seed)
+    return np.array([lib().or_mt_next(C.byref(m)) for _ in range(n)], np.uint64)
+
+
+def pcg4d(v):
+    a = np.ascontiguousarray(v, np.uint32).copy()
+    lib().or_pcg4d(_p(a))
+    return a
+
+
+def bsdf_eval(mat, wo, wi, ray_ior=1.0, ex_ior=1.0, back=0):
+    f = np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    lib().or_bsdf_eval(C.byref(mat), v3(wo), v3(wi), ray_ior, ex_ior, back, _p(f), _p(pdf))
+    return f, float(pdf[0])
+
+
+def bsdf_sample(mat, wo, u, ray_ior=1.0, ex_ior=1.0, back=0):
+    w = np.zeros(3, np.float32)
+    wi = np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    ok = lib().or_bsdf_sample(C.byref(mat), v3(wo), ray_ior, ex_ior, back, float(u[0]), float(u[1]),
+                              float(u[2]), _p(w), _p(wi), _p(pdf))
+    return bool(ok), w, wi, float(pdf[0])
+
+
+def hg_eval(g, wo, wi):
+    pdf = np.zeros(1, np.float32)
+    return lib().or_hg_eval(g, v3(wo), v3(wi), _p(pdf))
+
+
+def hg_sample(g, u0, u1, wo):
+    wi = np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    lib().or_hg_sample(g, u0, u1, v3(wo), _p(wi), _p(pdf))
+    return wi, float(pdf[0])
+
+
+def srgb_to_linear(s):
+    return lib().or_srgb_to_linear(s)
+
+
+def accumulate(noise, accum, frame):
+    h, w, _ = noise.shape
+    lib().or_accumulate(_p(noise), _p(accum), w, h, frame)
+
+
+def tonemap(img, gamma, exposure):
+    h, w, _ = img.shape
+    lib().or_tonemap(_p(img), w, h, gamma, exposure)

@@ -1,0 +1,1624 @@
+/*
+ * or_shade.c -- oracle restatement of YMRT's shading and path loop.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker for the HIP path tracer; never shipped).
+ *
+ *   - Camera basis / GenerateRay: src/Camera.h:164-170, 311-322
+ *   - InitCameraRayTransfer: src/Camera.cpp:15-78
+ *   - ray generation (pixel/sample -> Halton or PCG sampler, jitter):
+ *     src/Shading.cu:1634-1725
+ *   - trace stage with volume free flight, miss -> background:
+ *     src/Shading.cu:1765-1901, Background 35-38
+ *   - shading stage: src/Shading.cu:1953-2302
+ *   - NEE: EvalShapeLight 294-417, EvalVolumeShapeLight 419-513, PowerHeuristic
+ *     40-43, indirect_light_clamp 61-81
+ *   - FetchMeshShadingData: src/ShadingUtilities.cuh:713-875
+ *   - BoundaryTransitionBunch: src/ShadingUtilities.cuh:954-990, RayTransfer
+ *     src/Ray.h:106-260
+ *   - BSDF: src/BSDF.cuh:20-899 (lobes, DefaultMtlBSDF, MaterialBSDF without
+ *     normal or bump maps)
+ *   - lights: src/Light.h:14-40 (RayIntersectTriangle), 87-321 (ShapeLight
+ *     mesh branch), 398-416 (SampleLightPower); CDF
+ *     src/core/SceneModule.cpp:1203-1304
+ *   - sampling helpers: src/SampleUtilities.h:11-48
+ *   - volumes: src/VolumeEval.cuh:18-538, src/Volume.h:21-48
+ *   - image textures: src/TextureEval.cuh:162-237, decode src/FileIO.cpp:17-45,
+ *     src/MathCommon.h:136-139
+ *   - accumulate / tonemap: src/Shading.cu:2354-2425
+ *
+ * Two documented departures, shared with the HIP path:
+ *   - heterogeneous media use a voxel grid (the reference has none); the
+ *     shadow-ray transmittance through such a grid draws its random numbers
+ *     from a PCG hash of (pixel, sample, frame, vertex, ray, step) instead of
+ *     the path's Halton stream;
+ *   - the shape-light list follows input instance order (the reference's
+ *     follows its BVH-reordered instance order).
+ */
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include "oracle.h"
+
+/* ------------------------------------------------------------------ setup */
+float or_srgb_to_linear(float s)
+{
+    return s > 0.04045 ? (float)pow(((double)s + 0.055) / 1.055, (double)2.4f) : (s / 12.92f);
+}
+
+static void camera_basis(or_scene *sc)
+{
+    v3 w = mk3(sc->camera.w[0], sc->camera.w[1], sc->camera.w[2]);
+    w = scl3(w, safe_rcp(length3(w)));
+    v3 up = mk3(0.0f, 1.0f, 0.0f);
+    double mu[3], mv[3];
+    mu[0] = (double)up.y * (double)w.z - (double)up.z * (double)w.y;
+    mu[1] = -((double)up.x * (double)w.z - (double)up.z * (double)w.x);
+    mu[2] = (double)up.x * (double)w.y - (double)up.y * (double)w.x;
+    v3 u = scl3(mk3((float)mu[0], (float)mu[1], (float)mu[2]),
+                safe_rcp((float)sqrt(mu[0] * mu[0] + mu[1] * mu[1] + mu[2] * mu[2])));
+    mv[0] = (double)w.y * (double)u.z - (double)w.z * (double)u.y;
+    mv[1] = -((double)w.x * (double)u.z - (double)w.z * (double)u.x);
+    mv[2] = (double)w.x * (double)u.y - (double)w.y * (double)u.x;
+    v3 v = scl3(mk3((float)mv[0], (float)mv[1], (float)mv[2]),
+                safe_rcp((float)sqrt(mv[0] * mv[0] + mv[1] * mv[1] + mv[2] * mv[2])));
+    sc->cam_u = u; sc->cam_v = v; sc->cam_w = w;
+    sc->tan_half_fov = tanf(0.5f * sc->camera.fov);
+}
+
+static or_ray make_ray(v3 o, v3 d, uint32_t li, uint32_t lt)
+{
+    or_ray r;
+    r.o = o; r.d = normalize3(d); r.t = OR_TMAX; r.last_inst = li; r.last_tri = lt;
+    return r;
+}
+
+static or_ray camera_ray(const or_scene *sc, float nx, float ny)
+{
+    float height = 2.0f * sc->tan_half_fov;
+    float width = height * sc->camera.aspect_ratio;
+    v3 dir = add3(add3(neg3(sc->cam_w), sclf3((nx - 0.5f) * width, sc->cam_u)),
+                  sclf3((ny - 0.5f) * height, sc->cam_v));
+    return make_ray(mk3(sc->camera.position[0], sc->camera.position[1], sc->camera.position[2]), dir,
+                    OR_EMPTY, OR_EMPTY);
+}
+
+/* --------------------------------------------------------------- transfer */
+static float fetch_ior(const ymrt_material_desc *m, uint32_t *prio)
+{
+    if (m->type == YMRT_MATERIAL_DEFAULT) { *prio = m->ior_priority; return m->ior_n; }
+    *prio = 0;
+    return 1.0f;
+}
+
+static int tr_find_max(const or_transfer *t)
+{
+    if (t->count == 0) return -1;
+    uint32_t mp = t->rec[0].priority;
+    int mi = 0;
+    for (int i = 1; i < t->count; ++i)
+        if (t->rec[i].priority >= mp) { mp = t->rec[i].priority; mi = i; }
+    return mi;
+}
+static int tr_find_prim(const or_transfer *t, uint32_t prim)
+{
+    for (int i = t->count - 1; i >= 0; --i) if (t->rec[i].prim == prim) return i;
+    return -1;
+}
+static int tr_push(or_transfer *t, uint32_t prim, float ior, uint32_t prio, int vol)
+{
+    if (t->count == OR_MAX_BOUNDARY_RECORD) return 0;
+    or_record *r = &t->rec[t->count];
+    r->prim = prim; r->ior = ior; r->priority = prio; r->vol = vol;
+    ++t->count;
+    if (t->count == 1) { t->max_idx = 0; return 1; }
+    if (prio >= t->rec[t->max_idx].priority) t->max_idx = t->count - 1;
+    return 1;
+}
+static int tr_pop(or_transfer *t, int idx)
+{
+    if (t->count == 0) return 0;
+    if (!(idx >= 0 && idx < t->count)) return 0;
+    for (int i = idx + 1; i < t->count; ++i) t->rec[i - 1] = t->rec[i];
+    --t->count;
+    t->max_idx = tr_find_max(t);
+    return 1;
+}
+static int tr_volumes(const or_transfer *t, int *vols, uint32_t volume_count)
+{
+    if (t->count == 0 || t->max_idx == -1) return 0;
+    uint32_t mp = t->rec[t->max_idx].priority;
+    int n = 0;
+    for (int i = t->count - 1; i >= 0; --i)
+        if (t->rec[i].priority == mp && t->rec[i].vol >= 0 && (uint32_t)t->rec[i].vol < volume_count)
+            vols[n++] = t->rec[i].vol;
+    return n;
+}
+static float tr_ray_ior(const or_transfer *t) { return t->count > 0 ? t->rec[t->max_idx].ior : 1.0f; }
+static float tr_ex_ior(const or_transfer *t, uint32_t hit_prim)
+{
+    uint32_t mp = 0;
+    int mi = -1;
+    for (int i = 0; i < t->count; ++i) {
+        if (t->rec[i].prim == hit_prim) continue;
+        if (t->rec[i].priority >= mp) { mp = t->rec[i].priority; mi = i; }
+    }
+    return mi == -1 ? 1.0f : t->rec[mi].ior;
+}
+
+static void boundary_bunch(const or_scene *sc, or_transfer *t, const or_nearby *nb, int n)
+{
+    for (int i = 0; i < n; ++i) {
+        if (nb[i].back) continue;
+        const ymrt_instance_desc *in = &sc->instances[nb[i].prim];
+        uint32_t prio;
+        float ior = fetch_ior(&sc->materials[in->material], &prio);
+        if (tr_find_prim(t, nb[i].prim) == -1) tr_push(t, nb[i].prim, ior, prio, in->inner_volume);
+    }
+    for (int i = 0; i < n; ++i) {
+        if (!nb[i].back) continue;
+        int ri = tr_find_prim(t, nb[i].prim);
+        if (ri != -1) tr_pop(t, ri);
+    }
+}
+
+/* src/Camera.cpp:15-78.  The reference walks an unordered_map; here the
+ * distinct instances are visited in the order the centre ray meets them. */
+static void camera_transfer(or_scene *sc)
+{
+    or_transfer *ct = &sc->camera_transfer;
+    ct->count = 0; ct->max_idx = -1;
+    uint32_t seen[48];
+    int ns = 0;
+    or_ray ray = camera_ray(sc, 0.5f, 0.5f);
+    for (int depth = 0; depth < 48; ++depth) {
+        or_hit h;
+        memset(&h, 0, sizeof(h));
+        h.t = OR_TMAX; h.inst = OR_EMPTY; h.tri = OR_EMPTY;
+        or_nearby nb[OR_MAX_NEARBY];
+        int nc;
+        if (!or_trace_ray(sc, &ray, &h, &nc, nb)) break;
+        v3 p, perr, ng;
+        or_fetch_geometry_normal(sc, &h, &p, &perr, &ng);
+        int found = 0;
+        for (int i = 0; i < ns; ++i) if (seen[i] == h.inst) found = 1;
+        if (!found) seen[ns++] = h.inst;
+        ray = make_ray(or_offset_ray_origin(p, perr, ray.d, ng), ray.d, h.inst, h.tri);
+    }
+    for (int i = 0; i < ns; ++i) {
+        const ymrt_instance_desc *in = &sc->instances[seen[i]];
+        or_ray test = camera_ray(sc, 0.5f, 0.5f);
+        test.last_inst = OR_EMPTY; test.last_tri = OR_EMPTY;
+        or_ray pos, negr;
+        pos.o = m4_point(in->world_to_object, test.o);
+        pos.d = m4_dir(in->world_to_object, test.d);
+        pos.t = test.t; pos.last_inst = OR_EMPTY; pos.last_tri = OR_EMPTY;
+        negr = pos;
+        negr.d = neg3(pos.d);
+        /* any-hit against this instance only */
+        int hp = 0, hn = 0;
+        const or_mesh *m = &sc->meshes[in->mesh];
+        for (uint32_t t = 0; t < m->tri_count && !(hp && hn); ++t) {
+            v3 p0 = mk3(m->pos[3 * m->pidx[3 * t]], m->pos[3 * m->pidx[3 * t] + 1], m->pos[3 * m->pidx[3 * t] + 2]);
+            v3 p1 = mk3(m->pos[3 * m->pidx[3 * t + 1]], m->pos[3 * m->pidx[3 * t + 1] + 1], m->pos[3 * m->pidx[3 * t + 1] + 2]);
+            v3 p2 = mk3(m->pos[3 * m->pidx[3 * t + 2]], m->pos[3 * m->pidx[3 * t + 2] + 1], m->pos[3 * m->pidx[3 * t + 2] + 2]);
+            float th; v3 b; int bk;
+            if (!hp && or_intersect_triangle(p0, p1, p2, &pos, &th, &b, &bk) && th >= OR_TMIN && th <= pos.t) hp = 1;
+            if (!hn && or_intersect_triangle(p0, p1, p2, &negr, &th, &b, &bk) && th >= OR_TMIN && th <= negr.t) hn = 1;
+        }
+        if (hp && hn) {
+            uint32_t prio;
+            float ior = fetch_ior(&sc->materials[in->material], &prio);
+            if (!tr_push(ct, seen[i], ior, prio, in->inner_volume)) break;
+        }
+    }
+}
+
+static v3 ld3(const float *a, uint32_t i) { return mk3(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
+
+void or_shading_init(or_scene *sc, const ymrt_scene_desc *d)
+{
+    sc->material_count = d->material_count;
+    sc->materials = d->materials;
+    sc->volume_count = d->volume_count;
+    sc->volumes = d->volumes;
+    sc->camera = d->camera;
+    /* textures */
+    sc->texture_count = d->texture_count;
+    sc->textures = (or_texture *)calloc(d->texture_count ? d->texture_count : 1, sizeof(or_texture));
+    float lut[2][256];
+    for (int i = 0; i < 256; ++i) {
+        float v = (float)i * (1.0f / 255.0f);
+        lut[0][i] = or_srgb_to_linear(v);
+        lut[1][i] = v;
+    }
+    for (uint32_t ti = 0; ti < d->texture_count; ++ti) {
+        const ymrt_texture_desc *td = &d->textures[ti];
+        or_texture *t = &sc->textures[ti];
+        t->width = td->width; t->height = td->height; t->channels = td->channels;
+        t->wrap = td->wrap_mode; t->color_space = td->color_space;
+        t->su = td->scale_u; t->sv = td->scale_v; t->ou = td->offset_u; t->ov = td->offset_v;
+        size_t n = (size_t)td->width * td->height * td->channels;
+        t->texels = (float *)malloc(sizeof(float) * (n ? n : 1));
+        const float *l = lut[td->color_space == YMRT_COLOR_SRGB ? 0 : 1];
+        for (size_t i = 0; i < n; ++i) t->texels[i] = l[td->texels[i]];
+    }
+    sc->grid_count = d->grid_count;
+    sc->grids = (or_grid *)calloc(d->grid_count ? d->grid_count : 1, sizeof(or_grid));
+    for (uint32_t gi = 0; gi < d->grid_count; ++gi) {
+        or_grid *g = &sc->grids[gi];
+        g->nx = d->grids[gi].nx; g->ny = d->grids[gi].ny; g->nz = d->grids[gi].nz;
+        g->density = d->grids[gi].density;
+        float mx = 0.0f;
+        size_t n = (size_t)g->nx * g->ny * g->nz;
+        for (size_t i = 0; i < n; ++i) if (g->density[i] > mx) mx = g->density[i];
+        g->max_density = mx;
+    }
+    /* shape lights: src/core/SceneModule.cpp:1203-1304 */
+    uint32_t nl = 0;
+    for (uint32_t i = 0; i < d->instance_count; ++i) {
+        const ymrt_instance_desc *in = &d->instances[i];
+        if (in->material < d->material_count && d->materials[in->material].type == YMRT_MATERIAL_LIGHT)
+            nl += d->meshes[in->mesh].triangle_count;
+    }
+    sc->light_count = nl;
+    sc->lights = (or_shape_light *)calloc(nl ? nl : 1, sizeof(or_shape_light));
+    sc->light_cdf = (float *)calloc(nl ? nl : 1, sizeof(float));
+    nl = 0;
+    for (uint32_t i = 0; i < d->instance_count; ++i) {
+        const ymrt_instance_desc *in = &d->instances[i];
+        if (!(in->material < d->material_count && d->materials[in->material].type == YMRT_MATERIAL_LIGHT)) continue;
+        const ymrt_material_desc *lm = &d->materials[in->material];
+        float irr = (lm->light_color[0] + lm->light_color[1] + lm->light_color[2]) * lm->intensity * OR_PI;
+        const ymrt_mesh_desc *md = &d->meshes[in->mesh];
+        for (uint32_t t = 0; t < md->triangle_count; ++t) {
+            v3 p0 = transform_position(in->object_to_world, ld3(md->positions, md->position_indices[3 * t]), NULL);
+            v3 p1 = transform_position(in->object_to_world, ld3(md->positions, md->position_indices[3 * t + 1]), NULL);
+            v3 p2 = transform_position(in->object_to_world, ld3(md->positions, md->position_indices[3 * t + 2]), NULL);
+            float area = length3(cross3(sub3(p1, p0), sub3(p2, p0)));
+            sc->lights[nl].instance = i;
+            sc->lights[nl].triangle = t;
+            sc->lights[nl].power = irr * area;
+            sc->lights[nl].area = 0.5f * area;
+            ++nl;
+        }
+    }
+    float total = 0.0f;
+    for (uint32_t i = 0; i < nl; ++i) { total += sc->lights[i].power; sc->light_cdf[i] = total; }
+    float trcp = safe_rcp(total);
+    for (uint32_t i = 0; i < nl; ++i) sc->light_cdf[i] = total == 0.0f ? 0.0f : sc->light_cdf[i] * trcp;
+    /* sampler table */
+    sc->halton = (uint32_t *)malloc(sizeof(uint32_t) * or_halton_table_size());
+    or_halton_table(d->halton_seed, sc->halton);
+    camera_basis(sc);
+    camera_transfer(sc);
+}
+
+void or_shading_free(or_scene *sc)
+{
+    for (uint32_t i = 0; i < sc->texture_count; ++i) free(sc->textures[i].texels);
+    free(sc->textures);
+    free(sc->grids);
+    free(sc->lights);
+    free(sc->light_cdf);
+    free(sc->halton);
+}
+
+/* ---------------------------------------------------------------- textures */
+static v3 texel(const or_texture *t, int x, int y)
+{
+    int w = (int)t->width, h = (int)t->height;
+    if (t->wrap == YMRT_WRAP_CLAMP) {
+        x = x < 0 ? 0 : (x > w - 1 ? w - 1 : x);
+        y = y < 0 ? 0 : (y > h - 1 ? h - 1 : y);
+    } else {
+        x = ((x % w) + w) % w;
+        y = ((y % h) + h) % h;
+    }
+    const float *p = t->texels + ((size_t)y * w + x) * t->channels;
+    if (t->channels == 1) return s3(p[0]);
+    v3 r = s3(0.0f);
+    for (uint32_t c = 0; c < 3 && c < t->channels; ++c) setc3(&r, (int)c, p[c]);
+    return r;
+}
+
+v3 or_texture_eval(const or_scene *sc, uint32_t ti, float u, float v)
+{
+    const or_texture *t = &sc->textures[ti];
+    float tu = u * t->su + t->ou;
+    if (tu < 0.0f) tu = fmaxr(0.0f, tu + fabsf(floorf(tu)));
+    float tv = v * t->sv + t->ov;
+    if (tv < 0.0f) tv = fmaxr(0.0f, tv + fabsf(floorf(tv)));
+    float px = tu * (float)t->width, py = tv * (float)t->height;
+    float cx = px - 0.5f, cy = py - 0.5f;
+    int x0 = (int)floorf(cx), y0 = (int)floorf(cy);
+    float fx = cx - (float)x0, fy = cy - (float)y0;
+    int x1 = x0 + 1, y1 = y0 + 1;
+    v3 r = sclf3((1.0f - fx) * (1.0f - fy), texel(t, x0, y0));
+    r = add3(r, sclf3(fx * (1.0f - fy), texel(t, x1, y0)));
+    r = add3(r, sclf3((1.0f - fx) * fy, texel(t, x0, y1)));
+    r = add3(r, sclf3(fx * fy, texel(t, x1, y1)));
+    return r;
+}
+
+/* -------------------------------------------------------------------- BSDF */
+enum { LOBE_LAMBERT = 0, LOBE_REFLECT = 1, LOBE_TRANSMIT = 2 };
+typedef struct {
+    int type;
+    v3 albedo;          /* diffuse / specular / transmission albedo */
+    v3 nt, kt;          /* reflection */
+    float ni, nts;      /* ni; transmission nt */
+    float ax, ay;
+    int metal;
+} lobe;
+typedef struct {
+    v3 t, b, n;
+    int is_default;
+    lobe l[6];
+    float w[6];
+    int count;
+} mbsdf;
+
+static int same_hemi(v3 a, v3 b) { return a.z * b.z > 0.0f; }
+static v3 reflect3(v3 wo, v3 n) { return add3(neg3(wo), scl3(sclf3(2.0f, n), dot3(n, wo))); }
+static int refract3(v3 wo, v3 n, float ni, float nt, v3 *wi)
+{
+    float ci = fmaxr(dot3(wo, n), 0.0f);
+    float ci2 = sqrf(ci);
+    float si2 = 1.0f - ci2;
+    float ct2 = 1.0f - sqrf(ni / nt) * si2;
+    if (!(ct2 > 0.0f)) return 0;
+    float ct = sqrtf(ct2);
+    *wi = normalize3(sub3(sclf3(-(ni / nt), sub3(wo, scl3(n, ci))), scl3(n, ct)));
+    return 1;
+}
+static float fresnel_dd(float eta, float c)
+{
+    c = fmaxr(0.0f, c);
+    float s2 = 1.f - c * c;
+    float t0 = sqrtf(fmaxr(1.0f - (s2 / (eta * eta)), 0.0f));
+    float t1 = eta * t0;
+    float t2 = eta * c;
+    float rs = (c - t1) / fmaxr(c + t1, 1E-12f);
+    float rp = (t0 - t2) / fmaxr(t0 + t2, 1E-12f);
+    return (float)(0.5 * (double)(rs * rs + rp * rp));
+}
+static v3 fresnel_dc(v3 eta, v3 etak, float c)
+{
+    float c2 = c * c, s2 = 1.0f - c2;
+    v3 e2 = mul3(eta, eta), k2 = mul3(etak, etak);
+    v3 t0 = sub3(sub3(e2, k2), s3(s2));
+    v3 ab = mk3(sqrtf(t0.x * t0.x + 4.0f * e2.x * k2.x), sqrtf(t0.y * t0.y + 4.0f * e2.y * k2.y),
+                sqrtf(t0.z * t0.z + 4.0f * e2.z * k2.z));
+    v3 t1 = add3(ab, s3(c2));
+    v3 a = mk3(sqrtf(0.5f * (ab.x + t0.x)), sqrtf(0.5f * (ab.y + t0.y)), sqrtf(0.5f * (ab.z + t0.z)));
+    v3 t2 = scl3(sclf3(2.0f, a), c);
+    v3 rs = div3(sub3(t1, t2), add3(t1, t2));
+    v3 t3 = add3(sclf3(c2, ab), s3(s2 * s2));
+    v3 t4 = scl3(t2, s2);
+    v3 rp = div3(mul3(rs, sub3(t3, t4)), add3(t3, t4));
+    return sclf3(0.5f, add3(rp, rs));
+}
+static void edge_tint(v3 refl, v3 tint, v3 *n, v3 *k)
+{
+    v3 r = mk3(clampf(refl.x, 0.f, 0.99f), clampf(refl.y, 0.f, 0.99f), clampf(refl.z, 0.f, 0.99f));
+    v3 rs = mk3(sqrtf(r.x), sqrtf(r.y), sqrtf(r.z));
+    v3 one = s3(1.0f);
+    v3 nmin = div3(sub3(one, r), add3(one, r));
+    v3 nmax = div3(add3(one, rs), sub3(one, rs));
+    *n = mixv3(nmax, nmin, tint);
+    v3 np1 = add3(*n, one), nm1 = sub3(*n, one);
+    v3 k2 = div3(sub3(mul3(mul3(np1, np1), r), mul3(nm1, nm1)), sub3(one, r));
+    k2 = max3(k2, s3(0.0f));
+    *k = mk3(sqrtf(k2.x), sqrtf(k2.y), sqrtf(k2.z));
+}
+static float ggx_d(v3 h, float ax, float ay)
+{
+    float den = OR_PI * ax * ay * sqrf(sqrf(h.x / ax) + sqrf(h.y / ay) + sqrf(h.z));
+    return 1.0f / fmaxr(den, 1E-12f);
+}
+static float ggx_lambda(v3 w, float ax, float ay)
+{
+    if (fabsf(w.z) < 1E-8f) return 0.0f;
+    return (-1.0f + sqrtf(1.0f + (sqrf(w.x * ax) + sqrf(w.y * ay)) / sqrf(w.z))) * 0.5f;
+}
+static float ggx_g1(v3 w, float ax, float ay) { return 1.0f / (1.0f + ggx_lambda(w, ax, ay)); }
+static float ggx_g2(v3 a, v3 b, float ax, float ay)
+{
+    return 1.0f / (1.0f + ggx_lambda(a, ax, ay) + ggx_lambda(b, ax, ay));
+}
+static v3 ggx_sample_wh(v3 wo, float u0, float u1, float ax, float ay)
+{
+    v3 wh = normalize3(mk3(ax * wo.x, ay * wo.y, wo.z));
+    float lensq = wh.x * wh.x + wh.y * wh.y;
+    v3 T1 = lensq > 0 ? scl3(mk3(-wh.y, wh.x, 0), 1.0f / sqrtf(lensq)) : mk3(1, 0, 0);
+    v3 T2 = cross3(wh, T1);
+    float r = sqrtf(fmaxr(u0, 0.0f));
+    float phi = 2.0f * OR_PI * u1;
+    float t1 = r * cosf(phi);
+    float t2 = r * sinf(phi);
+    float s = (float)(0.5 * (1.0 + (double)wh.z));
+    t2 = (float)((1.0 - (double)s) * sqrt(1.0 - (double)(t1 * t1)) + (double)s * (double)t2);
+    v3 Nh = add3(add3(sclf3(t1, T1), sclf3(t2, T2)), sclf3(sqrtf(fmaxr(0.0f, 1.0f - t1 * t1 - t2 * t2)), wh));
+    return normalize3(mk3(ax * Nh.x, ay * Nh.y, fmaxr(0.0f, Nh.z)));
+}
+static v3 sample_cosine(float u0, float u1)
+{
+    float r = sqrtf(u0);
+    float phi = u1 * OR_TWO_PI;
+    float x = r * cosf(phi), y = r * sinf(phi);
+    float r2 = x * x + y * y;
+    return mk3(x, y, r2 < 1.0f ? sqrtf(1.0f - r2) : 0.0f);
+}
+
+static v3 lobe_eval(const lobe *l, v3 wo, v3 wi, float *pdf)
+{
+    if (l->type == LOBE_LAMBERT) {
+        if (!same_hemi(wo, wi)) { *pdf = 0.0f; return s3(0.0f); }
+        float c = fmaxr(wi.z, 0.0f);
+        *pdf = c * OR_INV_PI;
+        return scl3(scl3(l->albedo, c), OR_INV_PI);
+    }
+    if (l->type == LOBE_REFLECT) {
+        if (!same_hemi(wo, wi) || wo.z < 0.f || wi.z < 0.f) { *pdf = 0.0f; return s3(0.0f); }
+        v3 wh = add3(wi, wo);
+        float len = length3(wh);
+        if (fabsf(len) < 1E-8f) { *pdf = 0.0f; return s3(0.0f); }
+        wh = divs3(wh, len);
+        float D = ggx_d(wh, l->ax, l->ay);
+        float G1 = ggx_g1(wo, l->ax, l->ay);
+        float G2 = ggx_g2(wo, wi, l->ax, l->ay);
+        float ini = safe_rcp(l->ni);
+        float c = fmaxr(dot3(wo, wh), 0.0f);
+        v3 fr = l->metal ? fresnel_dc(scl3(l->nt, ini), scl3(l->kt, ini), c) : s3(fresnel_dd(l->nt.x * ini, c));
+        *pdf = D * G1 * 0.25f / fmaxr(wo.z, 1E-8f);
+        return divs3(mul3(scl3(scl3(scl3(l->albedo, D), G2), 0.25f), fr), fmaxr(wo.z, 1E-8f));
+    }
+    /* transmission */
+    if (same_hemi(wo, wi)) { *pdf = 0.0f; return s3(0.0f); }
+    v3 wh = neg3(add3(sclf3(l->ni, wo), sclf3(l->nts, wi)));
+    if (!same_hemi(wo, wh)) { *pdf = 0.0f; return s3(0.0f); }
+    float len = length3(wh);
+    if (fabsf(len) < 1E-8f) { *pdf = 0.0f; return s3(0.0f); }
+    wh = divs3(wh, len);
+    float D = ggx_d(wh, l->ax, l->ay);
+    float G1 = ggx_g1(wo, l->ax, l->ay);
+    float G2 = ggx_g2(wo, wi, l->ax, l->ay);
+    float dwh = sqrf(l->nts) * fabsf(dot3(wi, wh)) /
+                fmaxr(sqrf(l->ni * dot3(wo, wh) + l->nts * dot3(wi, wh)), 1E-12f);
+    float ini = safe_rcp(l->ni);
+    float c = fmaxr(dot3(wo, wh), 0.0f);
+    float fr = fresnel_dd(l->nts * ini, c);
+    *pdf = D * G1 * fabsf(dot3(wo, wh)) * dwh / fmaxr(fabsf(wo.z), 1E-8f);
+    v3 r = scl3(l->albedo, 1.0f - fr);
+    r = scl3(r, fabsf(dot3(wo, wh) / fmaxr(fabsf(wo.z), 1E-8f)));
+    r = scl3(r, G2); r = scl3(r, D); r = scl3(r, dwh); r = scl3(r, sqrf(l->ni / l->nts));
+    return r;
+}
+
+static int lobe_sample(const lobe *l, float u0, float u1, v3 wo, v3 *weight, v3 *wi, float *pdf)
+{
+    if (l->type == LOBE_LAMBERT) {
+        *wi = sample_cosine(u0, u1);
+        float c = fmaxr(wi->z, 0.0f);
+        *pdf = c * OR_INV_PI;
+        *weight = l->albedo;
+        return 1;
+    }
+    v3 wh = ggx_sample_wh(wo, u0, u1, l->ax, l->ay);
+    if (!same_hemi(wo, wh) || dot3(wo, wh) < 0.0f) return 0;
+    if (l->type == LOBE_REFLECT) {
+        *wi = reflect3(wo, wh);
+        if (!same_hemi(*wi, wo)) return 0;
+        float D = ggx_d(wh, l->ax, l->ay);
+        float G1 = ggx_g1(wo, l->ax, l->ay);
+        float G2 = ggx_g2(wo, *wi, l->ax, l->ay);
+        float ini = safe_rcp(l->ni);
+        float c = fmaxr(dot3(wo, wh), 0.0f);
+        v3 fr = l->metal ? fresnel_dc(scl3(l->nt, ini), scl3(l->kt, ini), c) : s3(fresnel_dd(l->nt.x * ini, c));
+        *pdf = D * G1 * 0.25f / fmaxr(wo.z, 1E-8f);
+        *weight = divs3(mul3(scl3(l->albedo, G2), fr), G1);
+        return 1;
+    }
+    if (!refract3(wo, wh, l->ni, l->nts, wi)) return 0;
+    if (same_hemi(*wi, wo)) return 0;
+    float D = ggx_d(wh, l->ax, l->ay);
+    float G1 = ggx_g1(wo, l->ax, l->ay);
+    float G2 = ggx_g2(*wi, wo, l->ax, l->ay);
+    float dwh = sqrf(l->nts) * fabsf(dot3(*wi, wh)) /
+                fmaxr(sqrf(l->ni * dot3(wo, wh) + l->nts * dot3(*wi, wh)), 1E-12f);
+    float ini = safe_rcp(l->ni);
+    float c = fmaxr(dot3(wo, wh), 0.0f);
+    float fr = fresnel_dd(l->nts * ini, c);
+    *pdf = D * G1 * fabsf(dot3(wo, wh)) * dwh / fmaxr(fabsf(wo.z), 1E-8f);
+    *weight = divs3(scl3(scl3(scl3(l->albedo, 1.0f - fr), sqrf(l->ni / l->nts)), G2), G1);
+    return 1;
+}
+
+static float tex_float(const or_scene *sc, uint32_t tex, float def, v2 uv)
+{
+    return tex != OR_EMPTY ? or_texture_eval(sc, tex, uv.x, uv.y).x : def;
+}
+static v3 tex_color(const or_scene *sc, uint32_t tex, const float *def, v2 uv)
+{
+    return tex != OR_EMPTY ? or_texture_eval(sc, tex, uv.x, uv.y) : mk3(def[0], def[1], def[2]);
+}
+
+/* src/BSDF.cuh:453-618 */
+static void init_default_bsdf(const or_scene *sc, mbsdf *b, const ymrt_material_desc *m, v2 uv,
+                              v3 ray_d, int hit_back, float ray_ior, float ex_ior)
+{
+    const float ior_min = 0.0001f;
+    float coat_w = tex_float(sc, m->coat_weight_tex, m->coat_weight, uv);
+    float ior_in = fmaxr(ray_ior, ior_min);
+    float ior_coat = fmaxr(m->coat_ior, ior_min);
+    float ior_out = fmaxr(hit_back ? ex_ior : m->ior_n, ior_min);
+    float eta_ic = ior_coat * safe_rcp(ior_in);
+    float eta_io = ior_out * safe_rcp(ior_in);
+    float cos_in = fmaxr(dot3(b->n, neg3(ray_d)), 0.0f);
+    float fr_ic = coat_w > 0.0f ? fresnel_dd(eta_ic, cos_in) : 1.0f;
+    float fr_io = fresnel_dd(eta_io, cos_in);
+    float metal = tex_float(sc, m->metalness_tex, m->metalness, uv);
+    float ax = sqrf(clampf(tex_float(sc, m->roughness_x_tex, m->roughness_x, uv), 0.001f, 0.999f));
+    float ay = sqrf(clampf(tex_float(sc, m->roughness_y_tex, m->roughness_y, uv), 0.001f, 0.999f));
+    float spec_w = tex_float(sc, m->specular_weight_tex, m->specular_weight, uv);
+    float trans_w = tex_float(sc, m->transmission_weight_tex, m->transmission_weight, uv);
+    v3 dif = tex_color(sc, m->diffuse_albedo_tex, m->diffuse_albedo, uv);
+    v3 spe = tex_color(sc, m->specular_albedo_tex, m->specular_albedo, uv);
+    float coat_th = tex_float(sc, m->coat_thickness_tex, m->coat_thickness, uv);
+    float cax = sqrf(clampf(tex_float(sc, m->coat_roughness_x_tex, m->coat_roughness_x, uv), 0.001f, 0.999f));
+    float cay = sqrf(clampf(tex_float(sc, m->coat_roughness_y_tex, m->coat_roughness_y, uv), 0.001f, 0.999f));
+    v3 coat_alb = tex_color(sc, m->coat_albedo_tex, m->coat_albedo, uv);
+    v3 ca = max3(coat_alb, s3(1E-8f));
+    v3 sigma = scl3(neg3(mk3(logf(ca.x), logf(ca.y), logf(ca.z))), coat_th);
+    float rc = safe_rcp(cos_in);
+    v3 ns = neg3(sigma);
+    v3 coat_tr = mk3(expf(ns.x * rc), expf(ns.y * rc), expf(ns.z * rc));
+    float wsum = 0.0f;
+    b->count = 0;
+    v3 coat_color = s3(coat_w * fr_ic);
+    float coat_lum = luminance3(coat_color);
+    if (coat_lum > 0.0f) {
+        lobe *l = &b->l[b->count];
+        l->type = LOBE_REFLECT; l->albedo = coat_color; l->ax = cax; l->ay = cay;
+        l->nt = s3(ior_coat); l->kt = s3(0.0f); l->ni = ior_in; l->metal = 0;
+        b->w[b->count] = coat_lum; wsum += coat_lum; ++b->count;
+    }
+    /* LinearLerp(vec3(1), coat_tr, coat_w) = (1 - w) * 1 + coat_tr * w */
+    v3 lerp = add3(sclf3(1.0f - coat_w, s3(1.0f)), scl3(coat_tr, coat_w));
+    v3 tint = sclf3(1.0f - coat_w * fr_ic, lerp);
+    v3 metal_c = scl3(tint, metal);
+    float metal_lum = luminance3(metal_c);
+    if (metal_lum > 0.0f) {
+        v3 mn, mk;
+        edge_tint(dif, spe, &mn, &mk);
+        lobe *l = &b->l[b->count];
+        l->type = LOBE_REFLECT; l->albedo = metal_c; l->ax = ax; l->ay = ay;
+        l->nt = mn; l->kt = mk; l->ni = ior_in; l->metal = 1;
+        b->w[b->count] = metal_lum; wsum += metal_lum; ++b->count;
+    }
+    v3 spec_c = mul3(scl3(scl3(scl3(tint, 1.0f - metal), fr_io), spec_w), spe);
+    float spec_lum = luminance3(spec_c);
+    if (spec_lum > 0.0f) {
+        lobe *l = &b->l[b->count];
+        l->type = LOBE_REFLECT; l->albedo = spec_c; l->ax = ax; l->ay = ay;
+        l->nt = s3(ior_out); l->kt = s3(0.0f); l->ni = ior_in; l->metal = 0;
+        b->w[b->count] = spec_lum; wsum += spec_lum; ++b->count;
+    }
+    v3 trans_c = mul3(scl3(scl3(scl3(tint, 1.0f - metal), 1.0f - fr_io * spec_w), trans_w), spe);
+    float trans_lum = luminance3(trans_c);
+    if (trans_lum > 0.0f) {
+        lobe *l = &b->l[b->count];
+        l->type = LOBE_TRANSMIT; l->albedo = trans_c; l->ax = ax; l->ay = ay;
+        l->ni = ior_in; l->nts = ior_out;
+        b->w[b->count] = trans_lum; wsum += trans_lum; ++b->count;
+    }
+    v3 dif_c = mul3(scl3(scl3(scl3(tint, 1.0f - metal), 1.0f - fr_io * spec_w), 1.0f - trans_w), dif);
+    float dif_lum = luminance3(dif_c);
+    {
+        lobe *l = &b->l[b->count];
+        l->type = LOBE_LAMBERT; l->albedo = dif_c;
+        b->w[b->count] = dif_lum; wsum += dif_lum; ++b->count;
+    }
+    if (wsum > 0.0f) {
+        float iw = safe_rcp(wsum);
+        for (int i = 0; i < b->count; ++i) b->w[i] *= iw;
+    } else {
+        float iw = 1.0f / (float)b->count;
+        for (int i = 0; i < b->count; ++i) b->w[i] = iw;
+    }
+}
+
+static v3 bsdf_eval(const mbsdf *b, v3 wo, v3 wi, float *pdf)
+{
+    v3 f = s3(0.0f);
+    float p = 0.0f;
+    if (!b->is_default) { *pdf = 0.0f; return f; }
+    for (int i = 0; i < b->count; ++i) {
+        float lp = 0.0f;
+        f = add3(f, lobe_eval(&b->l[i], wo, wi, &lp));
+        p += b->w[i] * lp;
+    }
+    *pdf = p;
+    return f;
+}
+
+static int bsdf_sample(const mbsdf *b, v3 wo, v3 *weight, v3 *wi, float *pdf, float u0, float u1, float u2)
+{
+    if (!b->is_default) return 0;
+    float cdf = 0.0f;
+    int sel = -1;
+    for (int i = 0; i < b->count; ++i) {
+        if (u0 < cdf + b->w[i]) { sel = i; break; }
+        cdf += b->w[i];
+    }
+    if (sel == -1) sel = b->count - 1;
+    if (sel == -1) return 0;
+    float lp = 0.0f;
+    if (!lobe_sample(&b->l[sel], u1, u2, wo, weight, wi, &lp)) return 0;
+    float mp = lp * b->w[sel];
+    v3 mf = scl3(*weight, lp);
+    for (int i = 0; i < b->count; ++i) {
+        if (i == sel) continue;
+        float p = 0.0f;
+        mf = add3(mf, lobe_eval(&b->l[i], wo, *wi, &p));
+        mp += b->w[i] * p;
+    }
+    *pdf = mp;
+    *weight = scl3(mf, safe_rcp(mp));
+    return 1;
+}
+
+static v3 to_local(const mbsdf *b, v3 w) { return mk3(dot3(w, b->t), dot3(w, b->b), dot3(w, b->n)); }
+static v3 to_world(const mbsdf *b, v3 w)
+{
+    return add3(add3(sclf3(w.x, b->t), sclf3(w.y, b->b)), sclf3(w.z, b->n));
+}
+
+/* src/BSDF.cuh:694-807 without normal/bump maps */
+static void init_shading_space(mbsdf *b, int hit_back, v3 ray_d, v3 ng, v3 sn_in, v3 dpdu, v3 dpdv)
+{
+    v3 sn = sn_in;
+    if (dot3(sn_in, ng) < 0.0f) sn = neg3(sn_in);
+    v3 su = dpdu;
+    if (dot3(cross3(dpdu, dpdv), ng) < 0.0f) su = neg3(dpdu);
+    su = sub3(su, scl3(sn, dot3(su, sn)));
+    if (!hit_back) {
+        b->n = sn; b->t = normalize3(su); b->b = cross3(b->n, b->t);
+    } else {
+        b->n = neg3(sn); b->t = neg3(normalize3(su)); b->b = neg3(cross3(b->n, b->t));
+    }
+    if (dot3(neg3(ray_d), b->n) < 0.0f) {
+        b->n = reflect3(b->n, dot3(b->n, ng) > 0.0f ? ng : neg3(ng));
+        b->t = normalize3(sub3(b->t, scl3(b->n, dot3(b->t, b->n))));
+        b->b = hit_back ? neg3(cross3(b->n, b->t)) : cross3(b->n, b->t);
+    }
+}
+
+int or_bsdf_eval(const ymrt_material_desc *m, v3 wo, v3 wi, float ray_ior, float ex_ior, int hit_back,
+                 float *f_rgb, float *pdf)
+{
+    mbsdf b;
+    memset(&b, 0, sizeof(b));
+    b.t = mk3(1, 0, 0); b.b = mk3(0, 1, 0); b.n = mk3(0, 0, 1);
+    b.is_default = m->type == YMRT_MATERIAL_DEFAULT;
+    or_scene dummy;
+    memset(&dummy, 0, sizeof(dummy));
+    if (b.is_default) init_default_bsdf(&dummy, &b, m, mk2(0, 0), neg3(wo), hit_back, ray_ior, ex_ior);
+    v3 f = bsdf_eval(&b, wo, wi, pdf);
+    f_rgb[0] = f.x; f_rgb[1] = f.y; f_rgb[2] = f.z;
+    return b.count;
+}
+
+int or_bsdf_sample(const ymrt_material_desc *m, v3 wo, float ray_ior, float ex_ior, int hit_back,
+                   float u0, float u1, float u2, float *w_rgb, float *wi_out, float *pdf)
+{
+    mbsdf b;
+    memset(&b, 0, sizeof(b));
+    b.t = mk3(1, 0, 0); b.b = mk3(0, 1, 0); b.n = mk3(0, 0, 1);
+    b.is_default = m->type == YMRT_MATERIAL_DEFAULT;
+    or_scene dummy;
+    memset(&dummy, 0, sizeof(dummy));
+    if (b.is_default) init_default_bsdf(&dummy, &b, m, mk2(0, 0), neg3(wo), hit_back, ray_ior, ex_ior);
+    v3 w = s3(0.0f), wi = s3(0.0f);
+    *pdf = 0.0f;
+    int ok = bsdf_sample(&b, wo, &w, &wi, pdf, u0, u1, u2);
+    w_rgb[0] = w.x; w_rgb[1] = w.y; w_rgb[2] = w.z;
+    wi_out[0] = wi.x; wi_out[1] = wi.y; wi_out[2] = wi.z;
+    return ok;
+}
+
+/* ------------------------------------------------------------------ phases */
+float or_hg_eval(float g, v3 wo, v3 wi, float *pdf)
+{
+    float c = dot3(wo, wi);
+    float t = 1.f + g * g + 2.f * g * c;
+    *pdf = 0.25f * OR_INV_PI * (1.f - g * g) * safe_rcp(t * sqrtf(fmaxr(t, 0.0f)));
+    return *pdf;
+}
+
+int or_hg_sample(float g, float u0, float u1, v3 wo, v3 *wi, float *pdf)
+{
+    float c;
+    if (fabsf(g) < 1e-4f) {
+        c = 1.f - 2.f * u0;
+    } else {
+        float t = (1.f - g * g) * safe_rcp(1.f - g + 2.f * g * u0);
+        c = (1.f + g * g - t * t) / (2.f * g);
+    }
+    float s = sqrtf(fmaxr(0.0f, 1.0f - c * c));
+    float phi = 2.f * OR_PI * u1;
+    v3 w = wo, u, v;
+    orthogonal_basis(&w, &u, &v);
+    *wi = add3(add3(scl3(scl3(u, s), cosf(phi)), scl3(scl3(v, s), sinf(phi))), scl3(w, c));
+    or_hg_eval(g, wo, *wi, pdf);
+    return 1;
+}
+
+static int mixed_phase_sample(const float *vw, const float *gs, int n, float u0, float u1, v3 wo,
+                              float *weight, v3 *wi, float *pdf)
+{
+    if (n == 0) return 0;
+    float cdf = 0.0f;
+    int sel = -1;
+    for (int i = 0; i < n; ++i) {
+        if (u0 < cdf + vw[i]) { sel = i; u0 = (u0 - cdf) * safe_rcp(vw[i]); break; }
+        cdf += vw[i];
+    }
+    if (sel == -1) sel = n - 1;
+    float pp = 0.0f;
+    or_hg_sample(gs[sel], u0, u1, wo, wi, &pp);
+    float w1 = 1.0f;
+    float mp = pp * vw[sel];
+    float mg = w1 * pp * vw[sel];
+    for (int i = 0; i < n; ++i) {
+        if (i == sel) continue;
+        float p;
+        mg += vw[i] * or_hg_eval(gs[i], wo, *wi, &p);
+        mp += vw[i] * p;
+    }
+    *pdf = mp;
+    *weight = mg * safe_rcp(mp);
+    return 1;
+}
+
+static float mixed_phase_eval(const float *vw, const float *gs, int n, v3 wo, v3 wi, float *pdf)
+{
+    if (n == 0) { *pdf = 0.0f; return 0.0f; }
+    float mg = 0.0f, mp = 0.0f;
+    for (int i = 0; i < n; ++i) {
+        float p = 0.0f;
+        float g = or_hg_eval(gs[i], wo, wi, &p);
+        mg += vw[i] * g;
+        mp += vw[i] * p;
+    }
+    *pdf = mp;
+    return mg;
+}
+
+/* ------------------------------------------------------------------ volumes */
+typedef struct {
+    or_sampler *s;        /* path sampler (free-flight sampling)            */
+    uint32_t hash_key[4]; /* shadow transmittance stream for grid media     */
+    uint32_t hash_ctr;
+} rng_ctx;
+
+static float shadow_rand(rng_ctx *r)
+{
+    uint32_t v[4] = {r->hash_key[0], r->hash_key[1] ^ (r->hash_ctr << 8), r->hash_key[2], r->hash_key[3]};
+    r->hash_ctr++;
+    or_pcg4d(v);
+    return (float)(v[0] >> 8) * (1.0f / 16777216.0f);
+}
+
+static float grid_density(const or_scene *sc, int gi, v3 p)
+{
+    const or_grid *g = &sc->grids[gi];
+    if (!(p.x >= 0.0f && p.x < 1.0f && p.y >= 0.0f && p.y < 1.0f && p.z >= 0.0f && p.z < 1.0f)) return 0.0f;
+    uint32_t ix = (uint32_t)(p.x * (float)g->nx), iy = (uint32_t)(p.y * (float)g->ny), iz = (uint32_t)(p.z * (float)g->nz);
+    if (ix >= g->nx) ix = g->nx - 1;
+    if (iy >= g->ny) iy = g->ny - 1;
+    if (iz >= g->nz) iz = g->nz - 1;
+    return g->density[((size_t)iz * g->ny + iy) * g->nx + ix];
+}
+
+static int vol_is_het(const or_scene *sc, int v)
+{
+    int gi = sc->volumes[v].density_grid;
+    return gi >= 0 && (uint32_t)gi < sc->grid_count;
+}
+
+static v3 vsig(const ymrt_volume_desc *v) { return mk3(v->sigma_t[0], v->sigma_t[1], v->sigma_t[2]); }
+static v3 valb(const ymrt_volume_desc *v) { return mk3(v->albedo[0], v->albedo[1], v->albedo[2]); }
+
+/* src/VolumeEval.cuh:18-78 */
+static int sample_homogeneous(const or_scene *sc, int n, const int *vi, float tmax, v3 *trw, float *dist,
+                              float *vw, or_sampler *s)
+{
+    *trw = s3(1.0f);
+    *dist = OR_FLOAT_MAX;
+    v3 st = s3(0.0f), ss = s3(0.0f);
+    for (int i = 0; i < n; ++i) {
+        const ymrt_volume_desc *v = &sc->volumes[vi[i]];
+        st = add3(st, vsig(v));
+        ss = add3(ss, mul3(vsig(v), valb(v)));
+    }
+    float csum = st.x + st.y + st.z;
+    v3 cw = csum > 0.0f ? divs3(st, csum) : s3(1.0f / 3.0f);
+    float u0 = or_sampler_next(s);
+    int ch;
+    if (u0 < cw.x) { ch = 0; u0 = u0 * safe_rcp(cw.x); }
+    else if (u0 < cw.x + cw.y) { ch = 1; u0 = (u0 - cw.x) * safe_rcp(cw.y); }
+    else { ch = 2; u0 = (u0 - cw.x - cw.y) * safe_rcp(cw.z); }
+    float sc_ = comp3(st, ch);
+    float fpl = sc_ > 0.0f ? (-logf(1.0f - u0) / sc_) : OR_FLOAT_MAX;
+    int scatter = fpl < tmax;
+    float m = fminr(tmax, fpl);
+    v3 tr = mk3(expf(-st.x * m), expf(-st.y * m), expf(-st.z * m));
+    float pdf = scatter ? (cw.x * st.x * tr.x + cw.y * st.y * tr.y + cw.z * st.z * tr.z)
+                        : (cw.x * tr.x + cw.y * tr.y + cw.z * tr.z);
+    if (!(fabsf(pdf) > 0.0f)) return 0;
+    float w = 1.0f / (float)n;
+    for (int i = 0; i < n; ++i) vw[i] = w;
+    *trw = scatter ? divs3(mul3(tr, ss), pdf) : divs3(tr, pdf);
+    *dist = fpl;
+    return scatter;
+}
+
+/* src/VolumeEval.cuh:80-214, densities from the voxel grid */
+static int sample_heterogeneous(const or_scene *sc, int n, const int *vi, const or_ray *ray, float tmax,
+                                v3 *trw, float *dist, float *vw, or_sampler *s)
+{
+    *trw = s3(1.0f);
+    *dist = OR_FLOAT_MAX;
+    v3 ro[OR_MAX_BOUNDARY_RECORD], rd[OR_MAX_BOUNDARY_RECORD];
+    v3 sm = s3(0.0f);
+    for (int i = 0; i < n; ++i) {
+        const ymrt_volume_desc *v = &sc->volumes[vi[i]];
+        float ub = 1.0f;
+        if (vol_is_het(sc, vi[i])) {
+            ro[i] = m4_point(v->world_to_object, ray->o);
+            rd[i] = m4_dir(v->world_to_object, ray->d);
+            ub = sc->grids[v->density_grid].max_density;
+        }
+        sm = add3(sm, scl3(vsig(v), ub));
+    }
+    float csum = sm.x + sm.y + sm.z;
+    v3 cw = csum > 0.0f ? divs3(sm, csum) : s3(1.0f / 3.0f);
+    float u0 = or_sampler_next(s);
+    int ch = u0 < cw.x ? 0 : (u0 < cw.x + cw.y ? 1 : 2);
+    if (!(comp3(sm, ch) > 0.0f)) return 0;
+    v3 nf = mk3(safe_rcp(sm.x + 1.0f), safe_rcp(sm.y + 1.0f), safe_rcp(sm.z + 1.0f));
+    v3 smr = mk3(safe_rcp(sm.x), safe_rcp(sm.y), safe_rcp(sm.z));
+    v3 tr = s3(1.0f), pdf = s3(1.0f);
+    float acc = 0.0f;
+    int it = 0;
+    for (;;) {
+        if (it >= OR_MAX_FREE_PATH_LENGTH) break;
+        float fpl = -logf(1.0f - or_sampler_next(s)) / comp3(sm, ch);
+        if (acc + fpl > tmax) {
+            float dd = tmax - acc;
+            v3 ptr = mk3(expf(-sm.x * dd), expf(-sm.y * dd), expf(-sm.z * dd));
+            tr = mul3(tr, mul3(ptr, nf));
+            pdf = mul3(pdf, mul3(ptr, nf));
+            acc += fpl;
+            break;
+        }
+        acc += fpl;
+        v3 st = s3(0.0f), ss = s3(0.0f);
+        for (int i = 0; i < n; ++i) {
+            const ymrt_volume_desc *v = &sc->volumes[vi[i]];
+            float dens = 1.0f;
+            if (vol_is_het(sc, vi[i])) dens = fmaxr(grid_density(sc, v->density_grid, add3(ro[i], scl3(rd[i], acc))), 0.0f);
+            st = add3(st, scl3(vsig(v), dens));
+            ss = add3(ss, scl3(mul3(vsig(v), valb(v)), dens));
+        }
+        v3 rp;
+        rp.x = fminr(sm.x > 0.0f ? st.x * smr.x : 0.0f, 1.0f);
+        rp.y = fminr(sm.y > 0.0f ? st.y * smr.y : 0.0f, 1.0f);
+        rp.z = fminr(sm.z > 0.0f ? st.z * smr.z : 0.0f, 1.0f);
+        v3 fp = sub3(s3(1.0f), rp);
+        v3 ptr = mk3(expf(-sm.x * fpl), expf(-sm.y * fpl), expf(-sm.z * fpl));
+        if (or_sampler_next(s) < comp3(rp, ch)) {
+            tr = mul3(tr, mul3(mul3(ptr, ss), nf));
+            pdf = mul3(pdf, mul3(mul3(mul3(ptr, sm), rp), nf));
+            break;
+        }
+        tr = mul3(tr, mul3(mul3(mul3(ptr, sm), fp), nf));
+        pdf = mul3(pdf, mul3(mul3(mul3(ptr, sm), fp), nf));
+        ++it;
+    }
+    float w = 1.0f / (float)n;
+    for (int i = 0; i < n; ++i) vw[i] = w;
+    *trw = scl3(tr, safe_rcp(dot3(cw, pdf)));
+    *dist = acc;
+    return (acc > tmax || it >= OR_MAX_FREE_PATH_LENGTH) ? 0 : 1;
+}
+
+static int sample_volume(const or_scene *sc, const or_ray *ray, int n, const int *vi, float tmax,
+                         v3 *trw, float *dist, float *vw, or_sampler *s)
+{
+    if (sc->volume_count == 0 || n == 0) return 0;
+    int het = 0;
+    for (int i = 0; i < n; ++i) if (vol_is_het(sc, vi[i])) { het = 1; break; }
+    if (het) return sample_heterogeneous(sc, n, vi, ray, tmax, trw, dist, vw, s);
+    return sample_homogeneous(sc, n, vi, tmax, trw, dist, vw, s);
+}
+
+/* src/VolumeEval.cuh:253-385 */
+static v3 eval_tr(const or_scene *sc, const or_ray *ray, const or_transfer *t, float tmax, rng_ctx *rng)
+{
+    if (sc->volume_count == 0) return s3(1.0f);
+    int vi[OR_MAX_BOUNDARY_RECORD];
+    int n = tr_volumes(t, vi, sc->volume_count);
+    if (n == 0) return s3(1.0f);
+    int het = 0;
+    for (int i = 0; i < n; ++i) if (vol_is_het(sc, vi[i])) { het = 1; break; }
+    if (!het) {
+        v3 st = s3(0.0f);
+        for (int i = 0; i < n; ++i) st = add3(st, vsig(&sc->volumes[vi[i]]));
+        return mk3(expf(-st.x * tmax), expf(-st.y * tmax), expf(-st.z * tmax));
+    }
+    v3 ro[OR_MAX_BOUNDARY_RECORD], rd[OR_MAX_BOUNDARY_RECORD];
+    v3 sm = s3(0.0f);
+    for (int i = 0; i < n; ++i) {
+        const ymrt_volume_desc *v = &sc->volumes[vi[i]];
+        float ub = 1.0f;
+        if (vol_is_het(sc, vi[i])) {
+            ro[i] = m4_point(v->world_to_object, ray->o);
+            rd[i] = m4_dir(v->world_to_object, ray->d);
+            ub = sc->grids[v->density_grid].max_density;
+        }
+        sm = add3(sm, scl3(vsig(v), ub));
+    }
+    float csum = sm.x + sm.y + sm.z;
+    v3 cw = csum > 0.0f ? divs3(sm, csum) : s3(1.0f / 3.0f);
+    float u0 = shadow_rand(rng);
+    int ch = u0 < cw.x ? 0 : (u0 < cw.x + cw.y ? 1 : 2);
+    if (!(comp3(sm, ch) > 0.0f)) return s3(1.0f);
+    v3 nf = mk3(safe_rcp(sm.x + 1.0f), safe_rcp(sm.y + 1.0f), safe_rcp(sm.z + 1.0f));
+    v3 smr = mk3(safe_rcp(sm.x), safe_rcp(sm.y), safe_rcp(sm.z));
+    v3 tr = s3(1.0f), pdf = s3(1.0f);
+    float acc = 0.0f;
+    int it = 0;
+    for (;;) {
+        if (it >= OR_MAX_FREE_PATH_LENGTH) break;
+        float fpl = -logf(1.0f - shadow_rand(rng)) / comp3(sm, ch);
+        if (acc + fpl > tmax) {
+            float dd = tmax - acc;
+            v3 ptr = mk3(expf(-sm.x * dd), expf(-sm.y * dd), expf(-sm.z * dd));
+            tr = mul3(tr, mul3(ptr, nf));
+            pdf = mul3(pdf, mul3(ptr, nf));
+            break;
+        }
+        acc += fpl;
+        v3 st = s3(0.0f);
+        for (int i = 0; i < n; ++i) {
+            const ymrt_volume_desc *v = &sc->volumes[vi[i]];
+            float dens = 1.0f;
+            if (vol_is_het(sc, vi[i])) dens = fmaxr(grid_density(sc, v->density_grid, add3(ro[i], scl3(rd[i], acc))), 0.0f);
+            st = add3(st, scl3(vsig(v), dens));
+        }
+        v3 rp;
+        rp.x = fminr(sm.x > 0.0f ? st.x * smr.x : 0.0f, 1.0f);
+        rp.y = fminr(sm.y > 0.0f ? st.y * smr.y : 0.0f, 1.0f);
+        rp.z = fminr(sm.z > 0.0f ? st.z * smr.z : 0.0f, 1.0f);
+        v3 fp = sub3(s3(1.0f), rp);
+        v3 ptr = mk3(expf(-sm.x * fpl), expf(-sm.y * fpl), expf(-sm.z * fpl));
+        tr = mul3(tr, mul3(mul3(mul3(ptr, sm), fp), nf));
+        pdf = mul3(pdf, mul3(mul3(ptr, sm), nf));
+        if (it + 1 > 32) {
+            v3 pt = scl3(tr, safe_rcp(dot3(cw, pdf)));
+            float rr = fminr(fmaxr(pt.x, fmaxr(pt.y, pt.z)), 1.0f);
+            if (shadow_rand(rng) < rr) tr = divs3(tr, fmaxr(rr, OR_MIN_COLOR_EPSILON));
+            else break;
+        }
+        ++it;
+    }
+    return scl3(tr, safe_rcp(dot3(cw, pdf)));
+}
+
+/* src/VolumeEval.cuh:420-468 */
+static v3 trace_tr(const or_scene *sc, const or_ray *ray, or_transfer *t, rng_ctx *rng, uint64_t *counts)
+{
+    if (sc->volume_count == 0) return s3(1.0f);
+    v3 tr = s3(1.0f);
+    or_ray r = *ray;
+    float maxd = ray->t;
+    float traced = OR_TMIN;
+    while (traced < maxd) {
+        or_hit h;
+        memset(&h, 0, sizeof(h));
+        h.t = OR_TMAX; h.inst = OR_EMPTY; h.tri = OR_EMPTY;
+        or_nearby nb[OR_MAX_NEARBY];
+        int nc = 0;
+        if (counts) counts[3]++;
+        int hb = or_trace_volume(sc, &r, &h, &nc, nb);
+        if (!hb) {
+            tr = mul3(tr, eval_tr(sc, &r, t, fmaxr(maxd - traced, 0.0f), rng));
+            break;
+        }
+        tr = mul3(tr, eval_tr(sc, &r, t, h.t, rng));
+        traced += h.t;
+        v3 p, perr, ng;
+        or_fetch_geometry_normal(sc, &h, &p, &perr, &ng);
+        boundary_bunch(sc, t, nb, nc);
+        r = make_ray(or_offset_ray_origin(p, perr, r.d, ng), r.d, h.inst, h.tri);
+    }
+    return tr;
+}
+
+/* ------------------------------------------------------------------ lights */
+static int sample_light_power(const or_scene *sc, float u, float *pdf)
+{
+    int left = 0, right = (int)sc->light_count;
+    while (left < right) {
+        int mid = (left + right) / 2;
+        if (sc->light_cdf[mid] <= u) left = mid + 1; else right = mid;
+    }
+    int idx = left;
+    if (idx > (int)sc->light_count - 1) idx = (int)sc->light_count - 1;
+    *pdf = idx == 0 ? sc->light_cdf[idx] : (sc->light_cdf[idx] - sc->light_cdf[idx - 1]);
+    return idx;
+}
+
+typedef struct {
+    v3 p0, p1, p2, n0, n1, n2;
+    const ymrt_instance_desc *in;
+    const ymrt_material_desc *mtl;
+} light_tri;
+
+static void load_light(const or_scene *sc, const or_shape_light *l, light_tri *lt)
+{
+    lt->in = &sc->instances[l->instance];
+    lt->mtl = &sc->materials[lt->in->material];
+    const or_mesh *m = &sc->meshes[lt->in->mesh];
+    uint32_t t = l->triangle;
+    lt->p0 = ld3(m->pos, m->pidx[3 * t]);
+    lt->p1 = ld3(m->pos, m->pidx[3 * t + 1]);
+    lt->p2 = ld3(m->pos, m->pidx[3 * t + 2]);
+    if (m->nidx && m->nrm) {
+        lt->n0 = ld3(m->nrm, m->nidx[3 * t]);
+        lt->n1 = ld3(m->nrm, m->nidx[3 * t + 1]);
+        lt->n2 = ld3(m->nrm, m->nidx[3 * t + 2]);
+    } else {
+        v3 g = cross3(sub3(lt->p1, lt->p0), sub3(lt->p2, lt->p0));
+        lt->n0 = g; lt->n1 = g; lt->n2 = g;
+    }
+}
+
+static v3 light_le(const light_tri *lt)
+{
+    return scl3(mk3(lt->mtl->light_color[0], lt->mtl->light_color[1], lt->mtl->light_color[2]), lt->mtl->intensity);
+}
+
+static v3 interp3(v3 a, v3 b, v3 c, v3 w) { return add3(add3(scl3(a, w.x), scl3(b, w.y)), scl3(c, w.z)); }
+
+/* src/Light.h:207-281 (mesh) */
+static int light_sample_li(const or_scene *sc, const or_shape_light *l, v3 pos, float u0, float u1, v3 *wi,
+                           v3 *lpos, v3 *lerr, v3 *lng, v3 *color, float *pdf)
+{
+    light_tri lt;
+    load_light(sc, l, &lt);
+    v3 gn = cross3(sub3(lt.p1, lt.p0), sub3(lt.p2, lt.p0));
+    float su = sqrtf(fmaxr(0.0f, u0));
+    float bu = 1.0f - su, bv = u1 * su;
+    v3 b = mk3(bu, bv, 1.0f - bu - bv);
+    float xs = fabsf(lt.p0.x * b.x) + fabsf(lt.p1.x * b.y) + fabsf(lt.p2.x * b.z);
+    float ys = fabsf(lt.p0.y * b.x) + fabsf(lt.p1.y * b.y) + fabsf(lt.p2.y * b.z);
+    float zs = fabsf(lt.p0.z * b.x) + fabsf(lt.p1.z * b.y) + fabsf(lt.p2.z * b.z);
+    v3 sp = interp3(lt.p0, lt.p1, lt.p2, b);
+    v3 se = sclf3(error_gamma(7), mk3(xs, ys, zs));
+    v3 sn = interp3(lt.n0, lt.n1, lt.n2, b);
+    v3 wn = normalize3(m4_transpose_dir(lt.in->world_to_object, sn));
+    *lpos = transform_position(lt.in->object_to_world, sp, &se);
+    *lerr = se;
+    *lng = normalize3(transpose_transform_vector(lt.in->world_to_object, gn));
+    float dist = length3(sub3(*lpos, pos));
+    *wi = scl3(sub3(*lpos, pos), safe_rcp(dist));
+    float ct = -dot3(wn, *wi);
+    if (ct <= 0.0f) { *pdf = 0.0f; return 0; }
+    *pdf = sqrf(dist) * safe_rcp(l->area * ct);
+    *color = divs3(light_le(&lt), *pdf);
+    return 1;
+}
+
+/* src/Light.h:14-40 with glm 0.9.8's mat3 inverse (func_matrix.inl:272-294) */
+static int ray_tri_light(v3 o, v3 d, v3 p0, v3 p1, v3 p2, v3 *bary)
+{
+    v3 a = sub3(p0, p2), b = sub3(p1, p2);
+    float m[3][3];
+    m[0][0] = a.x; m[0][1] = a.y; m[0][2] = a.z;
+    m[1][0] = b.x; m[1][1] = b.y; m[1][2] = b.z;
+    m[2][0] = -d.x; m[2][1] = -d.y; m[2][2] = -d.z;
+    float ood = 1.0f / (+m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2])
+                        - m[1][0] * (m[0][1] * m[2][2] - m[2][1] * m[0][2])
+                        + m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]));
+    float iv[3][3];
+    iv[0][0] = +(m[1][1] * m[2][2] - m[2][1] * m[1][2]) * ood;
+    iv[1][0] = -(m[1][0] * m[2][2] - m[2][0] * m[1][2]) * ood;
+    iv[2][0] = +(m[1][0] * m[2][1] - m[2][0] * m[1][1]) * ood;
+    iv[0][1] = -(m[0][1] * m[2][2] - m[2][1] * m[0][2]) * ood;
+    iv[1][1] = +(m[0][0] * m[2][2] - m[2][0] * m[0][2]) * ood;
+    iv[2][1] = -(m[0][0] * m[2][1] - m[2][0] * m[0][1]) * ood;
+    iv[0][2] = +(m[0][1] * m[1][2] - m[1][1] * m[0][2]) * ood;
+    iv[1][2] = -(m[0][0] * m[1][2] - m[1][0] * m[0][2]) * ood;
+    iv[2][2] = +(m[0][0] * m[1][1] - m[1][0] * m[0][1]) * ood;
+    v3 r = sub3(o, p2);
+    float u = iv[0][0] * r.x + iv[1][0] * r.y + iv[2][0] * r.z;
+    float v = iv[0][1] * r.x + iv[1][1] * r.y + iv[2][1] * r.z;
+    float t = iv[0][2] * r.x + iv[1][2] * r.y + iv[2][2] * r.z;
+    float w = 1.0f - u - v;
+    if (t < OR_TMIN) return 0;
+    *bary = mk3(u, v, w);
+    return (u > 0.0f && u < 1.0f) && (v > 0.0f && v < 1.0f) && (w > 0.0f && w < 1.0f);
+}
+
+/* src/Light.h:87-162 (mesh) */
+static v3 light_eval_li(const or_scene *sc, const or_shape_light *l, v3 pos, v3 wi, v3 *lpos, v3 *lerr,
+                        v3 *lng, float *pdf)
+{
+    light_tri lt;
+    load_light(sc, l, &lt);
+    v3 gn = cross3(sub3(lt.p1, lt.p0), sub3(lt.p2, lt.p0));
+    v3 owi = transform_vector(lt.in->world_to_object, wi);
+    v3 opos = transform_position(lt.in->world_to_object, pos, NULL);
+    v3 b = s3(0.0f);
+    if (!ray_tri_light(opos, owi, lt.p0, lt.p1, lt.p2, &b)) { *pdf = 0.0f; return s3(0.0f); }
+    float xs = fabsf(lt.p0.x * b.x) + fabsf(lt.p1.x * b.y) + fabsf(lt.p2.x * b.z);
+    float ys = fabsf(lt.p0.y * b.x) + fabsf(lt.p1.y * b.y) + fabsf(lt.p2.y * b.z);
+    float zs = fabsf(lt.p0.z * b.x) + fabsf(lt.p1.z * b.y) + fabsf(lt.p2.z * b.z);
+    v3 sp = interp3(lt.p0, lt.p1, lt.p2, b);
+    v3 se = sclf3(error_gamma(7), mk3(xs, ys, zs));
+    v3 sn = interp3(lt.n0, lt.n1, lt.n2, b);
+    v3 wn = normalize3(m4_transpose_dir(lt.in->world_to_object, sn));
+    float ct = -dot3(wn, wi);
+    if (ct <= 0.0f) { *pdf = 0.0f; return s3(0.0f); }
+    *lpos = transform_position(lt.in->object_to_world, sp, &se);
+    *lerr = se;
+    *lng = normalize3(transpose_transform_vector(lt.in->world_to_object, gn));
+    float dist = length3(sub3(*lpos, pos));
+    *pdf = sqrf(dist) * safe_rcp(l->area * ct);
+    return light_le(&lt);
+}
+
+static float power_heuristic(float main_pdf, float vice_pdf)
+{
+    return safe_rcp(1.0f + sqrf(vice_pdf * safe_rcp(main_pdf)));
+}
+
+static v3 clampv(v3 v) { return min3(v, s3(OR_MAX_COLOR_CLAMP)); }
+
+typedef struct {
+    const or_scene *sc;
+    const ymrt_render_settings *rs;
+    or_sampler *s;
+    rng_ctx *rng;
+    uint64_t *counts;
+} path_ctx;
+
+/* src/Shading.cu:294-417 */
+static v3 eval_shape_light(path_ctx *c, mbsdf *b, v3 ray_d, const or_hit *h, const or_transfer *t,
+                           int nc, const or_nearby *nb, v3 pos, v3 perr, v3 ng)
+{
+    const or_scene *sc = c->sc;
+    if (sc->light_count == 0) return s3(0.0f);
+    v3 direct = s3(0.0f);
+    v3 wo = to_local(b, neg3(ray_d));
+    float sel_pdf = 0.0f;
+    int li = sample_light_power(sc, or_sampler_next(c->s), &sel_pdf);
+    const or_shape_light *l = &sc->lights[li];
+    {
+        v3 ldir = s3(0.0f), lpos = s3(0.0f), lerr = s3(0.0f), lng = s3(0.0f), Li = s3(0.0f);
+        float lpdf = 0.0f;
+        float u0 = or_sampler_next(c->s);
+        float u1 = or_sampler_next(c->s);
+        int valid = light_sample_li(sc, l, pos, u0, u1, &ldir, &lpos, &lerr, &lng, &Li, &lpdf);
+        v3 lwi = to_local(b, ldir);
+        float wpdf = 0.0f;
+        v3 f = bsdf_eval(b, wo, lwi, &wpdf);
+        if (maxcomp3(f) > OR_MIN_COLOR_EPSILON && valid) {
+            v3 so = or_offset_ray_origin(pos, perr, ldir, ng);
+            or_ray sr = make_ray(so, ldir, OR_EMPTY, OR_EMPTY);
+            float maxd = length3(sub3(or_offset_ray_origin(lpos, lerr, neg3(ldir), lng), pos)) * OR_SHADOW_RAY_CLAMP;
+            sr.t = maxd;
+            int transmit = dot3(neg3(ray_d), ng) * dot3(ldir, ng) < 0.0f;
+            int invalid = (!same_hemi(wo, lwi) && !transmit) || (same_hemi(wo, lwi) && transmit);
+            if (!invalid) {
+                if (c->counts) c->counts[2]++;
+                if (!or_trace_shadow(sc, &sr)) {
+                    v3 tr = s3(1.0f);
+                    if (c->rs->enable_volume_scattering) {
+                        or_transfer st = *t;
+                        if (transmit) boundary_bunch(sc, &st, nb, nc);
+                        sr = make_ray(so, ldir, h->inst, h->tri);
+                        sr.t = maxd;
+                        tr = trace_tr(sc, &sr, &st, c->rng, c->counts);
+                    }
+                    float mis = power_heuristic(lpdf, wpdf);
+                    direct = add3(direct, mul3(mul3(sclf3(mis, tr), clampv(f)), clampv(Li)));
+                }
+            }
+        }
+    }
+    {
+        v3 bw = s3(0.0f), bwi = s3(0.0f);
+        float bpdf = 0.0f;
+        float u0 = or_sampler_next(c->s);
+        float u1 = or_sampler_next(c->s);
+        float u2 = or_sampler_next(c->s);
+        if (bsdf_sample(b, wo, &bw, &bwi, &bpdf, u0, u1, u2)) {
+            v3 ldir = to_world(b, bwi);
+            v3 lpos = s3(0.0f), lerr = s3(0.0f), lng = s3(0.0f);
+            float lpdf = 0.0f;
+            v3 Li = light_eval_li(sc, l, pos, ldir, &lpos, &lerr, &lng, &lpdf);
+            if (maxcomp3(Li) > OR_MIN_COLOR_EPSILON) {
+                v3 so = or_offset_ray_origin(pos, perr, ldir, ng);
+                or_ray sr = make_ray(so, ldir, OR_EMPTY, OR_EMPTY);
+                float maxd = length3(sub3(or_offset_ray_origin(lpos, lerr, neg3(ldir), lng), pos)) * OR_SHADOW_RAY_CLAMP;
+                sr.t = maxd;
+                int transmit = dot3(neg3(ray_d), ng) * dot3(ldir, ng) < 0.0f;
+                int invalid = (!same_hemi(wo, bwi) && !transmit) || (same_hemi(wo, bwi) && transmit);
+                if (!invalid) {
+                    if (c->counts) c->counts[2]++;
+                    if (!or_trace_shadow(sc, &sr)) {
+                        v3 tr = s3(1.0f);
+                        if (c->rs->enable_volume_scattering) {
+                            or_transfer st = *t;
+                            if (transmit) boundary_bunch(sc, &st, nb, nc);
+                            sr = make_ray(so, ldir, h->inst, h->tri);
+                            sr.t = maxd;
+                            tr = trace_tr(sc, &sr, &st, c->rng, c->counts);
+                        }
+                        float mis = power_heuristic(bpdf, lpdf);
+                        direct = add3(direct, mul3(mul3(sclf3(mis, tr), clampv(bw)), clampv(Li)));
+                    }
+                }
+            }
+        }
+    }
+    return scl3(direct, safe_rcp(sel_pdf));
+}
+
+/* src/Shading.cu:419-513 */
+static v3 eval_volume_light(path_ctx *c, int n, const float *gs, const float *vw, const or_ray *ray,
+                            const or_transfer *t, v3 vpos)
+{
+    const or_scene *sc = c->sc;
+    if (sc->light_count == 0 || n == 0) return s3(0.0f);
+    v3 direct = s3(0.0f);
+    float sel_pdf = 0.0f;
+    int li = sample_light_power(sc, or_sampler_next(c->s), &sel_pdf);
+    const or_shape_light *l = &sc->lights[li];
+    {
+        v3 ldir = s3(0.0f), lpos = s3(0.0f), lerr = s3(0.0f), lng = s3(0.0f), Li = s3(0.0f);
+        float lpdf = 0.0f;
+        float u0 = or_sampler_next(c->s);
+        float u1 = or_sampler_next(c->s);
+        int valid = light_sample_li(sc, l, vpos, u0, u1, &ldir, &lpos, &lerr, &lng, &Li, &lpdf);
+        float wpdf = 0.0f;
+        float ph = mixed_phase_eval(vw, gs, n, neg3(ray->d), ldir, &wpdf);
+        if (ph > OR_MIN_COLOR_EPSILON && valid) {
+            or_ray sr = make_ray(vpos, ldir, OR_EMPTY, OR_EMPTY);
+            float maxd = length3(sub3(or_offset_ray_origin(lpos, lerr, neg3(ldir), lng), vpos)) * OR_SHADOW_RAY_CLAMP;
+            sr.t = maxd;
+            if (c->counts) c->counts[2]++;
+            if (!or_trace_shadow(sc, &sr)) {
+                or_transfer st = *t;
+                sr = make_ray(vpos, ldir, OR_EMPTY, OR_EMPTY);
+                sr.t = maxd;
+                v3 tr = trace_tr(sc, &sr, &st, c->rng, c->counts);
+                float mis = power_heuristic(lpdf, wpdf);
+                direct = add3(direct, mul3(scl3(sclf3(mis, tr), fminr(ph, OR_MAX_COLOR_CLAMP)), clampv(Li)));
+            }
+        }
+    }
+    {
+        v3 pwi = s3(0.0f);
+        float ppdf = 0.0f, pw = 0.0f;
+        float u0 = or_sampler_next(c->s);
+        float u1 = or_sampler_next(c->s);
+        if (mixed_phase_sample(vw, gs, n, u0, u1, neg3(ray->d), &pw, &pwi, &ppdf)) {
+            v3 ldir = pwi;
+            v3 lpos = s3(0.0f), lerr = s3(0.0f), lng = s3(0.0f);
+            float lpdf = 0.0f;
+            v3 Li = light_eval_li(sc, l, vpos, ldir, &lpos, &lerr, &lng, &lpdf);
+            if (maxcomp3(Li) > OR_MIN_COLOR_EPSILON) {
+                or_ray sr = make_ray(vpos, ldir, OR_EMPTY, OR_EMPTY);
+                float maxd = length3(sub3(or_offset_ray_origin(lpos, lerr, neg3(ldir), lng), vpos)) * OR_SHADOW_RAY_CLAMP;
+                sr.t = maxd;
+                if (c->counts) c->counts[2]++;
+                if (!or_trace_shadow(sc, &sr)) {
+                    or_transfer st = *t;
+                    sr = make_ray(vpos, ldir, OR_EMPTY, OR_EMPTY);
+                    sr.t = maxd;
+                    v3 tr = trace_tr(sc, &sr, &st, c->rng, c->counts);
+                    float mis = power_heuristic(ppdf, lpdf);
+                    direct = add3(direct, mul3(scl3(sclf3(mis, tr), fminr(pw, OR_MAX_COLOR_CLAMP)), clampv(Li)));
+                }
+            }
+        }
+    }
+    return sel_pdf > 0.0f ? divs3(direct, sel_pdf) : s3(0.0f);
+}
+
+static v3 background(v3 d)
+{
+    return mix3(s3(1.0f), mk3(0.05f, 0.2f, 0.85f), d.y * 0.5f + 0.5f);
+}
+
+static v3 light_clamp(const ymrt_render_settings *rs, v3 c, int depth)
+{
+    if (!(rs->enable_indirect_clamp && depth > 1)) return c;
+    if (!(rs->indirect_limit > 0.0f)) return s3(0.0f);
+    float lum = luminance3(c);
+    if (lum <= 1E-10f) return c;
+    float lim = rs->indirect_limit;
+    float pw = fminr(0.05f + 0.25f * powf(lum / lim, 1.5f), 1.0f);
+    float sf = powf(lim / (lim + lum), pw);
+    return scl3(c, sf);
+}
+
+/* FetchMeshShadingData + world transform (src/ShadingUtilities.cuh:713-875) */
+static void fetch_shading(const or_scene *sc, const or_hit *h, v3 *p, v3 *perr, v3 *sn, v3 *ng, v2 *uv,
+                          v3 *dpdu, v3 *dpdv)
+{
+    const ymrt_instance_desc *in = &sc->instances[h->inst];
+    const or_mesh *m = &sc->meshes[in->mesh];
+    uint32_t t = h->tri;
+    v3 p0 = ld3(m->pos, m->pidx[3 * t]), p1 = ld3(m->pos, m->pidx[3 * t + 1]), p2 = ld3(m->pos, m->pidx[3 * t + 2]);
+    v3 b = h->bary;
+    float xs = fabsf(p0.x * b.x) + fabsf(p1.x * b.y) + fabsf(p2.x * b.z);
+    float ys = fabsf(p0.y * b.x) + fabsf(p1.y * b.y) + fabsf(p2.y * b.z);
+    float zs = fabsf(p0.z * b.x) + fabsf(p1.z * b.y) + fabsf(p2.z * b.z);
+    *p = interp3(p0, p1, p2, b);
+    *perr = sclf3(error_gamma(7), mk3(xs, ys, zs));
+    *ng = cross3(sub3(p1, p0), sub3(p2, p0));
+    if (m->nidx && m->nrm) {
+        *sn = interp3(ld3(m->nrm, m->nidx[3 * t]), ld3(m->nrm, m->nidx[3 * t + 1]), ld3(m->nrm, m->nidx[3 * t + 2]), b);
+    } else {
+        *sn = *ng;
+    }
+    int has_uv = m->tidx != NULL && m->uv != NULL;
+    v2 uv0 = has_uv ? mk2(m->uv[2 * m->tidx[3 * t]], m->uv[2 * m->tidx[3 * t] + 1]) : mk2(0.0f, 0.0f);
+    v2 uv1 = has_uv ? mk2(m->uv[2 * m->tidx[3 * t + 1]], m->uv[2 * m->tidx[3 * t + 1] + 1]) : mk2(1.0f, 0.0f);
+    v2 uv2 = has_uv ? mk2(m->uv[2 * m->tidx[3 * t + 2]], m->uv[2 * m->tidx[3 * t + 2] + 1]) : mk2(1.0f, 1.0f);
+    uv->x = uv0.x * b.x + uv1.x * b.y + uv2.x * b.z;
+    uv->y = uv0.y * b.x + uv1.y * b.y + uv2.y * b.z;
+    float du01 = uv1.x - uv0.x, dv01 = uv1.y - uv0.y;
+    float du02 = uv2.x - uv0.x, dv02 = uv2.y - uv0.y;
+    float det = du01 * dv02 - dv01 * du02;
+    if (fabsf(det) < 1E-8f) { du01 = 1.0f; dv01 = 0.0f; du02 = 0.0f; dv02 = 1.0f; det = 1.0f; }
+    float idet = 1.0f / det;
+    v3 dp01 = sub3(p1, p0), dp02 = sub3(p2, p0);
+    *dpdu = scl3(sub3(sclf3(dv02, dp01), sclf3(dv01, dp02)), idet);
+    *dpdv = scl3(add3(sclf3(-du02, dp01), sclf3(du01, dp02)), idet);
+    *p = transform_position(in->object_to_world, *p, perr);
+    *sn = normalize3(transpose_transform_vector(in->world_to_object, *sn));
+    *ng = normalize3(transpose_transform_vector(in->world_to_object, *ng));
+    *dpdu = transform_vector(in->object_to_world, *dpdu);
+    *dpdv = transform_vector(in->object_to_world, *dpdv);
+}
+
+/* ------------------------------------------------------------- path tracer */
+v3 or_trace_path(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t px, uint32_t py,
+                 uint32_t sample, uint64_t *counts)
+{
+    /* ray generation: src/Shading.cu:1634-1725 */
+    or_sampler s;
+    memset(&s, 0, sizeof(s));
+    v2 off;
+    uint32_t pixel_index = py * rs->width + px;
+    if (rs->sampler_type == OR_SAMPLER_HALTON) {
+        or_enumerator e;
+        or_enum_init(&e, (int)rs->width, (int)rs->height);
+        uint64_t idx = or_enum_index(&e, (int)px, (int)py, (frame - 1) * rs->spp + (int)sample);
+        s.type = OR_SAMPLER_HALTON; s.index = idx; s.dim = 2; s.table = sc->halton;
+        float ox = or_radical_inverse2(idx), oy = or_radical_inverse3(idx);
+        ox = fmaf(ox, e.x_scale, -(float)px);
+        oy = fmaf(oy, e.y_scale, -(float)py);
+        off.x = clampf(ox, 0.000001f, 0.999999f);
+        off.y = clampf(oy, 0.000001f, 0.999999f);
+    } else {
+        s.type = OR_SAMPLER_PCG; s.pixel = pixel_index; s.sample = sample; s.dim = 0; s.frame = (uint32_t)(frame - 1);
+        off.x = or_sampler_next(&s);
+        off.y = or_sampler_next(&s);
+    }
+    rng_ctx rng;
+    rng.s = &s;
+    rng.hash_key[0] = pixel_index; rng.hash_key[1] = 0; rng.hash_key[2] = sample; rng.hash_key[3] = (uint32_t)frame;
+    rng.hash_ctr = 0;
+    path_ctx c = {sc, rs, &s, &rng, counts};
+
+    or_ray ray = camera_ray(sc, ((float)px + off.x) / (float)rs->width, ((float)py + off.y) / (float)rs->height);
+    v3 L = s3(0.0f), beta = s3(1.0f);
+    int depth = 1, never_scatter = 1;
+    or_transfer t = sc->camera_transfer;
+    uint32_t vertex = 0;
+
+    for (;;) {
+        /* ---- trace stage */
+        if (counts) { counts[1]++; if (depth == 1 && never_scatter) counts[0]++; }
+        or_hit h;
+        memset(&h, 0, sizeof(h));
+        h.bary = s3(1.0f); h.t = OR_TMAX; h.inst = OR_EMPTY; h.tri = OR_EMPTY;
+        or_nearby nb[OR_MAX_NEARBY];
+        int nc = 0;
+        or_ray tr_ray = ray;
+        int hit = or_trace_ray(sc, &tr_ray, &h, &nc, nb);
+        float vw[OR_MAX_BOUNDARY_RECORD];
+        int vi[OR_MAX_BOUNDARY_RECORD];
+        float gs[OR_MAX_BOUNDARY_RECORD];
+        int nv = 0;
+        if (rs->enable_volume_scattering) {
+            v3 trw = s3(1.0f);
+            float dist = 0.0f;
+            nv = tr_volumes(&t, vi, sc->volume_count);
+            for (int i = 0; i < nv; ++i) gs[i] = sc->volumes[vi[i]].g;
+            int hv = sample_volume(sc, &ray, nv, vi, hit ? h.t : OR_TMAX, &trw, &dist, vw, &s);
+            hit = hit || hv;
+            beta = mul3(beta, trw);
+            if (hv) { h.volume = 1; h.bary = add3(ray.o, scl3(ray.d, dist)); }
+        }
+        if (!hit) {
+            if (rs->enable_env_light) L = add3(L, light_clamp(rs, mul3(beta, background(ray.d)), depth));
+            return L;
+        }
+        /* ---- shading stage */
+        rng.hash_key[1] = ++vertex;
+        rng.hash_ctr = 0;
+        if (rs->enable_volume_scattering && h.volume) {
+            v3 vpos = h.bary;
+            if (rs->enable_shape_light)
+                L = add3(L, light_clamp(rs, mul3(beta, eval_volume_light(&c, nv, gs, vw, &ray, &t, vpos)), depth));
+            v3 wi = s3(0.0f);
+            float pdf = 0.0f, pw = 0.0f;
+            float u0 = or_sampler_next(&s);
+            float u1 = or_sampler_next(&s);
+            if (!mixed_phase_sample(vw, gs, nv, u0, u1, neg3(ray.d), &pw, &wi, &pdf)) return L;
+            beta = scl3(beta, pw);
+            float rr = fminr(fmaxr(beta.x, fmaxr(beta.y, beta.z)), 1.0f);
+            if (depth + 1 > rs->ray_depth) {
+                if (rs->enable_russian_roulette && or_sampler_next(&s) < rr) beta = divs3(beta, fmaxr(rr, OR_MIN_COLOR_EPSILON));
+                else return L;
+            }
+            ray = make_ray(vpos, wi, OR_EMPTY, OR_EMPTY);
+            never_scatter = 0;
+            depth += 1;
+            if (depth > OR_MAX_RAY_DEPTH) return L;
+            continue;
+        }
+        v3 p, perr, sn, ng, dpdu, dpdv;
+        v2 uv;
+        fetch_shading(sc, &h, &p, &perr, &sn, &ng, &uv, &dpdu, &dpdv);
+        const ymrt_instance_desc *in = &sc->instances[h.inst];
+        if (!(in->material < sc->material_count)) return L;
+        const ymrt_material_desc *m = &sc->materials[in->material];
+        uint32_t prio;
+        fetch_ior(m, &prio);
+        if ((t.count != 0 && prio < t.rec[t.max_idx].priority) || in->treat_as_boundary) {
+            boundary_bunch(sc, &t, nb, nc);
+            ray = make_ray(or_offset_ray_origin(p, perr, ray.d, ng), ray.d, h.inst, h.tri);
+            continue;
+        }
+        if (m->type == YMRT_MATERIAL_LIGHT) {
+            if (never_scatter)
+                L = add3(L, light_clamp(rs, mul3(beta, scl3(mk3(m->light_color[0], m->light_color[1], m->light_color[2]), m->intensity)), depth));
+            return L;
+        }
+        mbsdf b;
+        memset(&b, 0, sizeof(b));
+        init_shading_space(&b, h.back, ray.d, ng, sn, dpdu, dpdv);
+        b.is_default = m->type == YMRT_MATERIAL_DEFAULT;
+        if (b.is_default) init_default_bsdf(sc, &b, m, uv, ray.d, h.back, tr_ray_ior(&t), tr_ex_ior(&t, h.inst));
+        if (rs->enable_shape_light)
+            L = add3(L, light_clamp(rs, mul3(beta, eval_shape_light(&c, &b, ray.d, &h, &t, nc, nb, p, perr, ng)), depth));
+        v3 wo = to_local(&b, neg3(ray.d)), wi = s3(0.0f), bw = s3(0.0f);
+        float pdf = 0.0f;
+        float u0 = or_sampler_next(&s);
+        float u1 = or_sampler_next(&s);
+        float u2 = or_sampler_next(&s);
+        if (!bsdf_sample(&b, wo, &bw, &wi, &pdf, u0, u1, u2)) return L;
+        beta = mul3(beta, bw);
+        float rr = fminr(fmaxr(beta.x, fmaxr(beta.y, beta.z)), 1.0f);
+        if (depth + 1 > rs->ray_depth) {
+            if (rs->enable_russian_roulette && or_sampler_next(&s) < rr) beta = divs3(beta, fmaxr(rr, OR_MIN_COLOR_EPSILON));
+            else return L;
+        }
+        v3 nd = to_world(&b, wi);
+        int transmit = dot3(neg3(ray.d), ng) * dot3(nd, ng) < 0.0f;
+        int invalid = (!same_hemi(wo, wi) && !transmit) || (same_hemi(wo, wi) && transmit);
+        if (invalid) return L;
+        if (transmit) boundary_bunch(sc, &t, nb, nc);
+        ray = make_ray(or_offset_ray_origin(p, perr, nd, ng), nd, h.inst, h.tri);
+        never_scatter = 0;
+        depth += 1;
+        if (depth > OR_MAX_RAY_DEPTH) return L;
+    }
+}
+
+void or_render_rows(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t row0, uint32_t row1,
+                    float *out, uint64_t *counts)
+{
+    float inv_spp = 1.0f / (float)(rs->spp > 1 ? rs->spp : 1);
+    for (uint32_t y = row0; y < row1 && y < rs->height; ++y) {
+        for (uint32_t x = 0; x < rs->width; ++x) {
+            float acc[4] = {0, 0, 0, 0};
+            for (int smp = 0; smp < rs->spp; ++smp) {
+                v3 L = or_trace_path(sc, rs, frame, x, y, (uint32_t)smp, counts);
+                acc[0] += L.x * inv_spp;
+                acc[1] += L.y * inv_spp;
+                acc[2] += L.z * inv_spp;
+                acc[3] += inv_spp;
+            }
+            float *o = out + 4 * ((size_t)y * rs->width + x);
+            o[0] = acc[0]; o[1] = acc[1]; o[2] = acc[2]; o[3] = acc[3];
+            if (counts) counts[4]++;
+        }
+    }
+}
+
+/* src/Shading.cu:2354-2373 */
+void or_accumulate(const float *noise, float *accum, uint32_t w, uint32_t h, int frame)
+{
+    size_t n = (size_t)w * h * 4;
+    for (size_t i = 0; i < n; ++i) {
+        if (frame == 1) accum[i] = noise[i];
+        else accum[i] = (accum[i] * (float)(frame - 1) + noise[i]) / (float)frame;
+    }
+}
+
+/* src/Shading.cu:2396-2425 */
+void or_tonemap(float *rgba, uint32_t w, uint32_t h, float gamma, float exposure)
+{
+    size_t n = (size_t)w * h;
+    for (size_t i = 0; i < n; ++i) {
+        float *p = rgba + 4 * i;
+        for (int c = 0; c < 3; ++c) {
+            float x = p[c] * exposure;
+            float y = (x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f);
+            p[c] = powf(y, 1.0f / gamma);
+        }
+        p[3] = 1.0f;
+    }
+}
+
+/* ------------------------------------------------- probes for closed-form tests */
+float or_probe_ggx_d(float hx, float hy, float hz, float ax, float ay) { return ggx_d(mk3(hx, hy, hz), ax, ay); }
+float or_probe_fresnel_dd(float eta, float c) { return fresnel_dd(eta, c); }
+void or_probe_fresnel_dc(const float *eta, const float *k, float c, float *out)
+{
+    v3 r = fresnel_dc(mk3(eta[0], eta[1], eta[2]), mk3(k[0], k[1], k[2]), c);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+void or_probe_edge_tint(const float *refl, const float *tint, float *n, float *k)
+{
+    v3 nn, kk;
+    edge_tint(mk3(refl[0], refl[1], refl[2]), mk3(tint[0], tint[1], tint[2]), &nn, &kk);
+    n[0] = nn.x; n[1] = nn.y; n[2] = nn.z;
+    k[0] = kk.x; k[1] = kk.y; k[2] = kk.z;
+}
+/* SampleTrHomogeneous with one volume and a fixed uniform u: distance and weight */
+int or_probe_free_flight(const float *sigma_t, const float *albedo, float tmax, float u, float *dist, float *w)
+{
+    or_scene sc;
+    memset(&sc, 0, sizeof(sc));
+    ymrt_volume_desc v;
+    memset(&v, 0, sizeof(v));
+    for (int i = 0; i < 3; ++i) { v.sigma_t[i] = sigma_t[i]; v.albedo[i] = albedo[i]; }
+    v.density_grid = -1;
+    sc.volumes = &v;
+    sc.volume_count = 1;
+    /* a one-value sampler: Halton index 0 of a table whose prime-2 permutation
+     * maps digit 0 to 0 returns the clamp 1e-6; instead feed u through PCG-free
+     * path by overriding the sampler below */
+    or_sampler s;
+    memset(&s, 0, sizeof(s));
+    s.type = -1;  /* see or_sampler_next: non-Halton, handled below */
+    int vi = 0;
+    v3 trw;
+    float vw[OR_MAX_BOUNDARY_RECORD];
+    g_probe_u = u;
+    int r = sample_homogeneous(&sc, 1, &vi, tmax, &trw, dist, vw, &s);
+    w[0] = trw.x; w[1] = trw.y; w[2] = trw.z;
+    return r;
+}

@@ -1,0 +1,133 @@
+"""Pins the oracle's ray queries (the checker for the HIP traversal).
+
+  * the watertight triangle test against an FP64 Moller-Trumbore restatement
+    on random rays and triangles (hit/miss away from edges, t within 1e-5);
+  * watertightness: rays from inside a closed mesh always hit it;
+  * BVH closest hit == brute force over every triangle (same ids and t bits;
+    exact-t ties excepted, which the reference also resolves by visit order);
+  * any hit <=> brute force finds a hit in [tmin, tmax] (boundary instances
+    skipped).
+"""
+import numpy as np
+
+import oracle
+from ymrt_amd import abi, scenes
+
+
+def moller_trumbore(o, d, p0, p1, p2):
+    e1, e2 = p1 - p0, p2 - p0
+    pv = np.cross(d, e2)
+    det = e1 @ pv
+    if abs(det) < 1e-300:
+        return None
+    inv = 1.0 / det
+    tv = o - p0
+    u = (tv @ pv) * inv
+    qv = np.cross(tv, e1)
+    v = (d @ qv) * inv
+    t = (e2 @ qv) * inv
+    return t, u, v
+
+
+def single_tri_scene(p0, p1, p2):
+    sa = abi.SceneArrays()
+    m = sa.add_mesh(np.array([p0, p1, p2], np.float32), np.array([[0, 1, 2]], np.uint32))
+    sa.add_instance(m, sa.add_material(abi.default_material()))
+    return sa
+
+
+def test_triangle_test_vs_fp64():
+    rng = np.random.RandomState(5)
+    agree = total = 0
+    for k in range(60):
+        tri = rng.uniform(-1, 1, (3, 3)).astype(np.float32)
+        sc = oracle.Scene(single_tri_scene(*tri))
+        n = 200
+        rays = np.zeros(n, abi.RAY_DTYPE)
+        o = rng.uniform(-2, 2, (n, 3))
+        tgt = tri.mean(0) + rng.uniform(-0.6, 0.6, (n, 3))
+        d = tgt - o
+        d /= np.linalg.norm(d, axis=1)[:, None]
+        rays["o"], rays["d"], rays["tmin"], rays["tmax"] = o, d, 1e-7, 1e36
+        rays = rays.astype(abi.RAY_DTYPE)
+        hits = sc.trace_closest(rays)
+        for i in range(n):
+            r = moller_trumbore(rays["o"][i].astype(np.float64), rays["d"][i].astype(np.float64),
+                                *tri.astype(np.float64))
+            inside = r is not None and r[0] >= 1e-7 and r[1] > 1e-5 and r[2] > 1e-5 and r[1] + r[2] < 1 - 1e-5
+            outside = r is None or r[0] < 0 or r[1] < -1e-5 or r[2] < -1e-5 or r[1] + r[2] > 1 + 1e-5
+            got = hits["instance"][i] != 0xFFFFFFFF
+            if inside:
+                total += 1
+                assert got, (k, i)
+                nrm = np.cross(tri[1] - tri[0], tri[2] - tri[0]).astype(np.float64)
+                cosang = abs(nrm @ rays["d"][i]) / np.linalg.norm(nrm)
+                # FP32 conditioning: the error grows as the ray grazes the plane
+                assert abs(hits["t"][i] - r[0]) <= 2e-5 * max(1.0, r[0]) / max(cosang, 0.05)
+                agree += 1
+            elif outside:
+                assert not got, (k, i)
+    assert total > 2000
+
+
+def test_watertight_closed_sphere():
+    sa = abi.SceneArrays()
+    pos, idx, nrm, uv = scenes.uv_sphere((0.0, 0.0, 0.0), 1.0, 48, 24)
+    sa.add_instance(sa.add_mesh(pos, idx), sa.add_material(abi.default_material()))
+    sc = oracle.Scene(sa)
+    rng = np.random.RandomState(1)
+    n = 20000
+    rays = np.zeros(n, abi.RAY_DTYPE)
+    rays["o"] = rng.uniform(-0.3, 0.3, (n, 3))
+    d = rng.normal(size=(n, 3))
+    rays["d"] = d / np.linalg.norm(d, axis=1)[:, None]
+    rays["tmin"], rays["tmax"] = 1e-7, 1e36
+    # also rays aimed exactly at vertices and edge midpoints
+    hits = sc.trace_closest(rays)
+    assert (hits["instance"] != 0xFFFFFFFF).all()
+    v = pos[idx[::7, 0]].astype(np.float64)
+    m = 0.5 * (pos[idx[::7, 0]].astype(np.float64) + pos[idx[::7, 1]].astype(np.float64))
+    tg = np.concatenate([v, m])
+    r2 = np.zeros(len(tg), abi.RAY_DTYPE)
+    r2["d"] = tg / np.linalg.norm(tg, axis=1)[:, None]
+    r2["tmin"], r2["tmax"] = 1e-7, 1e36
+    # The triangle test itself is watertight through vertices and edges ...
+    h2 = sc.trace_closest(r2, brute=True)
+    assert (h2["instance"] != 0xFFFFFFFF).all()
+    # ... while the reference's slab test (src/BoundingBox.h:119-155) pads the
+    # far distance only on axes it swapped, so a ray leaving a leaf box exactly
+    # through its corner can be culled: the restated BVH walk reproduces that.
+    h3 = sc.trace_closest(r2)
+    assert (h3["instance"] != 0xFFFFFFFF).mean() > 0.9
+
+
+def test_bvh_closest_equals_brute_force():
+    sa = scenes.cornell_box()
+    sc = oracle.Scene(sa)
+    closest, _ = scenes.stress_rays(sa, 3000, 12, 13, 14)
+    a = sc.trace_closest(closest)
+    b = sc.trace_closest(closest, brute=True)
+    same_t = a["t"] == b["t"]
+    assert same_t.all()
+    ids = (a["instance"] == b["instance"]) & (a["triangle"] == b["triangle"])
+    # every id mismatch is an exact tie in t
+    assert ids.mean() > 0.99
+    hf = scenes.heightfield(41)
+    sc2 = oracle.Scene(hf)
+    c2, _ = scenes.stress_rays(hf, 3000, 22, 23, 24)
+    a2 = sc2.trace_closest(c2)
+    b2 = sc2.trace_closest(c2, brute=True)
+    assert (a2["t"] == b2["t"]).all()
+    assert ((a2["triangle"] == b2["triangle"]) | (a2["t"] == b2["t"])).all()
+
+
+def test_any_hit_equals_brute_force():
+    sa = scenes.cornell_box()
+    sc = oracle.Scene(sa)
+    _, shadow = scenes.stress_rays(sa, 3000, 32, 33, 34)
+    occ = sc.trace_any(shadow)
+    # brute: closest hit within the segment
+    c = shadow.copy()
+    bh = sc.trace_closest(c, brute=True)
+    brute = (bh["instance"] != 0xFFFFFFFF) & (bh["t"] <= shadow["tmax"])
+    assert (occ.astype(bool) == brute).all()

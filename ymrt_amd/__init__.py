@@ -1,0 +1,203 @@
+"""ymrt_amd -- MI355X (gfx950) wavefront path tracer behind YMRT's scene API.
+
+Python binding over the C ABI of ``lib/libymrt_hip.so`` (declared in
+include/ymrt.h).  Device buffers are torch tensors on ``cuda`` (plumbing only:
+every kernel is the library's own HIP code).  There is no CPU fallback: when
+the library is missing, ``lib()`` raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libymrt_hip.so")
+CSRC = os.path.join(_HERE, "csrc")
+
+# Every symbol include/ymrt.h declares.
+EXPORTS = [
+    "ymrt_last_error", "ymrt_abi_version", "ymrt_context_create", "ymrt_context_destroy",
+    "ymrt_scene_create", "ymrt_scene_destroy", "ymrt_scene_get_info", "ymrt_trace_closest",
+    "ymrt_trace_any", "ymrt_trace_closest_counted", "ymrt_render_frame", "ymrt_accumulate",
+    "ymrt_tonemap", "ymrt_halton_values", "ymrt_halton_enumerate",
+]
+
+_lib = None
+
+
+class YmrtError(RuntimeError):
+    pass
+
+
+def build(jobs=8):
+    import subprocess
+    subprocess.check_call(["make", "-s", "-j%d" % jobs, "-C", CSRC])
+
+
+def lib():
+    """The loaded C-ABI library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise YmrtError("libymrt_hip.so is not built (run __graft_entry__.build() or make -C %s)" % CSRC)
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.ymrt_last_error.restype = C.c_char_p
+        L.ymrt_abi_version.restype = C.c_int
+        L.ymrt_context_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.ymrt_context_destroy.argtypes = [vp]
+        L.ymrt_scene_create.argtypes = [vp, C.POINTER(abi.SceneDesc), C.POINTER(vp)]
+        L.ymrt_scene_destroy.argtypes = [vp]
+        L.ymrt_scene_get_info.argtypes = [vp, C.POINTER(abi.SceneInfo)]
+        L.ymrt_trace_closest.argtypes = [vp, vp, C.c_uint32, vp, vp]
+        L.ymrt_trace_any.argtypes = [vp, vp, C.c_uint32, vp, vp]
+        L.ymrt_trace_closest_counted.argtypes = [vp, vp, C.c_uint32, vp, vp, vp]
+        L.ymrt_render_frame.argtypes = [vp, C.POINTER(abi.RenderSettings), C.c_int, vp, vp, C.c_int,
+                                        C.POINTER(abi.FrameStats)]
+        L.ymrt_accumulate.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_int, C.c_int, vp]
+        L.ymrt_tonemap.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_float, C.c_float, vp]
+        L.ymrt_halton_values.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp]
+        L.ymrt_halton_enumerate.argtypes = [C.c_uint32, C.c_uint32, vp, vp, vp, C.c_uint32, vp, vp]
+        for name in EXPORTS:
+            if name not in ("ymrt_last_error", "ymrt_abi_version"):
+                getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise YmrtError("ymrt error %d: %s" % (rc, lib().ymrt_last_error().decode()))
+
+
+def _dev_ptr(t):
+    import torch
+    assert t.is_cuda and t.is_contiguous(), "expected a contiguous cuda tensor"
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Context:
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        _check(lib().ymrt_context_create(device, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().ymrt_context_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """Device-resident scene (ymrt_scene_create) built from abi.SceneArrays."""
+
+    def __init__(self, ctx, arrays):
+        self.ctx = ctx
+        self.arrays = arrays
+        self._desc = arrays.desc()
+        self.h = C.c_void_p()
+        _check(lib().ymrt_scene_create(ctx.h, C.byref(self._desc), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().ymrt_scene_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        i = abi.SceneInfo()
+        _check(lib().ymrt_scene_get_info(self.h, C.byref(i)))
+        return i
+
+    # ----------------------------------------------------------- ray queries
+    @staticmethod
+    def rays_to_device(rays):
+        import torch
+        a = np.ascontiguousarray(rays).view(np.float32).reshape(-1, 8)
+        return torch.from_numpy(a.copy()).cuda()
+
+    def trace_closest(self, rays_dev, n=None, counted=False):
+        """rays_dev: (n, 8) float32 cuda tensor; returns (n, 4) int32 cuda tensor
+        of (t bits, triangle, instance, back) and, if counted, (n, 2) visits."""
+        import torch
+        n = rays_dev.shape[0] if n is None else n
+        hits = torch.empty((n, 4), dtype=torch.int32, device=rays_dev.device)
+        if counted:
+            visits = torch.empty((n, 2), dtype=torch.int32, device=rays_dev.device)
+            _check(lib().ymrt_trace_closest_counted(self.h, _dev_ptr(rays_dev), n, _dev_ptr(hits),
+                                                    _dev_ptr(visits), _stream()))
+            return hits, visits
+        _check(lib().ymrt_trace_closest(self.h, _dev_ptr(rays_dev), n, _dev_ptr(hits), _stream()))
+        return hits
+
+    def trace_any(self, rays_dev, n=None):
+        import torch
+        n = rays_dev.shape[0] if n is None else n
+        occ = torch.empty((n,), dtype=torch.int32, device=rays_dev.device)
+        _check(lib().ymrt_trace_any(self.h, _dev_ptr(rays_dev), n, _dev_ptr(occ), _stream()))
+        return occ
+
+    @staticmethod
+    def hits_to_numpy(hits):
+        h = hits.cpu().numpy().astype(np.int32)
+        out = np.zeros(len(h), abi.HIT_DTYPE)
+        out["t"] = h[:, 0].view(np.float32)
+        out["triangle"] = h[:, 1].view(np.uint32)
+        out["instance"] = h[:, 2].view(np.uint32)
+        out["back"] = h[:, 3].view(np.uint32)
+        return out
+
+    # -------------------------------------------------------------- sampler
+    def halton_values(self, indices, dim0, dims):
+        import torch
+        idx = torch.from_numpy(np.ascontiguousarray(indices, np.int64)).cuda()
+        out = torch.empty((len(indices), dims), dtype=torch.float32, device="cuda")
+        _check(lib().ymrt_halton_values(self.h, _dev_ptr(idx), len(indices), dim0, dims, _dev_ptr(out), _stream()))
+        return out.cpu().numpy()
+
+    # ------------------------------------------------------------ rendering
+    def render_frame(self, settings, frame=1, noise=None, timing=False):
+        import torch
+        if noise is None:
+            noise = torch.empty((settings.height, settings.width, 4), dtype=torch.float32, device="cuda")
+        stats = abi.FrameStats()
+        _check(lib().ymrt_render_frame(self.h, C.byref(settings), frame, _dev_ptr(noise), _stream(),
+                                       1 if timing else 0, C.byref(stats)))
+        return noise, stats
+
+
+def halton_enumerate(width, height, px, py, sample):
+    import torch
+    n = len(px)
+    t = [torch.from_numpy(np.ascontiguousarray(a, np.int32)).cuda() for a in (px, py, sample)]
+    out = torch.empty((n,), dtype=torch.int64, device="cuda")
+    _check(lib().ymrt_halton_enumerate(width, height, _dev_ptr(t[0]), _dev_ptr(t[1]), _dev_ptr(t[2]), n,
+                                       _dev_ptr(out), _stream()))
+    return out.cpu().numpy().view(np.uint64)
+
+
+def accumulate(noise, accum, frame, max_frames=-1):
+    h, w, _ = noise.shape
+    _check(lib().ymrt_accumulate(_dev_ptr(noise), _dev_ptr(accum), w, h, frame, max_frames, _stream()))
+
+
+def tonemap(img, gamma=2.2, exposure=1.0):
+    h, w, _ = img.shape
+    _check(lib().ymrt_tonemap(_dev_ptr(img), w, h, gamma, exposure, _stream()))

@@ -1,0 +1,145 @@
+// dev_scene.hpp -- HBM layout of a scene as the gfx950 kernels read it.
+//
+// The reference keeps a binary BVH per mesh (32-B nodes, leaf size 1,
+// src/BottomBVH.h:22-30) and fetches triangle corners through two levels of
+// index indirection (src/ShadingUtilities.cuh:62-68).  Here:
+//   * one array of 4-wide nodes (WNode, 128 B = one L2 line) holds every
+//     bottom-level tree and the top-level tree;
+//   * traversal triangles are pre-gathered in leaf order, 48 B each (three
+//     float4: corner xyz + original triangle index in v0.w);
+//   * shading triangles (positions, normals, uvs of the three corners) are
+//     pre-gathered by original triangle index, 96 B each, read once per hit.
+#pragma once
+
+#include "dev_math.hpp"
+
+namespace ym {
+
+// Child encoding in WNode::child and on the traversal stack (32 bits):
+//   bit 31 clear        internal node index
+//   bit 31 set          leaf: bits 30..3 = first primitive, bits 2..0 = count-1
+//   0xFFFFFFFF          instance sentinel on the stack / empty child
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kSentinel = 0xFFFFFFFFu;
+constexpr uint32_t kMaxLeafPrims = 8;
+
+YM_HD uint32_t leaf_code(uint32_t first, uint32_t count) { return kLeafBit | (first << 3) | (count - 1u); }
+YM_HD uint32_t leaf_first(uint32_t code) { return (code & 0x7FFFFFFFu) >> 3; }
+YM_HD uint32_t leaf_count(uint32_t code) { return (code & 7u) + 1u; }
+
+struct alignas(128) WNode {
+    float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
+    uint32_t child[4];
+    uint32_t pad[4];
+};
+static_assert(sizeof(WNode) == 128, "WNode must be one 128-B line");
+
+struct alignas(16) DevInstance {  // 128 B
+    Affine w2o;        // world -> object
+    Affine o2w;        // object -> world
+    uint32_t blas_root;
+    uint32_t mesh;
+    uint32_t material;
+    int32_t inner_volume;
+    uint32_t flags;    // bit 0: treat_as_boundary
+    uint32_t shade_base;   // first shading triangle of the mesh
+    uint32_t tri_count;
+    uint32_t pad;
+};
+static_assert(sizeof(DevInstance) == 128, "DevInstance size");
+
+constexpr uint32_t kInstBoundary = 1u;
+constexpr uint32_t kInstNoNormals = 2u;   // mesh without normals: shade with the face normal
+
+struct alignas(16) ShadeTri {      // 96 B, by original triangle index
+    float p[9];                    // p0, p1, p2
+    float n[9];                    // n0, n1, n2 (face normal repeated when the mesh has none)
+    float uv[6];                   // uv0, uv1, uv2 (reference defaults when absent)
+};
+static_assert(sizeof(ShadeTri) == 96, "ShadeTri size");
+
+struct DevMaterial {               // ymrt_material_desc, 128 B
+    uint32_t type;
+    float diffuse_albedo[3]; uint32_t diffuse_albedo_tex;
+    float specular_albedo[3]; uint32_t specular_albedo_tex;
+    float roughness_x; uint32_t roughness_x_tex;
+    float roughness_y; uint32_t roughness_y_tex;
+    float specular_weight; uint32_t specular_weight_tex;
+    float metalness; uint32_t metalness_tex;
+    float transmission_weight; uint32_t transmission_weight_tex;
+    float ior_n; uint32_t ior_priority;
+    float coat_albedo[3]; uint32_t coat_albedo_tex;
+    float coat_weight; uint32_t coat_weight_tex;
+    float coat_thickness; uint32_t coat_thickness_tex;
+    float coat_roughness_x; uint32_t coat_roughness_x_tex;
+    float coat_roughness_y; uint32_t coat_roughness_y_tex;
+    float coat_ior;
+    float light_color[3];
+    float intensity;
+};
+
+struct DevTexture {
+    uint32_t width, height, channels;
+    int32_t wrap, color_space;
+    float su, sv, ou, ov;
+    uint64_t offset;               // first texel (uint32 RGBA8 words) in the texel pool
+};
+
+struct DevGrid {
+    uint32_t nx, ny, nz;
+    float max_density;
+    uint64_t offset;               // first voxel in the density pool
+};
+
+struct DevVolume {
+    float sigma_t[3];
+    float albedo[3];
+    float g;
+    int32_t grid;
+    Affine w2o;
+};
+
+struct DevLight {
+    uint32_t instance, triangle;
+    float power, area;
+};
+
+struct CameraRecord {
+    uint32_t prim;
+    float ior;
+    uint32_t priority;
+    int32_t vol;
+};
+
+struct DevCamera {
+    f3 position, u, v, w;
+    float tan_half_fov, aspect;
+    int32_t record_count, max_idx;
+    CameraRecord rec[kMaxBoundaryRecord];
+};
+
+// Pointers and counts handed to every kernel by value.
+struct DevScene {
+    const WNode *nodes;
+    const float4 *tris;            // 3 float4 per traversal triangle
+    const DevInstance *instances;
+    const ShadeTri *shade;
+    const DevMaterial *materials;
+    const DevTexture *textures;
+    const uint32_t *texels;
+    const float *tex_lut;          // 2 x 256 decode tables (sRGB, linear)
+    const DevGrid *grids;
+    const float *voxels;
+    const DevVolume *volumes;
+    const DevLight *lights;
+    const float *light_cdf;
+    const uint32_t *halton;        // permutation table
+    const uint32_t *primes;        // first 1000 primes
+    const uint32_t *prime_sums;    // offset of each prime's permutation
+    uint32_t tlas_root;            // node index, or leaf code for a single instance
+    uint32_t instance_count, material_count, texture_count, grid_count, volume_count, light_count;
+    uint32_t has_boundaries;       // any instance with treat_as_boundary
+    DevCamera camera;
+};
+
+}  // namespace ym

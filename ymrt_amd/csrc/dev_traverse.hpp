@@ -1,0 +1,297 @@
+// dev_traverse.hpp -- two-level wide-BVH traversal for gfx950.
+//
+// Restates the queries of src/ShadingUtilities.cuh:
+//   closest hit  TraceRay            (310-431) incl. the +-2 ulp nearby-hit list
+//   any hit      BVHTraverseShadow   (563-635), boundary instances skipped
+//   boundaries   TraceVolume         (434-559), only boundary instances
+// over a 4-wide tree (one 128-B node per visit, children visited nearest
+// first, far children pushed on a per-lane stack in LDS laid out
+// [slot][lane] so the 64 lanes of a wave hit 64 distinct banks).  The
+// triangle test is the reference's watertight shear test
+// (src/ShadingUtilities.cuh:54-134) evaluated in the same operation order with
+// contraction off, so t and barycentrics are the reference's floats.  Box
+// tests use (b - o) * (1/d) like src/BoundingBox.h:119-155 and scale every
+// far distance by 1 + 2*gamma(3), which keeps them conservative.
+#pragma once
+
+#include "dev_scene.hpp"
+
+namespace ym {
+
+enum : int { kModeAll = 0, kModeSkipBoundary = 1, kModeBoundaryOnly = 2 };
+constexpr int kNearbyCap = 4;
+
+struct RayPre {
+    f3 o, d, inv;
+    int kx, ky, kz;
+    float sx, sy, sz;
+};
+
+YM_HD RayPre make_pre(f3 o, f3 d)
+{
+    RayPre r;
+    r.o = o;
+    r.d = d;
+    r.inv = f3{1.f / d.x, 1.f / d.y, 1.f / d.z};
+    f3 a = abs3(d);
+    int kz = 0;
+    if (a.y > comp(a, kz)) kz = 1;
+    if (a.z > comp(a, kz)) kz = 2;
+    int kx = kz + 1;
+    if (kx == 3) kx = 0;
+    int ky = kx + 1;
+    if (ky == 3) ky = 0;
+    r.kx = kx;
+    r.ky = ky;
+    r.kz = kz;
+    float dx = comp(d, kx), dy = comp(d, ky), dz = comp(d, kz);
+    r.sx = -dx / dz;
+    r.sy = -dy / dz;
+    r.sz = 1.f / dz;
+    return r;
+}
+
+struct TriHit {
+    float t, e0, e1, e2, inv;
+};
+
+// Returns true when the sheared plane crossing lies inside the triangle and
+// the denominator is non-zero; t is not range-checked.
+YM_HD bool tri_test(const RayPre &r, f3 q0, f3 q1, f3 q2, TriHit &h)
+{
+    f3 p0 = q0 - r.o, p1 = q1 - r.o, p2 = q2 - r.o;
+    f3 a{comp(p0, r.kx), comp(p0, r.ky), comp(p0, r.kz)};
+    f3 b{comp(p1, r.kx), comp(p1, r.ky), comp(p1, r.kz)};
+    f3 c{comp(p2, r.kx), comp(p2, r.ky), comp(p2, r.kz)};
+    a.x += r.sx * a.z;
+    a.y += r.sy * a.z;
+    b.x += r.sx * b.z;
+    b.y += r.sy * b.z;
+    c.x += r.sx * c.z;
+    c.y += r.sy * c.z;
+    float e0 = b.x * c.y - b.y * c.x;
+    float e1 = c.x * a.y - c.y * a.x;
+    float e2 = a.x * b.y - a.y * b.x;
+    if ((e0 < 0 || e1 < 0 || e2 < 0) && (e0 > 0 || e1 > 0 || e2 > 0)) return false;
+    float den = e0 + e1 + e2;
+    if (den == 0) return false;
+    a.z *= r.sz;
+    b.z *= r.sz;
+    c.z *= r.sz;
+    float inv = 1.0f / den;
+    float t = e0 * a.z + e1 * b.z + e2 * c.z;
+    t *= inv;
+    h.t = t;
+    h.e0 = e0;
+    h.e1 = e1;
+    h.e2 = e2;
+    h.inv = inv;
+    return true;
+}
+
+YM_HD bool tri_back(f3 q0, f3 q1, f3 q2, f3 d)
+{
+    f3 gn = normalize(cross(q1 - q0, q2 - q0));
+    return dot(gn, d) > 0.0f;
+}
+
+struct Query {
+    f3 o, d;
+    float tmin, tmax;
+    uint32_t last_inst, last_tri;
+};
+
+struct Hit {
+    float t;
+    uint32_t inst, tri;
+    float b0, b1, b2;
+    int back;
+};
+
+// Per-lane LDS scratch for one traversal: the stack and the nearby list.
+template <int kBlock>
+struct LaneScratch {
+    uint32_t *stack;        // [kStack][kBlock]
+    uint32_t *nb_prim;      // [kNearbyCap][kBlock]
+    float *nb_t;
+    int *nb_back;
+    int lane;
+    YM_HD uint32_t &st(int i) { return stack[i * kBlock + lane]; }
+};
+
+struct TraceCounters {
+    uint32_t nodes, tris;
+};
+
+#define YM_SORT2(ka, va, kb, vb)                      \
+    do {                                              \
+        if (kb < ka) {                                \
+            float tk = ka; ka = kb; kb = tk;          \
+            uint32_t tv = va; va = vb; vb = tv;       \
+        }                                             \
+    } while (0)
+
+// Returns 1 on hit, 0 on miss, -1 on stack overflow (the ray is abandoned).
+template <int kMode, bool kAny, bool kNearby, bool kCount, int kBlock, int kStack>
+YM_HD int traverse(const DevScene &sc, const Query &q, Hit &hit, int &nb_count,
+                                        LaneScratch<kBlock> &ls, TraceCounters &cnt)
+{
+    if (sc.instance_count == 0) return 0;
+    const bool skip_enabled = q.last_inst != kEmpty && q.last_tri != kEmpty;
+    RayPre pre = make_pre(q.o, q.d);
+    float tbest = q.tmax;
+    bool found = false;
+    bool in_blas = false;
+    uint32_t cur_inst = kEmpty;
+    // nearby-list state (TraceRay, src/ShadingUtilities.cuh:337-394)
+    bool inst_hit = false, ex_hit = false;
+    float ex_t = kTMax;
+    int ex_back = 0;
+    float tub = 0.0f, tlb = 0.0f;
+    int nbn = 0;
+    int sp = 0;
+    uint32_t node = sc.tlas_root;
+
+    for (;;) {
+        if (!(node & kLeafBit)) {
+            const WNode *n = sc.nodes + node;
+            const float4 lx = *reinterpret_cast<const float4 *>(n->lo_x);
+            const float4 hx = *reinterpret_cast<const float4 *>(n->hi_x);
+            const float4 ly = *reinterpret_cast<const float4 *>(n->lo_y);
+            const float4 hy = *reinterpret_cast<const float4 *>(n->hi_y);
+            const float4 lz = *reinterpret_cast<const float4 *>(n->lo_z);
+            const float4 hz = *reinterpret_cast<const float4 *>(n->hi_z);
+            const uint4 ch = *reinterpret_cast<const uint4 *>(n->child);
+            if (kCount) cnt.nodes++;
+            constexpr float kRobust = 1.0f + 2.0f * error_gamma(3);
+            float k0, k1, k2, k3;
+#define YM_SLAB(K, LX, HX, LY, HY, LZ, HZ, C)                                   \
+    {                                                                           \
+        float a0 = (LX - pre.o.x) * pre.inv.x, a1 = (HX - pre.o.x) * pre.inv.x; \
+        float b0 = (LY - pre.o.y) * pre.inv.y, b1 = (HY - pre.o.y) * pre.inv.y; \
+        float c0 = (LZ - pre.o.z) * pre.inv.z, c1 = (HZ - pre.o.z) * pre.inv.z; \
+        float tn = fmaxf(fmaxf(fminf(a0, a1), fminf(b0, b1)), fminf(c0, c1));   \
+        float tf = fminf(fminf(fmaxf(a0, a1), fmaxf(b0, b1)), fmaxf(c0, c1));   \
+        tf *= kRobust;                                                          \
+        bool h = (C != kSentinel) && (tf > tn) && (tf > 0.0f) && (tn <= tbest); \
+        K = h ? (tn < 0.0f ? 0.0f : tn) : INFINITY;                             \
+    }
+            YM_SLAB(k0, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ch.x)
+            YM_SLAB(k1, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ch.y)
+            YM_SLAB(k2, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ch.z)
+            YM_SLAB(k3, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ch.w)
+#undef YM_SLAB
+            uint32_t v0 = ch.x, v1 = ch.y, v2 = ch.z, v3 = ch.w;
+            YM_SORT2(k0, v0, k1, v1);
+            YM_SORT2(k2, v2, k3, v3);
+            YM_SORT2(k0, v0, k2, v2);
+            YM_SORT2(k1, v1, k3, v3);
+            YM_SORT2(k1, v1, k2, v2);
+            if (k0 != INFINITY) {
+                int need = (k1 != INFINITY) + (k2 != INFINITY) + (k3 != INFINITY);
+                if (sp + need > kStack) return -1;
+                if (k3 != INFINITY) ls.st(sp++) = v3;
+                if (k2 != INFINITY) ls.st(sp++) = v2;
+                if (k1 != INFINITY) ls.st(sp++) = v1;
+                node = v0;
+                continue;
+            }
+        } else if (!in_blas) {
+            // top-level leaf: one instance
+            const uint32_t ii = leaf_first(node);
+            const DevInstance &in = sc.instances[ii];
+            bool take = true;
+            if (kMode == kModeSkipBoundary) take = !(in.flags & kInstBoundary);
+            if (kMode == kModeBoundaryOnly) take = (in.flags & kInstBoundary) != 0;
+            if (take) {
+                if (sp + 1 > kStack) return -1;
+                ls.st(sp++) = kSentinel;
+                pre = make_pre(xf_point(in.w2o, q.o), xf_dir(in.w2o, q.d));
+                in_blas = true;
+                cur_inst = ii;
+                inst_hit = false;
+                ex_hit = false;
+                ex_t = kTMax;
+                node = in.blas_root;
+                continue;
+            }
+        } else {
+            const uint32_t first = leaf_first(node), count = leaf_count(node);
+            for (uint32_t k = 0; k < count; ++k) {
+                const float4 *tp = sc.tris + 3u * (first + k);
+                const float4 w0 = tp[0], w1 = tp[1], w2 = tp[2];
+                const uint32_t tri = f2u(w0.w);
+                if (kCount) cnt.tris++;
+                if (skip_enabled && cur_inst == q.last_inst && tri == q.last_tri) continue;
+                const f3 q0{w0.x, w0.y, w0.z}, q1{w1.x, w1.y, w1.z}, q2{w2.x, w2.y, w2.z};
+                TriHit th;
+                if (!tri_test(pre, q0, q1, q2, th)) continue;
+                if (kNearby && th.t >= q.tmin) {
+                    ex_hit = true;
+                    ex_t = gmin(ex_t, th.t);
+                    ex_back = tri_back(q0, q1, q2, pre.d);
+                }
+                if (th.t < q.tmin || th.t > tbest) continue;
+                tbest = th.t;
+                found = true;
+                hit.t = th.t;
+                hit.inst = cur_inst;
+                hit.tri = tri;
+                if (!kAny) {
+                    hit.b0 = th.e0 * th.inv;
+                    hit.b1 = th.e1 * th.inv;
+                    hit.b2 = th.e2 * th.inv;
+                    hit.back = tri_back(q0, q1, q2, pre.d);
+                }
+                inst_hit = true;
+                if (kAny) return 1;
+            }
+        }
+        // pop
+        for (;;) {
+            if (sp == 0) {
+                return found ? 1 : 0;
+            }
+            node = ls.st(--sp);
+            if (node != kSentinel) break;
+            // leaving an instance: back to world space
+            pre = make_pre(q.o, q.d);
+            in_blas = false;
+            if (kNearby) {
+                if (inst_hit) {
+                    tub = next_float_up(next_float_up(tbest));
+                    tlb = next_float_down(next_float_down(tbest));
+                    int w = 0;
+                    for (int i = 0; i < nbn; ++i) {
+                        const float ti = ls.nb_t[i * kBlock + ls.lane];
+                        if (ti > tlb && ti < tub) {
+                            ls.nb_t[w * kBlock + ls.lane] = ti;
+                            ls.nb_prim[w * kBlock + ls.lane] = ls.nb_prim[i * kBlock + ls.lane];
+                            ls.nb_back[w * kBlock + ls.lane] = ls.nb_back[i * kBlock + ls.lane];
+                            ++w;
+                        }
+                    }
+                    nbn = w;
+                    if (nbn < kNearbyCap) {
+                        ls.nb_prim[nbn * kBlock + ls.lane] = cur_inst;
+                        ls.nb_t[nbn * kBlock + ls.lane] = tbest;
+                        ls.nb_back[nbn * kBlock + ls.lane] = hit.back;
+                        ++nbn;
+                    }
+                } else if (found && ex_hit && ex_t > tlb && ex_t < tub) {
+                    if (nbn < kNearbyCap) {
+                        ls.nb_prim[nbn * kBlock + ls.lane] = cur_inst;
+                        ls.nb_t[nbn * kBlock + ls.lane] = ex_t;
+                        ls.nb_back[nbn * kBlock + ls.lane] = ex_back;
+                        ++nbn;
+                    }
+                }
+                nb_count = nbn;
+            }
+        }
+    }
+}
+
+#undef YM_SORT2
+
+}  // namespace ym

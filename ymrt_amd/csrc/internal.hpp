@@ -1,0 +1,68 @@
+// internal.hpp -- library-private types behind the C ABI (include/ymrt.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/ymrt.h"
+#include "dev_scene.hpp"
+
+namespace ym {
+
+void set_error(const std::string &msg);
+
+#define YM_HIP(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) {                                                                        \
+            ::ym::set_error(std::string(#expr) + ": " + hipGetErrorString(e_) + " (" __FILE__ ":" +    \
+                            std::to_string(__LINE__) + ")");                                           \
+            return e_ == hipErrorOutOfMemory ? YMRT_ERR_OUT_OF_MEMORY : YMRT_ERR_DEVICE;               \
+        }                                                                                              \
+    } while (0)
+
+#define YM_REQUIRE(cond, msg)                                                                          \
+    do {                                                                                               \
+        if (!(cond)) {                                                                                 \
+            ::ym::set_error(msg);                                                                      \
+            return YMRT_ERR_INVALID_ARGUMENT;                                                          \
+        }                                                                                              \
+    } while (0)
+
+struct RenderBuffers;  // render.cpp
+
+}  // namespace ym
+
+struct ymrt_context {
+    int device = 0;
+};
+
+struct ymrt_scene {
+    ymrt_context *ctx = nullptr;
+    ym::DevScene dev{};
+    std::vector<void *> allocations;
+    ymrt_scene_info info{};
+    uint32_t max_stack = 0;
+    uint32_t stack_slots = 0;     // LDS stack entries used by the kernels
+    ym::RenderBuffers *render = nullptr;
+    uint32_t *d_error = nullptr;  // device flag: traversal stack overflow
+};
+
+namespace ym {
+
+// kernels_trace.hip
+int launch_trace(const ymrt_scene *sc, const ymrt_ray *rays, uint32_t n, ymrt_hit *hits, uint32_t *occluded,
+                 uint32_t *visits, int any, hipStream_t stream);
+int launch_halton_values(const ymrt_scene *sc, const uint64_t *idx, uint32_t n, uint32_t dim0, uint32_t dims,
+                         float *out, hipStream_t stream);
+int launch_halton_enumerate(uint32_t w, uint32_t h, const uint32_t *px, const uint32_t *py, const uint32_t *s,
+                            uint32_t n, uint64_t *out, hipStream_t stream);
+int upload_sampler_tables(ymrt_scene *sc);
+
+// render.cpp
+void destroy_render_buffers(ymrt_scene *sc);
+
+}  // namespace ym

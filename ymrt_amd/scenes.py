@@ -1,0 +1,209 @@
+"""Procedural scenes and ray sets for the workloads the tracer is measured on.
+
+Every generator is deterministic (numpy PCG64 / RandomState seeded as stated)
+and returns an ``abi.SceneArrays`` that both the HIP path and the CPU oracle
+consume, so the two see bit-identical float inputs.
+
+  cornell_box   -- 5 wall quads, two boxes, a 32x16 UV sphere, one 0.3 m
+                   area light of radiance 15; Lambert / GGX conductor /
+                   dielectric materials.
+  interior      -- about 260k triangles in 40 instances of 8 procedural meshes
+                   with random affine transforms, four 2048^2 RGBA8
+                   value-noise textures, two quad lights, homogeneous fog.
+  heightfield   -- 1001x1001-vertex fBm height field (2M triangles) and the
+                   incoherent ray sets of the traversal stress test.
+"""
+import numpy as np
+
+from . import abi
+
+
+# ----------------------------------------------------------------- primitives
+def quad(p0, p1, p2, p3):
+    """Two triangles (p0,p1,p2), (p0,p2,p3) with the face normal of the first."""
+    pos = np.array([p0, p1, p2, p3], np.float32)
+    n = np.cross(pos[1] - pos[0], pos[2] - pos[0])
+    n = (n / np.linalg.norm(n)).astype(np.float32)
+    nrm = np.repeat(n[None], 4, axis=0)
+    uv = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32)
+    idx = np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+    return pos, idx, nrm, uv
+
+
+def box(lo, hi):
+    """Axis-aligned box, 12 outward-facing triangles, per-face normals."""
+    lo = np.asarray(lo, np.float32)
+    hi = np.asarray(hi, np.float32)
+    c = [np.array([hi[0] if i & 1 else lo[0], hi[1] if i & 2 else lo[1], hi[2] if i & 4 else lo[2]], np.float32)
+         for i in range(8)]
+    faces = [(0, 2, 3, 1), (4, 5, 7, 6), (0, 1, 5, 4), (2, 6, 7, 3), (0, 4, 6, 2), (1, 3, 7, 5)]
+    pos, idx, nrm, uv = [], [], [], []
+    for f in faces:
+        p, i, n, t = quad(*[c[k] for k in f])
+        idx.append(i + len(pos) * 4)
+        pos.append(p)
+        nrm.append(n)
+        uv.append(t)
+    return (np.concatenate(pos), np.concatenate(idx).astype(np.uint32), np.concatenate(nrm), np.concatenate(uv))
+
+
+def uv_sphere(center, radius, nu=32, nv=16):
+    """Latitude-longitude sphere with nu*nv*2 - 2*nu triangles (960 for 32x16)."""
+    center = np.asarray(center, np.float64)
+    verts, nrms, uvs = [], [], []
+    for j in range(nv + 1):
+        th = np.pi * j / nv
+        for i in range(nu + 1):
+            ph = 2 * np.pi * (i % nu) / nu  # the seam column repeats column 0 exactly
+            n = np.array([np.sin(th) * np.cos(ph), np.cos(th), np.sin(th) * np.sin(ph)])
+            if j == 0 or j == nv:  # poles: one exact point
+                n = np.array([0.0, 1.0 if j == 0 else -1.0, 0.0])
+            verts.append(center + radius * n)
+            nrms.append(n)
+            uvs.append([i / nu, j / nv])
+    idx = []
+    for j in range(nv):
+        for i in range(nu):
+            a = j * (nu + 1) + i
+            b = a + nu + 1
+            if j != 0:
+                idx.append([a, a + 1, b])
+            if j != nv - 1:
+                idx.append([a + 1, b + 1, b])
+    return (np.array(verts, np.float32), np.array(idx, np.uint32), np.array(nrms, np.float32),
+            np.array(uvs, np.float32))
+
+
+def _add(sa, geom):
+    pos, idx, nrm, uv = geom
+    return sa.add_mesh(pos, idx, nrm=nrm, nidx=idx, uv=uv, tidx=idx)
+
+
+# -------------------------------------------------------------- Cornell box
+def cornell_box(width=256, height=256, halton_seed=0):
+    """Unit Cornell box [0,1]^3 open at z=1 (the camera side)."""
+    sa = abi.SceneArrays()
+    sa.halton_seed = halton_seed
+    white = sa.add_material(abi.default_material(diffuse_albedo=(0.7, 0.7, 0.7), specular_weight=0.0))
+    red = sa.add_material(abi.default_material(diffuse_albedo=(0.7, 0.1, 0.1), specular_weight=0.0))
+    green = sa.add_material(abi.default_material(diffuse_albedo=(0.1, 0.7, 0.1), specular_weight=0.0))
+    metal = sa.add_material(abi.default_material(diffuse_albedo=(0.9, 0.9, 0.9), specular_albedo=(1.0, 1.0, 1.0),
+                                                 roughness_x=0.3, roughness_y=0.3, metalness=1.0))
+    glass = sa.add_material(abi.default_material(diffuse_albedo=(1.0, 1.0, 1.0), specular_albedo=(1.0, 1.0, 1.0),
+                                                 roughness_x=0.05, roughness_y=0.05, transmission_weight=1.0,
+                                                 ior_n=1.5, ior_priority=1))
+    light = sa.add_material(abi.light_material((1.0, 1.0, 1.0), 15.0))
+    walls = [
+        (quad((0, 0, 0), (0, 0, 1), (1, 0, 1), (1, 0, 0)), white),      # floor, normal +y
+        (quad((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)), white),      # ceiling, normal -y
+        (quad((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0)), white),      # back wall, normal +z
+        (quad((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1)), red),        # left wall, normal +x
+        (quad((1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0)), green),      # right wall, normal -x
+    ]
+    for geom, mat in walls:
+        sa.add_instance(_add(sa, geom), mat)
+    sa.add_instance(_add(sa, box((0.55, 0.0, 0.45), (0.85, 0.3, 0.75))), white)
+    sa.add_instance(_add(sa, box((0.15, 0.0, 0.15), (0.45, 0.6, 0.45))), metal)
+    sa.add_instance(_add(sa, uv_sphere((0.65, 0.45, 0.55), 0.15)), glass)
+    lq = quad((0.35, 0.999, 0.35), (0.65, 0.999, 0.35), (0.65, 0.999, 0.65), (0.35, 0.999, 0.65))
+    sa.add_instance(_add(sa, lq), light)
+    sa.set_camera((0.5, 0.5, 2.2), (0.5, 0.5, 0.0), 40.0, width / height)
+    return sa
+
+
+# --------------------------------------------------------------- value noise
+def value_noise_2d(n, octaves, seed, lattice=16):
+    """fBm of bilinearly interpolated value noise on an n x n grid, in [0, 1]."""
+    rng = np.random.RandomState(seed)
+    out = np.zeros((n, n), np.float64)
+    amp, total = 1.0, 0.0
+    for o in range(octaves):
+        cells = lattice * (2 ** o)
+        lat = rng.rand(cells + 1, cells + 1)
+        x = np.linspace(0, cells, n, endpoint=False)
+        i0 = np.floor(x).astype(int)
+        f = x - i0
+        f = f * f * (3 - 2 * f)
+        a = lat[i0][:, i0]
+        b = lat[i0][:, i0 + 1]
+        c = lat[i0 + 1][:, i0]
+        d = lat[i0 + 1][:, i0 + 1]
+        fy = f[:, None]
+        fx = f[None, :]
+        out += amp * ((a * (1 - fx) + b * fx) * (1 - fy) + (c * (1 - fx) + d * fx) * fy)
+        total += amp
+        amp *= 0.5
+    return out / total
+
+
+# -------------------------------------------------------------- height field
+def heightfield(verts_per_side=1001, octaves=6, amplitude=0.1, seed=1):
+    """(V-1)^2*2 triangles on [0,1]^2 (x,z), height y from a 512^2 fBm lattice."""
+    sa = abi.SceneArrays()
+    n = verts_per_side
+    lat = value_noise_2d(512, octaves, seed)
+    g = np.linspace(0.0, 1.0, n)
+    # bilinear sample of the 512^2 lattice at the vertex positions
+    s = g * 511.0
+    i0 = np.minimum(np.floor(s).astype(int), 510)
+    f = s - i0
+    a = lat[i0][:, i0]
+    b = lat[i0][:, i0 + 1]
+    c = lat[i0 + 1][:, i0]
+    d = lat[i0 + 1][:, i0 + 1]
+    fz = f[:, None]
+    fx = f[None, :]
+    h = amplitude * ((a * (1 - fx) + b * fx) * (1 - fz) + (c * (1 - fx) + d * fx) * fz)
+    X, Z = np.meshgrid(g, g)
+    pos = np.stack([X, h, Z], axis=-1).reshape(-1, 3).astype(np.float32)
+    k = np.arange(n - 1)
+    I, J = np.meshgrid(k, k)
+    v00 = (J * n + I).reshape(-1)
+    v10 = v00 + 1
+    v01 = v00 + n
+    v11 = v01 + 1
+    tri = np.empty((2 * len(v00), 3), np.uint32)
+    tri[0::2] = np.stack([v00, v01, v11], axis=-1)
+    tri[1::2] = np.stack([v00, v11, v10], axis=-1)
+    mesh = sa.add_mesh(pos, tri)
+    mat = sa.add_material(abi.default_material())
+    sa.add_instance(mesh, mat)
+    sa.set_camera((0.5, 0.5, 2.0), (0.5, 0.0, 0.5), 45.0, 1.0)
+    return sa
+
+
+def bounds(sa):
+    lo = np.full(3, np.inf)
+    hi = np.full(3, -np.inf)
+    for it in sa.instances:
+        p = sa.meshes[it["mesh"]]["pos"].astype(np.float64)
+        ph = np.c_[p, np.ones(len(p))] @ it["o2w"].astype(np.float64).T
+        lo = np.minimum(lo, ph[:, :3].min(0))
+        hi = np.maximum(hi, ph[:, :3].max(0))
+    return lo, hi
+
+
+def stress_rays(sa, n, seed_origin=2, seed_dir=3, seed_target=4):
+    """Closest-hit rays (origins uniform in the scene AABB, directions uniform
+    on S^2) and any-hit rays (segments to uniform points in the AABB)."""
+    lo, hi = bounds(sa)
+    o = lo + np.random.RandomState(seed_origin).rand(n, 3) * (hi - lo)
+    u = np.random.RandomState(seed_dir).rand(n, 2)
+    z = 1.0 - 2.0 * u[:, 0]
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    ph = 2.0 * np.pi * u[:, 1]
+    d = np.stack([r * np.cos(ph), r * np.sin(ph), z], axis=-1)
+    closest = np.zeros(n, abi.RAY_DTYPE)
+    closest["o"] = o.astype(np.float32)
+    closest["d"] = d.astype(np.float32)
+    closest["tmin"] = np.float32(1e-7)
+    closest["tmax"] = np.float32(1e36)
+    tgt = lo + np.random.RandomState(seed_target).rand(n, 3) * (hi - lo)
+    seg = tgt - o
+    length = np.linalg.norm(seg, axis=1)
+    shadow = np.zeros(n, abi.RAY_DTYPE)
+    shadow["o"] = o.astype(np.float32)
+    shadow["d"] = (seg / length[:, None]).astype(np.float32)
+    shadow["tmin"] = np.float32(1e-7)
+    shadow["tmax"] = length.astype(np.float32)
+    return closest, shadow
