@@ -1502,7 +1502,7 @@ v3 or_trace_path(const or_scene *sc, const ymrt_render_settings *rs, int frame, 
         }
         if (m->type == YMRT_MATERIAL_LIGHT) {
             if (never_scatter)
-                L = add3(L, light_clamp(rs, mul3(beta, scl3(mk3(m->light_color[0], m->light_color[1], m->light_color[2]), m->intensity)), depth));
+                L = add3(L, light_clamp(rs, scl3(mul3(beta, mk3(m->light_color[0], m->light_color[1], m->light_color[2])), m->intensity), depth));
             return L;
         }
         mbsdf b;

@@ -1,0 +1,1022 @@
+// dev_shade.hpp -- device shading library: medium records, image textures,
+// the DefaultMtl BSDF, shape lights, Henyey-Greenstein phases and free-flight
+// sampling.  Each function restates the reference function cited beside it in
+// the same floating-point operation order (contraction off), so the HIP path
+// and the CPU oracle compute the same floats wherever the math library does.
+#pragma once
+
+#include "../../include/ymrt.h"
+#include "dev_sampler.hpp"
+
+namespace ym {
+
+// ---------------------------------------------------------------- RayTransfer
+// src/Ray.h:106-260 (records of (instance, ior, priority, volume))
+struct Transfer {
+    int max_idx, count;
+    CameraRecord rec[kMaxBoundaryRecord];
+
+    YM_HD void clear()
+    {
+        max_idx = -1;
+        count = 0;
+    }
+    YM_HD int find_max() const
+    {
+        if (count == 0) return -1;
+        uint32_t mp = rec[0].priority;
+        int mi = 0;
+        for (int i = 1; i < count; ++i)
+            if (rec[i].priority >= mp) {
+                mp = rec[i].priority;
+                mi = i;
+            }
+        return mi;
+    }
+    YM_HD int find_prim(uint32_t prim) const
+    {
+        for (int i = count - 1; i >= 0; --i)
+            if (rec[i].prim == prim) return i;
+        return -1;
+    }
+    YM_HD bool push(uint32_t prim, float ior, uint32_t prio, int vol)
+    {
+        if (count == kMaxBoundaryRecord) return false;
+        rec[count] = CameraRecord{prim, ior, prio, vol};
+        ++count;
+        if (count == 1) {
+            max_idx = 0;
+            return true;
+        }
+        if (prio >= rec[max_idx].priority) max_idx = count - 1;
+        return true;
+    }
+    YM_HD bool pop(int idx)
+    {
+        if (count == 0) return false;
+        if (!(idx >= 0 && idx < count)) return false;
+        for (int i = idx + 1; i < count; ++i) rec[i - 1] = rec[i];
+        --count;
+        max_idx = find_max();
+        return true;
+    }
+    YM_HD int volumes(int *vols, uint32_t volume_count) const
+    {
+        if (count == 0 || max_idx == -1) return 0;
+        const uint32_t mp = rec[max_idx].priority;
+        int n = 0;
+        for (int i = count - 1; i >= 0; --i)
+            if (rec[i].priority == mp && rec[i].vol >= 0 && (uint32_t)rec[i].vol < volume_count) vols[n++] = rec[i].vol;
+        return n;
+    }
+    YM_HD float ray_ior() const { return count > 0 ? rec[max_idx].ior : 1.0f; }
+    YM_HD float ex_ior(uint32_t hit_prim) const
+    {
+        uint32_t mp = 0;
+        int mi = -1;
+        for (int i = 0; i < count; ++i) {
+            if (rec[i].prim == hit_prim) continue;
+            if (rec[i].priority >= mp) {
+                mp = rec[i].priority;
+                mi = i;
+            }
+        }
+        return mi == -1 ? 1.0f : rec[mi].ior;
+    }
+};
+
+YM_HD float fetch_ior(const DevMaterial &m, uint32_t *prio)
+{
+    if (m.type == YMRT_MATERIAL_DEFAULT) {
+        *prio = m.ior_priority;
+        return m.ior_n;
+    }
+    *prio = 0;
+    return 1.0f;
+}
+
+struct NearbyList {
+    int n;
+    uint32_t prim[4];
+    float t[4];
+    int back[4];
+};
+
+// BoundaryTransitionBunch (src/ShadingUtilities.cuh:954-990)
+__device__ __forceinline__ void boundary_bunch(const DevScene &sc, Transfer &t, const NearbyList &nb)
+{
+    for (int i = 0; i < nb.n; ++i) {
+        if (nb.back[i]) continue;
+        const DevInstance &in = sc.instances[nb.prim[i]];
+        uint32_t prio;
+        float ior = fetch_ior(sc.materials[in.material], &prio);
+        if (t.find_prim(nb.prim[i]) == -1) t.push(nb.prim[i], ior, prio, in.inner_volume);
+    }
+    for (int i = 0; i < nb.n; ++i) {
+        if (!nb.back[i]) continue;
+        int ri = t.find_prim(nb.prim[i]);
+        if (ri != -1) t.pop(ri);
+    }
+}
+
+// ---------------------------------------------------------------- textures
+// EvalTexImage (src/TextureEval.cuh:162-237) over RGBA8 texels decoded through
+// the sRGB / linear table (LoadPng8, src/FileIO.cpp:17-45).
+__device__ __forceinline__ f3 texel(const DevScene &sc, const DevTexture &t, int x, int y)
+{
+    const int w = (int)t.width, h = (int)t.height;
+    if (t.wrap == YMRT_WRAP_CLAMP) {
+        x = x < 0 ? 0 : (x > w - 1 ? w - 1 : x);
+        y = y < 0 ? 0 : (y > h - 1 ? h - 1 : y);
+    } else {
+        x = ((x % w) + w) % w;
+        y = ((y % h) + h) % h;
+    }
+    const uint32_t word = sc.texels[t.offset + (uint64_t)y * w + x];
+    const float *lut = sc.tex_lut + (t.color_space == YMRT_COLOR_SRGB ? 0 : 256);
+    if (t.channels == 1) return s3(lut[word & 0xFFu]);
+    f3 r = s3(0.0f);
+    r.x = lut[word & 0xFFu];
+    if (t.channels > 1) r.y = lut[(word >> 8) & 0xFFu];
+    if (t.channels > 2) r.z = lut[(word >> 16) & 0xFFu];
+    return r;
+}
+
+__device__ __forceinline__ f3 texture_eval(const DevScene &sc, uint32_t ti, f2 uv)
+{
+    const DevTexture t = sc.textures[ti];
+    float tu = uv.x * t.su + t.ou;
+    if (tu < 0.0f) tu = gmax(0.0f, tu + fabsf(floorf(tu)));
+    float tv = uv.y * t.sv + t.ov;
+    if (tv < 0.0f) tv = gmax(0.0f, tv + fabsf(floorf(tv)));
+    const float cx = tu * (float)t.width - 0.5f, cy = tv * (float)t.height - 0.5f;
+    const int x0 = (int)floorf(cx), y0 = (int)floorf(cy);
+    const float fx = cx - (float)x0, fy = cy - (float)y0;
+    f3 r = ((1.0f - fx) * (1.0f - fy)) * texel(sc, t, x0, y0);
+    r = r + (fx * (1.0f - fy)) * texel(sc, t, x0 + 1, y0);
+    r = r + ((1.0f - fx) * fy) * texel(sc, t, x0, y0 + 1);
+    r = r + (fx * fy) * texel(sc, t, x0 + 1, y0 + 1);
+    return r;
+}
+
+// ---------------------------------------------------------------- BSDF lobes
+// src/BSDF.cuh:20-375
+YM_HD bool same_hemi(f3 a, f3 b) { return a.z * b.z > 0.0f; }
+YM_HD f3 reflect3(f3 wo, f3 n) { return -wo + (2.0f * n) * dot(n, wo); }
+YM_HD bool refract3(f3 wo, f3 n, float ni, float nt, f3 &wi)
+{
+    float ci = gmax(dot(wo, n), 0.0f);
+    float ci2 = sqr(ci);
+    float si2 = 1.0f - ci2;
+    float ct2 = 1.0f - sqr(ni / nt) * si2;
+    if (!(ct2 > 0.0f)) return false;
+    float ct = sqrtf(ct2);
+    wi = normalize(-(ni / nt) * (wo - n * ci) - n * ct);
+    return true;
+}
+YM_HD float fresnel_dd(float eta, float c)
+{
+    c = gmax(0.0f, c);
+    float s2 = 1.f - c * c;
+    float t0 = sqrtf(gmax(1.0f - (s2 / (eta * eta)), 0.0f));
+    float t1 = eta * t0;
+    float t2 = eta * c;
+    float rs = (c - t1) / gmax(c + t1, 1E-12f);
+    float rp = (t0 - t2) / gmax(t0 + t2, 1E-12f);
+    return (float)(0.5 * (double)(rs * rs + rp * rp));
+}
+YM_HD f3 sqrt3(f3 a) { return f3{sqrtf(a.x), sqrtf(a.y), sqrtf(a.z)}; }
+YM_HD f3 fresnel_dc(f3 eta, f3 etak, float c)
+{
+    float c2 = c * c, s2 = 1.0f - c2;
+    f3 e2 = eta * eta, k2 = etak * etak;
+    f3 t0 = e2 - k2 - s3(s2);
+    f3 ab = f3{sqrtf(t0.x * t0.x + 4.0f * e2.x * k2.x), sqrtf(t0.y * t0.y + 4.0f * e2.y * k2.y),
+               sqrtf(t0.z * t0.z + 4.0f * e2.z * k2.z)};
+    f3 t1 = ab + s3(c2);
+    f3 a = f3{sqrtf(0.5f * (ab.x + t0.x)), sqrtf(0.5f * (ab.y + t0.y)), sqrtf(0.5f * (ab.z + t0.z))};
+    f3 t2 = (2.0f * a) * c;
+    f3 rs = (t1 - t2) / (t1 + t2);
+    f3 t3 = c2 * ab + s3(s2 * s2);
+    f3 t4 = t2 * s2;
+    f3 rp = rs * (t3 - t4) / (t3 + t4);
+    return 0.5f * (rp + rs);
+}
+YM_HD void edge_tint(f3 refl, f3 tint, f3 &n, f3 &k)
+{
+    f3 r{gclamp(refl.x, 0.f, 0.99f), gclamp(refl.y, 0.f, 0.99f), gclamp(refl.z, 0.f, 0.99f)};
+    f3 rs = sqrt3(r);
+    const f3 one = s3(1.0f);
+    f3 nmin = (one - r) / (one + r);
+    f3 nmax = (one + rs) / (one - rs);
+    n = mixv(nmax, nmin, tint);
+    f3 np1 = n + one, nm1 = n - one;
+    f3 k2 = (np1 * np1 * r - nm1 * nm1) / (one - r);
+    k2 = max3(k2, s3(0.0f));
+    k = sqrt3(k2);
+}
+YM_HD float ggx_d(f3 h, float ax, float ay)
+{
+    float den = kPi * ax * ay * sqr(sqr(h.x / ax) + sqr(h.y / ay) + sqr(h.z));
+    return 1.0f / gmax(den, 1E-12f);
+}
+YM_HD float ggx_lambda(f3 w, float ax, float ay)
+{
+    if (fabsf(w.z) < 1E-8f) return 0.0f;
+    return (-1.0f + sqrtf(1.0f + (sqr(w.x * ax) + sqr(w.y * ay)) / sqr(w.z))) * 0.5f;
+}
+YM_HD float ggx_g1(f3 w, float ax, float ay) { return 1.0f / (1.0f + ggx_lambda(w, ax, ay)); }
+YM_HD float ggx_g2(f3 a, f3 b, float ax, float ay)
+{
+    return 1.0f / (1.0f + ggx_lambda(a, ax, ay) + ggx_lambda(b, ax, ay));
+}
+__device__ __forceinline__ f3 ggx_sample_wh(f3 wo, float u0, float u1, float ax, float ay)
+{
+    f3 wh = normalize(f3{ax * wo.x, ay * wo.y, wo.z});
+    float lensq = wh.x * wh.x + wh.y * wh.y;
+    f3 T1 = lensq > 0 ? f3{-wh.y, wh.x, 0} * (1.0f / sqrtf(lensq)) : f3{1, 0, 0};
+    f3 T2 = cross(wh, T1);
+    float r = sqrtf(gmax(u0, 0.0f));
+    float phi = 2.0f * kPi * u1;
+    float t1 = r * cosf(phi);
+    float t2 = r * sinf(phi);
+    float s = (float)(0.5 * (1.0 + (double)wh.z));
+    t2 = (float)((1.0 - (double)s) * sqrt(1.0 - (double)(t1 * t1)) + (double)s * (double)t2);
+    f3 Nh = t1 * T1 + t2 * T2 + sqrtf(gmax(0.0f, 1.0f - t1 * t1 - t2 * t2)) * wh;
+    return normalize(f3{ax * Nh.x, ay * Nh.y, gmax(0.0f, Nh.z)});
+}
+__device__ __forceinline__ f3 sample_cosine(float u0, float u1)
+{
+    float r = sqrtf(u0);
+    float phi = u1 * kTwoPi;
+    float x = r * cosf(phi), y = r * sinf(phi);
+    float r2 = x * x + y * y;
+    return f3{x, y, r2 < 1.0f ? sqrtf(1.0f - r2) : 0.0f};
+}
+
+enum : int { kLobeLambert = 0, kLobeReflect = 1, kLobeTransmit = 2 };
+
+struct Lobe {
+    int type;
+    int metal;
+    f3 albedo;
+    f3 nt, kt;   // reflection: eta, k (transmission uses nt.x)
+    float ni;
+    float ax, ay;
+};
+
+YM_HD f3 lobe_eval(const Lobe &l, f3 wo, f3 wi, float &pdf)
+{
+    if (l.type == kLobeLambert) {
+        if (!same_hemi(wo, wi)) {
+            pdf = 0.0f;
+            return s3(0.0f);
+        }
+        float c = gmax(wi.z, 0.0f);
+        pdf = c * kInvPi;
+        return l.albedo * c * kInvPi;
+    }
+    if (l.type == kLobeReflect) {
+        if (!same_hemi(wo, wi) || wo.z < 0.f || wi.z < 0.f) {
+            pdf = 0.0f;
+            return s3(0.0f);
+        }
+        f3 wh = wi + wo;
+        float len = length(wh);
+        if (fabsf(len) < 1E-8f) {
+            pdf = 0.0f;
+            return s3(0.0f);
+        }
+        wh = wh / len;
+        float D = ggx_d(wh, l.ax, l.ay);
+        float G1 = ggx_g1(wo, l.ax, l.ay);
+        float G2 = ggx_g2(wo, wi, l.ax, l.ay);
+        float ini = safe_rcp(l.ni);
+        float c = gmax(dot(wo, wh), 0.0f);
+        f3 fr = l.metal ? fresnel_dc(l.nt * ini, l.kt * ini, c) : s3(fresnel_dd(l.nt.x * ini, c));
+        pdf = D * G1 * 0.25f / gmax(wo.z, 1E-8f);
+        return l.albedo * D * G2 * 0.25f * fr / gmax(wo.z, 1E-8f);
+    }
+    if (same_hemi(wo, wi)) {
+        pdf = 0.0f;
+        return s3(0.0f);
+    }
+    const float nts = l.nt.x;
+    f3 wh = -(l.ni * wo + nts * wi);
+    if (!same_hemi(wo, wh)) {
+        pdf = 0.0f;
+        return s3(0.0f);
+    }
+    float len = length(wh);
+    if (fabsf(len) < 1E-8f) {
+        pdf = 0.0f;
+        return s3(0.0f);
+    }
+    wh = wh / len;
+    float D = ggx_d(wh, l.ax, l.ay);
+    float G1 = ggx_g1(wo, l.ax, l.ay);
+    float G2 = ggx_g2(wo, wi, l.ax, l.ay);
+    float dwh = sqr(nts) * fabsf(dot(wi, wh)) / gmax(sqr(l.ni * dot(wo, wh) + nts * dot(wi, wh)), 1E-12f);
+    float ini = safe_rcp(l.ni);
+    float c = gmax(dot(wo, wh), 0.0f);
+    float fr = fresnel_dd(nts * ini, c);
+    pdf = D * G1 * fabsf(dot(wo, wh)) * dwh / gmax(fabsf(wo.z), 1E-8f);
+    f3 r = l.albedo * (1.0f - fr);
+    r = r * fabsf(dot(wo, wh) / gmax(fabsf(wo.z), 1E-8f));
+    r = r * G2;
+    r = r * D;
+    r = r * dwh;
+    r = r * sqr(l.ni / nts);
+    return r;
+}
+
+__device__ __forceinline__ bool lobe_sample(const Lobe &l, float u0, float u1, f3 wo, f3 &weight, f3 &wi,
+                                            float &pdf)
+{
+    if (l.type == kLobeLambert) {
+        wi = sample_cosine(u0, u1);
+        float c = gmax(wi.z, 0.0f);
+        pdf = c * kInvPi;
+        weight = l.albedo;
+        return true;
+    }
+    f3 wh = ggx_sample_wh(wo, u0, u1, l.ax, l.ay);
+    if (!same_hemi(wo, wh) || dot(wo, wh) < 0.0f) return false;
+    if (l.type == kLobeReflect) {
+        wi = reflect3(wo, wh);
+        if (!same_hemi(wi, wo)) return false;
+        float D = ggx_d(wh, l.ax, l.ay);
+        float G1 = ggx_g1(wo, l.ax, l.ay);
+        float G2 = ggx_g2(wo, wi, l.ax, l.ay);
+        float ini = safe_rcp(l.ni);
+        float c = gmax(dot(wo, wh), 0.0f);
+        f3 fr = l.metal ? fresnel_dc(l.nt * ini, l.kt * ini, c) : s3(fresnel_dd(l.nt.x * ini, c));
+        pdf = D * G1 * 0.25f / gmax(wo.z, 1E-8f);
+        weight = l.albedo * G2 * fr / G1;
+        return true;
+    }
+    const float nts = l.nt.x;
+    if (!refract3(wo, wh, l.ni, nts, wi)) return false;
+    if (same_hemi(wi, wo)) return false;
+    float D = ggx_d(wh, l.ax, l.ay);
+    float G1 = ggx_g1(wo, l.ax, l.ay);
+    float G2 = ggx_g2(wi, wo, l.ax, l.ay);
+    float dwh = sqr(nts) * fabsf(dot(wi, wh)) / gmax(sqr(l.ni * dot(wo, wh) + nts * dot(wi, wh)), 1E-12f);
+    float ini = safe_rcp(l.ni);
+    float c = gmax(dot(wo, wh), 0.0f);
+    float fr = fresnel_dd(nts * ini, c);
+    pdf = D * G1 * fabsf(dot(wo, wh)) * dwh / gmax(fabsf(wo.z), 1E-8f);
+    weight = l.albedo * (1.0f - fr) * sqr(l.ni / nts) * G2 / G1;
+    return true;
+}
+
+// DefaultMtlBSDF + MaterialBSDF (src/BSDF.cuh:432-807), no normal/bump maps
+struct Bsdf {
+    f3 t, b, n;
+    int is_default;
+    int count;
+    Lobe l[5];
+    float w[5];
+
+    YM_HD f3 to_local(f3 v) const { return f3{dot(v, t), dot(v, b), dot(v, n)}; }
+    YM_HD f3 to_world(f3 v) const { return v.x * t + v.y * b + v.z * n; }
+};
+
+YM_HD void init_shading_space(Bsdf &bs, int hit_back, f3 ray_d, f3 ng, f3 sn_in, f3 dpdu, f3 dpdv)
+{
+    f3 sn = sn_in;
+    if (dot(sn_in, ng) < 0.0f) sn = -sn_in;
+    f3 su = dpdu;
+    if (dot(cross(dpdu, dpdv), ng) < 0.0f) su = -dpdu;
+    su = su - sn * dot(su, sn);
+    if (!hit_back) {
+        bs.n = sn;
+        bs.t = normalize(su);
+        bs.b = cross(bs.n, bs.t);
+    } else {
+        bs.n = -sn;
+        bs.t = -normalize(su);
+        bs.b = -cross(bs.n, bs.t);
+    }
+    if (dot(-ray_d, bs.n) < 0.0f) {
+        bs.n = reflect3(bs.n, dot(bs.n, ng) > 0.0f ? ng : -ng);
+        bs.t = normalize(bs.t - bs.n * dot(bs.t, bs.n));
+        bs.b = hit_back ? -cross(bs.n, bs.t) : cross(bs.n, bs.t);
+    }
+}
+
+__device__ __forceinline__ float tex_float(const DevScene &sc, uint32_t tex, float def, f2 uv)
+{
+    return tex != kEmpty ? texture_eval(sc, tex, uv).x : def;
+}
+__device__ __forceinline__ f3 tex_color(const DevScene &sc, uint32_t tex, const float *def, f2 uv)
+{
+    return tex != kEmpty ? texture_eval(sc, tex, uv) : f3{def[0], def[1], def[2]};
+}
+
+__device__ __forceinline__ void init_default_bsdf(const DevScene &sc, Bsdf &b, const DevMaterial &m, f2 uv,
+                                                  f3 ray_d, int hit_back, float ray_ior, float ex_ior)
+{
+    const float ior_min = 0.0001f;
+    const float coat_w = tex_float(sc, m.coat_weight_tex, m.coat_weight, uv);
+    const float ior_in = gmax(ray_ior, ior_min);
+    const float ior_coat = gmax(m.coat_ior, ior_min);
+    const float ior_out = gmax(hit_back ? ex_ior : m.ior_n, ior_min);
+    const float eta_ic = ior_coat * safe_rcp(ior_in);
+    const float eta_io = ior_out * safe_rcp(ior_in);
+    const float cos_in = gmax(dot(b.n, -ray_d), 0.0f);
+    const float fr_ic = coat_w > 0.0f ? fresnel_dd(eta_ic, cos_in) : 1.0f;
+    const float fr_io = fresnel_dd(eta_io, cos_in);
+    const float metal = tex_float(sc, m.metalness_tex, m.metalness, uv);
+    const float ax = sqr(gclamp(tex_float(sc, m.roughness_x_tex, m.roughness_x, uv), 0.001f, 0.999f));
+    const float ay = sqr(gclamp(tex_float(sc, m.roughness_y_tex, m.roughness_y, uv), 0.001f, 0.999f));
+    const float spec_w = tex_float(sc, m.specular_weight_tex, m.specular_weight, uv);
+    const float trans_w = tex_float(sc, m.transmission_weight_tex, m.transmission_weight, uv);
+    const f3 dif = tex_color(sc, m.diffuse_albedo_tex, m.diffuse_albedo, uv);
+    const f3 spe = tex_color(sc, m.specular_albedo_tex, m.specular_albedo, uv);
+    const float coat_th = tex_float(sc, m.coat_thickness_tex, m.coat_thickness, uv);
+    const float cax = sqr(gclamp(tex_float(sc, m.coat_roughness_x_tex, m.coat_roughness_x, uv), 0.001f, 0.999f));
+    const float cay = sqr(gclamp(tex_float(sc, m.coat_roughness_y_tex, m.coat_roughness_y, uv), 0.001f, 0.999f));
+    const f3 coat_alb = tex_color(sc, m.coat_albedo_tex, m.coat_albedo, uv);
+    const f3 ca = max3(coat_alb, s3(1E-8f));
+    const f3 sigma = (-f3{logf(ca.x), logf(ca.y), logf(ca.z)}) * coat_th;
+    const float rc = safe_rcp(cos_in);
+    const f3 ns = -sigma;
+    const f3 coat_tr{expf(ns.x * rc), expf(ns.y * rc), expf(ns.z * rc)};
+    float wsum = 0.0f;
+    b.count = 0;
+    const f3 coat_color = s3(coat_w * fr_ic);
+    const float coat_lum = luminance(coat_color);
+    if (coat_lum > 0.0f) {
+        Lobe &l = b.l[b.count];
+        l.type = kLobeReflect;
+        l.albedo = coat_color;
+        l.ax = cax;
+        l.ay = cay;
+        l.nt = s3(ior_coat);
+        l.kt = s3(0.0f);
+        l.ni = ior_in;
+        l.metal = 0;
+        b.w[b.count] = coat_lum;
+        wsum += coat_lum;
+        ++b.count;
+    }
+    const f3 lerp = (1.0f - coat_w) * s3(1.0f) + coat_tr * coat_w;
+    const f3 tint = (1.0f - coat_w * fr_ic) * lerp;
+    const f3 metal_c = tint * metal;
+    const float metal_lum = luminance(metal_c);
+    if (metal_lum > 0.0f) {
+        f3 mn, mk;
+        edge_tint(dif, spe, mn, mk);
+        Lobe &l = b.l[b.count];
+        l.type = kLobeReflect;
+        l.albedo = metal_c;
+        l.ax = ax;
+        l.ay = ay;
+        l.nt = mn;
+        l.kt = mk;
+        l.ni = ior_in;
+        l.metal = 1;
+        b.w[b.count] = metal_lum;
+        wsum += metal_lum;
+        ++b.count;
+    }
+    const f3 spec_c = tint * (1.0f - metal) * fr_io * spec_w * spe;
+    const float spec_lum = luminance(spec_c);
+    if (spec_lum > 0.0f) {
+        Lobe &l = b.l[b.count];
+        l.type = kLobeReflect;
+        l.albedo = spec_c;
+        l.ax = ax;
+        l.ay = ay;
+        l.nt = s3(ior_out);
+        l.kt = s3(0.0f);
+        l.ni = ior_in;
+        l.metal = 0;
+        b.w[b.count] = spec_lum;
+        wsum += spec_lum;
+        ++b.count;
+    }
+    const f3 trans_c = tint * (1.0f - metal) * (1.0f - fr_io * spec_w) * trans_w * spe;
+    const float trans_lum = luminance(trans_c);
+    if (trans_lum > 0.0f) {
+        Lobe &l = b.l[b.count];
+        l.type = kLobeTransmit;
+        l.albedo = trans_c;
+        l.ax = ax;
+        l.ay = ay;
+        l.ni = ior_in;
+        l.nt = s3(ior_out);
+        l.kt = s3(0.0f);
+        l.metal = 0;
+        b.w[b.count] = trans_lum;
+        wsum += trans_lum;
+        ++b.count;
+    }
+    const f3 dif_c = tint * (1.0f - metal) * (1.0f - fr_io * spec_w) * (1.0f - trans_w) * dif;
+    const float dif_lum = luminance(dif_c);
+    {
+        Lobe &l = b.l[b.count];
+        l.type = kLobeLambert;
+        l.albedo = dif_c;
+        l.metal = 0;
+        b.w[b.count] = dif_lum;
+        wsum += dif_lum;
+        ++b.count;
+    }
+    if (wsum > 0.0f) {
+        const float iw = safe_rcp(wsum);
+        for (int i = 0; i < b.count; ++i) b.w[i] *= iw;
+    } else {
+        const float iw = 1.0f / (float)b.count;
+        for (int i = 0; i < b.count; ++i) b.w[i] = iw;
+    }
+}
+
+YM_HD f3 bsdf_eval(const Bsdf &b, f3 wo, f3 wi, float &pdf)
+{
+    f3 f = s3(0.0f);
+    float p = 0.0f;
+    if (!b.is_default) {
+        pdf = 0.0f;
+        return f;
+    }
+    for (int i = 0; i < b.count; ++i) {
+        float lp = 0.0f;
+        f = f + lobe_eval(b.l[i], wo, wi, lp);
+        p += b.w[i] * lp;
+    }
+    pdf = p;
+    return f;
+}
+
+__device__ __forceinline__ bool bsdf_sample(const Bsdf &b, f3 wo, f3 &weight, f3 &wi, float &pdf, float u0,
+                                            float u1, float u2)
+{
+    if (!b.is_default) return false;
+    float cdf = 0.0f;
+    int sel = -1;
+    for (int i = 0; i < b.count; ++i) {
+        if (u0 < cdf + b.w[i]) {
+            sel = i;
+            break;
+        }
+        cdf += b.w[i];
+    }
+    if (sel == -1) sel = b.count - 1;
+    if (sel == -1) return false;
+    float lp = 0.0f;
+    if (!lobe_sample(b.l[sel], u1, u2, wo, weight, wi, lp)) return false;
+    float mp = lp * b.w[sel];
+    f3 mf = weight * lp;
+    for (int i = 0; i < b.count; ++i) {
+        if (i == sel) continue;
+        float p = 0.0f;
+        mf = mf + lobe_eval(b.l[i], wo, wi, p);
+        mp += b.w[i] * p;
+    }
+    pdf = mp;
+    weight = mf * safe_rcp(mp);
+    return true;
+}
+
+// ---------------------------------------------------------------- phases
+// src/Volume.h:21-48, src/VolumeEval.cuh:470-538
+YM_HD float hg_eval(float g, f3 wo, f3 wi, float &pdf)
+{
+    float c = dot(wo, wi);
+    float t = 1.f + g * g + 2.f * g * c;
+    pdf = 0.25f * kInvPi * (1.f - g * g) * safe_rcp(t * sqrtf(gmax(t, 0.0f)));
+    return pdf;
+}
+__device__ __forceinline__ void hg_sample(float g, float u0, float u1, f3 wo, f3 &wi, float &pdf)
+{
+    float c;
+    if (fabsf(g) < 1e-4f) {
+        c = 1.f - 2.f * u0;
+    } else {
+        float t = (1.f - g * g) * safe_rcp(1.f - g + 2.f * g * u0);
+        c = (1.f + g * g - t * t) / (2.f * g);
+    }
+    float s = sqrtf(gmax(0.0f, 1.0f - c * c));
+    float phi = 2.f * kPi * u1;
+    f3 w = wo, u, v;
+    orthogonal_basis(w, u, v);
+    wi = u * s * cosf(phi) + v * s * sinf(phi) + w * c;
+    hg_eval(g, wo, wi, pdf);
+}
+__device__ __forceinline__ bool mixed_phase_sample(const float *vw, const float *gs, int n, float u0, float u1,
+                                                   f3 wo, float &weight, f3 &wi, float &pdf)
+{
+    if (n == 0) return false;
+    float cdf = 0.0f;
+    int sel = -1;
+    for (int i = 0; i < n; ++i) {
+        if (u0 < cdf + vw[i]) {
+            sel = i;
+            u0 = (u0 - cdf) * safe_rcp(vw[i]);
+            break;
+        }
+        cdf += vw[i];
+    }
+    if (sel == -1) sel = n - 1;
+    float pp = 0.0f;
+    hg_sample(gs[sel], u0, u1, wo, wi, pp);
+    const float w1 = 1.0f;
+    float mp = pp * vw[sel];
+    float mg = w1 * pp * vw[sel];
+    for (int i = 0; i < n; ++i) {
+        if (i == sel) continue;
+        float p;
+        mg += vw[i] * hg_eval(gs[i], wo, wi, p);
+        mp += vw[i] * p;
+    }
+    pdf = mp;
+    weight = mg * safe_rcp(mp);
+    return true;
+}
+YM_HD float mixed_phase_eval(const float *vw, const float *gs, int n, f3 wo, f3 wi, float &pdf)
+{
+    if (n == 0) {
+        pdf = 0.0f;
+        return 0.0f;
+    }
+    float mg = 0.0f, mp = 0.0f;
+    for (int i = 0; i < n; ++i) {
+        float p = 0.0f;
+        float g = hg_eval(gs[i], wo, wi, p);
+        mg += vw[i] * g;
+        mp += vw[i] * p;
+    }
+    pdf = mp;
+    return mg;
+}
+
+// ---------------------------------------------------------------- media
+__device__ __forceinline__ float grid_density(const DevScene &sc, int gi, f3 p)
+{
+    const DevGrid g = sc.grids[gi];
+    if (!(p.x >= 0.0f && p.x < 1.0f && p.y >= 0.0f && p.y < 1.0f && p.z >= 0.0f && p.z < 1.0f)) return 0.0f;
+    uint32_t ix = (uint32_t)(p.x * (float)g.nx), iy = (uint32_t)(p.y * (float)g.ny), iz = (uint32_t)(p.z * (float)g.nz);
+    if (ix >= g.nx) ix = g.nx - 1;
+    if (iy >= g.ny) iy = g.ny - 1;
+    if (iz >= g.nz) iz = g.nz - 1;
+    return sc.voxels[g.offset + ((uint64_t)iz * g.ny + iy) * g.nx + ix];
+}
+
+__device__ __forceinline__ bool vol_is_het(const DevScene &sc, int v)
+{
+    const int gi = sc.volumes[v].grid;
+    return gi >= 0 && (uint32_t)gi < sc.grid_count;
+}
+
+__device__ __forceinline__ f3 vsig(const DevVolume &v) { return f3{v.sigma_t[0], v.sigma_t[1], v.sigma_t[2]}; }
+__device__ __forceinline__ f3 valb(const DevVolume &v) { return f3{v.albedo[0], v.albedo[1], v.albedo[2]}; }
+YM_HD f3 exp3(f3 a) { return f3{expf(a.x), expf(a.y), expf(a.z)}; }
+
+// Random source for the shadow-ray transmittance of grid media: PCG hash of
+// (pixel, vertex, sample, frame) and a per-vertex counter.
+struct HashRng {
+    uint32_t key[4];
+    uint32_t ctr;
+    __device__ __forceinline__ float next()
+    {
+        uint32_t v[4] = {key[0], key[1] ^ (ctr << 8), key[2], key[3]};
+        ctr++;
+        pcg4d(v);
+        return (float)(v[0] >> 8) * (1.0f / 16777216.0f);
+    }
+};
+
+// src/VolumeEval.cuh:18-78
+__device__ __forceinline__ bool sample_homogeneous(const DevScene &sc, int n, const int *vi, float tmax, f3 &trw,
+                                                   float &dist, Sampler &s)
+{
+    trw = s3(1.0f);
+    dist = kFloatMax;
+    f3 st = s3(0.0f), ss = s3(0.0f);
+    for (int i = 0; i < n; ++i) {
+        const DevVolume &v = sc.volumes[vi[i]];
+        st = st + vsig(v);
+        ss = ss + vsig(v) * valb(v);
+    }
+    const float csum = st.x + st.y + st.z;
+    const f3 cw = csum > 0.0f ? st / csum : s3(1.0f / 3.0f);
+    float u0 = sampler_next(sc, s);
+    int ch;
+    if (u0 < cw.x) {
+        ch = 0;
+        u0 = u0 * safe_rcp(cw.x);
+    } else if (u0 < cw.x + cw.y) {
+        ch = 1;
+        u0 = (u0 - cw.x) * safe_rcp(cw.y);
+    } else {
+        ch = 2;
+        u0 = (u0 - cw.x - cw.y) * safe_rcp(cw.z);
+    }
+    const float sc_ = comp(st, ch);
+    const float fpl = sc_ > 0.0f ? (-logf(1.0f - u0) / sc_) : kFloatMax;
+    const bool scatter = fpl < tmax;
+    const float m = gmin(tmax, fpl);
+    const f3 tr{expf(-st.x * m), expf(-st.y * m), expf(-st.z * m)};
+    const float pdf = scatter ? (cw.x * st.x * tr.x + cw.y * st.y * tr.y + cw.z * st.z * tr.z)
+                              : (cw.x * tr.x + cw.y * tr.y + cw.z * tr.z);
+    if (!(fabsf(pdf) > 0.0f)) return false;
+    trw = scatter ? (tr * ss) / pdf : tr / pdf;
+    dist = fpl;
+    return scatter;
+}
+
+// src/VolumeEval.cuh:80-214 with voxel-grid densities
+__device__ __forceinline__ bool sample_heterogeneous(const DevScene &sc, int n, const int *vi, f3 ro_w, f3 rd_w,
+                                                     float tmax, f3 &trw, float &dist, Sampler &s)
+{
+    trw = s3(1.0f);
+    dist = kFloatMax;
+    f3 ro[kMaxBoundaryRecord], rd[kMaxBoundaryRecord];
+    f3 sm = s3(0.0f);
+    for (int i = 0; i < n; ++i) {
+        const DevVolume &v = sc.volumes[vi[i]];
+        float ub = 1.0f;
+        if (vol_is_het(sc, vi[i])) {
+            ro[i] = xf_point(v.w2o, ro_w);
+            rd[i] = xf_dir(v.w2o, rd_w);
+            ub = sc.grids[v.grid].max_density;
+        }
+        sm = sm + vsig(v) * ub;
+    }
+    const float csum = sm.x + sm.y + sm.z;
+    const f3 cw = csum > 0.0f ? sm / csum : s3(1.0f / 3.0f);
+    const float u0 = sampler_next(sc, s);
+    const int ch = u0 < cw.x ? 0 : (u0 < cw.x + cw.y ? 1 : 2);
+    if (!(comp(sm, ch) > 0.0f)) return false;
+    const f3 nf{safe_rcp(sm.x + 1.0f), safe_rcp(sm.y + 1.0f), safe_rcp(sm.z + 1.0f)};
+    const f3 smr{safe_rcp(sm.x), safe_rcp(sm.y), safe_rcp(sm.z)};
+    f3 tr = s3(1.0f), pdf = s3(1.0f);
+    float acc = 0.0f;
+    int it = 0;
+    for (;;) {
+        if (it >= kMaxFreePathLength) break;
+        const float fpl = -logf(1.0f - sampler_next(sc, s)) / comp(sm, ch);
+        if (acc + fpl > tmax) {
+            const float dd = tmax - acc;
+            const f3 ptr{expf(-sm.x * dd), expf(-sm.y * dd), expf(-sm.z * dd)};
+            tr = tr * (ptr * nf);
+            pdf = pdf * (ptr * nf);
+            acc += fpl;
+            break;
+        }
+        acc += fpl;
+        f3 st = s3(0.0f), ss = s3(0.0f);
+        for (int i = 0; i < n; ++i) {
+            const DevVolume &v = sc.volumes[vi[i]];
+            float dens = 1.0f;
+            if (vol_is_het(sc, vi[i])) dens = gmax(grid_density(sc, v.grid, ro[i] + rd[i] * acc), 0.0f);
+            st = st + vsig(v) * dens;
+            ss = ss + (vsig(v) * valb(v)) * dens;
+        }
+        f3 rp;
+        rp.x = gmin(sm.x > 0.0f ? st.x * smr.x : 0.0f, 1.0f);
+        rp.y = gmin(sm.y > 0.0f ? st.y * smr.y : 0.0f, 1.0f);
+        rp.z = gmin(sm.z > 0.0f ? st.z * smr.z : 0.0f, 1.0f);
+        const f3 fp = s3(1.0f) - rp;
+        const f3 ptr{expf(-sm.x * fpl), expf(-sm.y * fpl), expf(-sm.z * fpl)};
+        if (sampler_next(sc, s) < comp(rp, ch)) {
+            tr = tr * ((ptr * ss) * nf);
+            pdf = pdf * (((ptr * sm) * rp) * nf);
+            break;
+        }
+        tr = tr * (((ptr * sm) * fp) * nf);
+        pdf = pdf * (((ptr * sm) * fp) * nf);
+        ++it;
+    }
+    trw = tr * safe_rcp(dot(cw, pdf));
+    dist = acc;
+    return (acc > tmax || it >= kMaxFreePathLength) ? false : true;
+}
+
+// EvalTr (src/VolumeEval.cuh:253-418): transmittance of one segment inside
+// the current media.
+__device__ __forceinline__ f3 eval_tr(const DevScene &sc, f3 ro_w, f3 rd_w, const Transfer &t, float tmax,
+                                      HashRng &rng)
+{
+    if (sc.volume_count == 0) return s3(1.0f);
+    int vi[kMaxBoundaryRecord];
+    const int n = t.volumes(vi, sc.volume_count);
+    if (n == 0) return s3(1.0f);
+    bool het = false;
+    for (int i = 0; i < n; ++i)
+        if (vol_is_het(sc, vi[i])) het = true;
+    if (!het) {
+        f3 st = s3(0.0f);
+        for (int i = 0; i < n; ++i) st = st + vsig(sc.volumes[vi[i]]);
+        return f3{expf(-st.x * tmax), expf(-st.y * tmax), expf(-st.z * tmax)};
+    }
+    f3 ro[kMaxBoundaryRecord], rd[kMaxBoundaryRecord];
+    f3 sm = s3(0.0f);
+    for (int i = 0; i < n; ++i) {
+        const DevVolume &v = sc.volumes[vi[i]];
+        float ub = 1.0f;
+        if (vol_is_het(sc, vi[i])) {
+            ro[i] = xf_point(v.w2o, ro_w);
+            rd[i] = xf_dir(v.w2o, rd_w);
+            ub = sc.grids[v.grid].max_density;
+        }
+        sm = sm + vsig(v) * ub;
+    }
+    const float csum = sm.x + sm.y + sm.z;
+    const f3 cw = csum > 0.0f ? sm / csum : s3(1.0f / 3.0f);
+    const float u0 = rng.next();
+    const int ch = u0 < cw.x ? 0 : (u0 < cw.x + cw.y ? 1 : 2);
+    if (!(comp(sm, ch) > 0.0f)) return s3(1.0f);
+    const f3 nf{safe_rcp(sm.x + 1.0f), safe_rcp(sm.y + 1.0f), safe_rcp(sm.z + 1.0f)};
+    const f3 smr{safe_rcp(sm.x), safe_rcp(sm.y), safe_rcp(sm.z)};
+    f3 tr = s3(1.0f), pdf = s3(1.0f);
+    float acc = 0.0f;
+    int it = 0;
+    for (;;) {
+        if (it >= kMaxFreePathLength) break;
+        const float fpl = -logf(1.0f - rng.next()) / comp(sm, ch);
+        if (acc + fpl > tmax) {
+            const float dd = tmax - acc;
+            const f3 ptr{expf(-sm.x * dd), expf(-sm.y * dd), expf(-sm.z * dd)};
+            tr = tr * (ptr * nf);
+            pdf = pdf * (ptr * nf);
+            break;
+        }
+        acc += fpl;
+        f3 st = s3(0.0f);
+        for (int i = 0; i < n; ++i) {
+            const DevVolume &v = sc.volumes[vi[i]];
+            float dens = 1.0f;
+            if (vol_is_het(sc, vi[i])) dens = gmax(grid_density(sc, v.grid, ro[i] + rd[i] * acc), 0.0f);
+            st = st + vsig(v) * dens;
+        }
+        f3 rp;
+        rp.x = gmin(sm.x > 0.0f ? st.x * smr.x : 0.0f, 1.0f);
+        rp.y = gmin(sm.y > 0.0f ? st.y * smr.y : 0.0f, 1.0f);
+        rp.z = gmin(sm.z > 0.0f ? st.z * smr.z : 0.0f, 1.0f);
+        const f3 fp = s3(1.0f) - rp;
+        const f3 ptr{expf(-sm.x * fpl), expf(-sm.y * fpl), expf(-sm.z * fpl)};
+        tr = tr * (((ptr * sm) * fp) * nf);
+        pdf = pdf * ((ptr * sm) * nf);
+        if (it + 1 > 32) {
+            const f3 pt = tr * safe_rcp(dot(cw, pdf));
+            const float rr = gmin(gmax(pt.x, gmax(pt.y, pt.z)), 1.0f);
+            if (rng.next() < rr) tr = tr / gmax(rr, kMinColorEpsilon);
+            else break;
+        }
+        ++it;
+    }
+    return tr * safe_rcp(dot(cw, pdf));
+}
+
+// ---------------------------------------------------------------- lights
+// SampleLightPower (src/Light.h:398-416)
+__device__ __forceinline__ int sample_light_power(const DevScene &sc, float u, float &pdf)
+{
+    int left = 0, right = (int)sc.light_count;
+    while (left < right) {
+        int mid = (left + right) / 2;
+        if (sc.light_cdf[mid] <= u) left = mid + 1;
+        else right = mid;
+    }
+    int idx = left;
+    if (idx > (int)sc.light_count - 1) idx = (int)sc.light_count - 1;
+    pdf = idx == 0 ? sc.light_cdf[idx] : (sc.light_cdf[idx] - sc.light_cdf[idx - 1]);
+    return idx;
+}
+
+struct LightTri {
+    f3 p0, p1, p2, n0, n1, n2;
+    const DevInstance *in;
+    f3 le;
+};
+
+__device__ __forceinline__ LightTri load_light(const DevScene &sc, const DevLight &l)
+{
+    LightTri lt;
+    lt.in = &sc.instances[l.instance];
+    const ShadeTri &st = sc.shade[lt.in->shade_base + l.triangle];
+    lt.p0 = f3{st.p[0], st.p[1], st.p[2]};
+    lt.p1 = f3{st.p[3], st.p[4], st.p[5]};
+    lt.p2 = f3{st.p[6], st.p[7], st.p[8]};
+    lt.n0 = f3{st.n[0], st.n[1], st.n[2]};
+    lt.n1 = f3{st.n[3], st.n[4], st.n[5]};
+    lt.n2 = f3{st.n[6], st.n[7], st.n[8]};
+    const DevMaterial &m = sc.materials[lt.in->material];
+    lt.le = f3{m.light_color[0], m.light_color[1], m.light_color[2]} * m.intensity;
+    return lt;
+}
+
+YM_HD f3 interp3(f3 a, f3 b, f3 c, f3 w) { return a * w.x + b * w.y + c * w.z; }
+
+// ShapeLight::SampleLi, mesh branch (src/Light.h:207-281)
+__device__ __forceinline__ bool light_sample_li(const DevScene &sc, const DevLight &l, f3 pos, float u0, float u1,
+                                                f3 &wi, f3 &lpos, f3 &lerr, f3 &lng, f3 &color, float &pdf)
+{
+    const LightTri lt = load_light(sc, l);
+    const f3 gn = cross(lt.p1 - lt.p0, lt.p2 - lt.p0);
+    const float su = sqrtf(gmax(0.0f, u0));
+    const float bu = 1.0f - su, bv = u1 * su;
+    const f3 b{bu, bv, 1.0f - bu - bv};
+    const float xs = fabsf(lt.p0.x * b.x) + fabsf(lt.p1.x * b.y) + fabsf(lt.p2.x * b.z);
+    const float ys = fabsf(lt.p0.y * b.x) + fabsf(lt.p1.y * b.y) + fabsf(lt.p2.y * b.z);
+    const float zs = fabsf(lt.p0.z * b.x) + fabsf(lt.p1.z * b.y) + fabsf(lt.p2.z * b.z);
+    const f3 sp = interp3(lt.p0, lt.p1, lt.p2, b);
+    f3 se = error_gamma(7) * f3{xs, ys, zs};
+    const f3 sn = interp3(lt.n0, lt.n1, lt.n2, b);
+    const f3 wn = normalize(xf_transpose_dir(lt.in->w2o, sn));
+    lpos = xf_position(lt.in->o2w, sp, &se);
+    lerr = se;
+    lng = normalize(xf_transpose_vector(lt.in->w2o, gn));
+    const float dist = length(lpos - pos);
+    wi = (lpos - pos) * safe_rcp(dist);
+    const float ct = -dot(wn, wi);
+    if (ct <= 0.0f) {
+        pdf = 0.0f;
+        return false;
+    }
+    pdf = sqr(dist) * safe_rcp(l.area * ct);
+    color = lt.le / pdf;
+    return true;
+}
+
+// RayIntersectTriangle (src/Light.h:14-40) with glm 0.9.8's mat3 inverse
+YM_HD bool ray_tri_light(f3 o, f3 d, f3 p0, f3 p1, f3 p2, f3 &bary)
+{
+    const f3 a = p0 - p2, b = p1 - p2;
+    float m[3][3];
+    m[0][0] = a.x;
+    m[0][1] = a.y;
+    m[0][2] = a.z;
+    m[1][0] = b.x;
+    m[1][1] = b.y;
+    m[1][2] = b.z;
+    m[2][0] = -d.x;
+    m[2][1] = -d.y;
+    m[2][2] = -d.z;
+    const float ood = 1.0f / (+m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) -
+                              m[1][0] * (m[0][1] * m[2][2] - m[2][1] * m[0][2]) +
+                              m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]));
+    float iv[3][3];
+    iv[0][0] = +(m[1][1] * m[2][2] - m[2][1] * m[1][2]) * ood;
+    iv[1][0] = -(m[1][0] * m[2][2] - m[2][0] * m[1][2]) * ood;
+    iv[2][0] = +(m[1][0] * m[2][1] - m[2][0] * m[1][1]) * ood;
+    iv[0][1] = -(m[0][1] * m[2][2] - m[2][1] * m[0][2]) * ood;
+    iv[1][1] = +(m[0][0] * m[2][2] - m[2][0] * m[0][2]) * ood;
+    iv[2][1] = -(m[0][0] * m[2][1] - m[2][0] * m[0][1]) * ood;
+    iv[0][2] = +(m[0][1] * m[1][2] - m[1][1] * m[0][2]) * ood;
+    iv[1][2] = -(m[0][0] * m[1][2] - m[1][0] * m[0][2]) * ood;
+    iv[2][2] = +(m[0][0] * m[1][1] - m[1][0] * m[0][1]) * ood;
+    const f3 r = o - p2;
+    const float u = iv[0][0] * r.x + iv[1][0] * r.y + iv[2][0] * r.z;
+    const float v = iv[0][1] * r.x + iv[1][1] * r.y + iv[2][1] * r.z;
+    const float t = iv[0][2] * r.x + iv[1][2] * r.y + iv[2][2] * r.z;
+    const float w = 1.0f - u - v;
+    if (t < kTMin) return false;
+    bary = f3{u, v, w};
+    return (u > 0.0f && u < 1.0f) && (v > 0.0f && v < 1.0f) && (w > 0.0f && w < 1.0f);
+}
+
+// ShapeLight::EvalLi, mesh branch (src/Light.h:87-162)
+__device__ __forceinline__ f3 light_eval_li(const DevScene &sc, const DevLight &l, f3 pos, f3 wi, f3 &lpos, f3 &lerr,
+                                            f3 &lng, float &pdf)
+{
+    const LightTri lt = load_light(sc, l);
+    const f3 gn = cross(lt.p1 - lt.p0, lt.p2 - lt.p0);
+    const f3 owi = xf_vector(lt.in->w2o, wi);
+    const f3 opos = xf_position(lt.in->w2o, pos, nullptr);
+    f3 b = s3(0.0f);
+    if (!ray_tri_light(opos, owi, lt.p0, lt.p1, lt.p2, b)) {
+        pdf = 0.0f;
+        return s3(0.0f);
+    }
+    const float xs = fabsf(lt.p0.x * b.x) + fabsf(lt.p1.x * b.y) + fabsf(lt.p2.x * b.z);
+    const float ys = fabsf(lt.p0.y * b.x) + fabsf(lt.p1.y * b.y) + fabsf(lt.p2.y * b.z);
+    const float zs = fabsf(lt.p0.z * b.x) + fabsf(lt.p1.z * b.y) + fabsf(lt.p2.z * b.z);
+    const f3 sp = interp3(lt.p0, lt.p1, lt.p2, b);
+    f3 se = error_gamma(7) * f3{xs, ys, zs};
+    const f3 sn = interp3(lt.n0, lt.n1, lt.n2, b);
+    const f3 wn = normalize(xf_transpose_dir(lt.in->w2o, sn));
+    const float ct = -dot(wn, wi);
+    if (ct <= 0.0f) {
+        pdf = 0.0f;
+        return s3(0.0f);
+    }
+    lpos = xf_position(lt.in->o2w, sp, &se);
+    lerr = se;
+    lng = normalize(xf_transpose_vector(lt.in->w2o, gn));
+    const float dist = length(lpos - pos);
+    pdf = sqr(dist) * safe_rcp(l.area * ct);
+    return lt.le;
+}
+
+YM_HD float power_heuristic(float main_pdf, float vice_pdf)
+{
+    return safe_rcp(1.0f + sqr(vice_pdf * safe_rcp(main_pdf)));
+}
+
+YM_HD f3 background(f3 d) { return mix3(s3(1.0f), f3{0.05f, 0.2f, 0.85f}, d.y * 0.5f + 0.5f); }
+
+}  // namespace ym

@@ -46,7 +46,7 @@ struct ymrt_scene {
     std::vector<void *> allocations;
     ymrt_scene_info info{};
     uint32_t max_stack = 0;
-    uint32_t stack_slots = 0;     // LDS stack entries used by the kernels
+    bool needs_nearby = false;    // boundaries, dielectrics or ior priorities present
     ym::RenderBuffers *render = nullptr;
     uint32_t *d_error = nullptr;  // device flag: traversal stack overflow
 };

@@ -1,0 +1,915 @@
+// kernels_pt.hip -- wavefront path tracing on gfx950.
+//
+// Stages of one bounce pass (the reference's Ray_generation / Ray_trace /
+// Ray_shading kernels, src/Shading.cu:1634-2302, with its inline shadow rays
+// split into their own stage):
+//   raygen  camera rays for a chunk of (pixel, sample) paths
+//   extend  closest hit + homogeneous/grid free-flight sampling, miss -> env
+//   shade   surface or volume vertex: NEE light and BSDF/phase samples written
+//           as up to two shadow rays, next direction, Russian roulette
+//   shadow  any-hit of each shadow ray (+ boundary-walk transmittance when
+//           media are on), adds the NEE term in the reference's order
+//   resolve per-pixel mean over the samples of the chunk (sample order fixed,
+//           so frames are deterministic -- no float atomics)
+// Path state is SoA in HBM indexed by the path's slot; queues hold slot ids
+// and are compacted per wave64 with a ballot + one atomic per wave.  Every
+// kernel fetches 64 items per wave from a device counter, so launches never
+// need the host to know queue lengths (no host round trip per stage).
+#include "dev_shade.hpp"
+#include "dev_traverse.hpp"
+#include "internal.hpp"
+#include "pt.hpp"
+
+namespace ym {
+
+constexpr int kBlock = 64;
+
+// ------------------------------------------------------------ wave helpers
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t wave_fetch(uint32_t *work)
+{
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(work, (uint32_t)kBlock);
+    return __shfl(base, 0);
+}
+
+// All 64 lanes must call; returns the slot for lanes with pred.
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred)
+{
+    const uint64_t mask = __ballot(pred);
+    const uint32_t total = (uint32_t)__popcll(mask);
+    uint32_t base = 0;
+    if (lane_id() == 0 && total) base = atomicAdd(counter, total);
+    base = __shfl(base, 0);
+    const uint64_t below = mask & ((1ull << lane_id()) - 1ull);
+    return base + (uint32_t)__popcll(below);
+}
+
+__device__ __forceinline__ void wave_add64(unsigned long long *counter, uint32_t v)
+{
+    // sum v over the wave, one atomic
+    uint32_t s = v;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane_id() == 0 && s) atomicAdd(counter, (unsigned long long)s);
+}
+
+// ------------------------------------------------------------ state access
+__device__ __forceinline__ f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
+__device__ __forceinline__ float4 f4(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+
+__device__ __forceinline__ Transfer load_transfer(const PathState &ps, uint32_t p, uint32_t cap)
+{
+    Transfer t;
+    const uint32_t h = ps.xhdr[p];
+    t.count = (int)(h & 0xFFu);
+    t.max_idx = (int)((h >> 8) & 0xFFu) - 1;
+    for (int i = 0; i < t.count; ++i) {
+        const uint4 r = ps.xrec[(size_t)i * cap + p];
+        t.rec[i] = CameraRecord{r.x, __uint_as_float(r.y), r.z, (int32_t)r.w};
+    }
+    return t;
+}
+__device__ __forceinline__ void store_transfer(const PathState &ps, uint32_t p, uint32_t cap, const Transfer &t)
+{
+    ps.xhdr[p] = (uint32_t)t.count | ((uint32_t)(t.max_idx + 1) << 8);
+    for (int i = 0; i < t.count; ++i)
+        ps.xrec[(size_t)i * cap + p] = make_uint4(t.rec[i].prim, __float_as_uint(t.rec[i].ior), t.rec[i].priority,
+                                                  (uint32_t)t.rec[i].vol);
+}
+__device__ __forceinline__ NearbyList load_nearby(const PathState &ps, uint32_t p, uint32_t cap, uint32_t hflags)
+{
+    NearbyList nb;
+    nb.n = (int)((hflags >> 4) & 0xFu);
+    for (int i = 0; i < nb.n; ++i) {
+        const uint2 e = ps.nb[(size_t)i * cap + p];
+        nb.prim[i] = e.x & 0x7FFFFFFFu;
+        nb.back[i] = (int)(e.x >> 31);
+        nb.t[i] = __uint_as_float(e.y);
+    }
+    return nb;
+}
+__device__ __forceinline__ Sampler load_sampler(const PathState &ps, uint32_t p)
+{
+    const uint4 s = ps.smp[p];
+    Sampler r;
+    r.index = (uint64_t)s.x | ((uint64_t)s.y << 32);
+    r.dim = s.z;
+    r.frame_type = s.w;
+    return r;
+}
+__device__ __forceinline__ void store_sampler(const PathState &ps, uint32_t p, const Sampler &s)
+{
+    ps.smp[p] = make_uint4((uint32_t)s.index, (uint32_t)(s.index >> 32), s.dim, s.frame_type);
+}
+
+// info word
+constexpr uint32_t kInfoNeverScatter = 1u << 8;
+constexpr uint32_t kInfoPendingBunch = 1u << 10;
+__device__ __forceinline__ int info_depth(uint32_t i) { return (int)(i & 0xFFu); }
+__device__ __forceinline__ uint32_t info_vertex(uint32_t i) { return i >> 16; }
+
+// hit flags
+constexpr uint32_t kHitBack = 1u;
+constexpr uint32_t kHitVolume = 2u;
+
+// shadow flags: bits 0-1 active rays, 2-3 transmit per ray, 4 volume vertex,
+// 5 path ends after this shadow stage, 8-15 vertex depth
+constexpr uint32_t kShVolume = 1u << 4;
+constexpr uint32_t kShTerminate = 1u << 5;
+
+__device__ __forceinline__ f3 light_clamp(const ymrt_render_settings &rs, f3 c, int depth)
+{
+    if (!(rs.enable_indirect_clamp && depth > 1)) return c;
+    if (!(rs.indirect_limit > 0.0f)) return s3(0.0f);
+    const float lum = luminance(c);
+    if (lum <= 1E-10f) return c;
+    const float lim = rs.indirect_limit;
+    const float pw = gmin(0.05f + 0.25f * powf(lum / lim, 1.5f), 1.0f);
+    const float sf = powf(lim / (lim + lum), pw);
+    return c * sf;
+}
+
+// ------------------------------------------------------------ raygen
+__global__ __launch_bounds__(kBlock) void k_raygen(DevScene sc, PtParams prm, PathState ps)
+{
+    uint32_t *cnt = prm.counters;  // pass 0 slots
+    for (;;) {
+        const uint32_t base = wave_fetch(cnt + kCtrWorkRaygen);
+        if (base >= prm.chunk_paths) break;
+        const uint32_t p = base + lane_id();
+        bool live = false;
+        if (p < prm.chunk_paths) {
+            const uint64_t gp = prm.chunk_base + p;
+            const uint32_t spp = (uint32_t)prm.rs.spp;
+            const uint64_t vpix = gp / spp;
+            const uint32_t s = (uint32_t)(gp - vpix * spp);
+            const uint32_t tile = (uint32_t)(vpix >> 8), within = (uint32_t)(vpix & 255u);
+            const uint32_t ty = tile / prm.tiles_x, tx = tile - ty * prm.tiles_x;
+            const uint32_t px = tx * 16u + (within & 15u), py = ty * 16u + (within >> 4);
+            if (px < prm.rs.width && py < prm.rs.height) {
+                live = true;
+                const uint32_t pixel_index = py * prm.rs.width + px;
+                Sampler smp;
+                float ox, oy;
+                if (prm.rs.sampler_type == YMRT_SAMPLER_HALTON) {
+                    smp.index = prm.henum.index(px, py, (uint32_t)((prm.frame - 1) * prm.rs.spp) + s);
+                    smp.dim = 2;
+                    smp.frame_type = kSamplerHalton;
+                    ox = radical_inverse_plain<2>(smp.index);
+                    oy = radical_inverse_plain<3>(smp.index);
+                    ox = __builtin_fmaf(ox, prm.henum.x_scale, -(float)px);
+                    oy = __builtin_fmaf(oy, prm.henum.y_scale, -(float)py);
+                    ox = gclamp(ox, 0.000001f, 0.999999f);
+                    oy = gclamp(oy, 0.000001f, 0.999999f);
+                } else {
+                    smp.index = (uint64_t)pixel_index | ((uint64_t)s << 32);
+                    smp.dim = 0;
+                    smp.frame_type = (uint32_t)(prm.frame - 1) << 1;
+                    ox = sampler_next(sc, smp);
+                    oy = sampler_next(sc, smp);
+                }
+                const DevCamera &cam = sc.camera;
+                const float nx = ((float)px + ox) / (float)prm.rs.width;
+                const float ny = ((float)py + oy) / (float)prm.rs.height;
+                const float height = 2.0f * cam.tan_half_fov;
+                const float width = height * cam.aspect;
+                const f3 dir = normalize(-cam.w + ((nx - 0.5f) * width) * cam.u + ((ny - 0.5f) * height) * cam.v);
+                ps.ray_o[p] = f4(cam.position, 0.0f);
+                ps.ray_d[p] = f4(dir, 0.0f);
+                ps.last[p] = make_uint2(kEmpty, kEmpty);
+                ps.beta[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+                ps.L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                store_sampler(ps, p, smp);
+                ps.info[p] = 1u | kInfoNeverScatter;
+                Transfer t;
+                t.count = cam.record_count;
+                t.max_idx = cam.max_idx;
+                for (int i = 0; i < kMaxBoundaryRecord; ++i) t.rec[i] = cam.rec[i];
+                store_transfer(ps, p, prm.cap, t);
+            }
+        }
+        const uint32_t slot = wave_append(cnt + kCtrActive, live);
+        if (live) ps.queue[0][slot] = p;
+    }
+}
+
+// ------------------------------------------------------------ extend
+template <bool kNearby, int kStack>
+__global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
+{
+    __shared__ uint32_t stack[kStack * kBlock];
+    __shared__ uint32_t nbp[kNearbyCap * kBlock];
+    __shared__ float nbt[kNearbyCap * kBlock];
+    __shared__ int nbb[kNearbyCap * kBlock];
+    uint32_t *cnt = prm.counters + pass * kCtrStride;
+    const uint32_t *qin = ps.queue[pass & 1];
+    const uint32_t n = cnt[kCtrActive];
+    LaneScratch<kBlock> ls{stack, nbp, nbt, nbb, (int)lane_id()};
+    const ymrt_render_settings &rs = prm.rs;
+    for (;;) {
+        const uint32_t base = wave_fetch(cnt + kCtrWorkExtend);
+        if (base >= n) break;
+        const uint32_t i = base + lane_id();
+        bool want_hit = false;
+        uint32_t p = 0;
+        uint32_t rays = 0, cam_rays = 0;
+        if (i < n) {
+            p = qin[i];
+            uint32_t info = ps.info[p];
+            Transfer t;
+            bool t_loaded = false;
+            if (info & kInfoPendingBunch) {
+                t = load_transfer(ps, p, prm.cap);
+                const NearbyList nb = load_nearby(ps, p, prm.cap, ps.hit_a[p].w);
+                boundary_bunch(sc, t, nb);
+                store_transfer(ps, p, prm.cap, t);
+                t_loaded = true;
+                info &= ~kInfoPendingBunch;
+                ps.info[p] = info;
+            }
+            const float4 o4 = ps.ray_o[p], d4 = ps.ray_d[p];
+            const uint2 last = ps.last[p];
+            Query q;
+            q.o = xyz(o4);
+            q.d = xyz(d4);
+            q.tmin = kTMin;
+            q.tmax = kTMax;
+            q.last_inst = last.x;
+            q.last_tri = last.y;
+            Hit h;
+            h.back = 0;
+            int nbc = 0;
+            TraceCounters tc{0, 0};
+            int r = traverse<kModeAll, false, kNearby, false, kBlock, kStack>(sc, q, h, nbc, ls, tc);
+            if (r < 0) {
+                atomicOr(prm.error, 1u);
+                r = 0;
+            }
+            rays = 1;
+            cam_rays = (info_depth(info) == 1 && (info & kInfoNeverScatter)) ? 1u : 0u;
+            bool hit = r == 1;
+            if (!hit) h.t = kTMax;
+            float4 beta4 = ps.beta[p];
+            f3 beta = xyz(beta4);
+            bool vol_hit = false;
+            f3 vpos = s3(0.0f);
+            if (rs.enable_volume_scattering && sc.volume_count > 0) {
+                if (!t_loaded) t = load_transfer(ps, p, prm.cap);
+                int vi[kMaxBoundaryRecord];
+                const int nv = t.volumes(vi, sc.volume_count);
+                if (nv > 0) {
+                    Sampler smp = load_sampler(ps, p);
+                    bool het = false;
+                    for (int k = 0; k < nv; ++k)
+                        if (vol_is_het(sc, vi[k])) het = true;
+                    f3 trw;
+                    float dist;
+                    const float tmax = hit ? h.t : kTMax;
+                    vol_hit = het ? sample_heterogeneous(sc, nv, vi, q.o, q.d, tmax, trw, dist, smp)
+                                  : sample_homogeneous(sc, nv, vi, tmax, trw, dist, smp);
+                    beta = beta * trw;
+                    ps.beta[p] = f4(beta, 0.0f);
+                    store_sampler(ps, p, smp);
+                    if (vol_hit) vpos = q.o + q.d * dist;
+                }
+            }
+            if (!hit && !vol_hit) {
+                float4 L = ps.L[p];
+                f3 Lv = xyz(L);
+                if (rs.enable_env_light) Lv = Lv + light_clamp(rs, beta * background(q.d), info_depth(info));
+                ps.out[p] = f4(Lv, 0.0f);
+            } else {
+                want_hit = true;
+                uint32_t flags = (vol_hit ? kHitVolume : 0u) | (h.back ? kHitBack : 0u);
+                if (kNearby && hit) {
+                    flags |= (uint32_t)nbc << 4;
+                    for (int k = 0; k < nbc; ++k)
+                        ps.nb[(size_t)k * prm.cap + p] =
+                            make_uint2(nbp[k * kBlock + lane_id()] | ((uint32_t)(nbb[k * kBlock + lane_id()] != 0) << 31),
+                                       __float_as_uint(nbt[k * kBlock + lane_id()]));
+                } else if (hit) {
+                    // without the list the closest instance is its own nearby hit
+                    flags |= 1u << 4;
+                    ps.nb[p] = make_uint2(h.inst | ((uint32_t)(h.back != 0) << 31), __float_as_uint(h.t));
+                }
+                ps.hit_a[p] = make_uint4(__float_as_uint(h.t), hit ? h.inst : kEmpty, h.tri, flags);
+                ps.hit_b[p] = vol_hit ? make_float4(vpos.x, vpos.y, vpos.z, 0.0f) : make_float4(h.b0, h.b1, h.b2, 0.0f);
+            }
+        }
+        const uint32_t slot = wave_append(cnt + kCtrHit, want_hit);
+        if (want_hit) ps.q_hit[slot] = p;
+        wave_add64(prm.stats + kStatExtension, rays);
+        wave_add64(prm.stats + kStatCamera, cam_rays);
+    }
+}
+
+// ------------------------------------------------------------ shade
+struct ShadowOut {
+    uint32_t flags;
+    f3 o[2], d[2];
+    float tmax[2];
+    f3 f[2], li[2];
+    float mis[2];
+};
+
+__device__ __forceinline__ void shadow_from_light(const DevScene &sc, f3 pos, f3 perr, f3 ng, bool offset,
+                                                  f3 ray_d, f3 ldir, f3 lpos, f3 lerr, f3 lng, int k, ShadowOut &so,
+                                                  bool transmit)
+{
+    const f3 orig = offset ? offset_ray_origin(pos, perr, ldir, ng) : pos;
+    const float maxd = length(offset_ray_origin(lpos, lerr, -ldir, lng) - pos) * kShadowRayClamp;
+    so.o[k] = orig;
+    so.d[k] = ldir;
+    so.tmax[k] = maxd;
+    so.flags |= (1u << k) | (transmit ? (4u << k) : 0u);
+    (void)sc;
+    (void)ray_d;
+}
+
+__global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
+{
+    uint32_t *cnt = prm.counters + pass * kCtrStride;
+    uint32_t *cnt_next = prm.counters + (pass + 1) * kCtrStride;
+    const uint32_t n = cnt[kCtrHit];
+    const ymrt_render_settings &rs = prm.rs;
+    const uint32_t cap = prm.cap;
+    for (;;) {
+        const uint32_t base = wave_fetch(cnt + kCtrWorkShade);
+        if (base >= n) break;
+        const uint32_t i = base + lane_id();
+        bool want_next = false, want_shadow = false;
+        uint32_t p = 0;
+        if (i < n) {
+            p = ps.q_hit[i];
+            uint32_t info = ps.info[p];
+            int depth = info_depth(info);
+            bool never_scatter = (info & kInfoNeverScatter) != 0;
+            const uint32_t vertex = info_vertex(info) + 1;
+            const uint4 ha = ps.hit_a[p];
+            const float4 hb = ps.hit_b[p];
+            const float4 o4 = ps.ray_o[p], d4 = ps.ray_d[p];
+            const f3 ray_o = xyz(o4), ray_d = xyz(d4);
+            f3 beta = xyz(ps.beta[p]);
+            f3 L = xyz(ps.L[p]);
+            Sampler smp = load_sampler(ps, p);
+            Transfer t = load_transfer(ps, p, cap);
+            ShadowOut so;
+            so.flags = (uint32_t)depth << 8;
+            bool alive = false;       // a next extension ray
+            bool nee = false;         // shadow rays were produced
+            f3 sh_beta = beta;
+            float sel_pdf = 0.0f;
+            f3 next_o = s3(0.0f), next_d = s3(0.0f);
+            uint32_t next_li = kEmpty, next_lt = kEmpty;
+            bool pending = false;
+            if (rs.enable_volume_scattering && (ha.w & kHitVolume)) {
+                // ---------------------------------------------- volume vertex
+                int vi[kMaxBoundaryRecord];
+                float gs[kMaxBoundaryRecord], vw[kMaxBoundaryRecord];
+                const int nv = t.volumes(vi, sc.volume_count);
+                for (int k = 0; k < nv; ++k) {
+                    gs[k] = sc.volumes[vi[k]].g;
+                    vw[k] = 1.0f / (float)nv;
+                }
+                const f3 vpos = xyz(hb);
+                so.flags |= kShVolume;
+                if (rs.enable_shape_light && sc.light_count > 0 && nv > 0) {
+                    const int li = sample_light_power(sc, sampler_next(sc, smp), sel_pdf);
+                    const DevLight light = sc.lights[li];
+                    {
+                        f3 ldir = s3(0.0f), lpos = s3(0.0f), lerr = s3(0.0f), lng = s3(0.0f), Li = s3(0.0f);
+                        float lpdf = 0.0f;
+                        const float u0 = sampler_next(sc, smp);
+                        const float u1 = sampler_next(sc, smp);
+                        const bool valid = light_sample_li(sc, light, vpos, u0, u1, ldir, lpos, lerr, lng, Li, lpdf);
+                        float wpdf = 0.0f;
+                        const float ph = mixed_phase_eval(vw, gs, nv, -ray_d, ldir, wpdf);
+                        if (ph > kMinColorEpsilon && valid) {
+                            shadow_from_light(sc, vpos, s3(0.0f), s3(0.0f), false, ray_d, ldir, lpos, lerr, lng, 0,
+                                              so, false);
+                            so.f[0] = s3(gmin(ph, kMaxColorClamp));
+                            so.li[0] = min3(Li, s3(kMaxColorClamp));
+                            so.mis[0] = power_heuristic(lpdf, wpdf);
+                        }
+                    }
+                    {
+                        f3 pwi = s3(0.0f);
+                        float ppdf = 0.0f, pw = 0.0f;
+                        const float u0 = sampler_next(sc, smp);
+                        const float u1 = sampler_next(sc, smp);
+                        if (mixed_phase_sample(vw, gs, nv, u0, u1, -ray_d, pw, pwi, ppdf)) {
+                            f3 lpos = s3(0.0f), lerr = s3(0.0f), lng = s3(0.0f);
+                            float lpdf = 0.0f;
+                            const f3 Li = light_eval_li(sc, light, vpos, pwi, lpos, lerr, lng, lpdf);
+                            if (maxcomp(Li) > kMinColorEpsilon) {
+                                shadow_from_light(sc, vpos, s3(0.0f), s3(0.0f), false, ray_d, pwi, lpos, lerr, lng, 1,
+                                                  so, false);
+                                so.f[1] = s3(gmin(pw, kMaxColorClamp));
+                                so.li[1] = min3(Li, s3(kMaxColorClamp));
+                                so.mis[1] = power_heuristic(ppdf, lpdf);
+                            }
+                        }
+                    }
+                    nee = (so.flags & 3u) != 0;
+                }
+                f3 wi = s3(0.0f);
+                float pdf = 0.0f, pw = 0.0f;
+                const float u0 = sampler_next(sc, smp);
+                const float u1 = sampler_next(sc, smp);
+                if (mixed_phase_sample(vw, gs, nv, u0, u1, -ray_d, pw, wi, pdf)) {
+                    beta = beta * pw;
+                    const float rr = gmin(gmax(beta.x, gmax(beta.y, beta.z)), 1.0f);
+                    bool cont = true;
+                    if (depth + 1 > rs.ray_depth) {
+                        if (rs.enable_russian_roulette && sampler_next(sc, smp) < rr) beta = beta / gmax(rr, kMinColorEpsilon);
+                        else cont = false;
+                    }
+                    if (cont && depth + 1 <= kMaxRayDepth) {
+                        alive = true;
+                        next_o = vpos;
+                        next_d = normalize(wi);
+                        never_scatter = false;
+                        depth += 1;
+                    }
+                }
+            } else {
+                // ---------------------------------------------- surface vertex
+                const uint32_t inst = ha.y, tri = ha.z;
+                const int back = (ha.w & kHitBack) ? 1 : 0;
+                const DevInstance in = sc.instances[inst];
+                const ShadeTri &st = sc.shade[in.shade_base + tri];
+                const f3 b{hb.x, hb.y, hb.z};
+                const f3 p0{st.p[0], st.p[1], st.p[2]}, p1{st.p[3], st.p[4], st.p[5]}, p2{st.p[6], st.p[7], st.p[8]};
+                const float xs = fabsf(p0.x * b.x) + fabsf(p1.x * b.y) + fabsf(p2.x * b.z);
+                const float ys = fabsf(p0.y * b.x) + fabsf(p1.y * b.y) + fabsf(p2.y * b.z);
+                const float zs = fabsf(p0.z * b.x) + fabsf(p1.z * b.y) + fabsf(p2.z * b.z);
+                f3 pos = interp3(p0, p1, p2, b);
+                f3 perr = error_gamma(7) * f3{xs, ys, zs};
+                f3 ng = cross(p1 - p0, p2 - p0);
+                f3 sn = (in.flags & kInstNoNormals)
+                            ? ng
+                            : interp3(f3{st.n[0], st.n[1], st.n[2]}, f3{st.n[3], st.n[4], st.n[5]},
+                                      f3{st.n[6], st.n[7], st.n[8]}, b);
+                const f2 uv0{st.uv[0], st.uv[1]}, uv1{st.uv[2], st.uv[3]}, uv2{st.uv[4], st.uv[5]};
+                const f2 uv{uv0.x * b.x + uv1.x * b.y + uv2.x * b.z, uv0.y * b.x + uv1.y * b.y + uv2.y * b.z};
+                float du01 = uv1.x - uv0.x, dv01 = uv1.y - uv0.y, du02 = uv2.x - uv0.x, dv02 = uv2.y - uv0.y;
+                float det = du01 * dv02 - dv01 * du02;
+                if (fabsf(det) < 1E-8f) {
+                    du01 = 1.0f;
+                    dv01 = 0.0f;
+                    du02 = 0.0f;
+                    dv02 = 1.0f;
+                    det = 1.0f;
+                }
+                const float idet = 1.0f / det;
+                const f3 dp01 = p1 - p0, dp02 = p2 - p0;
+                f3 dpdu = (dv02 * dp01 - dv01 * dp02) * idet;
+                f3 dpdv = (-du02 * dp01 + du01 * dp02) * idet;
+                pos = xf_position(in.o2w, pos, &perr);
+                sn = normalize(xf_transpose_vector(in.w2o, sn));
+                ng = normalize(xf_transpose_vector(in.w2o, ng));
+                dpdu = xf_vector(in.o2w, dpdu);
+                dpdv = xf_vector(in.o2w, dpdv);
+                const DevMaterial &m = sc.materials[in.material];
+                uint32_t prio;
+                fetch_ior(m, &prio);
+                if ((t.count != 0 && prio < t.rec[t.max_idx].priority) || (in.flags & kInstBoundary)) {
+                    // pass through without consuming depth; the medium records
+                    // change at the start of the next extend
+                    pending = true;
+                    alive = true;
+                    next_o = offset_ray_origin(pos, perr, ray_d, ng);
+                    next_d = normalize(ray_d);
+                    next_li = inst;
+                    next_lt = tri;
+                } else if (m.type == YMRT_MATERIAL_LIGHT) {
+                    if (never_scatter)
+                        L = L + light_clamp(rs, beta * f3{m.light_color[0], m.light_color[1], m.light_color[2]} *
+                                                    m.intensity,
+                                            depth);
+                } else {
+                    Bsdf bs;
+                    init_shading_space(bs, back, ray_d, ng, sn, dpdu, dpdv);
+                    bs.is_default = m.type == YMRT_MATERIAL_DEFAULT;
+                    bs.count = 0;
+                    if (bs.is_default)
+                        init_default_bsdf(sc, bs, m, uv, ray_d, back, t.ray_ior(), t.ex_ior(inst));
+                    const f3 wo = bs.to_local(-ray_d);
+                    if (rs.enable_shape_light && sc.light_count > 0) {
+                        const int li = sample_light_power(sc, sampler_next(sc, smp), sel_pdf);
+                        const DevLight light = sc.lights[li];
+                        {
+                            f3 ldir = s3(0.0f), lpos = s3(0.0f), lerr = s3(0.0f), lng = s3(0.0f), Li = s3(0.0f);
+                            float lpdf = 0.0f;
+                            const float u0 = sampler_next(sc, smp);
+                            const float u1 = sampler_next(sc, smp);
+                            const bool valid = light_sample_li(sc, light, pos, u0, u1, ldir, lpos, lerr, lng, Li, lpdf);
+                            const f3 lwi = bs.to_local(ldir);
+                            float wpdf = 0.0f;
+                            const f3 f = bsdf_eval(bs, wo, lwi, wpdf);
+                            if (maxcomp(f) > kMinColorEpsilon && valid) {
+                                const bool transmit = dot(-ray_d, ng) * dot(ldir, ng) < 0.0f;
+                                const bool invalid = (!same_hemi(wo, lwi) && !transmit) || (same_hemi(wo, lwi) && transmit);
+                                if (!invalid) {
+                                    shadow_from_light(sc, pos, perr, ng, true, ray_d, ldir, lpos, lerr, lng, 0, so,
+                                                      transmit);
+                                    so.f[0] = min3(f, s3(kMaxColorClamp));
+                                    so.li[0] = min3(Li, s3(kMaxColorClamp));
+                                    so.mis[0] = power_heuristic(lpdf, wpdf);
+                                }
+                            }
+                        }
+                        {
+                            f3 bw = s3(0.0f), bwi = s3(0.0f);
+                            float bpdf = 0.0f;
+                            const float u0 = sampler_next(sc, smp);
+                            const float u1 = sampler_next(sc, smp);
+                            const float u2 = sampler_next(sc, smp);
+                            if (bsdf_sample(bs, wo, bw, bwi, bpdf, u0, u1, u2)) {
+                                const f3 ldir = bs.to_world(bwi);
+                                f3 lpos = s3(0.0f), lerr = s3(0.0f), lng = s3(0.0f);
+                                float lpdf = 0.0f;
+                                const f3 Li = light_eval_li(sc, light, pos, ldir, lpos, lerr, lng, lpdf);
+                                if (maxcomp(Li) > kMinColorEpsilon) {
+                                    const bool transmit = dot(-ray_d, ng) * dot(ldir, ng) < 0.0f;
+                                    const bool invalid =
+                                        (!same_hemi(wo, bwi) && !transmit) || (same_hemi(wo, bwi) && transmit);
+                                    if (!invalid) {
+                                        shadow_from_light(sc, pos, perr, ng, true, ray_d, ldir, lpos, lerr, lng, 1, so,
+                                                          transmit);
+                                        so.f[1] = min3(bw, s3(kMaxColorClamp));
+                                        so.li[1] = min3(Li, s3(kMaxColorClamp));
+                                        so.mis[1] = power_heuristic(bpdf, lpdf);
+                                    }
+                                }
+                            }
+                        }
+                        nee = (so.flags & 3u) != 0;
+                    }
+                    f3 wi = s3(0.0f), bw = s3(0.0f);
+                    float pdf = 0.0f;
+                    const float u0 = sampler_next(sc, smp);
+                    const float u1 = sampler_next(sc, smp);
+                    const float u2 = sampler_next(sc, smp);
+                    if (bsdf_sample(bs, wo, bw, wi, pdf, u0, u1, u2)) {
+                        beta = beta * bw;
+                        const float rr = gmin(gmax(beta.x, gmax(beta.y, beta.z)), 1.0f);
+                        bool cont = true;
+                        if (depth + 1 > rs.ray_depth) {
+                            if (rs.enable_russian_roulette && sampler_next(sc, smp) < rr) beta = beta / gmax(rr, kMinColorEpsilon);
+                            else cont = false;
+                        }
+                        if (cont) {
+                            const f3 nd = bs.to_world(wi);
+                            const bool transmit = dot(-ray_d, ng) * dot(nd, ng) < 0.0f;
+                            const bool invalid = (!same_hemi(wo, wi) && !transmit) || (same_hemi(wo, wi) && transmit);
+                            if (!invalid && depth + 1 <= kMaxRayDepth) {
+                                pending = transmit;
+                                alive = true;
+                                next_o = offset_ray_origin(pos, perr, nd, ng);
+                                next_d = normalize(nd);
+                                next_li = inst;
+                                next_lt = tri;
+                                never_scatter = false;
+                                depth += 1;
+                            }
+                        }
+                    }
+                }
+            }
+            // ---------------------------------------------- write back
+            store_sampler(ps, p, smp);
+            ps.L[p] = f4(L, 0.0f);
+            if (nee) {
+                want_shadow = true;
+                if (!alive) so.flags |= kShTerminate;
+                ps.sh_flags[p] = so.flags;
+                ps.sh_c[p] = f4(sh_beta, sel_pdf);
+                for (int k = 0; k < 2; ++k) {
+                    if (!(so.flags & (1u << k))) continue;
+                    ps.sh_o[(size_t)k * cap + p] = f4(so.o[k], so.tmax[k]);
+                    ps.sh_d[(size_t)k * cap + p] = f4(so.d[k], 0.0f);
+                    ps.sh_f[(size_t)k * cap + p] = f4(so.f[k], so.mis[k]);
+                    ps.sh_li[(size_t)k * cap + p] = f4(so.li[k], 0.0f);
+                }
+            } else if (!alive) {
+                ps.out[p] = f4(L, 0.0f);
+            }
+            if (alive) {
+                want_next = true;
+                ps.ray_o[p] = f4(next_o, 0.0f);
+                ps.ray_d[p] = f4(next_d, 0.0f);
+                ps.last[p] = make_uint2(next_li, next_lt);
+                ps.beta[p] = f4(beta, 0.0f);
+            }
+            info = (uint32_t)depth | (never_scatter ? kInfoNeverScatter : 0u) | (pending ? kInfoPendingBunch : 0u) |
+                   (vertex << 16);
+            ps.info[p] = info;
+        }
+        const uint32_t s1 = wave_append(cnt + kCtrShadow, want_shadow);
+        if (want_shadow) ps.q_shadow[s1] = p;
+        const uint32_t s2 = wave_append(cnt_next + kCtrActive, want_next);
+        if (want_next) ps.queue[(pass + 1) & 1][s2] = p;
+    }
+}
+
+// ------------------------------------------------------------ shadow
+template <bool kMedia, int kStack>
+__global__ __launch_bounds__(kBlock) void k_shadow(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
+{
+    __shared__ uint32_t stack[kStack * kBlock];
+    __shared__ uint32_t nbp[kNearbyCap * kBlock];
+    __shared__ float nbt[kNearbyCap * kBlock];
+    __shared__ int nbb[kNearbyCap * kBlock];
+    uint32_t *cnt = prm.counters + pass * kCtrStride;
+    const uint32_t n = cnt[kCtrShadow];
+    const uint32_t cap = prm.cap;
+    LaneScratch<kBlock> ls{stack, nbp, nbt, nbb, (int)lane_id()};
+    for (;;) {
+        const uint32_t base = wave_fetch(cnt + kCtrWorkShadow);
+        if (base >= n) break;
+        const uint32_t i = base + lane_id();
+        uint32_t nsh = 0, ntr = 0;
+        if (i < n) {
+            const uint32_t p = ps.q_shadow[i];
+            const uint32_t fl = ps.sh_flags[p];
+            const float4 c4 = ps.sh_c[p];
+            f3 direct = s3(0.0f);
+            Transfer t0;
+            NearbyList nb0;
+            uint4 ha = make_uint4(0, kEmpty, kEmpty, 0);
+            const uint32_t info = ps.info[p];
+            HashRng rng;
+            if (kMedia) {
+                t0 = load_transfer(ps, p, cap);
+                ha = ps.hit_a[p];
+                nb0 = load_nearby(ps, p, cap, ha.w);
+                const uint32_t gp = (uint32_t)(prm.chunk_base + p);
+                const uint32_t spp = (uint32_t)prm.rs.spp;
+                const uint32_t vpix = gp / spp, s = gp - vpix * spp;
+                const uint32_t tile = vpix >> 8, within = vpix & 255u;
+                const uint32_t ty = tile / prm.tiles_x, tx = tile - ty * prm.tiles_x;
+                const uint32_t px = tx * 16u + (within & 15u), py = ty * 16u + (within >> 4);
+                rng.key[0] = py * prm.rs.width + px;
+                rng.key[1] = info_vertex(info);
+                rng.key[2] = s;
+                rng.key[3] = (uint32_t)prm.frame;
+                rng.ctr = 0;
+            }
+            for (int k = 0; k < 2; ++k) {
+                if (!(fl & (1u << k))) continue;
+                const float4 o4 = ps.sh_o[(size_t)k * cap + p];
+                const float4 d4 = ps.sh_d[(size_t)k * cap + p];
+                Query q;
+                q.o = xyz(o4);
+                q.d = xyz(d4);
+                q.tmin = kTMin;
+                q.tmax = o4.w;
+                q.last_inst = kEmpty;
+                q.last_tri = kEmpty;
+                Hit h;
+                int nbc = 0;
+                TraceCounters tc{0, 0};
+                int r = traverse<kModeSkipBoundary, true, false, false, kBlock, kStack>(sc, q, h, nbc, ls, tc);
+                ++nsh;
+                if (r < 0) {
+                    atomicOr(prm.error, 1u);
+                    continue;
+                }
+                if (r == 1) continue;
+                f3 tr = s3(1.0f);
+                if (kMedia && prm.rs.enable_volume_scattering) {
+                    // TraceTr (src/VolumeEval.cuh:420-468)
+                    Transfer t = t0;
+                    if (fl & (4u << k)) boundary_bunch(sc, t, nb0);
+                    const bool volv = (fl & kShVolume) != 0;
+                    Query tq = q;
+                    tq.last_inst = volv ? kEmpty : ha.y;
+                    tq.last_tri = volv ? kEmpty : ha.z;
+                    const float maxd = o4.w;
+                    float traced = kTMin;
+                    int guard = 0;
+                    while (traced < maxd && sc.volume_count > 0) {
+                        Hit bh;
+                        bh.back = 0;
+                        int bnc = 0;
+                        int br = 0;
+                        ++ntr;
+                        if (sc.has_boundaries)
+                            br = traverse<kModeBoundaryOnly, false, true, false, kBlock, kStack>(sc, tq, bh, bnc, ls, tc);
+                        if (br < 0) {
+                            atomicOr(prm.error, 1u);
+                            break;
+                        }
+                        if (br == 0) {
+                            tr = tr * eval_tr(sc, tq.o, tq.d, t, gmax(maxd - traced, 0.0f), rng);
+                            break;
+                        }
+                        tr = tr * eval_tr(sc, tq.o, tq.d, t, bh.t, rng);
+                        traced += bh.t;
+                        const DevInstance &bin = sc.instances[bh.inst];
+                        const ShadeTri &st = sc.shade[bin.shade_base + bh.tri];
+                        const f3 p0{st.p[0], st.p[1], st.p[2]}, p1{st.p[3], st.p[4], st.p[5]},
+                            p2{st.p[6], st.p[7], st.p[8]};
+                        const float xs = fabsf(p0.x * bh.b0) + fabsf(p1.x * bh.b1) + fabsf(p2.x * bh.b2);
+                        const float ys = fabsf(p0.y * bh.b0) + fabsf(p1.y * bh.b1) + fabsf(p2.y * bh.b2);
+                        const float zs = fabsf(p0.z * bh.b0) + fabsf(p1.z * bh.b1) + fabsf(p2.z * bh.b2);
+                        f3 bp = p0 * bh.b0 + p1 * bh.b1 + p2 * bh.b2;
+                        f3 bperr = error_gamma(7) * f3{xs, ys, zs};
+                        f3 bng = cross(p1 - p0, p2 - p0);
+                        bp = xf_position(bin.o2w, bp, &bperr);
+                        bng = normalize(xf_transpose_vector(bin.w2o, bng));
+                        NearbyList bnb;
+                        bnb.n = bnc;
+                        for (int j = 0; j < bnc; ++j) {
+                            bnb.prim[j] = nbp[j * kBlock + lane_id()];
+                            bnb.t[j] = nbt[j * kBlock + lane_id()];
+                            bnb.back[j] = nbb[j * kBlock + lane_id()];
+                        }
+                        boundary_bunch(sc, t, bnb);
+                        tq.o = offset_ray_origin(bp, bperr, tq.d, bng);
+                        tq.tmax = kTMax;
+                        tq.last_inst = bh.inst;
+                        tq.last_tri = bh.tri;
+                        if (++guard > 64) break;
+                    }
+                }
+                const float4 f4v = ps.sh_f[(size_t)k * cap + p];
+                const float4 li4 = ps.sh_li[(size_t)k * cap + p];
+                direct = direct + ((f4v.w * tr) * xyz(f4v)) * xyz(li4);
+            }
+            // EvalShapeLight returns direct * SafeRcp(select pdf); the volume
+            // variant divides when the pdf is positive.
+            const f3 sel = (fl & kShVolume) ? (c4.w > 0.0f ? direct / c4.w : s3(0.0f)) : direct * safe_rcp(c4.w);
+            f3 L = xyz(ps.L[p]);
+            L = L + light_clamp(prm.rs, xyz(c4) * sel, (int)((fl >> 8) & 0xFFu));
+            ps.L[p] = f4(L, 0.0f);
+            if (fl & kShTerminate) ps.out[p] = f4(L, 0.0f);
+        }
+        wave_add64(prm.stats + kStatShadow, nsh);
+        wave_add64(prm.stats + kStatTransmittance, ntr);
+    }
+}
+
+// ------------------------------------------------------------ flush
+// Paths still queued after the last pass end with the radiance they carry.
+__global__ __launch_bounds__(kBlock) void k_flush(PtParams prm, PathState ps, uint32_t pass)
+{
+    uint32_t *cnt = prm.counters + pass * kCtrStride;
+    const uint32_t n = cnt[kCtrActive];
+    const uint32_t *q = ps.queue[pass & 1];
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const uint32_t p = q[i];
+        ps.out[p] = ps.L[p];
+    }
+}
+
+// ------------------------------------------------------------ resolve
+__global__ void k_resolve(PtParams prm, PathState ps, float4 *noise)
+{
+    const uint32_t spp = (uint32_t)prm.rs.spp;
+    const uint32_t pix_in_chunk = prm.chunk_paths / spp;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= pix_in_chunk) return;
+    const uint64_t vpix = prm.chunk_base / spp + k;
+    const uint32_t tile = (uint32_t)(vpix >> 8), within = (uint32_t)(vpix & 255u);
+    const uint32_t ty = tile / prm.tiles_x, tx = tile - ty * prm.tiles_x;
+    const uint32_t px = tx * 16u + (within & 15u), py = ty * 16u + (within >> 4);
+    if (px >= prm.rs.width || py >= prm.rs.height) return;
+    const float inv = 1.0f / (float)(prm.rs.spp > 1 ? prm.rs.spp : 1);
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+    const float4 *o = ps.out + (size_t)k * spp;
+    for (uint32_t s = 0; s < spp; ++s) {
+        const float4 v = o[s];
+        a0 += v.x * inv;
+        a1 += v.y * inv;
+        a2 += v.z * inv;
+        a3 += inv;
+    }
+    noise[(size_t)py * prm.rs.width + px] = make_float4(a0, a1, a2, a3);
+}
+
+// ------------------------------------------------------------ post
+__global__ void k_accumulate(const float4 *noise, float4 *accum, uint32_t n, int frame)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 v = noise[i];
+    if (frame == 1) {
+        accum[i] = v;
+        return;
+    }
+    const float4 a = accum[i];
+    const float fm1 = (float)(frame - 1), fr = (float)frame;
+    accum[i] = make_float4((a.x * fm1 + v.x) / fr, (a.y * fm1 + v.y) / fr, (a.z * fm1 + v.z) / fr,
+                           (a.w * fm1 + v.w) / fr);
+}
+
+__global__ void k_tonemap(float4 *img, uint32_t n, float gamma, float exposure)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 v = img[i];
+    float c[3] = {v.x, v.y, v.z};
+    for (int k = 0; k < 3; ++k) {
+        const float x = c[k] * exposure;
+        const float y = (x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f);
+        c[k] = powf(y, 1.0f / gamma);
+    }
+    img[i] = make_float4(c[0], c[1], c[2], 1.0f);
+}
+
+// ------------------------------------------------------------ launchers
+static dim3 persistent_grid(uint32_t items)
+{
+    // enough waves to fill 256 CUs several times over, never more than items need
+    uint32_t waves = (items + kBlock - 1) / kBlock;
+    uint32_t cap = 256u * 16u;
+    return dim3(waves < cap ? (waves ? waves : 1) : cap);
+}
+
+int launch_raygen(const DevScene &sc, const PtParams &prm, const PathState &ps, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_raygen, persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps);
+    YM_HIP(hipGetLastError());
+    return YMRT_OK;
+}
+
+template <bool kNearby, int kStack>
+static int launch_extend_t(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_extend<kNearby, kStack>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps,
+                       pass);
+    YM_HIP(hipGetLastError());
+    return YMRT_OK;
+}
+
+template <bool kMedia, int kStack>
+static int launch_shadow_t(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_shadow<kMedia, kStack>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps,
+                       pass);
+    YM_HIP(hipGetLastError());
+    return YMRT_OK;
+}
+
+int launch_extend(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool nearby,
+                  uint32_t stack, hipStream_t s)
+{
+    if (stack <= 32) return nearby ? launch_extend_t<true, 32>(sc, prm, ps, pass, s) : launch_extend_t<false, 32>(sc, prm, ps, pass, s);
+    if (stack <= 48) return nearby ? launch_extend_t<true, 48>(sc, prm, ps, pass, s) : launch_extend_t<false, 48>(sc, prm, ps, pass, s);
+    return nearby ? launch_extend_t<true, 64>(sc, prm, ps, pass, s) : launch_extend_t<false, 64>(sc, prm, ps, pass, s);
+}
+
+int launch_shade(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_shade, persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps, pass);
+    YM_HIP(hipGetLastError());
+    return YMRT_OK;
+}
+
+int launch_shadow(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool media,
+                  uint32_t stack, hipStream_t s)
+{
+    if (stack <= 32) return media ? launch_shadow_t<true, 32>(sc, prm, ps, pass, s) : launch_shadow_t<false, 32>(sc, prm, ps, pass, s);
+    if (stack <= 48) return media ? launch_shadow_t<true, 48>(sc, prm, ps, pass, s) : launch_shadow_t<false, 48>(sc, prm, ps, pass, s);
+    return media ? launch_shadow_t<true, 64>(sc, prm, ps, pass, s) : launch_shadow_t<false, 64>(sc, prm, ps, pass, s);
+}
+
+int launch_flush(const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_flush, dim3(1024), dim3(kBlock), 0, s, prm, ps, pass);
+    YM_HIP(hipGetLastError());
+    return YMRT_OK;
+}
+
+int launch_resolve(const PtParams &prm, const PathState &ps, float *noise, hipStream_t s)
+{
+    const uint32_t pix = prm.chunk_paths / (uint32_t)prm.rs.spp;
+    hipLaunchKernelGGL(k_resolve, dim3((pix + 255) / 256), dim3(256), 0, s, prm, ps, reinterpret_cast<float4 *>(noise));
+    YM_HIP(hipGetLastError());
+    return YMRT_OK;
+}
+
+int accumulate(const float *noise, float *accum, uint32_t w, uint32_t h, int frame, hipStream_t s)
+{
+    const uint32_t n = w * h;
+    if (n == 0) return YMRT_OK;
+    hipLaunchKernelGGL(k_accumulate, dim3((n + 255) / 256), dim3(256), 0, s, reinterpret_cast<const float4 *>(noise),
+                       reinterpret_cast<float4 *>(accum), n, frame);
+    YM_HIP(hipGetLastError());
+    return YMRT_OK;
+}
+
+int tonemap(float *rgba, uint32_t w, uint32_t h, float gamma, float exposure, hipStream_t s)
+{
+    const uint32_t n = w * h;
+    if (n == 0) return YMRT_OK;
+    hipLaunchKernelGGL(k_tonemap, dim3((n + 255) / 256), dim3(256), 0, s, reinterpret_cast<float4 *>(rgba), n, gamma,
+                       exposure);
+    YM_HIP(hipGetLastError());
+    return YMRT_OK;
+}
+
+}  // namespace ym

@@ -1,0 +1,61 @@
+// pt.hpp -- wavefront path-tracing state shared by render.cpp and kernels_pt.hip.
+#pragma once
+
+#include "../../include/ymrt.h"
+#include "dev_sampler.hpp"
+
+namespace ym {
+
+// Per-pass device counters: counters[pass * kCtrStride + k]
+constexpr uint32_t kCtrActive = 0;       // extension rays queued for this pass
+constexpr uint32_t kCtrHit = 1;          // hits queued for shading
+constexpr uint32_t kCtrShadow = 2;       // paths with shadow rays
+constexpr uint32_t kCtrWorkRaygen = 3;   // work-fetch cursors
+constexpr uint32_t kCtrWorkExtend = 4;
+constexpr uint32_t kCtrWorkShade = 5;
+constexpr uint32_t kCtrWorkShadow = 6;
+constexpr uint32_t kCtrWorkFlush = 7;
+constexpr uint32_t kCtrStride = 8;
+constexpr uint32_t kMaxPasses = 4 * kMaxRayDepth;
+
+// Frame statistics (device, 64-bit)
+constexpr uint32_t kStatCamera = 0;
+constexpr uint32_t kStatExtension = 1;
+constexpr uint32_t kStatShadow = 2;
+constexpr uint32_t kStatTransmittance = 3;
+constexpr uint32_t kStatCount = 8;
+
+// Structure-of-arrays path state for `cap` paths in flight (indexed by slot).
+struct PathState {
+    float4 *ray_o, *ray_d;
+    uint2 *last;
+    float4 *beta, *L;
+    uint4 *smp;
+    uint32_t *info;
+    uint32_t *xhdr;
+    uint4 *xrec;        // [kMaxBoundaryRecord][cap]
+    uint4 *hit_a;       // t bits, instance, triangle, flags
+    float4 *hit_b;      // barycentrics or volume position
+    uint2 *nb;          // [4][cap] nearby (instance | back << 31, t bits)
+    float4 *sh_o, *sh_d, *sh_f, *sh_li;  // [2][cap]
+    float4 *sh_c;       // throughput at the vertex, light-select pdf
+    uint32_t *sh_flags;
+    float4 *out;        // final radiance per path
+    uint32_t *queue[2];
+    uint32_t *q_hit, *q_shadow;
+};
+
+struct PtParams {
+    ymrt_render_settings rs;
+    HaltonEnum henum;
+    int frame;
+    uint32_t tiles_x;
+    uint64_t chunk_base;     // first (virtual pixel * spp + sample) of the chunk
+    uint32_t chunk_paths;
+    uint32_t cap;
+    uint32_t *counters;
+    unsigned long long *stats;
+    uint32_t *error;
+};
+
+}  // namespace ym

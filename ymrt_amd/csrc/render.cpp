@@ -1,9 +1,48 @@
-// render.cpp -- frame driver (filled in with the wavefront path tracer).
+// render.cpp -- host driver of one path-traced frame.
+//
+// Replaces RenderModule::PathTracingExecuteTask (src/core/RenderModule.cpp:494-653).
+// The reference regenerates paths into a 1M-ray queue and round-trips three
+// counters through the host after every stage (8 synchronisations per batch).
+// Here the frame is cut into chunks of whole pixels (capacity paths each, 4M
+// by default: ~1.2 GB of state, a sliver of the 288 GB HBM); each chunk runs
+// raygen, then extend/shade/shadow passes whose queue lengths live in device
+// counters, then a resolve into the noise image.  The host reads one counter
+// per chunk, after ray_depth + 1 passes, to decide whether boundary
+// pass-throughs need more passes.
+#include <algorithm>
+#include <cstring>
+
 #include "internal.hpp"
+#include "pt.hpp"
 
 namespace ym {
 
-struct RenderBuffers {};
+int launch_raygen(const DevScene &sc, const PtParams &prm, const PathState &ps, hipStream_t s);
+int launch_extend(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool nearby,
+                  uint32_t stack, hipStream_t s);
+int launch_shade(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s);
+int launch_shadow(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool media,
+                  uint32_t stack, hipStream_t s);
+int launch_flush(const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s);
+int launch_resolve(const PtParams &prm, const PathState &ps, float *noise, hipStream_t s);
+
+constexpr uint32_t kDefaultCapacity = 1u << 22;
+
+struct RenderBuffers {
+    uint32_t cap = 0;
+    PathState ps{};
+    std::vector<void *> allocs;
+    uint32_t *counters = nullptr;
+    unsigned long long *stats = nullptr;
+    std::vector<hipEvent_t> events;
+    size_t event_next = 0;
+
+    ~RenderBuffers()
+    {
+        for (void *p : allocs) (void)hipFree(p);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    }
+};
 
 void destroy_render_buffers(ymrt_scene *sc)
 {
@@ -11,20 +50,201 @@ void destroy_render_buffers(ymrt_scene *sc)
     sc->render = nullptr;
 }
 
-int render_frame(ymrt_scene *, const ymrt_render_settings *, int, float *, hipStream_t, int, ymrt_frame_stats *)
+template <typename T>
+static int alloc(RenderBuffers *rb, T **p, size_t count)
 {
-    set_error("path tracing not built yet");
-    return YMRT_ERR_UNSUPPORTED;
+    void *m = nullptr;
+    YM_HIP(hipMalloc(&m, sizeof(T) * std::max<size_t>(count, 1)));
+    rb->allocs.push_back(m);
+    *p = static_cast<T *>(m);
+    return YMRT_OK;
 }
-int accumulate(const float *, float *, uint32_t, uint32_t, int, hipStream_t)
+
+static int ensure_buffers(ymrt_scene *sc, uint32_t cap)
 {
-    set_error("not built yet");
-    return YMRT_ERR_UNSUPPORTED;
+    if (sc->render && sc->render->cap >= cap) return YMRT_OK;
+    destroy_render_buffers(sc);
+    RenderBuffers *rb = new RenderBuffers;
+    sc->render = rb;
+    rb->cap = cap;
+    PathState &ps = rb->ps;
+    int rc = 0;
+    const size_t n = cap;
+    rc |= alloc(rb, &ps.ray_o, n);
+    rc |= alloc(rb, &ps.ray_d, n);
+    rc |= alloc(rb, &ps.last, n);
+    rc |= alloc(rb, &ps.beta, n);
+    rc |= alloc(rb, &ps.L, n);
+    rc |= alloc(rb, &ps.smp, n);
+    rc |= alloc(rb, &ps.info, n);
+    rc |= alloc(rb, &ps.xhdr, n);
+    rc |= alloc(rb, &ps.xrec, n * kMaxBoundaryRecord);
+    rc |= alloc(rb, &ps.hit_a, n);
+    rc |= alloc(rb, &ps.hit_b, n);
+    rc |= alloc(rb, &ps.nb, n * 4);
+    rc |= alloc(rb, &ps.sh_o, n * 2);
+    rc |= alloc(rb, &ps.sh_d, n * 2);
+    rc |= alloc(rb, &ps.sh_f, n * 2);
+    rc |= alloc(rb, &ps.sh_li, n * 2);
+    rc |= alloc(rb, &ps.sh_c, n);
+    rc |= alloc(rb, &ps.sh_flags, n);
+    rc |= alloc(rb, &ps.out, n);
+    rc |= alloc(rb, &ps.queue[0], n);
+    rc |= alloc(rb, &ps.queue[1], n);
+    rc |= alloc(rb, &ps.q_hit, n);
+    rc |= alloc(rb, &ps.q_shadow, n);
+    rc |= alloc(rb, &rb->counters, (size_t)(kMaxPasses + 2) * kCtrStride);
+    rc |= alloc(rb, &rb->stats, kStatCount);
+    if (rc) {
+        destroy_render_buffers(sc);
+        return rc;
+    }
+    return YMRT_OK;
 }
-int tonemap(float *, uint32_t, uint32_t, float, float, hipStream_t)
+
+static hipEvent_t next_event(RenderBuffers *rb)
 {
-    set_error("not built yet");
-    return YMRT_ERR_UNSUPPORTED;
+    if (rb->event_next == rb->events.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        rb->events.push_back(e);
+    }
+    return rb->events[rb->event_next++];
+}
+
+int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, float *noise, hipStream_t stream,
+                 int timing, ymrt_frame_stats *stats)
+{
+    YM_REQUIRE(rs->width > 0 && rs->height > 0, "image size must be positive");
+    YM_REQUIRE(rs->spp >= 1, "spp must be >= 1");
+    YM_REQUIRE(rs->ray_depth >= 1 && rs->ray_depth <= kMaxRayDepth, "ray_depth must be in 1..64");
+    YM_REQUIRE(rs->sampler_type == YMRT_SAMPLER_HALTON || rs->sampler_type == YMRT_SAMPLER_PCG,
+               "sampler_type must be PCG (0) or HALTON (1)");
+    YM_HIP(hipSetDevice(sc->ctx->device));
+    const uint32_t spp = (uint32_t)rs->spp;
+    uint32_t cap = rs->path_capacity ? rs->path_capacity : kDefaultCapacity;
+    YM_REQUIRE(cap >= spp, "path_capacity must hold one pixel's samples");
+    const uint32_t chunk_paths_max = (cap / spp) * spp;
+    int rc = ensure_buffers(sc, chunk_paths_max);
+    if (rc) return rc;
+    RenderBuffers *rb = sc->render;
+
+    PtParams prm;
+    std::memset(&prm, 0, sizeof(prm));
+    prm.rs = *rs;
+    prm.henum = HaltonEnum::make((int)rs->width, (int)rs->height);
+    prm.frame = frame;
+    prm.tiles_x = (rs->width + 15) / 16;
+    const uint32_t tiles_y = (rs->height + 15) / 16;
+    prm.cap = rb->cap;
+    prm.counters = rb->counters;
+    prm.stats = rb->stats;
+    prm.error = sc->d_error;
+    const uint64_t total_paths = (uint64_t)prm.tiles_x * tiles_y * 256ull * spp;
+    // The coincident-surface list only changes results where medium records
+    // change: boundaries, dielectrics, priorities.
+    const bool nearby = sc->needs_nearby;
+    const bool media = rs->enable_volume_scattering && sc->dev.volume_count > 0;
+    const uint32_t stack = sc->max_stack;
+    YM_REQUIRE(stack <= 64, "scene BVH too deep for the path-tracing traversal stack");
+
+    YM_HIP(hipMemsetAsync(rb->stats, 0, sizeof(unsigned long long) * kStatCount, stream));
+    rb->event_next = 0;
+    struct Mark {
+        int stage;
+        hipEvent_t a, b;
+    };
+    std::vector<Mark> marks;
+    auto timed = [&](int stage, auto &&fn) -> int {
+        hipEvent_t a = nullptr, b = nullptr;
+        if (timing) {
+            a = next_event(rb);
+            b = next_event(rb);
+            if (a) (void)hipEventRecord(a, stream);
+        }
+        int r = fn();
+        if (timing && b) {
+            (void)hipEventRecord(b, stream);
+            marks.push_back(Mark{stage, a, b});
+        }
+        return r;
+    };
+    hipEvent_t frame_a = nullptr, frame_b = nullptr;
+    if (timing) {
+        frame_a = next_event(rb);
+        frame_b = next_event(rb);
+        YM_HIP(hipEventRecord(frame_a, stream));
+    }
+    uint32_t passes_total = 0, chunks = 0;
+    for (uint64_t base = 0; base < total_paths; base += chunk_paths_max) {
+        prm.chunk_base = base;
+        prm.chunk_paths = (uint32_t)std::min<uint64_t>(chunk_paths_max, total_paths - base);
+        ++chunks;
+        YM_HIP(hipMemsetAsync(rb->counters, 0, sizeof(uint32_t) * (kMaxPasses + 2) * kCtrStride, stream));
+        if ((rc = timed(0, [&] { return launch_raygen(sc->dev, prm, rb->ps, stream); }))) return rc;
+        uint32_t pass = 0;
+        uint32_t next_check = (uint32_t)rs->ray_depth + 1;
+        for (;;) {
+            if ((rc = timed(1, [&] { return launch_extend(sc->dev, prm, rb->ps, pass, nearby, stack, stream); })))
+                return rc;
+            if ((rc = timed(2, [&] { return launch_shade(sc->dev, prm, rb->ps, pass, stream); }))) return rc;
+            if ((rc = timed(3, [&] { return launch_shadow(sc->dev, prm, rb->ps, pass, media, stack, stream); })))
+                return rc;
+            ++pass;
+            if (pass == next_check || pass >= kMaxPasses) {
+                uint32_t active = 0;
+                YM_HIP(hipMemcpyAsync(&active, rb->counters + pass * kCtrStride + kCtrActive, sizeof(uint32_t),
+                                      hipMemcpyDeviceToHost, stream));
+                YM_HIP(hipStreamSynchronize(stream));
+                if (active == 0) break;
+                if (pass >= kMaxPasses) {
+                    if ((rc = launch_flush(prm, rb->ps, pass, stream))) return rc;
+                    break;
+                }
+                next_check = pass + 2;
+            }
+        }
+        passes_total += pass;
+        if ((rc = timed(4, [&] { return launch_resolve(prm, rb->ps, noise, stream); }))) return rc;
+    }
+    if (timing) YM_HIP(hipEventRecord(frame_b, stream));
+    if (stats) {
+        unsigned long long st[kStatCount];
+        YM_HIP(hipMemcpyAsync(st, rb->stats, sizeof(st), hipMemcpyDeviceToHost, stream));
+        YM_HIP(hipStreamSynchronize(stream));
+        std::memset(stats, 0, sizeof(*stats));
+        stats->camera_rays = st[kStatCamera];
+        stats->extension_rays = st[kStatExtension];
+        stats->shadow_rays = st[kStatShadow];
+        stats->transmittance_rays = st[kStatTransmittance];
+        stats->paths = (uint64_t)rs->width * rs->height * spp;
+        stats->passes = passes_total;
+        stats->chunks = chunks;
+        if (timing) {
+            float ms[5] = {0, 0, 0, 0, 0};
+            uint32_t launches[5] = {0, 0, 0, 0, 0};
+            for (const Mark &m : marks) {
+                float t = 0.0f;
+                YM_HIP(hipEventElapsedTime(&t, m.a, m.b));
+                ms[m.stage] += t;
+                launches[m.stage]++;
+            }
+            stats->ms_raygen = ms[0];
+            stats->ms_extend = ms[1];
+            stats->ms_shade = ms[2];
+            stats->ms_shadow = ms[3];
+            stats->ms_resolve = ms[4];
+            stats->launches_extend = launches[1];
+            stats->launches_shade = launches[2];
+            stats->launches_shadow = launches[3];
+            float tot = 0.0f;
+            YM_HIP(hipEventElapsedTime(&tot, frame_a, frame_b));
+            stats->ms_total = tot;
+        }
+    } else {
+        YM_HIP(hipStreamSynchronize(stream));
+    }
+    return YMRT_OK;
 }
 
 }  // namespace ym

@@ -333,6 +333,11 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
         tlas_root = collapse_wide(bin, h.nodes, 0, &tlas_depth);
     }
     sc->max_stack = 3 * max_blas_depth + 3 * tlas_depth + 2;
+    sc->needs_nearby = has_boundaries != 0;
+    for (const DevMaterial &m : h.materials)
+        if (m.type == YMRT_MATERIAL_DEFAULT &&
+            (m.transmission_weight > 0.0f || m.transmission_weight_tex != kEmpty || m.ior_priority != 0))
+            sc->needs_nearby = true;
     sc->info.triangle_count = total_tris;
     sc->info.node_count = h.nodes.size();
 
