@@ -2,9 +2,10 @@
 
 The oracle restates the reference's path loop per path; the HIP path runs it
 as a wavefront.  Both use the same Halton table, scene floats and operation
-order, so most paths are identical to the last bit; the rest diverge where
-GPU and glibc transcendentals (sin, cos, log, exp) round differently.  Bars:
-  * >= 90% of pixels bit-identical, relative RMSE <= 1% (Cornell, 8 spp);
+order, so paths follow the same vertices; their floats differ in the last bits
+where GPU and glibc transcendentals (sin, cos, log, exp) round differently,
+and a few paths diverge outright.  Bars:
+  * >= 95% of pixels within 1e-4 relative, relative RMSE <= 1% (Cornell, 8 spp);
   * relative RMSE <= 2% for the textured, fogged interior (homogeneous media);
   * accumulate and tonemap equal to the oracle's;
   * two GPU frames bit-identical (no float atomics anywhere).
@@ -28,6 +29,11 @@ def render_both(gpu_ctx, sa, rs, frame=1):
     return g, o, stats
 
 
+def close_frac(a, b):
+    d = np.abs(a[..., :3] - b[..., :3])
+    return float(np.all(d <= 1e-4 * np.abs(b[..., :3]) + 1e-6, axis=-1).mean())
+
+
 def rel_rmse(a, b):
     return float(np.sqrt(np.mean((a[..., :3] - b[..., :3]) ** 2)) / max(np.mean(np.abs(b[..., :3])), 1e-12))
 
@@ -38,11 +44,11 @@ def test_cornell_halton_parity(gpu_ctx):
     g, o, stats = render_both(gpu_ctx, sa, rs)
     assert np.isfinite(g).all()
     assert np.allclose(g[..., 3], 1.0)
-    same = np.all(g == o, axis=-1).mean()
+    same = close_frac(g, o)
     err = rel_rmse(g, o)
-    print("identical pixels %.4f  rel rmse %.5f  stats ext %d shadow %d" %
+    print("close pixels %.4f  rel rmse %.5f  stats ext %d shadow %d" %
           (same, err, stats.extension_rays, stats.shadow_rays))
-    assert same >= 0.90
+    assert same >= 0.95
     assert err <= 0.01
 
 
@@ -50,7 +56,7 @@ def test_cornell_pcg_parity(gpu_ctx):
     sa = scenes.cornell_box(48, 32)
     rs = abi.render_settings(48, 32, 4, 6, sampler=abi.SAMPLER_PCG)
     g, o, _ = render_both(gpu_ctx, sa, rs, frame=3)
-    assert np.all(g == o, axis=-1).mean() >= 0.85
+    assert close_frac(g, o) >= 0.90
     assert rel_rmse(g, o) <= 0.02
 
 

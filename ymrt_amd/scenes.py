@@ -207,3 +207,145 @@ def stress_rays(sa, n, seed_origin=2, seed_dir=3, seed_target=4):
     shadow["tmin"] = np.float32(1e-7)
     shadow["tmax"] = length.astype(np.float32)
     return closest, shadow
+
+
+# ------------------------------------------------------------- interior (C2)
+def _grid_mesh(nu, nv, fn):
+    """Parametric surface p = fn(u, v) on an (nu+1) x (nv+1) vertex grid."""
+    u = np.linspace(0.0, 1.0, nu + 1)
+    v = np.linspace(0.0, 1.0, nv + 1)
+    U, V = np.meshgrid(u, v)
+    P = fn(U, V)
+    pos = np.stack(P, axis=-1).reshape(-1, 3)
+    uv = np.stack([U, V], axis=-1).reshape(-1, 2)
+    i = np.arange(nu)
+    j = np.arange(nv)
+    I, J = np.meshgrid(i, j)
+    a = (J * (nu + 1) + I).reshape(-1)
+    b = a + 1
+    c = a + nu + 1
+    d = c + 1
+    idx = np.empty((2 * len(a), 3), np.int64)
+    idx[0::2] = np.stack([a, c, d], -1)
+    idx[1::2] = np.stack([a, d, b], -1)
+    return pos, idx, uv
+
+
+def _smooth_normals(pos, idx):
+    n = np.zeros_like(pos)
+    fn = np.cross(pos[idx[:, 1]] - pos[idx[:, 0]], pos[idx[:, 2]] - pos[idx[:, 0]])
+    for k in range(3):
+        np.add.at(n, idx[:, k], fn)
+    ln = np.linalg.norm(n, axis=1)
+    ln[ln == 0] = 1.0
+    return n / ln[:, None]
+
+
+def _finish(pos, idx, uv):
+    pos = pos - 0.5 * (pos.min(0) + pos.max(0))
+    pos = pos / np.abs(pos).max()
+    nrm = _smooth_normals(pos, idx)
+    return (pos.astype(np.float32), idx.astype(np.uint32), nrm.astype(np.float32), uv.astype(np.float32))
+
+
+def interior_meshes():
+    """Eight procedural meshes of about 6.5k triangles each, unit extent."""
+    tau = 2 * np.pi
+    out = []
+    # 1 sphere
+    out.append(_finish(*_grid_mesh(64, 51, lambda u, v: (np.sin(np.pi * v) * np.cos(tau * u), np.cos(np.pi * v),
+                                                          np.sin(np.pi * v) * np.sin(tau * u)))))
+    # 2 torus
+    out.append(_finish(*_grid_mesh(64, 51, lambda u, v: ((1 + 0.35 * np.cos(tau * v)) * np.cos(tau * u),
+                                                          0.35 * np.sin(tau * v),
+                                                          (1 + 0.35 * np.cos(tau * v)) * np.sin(tau * u)))))
+    # 3 displaced grid
+    hf = value_noise_2d(65, 3, 101, lattice=4)
+    out.append(_finish(*_grid_mesh(57, 57, lambda u, v: (u, 0.25 * hf[(v * 57).astype(int), (u * 57).astype(int)], v))))
+    # 4 cylinder (open)
+    out.append(_finish(*_grid_mesh(64, 51, lambda u, v: (np.cos(tau * u), 2 * v - 1, np.sin(tau * u)))))
+    # 5 bumpy sphere
+    bump = value_noise_2d(129, 3, 102, lattice=8)
+    out.append(_finish(*_grid_mesh(64, 51, lambda u, v: tuple(
+        (1 + 0.15 * bump[(v * 128).astype(int), (u * 128).astype(int)]) * c
+        for c in (np.sin(np.pi * v) * np.cos(tau * u), np.cos(np.pi * v), np.sin(np.pi * v) * np.sin(tau * u))))))
+    # 6 cone
+    out.append(_finish(*_grid_mesh(64, 51, lambda u, v: ((1 - v) * np.cos(tau * u), 2 * v - 1, (1 - v) * np.sin(tau * u)))))
+    # 7 twisted ribbon
+    out.append(_finish(*_grid_mesh(64, 51, lambda u, v: ((v - 0.5) * np.cos(tau * u), 2 * u - 1,
+                                                          (v - 0.5) * np.sin(tau * u)))))
+    # 8 spring (tube along a helix)
+    def spring(u, v):
+        t = 3 * tau * u
+        c = np.stack([np.cos(t), 0.15 * t - 1.4, np.sin(t)])
+        r = 0.2
+        return (c[0] + r * np.cos(tau * v) * np.cos(t), c[1] + r * np.sin(tau * v), c[2] + r * np.cos(tau * v) * np.sin(t))
+    out.append(_finish(*_grid_mesh(127, 25, spring)))
+    return out
+
+
+def value_noise_texture(n, seed, octaves=4):
+    """RGBA8 value-noise texture (n x n): three fBm channels, alpha 255."""
+    chans = [value_noise_2d(n, octaves, seed * 3 + c) for c in range(3)]
+    rgb = np.stack(chans, axis=-1)
+    lo, hi = rgb.min(), rgb.max()
+    rgb = (rgb - lo) / max(hi - lo, 1e-9)
+    tex = np.empty((n, n, 4), np.uint8)
+    tex[..., :3] = np.clip(rgb * 255.0 + 0.5, 0, 255).astype(np.uint8)
+    tex[..., 3] = 255
+    return tex
+
+
+def _rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def interior(width=1920, height=1080, tex_size=2048, instances=40, halton_seed=0, fog=True):
+    """Procedural room: 40 random affine instances of 8 meshes (~260k
+    triangles), four value-noise RGBA8 textures, two ceiling quad lights and
+    homogeneous fog (sigma_t 0.05, albedo 0.8, g 0.3) filling a boundary box
+    that contains the camera."""
+    sa = abi.SceneArrays()
+    sa.halton_seed = halton_seed
+    texs = [sa.add_texture(value_noise_texture(tex_size, s)) for s in (7, 8, 9, 10)]
+    meshes = [_add(sa, m) for m in interior_meshes()]
+    rng = np.random.RandomState(42)
+    mrng = np.random.RandomState(43)
+    for i in range(instances):
+        if mrng.rand() < 0.5:
+            mat = abi.default_material(diffuse_albedo=(1.0, 1.0, 1.0), diffuse_albedo_tex=texs[i % 4],
+                                       specular_weight=0.0, ior_n=1.0)
+        else:
+            r = float(mrng.uniform(0.05, 0.8))
+            mat = abi.default_material(diffuse_albedo=tuple(mrng.uniform(0.4, 0.95, 3)),
+                                       specular_albedo=(1.0, 1.0, 1.0), roughness_x=r, roughness_y=r,
+                                       metalness=1.0, ior_n=1.0)
+            if i % 3 == 0:
+                mat.diffuse_albedo_tex = texs[(i + 1) % 4]
+        m = np.eye(4)
+        m[:3, :3] = _rotation(rng) * rng.uniform(0.5, 2.0) * 0.45
+        m[:3, 3] = (rng.uniform(-4.0, 4.0), rng.uniform(0.6, 3.0), rng.uniform(-4.0, 2.5))
+        sa.add_instance(meshes[i % len(meshes)], sa.add_material(mat), m.astype(np.float32))
+    # room: inward-facing box [-5,5] x [0,4] x [-5,5], textured walls
+    pos, idx, nrm, uv = box((-5.0, 0.0, -5.0), (5.0, 4.0, 5.0))
+    idx = idx[:, ::-1].copy()
+    room = sa.add_mesh(pos, idx, nrm=-nrm, nidx=idx, uv=uv * 4.0, tidx=idx)
+    wall = sa.add_material(abi.default_material(diffuse_albedo=(0.8, 0.8, 0.8), diffuse_albedo_tex=texs[0],
+                                                specular_weight=0.0, ior_n=1.0))
+    sa.add_instance(room, wall)
+    light = sa.add_material(abi.light_material((1.0, 0.95, 0.85), 12.0))
+    for x0, z0 in ((-2.5, -1.5), (1.5, 0.5)):
+        _ = sa.add_instance(_add(sa, quad((x0, 3.99, z0), (x0 + 1.0, 3.99, z0), (x0 + 1.0, 3.99, z0 + 1.0),
+                                          (x0, 3.99, z0 + 1.0))), light)
+    if fog:
+        vol = sa.add_volume((0.05, 0.05, 0.05), (0.8, 0.8, 0.8), 0.3)
+        fb = box((-5.2, -0.2, -5.2), (5.2, 4.2, 5.2))
+        fogmat = sa.add_material(abi.default_material(ior_n=1.0))
+        sa.add_instance(_add(sa, fb), fogmat, inner_volume=vol, boundary=True)
+    sa.set_camera((0.0, 1.7, 4.6), (0.0, 1.4, 0.0), 60.0, width / height)
+    return sa
