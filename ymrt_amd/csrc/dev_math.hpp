@@ -9,8 +9,23 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cmath>
+#include <type_traits>
 
 #define YM_HD __host__ __device__ __forceinline__
+
+// Calls f(integral_constant<int, I>) for I in [B, E).  Small fixed-size arrays
+// whose elements are picked by a run-time index stay in registers only when
+// every access uses a compile-time index: an unrolled loop is not enough, as
+// the optimiser may first fold "if (i == k) a[i] = v" into a store through a
+// computed address, which pins the array in scratch memory.
+template <int B, int E, typename F>
+YM_HD void static_for(F &&f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
 
 namespace ym {
 

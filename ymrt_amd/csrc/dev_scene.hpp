@@ -136,7 +136,9 @@ struct DevScene {
     const uint32_t *halton;        // permutation table
     const uint32_t *primes;        // first 1000 primes
     const uint32_t *prime_sums;    // offset of each prime's permutation
-    uint32_t tlas_root;            // node index, or leaf code for a single instance
+    uint32_t tlas_root;            // all instances: node index, leaf code, or kSentinel
+    uint32_t tlas_root_solid;      // instances that are not medium boundaries (shadow rays)
+    uint32_t tlas_root_boundary;   // medium boundaries only (transmittance walk)
     uint32_t instance_count, material_count, texture_count, grid_count, volume_count, light_count;
     uint32_t has_boundaries;       // any instance with treat_as_boundary
     DevCamera camera;

@@ -11,7 +11,16 @@
 namespace ym {
 
 // ---------------------------------------------------------------- RayTransfer
-// src/Ray.h:106-260 (records of (instance, ior, priority, volume))
+// src/Ray.h:106-260 (records of (instance, ior, priority, volume)).  Every
+// access to the six fixed slots uses a compile-time index (static_for), is
+// unconditional, and picks by selecting values: conditional loads or stores
+// at different slots get merged by the optimiser into one access through a
+// computed address, which would pin the records in scratch memory.
+YM_HD CameraRecord select_rec(bool c, const CameraRecord &a, const CameraRecord &b)
+{
+    return CameraRecord{c ? a.prim : b.prim, c ? a.ior : b.ior, c ? a.priority : b.priority, c ? a.vol : b.vol};
+}
+
 struct Transfer {
     int max_idx, count;
     CameraRecord rec[kMaxBoundaryRecord];
@@ -21,67 +30,96 @@ struct Transfer {
         max_idx = -1;
         count = 0;
     }
+    YM_HD CameraRecord get(int idx) const
+    {
+        CameraRecord r = rec[0];
+        static_for<1, kMaxBoundaryRecord>([&](auto I) {
+            const CameraRecord c = rec[I()];
+            r = select_rec(I() == idx, c, r);
+        });
+        return r;
+    }
     YM_HD int find_max() const
     {
-        if (count == 0) return -1;
         uint32_t mp = rec[0].priority;
         int mi = 0;
-        for (int i = 1; i < count; ++i)
-            if (rec[i].priority >= mp) {
-                mp = rec[i].priority;
-                mi = i;
-            }
-        return mi;
+        static_for<1, kMaxBoundaryRecord>([&](auto I) {
+            const uint32_t pr = rec[I()].priority;
+            const bool take = I() < count && pr >= mp;
+            mp = take ? pr : mp;
+            mi = take ? (int)I() : mi;
+        });
+        return count == 0 ? -1 : mi;
     }
+    // RayTransfer::FindRecordWithPrim scans down from the top: the highest match wins
     YM_HD int find_prim(uint32_t prim) const
     {
-        for (int i = count - 1; i >= 0; --i)
-            if (rec[i].prim == prim) return i;
-        return -1;
+        int r = -1;
+        static_for<0, kMaxBoundaryRecord>([&](auto I) {
+            const uint32_t pp = rec[I()].prim;
+            r = (I() < count && pp == prim) ? (int)I() : r;
+        });
+        return r;
     }
     YM_HD bool push(uint32_t prim, float ior, uint32_t prio, int vol)
     {
         if (count == kMaxBoundaryRecord) return false;
-        rec[count] = CameraRecord{prim, ior, prio, vol};
+        const CameraRecord nr{prim, ior, prio, vol};
+        static_for<0, kMaxBoundaryRecord>([&](auto I) {
+            const CameraRecord c = rec[I()];
+            rec[I()] = select_rec(I() == count, nr, c);
+        });
         ++count;
         if (count == 1) {
             max_idx = 0;
             return true;
         }
-        if (prio >= rec[max_idx].priority) max_idx = count - 1;
+        if (prio >= get(max_idx).priority) max_idx = count - 1;
         return true;
     }
     YM_HD bool pop(int idx)
     {
         if (count == 0) return false;
         if (!(idx >= 0 && idx < count)) return false;
-        for (int i = idx + 1; i < count; ++i) rec[i - 1] = rec[i];
+        static_for<0, kMaxBoundaryRecord - 1>([&](auto I) {
+            const CameraRecord c = rec[I()], nx = rec[I() + 1];
+            rec[I()] = select_rec(I() >= idx && I() + 1 < count, nx, c);
+        });
         --count;
         max_idx = find_max();
         return true;
     }
+    // GetCurrentVolumeIndices: records of the top priority, from the top down
     YM_HD int volumes(int *vols, uint32_t volume_count) const
     {
+        static_for<0, kMaxBoundaryRecord>([&](auto J) { vols[J()] = 0; });
         if (count == 0 || max_idx == -1) return 0;
-        const uint32_t mp = rec[max_idx].priority;
+        const uint32_t mp = get(max_idx).priority;
         int n = 0;
-        for (int i = count - 1; i >= 0; --i)
-            if (rec[i].priority == mp && rec[i].vol >= 0 && (uint32_t)rec[i].vol < volume_count) vols[n++] = rec[i].vol;
+        static_for<0, kMaxBoundaryRecord>([&](auto I) {
+            constexpr int i = kMaxBoundaryRecord - 1 - I();
+            const CameraRecord c = rec[i];
+            const bool take = i < count && c.priority == mp && c.vol >= 0 && (uint32_t)c.vol < volume_count;
+            static_for<0, kMaxBoundaryRecord>([&](auto J) {
+                const int v = vols[J()];
+                vols[J()] = (take && J() == n) ? c.vol : v;
+            });
+            n += take ? 1 : 0;
+        });
         return n;
     }
-    YM_HD float ray_ior() const { return count > 0 ? rec[max_idx].ior : 1.0f; }
+    YM_HD float ray_ior() const { return count > 0 ? get(max_idx).ior : 1.0f; }
     YM_HD float ex_ior(uint32_t hit_prim) const
     {
         uint32_t mp = 0;
         int mi = -1;
-        for (int i = 0; i < count; ++i) {
-            if (rec[i].prim == hit_prim) continue;
-            if (rec[i].priority >= mp) {
-                mp = rec[i].priority;
-                mi = i;
-            }
-        }
-        return mi == -1 ? 1.0f : rec[mi].ior;
+        static_for<0, kMaxBoundaryRecord>([&](auto I) {
+            const CameraRecord c = rec[I()];
+            const bool take = I() < count && c.prim != hit_prim && c.priority >= mp;
+            mp = take ? c.priority : mp;
+            mi = take ? (int)I() : mi;
+        });
+        return mi == -1 ? 1.0f : get(mi).ior;
     }
 };
 
@@ -95,25 +133,30 @@ YM_HD float fetch_ior(const DevMaterial &m, uint32_t *prio)
     return 1.0f;
 }
 
+constexpr int kNearbyMax = 4;
+
 struct NearbyList {
     int n;
-    uint32_t prim[4];
-    float t[4];
-    int back[4];
+    uint32_t prim[kNearbyMax];
+    float t[kNearbyMax];
+    int back[kNearbyMax];
 };
 
-// BoundaryTransitionBunch (src/ShadingUtilities.cuh:954-990)
+// BoundaryTransitionBunch (src/ShadingUtilities.cuh:954-990): pushes for the
+// front-face hits first, then pops for the back-face hits.
 __device__ __forceinline__ void boundary_bunch(const DevScene &sc, Transfer &t, const NearbyList &nb)
 {
-    for (int i = 0; i < nb.n; ++i) {
-        if (nb.back[i]) continue;
+#pragma unroll
+    for (int i = 0; i < kNearbyMax; ++i) {
+        if (i >= nb.n || nb.back[i]) continue;
         const DevInstance &in = sc.instances[nb.prim[i]];
         uint32_t prio;
         float ior = fetch_ior(sc.materials[in.material], &prio);
         if (t.find_prim(nb.prim[i]) == -1) t.push(nb.prim[i], ior, prio, in.inner_volume);
     }
-    for (int i = 0; i < nb.n; ++i) {
-        if (!nb.back[i]) continue;
+#pragma unroll
+    for (int i = 0; i < kNearbyMax; ++i) {
+        if (i >= nb.n || !nb.back[i]) continue;
         int ri = t.find_prim(nb.prim[i]);
         if (ri != -1) t.pop(ri);
     }
@@ -370,16 +413,56 @@ __device__ __forceinline__ bool lobe_sample(const Lobe &l, float u0, float u1, f
     return true;
 }
 
-// DefaultMtlBSDF + MaterialBSDF (src/BSDF.cuh:432-807), no normal/bump maps
+// DefaultMtlBSDF + MaterialBSDF (src/BSDF.cuh:432-807), no normal/bump maps.
+// The reference appends up to five lobes to an array (coat, metal, specular,
+// transmission, diffuse -- always in that order, absent ones skipped).  Here
+// each lobe has a fixed slot and a presence bit; eval/sample walk the slots in
+// the same order, so sums and the lobe-selection CDF are the reference's, and
+// the per-lobe data (colours plus the shared roughness/IOR terms) stays in
+// registers instead of a dynamically indexed array.
+enum : int { kSlotCoat = 0, kSlotMetal = 1, kSlotSpec = 2, kSlotTrans = 3, kSlotDiffuse = 4, kSlots = 5 };
+
 struct Bsdf {
     f3 t, b, n;
-    int is_default;
-    int count;
-    Lobe l[5];
-    float w[5];
+    uint32_t mask;      // presence bit per slot; 0 for non-default materials
+    float w[kSlots];
+    f3 col[kSlots];
+    float ni, ax, ay, cax, cay, ior_coat, ior_out;
+    f3 metal_n, metal_k;
 
     YM_HD f3 to_local(f3 v) const { return f3{dot(v, t), dot(v, b), dot(v, n)}; }
     YM_HD f3 to_world(f3 v) const { return v.x * t + v.y * b + v.z * n; }
+    YM_HD Lobe lobe(int slot) const
+    {
+        Lobe l;
+        l.albedo = col[slot];
+        l.ni = ni;
+        l.kt = s3(0.0f);
+        l.metal = 0;
+        l.ax = ax;
+        l.ay = ay;
+        if (slot == kSlotCoat) {
+            l.type = kLobeReflect;
+            l.ax = cax;
+            l.ay = cay;
+            l.nt = s3(ior_coat);
+        } else if (slot == kSlotMetal) {
+            l.type = kLobeReflect;
+            l.nt = metal_n;
+            l.kt = metal_k;
+            l.metal = 1;
+        } else if (slot == kSlotSpec) {
+            l.type = kLobeReflect;
+            l.nt = s3(ior_out);
+        } else if (slot == kSlotTrans) {
+            l.type = kLobeTransmit;
+            l.nt = s3(ior_out);
+        } else {
+            l.type = kLobeLambert;
+            l.nt = s3(0.0f);
+        }
+        return l;
+    }
 };
 
 YM_HD void init_shading_space(Bsdf &bs, int hit_back, f3 ray_d, f3 ng, f3 sn_in, f3 dpdu, f3 dpdv)
@@ -414,6 +497,7 @@ __device__ __forceinline__ f3 tex_color(const DevScene &sc, uint32_t tex, const 
     return tex != kEmpty ? texture_eval(sc, tex, uv) : f3{def[0], def[1], def[2]};
 }
 
+// DefaultMtlBSDF::InitBSDFSettings (src/BSDF.cuh:453-618)
 __device__ __forceinline__ void init_default_bsdf(const DevScene &sc, Bsdf &b, const DevMaterial &m, f2 uv,
                                                   f3 ray_d, int hit_back, float ray_ior, float ex_ior)
 {
@@ -443,136 +527,128 @@ __device__ __forceinline__ void init_default_bsdf(const DevScene &sc, Bsdf &b, c
     const float rc = safe_rcp(cos_in);
     const f3 ns = -sigma;
     const f3 coat_tr{expf(ns.x * rc), expf(ns.y * rc), expf(ns.z * rc)};
+    b.ni = ior_in;
+    b.ax = ax;
+    b.ay = ay;
+    b.cax = cax;
+    b.cay = cay;
+    b.ior_coat = ior_coat;
+    b.ior_out = ior_out;
+    b.metal_n = s3(0.0f);
+    b.metal_k = s3(0.0f);
+    b.mask = 0;
     float wsum = 0.0f;
-    b.count = 0;
+#pragma unroll
+    for (int i = 0; i < kSlots; ++i) {
+        b.w[i] = 0.0f;
+        b.col[i] = s3(0.0f);
+    }
     const f3 coat_color = s3(coat_w * fr_ic);
     const float coat_lum = luminance(coat_color);
     if (coat_lum > 0.0f) {
-        Lobe &l = b.l[b.count];
-        l.type = kLobeReflect;
-        l.albedo = coat_color;
-        l.ax = cax;
-        l.ay = cay;
-        l.nt = s3(ior_coat);
-        l.kt = s3(0.0f);
-        l.ni = ior_in;
-        l.metal = 0;
-        b.w[b.count] = coat_lum;
+        b.mask |= 1u << kSlotCoat;
+        b.col[kSlotCoat] = coat_color;
+        b.w[kSlotCoat] = coat_lum;
         wsum += coat_lum;
-        ++b.count;
     }
     const f3 lerp = (1.0f - coat_w) * s3(1.0f) + coat_tr * coat_w;
     const f3 tint = (1.0f - coat_w * fr_ic) * lerp;
     const f3 metal_c = tint * metal;
     const float metal_lum = luminance(metal_c);
     if (metal_lum > 0.0f) {
-        f3 mn, mk;
-        edge_tint(dif, spe, mn, mk);
-        Lobe &l = b.l[b.count];
-        l.type = kLobeReflect;
-        l.albedo = metal_c;
-        l.ax = ax;
-        l.ay = ay;
-        l.nt = mn;
-        l.kt = mk;
-        l.ni = ior_in;
-        l.metal = 1;
-        b.w[b.count] = metal_lum;
+        edge_tint(dif, spe, b.metal_n, b.metal_k);
+        b.mask |= 1u << kSlotMetal;
+        b.col[kSlotMetal] = metal_c;
+        b.w[kSlotMetal] = metal_lum;
         wsum += metal_lum;
-        ++b.count;
     }
     const f3 spec_c = tint * (1.0f - metal) * fr_io * spec_w * spe;
     const float spec_lum = luminance(spec_c);
     if (spec_lum > 0.0f) {
-        Lobe &l = b.l[b.count];
-        l.type = kLobeReflect;
-        l.albedo = spec_c;
-        l.ax = ax;
-        l.ay = ay;
-        l.nt = s3(ior_out);
-        l.kt = s3(0.0f);
-        l.ni = ior_in;
-        l.metal = 0;
-        b.w[b.count] = spec_lum;
+        b.mask |= 1u << kSlotSpec;
+        b.col[kSlotSpec] = spec_c;
+        b.w[kSlotSpec] = spec_lum;
         wsum += spec_lum;
-        ++b.count;
     }
     const f3 trans_c = tint * (1.0f - metal) * (1.0f - fr_io * spec_w) * trans_w * spe;
     const float trans_lum = luminance(trans_c);
     if (trans_lum > 0.0f) {
-        Lobe &l = b.l[b.count];
-        l.type = kLobeTransmit;
-        l.albedo = trans_c;
-        l.ax = ax;
-        l.ay = ay;
-        l.ni = ior_in;
-        l.nt = s3(ior_out);
-        l.kt = s3(0.0f);
-        l.metal = 0;
-        b.w[b.count] = trans_lum;
+        b.mask |= 1u << kSlotTrans;
+        b.col[kSlotTrans] = trans_c;
+        b.w[kSlotTrans] = trans_lum;
         wsum += trans_lum;
-        ++b.count;
     }
     const f3 dif_c = tint * (1.0f - metal) * (1.0f - fr_io * spec_w) * (1.0f - trans_w) * dif;
     const float dif_lum = luminance(dif_c);
-    {
-        Lobe &l = b.l[b.count];
-        l.type = kLobeLambert;
-        l.albedo = dif_c;
-        l.metal = 0;
-        b.w[b.count] = dif_lum;
-        wsum += dif_lum;
-        ++b.count;
-    }
+    b.mask |= 1u << kSlotDiffuse;
+    b.col[kSlotDiffuse] = dif_c;
+    b.w[kSlotDiffuse] = dif_lum;
+    wsum += dif_lum;
     if (wsum > 0.0f) {
         const float iw = safe_rcp(wsum);
-        for (int i = 0; i < b.count; ++i) b.w[i] *= iw;
+#pragma unroll
+        for (int i = 0; i < kSlots; ++i)
+            if (b.mask & (1u << i)) b.w[i] *= iw;
     } else {
-        const float iw = 1.0f / (float)b.count;
-        for (int i = 0; i < b.count; ++i) b.w[i] = iw;
+        const float iw = 1.0f / (float)__popc(b.mask);
+#pragma unroll
+        for (int i = 0; i < kSlots; ++i)
+            if (b.mask & (1u << i)) b.w[i] = iw;
     }
 }
 
+// DefaultMtlBSDF::EvalWi (src/BSDF.cuh:619-632)
 YM_HD f3 bsdf_eval(const Bsdf &b, f3 wo, f3 wi, float &pdf)
 {
     f3 f = s3(0.0f);
     float p = 0.0f;
-    if (!b.is_default) {
-        pdf = 0.0f;
-        return f;
-    }
-    for (int i = 0; i < b.count; ++i) {
+#pragma unroll
+    for (int i = 0; i < kSlots; ++i) {
+        if (!(b.mask & (1u << i))) continue;
         float lp = 0.0f;
-        f = f + lobe_eval(b.l[i], wo, wi, lp);
+        f = f + lobe_eval(b.lobe(i), wo, wi, lp);
         p += b.w[i] * lp;
     }
     pdf = p;
     return f;
 }
 
+// DefaultMtlBSDF::SampleWi (src/BSDF.cuh:633-679)
 __device__ __forceinline__ bool bsdf_sample(const Bsdf &b, f3 wo, f3 &weight, f3 &wi, float &pdf, float u0,
                                             float u1, float u2)
 {
-    if (!b.is_default) return false;
+    if (!b.mask) return false;
     float cdf = 0.0f;
-    int sel = -1;
-    for (int i = 0; i < b.count; ++i) {
-        if (u0 < cdf + b.w[i]) {
-            sel = i;
-            break;
+    int sel = -1, last = -1;
+#pragma unroll
+    for (int i = 0; i < kSlots; ++i) {
+        if (!(b.mask & (1u << i))) continue;
+        last = i;
+        if (sel < 0) {
+            if (u0 < cdf + b.w[i]) sel = i;
+            else cdf += b.w[i];
         }
-        cdf += b.w[i];
     }
-    if (sel == -1) sel = b.count - 1;
-    if (sel == -1) return false;
+    if (sel == -1) sel = last;
     float lp = 0.0f;
-    if (!lobe_sample(b.l[sel], u1, u2, wo, weight, wi, lp)) return false;
-    float mp = lp * b.w[sel];
-    f3 mf = weight * lp;
-    for (int i = 0; i < b.count; ++i) {
-        if (i == sel) continue;
+    bool ok = false;
+#pragma unroll
+    for (int i = 0; i < kSlots; ++i)
+        if (i == sel) ok = lobe_sample(b.lobe(i), u1, u2, wo, weight, wi, lp);
+    if (!ok) return false;
+    float mp = 0.0f;
+    f3 mf = s3(0.0f);
+#pragma unroll
+    for (int i = 0; i < kSlots; ++i)
+        if (i == sel) {
+            mp = lp * b.w[i];
+            mf = weight * lp;
+        }
+#pragma unroll
+    for (int i = 0; i < kSlots; ++i) {
+        if (!(b.mask & (1u << i)) || i == sel) continue;
         float p = 0.0f;
-        mf = mf + lobe_eval(b.l[i], wo, wi, p);
+        mf = mf + lobe_eval(b.lobe(i), wo, wi, p);
         mp += b.w[i] * p;
     }
     pdf = mp;
@@ -605,28 +681,40 @@ __device__ __forceinline__ void hg_sample(float g, float u0, float u1, f3 wo, f3
     wi = u * s * cosf(phi) + v * s * sinf(phi) + w * c;
     hg_eval(g, wo, wi, pdf);
 }
+// SampleMixedPhases (src/VolumeEval.cuh:470-514); loops over the six fixed
+// slots with a count guard so vw/gs stay in registers.
 __device__ __forceinline__ bool mixed_phase_sample(const float *vw, const float *gs, int n, float u0, float u1,
                                                    f3 wo, float &weight, f3 &wi, float &pdf)
 {
     if (n == 0) return false;
     float cdf = 0.0f;
     int sel = -1;
-    for (int i = 0; i < n; ++i) {
+#pragma unroll
+    for (int i = 0; i < kMaxBoundaryRecord; ++i) {
+        if (i >= n || sel >= 0) continue;
         if (u0 < cdf + vw[i]) {
             sel = i;
             u0 = (u0 - cdf) * safe_rcp(vw[i]);
-            break;
+        } else {
+            cdf += vw[i];
         }
-        cdf += vw[i];
     }
     if (sel == -1) sel = n - 1;
+    float gsel = gs[0], vsel = vw[0];
+#pragma unroll
+    for (int i = 1; i < kMaxBoundaryRecord; ++i)
+        if (i == sel) {
+            gsel = gs[i];
+            vsel = vw[i];
+        }
     float pp = 0.0f;
-    hg_sample(gs[sel], u0, u1, wo, wi, pp);
+    hg_sample(gsel, u0, u1, wo, wi, pp);
     const float w1 = 1.0f;
-    float mp = pp * vw[sel];
-    float mg = w1 * pp * vw[sel];
-    for (int i = 0; i < n; ++i) {
-        if (i == sel) continue;
+    float mp = pp * vsel;
+    float mg = w1 * pp * vsel;
+#pragma unroll
+    for (int i = 0; i < kMaxBoundaryRecord; ++i) {
+        if (i >= n || i == sel) continue;
         float p;
         mg += vw[i] * hg_eval(gs[i], wo, wi, p);
         mp += vw[i] * p;
@@ -635,6 +723,7 @@ __device__ __forceinline__ bool mixed_phase_sample(const float *vw, const float 
     weight = mg * safe_rcp(mp);
     return true;
 }
+// EvalMixedPhases (src/VolumeEval.cuh:516-538)
 YM_HD float mixed_phase_eval(const float *vw, const float *gs, int n, f3 wo, f3 wi, float &pdf)
 {
     if (n == 0) {
@@ -642,7 +731,9 @@ YM_HD float mixed_phase_eval(const float *vw, const float *gs, int n, f3 wo, f3 
         return 0.0f;
     }
     float mg = 0.0f, mp = 0.0f;
-    for (int i = 0; i < n; ++i) {
+#pragma unroll
+    for (int i = 0; i < kMaxBoundaryRecord; ++i) {
+        if (i >= n) continue;
         float p = 0.0f;
         float g = hg_eval(gs[i], wo, wi, p);
         mg += vw[i] * g;
@@ -695,7 +786,9 @@ __device__ __forceinline__ bool sample_homogeneous(const DevScene &sc, int n, co
     trw = s3(1.0f);
     dist = kFloatMax;
     f3 st = s3(0.0f), ss = s3(0.0f);
-    for (int i = 0; i < n; ++i) {
+#pragma unroll
+    for (int i = 0; i < kMaxBoundaryRecord; ++i) {
+        if (i >= n) continue;
         const DevVolume &v = sc.volumes[vi[i]];
         st = st + vsig(v);
         ss = ss + vsig(v) * valb(v);
@@ -735,7 +828,9 @@ __device__ __forceinline__ bool sample_heterogeneous(const DevScene &sc, int n, 
     dist = kFloatMax;
     f3 ro[kMaxBoundaryRecord], rd[kMaxBoundaryRecord];
     f3 sm = s3(0.0f);
-    for (int i = 0; i < n; ++i) {
+#pragma unroll
+    for (int i = 0; i < kMaxBoundaryRecord; ++i) {
+        if (i >= n) continue;
         const DevVolume &v = sc.volumes[vi[i]];
         float ub = 1.0f;
         if (vol_is_het(sc, vi[i])) {
@@ -768,7 +863,9 @@ __device__ __forceinline__ bool sample_heterogeneous(const DevScene &sc, int n, 
         }
         acc += fpl;
         f3 st = s3(0.0f), ss = s3(0.0f);
-        for (int i = 0; i < n; ++i) {
+#pragma unroll
+        for (int i = 0; i < kMaxBoundaryRecord; ++i) {
+            if (i >= n) continue;
             const DevVolume &v = sc.volumes[vi[i]];
             float dens = 1.0f;
             if (vol_is_het(sc, vi[i])) dens = gmax(grid_density(sc, v.grid, ro[i] + rd[i] * acc), 0.0f);
@@ -805,16 +902,21 @@ __device__ __forceinline__ f3 eval_tr(const DevScene &sc, f3 ro_w, f3 rd_w, cons
     const int n = t.volumes(vi, sc.volume_count);
     if (n == 0) return s3(1.0f);
     bool het = false;
-    for (int i = 0; i < n; ++i)
-        if (vol_is_het(sc, vi[i])) het = true;
+#pragma unroll
+    for (int i = 0; i < kMaxBoundaryRecord; ++i)
+        if (i < n && vol_is_het(sc, vi[i])) het = true;
     if (!het) {
         f3 st = s3(0.0f);
-        for (int i = 0; i < n; ++i) st = st + vsig(sc.volumes[vi[i]]);
+#pragma unroll
+        for (int i = 0; i < kMaxBoundaryRecord; ++i)
+            if (i < n) st = st + vsig(sc.volumes[vi[i]]);
         return f3{expf(-st.x * tmax), expf(-st.y * tmax), expf(-st.z * tmax)};
     }
     f3 ro[kMaxBoundaryRecord], rd[kMaxBoundaryRecord];
     f3 sm = s3(0.0f);
-    for (int i = 0; i < n; ++i) {
+#pragma unroll
+    for (int i = 0; i < kMaxBoundaryRecord; ++i) {
+        if (i >= n) continue;
         const DevVolume &v = sc.volumes[vi[i]];
         float ub = 1.0f;
         if (vol_is_het(sc, vi[i])) {
@@ -846,7 +948,9 @@ __device__ __forceinline__ f3 eval_tr(const DevScene &sc, f3 ro_w, f3 rd_w, cons
         }
         acc += fpl;
         f3 st = s3(0.0f);
-        for (int i = 0; i < n; ++i) {
+#pragma unroll
+        for (int i = 0; i < kMaxBoundaryRecord; ++i) {
+            if (i >= n) continue;
             const DevVolume &v = sc.volumes[vi[i]];
             float dens = 1.0f;
             if (vol_is_het(sc, vi[i])) dens = gmax(grid_density(sc, v.grid, ro[i] + rd[i] * acc), 0.0f);

@@ -136,7 +136,10 @@ template <int kMode, bool kAny, bool kNearby, bool kCount, int kBlock, int kStac
 YM_HD int traverse(const DevScene &sc, const Query &q, Hit &hit, int &nb_count,
                                         LaneScratch<kBlock> &ls, TraceCounters &cnt)
 {
-    if (sc.instance_count == 0) return 0;
+    const uint32_t root = kMode == kModeAll ? sc.tlas_root
+                          : kMode == kModeSkipBoundary ? sc.tlas_root_solid
+                                                       : sc.tlas_root_boundary;
+    if (sc.instance_count == 0 || root == kSentinel) return 0;
     const bool skip_enabled = q.last_inst != kEmpty && q.last_tri != kEmpty;
     RayPre pre = make_pre(q.o, q.d);
     float tbest = q.tmax;
@@ -150,7 +153,7 @@ YM_HD int traverse(const DevScene &sc, const Query &q, Hit &hit, int &nb_count,
     float tub = 0.0f, tlb = 0.0f;
     int nbn = 0;
     int sp = 0;
-    uint32_t node = sc.tlas_root;
+    uint32_t node = root;
 
     for (;;) {
         if (!(node & kLeafBit)) {

@@ -64,8 +64,10 @@ __device__ __forceinline__ Transfer load_transfer(const PathState &ps, uint32_t 
     const uint32_t h = ps.xhdr[p];
     t.count = (int)(h & 0xFFu);
     t.max_idx = (int)((h >> 8) & 0xFFu) - 1;
-    for (int i = 0; i < t.count; ++i) {
-        const uint4 r = ps.xrec[(size_t)i * cap + p];
+#pragma unroll
+    for (int i = 0; i < kMaxBoundaryRecord; ++i) {
+        uint4 r = make_uint4(0u, 0u, 0u, 0u);
+        if (i < t.count) r = ps.xrec[(size_t)i * cap + p];
         t.rec[i] = CameraRecord{r.x, __uint_as_float(r.y), r.z, (int32_t)r.w};
     }
     return t;
@@ -73,16 +75,20 @@ __device__ __forceinline__ Transfer load_transfer(const PathState &ps, uint32_t 
 __device__ __forceinline__ void store_transfer(const PathState &ps, uint32_t p, uint32_t cap, const Transfer &t)
 {
     ps.xhdr[p] = (uint32_t)t.count | ((uint32_t)(t.max_idx + 1) << 8);
-    for (int i = 0; i < t.count; ++i)
-        ps.xrec[(size_t)i * cap + p] = make_uint4(t.rec[i].prim, __float_as_uint(t.rec[i].ior), t.rec[i].priority,
-                                                  (uint32_t)t.rec[i].vol);
+#pragma unroll
+    for (int i = 0; i < kMaxBoundaryRecord; ++i)
+        if (i < t.count)
+            ps.xrec[(size_t)i * cap + p] = make_uint4(t.rec[i].prim, __float_as_uint(t.rec[i].ior),
+                                                      t.rec[i].priority, (uint32_t)t.rec[i].vol);
 }
 __device__ __forceinline__ NearbyList load_nearby(const PathState &ps, uint32_t p, uint32_t cap, uint32_t hflags)
 {
     NearbyList nb;
     nb.n = (int)((hflags >> 4) & 0xFu);
-    for (int i = 0; i < nb.n; ++i) {
-        const uint2 e = ps.nb[(size_t)i * cap + p];
+#pragma unroll
+    for (int i = 0; i < kNearbyMax; ++i) {
+        uint2 e = make_uint2(0u, 0u);
+        if (i < nb.n) e = ps.nb[(size_t)i * cap + p];
         nb.prim[i] = e.x & 0x7FFFFFFFu;
         nb.back[i] = (int)(e.x >> 31);
         nb.t[i] = __uint_as_float(e.y);
@@ -211,7 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, PtParams prm, Pa
         const uint32_t base = wave_fetch(cnt + kCtrWorkExtend);
         if (base >= n) break;
         const uint32_t i = base + lane_id();
-        bool want_hit = false;
+        bool want_hit = false, want_vol = false;
         uint32_t p = 0;
         uint32_t rays = 0, cam_rays = 0;
         if (i < n) {
@@ -261,8 +267,9 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, PtParams prm, Pa
                 if (nv > 0) {
                     Sampler smp = load_sampler(ps, p);
                     bool het = false;
-                    for (int k = 0; k < nv; ++k)
-                        if (vol_is_het(sc, vi[k])) het = true;
+#pragma unroll
+                    for (int k = 0; k < kMaxBoundaryRecord; ++k)
+                        if (k < nv && vol_is_het(sc, vi[k])) het = true;
                     f3 trw;
                     float dist;
                     const float tmax = hit ? h.t : kTMax;
@@ -281,11 +288,14 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, PtParams prm, Pa
                 ps.out[p] = f4(Lv, 0.0f);
             } else {
                 want_hit = true;
+                want_vol = vol_hit;
                 uint32_t flags = (vol_hit ? kHitVolume : 0u) | (h.back ? kHitBack : 0u);
                 if (kNearby && hit) {
                     flags |= (uint32_t)nbc << 4;
-                    for (int k = 0; k < nbc; ++k)
-                        ps.nb[(size_t)k * prm.cap + p] =
+#pragma unroll
+                    for (int k = 0; k < kNearbyCap; ++k)
+                        if (k < nbc)
+                            ps.nb[(size_t)k * prm.cap + p] =
                             make_uint2(nbp[k * kBlock + lane_id()] | ((uint32_t)(nbb[k * kBlock + lane_id()] != 0) << 31),
                                        __float_as_uint(nbt[k * kBlock + lane_id()]));
                 } else if (hit) {
@@ -297,8 +307,10 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, PtParams prm, Pa
                 ps.hit_b[p] = vol_hit ? make_float4(vpos.x, vpos.y, vpos.z, 0.0f) : make_float4(h.b0, h.b1, h.b2, 0.0f);
             }
         }
-        const uint32_t slot = wave_append(cnt + kCtrHit, want_hit);
-        if (want_hit) ps.q_hit[slot] = p;
+        const uint32_t slot = wave_append(cnt + kCtrHit, want_hit && !want_vol);
+        if (want_hit && !want_vol) ps.q_hit[slot] = p;
+        const uint32_t vslot = wave_append(cnt + kCtrHitVol, want_vol);
+        if (want_vol) ps.q_hit_vol[vslot] = p;
         wave_add64(prm.stats + kStatExtension, rays);
         wave_add64(prm.stats + kStatCamera, cam_rays);
     }
@@ -313,64 +325,65 @@ struct ShadowOut {
     float mis[2];
 };
 
-__device__ __forceinline__ void shadow_from_light(const DevScene &sc, f3 pos, f3 perr, f3 ng, bool offset,
-                                                  f3 ray_d, f3 ldir, f3 lpos, f3 lerr, f3 lng, int k, ShadowOut &so,
-                                                  bool transmit)
+__device__ __forceinline__ void emit_shadow(f3 orig, f3 pos, f3 ldir, f3 lpos, f3 lerr, f3 lng, int k, bool transmit,
+                                            f3 f, f3 li, float mis, ShadowOut &so)
 {
-    const f3 orig = offset ? offset_ray_origin(pos, perr, ldir, ng) : pos;
-    const float maxd = length(offset_ray_origin(lpos, lerr, -ldir, lng) - pos) * kShadowRayClamp;
     so.o[k] = orig;
     so.d[k] = ldir;
-    so.tmax[k] = maxd;
+    so.tmax[k] = length(offset_ray_origin(lpos, lerr, -ldir, lng) - pos) * kShadowRayClamp;
     so.flags |= (1u << k) | (transmit ? (4u << k) : 0u);
-    (void)sc;
-    (void)ray_d;
+    so.f[k] = f;
+    so.li[k] = li;
+    so.mis[k] = mis;
 }
 
+// kVolume: volume scattering vertices (Ray_shading_path_tracing's volume branch,
+// src/Shading.cu:2070-2130, with EvalVolumeShapeLight 419-513); otherwise
+// surface vertices (2132-2302 with EvalShapeLight 294-417).
+template <bool kVolume>
 __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
 {
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     uint32_t *cnt_next = prm.counters + (pass + 1) * kCtrStride;
-    const uint32_t n = cnt[kCtrHit];
+    const uint32_t n = cnt[kVolume ? kCtrHitVol : kCtrHit];
+    const uint32_t *q = kVolume ? ps.q_hit_vol : ps.q_hit;
     const ymrt_render_settings &rs = prm.rs;
     const uint32_t cap = prm.cap;
     for (;;) {
-        const uint32_t base = wave_fetch(cnt + kCtrWorkShade);
+        const uint32_t base = wave_fetch(cnt + (kVolume ? kCtrWorkShadeVol : kCtrWorkShade));
         if (base >= n) break;
         const uint32_t i = base + lane_id();
         bool want_next = false, want_shadow = false;
         uint32_t p = 0;
         if (i < n) {
-            p = ps.q_hit[i];
+            p = q[i];
             uint32_t info = ps.info[p];
             int depth = info_depth(info);
             bool never_scatter = (info & kInfoNeverScatter) != 0;
             const uint32_t vertex = info_vertex(info) + 1;
             const uint4 ha = ps.hit_a[p];
             const float4 hb = ps.hit_b[p];
-            const float4 o4 = ps.ray_o[p], d4 = ps.ray_d[p];
-            const f3 ray_o = xyz(o4), ray_d = xyz(d4);
+            const f3 ray_d = xyz(ps.ray_d[p]);
             f3 beta = xyz(ps.beta[p]);
             f3 L = xyz(ps.L[p]);
             Sampler smp = load_sampler(ps, p);
-            Transfer t = load_transfer(ps, p, cap);
             ShadowOut so;
             so.flags = (uint32_t)depth << 8;
             bool alive = false;       // a next extension ray
-            bool nee = false;         // shadow rays were produced
-            f3 sh_beta = beta;
+            const f3 sh_beta = beta;
             float sel_pdf = 0.0f;
             f3 next_o = s3(0.0f), next_d = s3(0.0f);
             uint32_t next_li = kEmpty, next_lt = kEmpty;
             bool pending = false;
-            if (rs.enable_volume_scattering && (ha.w & kHitVolume)) {
-                // ---------------------------------------------- volume vertex
+            if constexpr (kVolume) {
+                const Transfer t = load_transfer(ps, p, cap);
                 int vi[kMaxBoundaryRecord];
                 float gs[kMaxBoundaryRecord], vw[kMaxBoundaryRecord];
                 const int nv = t.volumes(vi, sc.volume_count);
-                for (int k = 0; k < nv; ++k) {
-                    gs[k] = sc.volumes[vi[k]].g;
-                    vw[k] = 1.0f / (float)nv;
+#pragma unroll
+                for (int k = 0; k < kMaxBoundaryRecord; ++k) {
+                    gs[k] = k < nv ? sc.volumes[vi[k]].g : 0.0f;
+                    vw[k] = k < nv ? 1.0f / (float)nv : 0.0f;
                 }
                 const f3 vpos = xyz(hb);
                 so.flags |= kShVolume;
@@ -385,13 +398,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                         const bool valid = light_sample_li(sc, light, vpos, u0, u1, ldir, lpos, lerr, lng, Li, lpdf);
                         float wpdf = 0.0f;
                         const float ph = mixed_phase_eval(vw, gs, nv, -ray_d, ldir, wpdf);
-                        if (ph > kMinColorEpsilon && valid) {
-                            shadow_from_light(sc, vpos, s3(0.0f), s3(0.0f), false, ray_d, ldir, lpos, lerr, lng, 0,
-                                              so, false);
-                            so.f[0] = s3(gmin(ph, kMaxColorClamp));
-                            so.li[0] = min3(Li, s3(kMaxColorClamp));
-                            so.mis[0] = power_heuristic(lpdf, wpdf);
-                        }
+                        if (ph > kMinColorEpsilon && valid)
+                            emit_shadow(vpos, vpos, ldir, lpos, lerr, lng, 0, false, s3(gmin(ph, kMaxColorClamp)),
+                                        min3(Li, s3(kMaxColorClamp)), power_heuristic(lpdf, wpdf), so);
                     }
                     {
                         f3 pwi = s3(0.0f);
@@ -402,16 +411,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                             f3 lpos = s3(0.0f), lerr = s3(0.0f), lng = s3(0.0f);
                             float lpdf = 0.0f;
                             const f3 Li = light_eval_li(sc, light, vpos, pwi, lpos, lerr, lng, lpdf);
-                            if (maxcomp(Li) > kMinColorEpsilon) {
-                                shadow_from_light(sc, vpos, s3(0.0f), s3(0.0f), false, ray_d, pwi, lpos, lerr, lng, 1,
-                                                  so, false);
-                                so.f[1] = s3(gmin(pw, kMaxColorClamp));
-                                so.li[1] = min3(Li, s3(kMaxColorClamp));
-                                so.mis[1] = power_heuristic(ppdf, lpdf);
-                            }
+                            if (maxcomp(Li) > kMinColorEpsilon)
+                                emit_shadow(vpos, vpos, pwi, lpos, lerr, lng, 1, false, s3(gmin(pw, kMaxColorClamp)),
+                                            min3(Li, s3(kMaxColorClamp)), power_heuristic(ppdf, lpdf), so);
                         }
                     }
-                    nee = (so.flags & 3u) != 0;
                 }
                 f3 wi = s3(0.0f);
                 float pdf = 0.0f, pw = 0.0f;
@@ -434,10 +438,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                     }
                 }
             } else {
-                // ---------------------------------------------- surface vertex
                 const uint32_t inst = ha.y, tri = ha.z;
                 const int back = (ha.w & kHitBack) ? 1 : 0;
-                const DevInstance in = sc.instances[inst];
+                const DevInstance &in = sc.instances[inst];
                 const ShadeTri &st = sc.shade[in.shade_base + tri];
                 const f3 b{hb.x, hb.y, hb.z};
                 const f3 p0{st.p[0], st.p[1], st.p[2]}, p1{st.p[3], st.p[4], st.p[5]}, p2{st.p[6], st.p[7], st.p[8]};
@@ -447,34 +450,18 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                 f3 pos = interp3(p0, p1, p2, b);
                 f3 perr = error_gamma(7) * f3{xs, ys, zs};
                 f3 ng = cross(p1 - p0, p2 - p0);
-                f3 sn = (in.flags & kInstNoNormals)
-                            ? ng
-                            : interp3(f3{st.n[0], st.n[1], st.n[2]}, f3{st.n[3], st.n[4], st.n[5]},
-                                      f3{st.n[6], st.n[7], st.n[8]}, b);
-                const f2 uv0{st.uv[0], st.uv[1]}, uv1{st.uv[2], st.uv[3]}, uv2{st.uv[4], st.uv[5]};
-                const f2 uv{uv0.x * b.x + uv1.x * b.y + uv2.x * b.z, uv0.y * b.x + uv1.y * b.y + uv2.y * b.z};
-                float du01 = uv1.x - uv0.x, dv01 = uv1.y - uv0.y, du02 = uv2.x - uv0.x, dv02 = uv2.y - uv0.y;
-                float det = du01 * dv02 - dv01 * du02;
-                if (fabsf(det) < 1E-8f) {
-                    du01 = 1.0f;
-                    dv01 = 0.0f;
-                    du02 = 0.0f;
-                    dv02 = 1.0f;
-                    det = 1.0f;
-                }
-                const float idet = 1.0f / det;
-                const f3 dp01 = p1 - p0, dp02 = p2 - p0;
-                f3 dpdu = (dv02 * dp01 - dv01 * dp02) * idet;
-                f3 dpdv = (-du02 * dp01 + du01 * dp02) * idet;
-                pos = xf_position(in.o2w, pos, &perr);
-                sn = normalize(xf_transpose_vector(in.w2o, sn));
-                ng = normalize(xf_transpose_vector(in.w2o, ng));
-                dpdu = xf_vector(in.o2w, dpdu);
-                dpdv = xf_vector(in.o2w, dpdv);
+                const uint32_t iflags = in.flags;
                 const DevMaterial &m = sc.materials[in.material];
+                const uint32_t mtype = m.type;
                 uint32_t prio;
                 fetch_ior(m, &prio);
-                if ((t.count != 0 && prio < t.rec[t.max_idx].priority) || (in.flags & kInstBoundary)) {
+                const Transfer t = load_transfer(ps, p, cap);
+                const bool pass_through = (t.count != 0 && prio < t.get(t.max_idx).priority) || (iflags & kInstBoundary);
+                if (pass_through || mtype == YMRT_MATERIAL_LIGHT) {
+                    pos = xf_position(in.o2w, pos, &perr);
+                    ng = normalize(xf_transpose_vector(in.w2o, ng));
+                }
+                if (pass_through) {
                     // pass through without consuming depth; the medium records
                     // change at the start of the next extend
                     pending = true;
@@ -483,17 +470,40 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                     next_d = normalize(ray_d);
                     next_li = inst;
                     next_lt = tri;
-                } else if (m.type == YMRT_MATERIAL_LIGHT) {
+                } else if (mtype == YMRT_MATERIAL_LIGHT) {
                     if (never_scatter)
                         L = L + light_clamp(rs, beta * f3{m.light_color[0], m.light_color[1], m.light_color[2]} *
                                                     m.intensity,
                                             depth);
                 } else {
+                    f3 sn = (iflags & kInstNoNormals)
+                                ? ng
+                                : interp3(f3{st.n[0], st.n[1], st.n[2]}, f3{st.n[3], st.n[4], st.n[5]},
+                                          f3{st.n[6], st.n[7], st.n[8]}, b);
+                    const f2 uv0{st.uv[0], st.uv[1]}, uv1{st.uv[2], st.uv[3]}, uv2{st.uv[4], st.uv[5]};
+                    const f2 uv{uv0.x * b.x + uv1.x * b.y + uv2.x * b.z, uv0.y * b.x + uv1.y * b.y + uv2.y * b.z};
+                    float du01 = uv1.x - uv0.x, dv01 = uv1.y - uv0.y, du02 = uv2.x - uv0.x, dv02 = uv2.y - uv0.y;
+                    float det = du01 * dv02 - dv01 * du02;
+                    if (fabsf(det) < 1E-8f) {
+                        du01 = 1.0f;
+                        dv01 = 0.0f;
+                        du02 = 0.0f;
+                        dv02 = 1.0f;
+                        det = 1.0f;
+                    }
+                    const float idet = 1.0f / det;
+                    const f3 dp01 = p1 - p0, dp02 = p2 - p0;
+                    f3 dpdu = (dv02 * dp01 - dv01 * dp02) * idet;
+                    f3 dpdv = (-du02 * dp01 + du01 * dp02) * idet;
+                    pos = xf_position(in.o2w, pos, &perr);
+                    sn = normalize(xf_transpose_vector(in.w2o, sn));
+                    ng = normalize(xf_transpose_vector(in.w2o, ng));
+                    dpdu = xf_vector(in.o2w, dpdu);
+                    dpdv = xf_vector(in.o2w, dpdv);
                     Bsdf bs;
                     init_shading_space(bs, back, ray_d, ng, sn, dpdu, dpdv);
-                    bs.is_default = m.type == YMRT_MATERIAL_DEFAULT;
-                    bs.count = 0;
-                    if (bs.is_default)
+                    bs.mask = 0;
+                    if (mtype == YMRT_MATERIAL_DEFAULT)
                         init_default_bsdf(sc, bs, m, uv, ray_d, back, t.ray_ior(), t.ex_ior(inst));
                     const f3 wo = bs.to_local(-ray_d);
                     if (rs.enable_shape_light && sc.light_count > 0) {
@@ -511,13 +521,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                             if (maxcomp(f) > kMinColorEpsilon && valid) {
                                 const bool transmit = dot(-ray_d, ng) * dot(ldir, ng) < 0.0f;
                                 const bool invalid = (!same_hemi(wo, lwi) && !transmit) || (same_hemi(wo, lwi) && transmit);
-                                if (!invalid) {
-                                    shadow_from_light(sc, pos, perr, ng, true, ray_d, ldir, lpos, lerr, lng, 0, so,
-                                                      transmit);
-                                    so.f[0] = min3(f, s3(kMaxColorClamp));
-                                    so.li[0] = min3(Li, s3(kMaxColorClamp));
-                                    so.mis[0] = power_heuristic(lpdf, wpdf);
-                                }
+                                if (!invalid)
+                                    emit_shadow(offset_ray_origin(pos, perr, ldir, ng), pos, ldir, lpos, lerr, lng, 0,
+                                                transmit, min3(f, s3(kMaxColorClamp)), min3(Li, s3(kMaxColorClamp)),
+                                                power_heuristic(lpdf, wpdf), so);
                             }
                         }
                         {
@@ -535,17 +542,13 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                                     const bool transmit = dot(-ray_d, ng) * dot(ldir, ng) < 0.0f;
                                     const bool invalid =
                                         (!same_hemi(wo, bwi) && !transmit) || (same_hemi(wo, bwi) && transmit);
-                                    if (!invalid) {
-                                        shadow_from_light(sc, pos, perr, ng, true, ray_d, ldir, lpos, lerr, lng, 1, so,
-                                                          transmit);
-                                        so.f[1] = min3(bw, s3(kMaxColorClamp));
-                                        so.li[1] = min3(Li, s3(kMaxColorClamp));
-                                        so.mis[1] = power_heuristic(bpdf, lpdf);
-                                    }
+                                    if (!invalid)
+                                        emit_shadow(offset_ray_origin(pos, perr, ldir, ng), pos, ldir, lpos, lerr, lng,
+                                                    1, transmit, min3(bw, s3(kMaxColorClamp)),
+                                                    min3(Li, s3(kMaxColorClamp)), power_heuristic(bpdf, lpdf), so);
                                 }
                             }
                         }
-                        nee = (so.flags & 3u) != 0;
                     }
                     f3 wi = s3(0.0f), bw = s3(0.0f);
                     float pdf = 0.0f;
@@ -579,6 +582,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                 }
             }
             // ---------------------------------------------- write back
+            const bool nee = (so.flags & 3u) != 0;
             store_sampler(ps, p, smp);
             ps.L[p] = f4(L, 0.0f);
             if (nee) {
@@ -586,6 +590,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                 if (!alive) so.flags |= kShTerminate;
                 ps.sh_flags[p] = so.flags;
                 ps.sh_c[p] = f4(sh_beta, sel_pdf);
+#pragma unroll
                 for (int k = 0; k < 2; ++k) {
                     if (!(so.flags & (1u << k))) continue;
                     ps.sh_o[(size_t)k * cap + p] = f4(so.o[k], so.tmax[k]);
@@ -862,9 +867,13 @@ int launch_extend(const DevScene &sc, const PtParams &prm, const PathState &ps, 
     return nearby ? launch_extend_t<true, 64>(sc, prm, ps, pass, s) : launch_extend_t<false, 64>(sc, prm, ps, pass, s);
 }
 
-int launch_shade(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
+int launch_shade(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool volume,
+                 hipStream_t s)
 {
-    hipLaunchKernelGGL(k_shade, persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps, pass);
+    if (volume)
+        hipLaunchKernelGGL(k_shade<true>, persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps, pass);
+    else
+        hipLaunchKernelGGL(k_shade<false>, persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps, pass);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }

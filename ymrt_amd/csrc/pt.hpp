@@ -8,14 +8,16 @@ namespace ym {
 
 // Per-pass device counters: counters[pass * kCtrStride + k]
 constexpr uint32_t kCtrActive = 0;       // extension rays queued for this pass
-constexpr uint32_t kCtrHit = 1;          // hits queued for shading
+constexpr uint32_t kCtrHit = 1;          // surface hits queued for shading
 constexpr uint32_t kCtrShadow = 2;       // paths with shadow rays
-constexpr uint32_t kCtrWorkRaygen = 3;   // work-fetch cursors
-constexpr uint32_t kCtrWorkExtend = 4;
-constexpr uint32_t kCtrWorkShade = 5;
-constexpr uint32_t kCtrWorkShadow = 6;
-constexpr uint32_t kCtrWorkFlush = 7;
-constexpr uint32_t kCtrStride = 8;
+constexpr uint32_t kCtrHitVol = 3;       // volume scattering vertices queued for shading
+constexpr uint32_t kCtrWorkRaygen = 4;   // work-fetch cursors
+constexpr uint32_t kCtrWorkExtend = 5;
+constexpr uint32_t kCtrWorkShade = 6;
+constexpr uint32_t kCtrWorkShadeVol = 7;
+constexpr uint32_t kCtrWorkShadow = 8;
+constexpr uint32_t kCtrWorkFlush = 9;
+constexpr uint32_t kCtrStride = 12;
 constexpr uint32_t kMaxPasses = 4 * kMaxRayDepth;
 
 // Frame statistics (device, 64-bit)
@@ -42,7 +44,7 @@ struct PathState {
     uint32_t *sh_flags;
     float4 *out;        // final radiance per path
     uint32_t *queue[2];
-    uint32_t *q_hit, *q_shadow;
+    uint32_t *q_hit, *q_hit_vol, *q_shadow;
 };
 
 struct PtParams {

@@ -20,7 +20,8 @@ namespace ym {
 int launch_raygen(const DevScene &sc, const PtParams &prm, const PathState &ps, hipStream_t s);
 int launch_extend(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool nearby,
                   uint32_t stack, hipStream_t s);
-int launch_shade(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s);
+int launch_shade(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool volume,
+                 hipStream_t s);
 int launch_shadow(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool media,
                   uint32_t stack, hipStream_t s);
 int launch_flush(const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s);
@@ -92,6 +93,7 @@ static int ensure_buffers(ymrt_scene *sc, uint32_t cap)
     rc |= alloc(rb, &ps.queue[0], n);
     rc |= alloc(rb, &ps.queue[1], n);
     rc |= alloc(rb, &ps.q_hit, n);
+    rc |= alloc(rb, &ps.q_hit_vol, n);
     rc |= alloc(rb, &ps.q_shadow, n);
     rc |= alloc(rb, &rb->counters, (size_t)(kMaxPasses + 2) * kCtrStride);
     rc |= alloc(rb, &rb->stats, kStatCount);
@@ -187,7 +189,9 @@ int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, floa
         for (;;) {
             if ((rc = timed(1, [&] { return launch_extend(sc->dev, prm, rb->ps, pass, nearby, stack, stream); })))
                 return rc;
-            if ((rc = timed(2, [&] { return launch_shade(sc->dev, prm, rb->ps, pass, stream); }))) return rc;
+            if ((rc = timed(2, [&] { return launch_shade(sc->dev, prm, rb->ps, pass, false, stream); }))) return rc;
+            if (media && (rc = timed(2, [&] { return launch_shade(sc->dev, prm, rb->ps, pass, true, stream); })))
+                return rc;
             if ((rc = timed(3, [&] { return launch_shadow(sc->dev, prm, rb->ps, pass, media, stack, stream); })))
                 return rc;
             ++pass;

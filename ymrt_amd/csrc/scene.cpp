@@ -322,16 +322,29 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
         bp.id = i;
         iprims.push_back(bp);
     }
+    // Three top-level trees: all instances (closest hit), non-boundary
+    // instances (shadow any-hit), boundary instances (transmittance walk).
     uint32_t tlas_depth = 0;
-    uint32_t tlas_root = kSentinel;
-    if (!iprims.empty()) {
-        BinaryBuilder tb(iprims, 1);
+    auto build_tlas = [&](int which) -> uint32_t {
+        std::vector<BuildPrim> sel;
+        for (const BuildPrim &bp : iprims) {
+            const bool boundary = d->instances[bp.id].treat_as_boundary != 0;
+            if (which == 0 || (which == 1 && !boundary) || (which == 2 && boundary)) sel.push_back(bp);
+        }
+        if (sel.empty()) return kSentinel;
+        BinaryBuilder tb(sel, 1);
         std::vector<BinNode> bin = tb.build();
         // leaves of size 1 referencing the original instance index
         for (BinNode &n : bin)
-            if (n.left < 0) n.first = iprims[n.first].id;
-        tlas_root = collapse_wide(bin, h.nodes, 0, &tlas_depth);
-    }
+            if (n.left < 0) n.first = sel[n.first].id;
+        uint32_t depth = 0;
+        uint32_t root = collapse_wide(bin, h.nodes, 0, &depth);
+        tlas_depth = std::max(tlas_depth, depth);
+        return root;
+    };
+    const uint32_t tlas_root = build_tlas(0);
+    const uint32_t tlas_root_solid = build_tlas(1);
+    const uint32_t tlas_root_boundary = build_tlas(2);
     sc->max_stack = 3 * max_blas_depth + 3 * tlas_depth + 2;
     sc->needs_nearby = has_boundaries != 0;
     for (const DevMaterial &m : h.materials)
@@ -344,6 +357,8 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
     DevScene &ds = sc->dev;
     ds = DevScene{};
     ds.tlas_root = tlas_root;
+    ds.tlas_root_solid = tlas_root_solid;
+    ds.tlas_root_boundary = tlas_root_boundary;
     ds.instance_count = (uint32_t)iprims.size() ? d->instance_count : 0;
     ds.material_count = d->material_count;
     ds.has_boundaries = has_boundaries;
