@@ -25,16 +25,18 @@ sys.path.insert(0, ROOT)
 METRIC = "Mrays/s & 1080p samples/s vs plain; adjoint/forward time at fixed RMSE"
 HBM_PEAK_GBS = 8000.0
 
-# Bytes a kernel must move per item ("Roofline accounting").
-# extend: queue id 4 + ray 32 + last hit 8 + info 4 + hit record 32 + nearby 8
-#         + hit-queue id 4, plus the BVH bytes it visits (128 B per wide node,
-#         48 B per triangle), counted by a separate counting frame.
-EXTEND_STATE_BYTES = 4 + 32 + 8 + 4 + 32 + 8 + 4
-# shade: queue id 4 + info 4 + hit 32 + ray 32 + beta 16 + L 16 + sampler 16
-#        + transfer header 4 + shading triangle 96 + instance 128 + material 128
-#        + writes (sampler 16, L 16, ray 32, last 8, beta 16, info 4,
-#        shadow rays 2*64 + 16 + 4, queue ids 8)
-SHADE_BYTES = 4 + 4 + 32 + 32 + 16 + 16 + 16 + 4 + 96 + 128 + 128 + 16 + 16 + 32 + 8 + 16 + 4 + 148 + 8
+# Algorithmic bytes per item of the traversal kernels ("Roofline
+# accounting").  extend, per closest-hit segment: queue id 4 + info 4 + ray 32
+# + last hit 8 + medium header 4 read; hit record 32 + nearby entry 8 + queue
+# id 4 written.  Each visited wide node costs 128 B and each tested triangle
+# 48 B; the kernel counts both on the device in every frame.
+EXTEND_STATE_BYTES = 4 + 4 + 32 + 8 + 4 + 32 + 8 + 4
+NODE_BYTES = 128
+TRI_BYTES = 48
+# shadow, per path with shadow rays: queue id 4 + flags 4 + vertex 16 + L 16
+# read/write 32, per shadow ray o/d/f/li 64; plus visited nodes and triangles.
+SHADOW_PATH_BYTES = 4 + 4 + 16 + 32
+SHADOW_RAY_BYTES = 64
 
 
 def parse():
@@ -52,37 +54,42 @@ def parse():
 
 
 def cpu_baseline(sa, rs_full, budget_s):
-    """Oracle (the CPU restatement) on a band of rows of the same frame at 1 spp,
-    sized to roughly budget_s seconds of work on the host's cores."""
-    import threading
-    import numpy as np
+    """The oracle (the CPU restatement) rendering the same frame on the host's
+    cores: a probe of 8 rows at 1 spp sizes a sample of about budget_s seconds
+    (whole frame at k spp, or a centred band of rows at 1 spp)."""
     import oracle
     from ymrt_amd import abi
 
     threads = max(1, min(16, os.cpu_count() or 1))
     sc = oracle.Scene(sa)
-    rs = abi.render_settings(rs_full.width, rs_full.height, 1, rs_full.ray_depth, volume=1)
-    # probe: 8 rows
+    w, h = rs_full.width, rs_full.height
+    rs1 = abi.render_settings(w, h, 1, rs_full.ray_depth, volume=1)
     t0 = time.time()
-    sc.render(rs, 1, rows=(536, 544), threads=threads)
-    dt = max(time.time() - t0, 1e-3)
-    rows = int(max(8, min(rs.height, 8 * budget_s / dt)))
-    r0 = max(0, rs.height // 2 - rows // 2)
+    sc.render(rs1, 1, rows=(h // 2 - 4, h // 2 + 4), threads=threads)
+    frame_s = max(time.time() - t0, 1e-3) * h / 8.0
+    if frame_s < budget_s:
+        spp = int(max(1, min(rs_full.spp, budget_s // frame_s)))
+        rows = (0, h)
+    else:
+        spp = 1
+        n = int(max(8, min(h, h * budget_s / frame_s)))
+        rows = (h // 2 - n // 2, h // 2 - n // 2 + n)
+    rs = abi.render_settings(w, h, spp, rs_full.ray_depth, volume=1)
     t0 = time.time()
-    sc.render(rs, 1, rows=(r0, r0 + rows), threads=threads)
+    sc.render(rs, 1, rows=rows, threads=threads)
     dt = time.time() - t0
     c = sc.last_counts
     rays = int(c[1] + c[2] + c[3])
-    paths = rows * rs.width
+    samples = (rows[1] - rows[0]) * w * spp
     sc.close()
     return {
-        "value": rays / dt / 1e6,
+        "value": round(rays / dt / 1e6, 3),
         "unit": "Mrays/s",
         "cores": threads,
         "kind": "port",
-        "samples_per_s": paths / dt,
-        "sample": "oracle, rows %d-%d of the 1920x1080 frame at 1 spp (%d paths, %d rays), %.1f s"
-                  % (r0, r0 + rows, paths, rays, dt),
+        "samples_per_s": round(samples / dt, 1),
+        "sample": "oracle, rows %d-%d of the %dx%d frame at %d spp (%d samples, %d rays), %.1f s on %d threads"
+                  % (rows[0], rows[1], w, h, spp, samples, rays, dt, threads),
     }
 
 
@@ -122,22 +129,24 @@ def main():
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     rays = 0
-    stage = dict(extend=0.0, shade=0.0, shadow=0.0, raygen=0.0, resolve=0.0)
-    launches = dict(extend=0, shade=0, shadow=0)
+    stage = dict(extend=0.0, medium=0.0, shade=0.0, shadow=0.0, raygen=0.0, resolve=0.0)
+    launches = dict(extend=0, medium=0, shade=0, shadow=0)
+    visits = dict(extend_nodes=0, extend_tris=0, shadow_nodes=0, shadow_tris=0)
+    ext_rays = sh_rays = tr_rays = 0
     t_wall = time.time()
     start.record()
     for k in range(args.steps):
         _, st = sc.render_frame(rs, 1 + args.warmup + k, noise, timing=True)
-        rays += st.camera_rays * 0 + st.extension_rays + st.shadow_rays + st.transmittance_rays
-        stage["extend"] += st.ms_extend
-        stage["shade"] += st.ms_shade
-        stage["shadow"] += st.ms_shadow
-        stage["raygen"] += st.ms_raygen
-        stage["resolve"] += st.ms_resolve
-        launches["extend"] += st.launches_extend
-        launches["shade"] += st.launches_shade
-        launches["shadow"] += st.launches_shadow
-        last = st
+        rays += st.extension_rays + st.shadow_rays + st.transmittance_rays
+        ext_rays += st.extension_rays
+        sh_rays += st.shadow_rays
+        tr_rays += st.transmittance_rays
+        for key in stage:
+            stage[key] += getattr(st, "ms_" + key)
+        for key in launches:
+            launches[key] += getattr(st, "launches_" + key)
+        for key in visits:
+            visits[key] += getattr(st, key)
         print("step %d: %.1f ms" % (k, st.ms_total), file=sys.stderr, flush=True)
     stop.record()
     torch.cuda.synchronize()
@@ -147,23 +156,19 @@ def main():
     mrays = rays / (ms / 1e3) / 1e6
     samples_per_s = args.width * args.height * args.spp * args.steps / (ms / 1e3)
 
-    # Roofline of the dominant stage.  Visited BVH bytes come from the
-    # extension rays' average visits, measured by the counting trace kernel on
-    # a sample of this frame's first-bounce rays is not available per stage, so
-    # the extend figure uses the compulsory state bytes; shade uses its fixed
-    # per-item bytes.
-    dom = max(("extend", "shade", "shadow"), key=lambda k: stage[k])
-    per_step = {k: v / args.steps for k, v in stage.items()}
-    if dom == "shade":
-        items = last.extension_rays  # hits shaded <= extension rays
-        bytes_per_step = SHADE_BYTES * items
-    elif dom == "extend":
-        items = last.extension_rays
-        bytes_per_step = EXTEND_STATE_BYTES * items
+    # Roofline of the dominant traversal kernel: algorithmic bytes (state +
+    # visited nodes and triangles, counted on the device) over its HIP-event
+    # time on the render stream.
+    ext_bytes = (EXTEND_STATE_BYTES * ext_rays + NODE_BYTES * visits["extend_nodes"]
+                 + TRI_BYTES * visits["extend_tris"])
+    sh_bytes = (SHADOW_RAY_BYTES * (sh_rays + tr_rays) + SHADOW_PATH_BYTES * sh_rays // 2
+                + NODE_BYTES * visits["shadow_nodes"] + TRI_BYTES * visits["shadow_tris"])
+    if stage["shadow"] > stage["extend"]:
+        dom, dom_bytes, dom_ms, dom_launch = "k_shadow", sh_bytes, stage["shadow"], launches["shadow"]
     else:
-        items = last.shadow_rays
-        bytes_per_step = 64 * items
-    achieved = bytes_per_step / (per_step[dom] / 1e3) / 1e9
+        dom, dom_bytes, dom_ms, dom_launch = "k_extend", ext_bytes, stage["extend"], launches["extend"]
+    achieved = dom_bytes / (dom_ms / 1e3) / 1e9
+    per_step = {k: v / args.steps for k, v in stage.items()}
     out = {
         "metric": METRIC,
         "value": round(mrays, 2),
@@ -185,7 +190,16 @@ def main():
         "rays_per_step": int(rays / args.steps),
         "stage_ms_per_step": {k: round(v, 2) for k, v in per_step.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": int(dom_bytes / max(dom_launch, 1)),
+                     "avg_launch_ms": round(dom_ms / max(dom_launch, 1), 4),
+                     "launches": dom_launch},
+        "traversal": {"extend_rays": ext_rays // args.steps,
+                      "nodes_per_extend_ray": round(visits["extend_nodes"] / max(ext_rays, 1), 2),
+                      "tris_per_extend_ray": round(visits["extend_tris"] / max(ext_rays, 1), 2),
+                      "shadow_rays": sh_rays // args.steps, "transmittance_rays": tr_rays // args.steps,
+                      "nodes_per_shadow_ray": round(visits["shadow_nodes"] / max(sh_rays + tr_rays, 1), 2),
+                      "tris_per_shadow_ray": round(visits["shadow_tris"] / max(sh_rays + tr_rays, 1), 2)},
         "wall_s": round(wall, 2),
     }
     if not args.no_cpu:

@@ -199,6 +199,11 @@ typedef struct ymrt_frame_stats {
     float ms_raygen, ms_extend, ms_shade, ms_shadow, ms_resolve;
     uint32_t launches_extend, launches_shade, launches_shadow;
     float ms_total;
+    float ms_medium;                 /* free-flight sampling (scenes with media) */
+    uint32_t launches_medium;
+    /* Traversal work, always counted: wide nodes visited and triangles
+     * tested by closest-hit (extend) and by shadow/transmittance queries. */
+    uint64_t extend_nodes, extend_tris, shadow_nodes, shadow_tris;
 } ymrt_frame_stats;
 
 /* ---------------------------------------------------------------- entry points */

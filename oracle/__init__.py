@@ -76,6 +76,14 @@ def lib():
             L.or_texture_eval.argtypes = [vp, C.c_uint32, C.c_float, C.c_float]
             L.or_accumulate.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_int]
             L.or_tonemap.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_float, C.c_float]
+            L.or_probe_ggx_d.restype = C.c_float
+            L.or_probe_ggx_d.argtypes = [C.c_float] * 5
+            L.or_probe_fresnel_dd.restype = C.c_float
+            L.or_probe_fresnel_dd.argtypes = [C.c_float, C.c_float]
+            L.or_probe_fresnel_dc.argtypes = [vp, vp, C.c_float, vp]
+            L.or_probe_edge_tint.argtypes = [vp, vp, vp, vp]
+            L.or_probe_free_flight.restype = C.c_int
+            L.or_probe_free_flight.argtypes = [vp, vp, C.c_float, C.c_float, vp, vp]
             _lib = L
     return _lib
 
@@ -231,3 +239,39 @@ def accumulate(noise, accum, frame):
 def tonemap(img, gamma, exposure):
     h, w, _ = img.shape
     lib().or_tonemap(_p(img), w, h, gamma, exposure)
+
+
+def ggx_d(h, ax, ay):
+    return lib().or_probe_ggx_d(float(h[0]), float(h[1]), float(h[2]), ax, ay)
+
+
+def fresnel_dielectric(eta, cos_i):
+    return lib().or_probe_fresnel_dd(eta, cos_i)
+
+
+def fresnel_conductor(eta, k, cos_i):
+    e = np.ascontiguousarray(eta, np.float32)
+    kk = np.ascontiguousarray(k, np.float32)
+    out = np.zeros(3, np.float32)
+    lib().or_probe_fresnel_dc(_p(e), _p(kk), cos_i, _p(out))
+    return out
+
+
+def edge_tint(reflectivity, tint):
+    r = np.ascontiguousarray(reflectivity, np.float32)
+    t = np.ascontiguousarray(tint, np.float32)
+    n = np.zeros(3, np.float32)
+    k = np.zeros(3, np.float32)
+    lib().or_probe_edge_tint(_p(r), _p(t), _p(n), _p(k))
+    return n, k
+
+
+def free_flight(sigma_t, albedo, tmax, u):
+    """SampleTrHomogeneous for one medium with the sampler value u:
+    (scattered, distance, weight rgb)."""
+    s = np.ascontiguousarray(sigma_t, np.float32)
+    a = np.ascontiguousarray(albedo, np.float32)
+    dist = np.zeros(1, np.float32)
+    w = np.zeros(3, np.float32)
+    r = lib().or_probe_free_flight(_p(s), _p(a), tmax, u, _p(dist), _p(w))
+    return bool(r), float(dist[0]), w

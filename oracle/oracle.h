@@ -8,6 +8,25 @@
  *
  * Scene input is the same POD descriptor the product's C ABI takes
  * (include/ymrt.h), so both sides see identical float inputs.
+ *
+ * What pins each part (tests/test_oracle_*.py) without trusting the oracle:
+ *   primes, prime sums, MT19937, permutation table, radical inverses (exact
+ *   rationals), Halton enumerator (brute force), PCG4d (published algorithm);
+ *   triangle test (float64), BVH closest/any hit (brute force over all
+ *   triangles), watertightness; sRGB decode (IEC 61966-2-1), bilinear image
+ *   lookup (ramps), Fresnel (textbook complex-index formulas), edge tint
+ *   (reflectivity identity), GGX D (normalisation), every DefaultMtl lobe
+ *   (sample/eval consistency, sampling histogram), Henyey-Greenstein (moments,
+ *   sampled density), homogeneous free flight (closed form), grid free flight
+ *   and transmittance (constant grid == homogeneous), surface NEE with MIS
+ *   (analytic irradiance of a square emitter), accumulate and ACES tonemap.
+ *
+ * Parity unpinned (checked only against the HIP path, no independent truth):
+ *   indirect_light_clamp, Russian roulette and the environment background
+ *   (all off in the measured workloads); next-event estimation at volume
+ *   vertices (EvalVolumeShapeLight) and the mixture of overlapping media;
+ *   the medium-record stack (RayTransfer priorities, BoundaryTransitionBunch,
+ *   camera medium records, nested-dielectric pass-through).
  */
 #ifndef ORACLE_H
 #define ORACLE_H

@@ -6,7 +6,9 @@ order, so paths follow the same vertices; their floats differ in the last bits
 where GPU and glibc transcendentals (sin, cos, log, exp) round differently,
 and a few paths diverge outright.  Bars:
   * >= 95% of pixels within 1e-4 relative, relative RMSE <= 1% (Cornell, 8 spp);
-  * relative RMSE <= 2% for the textured, fogged interior (homogeneous media);
+  * media (fogged textured interior, homogeneous and voxel-grid boxes):
+    relative RMSE <= 1-2%, with a looser per-pixel bar since every segment
+    in a medium runs expf/logf;
   * accumulate and tonemap equal to the oracle's;
   * two GPU frames bit-identical (no float atomics anywhere).
 """
@@ -87,3 +89,37 @@ def test_accumulate_tonemap(gpu_ctx):
     oracle.tonemap(tm_o, 2.2, 1.5)
     ymrt_amd.tonemap(acc_g, 2.2, 1.5)
     assert np.allclose(acc_g.cpu().numpy(), tm_o, rtol=2e-6, atol=1e-7)
+
+
+def _media_report(name, g, o, stats):
+    same, err = close_frac(g, o), rel_rmse(g, o)
+    print("%s: close pixels %.4f  rel rmse %.5f  ext %d shadow %d transmittance %d" %
+          (name, same, err, stats.extension_rays, stats.shadow_rays, stats.transmittance_rays))
+    return same, err
+
+
+def test_interior_fog_parity(gpu_ctx):
+    """Textured interior inside homogeneous fog (camera in the medium)."""
+    sa = scenes.interior(64, 40, tex_size=64, instances=8)
+    rs = abi.render_settings(64, 40, 4, 8, volume=1)
+    g, o, stats = render_both(gpu_ctx, sa, rs)
+    assert np.isfinite(g).all()
+    same, err = _media_report("interior fog", g, o, stats)
+    assert stats.transmittance_rays > 0
+    # every fog segment runs expf/logf, whose last bits differ between the GPU
+    # and glibc, so fewer pixels agree to 1e-4 while the image error stays low
+    assert same >= 0.70
+    assert err <= 0.01
+
+
+@pytest.mark.parametrize("grid", [False, True], ids=["homogeneous", "grid"])
+def test_cornell_medium_parity(gpu_ctx, grid):
+    """A medium box seen from outside: boundary pass-through, free flight
+    (homogeneous or voxel-grid delta tracking), transmittance of shadow rays."""
+    sa = scenes.cornell_media(48, 48, grid=grid)
+    rs = abi.render_settings(48, 48, 4, 6, volume=1)
+    g, o, stats = render_both(gpu_ctx, sa, rs)
+    assert np.isfinite(g).all()
+    same, err = _media_report("cornell medium grid=%d" % grid, g, o, stats)
+    assert same >= 0.50
+    assert err <= 0.02

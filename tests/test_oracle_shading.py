@@ -1,0 +1,403 @@
+"""Pins the oracle's shading, media, texture and output functions against
+closed forms, published algorithms and integral identities (no GPU).
+
+Each check is chosen so that a plausible slip in the restatement (a dropped
+cosine, a wrong Jacobian, a swapped sign or channel) fails it:
+  * sRGB decode against IEC 61966-2-1, for all 256 byte values;
+  * dielectric and conductor Fresnel against the textbook complex-index
+    formulas evaluated in float64;
+  * the edge-tint mapping (Gulbrandsen 2014) reproduces its input
+    reflectivity at normal incidence;
+  * GGX D integrates to 1 over projected solid angle, isotropic and not;
+  * Lambert eval/pdf equal albedo*cos/pi and cos/pi; every lobe's sampled
+    weight and pdf equal eval/pdf at the sampled direction; the sampled GGX
+    directions follow the eval pdf (histogram);
+  * Henyey-Greenstein: the eval pdf integrates to 1 with mean cosine g about
+    the forward direction; the sampler follows the mirrored density (a quirk
+    of src/Volume.h:21-48 that the oracle keeps on purpose);
+  * homogeneous free flight: distance = -ln(1-u')/sigma_c with the channel
+    picked by sigma/sum(sigma), scatter weight = albedo, pass weight = 1;
+  * bilinear image lookup on a ramp with repeat and clamp wrapping;
+  * PCG4d against the published algorithm (Jarzynski & Olano, JCGT 2020);
+  * ACES fit tonemap (Narkowicz 2015) and the running-mean accumulate;
+  * a rendered Lambert floor under a square area light equals the analytic
+    irradiance (form factor of a parallel rectangle) times albedo/pi.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from ymrt_amd import abi, scenes
+
+
+# ---------------------------------------------------------------- sRGB decode
+def test_srgb_decode_iec():
+    for b in range(256):
+        s = np.float32(b / 255.0)
+        ref = s / 12.92 if s <= 0.04045 else ((s + 0.055) / 1.055) ** 2.4
+        assert abs(oracle.srgb_to_linear(float(s)) - ref) <= 2e-7 + 2e-6 * ref, b
+
+
+# --------------------------------------------------------------------- Fresnel
+def _fresnel_dielectric_f64(eta, c):
+    s2 = 1.0 - c * c
+    t = 1.0 - s2 / (eta * eta)
+    if t <= 0.0:
+        return 1.0
+    ct = math.sqrt(t)
+    rs = (c - eta * ct) / (c + eta * ct)
+    rp = (ct - eta * c) / (ct + eta * c)
+    return 0.5 * (rs * rs + rp * rp)
+
+
+@pytest.mark.parametrize("eta", [1.5, 1 / 1.5, 1.33, 2.4, 1.0001])
+def test_fresnel_dielectric_textbook(eta):
+    for c in np.linspace(0.001, 1.0, 97):
+        got = oracle.fresnel_dielectric(eta, float(c))
+        assert abs(got - _fresnel_dielectric_f64(eta, float(c))) <= 2e-5, (eta, c)
+
+
+def _fresnel_conductor_c128(n, k, c):
+    m = complex(n, k)
+    s2 = 1.0 - c * c
+    root = np.sqrt(m * m - s2)
+    rs = (c - root) / (c + root)
+    rp = (m * m * c - root) / (m * m * c + root)
+    return 0.5 * (abs(rs) ** 2 + abs(rp) ** 2)
+
+
+def test_fresnel_conductor_textbook():
+    eta = np.array([0.2, 1.1, 2.9], np.float32)   # gold-, iron-, silicon-like
+    k = np.array([3.6, 2.9, 0.1], np.float32)
+    for c in np.linspace(0.01, 1.0, 50):
+        got = oracle.fresnel_conductor(eta, k, float(c))
+        ref = [_fresnel_conductor_c128(float(eta[i]), float(k[i]), float(c)) for i in range(3)]
+        assert np.allclose(got, ref, atol=3e-5), (c, got, ref)
+
+
+def test_edge_tint_reproduces_reflectivity():
+    rng = np.random.RandomState(1)
+    for _ in range(50):
+        r = rng.uniform(0.02, 0.98, 3).astype(np.float32)
+        g = rng.uniform(0.0, 1.0, 3).astype(np.float32)
+        n, k = oracle.edge_tint(r, g)
+        f0 = ((n - 1) ** 2 + k ** 2) / ((n + 1) ** 2 + k ** 2)
+        assert np.allclose(f0, r, rtol=2e-4, atol=1e-5), (r, g, f0)
+        # the oracle's conductor Fresnel at normal incidence agrees
+        assert np.allclose(oracle.fresnel_conductor(n, k, 1.0), r, rtol=5e-4, atol=2e-5)
+
+
+# ------------------------------------------------------------------------ GGX
+def _hemisphere_grid(nt=400, npk=256):
+    # theta = (pi/2) x^2 puts samples near the pole, where narrow lobes live
+    x = (np.arange(nt) + 0.5) / nt
+    th = 0.5 * np.pi * x * x
+    dth = np.pi * x / nt
+    ph = (np.arange(npk) + 0.5) / npk * 2 * np.pi
+    dph = 2 * np.pi / npk
+    return th, dth, ph, dph
+
+
+@pytest.mark.parametrize("ax,ay", [(0.09, 0.09), (0.3, 0.1), (0.5, 0.5), (0.02, 0.2)])
+def test_ggx_d_normalised(ax, ay):
+    th, dth, ph, dph = _hemisphere_grid()
+    total = 0.0
+    for t, dt in zip(th, dth):
+        st, ct = math.sin(t), math.cos(t)
+        for p in ph:
+            h = (st * math.cos(p), st * math.sin(p), ct)
+            total += oracle.ggx_d(h, ax, ay) * ct * st * dt * dph
+    assert abs(total - 1.0) < 5e-3, total
+
+
+def _lambert(albedo=(0.6, 0.3, 0.2)):
+    return abi.default_material(diffuse_albedo=albedo, specular_weight=0.0, metalness=0.0, ior_n=1.0)
+
+
+def test_lambert_eval_closed_form():
+    m = _lambert()
+    rng = np.random.RandomState(2)
+    wo = np.array([0.3, -0.2, 0.93]); wo /= np.linalg.norm(wo)
+    for _ in range(200):
+        wi = rng.normal(size=3); wi /= np.linalg.norm(wi)
+        f, pdf = oracle.bsdf_eval(m, wo, wi)
+        if wi[2] > 0:
+            assert np.allclose(f, np.array([0.6, 0.3, 0.2]) * wi[2] / np.pi, rtol=1e-5, atol=1e-7)
+            assert abs(pdf - wi[2] / np.pi) < 1e-6
+        else:
+            assert np.all(f == 0) and pdf == 0
+
+
+MATERIALS = {
+    "lambert": _lambert(),
+    "conductor": abi.default_material(diffuse_albedo=(0.9, 0.6, 0.3), specular_albedo=(1.0, 0.9, 0.8),
+                                      roughness_x=0.3, roughness_y=0.3, metalness=1.0),
+    "plastic": abi.default_material(diffuse_albedo=(0.5, 0.5, 0.5), roughness_x=0.4, roughness_y=0.4,
+                                    ior_n=1.5),
+    "glass": abi.default_material(diffuse_albedo=(1.0, 1.0, 1.0), roughness_x=0.2, roughness_y=0.2,
+                                  transmission_weight=1.0, ior_n=1.5),
+    "coated": abi.default_material(diffuse_albedo=(0.7, 0.2, 0.2), roughness_x=0.5, roughness_y=0.3,
+                                   coat_weight=1.0, coat_roughness_x=0.1, coat_roughness_y=0.1, ior_n=1.4),
+}
+
+
+@pytest.mark.parametrize("name", sorted(MATERIALS))
+def test_sample_weight_matches_eval(name):
+    """The mixture's sampled weight and pdf are eval(f)/eval(pdf) at the
+    sampled direction (src/BSDF.cuh:619-679)."""
+    m = MATERIALS[name]
+    rng = np.random.RandomState(3)
+    checked = 0
+    for _ in range(400):
+        wo = rng.normal(size=3); wo[2] = abs(wo[2]) + 0.05; wo /= np.linalg.norm(wo)
+        ok, w, wi, pdf = oracle.bsdf_sample(m, wo, rng.rand(3))
+        if not ok or pdf <= 1e-4:
+            continue
+        f, epdf = oracle.bsdf_eval(m, wo, wi)
+        assert abs(epdf - pdf) <= 2e-3 * pdf, (epdf, pdf)
+        assert np.allclose(w, f / epdf, rtol=3e-3, atol=1e-6), (w, f / epdf)
+        checked += 1
+    assert checked > 300
+
+
+def test_ggx_sampling_follows_pdf():
+    """Histogram of sampled directions (VNDF sampling, src/BSDF.cuh:158-176)
+    against the eval pdf integrated over the same cells."""
+    m = abi.default_material(diffuse_albedo=(0.9, 0.9, 0.9), specular_albedo=(1.0, 1.0, 1.0),
+                             roughness_x=0.5, roughness_y=0.5, metalness=1.0)
+    wo = np.array([0.5, 0.0, math.sqrt(0.75)])
+    rng = np.random.RandomState(11)
+    nt, nph = 8, 8
+    counts = np.zeros((nt, nph))
+    n = 120000
+    for _ in range(n):
+        ok, w, wi, pdf = oracle.bsdf_sample(m, wo, rng.rand(3))
+        if not ok:
+            continue
+        ct = min(max(wi[2], 0.0), 1.0)
+        it = min(int((1.0 - ct) * nt), nt - 1)
+        ip = int((math.atan2(wi[1], wi[0]) % (2 * np.pi)) / (2 * np.pi) * nph) % nph
+        counts[it, ip] += 1
+    expect = np.zeros((nt, nph))
+    sub = 8
+    for it in range(nt):
+        for ip in range(nph):
+            acc = 0.0
+            for a in range(sub):
+                ct = 1.0 - (it + (a + 0.5) / sub) / nt
+                st = math.sqrt(max(0.0, 1 - ct * ct))
+                for b in range(sub):
+                    p = (ip + (b + 0.5) / sub) / nph * 2 * np.pi
+                    _, pdf = oracle.bsdf_eval(m, wo, (st * math.cos(p), st * math.sin(p), ct))
+                    acc += pdf
+            expect[it, ip] = acc / sub / sub * (1.0 / nt) * (2 * np.pi / nph) * n
+    chi2 = float((((counts - expect) ** 2) / np.maximum(expect, 1.0)).sum())
+    # 64 cells: chi2 has mean 64 and sd 11.3 when the samples follow the pdf
+    assert chi2 < 110, chi2
+    assert abs(counts.sum() - expect.sum()) < 4 * math.sqrt(n)
+
+
+# ------------------------------------------------------------ Henyey-Greenstein
+@pytest.mark.parametrize("g", [0.3, -0.5, 0.8, 0.0])
+def test_hg_eval_moments(g):
+    wo = np.array([0.0, 0.0, 1.0])
+    x = (np.arange(4000) + 0.5) / 4000 * 2 - 1           # mu = cos(wo, wi)
+    tot = mean = 0.0
+    for mu in x:
+        p = oracle.hg_eval(g, wo, (math.sqrt(1 - mu * mu), 0.0, mu))
+        tot += p * 2 * np.pi * (2 / 4000)
+        mean += p * (-mu) * 2 * np.pi * (2 / 4000)       # forward = -wo
+    assert abs(tot - 1.0) < 2e-3
+    assert abs(mean - g) < 3e-3
+
+
+@pytest.mark.parametrize("g", [0.3, -0.6])
+def test_hg_sampler_mirrored_density(g):
+    """The sampler draws cos(wo, wi) with density (1-g^2)/(2(1+g^2-2g mu)^1.5):
+    the eval density mirrored, so E[cos(wo, wi)] = +g."""
+    rng = np.random.RandomState(5)
+    wo = np.array([0.2, 0.3, 0.0]); wo /= np.linalg.norm(wo)
+    mus = []
+    for _ in range(40000):
+        wi, _ = oracle.hg_sample(g, rng.rand(), rng.rand(), wo)
+        mus.append(float(np.dot(wi, wo)))
+    mus = np.array(mus)
+    assert abs(mus.mean() - g) < 0.01
+    hist, edges = np.histogram(mus, bins=10, range=(-1, 1))
+    cdf = lambda mu: (1 - g * g) / (2 * g) * (1 / np.sqrt(1 + g * g - 2 * g * mu) - 1 / (1 + g))
+    expect = (cdf(edges[1:]) - cdf(edges[:-1])) * len(mus)
+    assert np.all(np.abs(hist - expect) <= 5 * np.sqrt(expect) + 5)
+
+
+# ---------------------------------------------------------------- free flight
+def test_homogeneous_free_flight_closed_form():
+    sig = np.array([0.5, 1.0, 2.5], np.float32)
+    alb = np.array([0.8, 0.5, 0.3], np.float32)
+    cw = sig / sig.sum()
+    cdf = np.cumsum(cw)
+    for u in np.linspace(0.01, 0.99, 37):
+        ch = int(np.searchsorted(cdf, u, side="right"))
+        lo = 0.0 if ch == 0 else cdf[ch - 1]
+        uu = (u - lo) / cw[ch]
+        d_ref = -math.log(1 - uu) / sig[ch]
+        tmax = 10.0
+        scat, dist, w = oracle.free_flight(sig, alb, tmax, float(u))
+        assert abs(dist - d_ref) <= 1e-4 * max(1.0, d_ref)
+        assert scat == (d_ref < tmax)
+        tr = np.exp(-sig.astype(np.float64) * min(d_ref, tmax))
+        if scat:
+            expect = tr * sig * alb / np.dot(cw, sig * tr)
+        else:
+            expect = tr / np.dot(cw, tr)
+        assert np.allclose(w, expect, rtol=2e-4), (u, w, expect)
+    # gray medium: a scatter carries exactly the albedo, a pass-through weight 1
+    scat, dist, w = oracle.free_flight([1, 1, 1], [0.7, 0.7, 0.7], 100.0, 0.2)
+    assert scat and np.allclose(w, 0.7, rtol=1e-5)
+    scat, dist, w = oracle.free_flight([1, 1, 1], [0.7, 0.7, 0.7], 0.1, 0.9)
+    assert not scat and np.allclose(w, 1.0, rtol=1e-5)
+
+
+# ----------------------------------------------------------------- textures
+def _texture_scene(texels, wrap, cs):
+    sa = abi.SceneArrays()
+    sa.add_texture(texels, wrap=wrap, color_space=cs)
+    sa.add_instance(sa.add_mesh(*scenes.quad((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0))[:2]),
+                    sa.add_material(abi.default_material()))
+    sa.set_camera((0.5, 0.5, 2.0), (0.5, 0.5, 0.0), 40.0, 1.0)
+    return oracle.Scene(sa)
+
+
+@pytest.mark.parametrize("wrap,cs", [(abi.WRAP_REPEAT, abi.COLOR_LINEAR), (abi.WRAP_CLAMP, abi.COLOR_LINEAR),
+                                     (abi.WRAP_REPEAT, abi.COLOR_SRGB)])
+def test_bilinear_ramp(wrap, cs):
+    w = 8
+    row = (np.arange(w) * 30).astype(np.uint8)
+    tex = np.tile(row, (2, 1))
+    sc = _texture_scene(tex, wrap, cs)
+    vals = row.astype(np.float64) / 255.0
+    if cs == abi.COLOR_SRGB:
+        vals = np.where(vals <= 0.04045, vals / 12.92, ((vals + 0.055) / 1.055) ** 2.4)
+    for u in np.linspace(0.0, 0.999, 41):
+        x = u * w - 0.5
+        x0 = math.floor(x)
+        fx = x - x0
+        if wrap == abi.WRAP_REPEAT:
+            a, b = vals[x0 % w], vals[(x0 + 1) % w]
+        else:
+            a, b = vals[min(max(x0, 0), w - 1)], vals[min(max(x0 + 1, 0), w - 1)]
+        ref = (1 - fx) * a + fx * b
+        got = sc.texture_eval(0, float(u), 0.5)
+        assert abs(got[0] - ref) < 2e-6, (u, got[0], ref)
+    sc.close()
+
+
+# ------------------------------------------------------------------ PCG4d
+def _pcg4d_published(v):
+    v = v.astype(np.uint64)
+    m = np.uint64(0xFFFFFFFF)
+    v = (v * np.uint64(1664525) + np.uint64(1013904223)) & m
+    x, y, z, w = v
+    x = (x + y * w) & m; y = (y + z * x) & m; z = (z + x * y) & m; w = (w + y * z) & m
+    x ^= x >> np.uint64(16); y ^= y >> np.uint64(16); z ^= z >> np.uint64(16); w ^= w >> np.uint64(16)
+    x = (x + y * w) & m; y = (y + z * x) & m; z = (z + x * y) & m; w = (w + y * z) & m
+    return np.array([x, y, z, w], np.uint32)
+
+
+def test_pcg4d_published():
+    rng = np.random.RandomState(6)
+    for _ in range(500):
+        v = rng.randint(0, 2 ** 32, size=4, dtype=np.uint64).astype(np.uint32)
+        assert np.array_equal(oracle.pcg4d(v), _pcg4d_published(v))
+
+
+# --------------------------------------------------------- accumulate / tonemap
+def test_tonemap_aces_fit_and_accumulate():
+    rng = np.random.RandomState(7)
+    img = (rng.rand(5, 7, 4) * 4).astype(np.float32)
+    out = img.copy()
+    oracle.tonemap(out, 2.2, 0.8)
+    x = img[..., :3].astype(np.float64) * 0.8
+    y = (x * (2.51 * x + 0.03)) / (x * (2.43 * x + 0.59) + 0.14)
+    assert np.allclose(out[..., :3], y ** (1 / 2.2), rtol=3e-6, atol=1e-7)
+    assert np.all(out[..., 3] == 1.0)
+    frames = [(rng.rand(3, 4, 4) * 2).astype(np.float32) for _ in range(5)]
+    acc = np.zeros((3, 4, 4), np.float32)
+    for f, n in enumerate(frames, start=1):
+        oracle.accumulate(n, acc, f)
+    assert np.allclose(acc, np.mean(frames, axis=0), rtol=1e-5)
+
+
+# --------------------------------------------------- analytic direct lighting
+def _parallel_rect_factor(a, b, h):
+    """Form factor from a differential area to a parallel a x b rectangle at
+    height h whose corner is above it."""
+    x, y = a / h, b / h
+    return (x / math.sqrt(1 + x * x) * math.atan(y / math.sqrt(1 + x * x))
+            + y / math.sqrt(1 + y * y) * math.atan(x / math.sqrt(1 + y * y))) / (2 * math.pi)
+
+
+def test_direct_light_under_square_emitter():
+    """Lambert floor (albedo 0.5) under a 2x2 emitter of radiance 1 at height 1:
+    L = albedo/pi * E, E = pi * 4 * F(1, 1, 1).  Light sampling, the BSDF
+    sample tested against the chosen triangle, power-heuristic MIS and the
+    light-selection pdf all enter; depth 1 keeps only direct light."""
+    sa = abi.SceneArrays()
+    floor = sa.add_mesh(*scenes.quad((-50, 0, -50), (-50, 0, 50), (50, 0, 50), (50, 0, -50))[:2])
+    sa.add_instance(floor, sa.add_material(_lambert((0.5, 0.5, 0.5))))
+    lq = sa.add_mesh(*scenes.quad((-1, 1, -1), (1, 1, -1), (1, 1, 1), (-1, 1, 1))[:2])
+    sa.add_instance(lq, sa.add_material(abi.light_material((1.0, 1.0, 1.0), 1.0)))
+    sa.set_camera((0.0, 0.5, 0.0001), (0.0, 0.0, 0.0), 0.2, 1.0)
+    rs = abi.render_settings(8, 8, 512, 1, sampler=abi.SAMPLER_PCG)
+    sc = oracle.Scene(sa)
+    img = sc.render(rs, 1, threads=8)
+    sc.close()
+    expect = 0.5 / math.pi * (math.pi * 4 * _parallel_rect_factor(1.0, 1.0, 1.0))
+    got = float(img[..., :3].mean())
+    assert abs(got - expect) < 0.01 * expect, (got, expect)
+    # a camera ray that sees the emitter directly returns its radiance exactly
+    sa.set_camera((0.0, 0.5, 0.0001), (0.0, 2.0, 0.0), 0.2, 1.0)
+    sc = oracle.Scene(sa)
+    img = sc.render(abi.render_settings(4, 4, 1, 1, sampler=abi.SAMPLER_PCG), 1)
+    sc.close()
+    assert np.allclose(img[..., :3], 1.0)
+
+
+# ------------------------------------------------ grid media vs homogeneous
+def _medium_box_scene(grid):
+    sa = scenes.cornell_box(24, 24)
+    lo, hi = np.array((0.2, 0.05, 0.2)), np.array((0.8, 0.65, 0.8))
+    w2o = np.eye(4, dtype=np.float32)
+    w2o[:3, :3] = np.diag(1.0 / (hi - lo))
+    w2o[:3, 3] = -lo / (hi - lo)
+    if grid:
+        g = sa.add_grid(np.ones((4, 4, 4), np.float32))
+        vol = sa.add_volume((1.5, 1.5, 1.5), (0.9, 0.7, 0.5), 0.2, grid=g, world_to_object=w2o)
+    else:
+        vol = sa.add_volume((1.5, 1.5, 1.5), (0.9, 0.7, 0.5), 0.2, world_to_object=w2o)
+    sa.add_instance(sa.add_mesh(*scenes.box(tuple(lo), tuple(hi))[:2]), sa.add_material(abi.default_material(ior_n=1.0)),
+                    inner_volume=vol, boundary=True)
+    return sa
+
+
+def test_constant_grid_equals_homogeneous():
+    """Delta tracking through a grid of constant density 1 (majorant 1) is the
+    homogeneous medium in distribution: free-flight distances, scatter weights
+    and shadow transmittance agree in expectation (images within MC error)."""
+    rs = abi.render_settings(24, 24, 256, 4, sampler=abi.SAMPLER_PCG, volume=1)
+    blocks = {}
+    for grid in (False, True):
+        sc = oracle.Scene(_medium_box_scene(grid))
+        for frame in (1, 2):
+            img = sc.render(rs, frame, threads=8)[..., :3].astype(np.float64)
+            blocks[grid, frame] = img.reshape(4, 6, 4, 6, 3).mean(axis=(1, 3))   # 6x6-pixel blocks
+        sc.close()
+    dist = lambda a, b: float((np.abs(a - b) / np.maximum(a, 1e-3)).mean())
+    noise = max(dist(blocks[False, 1], blocks[False, 2]), dist(blocks[True, 1], blocks[True, 2]))
+    # the two media differ by no more than two independent frames of one medium
+    assert dist(blocks[False, 1], blocks[True, 1]) < 1.5 * noise
+    assert dist(blocks[False, 2], blocks[True, 2]) < 1.5 * noise
+    mh = blocks[False, 1].mean() + blocks[False, 2].mean()
+    mg = blocks[True, 1].mean() + blocks[True, 2].mean()
+    assert abs(mh - mg) < 0.01 * mh

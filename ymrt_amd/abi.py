@@ -129,6 +129,9 @@ class FrameStats(C.Structure):
         ("ms_shadow", C.c_float), ("ms_resolve", C.c_float),
         ("launches_extend", C.c_uint32), ("launches_shade", C.c_uint32),
         ("launches_shadow", C.c_uint32), ("ms_total", C.c_float),
+        ("ms_medium", C.c_float), ("launches_medium", C.c_uint32),
+        ("extend_nodes", C.c_uint64), ("extend_tris", C.c_uint64),
+        ("shadow_nodes", C.c_uint64), ("shadow_tris", C.c_uint64),
     ]
 
 

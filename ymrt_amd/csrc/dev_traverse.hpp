@@ -108,15 +108,48 @@ struct Hit {
     int back;
 };
 
-// Per-lane LDS scratch for one traversal: the stack and the nearby list.
-template <int kBlock>
+// Per-lane scratch for one traversal: the stack and the nearby list, in LDS
+// laid out [slot][lane].  With kWindow > 0 only the top kWindow stack entries
+// live in LDS (a ring indexed by depth); an entry pushed deeper than that
+// evicts the entry kWindow below it to a global spill area laid out
+// [depth - kWindow][thread], and popping restores it.  Most traversals never
+// leave the window, so the spill area costs nothing but its allocation while
+// the small window lets more waves share a CU's LDS.
+template <int kBlock, int kWindow = 0>
 struct LaneScratch {
-    uint32_t *stack;        // [kStack][kBlock]
+    static constexpr int kLanes = kBlock;
+    uint32_t *stack;        // [kWindow or kStack][kBlock]
     uint32_t *nb_prim;      // [kNearbyCap][kBlock]
     float *nb_t;
     int *nb_back;
     int lane;
-    YM_HD uint32_t &st(int i) { return stack[i * kBlock + lane]; }
+    uint32_t *spill;        // global [kMaxSpill][spill_stride], unused when kWindow == 0
+    uint32_t spill_stride, spill_index;
+
+    YM_HD int nbi(int i) const { return i * kBlock + lane; }
+    YM_HD void push(int sp, uint32_t v)
+    {
+        if constexpr (kWindow == 0) {
+            stack[sp * kBlock + lane] = v;
+        } else {
+            static_assert((kWindow & (kWindow - 1)) == 0, "window must be a power of two");
+            const int w = (sp & (kWindow - 1)) * kBlock + lane;
+            if (sp >= kWindow) spill[(size_t)(sp - kWindow) * spill_stride + spill_index] = stack[w];
+            stack[w] = v;
+        }
+    }
+    // sp is the depth of the entry being popped (already decremented)
+    YM_HD uint32_t pop(int sp)
+    {
+        if constexpr (kWindow == 0) {
+            return stack[sp * kBlock + lane];
+        } else {
+            const int w = (sp & (kWindow - 1)) * kBlock + lane;
+            const uint32_t v = stack[w];
+            if (sp >= kWindow) stack[w] = spill[(size_t)(sp - kWindow) * spill_stride + spill_index];
+            return v;
+        }
+    }
 };
 
 struct TraceCounters {
@@ -132,9 +165,8 @@ struct TraceCounters {
     } while (0)
 
 // Returns 1 on hit, 0 on miss, -1 on stack overflow (the ray is abandoned).
-template <int kMode, bool kAny, bool kNearby, bool kCount, int kBlock, int kStack>
-YM_HD int traverse(const DevScene &sc, const Query &q, Hit &hit, int &nb_count,
-                                        LaneScratch<kBlock> &ls, TraceCounters &cnt)
+template <int kMode, bool kAny, bool kNearby, bool kCount, int kStack, typename LS>
+YM_HD int traverse(const DevScene &sc, const Query &q, Hit &hit, int &nb_count, LS &ls, TraceCounters &cnt)
 {
     const uint32_t root = kMode == kModeAll ? sc.tlas_root
                           : kMode == kModeSkipBoundary ? sc.tlas_root_solid
@@ -193,9 +225,9 @@ YM_HD int traverse(const DevScene &sc, const Query &q, Hit &hit, int &nb_count,
             if (k0 != INFINITY) {
                 int need = (k1 != INFINITY) + (k2 != INFINITY) + (k3 != INFINITY);
                 if (sp + need > kStack) return -1;
-                if (k3 != INFINITY) ls.st(sp++) = v3;
-                if (k2 != INFINITY) ls.st(sp++) = v2;
-                if (k1 != INFINITY) ls.st(sp++) = v1;
+                if (k3 != INFINITY) ls.push(sp++, v3);
+                if (k2 != INFINITY) ls.push(sp++, v2);
+                if (k1 != INFINITY) ls.push(sp++, v1);
                 node = v0;
                 continue;
             }
@@ -208,7 +240,7 @@ YM_HD int traverse(const DevScene &sc, const Query &q, Hit &hit, int &nb_count,
             if (kMode == kModeBoundaryOnly) take = (in.flags & kInstBoundary) != 0;
             if (take) {
                 if (sp + 1 > kStack) return -1;
-                ls.st(sp++) = kSentinel;
+                ls.push(sp++, kSentinel);
                 pre = make_pre(xf_point(in.w2o, q.o), xf_dir(in.w2o, q.d));
                 in_blas = true;
                 cur_inst = ii;
@@ -255,7 +287,7 @@ YM_HD int traverse(const DevScene &sc, const Query &q, Hit &hit, int &nb_count,
             if (sp == 0) {
                 return found ? 1 : 0;
             }
-            node = ls.st(--sp);
+            node = ls.pop(--sp);
             if (node != kSentinel) break;
             // leaving an instance: back to world space
             pre = make_pre(q.o, q.d);
@@ -266,26 +298,26 @@ YM_HD int traverse(const DevScene &sc, const Query &q, Hit &hit, int &nb_count,
                     tlb = next_float_down(next_float_down(tbest));
                     int w = 0;
                     for (int i = 0; i < nbn; ++i) {
-                        const float ti = ls.nb_t[i * kBlock + ls.lane];
+                        const float ti = ls.nb_t[ls.nbi(i)];
                         if (ti > tlb && ti < tub) {
-                            ls.nb_t[w * kBlock + ls.lane] = ti;
-                            ls.nb_prim[w * kBlock + ls.lane] = ls.nb_prim[i * kBlock + ls.lane];
-                            ls.nb_back[w * kBlock + ls.lane] = ls.nb_back[i * kBlock + ls.lane];
+                            ls.nb_t[ls.nbi(w)] = ti;
+                            ls.nb_prim[ls.nbi(w)] = ls.nb_prim[ls.nbi(i)];
+                            ls.nb_back[ls.nbi(w)] = ls.nb_back[ls.nbi(i)];
                             ++w;
                         }
                     }
                     nbn = w;
                     if (nbn < kNearbyCap) {
-                        ls.nb_prim[nbn * kBlock + ls.lane] = cur_inst;
-                        ls.nb_t[nbn * kBlock + ls.lane] = tbest;
-                        ls.nb_back[nbn * kBlock + ls.lane] = hit.back;
+                        ls.nb_prim[ls.nbi(nbn)] = cur_inst;
+                        ls.nb_t[ls.nbi(nbn)] = tbest;
+                        ls.nb_back[ls.nbi(nbn)] = hit.back;
                         ++nbn;
                     }
                 } else if (found && ex_hit && ex_t > tlb && ex_t < tub) {
                     if (nbn < kNearbyCap) {
-                        ls.nb_prim[nbn * kBlock + ls.lane] = cur_inst;
-                        ls.nb_t[nbn * kBlock + ls.lane] = ex_t;
-                        ls.nb_back[nbn * kBlock + ls.lane] = ex_back;
+                        ls.nb_prim[ls.nbi(nbn)] = cur_inst;
+                        ls.nb_t[ls.nbi(nbn)] = ex_t;
+                        ls.nb_back[ls.nbi(nbn)] = ex_back;
                         ++nbn;
                     }
                 }

@@ -201,36 +201,40 @@ __global__ __launch_bounds__(kBlock) void k_raygen(DevScene sc, PtParams prm, Pa
 }
 
 // ------------------------------------------------------------ extend
-template <bool kNearby, int kStack>
+// Closest hit for every queued path (Ray_trace_path_tracing,
+// src/Shading.cu:1765-1901).  With kMedia, paths that carry medium records
+// hand their hit or miss to k_medium, which samples free flight; the others
+// resolve misses here and queue surface hits for shading.  Visited wide nodes
+// and tested triangles are summed per wave into the frame statistics.
+template <bool kNearby, bool kMedia, int kStack>
 __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
 {
-    __shared__ uint32_t stack[kStack * kBlock];
-    __shared__ uint32_t nbp[kNearbyCap * kBlock];
-    __shared__ float nbt[kNearbyCap * kBlock];
-    __shared__ int nbb[kNearbyCap * kBlock];
+    __shared__ uint32_t stack[kStackWindow * kBlock];
+    __shared__ uint32_t nbp[(kNearby ? kNearbyCap : 1) * kBlock];
+    __shared__ float nbt[(kNearby ? kNearbyCap : 1) * kBlock];
+    __shared__ int nbb[(kNearby ? kNearbyCap : 1) * kBlock];
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     const uint32_t *qin = ps.queue[pass & 1];
     const uint32_t n = cnt[kCtrActive];
-    LaneScratch<kBlock> ls{stack, nbp, nbt, nbb, (int)lane_id()};
+    LaneScratch<kBlock, kStackWindow> ls{stack, nbp, nbt, nbb, (int)lane_id(), ps.spill, kSpillThreads,
+                                         blockIdx.x * kBlock + threadIdx.x};
     const ymrt_render_settings &rs = prm.rs;
     for (;;) {
         const uint32_t base = wave_fetch(cnt + kCtrWorkExtend);
         if (base >= n) break;
         const uint32_t i = base + lane_id();
-        bool want_hit = false, want_vol = false;
+        bool to_hit = false, to_med = false;
         uint32_t p = 0;
         uint32_t rays = 0, cam_rays = 0;
+        TraceCounters tc{0, 0};
         if (i < n) {
             p = qin[i];
             uint32_t info = ps.info[p];
-            Transfer t;
-            bool t_loaded = false;
             if (info & kInfoPendingBunch) {
-                t = load_transfer(ps, p, prm.cap);
+                Transfer t = load_transfer(ps, p, prm.cap);
                 const NearbyList nb = load_nearby(ps, p, prm.cap, ps.hit_a[p].w);
                 boundary_bunch(sc, t, nb);
                 store_transfer(ps, p, prm.cap, t);
-                t_loaded = true;
                 info &= ~kInfoPendingBunch;
                 ps.info[p] = info;
             }
@@ -245,74 +249,122 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, PtParams prm, Pa
             q.last_tri = last.y;
             Hit h;
             h.back = 0;
+            h.inst = kEmpty;
+            h.tri = 0;
+            h.b0 = h.b1 = h.b2 = 0.0f;
             int nbc = 0;
-            TraceCounters tc{0, 0};
-            int r = traverse<kModeAll, false, kNearby, false, kBlock, kStack>(sc, q, h, nbc, ls, tc);
+            int r = traverse<kModeAll, false, kNearby, true, kStack>(sc, q, h, nbc, ls, tc);
             if (r < 0) {
                 atomicOr(prm.error, 1u);
                 r = 0;
             }
             rays = 1;
             cam_rays = (info_depth(info) == 1 && (info & kInfoNeverScatter)) ? 1u : 0u;
-            bool hit = r == 1;
-            if (!hit) h.t = kTMax;
-            float4 beta4 = ps.beta[p];
-            f3 beta = xyz(beta4);
-            bool vol_hit = false;
-            f3 vpos = s3(0.0f);
-            if (rs.enable_volume_scattering && sc.volume_count > 0) {
-                if (!t_loaded) t = load_transfer(ps, p, prm.cap);
-                int vi[kMaxBoundaryRecord];
-                const int nv = t.volumes(vi, sc.volume_count);
-                if (nv > 0) {
-                    Sampler smp = load_sampler(ps, p);
-                    bool het = false;
-#pragma unroll
-                    for (int k = 0; k < kMaxBoundaryRecord; ++k)
-                        if (k < nv && vol_is_het(sc, vi[k])) het = true;
-                    f3 trw;
-                    float dist;
-                    const float tmax = hit ? h.t : kTMax;
-                    vol_hit = het ? sample_heterogeneous(sc, nv, vi, q.o, q.d, tmax, trw, dist, smp)
-                                  : sample_homogeneous(sc, nv, vi, tmax, trw, dist, smp);
-                    beta = beta * trw;
-                    ps.beta[p] = f4(beta, 0.0f);
-                    store_sampler(ps, p, smp);
-                    if (vol_hit) vpos = q.o + q.d * dist;
-                }
-            }
-            if (!hit && !vol_hit) {
-                float4 L = ps.L[p];
-                f3 Lv = xyz(L);
-                if (rs.enable_env_light) Lv = Lv + light_clamp(rs, beta * background(q.d), info_depth(info));
+            const bool hit = r == 1;
+            const bool in_medium = kMedia && (ps.xhdr[p] & 0xFFu) != 0;
+            if (!hit && !in_medium) {
+                f3 Lv = xyz(ps.L[p]);
+                if (rs.enable_env_light)
+                    Lv = Lv + light_clamp(rs, xyz(ps.beta[p]) * background(q.d), info_depth(info));
                 ps.out[p] = f4(Lv, 0.0f);
             } else {
-                want_hit = true;
-                want_vol = vol_hit;
-                uint32_t flags = (vol_hit ? kHitVolume : 0u) | (h.back ? kHitBack : 0u);
-                if (kNearby && hit) {
-                    flags |= (uint32_t)nbc << 4;
-#pragma unroll
-                    for (int k = 0; k < kNearbyCap; ++k)
-                        if (k < nbc)
-                            ps.nb[(size_t)k * prm.cap + p] =
-                            make_uint2(nbp[k * kBlock + lane_id()] | ((uint32_t)(nbb[k * kBlock + lane_id()] != 0) << 31),
-                                       __float_as_uint(nbt[k * kBlock + lane_id()]));
-                } else if (hit) {
+                to_med = in_medium;
+                to_hit = !in_medium;
+                uint32_t flags = h.back ? kHitBack : 0u;
+                bool listed = false;
+                if constexpr (kNearby) {
+                    if (hit) {
+                        listed = true;
+                        flags |= (uint32_t)nbc << 4;
+                        static_for<0, kNearbyCap>([&](auto K) {
+                            if (K() < nbc)
+                                ps.nb[(size_t)K() * prm.cap + p] =
+                                    make_uint2(nbp[K() * kBlock + lane_id()] |
+                                                   ((uint32_t)(nbb[K() * kBlock + lane_id()] != 0) << 31),
+                                               __float_as_uint(nbt[K() * kBlock + lane_id()]));
+                        });
+                    }
+                }
+                if (!listed && hit) {
                     // without the list the closest instance is its own nearby hit
                     flags |= 1u << 4;
                     ps.nb[p] = make_uint2(h.inst | ((uint32_t)(h.back != 0) << 31), __float_as_uint(h.t));
                 }
-                ps.hit_a[p] = make_uint4(__float_as_uint(h.t), hit ? h.inst : kEmpty, h.tri, flags);
-                ps.hit_b[p] = vol_hit ? make_float4(vpos.x, vpos.y, vpos.z, 0.0f) : make_float4(h.b0, h.b1, h.b2, 0.0f);
+                ps.hit_a[p] = make_uint4(__float_as_uint(hit ? h.t : kTMax), hit ? h.inst : kEmpty, h.tri, flags);
+                ps.hit_b[p] = make_float4(h.b0, h.b1, h.b2, 0.0f);
             }
         }
-        const uint32_t slot = wave_append(cnt + kCtrHit, want_hit && !want_vol);
-        if (want_hit && !want_vol) ps.q_hit[slot] = p;
-        const uint32_t vslot = wave_append(cnt + kCtrHitVol, want_vol);
-        if (want_vol) ps.q_hit_vol[vslot] = p;
+        const uint32_t slot = wave_append(cnt + kCtrHit, to_hit);
+        if (to_hit) ps.q_hit[slot] = p;
+        if (kMedia) {
+            const uint32_t mslot = wave_append(cnt + kCtrMedium, to_med);
+            if (to_med) ps.q_medium[mslot] = p;
+        }
         wave_add64(prm.stats + kStatExtension, rays);
         wave_add64(prm.stats + kStatCamera, cam_rays);
+        wave_add64(prm.stats + kStatExtendNodes, tc.nodes);
+        wave_add64(prm.stats + kStatExtendTris, tc.tris);
+    }
+}
+
+// ------------------------------------------------------------ medium
+// Free flight through the media of the path's top-priority records
+// (SampleVolumeScattering, src/VolumeEval.cuh:217-251, called from
+// src/Shading.cu:1800-1840): scattering vertices go to the volume shading
+// queue, surviving rays to surface shading or, on a miss, to the environment.
+__global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
+{
+    uint32_t *cnt = prm.counters + pass * kCtrStride;
+    const uint32_t n = cnt[kCtrMedium];
+    const ymrt_render_settings &rs = prm.rs;
+    for (;;) {
+        const uint32_t base = wave_fetch(cnt + kCtrWorkMedium);
+        if (base >= n) break;
+        const uint32_t i = base + lane_id();
+        bool to_hit = false, to_vol = false;
+        uint32_t p = 0;
+        if (i < n) {
+            p = ps.q_medium[i];
+            const uint4 ha = ps.hit_a[p];
+            const bool hit = ha.y != kEmpty;
+            const f3 o = xyz(ps.ray_o[p]), d = xyz(ps.ray_d[p]);
+            f3 beta = xyz(ps.beta[p]);
+            bool vol_hit = false;
+            float dist = 0.0f;
+            const Transfer t = load_transfer(ps, p, prm.cap);
+            int vi[kMaxBoundaryRecord];
+            const int nv = t.volumes(vi, sc.volume_count);
+            if (nv > 0) {
+                Sampler smp = load_sampler(ps, p);
+                bool het = false;
+                static_for<0, kMaxBoundaryRecord>([&](auto K) {
+                    if (K() < nv && vol_is_het(sc, vi[K()])) het = true;
+                });
+                f3 trw;
+                const float tmax = hit ? __uint_as_float(ha.x) : kTMax;
+                vol_hit = het ? sample_heterogeneous(sc, nv, vi, o, d, tmax, trw, dist, smp)
+                              : sample_homogeneous(sc, nv, vi, tmax, trw, dist, smp);
+                beta = beta * trw;
+                ps.beta[p] = f4(beta, 0.0f);
+                store_sampler(ps, p, smp);
+            }
+            if (vol_hit) {
+                to_vol = true;
+                const f3 vpos = o + d * dist;
+                ps.hit_a[p] = make_uint4(ha.x, ha.y, ha.z, ha.w | kHitVolume);
+                ps.hit_b[p] = make_float4(vpos.x, vpos.y, vpos.z, 0.0f);
+            } else if (hit) {
+                to_hit = true;
+            } else {
+                f3 Lv = xyz(ps.L[p]);
+                if (rs.enable_env_light) Lv = Lv + light_clamp(rs, beta * background(d), info_depth(ps.info[p]));
+                ps.out[p] = f4(Lv, 0.0f);
+            }
+        }
+        const uint32_t slot = wave_append(cnt + kCtrHit, to_hit);
+        if (to_hit) ps.q_hit[slot] = p;
+        const uint32_t vslot = wave_append(cnt + kCtrHitVol, to_vol);
+        if (to_vol) ps.q_hit_vol[vslot] = p;
     }
 }
 
@@ -623,19 +675,21 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
 template <bool kMedia, int kStack>
 __global__ __launch_bounds__(kBlock) void k_shadow(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
 {
-    __shared__ uint32_t stack[kStack * kBlock];
-    __shared__ uint32_t nbp[kNearbyCap * kBlock];
-    __shared__ float nbt[kNearbyCap * kBlock];
-    __shared__ int nbb[kNearbyCap * kBlock];
+    __shared__ uint32_t stack[kStackWindow * kBlock];
+    __shared__ uint32_t nbp[(kMedia ? kNearbyCap : 1) * kBlock];
+    __shared__ float nbt[(kMedia ? kNearbyCap : 1) * kBlock];
+    __shared__ int nbb[(kMedia ? kNearbyCap : 1) * kBlock];
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     const uint32_t n = cnt[kCtrShadow];
     const uint32_t cap = prm.cap;
-    LaneScratch<kBlock> ls{stack, nbp, nbt, nbb, (int)lane_id()};
+    LaneScratch<kBlock, kStackWindow> ls{stack, nbp, nbt, nbb, (int)lane_id(), ps.spill, kSpillThreads,
+                                         blockIdx.x * kBlock + threadIdx.x};
     for (;;) {
         const uint32_t base = wave_fetch(cnt + kCtrWorkShadow);
         if (base >= n) break;
         const uint32_t i = base + lane_id();
         uint32_t nsh = 0, ntr = 0;
+        TraceCounters tc{0, 0};
         if (i < n) {
             const uint32_t p = ps.q_shadow[i];
             const uint32_t fl = ps.sh_flags[p];
@@ -675,8 +729,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene sc, PtParams prm, Pa
                 q.last_tri = kEmpty;
                 Hit h;
                 int nbc = 0;
-                TraceCounters tc{0, 0};
-                int r = traverse<kModeSkipBoundary, true, false, false, kBlock, kStack>(sc, q, h, nbc, ls, tc);
+                int r = traverse<kModeSkipBoundary, true, false, true, kStack>(sc, q, h, nbc, ls, tc);
                 ++nsh;
                 if (r < 0) {
                     atomicOr(prm.error, 1u);
@@ -684,60 +737,62 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene sc, PtParams prm, Pa
                 }
                 if (r == 1) continue;
                 f3 tr = s3(1.0f);
-                if (kMedia && prm.rs.enable_volume_scattering) {
-                    // TraceTr (src/VolumeEval.cuh:420-468)
-                    Transfer t = t0;
-                    if (fl & (4u << k)) boundary_bunch(sc, t, nb0);
-                    const bool volv = (fl & kShVolume) != 0;
-                    Query tq = q;
-                    tq.last_inst = volv ? kEmpty : ha.y;
-                    tq.last_tri = volv ? kEmpty : ha.z;
-                    const float maxd = o4.w;
-                    float traced = kTMin;
-                    int guard = 0;
-                    while (traced < maxd && sc.volume_count > 0) {
-                        Hit bh;
-                        bh.back = 0;
-                        int bnc = 0;
-                        int br = 0;
-                        ++ntr;
-                        if (sc.has_boundaries)
-                            br = traverse<kModeBoundaryOnly, false, true, false, kBlock, kStack>(sc, tq, bh, bnc, ls, tc);
-                        if (br < 0) {
-                            atomicOr(prm.error, 1u);
-                            break;
+                if constexpr (kMedia) {
+                    if (prm.rs.enable_volume_scattering) {
+                        // TraceTr (src/VolumeEval.cuh:420-468)
+                        Transfer t = t0;
+                        if (fl & (4u << k)) boundary_bunch(sc, t, nb0);
+                        const bool volv = (fl & kShVolume) != 0;
+                        Query tq = q;
+                        tq.last_inst = volv ? kEmpty : ha.y;
+                        tq.last_tri = volv ? kEmpty : ha.z;
+                        const float maxd = o4.w;
+                        float traced = kTMin;
+                        int guard = 0;
+                        while (traced < maxd && sc.volume_count > 0) {
+                            Hit bh;
+                            bh.back = 0;
+                            int bnc = 0;
+                            int br = 0;
+                            ++ntr;
+                            if (sc.has_boundaries)
+                                br = traverse<kModeBoundaryOnly, false, true, true, kStack>(sc, tq, bh, bnc, ls, tc);
+                            if (br < 0) {
+                                atomicOr(prm.error, 1u);
+                                break;
+                            }
+                            if (br == 0) {
+                                tr = tr * eval_tr(sc, tq.o, tq.d, t, gmax(maxd - traced, 0.0f), rng);
+                                break;
+                            }
+                            tr = tr * eval_tr(sc, tq.o, tq.d, t, bh.t, rng);
+                            traced += bh.t;
+                            const DevInstance &bin = sc.instances[bh.inst];
+                            const ShadeTri &st = sc.shade[bin.shade_base + bh.tri];
+                            const f3 p0{st.p[0], st.p[1], st.p[2]}, p1{st.p[3], st.p[4], st.p[5]},
+                                p2{st.p[6], st.p[7], st.p[8]};
+                            const float xs = fabsf(p0.x * bh.b0) + fabsf(p1.x * bh.b1) + fabsf(p2.x * bh.b2);
+                            const float ys = fabsf(p0.y * bh.b0) + fabsf(p1.y * bh.b1) + fabsf(p2.y * bh.b2);
+                            const float zs = fabsf(p0.z * bh.b0) + fabsf(p1.z * bh.b1) + fabsf(p2.z * bh.b2);
+                            f3 bp = p0 * bh.b0 + p1 * bh.b1 + p2 * bh.b2;
+                            f3 bperr = error_gamma(7) * f3{xs, ys, zs};
+                            f3 bng = cross(p1 - p0, p2 - p0);
+                            bp = xf_position(bin.o2w, bp, &bperr);
+                            bng = normalize(xf_transpose_vector(bin.w2o, bng));
+                            NearbyList bnb;
+                            bnb.n = bnc;
+                            static_for<0, kNearbyMax>([&](auto J) {
+                                bnb.prim[J()] = nbp[J() * kBlock + lane_id()];
+                                bnb.t[J()] = nbt[J() * kBlock + lane_id()];
+                                bnb.back[J()] = nbb[J() * kBlock + lane_id()];
+                            });
+                            boundary_bunch(sc, t, bnb);
+                            tq.o = offset_ray_origin(bp, bperr, tq.d, bng);
+                            tq.tmax = kTMax;
+                            tq.last_inst = bh.inst;
+                            tq.last_tri = bh.tri;
+                            if (++guard > 64) break;
                         }
-                        if (br == 0) {
-                            tr = tr * eval_tr(sc, tq.o, tq.d, t, gmax(maxd - traced, 0.0f), rng);
-                            break;
-                        }
-                        tr = tr * eval_tr(sc, tq.o, tq.d, t, bh.t, rng);
-                        traced += bh.t;
-                        const DevInstance &bin = sc.instances[bh.inst];
-                        const ShadeTri &st = sc.shade[bin.shade_base + bh.tri];
-                        const f3 p0{st.p[0], st.p[1], st.p[2]}, p1{st.p[3], st.p[4], st.p[5]},
-                            p2{st.p[6], st.p[7], st.p[8]};
-                        const float xs = fabsf(p0.x * bh.b0) + fabsf(p1.x * bh.b1) + fabsf(p2.x * bh.b2);
-                        const float ys = fabsf(p0.y * bh.b0) + fabsf(p1.y * bh.b1) + fabsf(p2.y * bh.b2);
-                        const float zs = fabsf(p0.z * bh.b0) + fabsf(p1.z * bh.b1) + fabsf(p2.z * bh.b2);
-                        f3 bp = p0 * bh.b0 + p1 * bh.b1 + p2 * bh.b2;
-                        f3 bperr = error_gamma(7) * f3{xs, ys, zs};
-                        f3 bng = cross(p1 - p0, p2 - p0);
-                        bp = xf_position(bin.o2w, bp, &bperr);
-                        bng = normalize(xf_transpose_vector(bin.w2o, bng));
-                        NearbyList bnb;
-                        bnb.n = bnc;
-                        for (int j = 0; j < bnc; ++j) {
-                            bnb.prim[j] = nbp[j * kBlock + lane_id()];
-                            bnb.t[j] = nbt[j * kBlock + lane_id()];
-                            bnb.back[j] = nbb[j * kBlock + lane_id()];
-                        }
-                        boundary_bunch(sc, t, bnb);
-                        tq.o = offset_ray_origin(bp, bperr, tq.d, bng);
-                        tq.tmax = kTMax;
-                        tq.last_inst = bh.inst;
-                        tq.last_tri = bh.tri;
-                        if (++guard > 64) break;
                     }
                 }
                 const float4 f4v = ps.sh_f[(size_t)k * cap + p];
@@ -754,6 +809,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene sc, PtParams prm, Pa
         }
         wave_add64(prm.stats + kStatShadow, nsh);
         wave_add64(prm.stats + kStatTransmittance, ntr);
+        wave_add64(prm.stats + kStatShadowNodes, tc.nodes);
+        wave_add64(prm.stats + kStatShadowTris, tc.tris);
     }
 }
 
@@ -828,9 +885,10 @@ __global__ void k_tonemap(float4 *img, uint32_t n, float gamma, float exposure)
 // ------------------------------------------------------------ launchers
 static dim3 persistent_grid(uint32_t items)
 {
-    // enough waves to fill 256 CUs several times over, never more than items need
+    // enough waves to fill 256 CUs several times over, never more than items
+    // need; the traversal spill area holds kSpillThreads threads
     uint32_t waves = (items + kBlock - 1) / kBlock;
-    uint32_t cap = 256u * 16u;
+    constexpr uint32_t cap = kSpillThreads / kBlock;
     return dim3(waves < cap ? (waves ? waves : 1) : cap);
 }
 
@@ -841,11 +899,11 @@ int launch_raygen(const DevScene &sc, const PtParams &prm, const PathState &ps, 
     return YMRT_OK;
 }
 
-template <bool kNearby, int kStack>
+template <bool kNearby, bool kMedia, int kStack>
 static int launch_extend_t(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_extend<kNearby, kStack>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps,
-                       pass);
+    hipLaunchKernelGGL((k_extend<kNearby, kMedia, kStack>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc,
+                       prm, ps, pass);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }
@@ -859,12 +917,23 @@ static int launch_shadow_t(const DevScene &sc, const PtParams &prm, const PathSt
     return YMRT_OK;
 }
 
+// The window stack spills to a kMaxStackDepth-deep area, so one depth bound
+// serves every scene the host accepted.
 int launch_extend(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool nearby,
-                  uint32_t stack, hipStream_t s)
+                  bool media, uint32_t, hipStream_t s)
 {
-    if (stack <= 32) return nearby ? launch_extend_t<true, 32>(sc, prm, ps, pass, s) : launch_extend_t<false, 32>(sc, prm, ps, pass, s);
-    if (stack <= 48) return nearby ? launch_extend_t<true, 48>(sc, prm, ps, pass, s) : launch_extend_t<false, 48>(sc, prm, ps, pass, s);
-    return nearby ? launch_extend_t<true, 64>(sc, prm, ps, pass, s) : launch_extend_t<false, 64>(sc, prm, ps, pass, s);
+    if (nearby)
+        return media ? launch_extend_t<true, true, kMaxStackDepth>(sc, prm, ps, pass, s)
+                     : launch_extend_t<true, false, kMaxStackDepth>(sc, prm, ps, pass, s);
+    return media ? launch_extend_t<false, true, kMaxStackDepth>(sc, prm, ps, pass, s)
+                 : launch_extend_t<false, false, kMaxStackDepth>(sc, prm, ps, pass, s);
+}
+
+int launch_medium(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_medium, persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps, pass);
+    YM_HIP(hipGetLastError());
+    return YMRT_OK;
 }
 
 int launch_shade(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool volume,
@@ -879,11 +948,10 @@ int launch_shade(const DevScene &sc, const PtParams &prm, const PathState &ps, u
 }
 
 int launch_shadow(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool media,
-                  uint32_t stack, hipStream_t s)
+                  uint32_t, hipStream_t s)
 {
-    if (stack <= 32) return media ? launch_shadow_t<true, 32>(sc, prm, ps, pass, s) : launch_shadow_t<false, 32>(sc, prm, ps, pass, s);
-    if (stack <= 48) return media ? launch_shadow_t<true, 48>(sc, prm, ps, pass, s) : launch_shadow_t<false, 48>(sc, prm, ps, pass, s);
-    return media ? launch_shadow_t<true, 64>(sc, prm, ps, pass, s) : launch_shadow_t<false, 64>(sc, prm, ps, pass, s);
+    return media ? launch_shadow_t<true, kMaxStackDepth>(sc, prm, ps, pass, s)
+                 : launch_shadow_t<false, kMaxStackDepth>(sc, prm, ps, pass, s);
 }
 
 int launch_flush(const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)

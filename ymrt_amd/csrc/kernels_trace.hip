@@ -28,16 +28,16 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(DevScene sc, const float4
     q.tmax = r1.w;
     q.last_inst = kEmpty;
     q.last_tri = kEmpty;
-    LaneScratch<kTraceBlock> ls{stack, nullptr, nullptr, nullptr, (int)threadIdx.x};
+    LaneScratch<kTraceBlock> ls{stack, nullptr, nullptr, nullptr, (int)threadIdx.x, nullptr, 0, 0};
     Hit h;
     h.back = 0;
     int nbc = 0;
     TraceCounters cnt{0, 0};
     int r;
     if (kAny)
-        r = traverse<kModeSkipBoundary, true, false, kCount, kTraceBlock, kStack>(sc, q, h, nbc, ls, cnt);
+        r = traverse<kModeSkipBoundary, true, false, kCount, kStack>(sc, q, h, nbc, ls, cnt);
     else
-        r = traverse<kModeAll, false, false, kCount, kTraceBlock, kStack>(sc, q, h, nbc, ls, cnt);
+        r = traverse<kModeAll, false, false, kCount, kStack>(sc, q, h, nbc, ls, cnt);
     if (r < 0) {
         atomicOr(err, 1u);
         r = 0;

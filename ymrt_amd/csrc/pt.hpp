@@ -17,14 +17,27 @@ constexpr uint32_t kCtrWorkShade = 6;
 constexpr uint32_t kCtrWorkShadeVol = 7;
 constexpr uint32_t kCtrWorkShadow = 8;
 constexpr uint32_t kCtrWorkFlush = 9;
+constexpr uint32_t kCtrMedium = 10;        // paths inside media queued for free flight
+constexpr uint32_t kCtrWorkMedium = 11;
 constexpr uint32_t kCtrStride = 12;
 constexpr uint32_t kMaxPasses = 4 * kMaxRayDepth;
+
+// Path-tracing traversals keep the top kStackWindow stack entries per lane in
+// LDS and spill deeper ones to a global area of (kMaxStackDepth - kStackWindow)
+// entries for each of kSpillThreads threads (the largest persistent grid).
+constexpr int kStackWindow = 16;
+constexpr int kMaxStackDepth = 64;
+constexpr uint32_t kSpillThreads = 256u * 16u * 64u;
 
 // Frame statistics (device, 64-bit)
 constexpr uint32_t kStatCamera = 0;
 constexpr uint32_t kStatExtension = 1;
 constexpr uint32_t kStatShadow = 2;
 constexpr uint32_t kStatTransmittance = 3;
+constexpr uint32_t kStatExtendNodes = 4;   // wide nodes visited by closest-hit traversals
+constexpr uint32_t kStatExtendTris = 5;    // triangles tested by closest-hit traversals
+constexpr uint32_t kStatShadowNodes = 6;   // same for shadow and transmittance traversals
+constexpr uint32_t kStatShadowTris = 7;
 constexpr uint32_t kStatCount = 8;
 
 // Structure-of-arrays path state for `cap` paths in flight (indexed by slot).
@@ -44,7 +57,8 @@ struct PathState {
     uint32_t *sh_flags;
     float4 *out;        // final radiance per path
     uint32_t *queue[2];
-    uint32_t *q_hit, *q_hit_vol, *q_shadow;
+    uint32_t *q_hit, *q_hit_vol, *q_medium, *q_shadow;
+    uint32_t *spill;    // traversal stack spill [kMaxStackDepth - kStackWindow][kSpillThreads]
 };
 
 struct PtParams {

@@ -19,7 +19,8 @@ namespace ym {
 
 int launch_raygen(const DevScene &sc, const PtParams &prm, const PathState &ps, hipStream_t s);
 int launch_extend(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool nearby,
-                  uint32_t stack, hipStream_t s);
+                  bool media, uint32_t stack, hipStream_t s);
+int launch_medium(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s);
 int launch_shade(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool volume,
                  hipStream_t s);
 int launch_shadow(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool media,
@@ -94,7 +95,9 @@ static int ensure_buffers(ymrt_scene *sc, uint32_t cap)
     rc |= alloc(rb, &ps.queue[1], n);
     rc |= alloc(rb, &ps.q_hit, n);
     rc |= alloc(rb, &ps.q_hit_vol, n);
+    rc |= alloc(rb, &ps.q_medium, n);
     rc |= alloc(rb, &ps.q_shadow, n);
+    rc |= alloc(rb, &ps.spill, (size_t)(kMaxStackDepth - kStackWindow) * kSpillThreads);
     rc |= alloc(rb, &rb->counters, (size_t)(kMaxPasses + 2) * kCtrStride);
     rc |= alloc(rb, &rb->stats, kStatCount);
     if (rc) {
@@ -148,7 +151,7 @@ int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, floa
     const bool nearby = sc->needs_nearby;
     const bool media = rs->enable_volume_scattering && sc->dev.volume_count > 0;
     const uint32_t stack = sc->max_stack;
-    YM_REQUIRE(stack <= 64, "scene BVH too deep for the path-tracing traversal stack");
+    YM_REQUIRE(stack <= (uint32_t)kMaxStackDepth, "scene BVH too deep for the path-tracing traversal stack");
 
     YM_HIP(hipMemsetAsync(rb->stats, 0, sizeof(unsigned long long) * kStatCount, stream));
     rb->event_next = 0;
@@ -187,7 +190,11 @@ int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, floa
         uint32_t pass = 0;
         uint32_t next_check = (uint32_t)rs->ray_depth + 1;
         for (;;) {
-            if ((rc = timed(1, [&] { return launch_extend(sc->dev, prm, rb->ps, pass, nearby, stack, stream); })))
+            if ((rc = timed(1, [&] {
+                     return launch_extend(sc->dev, prm, rb->ps, pass, nearby, media, stack, stream);
+                 })))
+                return rc;
+            if (media && (rc = timed(5, [&] { return launch_medium(sc->dev, prm, rb->ps, pass, stream); })))
                 return rc;
             if ((rc = timed(2, [&] { return launch_shade(sc->dev, prm, rb->ps, pass, false, stream); }))) return rc;
             if (media && (rc = timed(2, [&] { return launch_shade(sc->dev, prm, rb->ps, pass, true, stream); })))
@@ -221,12 +228,16 @@ int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, floa
         stats->extension_rays = st[kStatExtension];
         stats->shadow_rays = st[kStatShadow];
         stats->transmittance_rays = st[kStatTransmittance];
+        stats->extend_nodes = st[kStatExtendNodes];
+        stats->extend_tris = st[kStatExtendTris];
+        stats->shadow_nodes = st[kStatShadowNodes];
+        stats->shadow_tris = st[kStatShadowTris];
         stats->paths = (uint64_t)rs->width * rs->height * spp;
         stats->passes = passes_total;
         stats->chunks = chunks;
         if (timing) {
-            float ms[5] = {0, 0, 0, 0, 0};
-            uint32_t launches[5] = {0, 0, 0, 0, 0};
+            float ms[6] = {0, 0, 0, 0, 0, 0};
+            uint32_t launches[6] = {0, 0, 0, 0, 0, 0};
             for (const Mark &m : marks) {
                 float t = 0.0f;
                 YM_HIP(hipEventElapsedTime(&t, m.a, m.b));
@@ -238,6 +249,8 @@ int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, floa
             stats->ms_shade = ms[2];
             stats->ms_shadow = ms[3];
             stats->ms_resolve = ms[4];
+            stats->ms_medium = ms[5];
+            stats->launches_medium = launches[5];
             stats->launches_extend = launches[1];
             stats->launches_shade = launches[2];
             stats->launches_shadow = launches[3];

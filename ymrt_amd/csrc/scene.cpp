@@ -137,12 +137,12 @@ static void camera_media(const DevScene &hs, const HostScene &h, const ymrt_scen
     uint32_t nbp[kNearbyCap];
     float nbt[kNearbyCap];
     int nbb[kNearbyCap];
-    LaneScratch<1> ls{stack.data(), nbp, nbt, nbb, 0};
+    LaneScratch<1> ls{stack.data(), nbp, nbt, nbb, 0, nullptr, 0, 0};
     for (int depth = 0; depth < 48; ++depth) {
         Hit hit{};
         int nbc = 0;
         TraceCounters cnt{0, 0};
-        int r = traverse<kModeAll, false, false, false, 1, 1024>(hs, q, hit, nbc, ls, cnt);
+        int r = traverse<kModeAll, false, false, false, 1024>(hs, q, hit, nbc, ls, cnt);
         if (r != 1) break;
         const DevInstance &in = h.instances[hit.inst];
         const ShadeTri &st = h.shade[in.shade_base + hit.tri];

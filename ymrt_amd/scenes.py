@@ -111,6 +111,26 @@ def cornell_box(width=256, height=256, halton_seed=0):
     return sa
 
 
+def cornell_media(width=64, height=64, grid=True, halton_seed=0, sigma=2.0):
+    """Cornell box with a participating medium in the box [0.2,0.8] x [0.05,0.65]
+    x [0.2,0.8] (a boundary instance): an 8^3 voxel density grid in [0, 1]
+    scaled by sigma when grid is True, else homogeneous sigma * 0.5."""
+    sa = cornell_box(width, height, halton_seed)
+    lo, hi = np.array((0.2, 0.05, 0.2)), np.array((0.8, 0.65, 0.8))
+    w2o = np.eye(4, dtype=np.float32)
+    w2o[:3, :3] = np.diag(1.0 / (hi - lo))
+    w2o[:3, 3] = -lo / (hi - lo)
+    if grid:
+        rng = np.random.RandomState(5)
+        g = sa.add_grid(rng.rand(8, 8, 8).astype(np.float32))
+        vol = sa.add_volume((sigma, sigma, sigma), (0.9, 0.7, 0.5), 0.2, grid=g, world_to_object=w2o)
+    else:
+        vol = sa.add_volume((0.5 * sigma,) * 3, (0.9, 0.7, 0.5), 0.2, world_to_object=w2o)
+    mat = sa.add_material(abi.default_material(ior_n=1.0))
+    sa.add_instance(_add(sa, box(tuple(lo), tuple(hi))), mat, inner_volume=vol, boundary=True)
+    return sa
+
+
 # --------------------------------------------------------------- value noise
 def value_noise_2d(n, octaves, seed, lattice=16):
     """fBm of bilinearly interpolated value noise on an n x n grid, in [0, 1]."""
