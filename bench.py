@@ -48,39 +48,36 @@ def parse():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--depth", type=int, default=8)
+    ap.add_argument("--sampler", choices=["halton", "pcg"], default="halton")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="before rendering, copy MIB MiB device-to-device once (the known-size kernel that "
+                         "scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
 
 
 def cpu_baseline(sa, rs_full, budget_s):
     """The oracle (the CPU restatement) rendering the same frame on the host's
-    cores: a probe of 8 rows at 1 spp sizes a sample of about budget_s seconds
-    (whole frame at k spp, or a centred band of rows at 1 spp)."""
+    cores.  A first full frame at 1 spp (untimed for the result) sizes the
+    timed sample: the whole frame at the spp that takes about budget_s."""
     import oracle
     from ymrt_amd import abi
 
     threads = max(1, min(16, os.cpu_count() or 1))
     sc = oracle.Scene(sa)
     w, h = rs_full.width, rs_full.height
-    rs1 = abi.render_settings(w, h, 1, rs_full.ray_depth, volume=1)
     t0 = time.time()
-    sc.render(rs1, 1, rows=(h // 2 - 4, h // 2 + 4), threads=threads)
-    frame_s = max(time.time() - t0, 1e-3) * h / 8.0
-    if frame_s < budget_s:
-        spp = int(max(1, min(rs_full.spp, budget_s // frame_s)))
-        rows = (0, h)
-    else:
-        spp = 1
-        n = int(max(8, min(h, h * budget_s / frame_s)))
-        rows = (h // 2 - n // 2, h // 2 - n // 2 + n)
+    sc.render(abi.render_settings(w, h, 1, rs_full.ray_depth, volume=1), 2, threads=threads)
+    frame_s = max(time.time() - t0, 1e-3)
+    spp = int(max(1, min(rs_full.spp, round(budget_s / frame_s))))
     rs = abi.render_settings(w, h, spp, rs_full.ray_depth, volume=1)
     t0 = time.time()
-    sc.render(rs, 1, rows=rows, threads=threads)
+    sc.render(rs, 1, threads=threads)
     dt = time.time() - t0
     c = sc.last_counts
     rays = int(c[1] + c[2] + c[3])
-    samples = (rows[1] - rows[0]) * w * spp
+    samples = w * h * spp
     sc.close()
     return {
         "value": round(rays / dt / 1e6, 3),
@@ -88,8 +85,8 @@ def cpu_baseline(sa, rs_full, budget_s):
         "cores": threads,
         "kind": "port",
         "samples_per_s": round(samples / dt, 1),
-        "sample": "oracle, rows %d-%d of the %dx%d frame at %d spp (%d samples, %d rays), %.1f s on %d threads"
-                  % (rows[0], rows[1], w, h, spp, samples, rays, dt, threads),
+        "sample": "oracle, the full %dx%d frame at %d spp (%d samples, %d rays), %.1f s on %d threads"
+                  % (w, h, spp, samples, rays, dt, threads),
     }
 
 
@@ -113,11 +110,22 @@ def main():
     ctx = ymrt_amd.Context(0)
     sc = ymrt_amd.Scene(ctx, sa)
     info = sc.info()
-    rs = abi.render_settings(args.width, args.height, args.spp, args.depth, volume=1)
+    rs = abi.render_settings(args.width, args.height, args.spp, args.depth, volume=1,
+                             sampler=abi.SAMPLER_HALTON if args.sampler == "halton" else abi.SAMPLER_PCG)
     noise = torch.empty((args.height, args.width, 4), dtype=torch.float32, device="cuda")
     print("scene: %d tris, %d wide nodes, %.1f MB in HBM, build %.0f ms (gen %.1f s); stack %d"
           % (info.triangle_count, info.node_count, info.device_bytes / 1e6, info.build_ms, gen_s, info.max_stack),
           file=sys.stderr, flush=True)
+
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        print("copy calibration: %d MiB" % args.copy_calibration, file=sys.stderr, flush=True)
 
     for w in range(args.warmup):
         sc.render_frame(rs, 1 + w, noise)
@@ -129,9 +137,9 @@ def main():
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     rays = 0
-    stage = dict(extend=0.0, medium=0.0, shade=0.0, shadow=0.0, raygen=0.0, resolve=0.0)
-    launches = dict(extend=0, medium=0, shade=0, shadow=0)
-    visits = dict(extend_nodes=0, extend_tris=0, shadow_nodes=0, shadow_tris=0)
+    stage = dict(extend=0.0, medium=0.0, shade=0.0, shadow=0.0, transmit=0.0, raygen=0.0, resolve=0.0)
+    launches = dict(extend=0, medium=0, shade=0, shadow=0, transmit=0)
+    visits = dict(extend_nodes=0, extend_tris=0, shadow_nodes=0, shadow_tris=0, transmit_nodes=0, transmit_tris=0)
     ext_rays = sh_rays = tr_rays = 0
     t_wall = time.time()
     start.record()
@@ -161,7 +169,7 @@ def main():
     # time on the render stream.
     ext_bytes = (EXTEND_STATE_BYTES * ext_rays + NODE_BYTES * visits["extend_nodes"]
                  + TRI_BYTES * visits["extend_tris"])
-    sh_bytes = (SHADOW_RAY_BYTES * (sh_rays + tr_rays) + SHADOW_PATH_BYTES * sh_rays // 2
+    sh_bytes = (SHADOW_RAY_BYTES * sh_rays + SHADOW_PATH_BYTES * sh_rays // 2
                 + NODE_BYTES * visits["shadow_nodes"] + TRI_BYTES * visits["shadow_tris"])
     if stage["shadow"] > stage["extend"]:
         dom, dom_bytes, dom_ms, dom_launch = "k_shadow", sh_bytes, stage["shadow"], launches["shadow"]
@@ -198,8 +206,9 @@ def main():
                       "nodes_per_extend_ray": round(visits["extend_nodes"] / max(ext_rays, 1), 2),
                       "tris_per_extend_ray": round(visits["extend_tris"] / max(ext_rays, 1), 2),
                       "shadow_rays": sh_rays // args.steps, "transmittance_rays": tr_rays // args.steps,
-                      "nodes_per_shadow_ray": round(visits["shadow_nodes"] / max(sh_rays + tr_rays, 1), 2),
-                      "tris_per_shadow_ray": round(visits["shadow_tris"] / max(sh_rays + tr_rays, 1), 2)},
+                      "nodes_per_shadow_ray": round(visits["shadow_nodes"] / max(sh_rays, 1), 2),
+                      "tris_per_shadow_ray": round(visits["shadow_tris"] / max(sh_rays, 1), 2),
+                      "nodes_per_transmittance_ray": round(visits["transmit_nodes"] / max(tr_rays, 1), 2)},
         "wall_s": round(wall, 2),
     }
     if not args.no_cpu:

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define YMRT_ABI_VERSION 1
+#define YMRT_ABI_VERSION 2
 
 /* status codes */
 #define YMRT_OK 0
@@ -66,11 +66,23 @@ typedef struct ymrt_mesh_desc {
     const float *texcoords;          /* 2 * texcoord_count                    */
 } ymrt_mesh_desc;
 
+/* Analytic sphere centred at the object-space origin (Sphere,
+ * src/GeometryDefines.h; intersected by IntersectSphereObject,
+ * src/ShadingUtilities.cuh:219-275). */
+typedef struct ymrt_sphere_desc {
+    float radius;
+} ymrt_sphere_desc;
+
+#define YMRT_GEOMETRY_MESH 0u
+#define YMRT_GEOMETRY_SPHERE 1u
+
 typedef struct ymrt_instance_desc {
-    uint32_t mesh;                   /* index into meshes                     */
+    uint32_t geometry;               /* index into meshes or spheres          */
     uint32_t material;               /* index into materials                  */
     int32_t inner_volume;            /* index into volumes or -1              */
     int32_t treat_as_boundary;       /* 1: medium boundary, skipped by shading */
+    uint32_t geometry_type;          /* YMRT_GEOMETRY_MESH or _SPHERE         */
+    uint32_t reserved;
     float object_to_world[16];
     float world_to_object[16];
 } ymrt_instance_desc;
@@ -132,6 +144,16 @@ typedef struct ymrt_volume_desc {
     float world_to_object[16];       /* maps world to the grid's unit cube    */
 } ymrt_volume_desc;
 
+/* DistantLight (src/Light.h:324-386): radiance light_color * intensity from
+ * the directions within theta_max degrees of -y of the light's frame. */
+typedef struct ymrt_distant_light_desc {
+    float light_color[3];
+    float intensity;
+    float theta_max;                 /* cone half-angle, degrees              */
+    float object_to_world[16];       /* the light's frame                     */
+    float world_to_object[16];
+} ymrt_distant_light_desc;
+
 typedef struct ymrt_camera_desc {
     float position[3];
     float w[3];                      /* backward axis, Camera::SetDir         */
@@ -141,11 +163,13 @@ typedef struct ymrt_camera_desc {
 
 typedef struct ymrt_scene_desc {
     uint32_t mesh_count;       const ymrt_mesh_desc *meshes;
+    uint32_t sphere_count;     const ymrt_sphere_desc *spheres;
     uint32_t instance_count;   const ymrt_instance_desc *instances;
     uint32_t material_count;   const ymrt_material_desc *materials;
     uint32_t texture_count;    const ymrt_texture_desc *textures;
     uint32_t grid_count;       const ymrt_grid_desc *grids;
     uint32_t volume_count;     const ymrt_volume_desc *volumes;
+    uint32_t distant_light_count; const ymrt_distant_light_desc *distant_lights;
     ymrt_camera_desc camera;
     uint32_t halton_seed;      /* seed of the Halton digit permutations     */
 } ymrt_scene_desc;
@@ -176,6 +200,7 @@ typedef struct ymrt_render_settings {
     int32_t ray_depth;
     int32_t sampler_type;
     int32_t enable_shape_light;
+    int32_t enable_distant_light;
     int32_t enable_volume_scattering;
     int32_t enable_env_light;
     int32_t enable_russian_roulette;
@@ -202,8 +227,11 @@ typedef struct ymrt_frame_stats {
     float ms_medium;                 /* free-flight sampling (scenes with media) */
     uint32_t launches_medium;
     /* Traversal work, always counted: wide nodes visited and triangles
-     * tested by closest-hit (extend) and by shadow/transmittance queries. */
+     * tested by closest-hit (extend) and any-hit (shadow) queries. */
     uint64_t extend_nodes, extend_tris, shadow_nodes, shadow_tris;
+    float ms_transmit;               /* transmittance of shadow rays in media */
+    uint32_t launches_transmit;
+    uint64_t transmit_nodes, transmit_tris;   /* boundary walks of those rays */
 } ymrt_frame_stats;
 
 /* ---------------------------------------------------------------- entry points */

@@ -330,13 +330,22 @@ or_scene *or_scene_create(const ymrt_scene_desc *d)
         for (uint32_t t = 0; t < m->tri_count; ++t) m->order[t] = pi[t].idx;
         free(pi);
     }
+    sc->sphere_count = d->sphere_count;
+    sc->spheres = d->spheres;
     sc->instance_count = d->instance_count;
     sc->instances = d->instances;
     if (d->instance_count) {
         prim_info *pi = (prim_info *)malloc(sizeof(prim_info) * d->instance_count);
         for (uint32_t i = 0; i < d->instance_count; ++i) {
             const ymrt_instance_desc *in = &d->instances[i];
-            or_bbox ob = sc->meshes[in->mesh].bounds;
+            or_bbox ob;
+            if (in->geometry_type == YMRT_GEOMETRY_SPHERE) {
+                float r = d->spheres[in->geometry].radius;
+                ob.pmin = mk3(-r, -r, -r);
+                ob.pmax = mk3(r, r, r);
+            } else {
+                ob = sc->meshes[in->geometry].bounds;
+            }
             or_bbox wb = bb_empty();
             for (int c = 0; c < 8; ++c) {
                 v3 corner = mk3((c & 1) ? ob.pmax.x : ob.pmin.x, (c & 2) ? ob.pmax.y : ob.pmin.y,
@@ -415,6 +424,57 @@ int or_intersect_triangle(const v3 q0, const v3 q1, const v3 q2, const or_ray *r
 }
 
 typedef struct { float t; int hit; int back; } extra_info;
+static int intersect_mesh(const or_mesh *m, or_ray *ray, uint32_t inst, or_hit *hr, int any, extra_info *ex);
+
+/* IntersectSphereObject (src/ShadingUtilities.cuh:219-275), object space.  The
+ * roots are divided by 2a in double precision as the reference's 2.0 literal
+ * makes them; hr->bary receives the object-space hit point. */
+static int intersect_sphere(float radius, or_ray *ray, or_hit *hr, extra_info *ex)
+{
+    v3 o = ray->o, d = ray->d;
+    float a = d.x * d.x + d.y * d.y + d.z * d.z;
+    float b = 2.0f * (d.x * o.x + d.y * o.y + d.z * o.z);
+    float c = o.x * o.x + o.y * o.y + o.z * o.z - radius * radius;
+    float disc = b * b - 4.0f * a * c;
+    if (disc < 0) return 0;
+    float sq = sqrtf(disc);
+    float t1 = (float)((double)(-b - sq) / (2.0 * (double)a));
+    float t2 = (float)((double)(-b + sq) / (2.0 * (double)a));
+    if (ex) {
+        ex->hit = 1;
+        ex->t = t1 >= OR_TMIN ? t1 : t2;
+        ex->back = dot3(d, normalize3(add3(o, scl3(d, ex->t)))) > 0.0f;
+    }
+    if (t1 > ray->t || t2 < OR_TMIN) return 0;
+    float t = t1;
+    if (t < OR_TMIN) {
+        t = t2;
+        if (t > ray->t) return 0;
+    }
+    ray->t = t;
+    v3 po = add3(o, scl3(d, t));
+    float dist = length3(po);
+    po = dist > OR_FLOAT_EPSILON ? divs3(scl3(po, radius), dist) : s3(0.0f);
+    hr->t = t;
+    hr->bary = po;
+    hr->back = dot3(d, divs3(po, radius)) > 0.0f;
+    hr->tri = OR_EMPTY;
+    return 1;
+}
+
+int or_sphere_any(float radius, or_ray *ray)
+{
+    or_hit h;
+    return intersect_sphere(radius, ray, &h, NULL);
+}
+
+static int intersect_instance(const or_scene *sc, const ymrt_instance_desc *in, uint32_t ii, or_ray *orr,
+                              or_hit *ph, int any, extra_info *ex)
+{
+    if (in->geometry_type == YMRT_GEOMETRY_SPHERE)
+        return intersect_sphere(sc->spheres[in->geometry].radius, orr, ph, ex);
+    return intersect_mesh(&sc->meshes[in->geometry], orr, ii, ph, any, ex);
+}
 
 /* src/ShadingUtilities.cuh:136-217 with IntersectTri folded in. */
 static int intersect_mesh(const or_mesh *m, or_ray *ray, uint32_t inst, or_hit *hr,
@@ -505,7 +565,7 @@ static int trace_nearby(const or_scene *sc, or_ray *ray, or_hit *hr, int *ncount
                 memset(&ph, 0, sizeof(ph));
                 ph.t = OR_TMAX; ph.inst = OR_EMPTY; ph.tri = OR_EMPTY;
                 extra_info ex = {OR_TMAX, 0, 0};
-                if (intersect_mesh(&sc->meshes[in->mesh], &orr, ii, &ph, 0, &ex)) {
+                if (intersect_instance(sc, in, ii, &orr, &ph, 0, &ex)) {
                     hr->back = ph.back;
                     hr->bary = ph.bary;
                     hr->t = ph.t;
@@ -580,7 +640,7 @@ int or_trace_shadow(const or_scene *sc, or_ray *ray)
                 or_hit ph;
                 memset(&ph, 0, sizeof(ph));
                 ph.t = OR_TMAX;
-                if (intersect_mesh(&sc->meshes[in->mesh], &orr, ii, &ph, 1, NULL)) {
+                if (intersect_instance(sc, in, ii, &orr, &ph, 1, NULL)) {
                     ray->t = orr.t;
                     return 1;
                 }
@@ -602,20 +662,27 @@ int or_trace_shadow(const or_scene *sc, or_ray *ray)
     return 0;
 }
 
-/* src/ShadingUtilities.cuh:894-951 (mesh branch) */
+/* FetchGeometryNormal, src/ShadingUtilities.cuh:878-951 */
 void or_fetch_geometry_normal(const or_scene *sc, const or_hit *h, v3 *p, v3 *perr, v3 *ng)
 {
     const ymrt_instance_desc *in = &sc->instances[h->inst];
-    const or_mesh *m = &sc->meshes[in->mesh];
-    v3 p0, p1, p2;
-    tri_verts(m, h->tri, &p0, &p1, &p2);
-    v3 b = h->bary;
-    float xs = fabsf(p0.x * b.x) + fabsf(p1.x * b.y) + fabsf(p2.x * b.z);
-    float ys = fabsf(p0.y * b.x) + fabsf(p1.y * b.y) + fabsf(p2.y * b.z);
-    float zs = fabsf(p0.z * b.x) + fabsf(p1.z * b.y) + fabsf(p2.z * b.z);
-    *p = add3(add3(scl3(p0, b.x), scl3(p1, b.y)), scl3(p2, b.z));
-    *perr = sclf3(error_gamma(7), mk3(xs, ys, zs));
-    *ng = cross3(sub3(p1, p0), sub3(p2, p0));
+    if (in->geometry_type == YMRT_GEOMETRY_SPHERE) {
+        float radius = fmaxr(sc->spheres[in->geometry].radius, OR_FLOAT_EPSILON);
+        *p = h->bary;
+        *perr = sclf3(error_gamma(5), abs3(h->bary));
+        *ng = divs3(h->bary, radius);
+    } else {
+        const or_mesh *m = &sc->meshes[in->geometry];
+        v3 p0, p1, p2;
+        tri_verts(m, h->tri, &p0, &p1, &p2);
+        v3 b = h->bary;
+        float xs = fabsf(p0.x * b.x) + fabsf(p1.x * b.y) + fabsf(p2.x * b.z);
+        float ys = fabsf(p0.y * b.x) + fabsf(p1.y * b.y) + fabsf(p2.y * b.z);
+        float zs = fabsf(p0.z * b.x) + fabsf(p1.z * b.y) + fabsf(p2.z * b.z);
+        *p = add3(add3(scl3(p0, b.x), scl3(p1, b.y)), scl3(p2, b.z));
+        *perr = sclf3(error_gamma(7), mk3(xs, ys, zs));
+        *ng = cross3(sub3(p1, p0), sub3(p2, p0));
+    }
     *p = transform_position(in->object_to_world, *p, perr);
     *ng = normalize3(transpose_transform_vector(in->world_to_object, *ng));
 }
@@ -689,9 +756,18 @@ void or_brute_closest_batch(const or_scene *sc, const ymrt_ray *rays, uint32_t n
         uint32_t bi = OR_EMPTY, bt = OR_EMPTY, bb = 0;
         for (uint32_t ii = 0; ii < sc->instance_count; ++ii) {
             const ymrt_instance_desc *in = &sc->instances[ii];
-            const or_mesh *m = &sc->meshes[in->mesh];
             or_ray orr;
             transform_ray(&wr, in->world_to_object, &orr);
+            if (in->geometry_type == YMRT_GEOMETRY_SPHERE) {
+                or_hit sh;
+                orr.t = best;
+                if (intersect_sphere(sc->spheres[in->geometry].radius, &orr, &sh, NULL) &&
+                    !(sh.t == best && bi != OR_EMPTY)) {
+                    best = sh.t; bi = ii; bt = OR_EMPTY; bb = (uint32_t)sh.back;
+                }
+                continue;
+            }
+            const or_mesh *m = &sc->meshes[in->geometry];
             for (uint32_t t = 0; t < m->tri_count; ++t) {
                 v3 p0, p1, p2, bary;
                 float th; int back;

@@ -196,8 +196,12 @@ static void camera_transfer(or_scene *sc)
         negr.d = neg3(pos.d);
         /* any-hit against this instance only */
         int hp = 0, hn = 0;
-        const or_mesh *m = &sc->meshes[in->mesh];
-        for (uint32_t t = 0; t < m->tri_count && !(hp && hn); ++t) {
+        if (in->geometry_type == YMRT_GEOMETRY_SPHERE) {
+            hp = or_sphere_any(sc->spheres[in->geometry].radius, &pos);
+            hn = or_sphere_any(sc->spheres[in->geometry].radius, &negr);
+        }
+        const or_mesh *m = in->geometry_type == YMRT_GEOMETRY_SPHERE ? NULL : &sc->meshes[in->geometry];
+        for (uint32_t t = 0; m && t < m->tri_count && !(hp && hn); ++t) {
             v3 p0 = mk3(m->pos[3 * m->pidx[3 * t]], m->pos[3 * m->pidx[3 * t] + 1], m->pos[3 * m->pidx[3 * t] + 2]);
             v3 p1 = mk3(m->pos[3 * m->pidx[3 * t + 1]], m->pos[3 * m->pidx[3 * t + 1] + 1], m->pos[3 * m->pidx[3 * t + 1] + 2]);
             v3 p2 = mk3(m->pos[3 * m->pidx[3 * t + 2]], m->pos[3 * m->pidx[3 * t + 2] + 1], m->pos[3 * m->pidx[3 * t + 2] + 2]);
@@ -253,12 +257,18 @@ void or_shading_init(or_scene *sc, const ymrt_scene_desc *d)
         for (size_t i = 0; i < n; ++i) if (g->density[i] > mx) mx = g->density[i];
         g->max_density = mx;
     }
+    /* distant lights: DistantLight(), src/Light.h:335-339 */
+    sc->distant_count = d->distant_light_count;
+    sc->distant = d->distant_lights;
+    sc->distant_cos_max = (float *)calloc(d->distant_light_count ? d->distant_light_count : 1, sizeof(float));
+    for (uint32_t i = 0; i < d->distant_light_count; ++i)
+        sc->distant_cos_max[i] = cosf(d->distant_lights[i].theta_max * 0.01745329251994329576923690768489f);
     /* shape lights: src/core/SceneModule.cpp:1203-1304 */
     uint32_t nl = 0;
     for (uint32_t i = 0; i < d->instance_count; ++i) {
         const ymrt_instance_desc *in = &d->instances[i];
         if (in->material < d->material_count && d->materials[in->material].type == YMRT_MATERIAL_LIGHT)
-            nl += d->meshes[in->mesh].triangle_count;
+            nl += in->geometry_type == YMRT_GEOMETRY_SPHERE ? 1 : d->meshes[in->geometry].triangle_count;
     }
     sc->light_count = nl;
     sc->lights = (or_shape_light *)calloc(nl ? nl : 1, sizeof(or_shape_light));
@@ -269,7 +279,24 @@ void or_shading_init(or_scene *sc, const ymrt_scene_desc *d)
         if (!(in->material < d->material_count && d->materials[in->material].type == YMRT_MATERIAL_LIGHT)) continue;
         const ymrt_material_desc *lm = &d->materials[in->material];
         float irr = (lm->light_color[0] + lm->light_color[1] + lm->light_color[2]) * lm->intensity * OR_PI;
-        const ymrt_mesh_desc *md = &d->meshes[in->mesh];
+        if (in->geometry_type == YMRT_GEOMETRY_SPHERE) {
+            /* sphere branch: ellipsoid area estimate with the instance's axis
+             * scales (lengths of the object-to-world columns) */
+            const float *m = in->object_to_world;
+            float sx = sqrtf(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+            float sy = sqrtf(m[4] * m[4] + m[5] * m[5] + m[6] * m[6]);
+            float sz = sqrtf(m[8] * m[8] + m[9] * m[9] + m[10] * m[10]);
+            float radius = d->spheres[in->geometry].radius;
+            float a = radius * sx, b = radius * sy, c = radius * sz;
+            float area = (4.0f / 3.0f) * OR_PI * (a * b + b * c + c * a);
+            sc->lights[nl].instance = i;
+            sc->lights[nl].triangle = OR_EMPTY;
+            sc->lights[nl].power = irr * area;
+            sc->lights[nl].area = area;
+            ++nl;
+            continue;
+        }
+        const ymrt_mesh_desc *md = &d->meshes[in->geometry];
         for (uint32_t t = 0; t < md->triangle_count; ++t) {
             v3 p0 = transform_position(in->object_to_world, ld3(md->positions, md->position_indices[3 * t]), NULL);
             v3 p1 = transform_position(in->object_to_world, ld3(md->positions, md->position_indices[3 * t + 1]), NULL);
@@ -295,6 +322,7 @@ void or_shading_init(or_scene *sc, const ymrt_scene_desc *d)
 
 void or_shading_free(or_scene *sc)
 {
+    free(sc->distant_cos_max);
     for (uint32_t i = 0; i < sc->texture_count; ++i) free(sc->textures[i].texels);
     free(sc->textures);
     free(sc->grids);
@@ -1071,7 +1099,7 @@ static void load_light(const or_scene *sc, const or_shape_light *l, light_tri *l
 {
     lt->in = &sc->instances[l->instance];
     lt->mtl = &sc->materials[lt->in->material];
-    const or_mesh *m = &sc->meshes[lt->in->mesh];
+    const or_mesh *m = &sc->meshes[lt->in->geometry];
     uint32_t t = l->triangle;
     lt->p0 = ld3(m->pos, m->pidx[3 * t]);
     lt->p1 = ld3(m->pos, m->pidx[3 * t + 1]);
@@ -1093,10 +1121,101 @@ static v3 light_le(const light_tri *lt)
 
 static v3 interp3(v3 a, v3 b, v3 c, v3 w) { return add3(add3(scl3(a, w.x), scl3(b, w.y)), scl3(c, w.z)); }
 
+/* RayIntersectSphere, src/Light.h:42-65 (4.0 is a double literal) */
+static int ray_sphere_light(v3 o, v3 d, float radius, float *t0, float *t1)
+{
+    float a = d.x * d.x + d.y * d.y + d.z * d.z;
+    float b = 2.0f * (d.x * o.x + d.y * o.y + d.z * o.z);
+    float c = o.x * o.x + o.y * o.y + o.z * o.z - radius * radius;
+    float disc = (float)((double)(b * b) - 4.0 * (double)a * (double)c);
+    if (disc < 0.0f) return 0;
+    float sq = sqrtf(disc);
+    float ia = 1.0f / a;
+    *t0 = (-b - sq) * 0.5f * ia;
+    *t1 = (-b + sq) * 0.5f * ia;
+    if (*t1 < 1E-10f) return 0;
+    return 1;
+}
+
+/* SampleCone, src/SampleUtilities.h:33-39 */
+static v3 sample_cone(float u0, float u1, float ctm)
+{
+    float ct = (1.0f - u0) + u0 * ctm;
+    float st = sqrtf(fmaxr(0.0f, 1.0f - ct * ct));
+    float phi = u1 * 2.0f * OR_PI;
+    return mk3(cosf(phi) * st, sinf(phi) * st, ct);
+}
+
+static v3 inst_le(const or_scene *sc, const ymrt_instance_desc *in)
+{
+    const ymrt_material_desc *m = &sc->materials[in->material];
+    return scl3(mk3(m->light_color[0], m->light_color[1], m->light_color[2]), m->intensity);
+}
+
+static v3 m4_translate(const float *m) { return mk3(m[12], m[13], m[14]); }
+
+/* src/Light.h:282-317 (sphere) */
+static int sphere_light_sample_li(const or_scene *sc, const or_shape_light *l, v3 pos, float u0, float u1, v3 *wi,
+                                  v3 *lpos, v3 *lerr, v3 *lng, v3 *color, float *pdf)
+{
+    const ymrt_instance_desc *in = &sc->instances[l->instance];
+    v3 opos = transform_position(in->world_to_object, pos, NULL);
+    float lpo = length3(opos);
+    float radius = sc->spheres[in->geometry].radius;
+    if (lpo < radius) { *pdf = 0.0f; return 0; }
+    float ctm = sqrtf(fmaxr(sqrf(lpo) - sqrf(radius), 0.0f)) / lpo;
+    float cone = 2.0f * OR_PI * (1.0f - ctm);
+    v3 cs = sample_cone(u0, u1, ctm);
+    v3 w = neg3(opos), u, v;
+    orthogonal_basis(&w, &u, &v);
+    cs = add3(add3(sclf3(cs.x, u), sclf3(cs.y, v)), sclf3(cs.z, w));
+    float t0 = 0.0f, t1 = 0.0f;
+    ray_sphere_light(opos, cs, radius, &t0, &t1);
+    v3 ip = add3(opos, scl3(cs, t0));
+    float lio = length3(ip);
+    ip = scl3(ip, radius * safe_rcp(lio));
+    v3 ie = sclf3(error_gamma(5), abs3(ip));
+    *lpos = transform_position(in->object_to_world, ip, &ie);
+    *lerr = ie;
+    *lng = normalize3(sub3(*lpos, m4_translate(in->object_to_world)));
+    *wi = normalize3(transform_vector(in->object_to_world, cs));
+    *pdf = 1.0f / cone;
+    *color = scl3(inst_le(sc, in), cone);
+    return 1;
+}
+
+/* src/Light.h:163-199 (sphere) */
+static v3 sphere_light_eval_li(const or_scene *sc, const or_shape_light *l, v3 pos, v3 wi, v3 *lpos, v3 *lerr,
+                               v3 *lng, float *pdf)
+{
+    const ymrt_instance_desc *in = &sc->instances[l->instance];
+    v3 owi = transform_vector(in->world_to_object, wi);
+    v3 opos = transform_position(in->world_to_object, pos, NULL);
+    float lpo = length3(opos);
+    float radius = sc->spheres[in->geometry].radius;
+    if (lpo < radius) { *pdf = 0.0f; return s3(0.0f); }
+    float t0, t1;
+    if (!ray_sphere_light(opos, owi, radius, &t0, &t1)) { *pdf = 0.0f; return s3(0.0f); }
+    float ctm = sqrtf(fmaxr(sqrf(lpo) - sqrf(radius), 0.0f)) / lpo;
+    float cone = 2.0f * OR_PI * (1.0f - ctm);
+    float ct = dot3(normalize3(owi), divs3(neg3(opos), lpo));
+    if (ct < ctm) { *pdf = 0.0f; return s3(0.0f); }
+    v3 ip = add3(opos, scl3(owi, t0));
+    float lio = length3(ip);
+    ip = scl3(ip, radius * safe_rcp(lio));
+    v3 ie = sclf3(error_gamma(5), abs3(ip));
+    *lpos = transform_position(in->object_to_world, ip, &ie);
+    *lerr = ie;
+    *lng = normalize3(sub3(*lpos, m4_translate(in->object_to_world)));
+    *pdf = 1.0f / cone;
+    return inst_le(sc, in);
+}
+
 /* src/Light.h:207-281 (mesh) */
 static int light_sample_li(const or_scene *sc, const or_shape_light *l, v3 pos, float u0, float u1, v3 *wi,
                            v3 *lpos, v3 *lerr, v3 *lng, v3 *color, float *pdf)
 {
+    if (l->triangle == OR_EMPTY) return sphere_light_sample_li(sc, l, pos, u0, u1, wi, lpos, lerr, lng, color, pdf);
     light_tri lt;
     load_light(sc, l, &lt);
     v3 gn = cross3(sub3(lt.p1, lt.p0), sub3(lt.p2, lt.p0));
@@ -1157,6 +1276,7 @@ static int ray_tri_light(v3 o, v3 d, v3 p0, v3 p1, v3 p2, v3 *bary)
 static v3 light_eval_li(const or_scene *sc, const or_shape_light *l, v3 pos, v3 wi, v3 *lpos, v3 *lerr,
                         v3 *lng, float *pdf)
 {
+    if (l->triangle == OR_EMPTY) return sphere_light_eval_li(sc, l, pos, wi, lpos, lerr, lng, pdf);
     light_tri lt;
     load_light(sc, l, &lt);
     v3 gn = cross3(sub3(lt.p1, lt.p0), sub3(lt.p2, lt.p0));
@@ -1195,6 +1315,121 @@ typedef struct {
     rng_ctx *rng;
     uint64_t *counts;
 } path_ctx;
+
+/* DistantLight::SampleLi (src/Light.h:361-385) */
+static int distant_sample_li(const or_scene *sc, uint32_t i, float u0, float u1, v3 *wi, v3 *color, float *pdf)
+{
+    const ymrt_distant_light_desc *dl = &sc->distant[i];
+    float ctm = sc->distant_cos_max[i];
+    float cone = 2.0f * OR_PI * (1.0f - ctm);
+    v3 cs;
+    if (cone < 1E-10f) {
+        cs = mk3(0.0f, 1.0f, 0.0f);
+    } else {
+        cs = sample_cone(u0, u1, ctm);
+        cs = mk3(cs.x, cs.z, cs.y);
+    }
+    *wi = neg3(normalize3(transform_vector(dl->object_to_world, cs)));
+    *pdf = 1.0f / fmaxr(cone, 1E-10f);
+    *color = scl3(mk3(dl->light_color[0], dl->light_color[1], dl->light_color[2]), dl->intensity);
+    return 1;
+}
+
+/* DistantLight::EvalLi (src/Light.h:344-358): radiance times the cone pdf */
+static v3 distant_eval_li(const or_scene *sc, uint32_t i, v3 wi, float *pdf)
+{
+    const ymrt_distant_light_desc *dl = &sc->distant[i];
+    v3 lw = transform_vector(dl->world_to_object, wi);
+    float ct = -lw.y;
+    float ctm = sc->distant_cos_max[i];
+    if (ct < ctm) { *pdf = 0.0f; return s3(0.0f); }
+    float cone = 2.0f * OR_PI * (1.0f - ctm);
+    *pdf = 1.0f / cone;
+    return scl3(scl3(mk3(dl->light_color[0], dl->light_color[1], dl->light_color[2]), dl->intensity), *pdf);
+}
+
+static uint32_t pick_distant(const or_scene *sc, float u)
+{
+    int k = (int)(u * (float)sc->distant_count);
+    if (k > (int)sc->distant_count - 1) k = (int)sc->distant_count - 1;
+    if (k < 0) k = 0;
+    return (uint32_t)k;
+}
+
+/* EvalDistantLight, src/Shading.cu:83-198 */
+static v3 eval_distant_light(path_ctx *c, mbsdf *b, v3 ray_d, const or_hit *h, const or_transfer *t,
+                             int nc, const or_nearby *nb, v3 pos, v3 perr, v3 ng)
+{
+    const or_scene *sc = c->sc;
+    if (sc->distant_count == 0) return s3(0.0f);
+    v3 direct = s3(0.0f);
+    v3 wo = to_local(b, neg3(ray_d));
+    float sel_pdf = 1.0f / (float)sc->distant_count;
+    uint32_t li = pick_distant(sc, or_sampler_next(c->s));
+    {
+        v3 ldir = s3(0.0f), Li = s3(0.0f);
+        float lpdf = 0.0f;
+        float u0 = or_sampler_next(c->s);
+        float u1 = or_sampler_next(c->s);
+        int valid = distant_sample_li(sc, li, u0, u1, &ldir, &Li, &lpdf);
+        v3 lwi = to_local(b, ldir);
+        float wpdf = 0.0f;
+        v3 f = bsdf_eval(b, wo, lwi, &wpdf);
+        if (maxcomp3(f) > OR_MIN_COLOR_EPSILON && valid) {
+            v3 so = or_offset_ray_origin(pos, perr, ldir, ng);
+            or_ray sr = make_ray(so, ldir, OR_EMPTY, OR_EMPTY);
+            int transmit = dot3(neg3(ray_d), ng) * dot3(ldir, ng) < 0.0f;
+            int invalid = (!same_hemi(wo, lwi) && !transmit) || (same_hemi(wo, lwi) && transmit);
+            if (!invalid) {
+                if (c->counts) c->counts[2]++;
+                if (!or_trace_shadow(sc, &sr)) {
+                    v3 tr = s3(1.0f);
+                    if (c->rs->enable_volume_scattering) {
+                        or_transfer st = *t;
+                        if (transmit) boundary_bunch(sc, &st, nb, nc);
+                        sr = make_ray(so, ldir, h->inst, h->tri);
+                        tr = trace_tr(sc, &sr, &st, c->rng, c->counts);
+                    }
+                    float mis = power_heuristic(lpdf, wpdf);
+                    direct = add3(direct, mul3(mul3(sclf3(mis, tr), clampv(f)), clampv(Li)));
+                }
+            }
+        }
+    }
+    {
+        v3 bw = s3(0.0f), bwi = s3(0.0f);
+        float bpdf = 0.0f;
+        float u0 = or_sampler_next(c->s);
+        float u1 = or_sampler_next(c->s);
+        float u2 = or_sampler_next(c->s);
+        if (bsdf_sample(b, wo, &bw, &bwi, &bpdf, u0, u1, u2)) {
+            v3 ldir = to_world(b, bwi);
+            float lpdf = 0.0f;
+            v3 Li = distant_eval_li(sc, li, ldir, &lpdf);
+            if (maxcomp3(Li) > OR_MIN_COLOR_EPSILON) {
+                v3 so = or_offset_ray_origin(pos, perr, ldir, ng);
+                or_ray sr = make_ray(so, ldir, OR_EMPTY, OR_EMPTY);
+                int transmit = dot3(neg3(ray_d), ng) * dot3(ldir, ng) < 0.0f;
+                int invalid = (!same_hemi(wo, bwi) && !transmit) || (same_hemi(wo, bwi) && transmit);
+                if (!invalid) {
+                    if (c->counts) c->counts[2]++;
+                    if (!or_trace_shadow(sc, &sr)) {
+                        v3 tr = s3(1.0f);
+                        if (c->rs->enable_volume_scattering) {
+                            or_transfer st = *t;
+                            if (transmit) boundary_bunch(sc, &st, nb, nc);
+                            sr = make_ray(so, ldir, h->inst, h->tri);
+                            tr = trace_tr(sc, &sr, &st, c->rng, c->counts);
+                        }
+                        float mis = power_heuristic(bpdf, lpdf);
+                        direct = add3(direct, mul3(mul3(sclf3(mis, tr), clampv(bw)), clampv(Li)));
+                    }
+                }
+            }
+        }
+    }
+    return divs3(direct, sel_pdf);
+}
 
 /* src/Shading.cu:294-417 */
 static v3 eval_shape_light(path_ctx *c, mbsdf *b, v3 ray_d, const or_hit *h, const or_transfer *t,
@@ -1277,6 +1512,59 @@ static v3 eval_shape_light(path_ctx *c, mbsdf *b, v3 ray_d, const or_hit *h, con
         }
     }
     return scl3(direct, safe_rcp(sel_pdf));
+}
+
+/* EvalVolumeDistantLight, src/Shading.cu:202-290 */
+static v3 eval_volume_distant(path_ctx *c, int n, const float *gs, const float *vw, const or_ray *ray,
+                              const or_transfer *t, v3 vpos)
+{
+    const or_scene *sc = c->sc;
+    if (sc->distant_count == 0 || n == 0) return s3(0.0f);
+    v3 direct = s3(0.0f);
+    float sel_pdf = 1.0f / (float)sc->distant_count;
+    uint32_t li = pick_distant(sc, or_sampler_next(c->s));
+    {
+        v3 ldir = s3(0.0f), Li = s3(0.0f);
+        float lpdf = 0.0f;
+        float u0 = or_sampler_next(c->s);
+        float u1 = or_sampler_next(c->s);
+        int valid = distant_sample_li(sc, li, u0, u1, &ldir, &Li, &lpdf);
+        float wpdf = 0.0f;
+        float ph = mixed_phase_eval(vw, gs, n, neg3(ray->d), ldir, &wpdf);
+        if (ph > OR_MIN_COLOR_EPSILON && valid) {
+            or_ray sr = make_ray(vpos, ldir, OR_EMPTY, OR_EMPTY);
+            if (c->counts) c->counts[2]++;
+            if (!or_trace_shadow(sc, &sr)) {
+                or_transfer st = *t;
+                sr = make_ray(vpos, ldir, OR_EMPTY, OR_EMPTY);
+                v3 tr = trace_tr(sc, &sr, &st, c->rng, c->counts);
+                float mis = power_heuristic(lpdf, wpdf);
+                direct = add3(direct, mul3(scl3(sclf3(mis, tr), fminr(ph, OR_MAX_COLOR_CLAMP)), clampv(Li)));
+            }
+        }
+    }
+    {
+        v3 pwi = s3(0.0f);
+        float ppdf = 0.0f, pw = 0.0f;
+        float u0 = or_sampler_next(c->s);
+        float u1 = or_sampler_next(c->s);
+        if (mixed_phase_sample(vw, gs, n, u0, u1, neg3(ray->d), &pw, &pwi, &ppdf)) {
+            float lpdf = 0.0f;
+            v3 Li = distant_eval_li(sc, li, pwi, &lpdf);
+            if (maxcomp3(Li) > OR_MIN_COLOR_EPSILON) {
+                or_ray sr = make_ray(vpos, pwi, OR_EMPTY, OR_EMPTY);
+                if (c->counts) c->counts[2]++;
+                if (!or_trace_shadow(sc, &sr)) {
+                    or_transfer st = *t;
+                    sr = make_ray(vpos, pwi, OR_EMPTY, OR_EMPTY);
+                    v3 tr = trace_tr(sc, &sr, &st, c->rng, c->counts);
+                    float mis = power_heuristic(ppdf, lpdf);
+                    direct = add3(direct, mul3(scl3(sclf3(mis, tr), fminr(pw, OR_MAX_COLOR_CLAMP)), clampv(Li)));
+                }
+            }
+        }
+    }
+    return divs3(direct, sel_pdf);
 }
 
 /* src/Shading.cu:419-513 */
@@ -1363,7 +1651,35 @@ static void fetch_shading(const or_scene *sc, const or_hit *h, v3 *p, v3 *perr, 
                           v3 *dpdu, v3 *dpdv)
 {
     const ymrt_instance_desc *in = &sc->instances[h->inst];
-    const or_mesh *m = &sc->meshes[in->mesh];
+    if (in->geometry_type == YMRT_GEOMETRY_SPHERE) {
+        /* FetchSphereShadingData, src/ShadingUtilities.cuh:636-711 */
+        float radius = fmaxr(sc->spheres[in->geometry].radius, OR_FLOAT_EPSILON);
+        v3 po = h->bary;
+        v3 no = divs3(po, radius);
+        *p = po;
+        *perr = sclf3(error_gamma(5), abs3(po));
+        *sn = no;
+        *ng = no;
+        float theta = acosf(no.y);
+        float phi = atan2f(no.z, no.x);
+        if (phi < 0.0f) phi += OR_TWO_PI;
+        uv->x = 0.5f * phi * OR_INV_PI;
+        uv->y = theta * OR_INV_PI;
+        float sin_theta = 0.0f;
+        if (1.0f - no.y * no.y > 0.0f) sin_theta = sqrtf(1.0f - no.y * no.y);
+        float cos_phi = 0.0f, sin_phi = 0.0f;
+        if (sin_theta > 0.0f) { cos_phi = no.x / sin_theta; sin_phi = no.z / sin_theta; }
+        else { cos_phi = 1.0f; sin_phi = 0.0f; }
+        *dpdu = scl3(mk3(-po.z, 0.0f, po.x), OR_TWO_PI);
+        *dpdv = scl3(mk3(po.y * cos_phi, -radius * sin_theta, sin_phi * po.y), OR_PI);
+        *p = transform_position(in->object_to_world, *p, perr);
+        *sn = normalize3(transpose_transform_vector(in->world_to_object, *sn));
+        *ng = normalize3(transpose_transform_vector(in->world_to_object, *ng));
+        *dpdu = transform_vector(in->object_to_world, *dpdu);
+        *dpdv = transform_vector(in->object_to_world, *dpdv);
+        return;
+    }
+    const or_mesh *m = &sc->meshes[in->geometry];
     uint32_t t = h->tri;
     v3 p0 = ld3(m->pos, m->pidx[3 * t]), p1 = ld3(m->pos, m->pidx[3 * t + 1]), p2 = ld3(m->pos, m->pidx[3 * t + 2]);
     v3 b = h->bary;
@@ -1468,6 +1784,8 @@ v3 or_trace_path(const or_scene *sc, const ymrt_render_settings *rs, int frame, 
         rng.hash_ctr = 0;
         if (rs->enable_volume_scattering && h.volume) {
             v3 vpos = h.bary;
+            if (rs->enable_distant_light)
+                L = add3(L, light_clamp(rs, mul3(beta, eval_volume_distant(&c, nv, gs, vw, &ray, &t, vpos)), depth));
             if (rs->enable_shape_light)
                 L = add3(L, light_clamp(rs, mul3(beta, eval_volume_light(&c, nv, gs, vw, &ray, &t, vpos)), depth));
             v3 wi = s3(0.0f);
@@ -1510,6 +1828,8 @@ v3 or_trace_path(const or_scene *sc, const ymrt_render_settings *rs, int frame, 
         init_shading_space(&b, h.back, ray.d, ng, sn, dpdu, dpdv);
         b.is_default = m->type == YMRT_MATERIAL_DEFAULT;
         if (b.is_default) init_default_bsdf(sc, &b, m, uv, ray.d, h.back, tr_ray_ior(&t), tr_ex_ior(&t, h.inst));
+        if (rs->enable_distant_light)
+            L = add3(L, light_clamp(rs, mul3(beta, eval_distant_light(&c, &b, ray.d, &h, &t, nc, nb, p, perr, ng)), depth));
         if (rs->enable_shape_light)
             L = add3(L, light_clamp(rs, mul3(beta, eval_shape_light(&c, &b, ray.d, &h, &t, nc, nb, p, perr, ng)), depth));
         v3 wo = to_local(&b, neg3(ray.d)), wi = s3(0.0f), bw = s3(0.0f);

@@ -136,6 +136,8 @@ typedef struct {
 typedef struct {
     uint32_t mesh_count;
     or_mesh *meshes;
+    uint32_t sphere_count;
+    const ymrt_sphere_desc *spheres;
     uint32_t instance_count;
     const ymrt_instance_desc *instances; /* original order */
     uint32_t *inst_order;   /* top-leaf order -> original instance index */
@@ -149,6 +151,9 @@ typedef struct {
     or_grid *grids;
     uint32_t volume_count;
     const ymrt_volume_desc *volumes;
+    uint32_t distant_count;
+    const ymrt_distant_light_desc *distant;
+    float *distant_cos_max;   /* cos(radians(theta_max)) per distant light */
     uint32_t light_count;
     or_shape_light *lights;
     float *light_cdf;
@@ -185,6 +190,7 @@ void or_scene_destroy(or_scene *sc);
 
 int or_trace_ray(const or_scene *sc, or_ray *ray, or_hit *hit, int *nearby_count, or_nearby *nearby);
 int or_trace_shadow(const or_scene *sc, or_ray *ray);
+int or_sphere_any(float radius, or_ray *object_ray);
 int or_trace_volume(const or_scene *sc, or_ray *ray, or_hit *hit, int *nearby_count, or_nearby *nearby);
 int or_intersect_triangle(const v3 p0, const v3 p1, const v3 p2, const or_ray *ray,
                           float *t, v3 *bary, int *back);

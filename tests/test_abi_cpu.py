@@ -27,5 +27,5 @@ def test_library_exports_every_symbol():
 
 def test_abi_version_and_error_string():
     L = ymrt_amd.lib()
-    assert L.ymrt_abi_version() == 1
+    assert L.ymrt_abi_version() == 2
     assert isinstance(L.ymrt_last_error(), bytes)

@@ -123,3 +123,19 @@ def test_cornell_medium_parity(gpu_ctx, grid):
     same, err = _media_report("cornell medium grid=%d" % grid, g, o, stats)
     assert same >= 0.50
     assert err <= 0.02
+
+
+@pytest.mark.parametrize("kw", [dict(analytic_sphere=True), dict(sphere_light=True),
+                                dict(analytic_sphere=True, sphere_light=True)],
+                         ids=["glass-sphere", "sphere-light", "both"])
+def test_cornell_sphere_parity(gpu_ctx, kw):
+    """Analytic spheres: a glass sphere (shading data, nested dielectric) and a
+    sphere light (cone sampling, EvalLi)."""
+    sa = scenes.cornell_box(48, 48, **kw)
+    rs = abi.render_settings(48, 48, 8, 6)
+    g, o, stats = render_both(gpu_ctx, sa, rs)
+    assert np.isfinite(g).all()
+    same, err = close_frac(g, o), rel_rmse(g, o)
+    print("cornell %s: close pixels %.4f  rel rmse %.5f" % (kw, same, err))
+    assert same >= 0.90
+    assert err <= 0.01

@@ -131,3 +131,28 @@ def test_stress_full_size(gpu_ctx):
     seg_hits = ymrt_amd.Scene.hits_to_numpy(sc.trace_closest(ymrt_amd.Scene.rays_to_device(shadow)))
     inside = (seg_hits["instance"] != 0xFFFFFFFF)
     assert (occ == inside).mean() >= 0.99999
+
+
+def test_closest_spheres_and_meshes(gpu_ctx):
+    """Analytic sphere instances (scaled, rotated, some boundaries) mixed with
+    meshes: closest-hit ids and t, any-hit flags."""
+    sa = random_instances(24, seed=13)
+    rng = np.random.RandomState(14)
+    mat = sa.add_material(abi.default_material())
+    for i in range(12):
+        a = rng.normal(size=(3, 3))
+        q, _ = np.linalg.qr(a)
+        m = np.eye(4)
+        m[:3, :3] = q * rng.uniform(0.3, 1.2)
+        m[:3, 3] = rng.uniform(-4, 4, 3)
+        sa.add_instance(sa.add_sphere(float(rng.uniform(0.3, 1.0))), mat, m.astype(np.float32),
+                        boundary=(i % 5 == 2))
+    closest, shadow = scenes.stress_rays(sa, 100000, 15, 16, 17)
+    sc, gh = run_closest(gpu_ctx, sa, closest)
+    osc = oracle.Scene(sa)
+    oh = osc.trace_closest(closest)
+    compare_closest(gh, oh)
+    sphere_hits = (oh["instance"] != 0xFFFFFFFF) & (oh["triangle"] == 0xFFFFFFFF)
+    assert sphere_hits.sum() > 3000
+    occ = sc.trace_any(ymrt_amd.Scene.rays_to_device(shadow)).cpu().numpy()
+    assert (occ == osc.trace_any(shadow)).mean() >= 0.9999

@@ -401,3 +401,44 @@ def test_constant_grid_equals_homogeneous():
     mh = blocks[False, 1].mean() + blocks[False, 2].mean()
     mg = blocks[True, 1].mean() + blocks[True, 2].mean()
     assert abs(mh - mg) < 0.01 * mh
+
+
+def test_direct_light_under_sphere_emitter():
+    """Lambert floor (albedo 0.5) under a sphere light of radius R centred at
+    height D: E = pi * Le * (R / D)^2, so L = albedo * Le * (R / D)^2.  Cone
+    sampling, the sphere EvalLi and MIS all enter."""
+    sa = abi.SceneArrays()
+    floor = sa.add_mesh(*scenes.quad((-50, 0, -50), (-50, 0, 50), (50, 0, 50), (50, 0, -50))[:2])
+    sa.add_instance(floor, sa.add_material(_lambert((0.5, 0.5, 0.5))))
+    m = np.eye(4, dtype=np.float32)
+    m[:3, 3] = (0.0, 1.0, 0.0)
+    sa.add_instance(sa.add_sphere(0.2), sa.add_material(abi.light_material((1.0, 1.0, 1.0), 1.0)), m)
+    sa.set_camera((0.0, 0.5, 0.0001), (0.0, 0.0, 0.0), 0.2, 1.0)
+    sc = oracle.Scene(sa)
+    img = sc.render(abi.render_settings(8, 8, 512, 1, sampler=abi.SAMPLER_PCG), 1, threads=8)
+    sc.close()
+    expect = 0.5 * 1.0 * (0.2 / 1.0) ** 2
+    got = float(img[..., :3].mean())
+    assert abs(got - expect) < 0.01 * expect, (got, expect)
+
+
+def test_direct_light_distant_cone():
+    """Lambert floor (albedo 0.5) under a distant light of intensity 1 whose
+    cone (half-angle 30 degrees) is centred on the normal.  The reference's
+    DistantLight returns Le from SampleLi without dividing by the cone pdf and
+    Le * pdf from EvalLi (src/Light.h:344-385), so with MIS the estimate is the
+    cone average of f * Le: L = albedo / pi * Le * <cos> with
+    <cos> = (1 + cos(30)) / 2 over the cone."""
+    sa = abi.SceneArrays()
+    floor = sa.add_mesh(*scenes.quad((-50, 0, -50), (-50, 0, 50), (50, 0, 50), (50, 0, -50))[:2])
+    sa.add_instance(floor, sa.add_material(_lambert((0.5, 0.5, 0.5))))
+    flip = np.diag([1.0, -1.0, -1.0, 1.0]).astype(np.float32)   # light frame's -y points up
+    sa.add_distant_light((1.0, 1.0, 1.0), 1.0, 30.0, flip)
+    sa.set_camera((0.0, 0.5, 0.0001), (0.0, 0.0, 0.0), 0.2, 1.0)
+    sc = oracle.Scene(sa)
+    rs = abi.render_settings(8, 8, 512, 1, sampler=abi.SAMPLER_PCG, shape_light=0, distant_light=1)
+    img = sc.render(rs, 1, threads=8)
+    sc.close()
+    expect = 0.5 / math.pi * (1.0 + math.cos(math.radians(30.0))) / 2.0
+    got = float(img[..., :3].mean())
+    assert abs(got - expect) < 0.01 * expect, (got, expect)

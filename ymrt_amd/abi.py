@@ -32,10 +32,18 @@ class MeshDesc(C.Structure):
     ]
 
 
+class SphereDesc(C.Structure):
+    _fields_ = [("radius", C.c_float)]
+
+
+GEOMETRY_MESH, GEOMETRY_SPHERE = 0, 1
+
+
 class InstanceDesc(C.Structure):
     _fields_ = [
-        ("mesh", C.c_uint32), ("material", C.c_uint32),
+        ("geometry", C.c_uint32), ("material", C.c_uint32),
         ("inner_volume", C.c_int32), ("treat_as_boundary", C.c_int32),
+        ("geometry_type", C.c_uint32), ("reserved", C.c_uint32),
         ("object_to_world", C.c_float * 16), ("world_to_object", C.c_float * 16),
     ]
 
@@ -86,14 +94,23 @@ class CameraDesc(C.Structure):
                 ("aspect_ratio", C.c_float)]
 
 
+class DistantLightDesc(C.Structure):
+    _fields_ = [
+        ("light_color", C.c_float * 3), ("intensity", C.c_float), ("theta_max", C.c_float),
+        ("object_to_world", C.c_float * 16), ("world_to_object", C.c_float * 16),
+    ]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [
         ("mesh_count", C.c_uint32), ("meshes", C.POINTER(MeshDesc)),
+        ("sphere_count", C.c_uint32), ("spheres", C.POINTER(SphereDesc)),
         ("instance_count", C.c_uint32), ("instances", C.POINTER(InstanceDesc)),
         ("material_count", C.c_uint32), ("materials", C.POINTER(MaterialDesc)),
         ("texture_count", C.c_uint32), ("textures", C.POINTER(TextureDesc)),
         ("grid_count", C.c_uint32), ("grids", C.POINTER(GridDesc)),
         ("volume_count", C.c_uint32), ("volumes", C.POINTER(VolumeDesc)),
+        ("distant_light_count", C.c_uint32), ("distant_lights", C.POINTER(DistantLightDesc)),
         ("camera", CameraDesc),
         ("halton_seed", C.c_uint32),
     ]
@@ -113,7 +130,8 @@ class RenderSettings(C.Structure):
     _fields_ = [
         ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_int32),
         ("ray_depth", C.c_int32), ("sampler_type", C.c_int32),
-        ("enable_shape_light", C.c_int32), ("enable_volume_scattering", C.c_int32),
+        ("enable_shape_light", C.c_int32), ("enable_distant_light", C.c_int32),
+        ("enable_volume_scattering", C.c_int32),
         ("enable_env_light", C.c_int32), ("enable_russian_roulette", C.c_int32),
         ("enable_indirect_clamp", C.c_int32), ("indirect_limit", C.c_float),
         ("gamma", C.c_float), ("exposure", C.c_float), ("path_capacity", C.c_uint32),
@@ -132,6 +150,8 @@ class FrameStats(C.Structure):
         ("ms_medium", C.c_float), ("launches_medium", C.c_uint32),
         ("extend_nodes", C.c_uint64), ("extend_tris", C.c_uint64),
         ("shadow_nodes", C.c_uint64), ("shadow_tris", C.c_uint64),
+        ("ms_transmit", C.c_float), ("launches_transmit", C.c_uint32),
+        ("transmit_nodes", C.c_uint64), ("transmit_tris", C.c_uint64),
     ]
 
 
@@ -189,6 +209,10 @@ def light_material(color=(1.0, 1.0, 1.0), intensity=1.0):
     return m
 
 
+class SphereRef(int):
+    """Index of a sphere in SceneArrays.spheres (distinguishes it from a mesh index)."""
+
+
 class SceneArrays:
     """Numpy-backed scene; ``desc()`` returns a SceneDesc pointing into it.
 
@@ -200,11 +224,13 @@ class SceneArrays:
 
     def __init__(self):
         self.meshes = []
+        self.spheres = []
         self.instances = []
         self.materials = []
         self.textures = []
         self.grids = []
         self.volumes = []
+        self.distant_lights = []
         self.camera = dict(position=(0, 0, 0), w=(0, 0, 1), fov=np.radians(45.0), aspect=1.0)
         self.halton_seed = 0
         self._keep = []
@@ -226,6 +252,12 @@ class SceneArrays:
         self.materials.append(m)
         return len(self.materials) - 1
 
+    def add_sphere(self, radius):
+        """Analytic sphere of the given radius about the object-space origin;
+        pass the returned handle to add_instance like a mesh index."""
+        self.spheres.append(float(radius))
+        return SphereRef(len(self.spheres) - 1)
+
     def add_instance(self, mesh, material, object_to_world=None, world_to_object=None,
                      inner_volume=-1, boundary=False):
         o2w = np.eye(4, dtype=np.float32) if object_to_world is None else np.asarray(object_to_world, np.float32)
@@ -235,7 +267,8 @@ class SceneArrays:
         world_to_object = np.array(world_to_object, np.float32)
         for m in (o2w, world_to_object):  # exact affine bottom row
             m[3, :] = (0.0, 0.0, 0.0, 1.0)
-        self.instances.append(dict(mesh=mesh, material=material, o2w=o2w,
+        gtype = GEOMETRY_SPHERE if isinstance(mesh, SphereRef) else GEOMETRY_MESH
+        self.instances.append(dict(mesh=int(mesh), geometry_type=gtype, material=material, o2w=o2w,
                                    w2o=np.asarray(world_to_object, np.float32),
                                    inner_volume=inner_volume, boundary=int(boundary)))
         return len(self.instances) - 1
@@ -259,6 +292,17 @@ class SceneArrays:
         self.volumes.append(dict(sigma_t=sigma_t, albedo=albedo, g=g, grid=grid, w2o=w2o))
         return len(self.volumes) - 1
 
+    def add_distant_light(self, color, intensity=1.0, theta_max_deg=5.0, object_to_world=None):
+        """Distant light shining along -y of its frame (object_to_world)."""
+        o2w = np.eye(4, dtype=np.float32) if object_to_world is None else np.asarray(object_to_world, np.float32)
+        o2w = o2w.copy()
+        o2w[3, :] = (0.0, 0.0, 0.0, 1.0)
+        w2o = np.linalg.inv(o2w.astype(np.float64)).astype(np.float32)
+        w2o[3, :] = (0.0, 0.0, 0.0, 1.0)
+        self.distant_lights.append(dict(color=tuple(float(c) for c in color), intensity=float(intensity),
+                                        theta=float(theta_max_deg), o2w=o2w, w2o=w2o))
+        return len(self.distant_lights) - 1
+
     def set_camera(self, position, look_at, fov_deg, aspect):
         p = np.asarray(position, np.float32)
         w = p - np.asarray(look_at, np.float32)
@@ -267,7 +311,8 @@ class SceneArrays:
 
     @property
     def triangle_count(self):
-        return int(sum(self.meshes[i["mesh"]]["pidx"].shape[0] for i in self.instances))
+        return int(sum(self.meshes[i["mesh"]]["pidx"].shape[0] for i in self.instances
+                       if i["geometry_type"] == GEOMETRY_MESH))
 
     # -- descriptor -----------------------------------------------------
     def desc(self):
@@ -290,7 +335,8 @@ class SceneArrays:
         inst = (InstanceDesc * max(1, len(self.instances)))()
         for i, it in enumerate(self.instances):
             d = inst[i]
-            d.mesh, d.material = it["mesh"], it["material"]
+            d.geometry, d.material = it["mesh"], it["material"]
+            d.geometry_type = it.get("geometry_type", GEOMETRY_MESH)
             d.inner_volume, d.treat_as_boundary = it["inner_volume"], it["boundary"]
             # numpy row-major M[r, c] -> column-major m[c*4 + r]
             d.object_to_world[:] = it["o2w"].T.reshape(-1).tolist()
@@ -319,28 +365,41 @@ class SceneArrays:
             d.world_to_object[:] = v["w2o"].T.reshape(-1).tolist()
         sd = SceneDesc()
         sd.mesh_count, sd.meshes = len(self.meshes), md
+        sph = (SphereDesc * max(1, len(self.spheres)))()
+        for i, r in enumerate(self.spheres):
+            sph[i].radius = float(r)
+        sd.sphere_count, sd.spheres = len(self.spheres), sph
         sd.instance_count, sd.instances = len(self.instances), inst
         sd.material_count, sd.materials = len(self.materials), mats
         sd.texture_count, sd.textures = len(self.textures), texs
         sd.grid_count, sd.grids = len(self.grids), grids
         sd.volume_count, sd.volumes = len(self.volumes), vols
+        dls = (DistantLightDesc * max(1, len(self.distant_lights)))()
+        for i, dl in enumerate(self.distant_lights):
+            d = dls[i]
+            d.light_color[:] = list(dl["color"])
+            d.intensity, d.theta_max = dl["intensity"], dl["theta"]
+            d.object_to_world[:] = dl["o2w"].T.reshape(-1).tolist()
+            d.world_to_object[:] = dl["w2o"].T.reshape(-1).tolist()
+        sd.distant_light_count, sd.distant_lights = len(self.distant_lights), dls
         cam = self.camera
         sd.camera.position[:] = [float(x) for x in cam["position"]]
         sd.camera.w[:] = [float(x) for x in cam["w"]]
         sd.camera.fov = cam["fov"]
         sd.camera.aspect_ratio = cam["aspect"]
         sd.halton_seed = self.halton_seed
-        keep.extend([md, inst, mats, texs, grids, vols])
+        keep.extend([md, sph, inst, mats, texs, grids, vols, dls])
         self._keep = keep
         return sd
 
 
 def render_settings(width, height, spp, depth, sampler=SAMPLER_HALTON, shape_light=1,
-                    volume=0, env_light=0, rr=0, path_capacity=0):
+                    volume=0, env_light=0, rr=0, path_capacity=0, distant_light=0):
     rs = RenderSettings()
     rs.width, rs.height, rs.spp, rs.ray_depth = width, height, spp, depth
     rs.sampler_type = sampler
     rs.enable_shape_light = shape_light
+    rs.enable_distant_light = distant_light
     rs.enable_volume_scattering = volume
     rs.enable_env_light = env_light
     rs.enable_russian_roulette = rr

@@ -92,8 +92,21 @@ YM_HD float radical_inverse_plain(uint64_t index)
     return gmax(0.000001f, gmin((float)radical_sum * base_multiply, 0.999999f));
 }
 
-// Permuted radical inverse of dimension dim >= 2 (base = dim-th prime).
-__device__ __forceinline__ float radical_inverse(const DevScene &sc, uint64_t index, uint32_t dim)
+// Quotient of a 32-bit n by the prime whose magic is ceil(2^64 / p): the high
+// word of n * magic.  Exact because n * (magic - 2^64/p) / 2^64 < 2^-32 is
+// smaller than the gap 1/p between n/p and the next integer.
+YM_HD uint32_t div_magic(uint32_t n, uint64_t magic)
+{
+    const uint64_t lo = (uint64_t)n * (uint32_t)magic;
+    const uint64_t hi = (uint64_t)n * (uint32_t)(magic >> 32) + (lo >> 32);
+    return (uint32_t)(hi >> 32);
+}
+
+// Permuted radical inverse of dimension dim >= 2 (base = dim-th prime),
+// HaltonSampler::RadicalInversion (src/RandomSampler.cuh:442-472).  Indices
+// below 2^32 divide by the base with a multiply-high by ceil(2^64 / base)
+// instead of a 64-bit division.
+YM_HD float radical_inverse(const DevScene &sc, uint64_t index, uint32_t dim)
 {
     const uint32_t base = sc.primes[dim];
     const uint32_t *perm = sc.halton + sc.prime_sums[dim];
@@ -101,9 +114,10 @@ __device__ __forceinline__ float radical_inverse(const DevScene &sc, uint64_t in
     uint64_t radical_sum = 0;
     float base_multiply = 1.0f;
     if (index <= 0xFFFFFFFFull) {
+        const uint64_t magic = sc.prime_magic[dim];
         uint32_t idx = (uint32_t)index;
         while (idx) {
-            const uint32_t next = idx / base;
+            const uint32_t next = div_magic(idx, magic);
             const uint32_t digit = idx - next * base;
             radical_sum = radical_sum * base + perm[digit];
             base_multiply *= inv_base;

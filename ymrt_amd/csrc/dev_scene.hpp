@@ -44,12 +44,13 @@ struct alignas(16) DevInstance {  // 128 B
     uint32_t flags;    // bit 0: treat_as_boundary
     uint32_t shade_base;   // first shading triangle of the mesh
     uint32_t tri_count;
-    uint32_t pad;
+    float radius;          // sphere instances
 };
 static_assert(sizeof(DevInstance) == 128, "DevInstance size");
 
 constexpr uint32_t kInstBoundary = 1u;
 constexpr uint32_t kInstNoNormals = 2u;   // mesh without normals: shade with the face normal
+constexpr uint32_t kInstSphere = 4u;      // analytic sphere of DevInstance::radius, no bottom-level tree
 
 struct alignas(16) ShadeTri {      // 96 B, by original triangle index
     float p[9];                    // p0, p1, p2
@@ -104,6 +105,13 @@ struct DevLight {
     float power, area;
 };
 
+struct DevDistant {
+    f3 color;                      // light_color
+    float intensity;
+    float cos_max;                 // cos(radians(theta_max))
+    Affine o2w, w2o;               // the light's frame
+};
+
 struct CameraRecord {
     uint32_t prim;
     float ior;
@@ -133,14 +141,18 @@ struct DevScene {
     const DevVolume *volumes;
     const DevLight *lights;
     const float *light_cdf;
+    const DevDistant *distant;
     const uint32_t *halton;        // permutation table
     const uint32_t *primes;        // first 1000 primes
     const uint32_t *prime_sums;    // offset of each prime's permutation
+    const uint64_t *prime_magic;   // ceil(2^64 / prime): exact division of 32-bit values
     uint32_t tlas_root;            // all instances: node index, leaf code, or kSentinel
     uint32_t tlas_root_solid;      // instances that are not medium boundaries (shadow rays)
     uint32_t tlas_root_boundary;   // medium boundaries only (transmittance walk)
     uint32_t instance_count, material_count, texture_count, grid_count, volume_count, light_count;
+    uint32_t distant_count;
     uint32_t has_boundaries;       // any instance with treat_as_boundary
+    uint32_t has_spheres;          // any sphere instance
     DevCamera camera;
 };
 

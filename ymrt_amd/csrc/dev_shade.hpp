@@ -1015,10 +1015,112 @@ __device__ __forceinline__ LightTri load_light(const DevScene &sc, const DevLigh
 
 YM_HD f3 interp3(f3 a, f3 b, f3 c, f3 w) { return a * w.x + b * w.y + c * w.z; }
 
+// RayIntersectSphere (src/Light.h:42-65): the discriminant is formed in
+// double precision as the reference's 4.0 literal makes it.
+YM_HD bool ray_sphere_light(f3 o, f3 d, float radius, float &t0, float &t1)
+{
+    const float a = d.x * d.x + d.y * d.y + d.z * d.z;
+    const float b = 2.0f * (d.x * o.x + d.y * o.y + d.z * o.z);
+    const float c = o.x * o.x + o.y * o.y + o.z * o.z - radius * radius;
+    const float disc = (float)((double)(b * b) - 4.0 * (double)a * (double)c);
+    if (disc < 0.0f) return false;
+    const float sq = sqrtf(disc);
+    const float ia = 1.0f / a;
+    t0 = (-b - sq) * 0.5f * ia;
+    t1 = (-b + sq) * 0.5f * ia;
+    if (t1 < 1E-10f) return false;
+    return true;
+}
+
+// SampleCone (src/SampleUtilities.h:33-39)
+YM_HD f3 sample_cone(float u0, float u1, float cos_theta_max)
+{
+    const float ct = (1.0f - u0) + u0 * cos_theta_max;
+    const float st = sqrtf(gmax(0.0f, 1.0f - ct * ct));
+    const float phi = u1 * 2.0f * kPi;
+    return f3{cosf(phi) * st, sinf(phi) * st, ct};
+}
+
+YM_HD f3 light_emission(const DevScene &sc, const DevInstance &in)
+{
+    const DevMaterial &m = sc.materials[in.material];
+    return f3{m.light_color[0], m.light_color[1], m.light_color[2]} * m.intensity;
+}
+
+YM_HD f3 affine_translate(const Affine &m) { return f3{m.r[3], m.r[7], m.r[11]}; }
+
+// ShapeLight::SampleLi, sphere branch (src/Light.h:282-317): a direction in
+// the cone the sphere subtends, uniform in solid angle.
+__device__ __forceinline__ bool sphere_light_sample_li(const DevScene &sc, const DevLight &l, f3 pos, float u0,
+                                                       float u1, f3 &wi, f3 &lpos, f3 &lerr, f3 &lng, f3 &color,
+                                                       float &pdf)
+{
+    const DevInstance &in = sc.instances[l.instance];
+    const f3 opos = xf_position(in.w2o, pos, nullptr);
+    const float lpo = length(opos);
+    const float radius = in.radius;
+    if (lpo < radius) {
+        pdf = 0.0f;
+        return false;
+    }
+    const float ctm = sqrtf(gmax(sqr(lpo) - sqr(radius), 0.0f)) / lpo;
+    const float cone = 2.0f * kPi * (1.0f - ctm);
+    f3 cs = sample_cone(u0, u1, ctm);
+    f3 w = -opos, u, v;
+    orthogonal_basis(w, u, v);
+    cs = cs.x * u + cs.y * v + cs.z * w;
+    float t0 = 0.0f, t1 = 0.0f;
+    ray_sphere_light(opos, cs, radius, t0, t1);
+    f3 ip = opos + cs * t0;
+    const float lio = length(ip);
+    ip = ip * (radius * safe_rcp(lio));
+    f3 ie = error_gamma(5) * abs3(ip);
+    lpos = xf_position(in.o2w, ip, &ie);
+    lerr = ie;
+    lng = normalize(lpos - affine_translate(in.o2w));
+    wi = normalize(xf_vector(in.o2w, cs));
+    pdf = 1.0f / cone;
+    color = light_emission(sc, in) * cone;
+    return true;
+}
+
+// ShapeLight::EvalLi, sphere branch (src/Light.h:163-199)
+__device__ __forceinline__ f3 sphere_light_eval_li(const DevScene &sc, const DevLight &l, f3 pos, f3 wi, f3 &lpos,
+                                                   f3 &lerr, f3 &lng, float &pdf)
+{
+    const DevInstance &in = sc.instances[l.instance];
+    const f3 owi = xf_vector(in.w2o, wi);
+    const f3 opos = xf_position(in.w2o, pos, nullptr);
+    const float lpo = length(opos);
+    const float radius = in.radius;
+    float t0 = 0.0f, t1 = 0.0f;
+    if (lpo < radius || !ray_sphere_light(opos, owi, radius, t0, t1)) {
+        pdf = 0.0f;
+        return s3(0.0f);
+    }
+    const float ctm = sqrtf(gmax(sqr(lpo) - sqr(radius), 0.0f)) / lpo;
+    const float cone = 2.0f * kPi * (1.0f - ctm);
+    const float ct = dot(normalize(owi), -opos / lpo);
+    if (ct < ctm) {
+        pdf = 0.0f;
+        return s3(0.0f);
+    }
+    f3 ip = opos + owi * t0;
+    const float lio = length(ip);
+    ip = ip * (radius * safe_rcp(lio));
+    f3 ie = error_gamma(5) * abs3(ip);
+    lpos = xf_position(in.o2w, ip, &ie);
+    lerr = ie;
+    lng = normalize(lpos - affine_translate(in.o2w));
+    pdf = 1.0f / cone;
+    return light_emission(sc, in);
+}
+
 // ShapeLight::SampleLi, mesh branch (src/Light.h:207-281)
 __device__ __forceinline__ bool light_sample_li(const DevScene &sc, const DevLight &l, f3 pos, float u0, float u1,
                                                 f3 &wi, f3 &lpos, f3 &lerr, f3 &lng, f3 &color, float &pdf)
 {
+    if (l.triangle == kEmpty) return sphere_light_sample_li(sc, l, pos, u0, u1, wi, lpos, lerr, lng, color, pdf);
     const LightTri lt = load_light(sc, l);
     const f3 gn = cross(lt.p1 - lt.p0, lt.p2 - lt.p0);
     const float su = sqrtf(gmax(0.0f, u0));
@@ -1087,6 +1189,7 @@ YM_HD bool ray_tri_light(f3 o, f3 d, f3 p0, f3 p1, f3 p2, f3 &bary)
 __device__ __forceinline__ f3 light_eval_li(const DevScene &sc, const DevLight &l, f3 pos, f3 wi, f3 &lpos, f3 &lerr,
                                             f3 &lng, float &pdf)
 {
+    if (l.triangle == kEmpty) return sphere_light_eval_li(sc, l, pos, wi, lpos, lerr, lng, pdf);
     const LightTri lt = load_light(sc, l);
     const f3 gn = cross(lt.p1 - lt.p0, lt.p2 - lt.p0);
     const f3 owi = xf_vector(lt.in->w2o, wi);
@@ -1114,6 +1217,46 @@ __device__ __forceinline__ f3 light_eval_li(const DevScene &sc, const DevLight &
     const float dist = length(lpos - pos);
     pdf = sqr(dist) * safe_rcp(l.area * ct);
     return lt.le;
+}
+
+// DistantLight::SampleLi (src/Light.h:361-385): a direction uniform in the
+// cone; the returned radiance is not divided by the pdf (the reference's
+// distant light is normalised by its cone, see EvalLi).
+YM_HD void distant_sample_li(const DevDistant &dl, float u0, float u1, f3 &wi, f3 &color, float &pdf)
+{
+    const float cone = 2.0f * kPi * (1.0f - dl.cos_max);
+    f3 cs;
+    if (cone < 1E-10f) {
+        cs = f3{0.0f, 1.0f, 0.0f};
+    } else {
+        cs = sample_cone(u0, u1, dl.cos_max);
+        cs = f3{cs.x, cs.z, cs.y};
+    }
+    wi = -normalize(xf_vector(dl.o2w, cs));
+    pdf = 1.0f / gmax(cone, 1E-10f);
+    color = dl.color * dl.intensity;
+}
+
+// DistantLight::EvalLi (src/Light.h:344-358): radiance times the cone pdf
+YM_HD f3 distant_eval_li(const DevDistant &dl, f3 wi, float &pdf)
+{
+    const f3 lw = xf_vector(dl.w2o, wi);
+    const float ct = -lw.y;
+    if (ct < dl.cos_max) {
+        pdf = 0.0f;
+        return s3(0.0f);
+    }
+    const float cone = 2.0f * kPi * (1.0f - dl.cos_max);
+    pdf = 1.0f / cone;
+    return (dl.color * dl.intensity) * pdf;
+}
+
+// EvalDistantLight's light pick (src/Shading.cu:109-110)
+YM_HD uint32_t pick_distant(uint32_t count, float u)
+{
+    int k = (int)(u * (float)count);
+    k = k > (int)count - 1 ? (int)count - 1 : k;
+    return (uint32_t)(k < 0 ? 0 : k);
 }
 
 YM_HD float power_heuristic(float main_pdf, float vice_pdf)

@@ -95,6 +95,84 @@ YM_HD bool tri_back(f3 q0, f3 q1, f3 q2, f3 d)
     return dot(gn, d) > 0.0f;
 }
 
+// IntersectSphereObject (src/ShadingUtilities.cuh:219-275) in object space.
+// The reference divides the roots by 2.0 * a in double precision and rounds
+// to float; for a quotient of two floats that equals the correctly rounded
+// float quotient (53 >= 2 * 24 + 2 bits), which is what float division gives
+// here (2a is exact).  ex_* describe the first root at or beyond tmin whether
+// or not it is accepted (the nearby-hit bookkeeping needs it); on acceptance
+// t, the hit point on the sphere and the back-face flag are returned.
+struct SphereTest {
+    bool ex_hit;
+    float ex_t;
+    int ex_back;
+    float t;
+    f3 p;
+    int back;
+};
+
+YM_HD bool sphere_test(f3 o, f3 d, float radius, float tmin, float tmax, SphereTest &r)
+{
+    r.ex_hit = false;
+    const float a = d.x * d.x + d.y * d.y + d.z * d.z;
+    const float b = 2.0f * (d.x * o.x + d.y * o.y + d.z * o.z);
+    const float c = o.x * o.x + o.y * o.y + o.z * o.z - radius * radius;
+    const float disc = b * b - 4.0f * a * c;
+    if (disc < 0) return false;
+    const float sq = sqrtf(disc);
+    const float t1 = (-b - sq) / (2.0f * a);
+    const float t2 = (-b + sq) / (2.0f * a);
+    r.ex_hit = true;
+    r.ex_t = t1 >= tmin ? t1 : t2;
+    r.ex_back = dot(d, normalize(o + d * r.ex_t)) > 0.0f;
+    if (t1 > tmax || t2 < tmin) return false;
+    float t = t1;
+    if (t < tmin) {
+        t = t2;
+        if (t > tmax) return false;
+    }
+    f3 po = o + d * t;
+    const float dist = length(po);
+    po = dist > kFloatEpsilon ? (po * radius) / dist : s3(0.0f);
+    r.t = t;
+    r.p = po;
+    r.back = dot(d, po / radius) > 0.0f;
+    return true;
+}
+
+// Length of column c of an affine map's linear part: the scale an instance
+// applies along object axis c.
+YM_HD float sphere_axis_scale(const Affine &m, int c)
+{
+    return sqrtf(m.r[c] * m.r[c] + m.r[4 + c] * m.r[4 + c] + m.r[8 + c] * m.r[8 + c]);
+}
+
+// FetchGeometryNormal (src/ShadingUtilities.cuh:878-951): world position with
+// its rounding-error bound and the normalised geometric normal of a hit.  For
+// a sphere (b0, b1, b2) is the object-space hit point.
+YM_HD void hit_geometry(const DevScene &sc, uint32_t inst, uint32_t tri, float b0, float b1, float b2, f3 &p,
+                        f3 &perr, f3 &ng)
+{
+    const DevInstance &in = sc.instances[inst];
+    if (in.flags & kInstSphere) {
+        const float radius = gmax(in.radius, kFloatEpsilon);
+        p = f3{b0, b1, b2};
+        perr = error_gamma(5) * abs3(p);
+        ng = p / radius;
+    } else {
+        const ShadeTri &st = sc.shade[in.shade_base + tri];
+        const f3 p0{st.p[0], st.p[1], st.p[2]}, p1{st.p[3], st.p[4], st.p[5]}, p2{st.p[6], st.p[7], st.p[8]};
+        const float xs = fabsf(p0.x * b0) + fabsf(p1.x * b1) + fabsf(p2.x * b2);
+        const float ys = fabsf(p0.y * b0) + fabsf(p1.y * b1) + fabsf(p2.y * b2);
+        const float zs = fabsf(p0.z * b0) + fabsf(p1.z * b1) + fabsf(p2.z * b2);
+        p = p0 * b0 + p1 * b1 + p2 * b2;
+        perr = error_gamma(7) * f3{xs, ys, zs};
+        ng = cross(p1 - p0, p2 - p0);
+    }
+    p = xf_position(in.o2w, p, &perr);
+    ng = normalize(xf_transpose_vector(in.w2o, ng));
+}
+
 struct Query {
     f3 o, d;
     float tmin, tmax;
@@ -165,7 +243,9 @@ struct TraceCounters {
     } while (0)
 
 // Returns 1 on hit, 0 on miss, -1 on stack overflow (the ray is abandoned).
-template <int kMode, bool kAny, bool kNearby, bool kCount, int kStack, typename LS>
+// kSphere: the scene has sphere instances (their test is compiled in only
+// then; it costs registers in the traversal loop).
+template <int kMode, bool kAny, bool kNearby, bool kCount, bool kSphere, int kStack, typename LS>
 YM_HD int traverse(const DevScene &sc, const Query &q, Hit &hit, int &nb_count, LS &ls, TraceCounters &cnt)
 {
     const uint32_t root = kMode == kModeAll ? sc.tlas_root
@@ -186,6 +266,39 @@ YM_HD int traverse(const DevScene &sc, const Query &q, Hit &hit, int &nb_count, 
     int nbn = 0;
     int sp = 0;
     uint32_t node = root;
+    // nearby-list update when the traversal leaves an instance (TraceRay's
+    // per-instance block, src/ShadingUtilities.cuh:357-393)
+    auto nearby_exit = [&]() {
+        if (inst_hit) {
+            tub = next_float_up(next_float_up(tbest));
+            tlb = next_float_down(next_float_down(tbest));
+            int w = 0;
+            for (int i = 0; i < nbn; ++i) {
+                const float ti = ls.nb_t[ls.nbi(i)];
+                if (ti > tlb && ti < tub) {
+                    ls.nb_t[ls.nbi(w)] = ti;
+                    ls.nb_prim[ls.nbi(w)] = ls.nb_prim[ls.nbi(i)];
+                    ls.nb_back[ls.nbi(w)] = ls.nb_back[ls.nbi(i)];
+                    ++w;
+                }
+            }
+            nbn = w;
+            if (nbn < kNearbyCap) {
+                ls.nb_prim[ls.nbi(nbn)] = cur_inst;
+                ls.nb_t[ls.nbi(nbn)] = tbest;
+                ls.nb_back[ls.nbi(nbn)] = hit.back;
+                ++nbn;
+            }
+        } else if (found && ex_hit && ex_t > tlb && ex_t < tub) {
+            if (nbn < kNearbyCap) {
+                ls.nb_prim[ls.nbi(nbn)] = cur_inst;
+                ls.nb_t[ls.nbi(nbn)] = ex_t;
+                ls.nb_back[ls.nbi(nbn)] = ex_back;
+                ++nbn;
+            }
+        }
+        nb_count = nbn;
+    };
 
     for (;;) {
         if (!(node & kLeafBit)) {
@@ -238,7 +351,33 @@ YM_HD int traverse(const DevScene &sc, const Query &q, Hit &hit, int &nb_count, 
             bool take = true;
             if (kMode == kModeSkipBoundary) take = !(in.flags & kInstBoundary);
             if (kMode == kModeBoundaryOnly) take = (in.flags & kInstBoundary) != 0;
-            if (take) {
+            if (kSphere && take && (in.flags & kInstSphere)) {
+                SphereTest st;
+                const bool acc = sphere_test(xf_point(in.w2o, q.o), xf_dir(in.w2o, q.d), in.radius, q.tmin, tbest, st);
+                if (kCount) cnt.tris++;
+                if (acc) {
+                    tbest = st.t;
+                    found = true;
+                    hit.t = st.t;
+                    hit.inst = ii;
+                    hit.tri = kEmpty;
+                    if (!kAny) {
+                        hit.b0 = st.p.x;
+                        hit.b1 = st.p.y;
+                        hit.b2 = st.p.z;
+                        hit.back = st.back;
+                    }
+                    if (kAny) return 1;
+                }
+                if (kNearby) {
+                    cur_inst = ii;
+                    inst_hit = acc;
+                    ex_hit = st.ex_hit;
+                    ex_t = st.ex_t;
+                    ex_back = st.ex_back;
+                    nearby_exit();
+                }
+            } else if (take) {
                 if (sp + 1 > kStack) return -1;
                 ls.push(sp++, kSentinel);
                 pre = make_pre(xf_point(in.w2o, q.o), xf_dir(in.w2o, q.d));
@@ -292,37 +431,7 @@ YM_HD int traverse(const DevScene &sc, const Query &q, Hit &hit, int &nb_count, 
             // leaving an instance: back to world space
             pre = make_pre(q.o, q.d);
             in_blas = false;
-            if (kNearby) {
-                if (inst_hit) {
-                    tub = next_float_up(next_float_up(tbest));
-                    tlb = next_float_down(next_float_down(tbest));
-                    int w = 0;
-                    for (int i = 0; i < nbn; ++i) {
-                        const float ti = ls.nb_t[ls.nbi(i)];
-                        if (ti > tlb && ti < tub) {
-                            ls.nb_t[ls.nbi(w)] = ti;
-                            ls.nb_prim[ls.nbi(w)] = ls.nb_prim[ls.nbi(i)];
-                            ls.nb_back[ls.nbi(w)] = ls.nb_back[ls.nbi(i)];
-                            ++w;
-                        }
-                    }
-                    nbn = w;
-                    if (nbn < kNearbyCap) {
-                        ls.nb_prim[ls.nbi(nbn)] = cur_inst;
-                        ls.nb_t[ls.nbi(nbn)] = tbest;
-                        ls.nb_back[ls.nbi(nbn)] = hit.back;
-                        ++nbn;
-                    }
-                } else if (found && ex_hit && ex_t > tlb && ex_t < tub) {
-                    if (nbn < kNearbyCap) {
-                        ls.nb_prim[ls.nbi(nbn)] = cur_inst;
-                        ls.nb_t[ls.nbi(nbn)] = ex_t;
-                        ls.nb_back[ls.nbi(nbn)] = ex_back;
-                        ++nbn;
-                    }
-                }
-                nb_count = nbn;
-            }
+            if (kNearby) nearby_exit();
         }
     }
 }

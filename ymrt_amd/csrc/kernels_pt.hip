@@ -27,23 +27,63 @@ constexpr int kBlock = 64;
 // ------------------------------------------------------------ wave helpers
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Items a wave takes per work-fetch atomic: kSub sub-batches of 64.  Every
+// global counter here is touched by all waves of the grid, and device-scope
+// atomics on one address serialise across the chip, so each wave fetches
+// kSub * 64 items at once and publishes each output queue's survivors of all
+// sub-batches with one atomic.
+constexpr int kSub = 4;
+
 __device__ __forceinline__ uint32_t wave_fetch(uint32_t *work)
 {
     uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(work, (uint32_t)kBlock);
+    if (lane_id() == 0) base = atomicAdd(work, (uint32_t)(kBlock * kSub));
     return __shfl(base, 0);
 }
 
-// All 64 lanes must call; returns the slot for lanes with pred.
-__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred)
+// Persistent loop over n items (one wave per block): body(i, p, out) handles
+// item i < n, sets p (the path slot it appends) and out[q] for each output
+// queue q.  Survivors of a fetch's sub-batches are staged in LDS and then
+// published with one atomic per queue and a coalesced copy.
+template <int kQueues, typename Body>
+__device__ __forceinline__ void run_items(uint32_t *fetch, uint32_t n, uint32_t *const *counters,
+                                          uint32_t *const *queues, Body &&body)
 {
-    const uint64_t mask = __ballot(pred);
-    const uint32_t total = (uint32_t)__popcll(mask);
-    uint32_t base = 0;
-    if (lane_id() == 0 && total) base = atomicAdd(counter, total);
-    base = __shfl(base, 0);
-    const uint64_t below = mask & ((1ull << lane_id()) - 1ull);
-    return base + (uint32_t)__popcll(below);
+    constexpr int kQ = kQueues > 0 ? kQueues : 1;
+    __shared__ uint32_t stage[kQ * kSub * kBlock];
+    const uint64_t below = (1ull << lane_id()) - 1ull;
+    for (;;) {
+        const uint32_t base = wave_fetch(fetch);
+        if (base >= n) break;
+        uint32_t filled[kQ] = {};
+#pragma unroll 1
+        for (int k = 0; k < kSub; ++k) {
+            const uint32_t first = base + (uint32_t)k * kBlock;
+            if (first >= n) break;
+            const uint32_t i = first + lane_id();
+            uint32_t p = 0;
+            bool out[kQ] = {};
+            if (i < n) body(i, p, out);
+            static_for<0, kQueues>([&](auto Q) {
+                const uint64_t m = __ballot(out[Q()]);
+                if (out[Q()]) stage[Q() * kSub * kBlock + filled[Q()] + (uint32_t)__popcll(m & below)] = p;
+                filled[Q()] += (uint32_t)__popcll(m);
+            });
+        }
+        if constexpr (kQueues > 0) {
+            __syncthreads();
+            static_for<0, kQueues>([&](auto Q) {
+                if (filled[Q()]) {
+                    uint32_t dst = 0;
+                    if (lane_id() == 0) dst = atomicAdd(counters[Q()], filled[Q()]);
+                    dst = __shfl(dst, 0);
+                    for (uint32_t j = lane_id(); j < filled[Q()]; j += kBlock)
+                        queues[Q()][dst + j] = stage[Q() * kSub * kBlock + j];
+                }
+            });
+            __syncthreads();
+        }
+    }
 }
 
 __device__ __forceinline__ void wave_add64(unsigned long long *counter, uint32_t v)
@@ -120,9 +160,11 @@ constexpr uint32_t kHitBack = 1u;
 constexpr uint32_t kHitVolume = 2u;
 
 // shadow flags: bits 0-1 active rays, 2-3 transmit per ray, 4 volume vertex,
-// 5 path ends after this shadow stage, 8-15 vertex depth
+// 5 path ends after this shadow stage, 6-7 ray reached the light past solid
+// geometry, 8-15 vertex depth
 constexpr uint32_t kShVolume = 1u << 4;
 constexpr uint32_t kShTerminate = 1u << 5;
+constexpr uint32_t kShClear0 = 1u << 6;   // bits 6-7: shadow ray k is not occluded by solid geometry
 
 __device__ __forceinline__ f3 light_clamp(const ymrt_render_settings &rs, f3 c, int depth)
 {
@@ -140,12 +182,11 @@ __device__ __forceinline__ f3 light_clamp(const ymrt_render_settings &rs, f3 c, 
 __global__ __launch_bounds__(kBlock) void k_raygen(DevScene sc, PtParams prm, PathState ps)
 {
     uint32_t *cnt = prm.counters;  // pass 0 slots
-    for (;;) {
-        const uint32_t base = wave_fetch(cnt + kCtrWorkRaygen);
-        if (base >= prm.chunk_paths) break;
-        const uint32_t p = base + lane_id();
-        bool live = false;
-        if (p < prm.chunk_paths) {
+    uint32_t *const ctrs[1] = {cnt + kCtrActive};
+    uint32_t *const qs[1] = {ps.queue[0]};
+    run_items<1>(cnt + kCtrWorkRaygen, prm.chunk_paths, ctrs, qs, [&](uint32_t p, uint32_t &slot, bool(&out)[1]) {
+        slot = p;
+        {
             const uint64_t gp = prm.chunk_base + p;
             const uint32_t spp = (uint32_t)prm.rs.spp;
             const uint64_t vpix = gp / spp;
@@ -154,7 +195,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(DevScene sc, PtParams prm, Pa
             const uint32_t ty = tile / prm.tiles_x, tx = tile - ty * prm.tiles_x;
             const uint32_t px = tx * 16u + (within & 15u), py = ty * 16u + (within >> 4);
             if (px < prm.rs.width && py < prm.rs.height) {
-                live = true;
+                out[0] = true;
                 const uint32_t pixel_index = py * prm.rs.width + px;
                 Sampler smp;
                 float ox, oy;
@@ -195,9 +236,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(DevScene sc, PtParams prm, Pa
                 store_transfer(ps, p, prm.cap, t);
             }
         }
-        const uint32_t slot = wave_append(cnt + kCtrActive, live);
-        if (live) ps.queue[0][slot] = p;
-    }
+    });
 }
 
 // ------------------------------------------------------------ extend
@@ -206,8 +245,8 @@ __global__ __launch_bounds__(kBlock) void k_raygen(DevScene sc, PtParams prm, Pa
 // hand their hit or miss to k_medium, which samples free flight; the others
 // resolve misses here and queue surface hits for shading.  Visited wide nodes
 // and tested triangles are summed per wave into the frame statistics.
-template <bool kNearby, bool kMedia, int kStack>
-__global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
+template <bool kNearby, bool kMedia, bool kSphere, int kStack>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere ? 1 : 4))) void k_extend(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
 {
     __shared__ uint32_t stack[kStackWindow * kBlock];
     __shared__ uint32_t nbp[(kNearby ? kNearbyCap : 1) * kBlock];
@@ -219,15 +258,12 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, PtParams prm, Pa
     LaneScratch<kBlock, kStackWindow> ls{stack, nbp, nbt, nbb, (int)lane_id(), ps.spill, kSpillThreads,
                                          blockIdx.x * kBlock + threadIdx.x};
     const ymrt_render_settings &rs = prm.rs;
-    for (;;) {
-        const uint32_t base = wave_fetch(cnt + kCtrWorkExtend);
-        if (base >= n) break;
-        const uint32_t i = base + lane_id();
-        bool to_hit = false, to_med = false;
-        uint32_t p = 0;
-        uint32_t rays = 0, cam_rays = 0;
-        TraceCounters tc{0, 0};
-        if (i < n) {
+    uint32_t rays = 0, cam_rays = 0;
+    TraceCounters tc{0, 0};
+    uint32_t *const ctrs[2] = {cnt + kCtrHit, cnt + kCtrMedium};
+    uint32_t *const qs[2] = {ps.q_hit, ps.q_medium};
+    run_items<kMedia ? 2 : 1>(cnt + kCtrWorkExtend, n, ctrs, qs, [&](uint32_t i, uint32_t &p, bool(&out)[kMedia ? 2 : 1]) {
+        {
             p = qin[i];
             uint32_t info = ps.info[p];
             if (info & kInfoPendingBunch) {
@@ -253,13 +289,13 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, PtParams prm, Pa
             h.tri = 0;
             h.b0 = h.b1 = h.b2 = 0.0f;
             int nbc = 0;
-            int r = traverse<kModeAll, false, kNearby, true, kStack>(sc, q, h, nbc, ls, tc);
+            int r = traverse<kModeAll, false, kNearby, true, kSphere, kStack>(sc, q, h, nbc, ls, tc);
             if (r < 0) {
                 atomicOr(prm.error, 1u);
                 r = 0;
             }
-            rays = 1;
-            cam_rays = (info_depth(info) == 1 && (info & kInfoNeverScatter)) ? 1u : 0u;
+            rays += 1;
+            cam_rays += (info_depth(info) == 1 && (info & kInfoNeverScatter)) ? 1u : 0u;
             const bool hit = r == 1;
             const bool in_medium = kMedia && (ps.xhdr[p] & 0xFFu) != 0;
             if (!hit && !in_medium) {
@@ -268,8 +304,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, PtParams prm, Pa
                     Lv = Lv + light_clamp(rs, xyz(ps.beta[p]) * background(q.d), info_depth(info));
                 ps.out[p] = f4(Lv, 0.0f);
             } else {
-                to_med = in_medium;
-                to_hit = !in_medium;
+                out[0] = !in_medium;
+                if constexpr (kMedia) out[1] = in_medium;
                 uint32_t flags = h.back ? kHitBack : 0u;
                 bool listed = false;
                 if constexpr (kNearby) {
@@ -294,17 +330,11 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, PtParams prm, Pa
                 ps.hit_b[p] = make_float4(h.b0, h.b1, h.b2, 0.0f);
             }
         }
-        const uint32_t slot = wave_append(cnt + kCtrHit, to_hit);
-        if (to_hit) ps.q_hit[slot] = p;
-        if (kMedia) {
-            const uint32_t mslot = wave_append(cnt + kCtrMedium, to_med);
-            if (to_med) ps.q_medium[mslot] = p;
-        }
-        wave_add64(prm.stats + kStatExtension, rays);
-        wave_add64(prm.stats + kStatCamera, cam_rays);
-        wave_add64(prm.stats + kStatExtendNodes, tc.nodes);
-        wave_add64(prm.stats + kStatExtendTris, tc.tris);
-    }
+    });
+    wave_add64(prm.stats + kStatExtension, rays);
+    wave_add64(prm.stats + kStatCamera, cam_rays);
+    wave_add64(prm.stats + kStatExtendNodes, tc.nodes);
+    wave_add64(prm.stats + kStatExtendTris, tc.tris);
 }
 
 // ------------------------------------------------------------ medium
@@ -312,18 +342,18 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, PtParams prm, Pa
 // (SampleVolumeScattering, src/VolumeEval.cuh:217-251, called from
 // src/Shading.cu:1800-1840): scattering vertices go to the volume shading
 // queue, surviving rays to surface shading or, on a miss, to the environment.
+// kGrid: the scene has voxel-grid media (delta tracking); otherwise every
+// medium is homogeneous and the tracker is compiled out.
+template <bool kGrid>
 __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
 {
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     const uint32_t n = cnt[kCtrMedium];
     const ymrt_render_settings &rs = prm.rs;
-    for (;;) {
-        const uint32_t base = wave_fetch(cnt + kCtrWorkMedium);
-        if (base >= n) break;
-        const uint32_t i = base + lane_id();
-        bool to_hit = false, to_vol = false;
-        uint32_t p = 0;
-        if (i < n) {
+    uint32_t *const ctrs[2] = {cnt + kCtrHit, cnt + kCtrHitVol};
+    uint32_t *const qs[2] = {ps.q_hit, ps.q_hit_vol};
+    run_items<2>(cnt + kCtrWorkMedium, n, ctrs, qs, [&](uint32_t i, uint32_t &p, bool(&out)[2]) {
+        {
             p = ps.q_medium[i];
             const uint4 ha = ps.hit_a[p];
             const bool hit = ha.y != kEmpty;
@@ -337,35 +367,37 @@ __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PtParams prm, Pa
             if (nv > 0) {
                 Sampler smp = load_sampler(ps, p);
                 bool het = false;
-                static_for<0, kMaxBoundaryRecord>([&](auto K) {
-                    if (K() < nv && vol_is_het(sc, vi[K()])) het = true;
-                });
+                if constexpr (kGrid) {
+                    static_for<0, kMaxBoundaryRecord>([&](auto K) {
+                        if (K() < nv && vol_is_het(sc, vi[K()])) het = true;
+                    });
+                }
                 f3 trw;
                 const float tmax = hit ? __uint_as_float(ha.x) : kTMax;
-                vol_hit = het ? sample_heterogeneous(sc, nv, vi, o, d, tmax, trw, dist, smp)
-                              : sample_homogeneous(sc, nv, vi, tmax, trw, dist, smp);
+                if constexpr (kGrid) {
+                    vol_hit = het ? sample_heterogeneous(sc, nv, vi, o, d, tmax, trw, dist, smp)
+                                  : sample_homogeneous(sc, nv, vi, tmax, trw, dist, smp);
+                } else {
+                    vol_hit = sample_homogeneous(sc, nv, vi, tmax, trw, dist, smp);
+                }
                 beta = beta * trw;
                 ps.beta[p] = f4(beta, 0.0f);
                 store_sampler(ps, p, smp);
             }
             if (vol_hit) {
-                to_vol = true;
+                out[1] = true;
                 const f3 vpos = o + d * dist;
                 ps.hit_a[p] = make_uint4(ha.x, ha.y, ha.z, ha.w | kHitVolume);
                 ps.hit_b[p] = make_float4(vpos.x, vpos.y, vpos.z, 0.0f);
             } else if (hit) {
-                to_hit = true;
+                out[0] = true;
             } else {
                 f3 Lv = xyz(ps.L[p]);
                 if (rs.enable_env_light) Lv = Lv + light_clamp(rs, beta * background(d), info_depth(ps.info[p]));
                 ps.out[p] = f4(Lv, 0.0f);
             }
         }
-        const uint32_t slot = wave_append(cnt + kCtrHit, to_hit);
-        if (to_hit) ps.q_hit[slot] = p;
-        const uint32_t vslot = wave_append(cnt + kCtrHitVol, to_vol);
-        if (to_vol) ps.q_hit_vol[vslot] = p;
-    }
+    });
 }
 
 // ------------------------------------------------------------ shade
@@ -401,13 +433,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
     const uint32_t *q = kVolume ? ps.q_hit_vol : ps.q_hit;
     const ymrt_render_settings &rs = prm.rs;
     const uint32_t cap = prm.cap;
-    for (;;) {
-        const uint32_t base = wave_fetch(cnt + (kVolume ? kCtrWorkShadeVol : kCtrWorkShade));
-        if (base >= n) break;
-        const uint32_t i = base + lane_id();
-        bool want_next = false, want_shadow = false;
-        uint32_t p = 0;
-        if (i < n) {
+    uint32_t *const ctrs[2] = {cnt + kCtrShadow, cnt_next + kCtrActive};
+    uint32_t *const qs[2] = {ps.q_shadow, ps.queue[(pass + 1) & 1]};
+    run_items<2>(cnt + (kVolume ? kCtrWorkShadeVol : kCtrWorkShade), n, ctrs, qs,
+                 [&](uint32_t i, uint32_t &p, bool(&out)[2]) {
+        {
             p = q[i];
             uint32_t info = ps.info[p];
             int depth = info_depth(info);
@@ -493,16 +523,29 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                 const uint32_t inst = ha.y, tri = ha.z;
                 const int back = (ha.w & kHitBack) ? 1 : 0;
                 const DevInstance &in = sc.instances[inst];
-                const ShadeTri &st = sc.shade[in.shade_base + tri];
-                const f3 b{hb.x, hb.y, hb.z};
-                const f3 p0{st.p[0], st.p[1], st.p[2]}, p1{st.p[3], st.p[4], st.p[5]}, p2{st.p[6], st.p[7], st.p[8]};
-                const float xs = fabsf(p0.x * b.x) + fabsf(p1.x * b.y) + fabsf(p2.x * b.z);
-                const float ys = fabsf(p0.y * b.x) + fabsf(p1.y * b.y) + fabsf(p2.y * b.z);
-                const float zs = fabsf(p0.z * b.x) + fabsf(p1.z * b.y) + fabsf(p2.z * b.z);
-                f3 pos = interp3(p0, p1, p2, b);
-                f3 perr = error_gamma(7) * f3{xs, ys, zs};
-                f3 ng = cross(p1 - p0, p2 - p0);
                 const uint32_t iflags = in.flags;
+                const bool is_sphere = (iflags & kInstSphere) != 0;
+                const ShadeTri &st = sc.shade[in.shade_base + (is_sphere ? 0u : tri)];
+                const f3 b{hb.x, hb.y, hb.z};
+                f3 p0 = s3(0.0f), p1 = s3(0.0f), p2 = s3(0.0f);
+                f3 pos, perr, ng;
+                const float radius = gmax(in.radius, kFloatEpsilon);
+                if (is_sphere) {
+                    // FetchSphereShadingData (src/ShadingUtilities.cuh:636-711)
+                    pos = b;
+                    perr = error_gamma(5) * abs3(b);
+                    ng = b / radius;
+                } else {
+                    p0 = f3{st.p[0], st.p[1], st.p[2]};
+                    p1 = f3{st.p[3], st.p[4], st.p[5]};
+                    p2 = f3{st.p[6], st.p[7], st.p[8]};
+                    const float xs = fabsf(p0.x * b.x) + fabsf(p1.x * b.y) + fabsf(p2.x * b.z);
+                    const float ys = fabsf(p0.y * b.x) + fabsf(p1.y * b.y) + fabsf(p2.y * b.z);
+                    const float zs = fabsf(p0.z * b.x) + fabsf(p1.z * b.y) + fabsf(p2.z * b.z);
+                    pos = interp3(p0, p1, p2, b);
+                    perr = error_gamma(7) * f3{xs, ys, zs};
+                    ng = cross(p1 - p0, p2 - p0);
+                }
                 const DevMaterial &m = sc.materials[in.material];
                 const uint32_t mtype = m.type;
                 uint32_t prio;
@@ -528,25 +571,44 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                                                     m.intensity,
                                             depth);
                 } else {
-                    f3 sn = (iflags & kInstNoNormals)
-                                ? ng
-                                : interp3(f3{st.n[0], st.n[1], st.n[2]}, f3{st.n[3], st.n[4], st.n[5]},
-                                          f3{st.n[6], st.n[7], st.n[8]}, b);
-                    const f2 uv0{st.uv[0], st.uv[1]}, uv1{st.uv[2], st.uv[3]}, uv2{st.uv[4], st.uv[5]};
-                    const f2 uv{uv0.x * b.x + uv1.x * b.y + uv2.x * b.z, uv0.y * b.x + uv1.y * b.y + uv2.y * b.z};
-                    float du01 = uv1.x - uv0.x, dv01 = uv1.y - uv0.y, du02 = uv2.x - uv0.x, dv02 = uv2.y - uv0.y;
-                    float det = du01 * dv02 - dv01 * du02;
-                    if (fabsf(det) < 1E-8f) {
-                        du01 = 1.0f;
-                        dv01 = 0.0f;
-                        du02 = 0.0f;
-                        dv02 = 1.0f;
-                        det = 1.0f;
+                    f3 sn, dpdu, dpdv;
+                    f2 uv;
+                    if (is_sphere) {
+                        sn = ng;
+                        const float theta = acosf(ng.y);
+                        float phi = atan2f(ng.z, ng.x);
+                        if (phi < 0.0f) phi += kTwoPi;
+                        uv = f2{0.5f * phi * kInvPi, theta * kInvPi};
+                        float sin_theta = 0.0f;
+                        if (1.0f - ng.y * ng.y > 0.0f) sin_theta = sqrtf(1.0f - ng.y * ng.y);
+                        float cos_phi = 1.0f, sin_phi = 0.0f;
+                        if (sin_theta > 0.0f) {
+                            cos_phi = ng.x / sin_theta;
+                            sin_phi = ng.z / sin_theta;
+                        }
+                        dpdu = f3{-b.z, 0.0f, b.x} * kTwoPi;
+                        dpdv = f3{b.y * cos_phi, -radius * sin_theta, sin_phi * b.y} * kPi;
+                    } else {
+                        sn = (iflags & kInstNoNormals)
+                                 ? ng
+                                 : interp3(f3{st.n[0], st.n[1], st.n[2]}, f3{st.n[3], st.n[4], st.n[5]},
+                                           f3{st.n[6], st.n[7], st.n[8]}, b);
+                        const f2 uv0{st.uv[0], st.uv[1]}, uv1{st.uv[2], st.uv[3]}, uv2{st.uv[4], st.uv[5]};
+                        uv = f2{uv0.x * b.x + uv1.x * b.y + uv2.x * b.z, uv0.y * b.x + uv1.y * b.y + uv2.y * b.z};
+                        float du01 = uv1.x - uv0.x, dv01 = uv1.y - uv0.y, du02 = uv2.x - uv0.x, dv02 = uv2.y - uv0.y;
+                        float det = du01 * dv02 - dv01 * du02;
+                        if (fabsf(det) < 1E-8f) {
+                            du01 = 1.0f;
+                            dv01 = 0.0f;
+                            du02 = 0.0f;
+                            dv02 = 1.0f;
+                            det = 1.0f;
+                        }
+                        const float idet = 1.0f / det;
+                        const f3 dp01 = p1 - p0, dp02 = p2 - p0;
+                        dpdu = (dv02 * dp01 - dv01 * dp02) * idet;
+                        dpdv = (-du02 * dp01 + du01 * dp02) * idet;
                     }
-                    const float idet = 1.0f / det;
-                    const f3 dp01 = p1 - p0, dp02 = p2 - p0;
-                    f3 dpdu = (dv02 * dp01 - dv01 * dp02) * idet;
-                    f3 dpdv = (-du02 * dp01 + du01 * dp02) * idet;
                     pos = xf_position(in.o2w, pos, &perr);
                     sn = normalize(xf_transpose_vector(in.w2o, sn));
                     ng = normalize(xf_transpose_vector(in.w2o, ng));
@@ -638,7 +700,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
             store_sampler(ps, p, smp);
             ps.L[p] = f4(L, 0.0f);
             if (nee) {
-                want_shadow = true;
+                out[0] = true;
                 if (!alive) so.flags |= kShTerminate;
                 ps.sh_flags[p] = so.flags;
                 ps.sh_c[p] = f4(sh_beta, sel_pdf);
@@ -654,7 +716,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                 ps.out[p] = f4(L, 0.0f);
             }
             if (alive) {
-                want_next = true;
+                out[1] = true;
                 ps.ray_o[p] = f4(next_o, 0.0f);
                 ps.ray_d[p] = f4(next_d, 0.0f);
                 ps.last[p] = make_uint2(next_li, next_lt);
@@ -664,58 +726,60 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                    (vertex << 16);
             ps.info[p] = info;
         }
-        const uint32_t s1 = wave_append(cnt + kCtrShadow, want_shadow);
-        if (want_shadow) ps.q_shadow[s1] = p;
-        const uint32_t s2 = wave_append(cnt_next + kCtrActive, want_next);
-        if (want_next) ps.queue[(pass + 1) & 1][s2] = p;
-    }
+    });
 }
 
 // ------------------------------------------------------------ shadow
-template <bool kMedia, int kStack>
-__global__ __launch_bounds__(kBlock) void k_shadow(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
+// NEE contribution of a path's shadow rays (EvalShapeLight's tail,
+// src/Shading.cu:337-416): sum f * Li * mis * transmittance over the rays that
+// reached the light, divide by the light-selection pdf, clamp, deposit.
+__device__ __forceinline__ void shadow_finish(const PtParams &prm, const PathState &ps, uint32_t p, uint32_t fl,
+                                              f3 tr0, f3 tr1)
+{
+    const uint32_t cap = prm.cap;
+    const float4 c4 = ps.sh_c[p];
+    f3 direct = s3(0.0f);
+    if ((fl & 1u) && (fl & kShClear0)) {
+        const float4 f4v = ps.sh_f[p];
+        const float4 li4 = ps.sh_li[p];
+        direct = direct + ((f4v.w * tr0) * xyz(f4v)) * xyz(li4);
+    }
+    if ((fl & 2u) && (fl & (kShClear0 << 1))) {
+        const float4 f4v = ps.sh_f[(size_t)cap + p];
+        const float4 li4 = ps.sh_li[(size_t)cap + p];
+        direct = direct + ((f4v.w * tr1) * xyz(f4v)) * xyz(li4);
+    }
+    // EvalShapeLight returns direct * SafeRcp(select pdf); the volume variant
+    // divides when the pdf is positive.
+    const f3 sel = (fl & kShVolume) ? (c4.w > 0.0f ? direct / c4.w : s3(0.0f)) : direct * safe_rcp(c4.w);
+    f3 L = xyz(ps.L[p]);
+    L = L + light_clamp(prm.rs, xyz(c4) * sel, (int)((fl >> 8) & 0xFFu));
+    ps.L[p] = f4(L, 0.0f);
+    if (fl & kShTerminate) ps.out[p] = f4(L, 0.0f);
+}
+
+// Any-hit for each shadow ray against non-boundary instances
+// (BVHTraverseShadow, src/ShadingUtilities.cuh:563-635).  In scenes with media,
+// paths with an unoccluded ray go on to k_transmit for the transmittance along
+// it; the others are finished here.
+template <bool kMedia, bool kSphere, int kStack>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_shadow(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
 {
     __shared__ uint32_t stack[kStackWindow * kBlock];
-    __shared__ uint32_t nbp[(kMedia ? kNearbyCap : 1) * kBlock];
-    __shared__ float nbt[(kMedia ? kNearbyCap : 1) * kBlock];
-    __shared__ int nbb[(kMedia ? kNearbyCap : 1) * kBlock];
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     const uint32_t n = cnt[kCtrShadow];
     const uint32_t cap = prm.cap;
-    LaneScratch<kBlock, kStackWindow> ls{stack, nbp, nbt, nbb, (int)lane_id(), ps.spill, kSpillThreads,
+    LaneScratch<kBlock, kStackWindow> ls{stack, nullptr, nullptr, nullptr, (int)lane_id(), ps.spill, kSpillThreads,
                                          blockIdx.x * kBlock + threadIdx.x};
-    for (;;) {
-        const uint32_t base = wave_fetch(cnt + kCtrWorkShadow);
-        if (base >= n) break;
-        const uint32_t i = base + lane_id();
-        uint32_t nsh = 0, ntr = 0;
-        TraceCounters tc{0, 0};
-        if (i < n) {
-            const uint32_t p = ps.q_shadow[i];
-            const uint32_t fl = ps.sh_flags[p];
-            const float4 c4 = ps.sh_c[p];
-            f3 direct = s3(0.0f);
-            Transfer t0;
-            NearbyList nb0;
-            uint4 ha = make_uint4(0, kEmpty, kEmpty, 0);
-            const uint32_t info = ps.info[p];
-            HashRng rng;
-            if (kMedia) {
-                t0 = load_transfer(ps, p, cap);
-                ha = ps.hit_a[p];
-                nb0 = load_nearby(ps, p, cap, ha.w);
-                const uint32_t gp = (uint32_t)(prm.chunk_base + p);
-                const uint32_t spp = (uint32_t)prm.rs.spp;
-                const uint32_t vpix = gp / spp, s = gp - vpix * spp;
-                const uint32_t tile = vpix >> 8, within = vpix & 255u;
-                const uint32_t ty = tile / prm.tiles_x, tx = tile - ty * prm.tiles_x;
-                const uint32_t px = tx * 16u + (within & 15u), py = ty * 16u + (within >> 4);
-                rng.key[0] = py * prm.rs.width + px;
-                rng.key[1] = info_vertex(info);
-                rng.key[2] = s;
-                rng.key[3] = (uint32_t)prm.frame;
-                rng.ctr = 0;
-            }
+    uint32_t nsh = 0;
+    TraceCounters tc{0, 0};
+    uint32_t *const ctrs[1] = {cnt + kCtrTransmit};
+    uint32_t *const qs[1] = {ps.q_transmit};
+    run_items<kMedia ? 1 : 0>(cnt + kCtrWorkShadow, n, ctrs, qs, [&](uint32_t i, uint32_t &p, bool(&out)[1]) {
+        {
+            p = ps.q_shadow[i];
+            uint32_t fl = ps.sh_flags[p];
+#pragma unroll
             for (int k = 0; k < 2; ++k) {
                 if (!(fl & (1u << k))) continue;
                 const float4 o4 = ps.sh_o[(size_t)k * cap + p];
@@ -729,89 +793,125 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene sc, PtParams prm, Pa
                 q.last_tri = kEmpty;
                 Hit h;
                 int nbc = 0;
-                int r = traverse<kModeSkipBoundary, true, false, true, kStack>(sc, q, h, nbc, ls, tc);
+                const int r = traverse<kModeSkipBoundary, true, false, true, kSphere, kStack>(sc, q, h, nbc, ls, tc);
                 ++nsh;
-                if (r < 0) {
-                    atomicOr(prm.error, 1u);
-                    continue;
-                }
-                if (r == 1) continue;
-                f3 tr = s3(1.0f);
-                if constexpr (kMedia) {
-                    if (prm.rs.enable_volume_scattering) {
-                        // TraceTr (src/VolumeEval.cuh:420-468)
-                        Transfer t = t0;
-                        if (fl & (4u << k)) boundary_bunch(sc, t, nb0);
-                        const bool volv = (fl & kShVolume) != 0;
-                        Query tq = q;
-                        tq.last_inst = volv ? kEmpty : ha.y;
-                        tq.last_tri = volv ? kEmpty : ha.z;
-                        const float maxd = o4.w;
-                        float traced = kTMin;
-                        int guard = 0;
-                        while (traced < maxd && sc.volume_count > 0) {
-                            Hit bh;
-                            bh.back = 0;
-                            int bnc = 0;
-                            int br = 0;
-                            ++ntr;
-                            if (sc.has_boundaries)
-                                br = traverse<kModeBoundaryOnly, false, true, true, kStack>(sc, tq, bh, bnc, ls, tc);
-                            if (br < 0) {
-                                atomicOr(prm.error, 1u);
-                                break;
-                            }
-                            if (br == 0) {
-                                tr = tr * eval_tr(sc, tq.o, tq.d, t, gmax(maxd - traced, 0.0f), rng);
-                                break;
-                            }
-                            tr = tr * eval_tr(sc, tq.o, tq.d, t, bh.t, rng);
-                            traced += bh.t;
-                            const DevInstance &bin = sc.instances[bh.inst];
-                            const ShadeTri &st = sc.shade[bin.shade_base + bh.tri];
-                            const f3 p0{st.p[0], st.p[1], st.p[2]}, p1{st.p[3], st.p[4], st.p[5]},
-                                p2{st.p[6], st.p[7], st.p[8]};
-                            const float xs = fabsf(p0.x * bh.b0) + fabsf(p1.x * bh.b1) + fabsf(p2.x * bh.b2);
-                            const float ys = fabsf(p0.y * bh.b0) + fabsf(p1.y * bh.b1) + fabsf(p2.y * bh.b2);
-                            const float zs = fabsf(p0.z * bh.b0) + fabsf(p1.z * bh.b1) + fabsf(p2.z * bh.b2);
-                            f3 bp = p0 * bh.b0 + p1 * bh.b1 + p2 * bh.b2;
-                            f3 bperr = error_gamma(7) * f3{xs, ys, zs};
-                            f3 bng = cross(p1 - p0, p2 - p0);
-                            bp = xf_position(bin.o2w, bp, &bperr);
-                            bng = normalize(xf_transpose_vector(bin.w2o, bng));
-                            NearbyList bnb;
-                            bnb.n = bnc;
-                            static_for<0, kNearbyMax>([&](auto J) {
-                                bnb.prim[J()] = nbp[J() * kBlock + lane_id()];
-                                bnb.t[J()] = nbt[J() * kBlock + lane_id()];
-                                bnb.back[J()] = nbb[J() * kBlock + lane_id()];
-                            });
-                            boundary_bunch(sc, t, bnb);
-                            tq.o = offset_ray_origin(bp, bperr, tq.d, bng);
-                            tq.tmax = kTMax;
-                            tq.last_inst = bh.inst;
-                            tq.last_tri = bh.tri;
-                            if (++guard > 64) break;
-                        }
-                    }
-                }
-                const float4 f4v = ps.sh_f[(size_t)k * cap + p];
-                const float4 li4 = ps.sh_li[(size_t)k * cap + p];
-                direct = direct + ((f4v.w * tr) * xyz(f4v)) * xyz(li4);
+                if (r < 0) atomicOr(prm.error, 1u);
+                if (r == 0) fl |= kShClear0 << k;
             }
-            // EvalShapeLight returns direct * SafeRcp(select pdf); the volume
-            // variant divides when the pdf is positive.
-            const f3 sel = (fl & kShVolume) ? (c4.w > 0.0f ? direct / c4.w : s3(0.0f)) : direct * safe_rcp(c4.w);
-            f3 L = xyz(ps.L[p]);
-            L = L + light_clamp(prm.rs, xyz(c4) * sel, (int)((fl >> 8) & 0xFFu));
-            ps.L[p] = f4(L, 0.0f);
-            if (fl & kShTerminate) ps.out[p] = f4(L, 0.0f);
+            const bool walk = kMedia && (fl & (kShClear0 | (kShClear0 << 1)));
+            out[0] = walk;
+            if (walk) ps.sh_flags[p] = fl;
+            else shadow_finish(prm, ps, p, fl, s3(1.0f), s3(1.0f));
         }
-        wave_add64(prm.stats + kStatShadow, nsh);
-        wave_add64(prm.stats + kStatTransmittance, ntr);
-        wave_add64(prm.stats + kStatShadowNodes, tc.nodes);
-        wave_add64(prm.stats + kStatShadowTris, tc.tris);
-    }
+    });
+    wave_add64(prm.stats + kStatShadow, nsh);
+    wave_add64(prm.stats + kStatShadowNodes, tc.nodes);
+    wave_add64(prm.stats + kStatShadowTris, tc.tris);
+}
+
+// Transmittance of the unoccluded shadow rays of paths in media: TraceTr
+// (src/VolumeEval.cuh:420-468) walks the boundary instances along the ray,
+// updating the medium records at each crossing and multiplying the segment
+// transmittances (EvalTr, 253-418).
+template <bool kSphere, int kStack>
+__global__ __launch_bounds__(kBlock) void k_transmit(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
+{
+    __shared__ uint32_t stack[kStackWindow * kBlock];
+    __shared__ uint32_t nbp[kNearbyCap * kBlock];
+    __shared__ float nbt[kNearbyCap * kBlock];
+    __shared__ int nbb[kNearbyCap * kBlock];
+    uint32_t *cnt = prm.counters + pass * kCtrStride;
+    const uint32_t n = cnt[kCtrTransmit];
+    const uint32_t cap = prm.cap;
+    LaneScratch<kBlock, kStackWindow> ls{stack, nbp, nbt, nbb, (int)lane_id(), ps.spill, kSpillThreads,
+                                         blockIdx.x * kBlock + threadIdx.x};
+    uint32_t ntr = 0;
+    TraceCounters tc{0, 0};
+    run_items<0>(cnt + kCtrWorkTransmit, n, nullptr, nullptr, [&](uint32_t i, uint32_t &slot, bool(&)[1]) {
+        {
+            const uint32_t p = ps.q_transmit[i];
+            slot = p;
+            const uint32_t fl = ps.sh_flags[p];
+            const uint32_t info = ps.info[p];
+            const Transfer t0 = load_transfer(ps, p, cap);
+            const uint4 ha = ps.hit_a[p];
+            const NearbyList nb0 = load_nearby(ps, p, cap, ha.w);
+            HashRng rng;
+            {
+                const uint32_t gp = (uint32_t)(prm.chunk_base + p);
+                const uint32_t spp = (uint32_t)prm.rs.spp;
+                const uint32_t vpix = gp / spp, s = gp - vpix * spp;
+                const uint32_t tile = vpix >> 8, within = vpix & 255u;
+                const uint32_t ty = tile / prm.tiles_x, tx = tile - ty * prm.tiles_x;
+                const uint32_t px = tx * 16u + (within & 15u), py = ty * 16u + (within >> 4);
+                rng.key[0] = py * prm.rs.width + px;
+                rng.key[1] = info_vertex(info);
+                rng.key[2] = s;
+                rng.key[3] = (uint32_t)prm.frame;
+                rng.ctr = 0;
+            }
+            f3 trk[2] = {s3(1.0f), s3(1.0f)};
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                if (!(fl & (1u << k)) || !(fl & (kShClear0 << k))) continue;
+                const float4 o4 = ps.sh_o[(size_t)k * cap + p];
+                const float4 d4 = ps.sh_d[(size_t)k * cap + p];
+                f3 tr = s3(1.0f);
+                Transfer t = t0;
+                if (fl & (4u << k)) boundary_bunch(sc, t, nb0);
+                const bool volv = (fl & kShVolume) != 0;
+                Query tq;
+                tq.o = xyz(o4);
+                tq.d = xyz(d4);
+                tq.tmin = kTMin;
+                tq.tmax = o4.w;
+                tq.last_inst = volv ? kEmpty : ha.y;
+                tq.last_tri = volv ? kEmpty : ha.z;
+                const float maxd = o4.w;
+                float traced = kTMin;
+                int guard = 0;
+                while (traced < maxd) {
+                    Hit bh;
+                    bh.back = 0;
+                    int bnc = 0;
+                    int br = 0;
+                    ++ntr;
+                    if (sc.has_boundaries)
+                        br = traverse<kModeBoundaryOnly, false, true, true, kSphere, kStack>(sc, tq, bh, bnc, ls, tc);
+                    if (br < 0) {
+                        atomicOr(prm.error, 1u);
+                        break;
+                    }
+                    if (br == 0) {
+                        tr = tr * eval_tr(sc, tq.o, tq.d, t, gmax(maxd - traced, 0.0f), rng);
+                        break;
+                    }
+                    tr = tr * eval_tr(sc, tq.o, tq.d, t, bh.t, rng);
+                    traced += bh.t;
+                    f3 bp, bperr, bng;
+                    hit_geometry(sc, bh.inst, bh.tri, bh.b0, bh.b1, bh.b2, bp, bperr, bng);
+                    NearbyList bnb;
+                    bnb.n = bnc;
+                    static_for<0, kNearbyMax>([&](auto J) {
+                        bnb.prim[J()] = nbp[J() * kBlock + lane_id()];
+                        bnb.t[J()] = nbt[J() * kBlock + lane_id()];
+                        bnb.back[J()] = nbb[J() * kBlock + lane_id()];
+                    });
+                    boundary_bunch(sc, t, bnb);
+                    tq.o = offset_ray_origin(bp, bperr, tq.d, bng);
+                    tq.tmax = kTMax;
+                    tq.last_inst = bh.inst;
+                    tq.last_tri = bh.tri;
+                    if (++guard > 64) break;
+                }
+                trk[k] = tr;
+            }
+            shadow_finish(prm, ps, p, fl, trk[0], trk[1]);
+        }
+    });
+    wave_add64(prm.stats + kStatTransmittance, ntr);
+    wave_add64(prm.stats + kStatTransmitNodes, tc.nodes);
+    wave_add64(prm.stats + kStatTransmitTris, tc.tris);
 }
 
 // ------------------------------------------------------------ flush
@@ -899,22 +999,29 @@ int launch_raygen(const DevScene &sc, const PtParams &prm, const PathState &ps, 
     return YMRT_OK;
 }
 
-template <bool kNearby, bool kMedia, int kStack>
+template <bool kNearby, bool kMedia, bool kSphere>
 static int launch_extend_t(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_extend<kNearby, kMedia, kStack>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc,
-                       prm, ps, pass);
+    hipLaunchKernelGGL((k_extend<kNearby, kMedia, kSphere, kMaxStackDepth>), persistent_grid(prm.chunk_paths),
+                       dim3(kBlock), 0, s, sc, prm, ps, pass);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }
 
-template <bool kMedia, int kStack>
+template <bool kMedia, bool kSphere>
 static int launch_shadow_t(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_shadow<kMedia, kStack>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps,
-                       pass);
+    hipLaunchKernelGGL((k_shadow<kMedia, kSphere, kMaxStackDepth>), persistent_grid(prm.chunk_paths), dim3(kBlock),
+                       0, s, sc, prm, ps, pass);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
+}
+
+template <bool kNearby, bool kMedia>
+static int launch_extend_m(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
+{
+    return sc.has_spheres ? launch_extend_t<kNearby, kMedia, true>(sc, prm, ps, pass, s)
+                          : launch_extend_t<kNearby, kMedia, false>(sc, prm, ps, pass, s);
 }
 
 // The window stack spills to a kMaxStackDepth-deep area, so one depth bound
@@ -923,15 +1030,18 @@ int launch_extend(const DevScene &sc, const PtParams &prm, const PathState &ps, 
                   bool media, uint32_t, hipStream_t s)
 {
     if (nearby)
-        return media ? launch_extend_t<true, true, kMaxStackDepth>(sc, prm, ps, pass, s)
-                     : launch_extend_t<true, false, kMaxStackDepth>(sc, prm, ps, pass, s);
-    return media ? launch_extend_t<false, true, kMaxStackDepth>(sc, prm, ps, pass, s)
-                 : launch_extend_t<false, false, kMaxStackDepth>(sc, prm, ps, pass, s);
+        return media ? launch_extend_m<true, true>(sc, prm, ps, pass, s)
+                     : launch_extend_m<true, false>(sc, prm, ps, pass, s);
+    return media ? launch_extend_m<false, true>(sc, prm, ps, pass, s)
+                 : launch_extend_m<false, false>(sc, prm, ps, pass, s);
 }
 
 int launch_medium(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_medium, persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps, pass);
+    if (sc.grid_count > 0)
+        hipLaunchKernelGGL(k_medium<true>, persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps, pass);
+    else
+        hipLaunchKernelGGL(k_medium<false>, persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps, pass);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }
@@ -950,8 +1060,21 @@ int launch_shade(const DevScene &sc, const PtParams &prm, const PathState &ps, u
 int launch_shadow(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool media,
                   uint32_t, hipStream_t s)
 {
-    return media ? launch_shadow_t<true, kMaxStackDepth>(sc, prm, ps, pass, s)
-                 : launch_shadow_t<false, kMaxStackDepth>(sc, prm, ps, pass, s);
+    if (sc.has_spheres)
+        return media ? launch_shadow_t<true, true>(sc, prm, ps, pass, s) : launch_shadow_t<false, true>(sc, prm, ps, pass, s);
+    return media ? launch_shadow_t<true, false>(sc, prm, ps, pass, s) : launch_shadow_t<false, false>(sc, prm, ps, pass, s);
+}
+
+int launch_transmit(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
+{
+    if (sc.has_spheres)
+        hipLaunchKernelGGL((k_transmit<true, kMaxStackDepth>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s,
+                           sc, prm, ps, pass);
+    else
+        hipLaunchKernelGGL((k_transmit<false, kMaxStackDepth>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s,
+                           sc, prm, ps, pass);
+    YM_HIP(hipGetLastError());
+    return YMRT_OK;
 }
 
 int launch_flush(const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)

@@ -12,7 +12,7 @@ namespace ym {
 
 constexpr int kTraceBlock = 64;
 
-template <bool kAny, bool kCount, int kStack>
+template <bool kAny, bool kCount, bool kSphere, int kStack>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(DevScene sc, const float4 *__restrict__ rays, uint32_t n,
                                                         uint4 *__restrict__ hits, uint32_t *__restrict__ occluded,
                                                         uint2 *__restrict__ visits, uint32_t *__restrict__ err)
@@ -35,9 +35,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(DevScene sc, const float4
     TraceCounters cnt{0, 0};
     int r;
     if (kAny)
-        r = traverse<kModeSkipBoundary, true, false, kCount, kStack>(sc, q, h, nbc, ls, cnt);
+        r = traverse<kModeSkipBoundary, true, false, kCount, kSphere, kStack>(sc, q, h, nbc, ls, cnt);
     else
-        r = traverse<kModeAll, false, false, kCount, kStack>(sc, q, h, nbc, ls, cnt);
+        r = traverse<kModeAll, false, false, kCount, kSphere, kStack>(sc, q, h, nbc, ls, cnt);
     if (r < 0) {
         atomicOr(err, 1u);
         r = 0;
@@ -56,7 +56,12 @@ static int launch_one(const ymrt_scene *sc, const ymrt_ray *rays, uint32_t n, ym
                       uint32_t *visits, hipStream_t s)
 {
     dim3 grid((n + kTraceBlock - 1) / kTraceBlock);
-    hipLaunchKernelGGL((k_trace<kAny, kCount, kStack>), grid, dim3(kTraceBlock), 0, s, sc->dev,
+    if (sc->dev.has_spheres)
+        hipLaunchKernelGGL((k_trace<kAny, kCount, true, kStack>), grid, dim3(kTraceBlock), 0, s, sc->dev,
+                           reinterpret_cast<const float4 *>(rays), n, reinterpret_cast<uint4 *>(hits), occ,
+                           reinterpret_cast<uint2 *>(visits), sc->d_error);
+    else
+    hipLaunchKernelGGL((k_trace<kAny, kCount, false, kStack>), grid, dim3(kTraceBlock), 0, s, sc->dev,
                        reinterpret_cast<const float4 *>(rays), n, reinterpret_cast<uint4 *>(hits), occ,
                        reinterpret_cast<uint2 *>(visits), sc->d_error);
     YM_HIP(hipGetLastError());

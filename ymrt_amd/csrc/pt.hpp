@@ -19,7 +19,9 @@ constexpr uint32_t kCtrWorkShadow = 8;
 constexpr uint32_t kCtrWorkFlush = 9;
 constexpr uint32_t kCtrMedium = 10;        // paths inside media queued for free flight
 constexpr uint32_t kCtrWorkMedium = 11;
-constexpr uint32_t kCtrStride = 12;
+constexpr uint32_t kCtrTransmit = 12;      // paths whose unoccluded shadow rays cross media
+constexpr uint32_t kCtrWorkTransmit = 13;
+constexpr uint32_t kCtrStride = 16;
 constexpr uint32_t kMaxPasses = 4 * kMaxRayDepth;
 
 // Path-tracing traversals keep the top kStackWindow stack entries per lane in
@@ -36,9 +38,11 @@ constexpr uint32_t kStatShadow = 2;
 constexpr uint32_t kStatTransmittance = 3;
 constexpr uint32_t kStatExtendNodes = 4;   // wide nodes visited by closest-hit traversals
 constexpr uint32_t kStatExtendTris = 5;    // triangles tested by closest-hit traversals
-constexpr uint32_t kStatShadowNodes = 6;   // same for shadow and transmittance traversals
+constexpr uint32_t kStatShadowNodes = 6;   // same for any-hit shadow traversals
 constexpr uint32_t kStatShadowTris = 7;
-constexpr uint32_t kStatCount = 8;
+constexpr uint32_t kStatTransmitNodes = 8; // boundary walks of shadow rays in media
+constexpr uint32_t kStatTransmitTris = 9;
+constexpr uint32_t kStatCount = 12;
 
 // Structure-of-arrays path state for `cap` paths in flight (indexed by slot).
 struct PathState {
@@ -52,14 +56,19 @@ struct PathState {
     uint4 *hit_a;       // t bits, instance, triangle, flags
     float4 *hit_b;      // barycentrics or volume position
     uint2 *nb;          // [4][cap] nearby (instance | back << 31, t bits)
-    float4 *sh_o, *sh_d, *sh_f, *sh_li;  // [2][cap]
+    float4 *sh_o, *sh_d, *sh_f, *sh_li;  // [kShadowSlots][cap]
+    float4 *sh_tr;      // [kShadowSlots][cap] transmittance of shadow rays in media
     float4 *sh_c;       // throughput at the vertex, light-select pdf
     uint32_t *sh_flags;
     float4 *out;        // final radiance per path
     uint32_t *queue[2];
-    uint32_t *q_hit, *q_hit_vol, *q_medium, *q_shadow;
+    uint32_t *q_hit, *q_hit_vol, *q_medium, *q_shadow, *q_transmit;
     uint32_t *spill;    // traversal stack spill [kMaxStackDepth - kStackWindow][kSpillThreads]
 };
+
+// Shadow rays per path vertex: slots 0-1 shape light (light sample, BSDF or
+// phase sample), slots 2-3 distant light.
+constexpr int kShadowSlots = 4;
 
 struct PtParams {
     ymrt_render_settings rs;

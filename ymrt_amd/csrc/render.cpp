@@ -25,6 +25,7 @@ int launch_shade(const DevScene &sc, const PtParams &prm, const PathState &ps, u
                  hipStream_t s);
 int launch_shadow(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool media,
                   uint32_t stack, hipStream_t s);
+int launch_transmit(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s);
 int launch_flush(const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s);
 int launch_resolve(const PtParams &prm, const PathState &ps, float *noise, hipStream_t s);
 
@@ -84,10 +85,11 @@ static int ensure_buffers(ymrt_scene *sc, uint32_t cap)
     rc |= alloc(rb, &ps.hit_a, n);
     rc |= alloc(rb, &ps.hit_b, n);
     rc |= alloc(rb, &ps.nb, n * 4);
-    rc |= alloc(rb, &ps.sh_o, n * 2);
-    rc |= alloc(rb, &ps.sh_d, n * 2);
-    rc |= alloc(rb, &ps.sh_f, n * 2);
-    rc |= alloc(rb, &ps.sh_li, n * 2);
+    rc |= alloc(rb, &ps.sh_o, n * kShadowSlots);
+    rc |= alloc(rb, &ps.sh_d, n * kShadowSlots);
+    rc |= alloc(rb, &ps.sh_f, n * kShadowSlots);
+    rc |= alloc(rb, &ps.sh_li, n * kShadowSlots);
+    rc |= alloc(rb, &ps.sh_tr, n * kShadowSlots);
     rc |= alloc(rb, &ps.sh_c, n);
     rc |= alloc(rb, &ps.sh_flags, n);
     rc |= alloc(rb, &ps.out, n);
@@ -97,6 +99,7 @@ static int ensure_buffers(ymrt_scene *sc, uint32_t cap)
     rc |= alloc(rb, &ps.q_hit_vol, n);
     rc |= alloc(rb, &ps.q_medium, n);
     rc |= alloc(rb, &ps.q_shadow, n);
+    rc |= alloc(rb, &ps.q_transmit, n);
     rc |= alloc(rb, &ps.spill, (size_t)(kMaxStackDepth - kStackWindow) * kSpillThreads);
     rc |= alloc(rb, &rb->counters, (size_t)(kMaxPasses + 2) * kCtrStride);
     rc |= alloc(rb, &rb->stats, kStatCount);
@@ -201,6 +204,8 @@ int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, floa
                 return rc;
             if ((rc = timed(3, [&] { return launch_shadow(sc->dev, prm, rb->ps, pass, media, stack, stream); })))
                 return rc;
+            if (media && (rc = timed(6, [&] { return launch_transmit(sc->dev, prm, rb->ps, pass, stream); })))
+                return rc;
             ++pass;
             if (pass == next_check || pass >= kMaxPasses) {
                 uint32_t active = 0;
@@ -232,12 +237,14 @@ int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, floa
         stats->extend_tris = st[kStatExtendTris];
         stats->shadow_nodes = st[kStatShadowNodes];
         stats->shadow_tris = st[kStatShadowTris];
+        stats->transmit_nodes = st[kStatTransmitNodes];
+        stats->transmit_tris = st[kStatTransmitTris];
         stats->paths = (uint64_t)rs->width * rs->height * spp;
         stats->passes = passes_total;
         stats->chunks = chunks;
         if (timing) {
-            float ms[6] = {0, 0, 0, 0, 0, 0};
-            uint32_t launches[6] = {0, 0, 0, 0, 0, 0};
+            float ms[7] = {0, 0, 0, 0, 0, 0, 0};
+            uint32_t launches[7] = {0, 0, 0, 0, 0, 0, 0};
             for (const Mark &m : marks) {
                 float t = 0.0f;
                 YM_HIP(hipEventElapsedTime(&t, m.a, m.b));
@@ -251,6 +258,8 @@ int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, floa
             stats->ms_resolve = ms[4];
             stats->ms_medium = ms[5];
             stats->launches_medium = launches[5];
+            stats->ms_transmit = ms[6];
+            stats->launches_transmit = launches[6];
             stats->launches_extend = launches[1];
             stats->launches_shade = launches[2];
             stats->launches_shadow = launches[3];

@@ -142,19 +142,10 @@ static void camera_media(const DevScene &hs, const HostScene &h, const ymrt_scen
         Hit hit{};
         int nbc = 0;
         TraceCounters cnt{0, 0};
-        int r = traverse<kModeAll, false, false, false, 1024>(hs, q, hit, nbc, ls, cnt);
+        int r = traverse<kModeAll, false, false, false, true, 1024>(hs, q, hit, nbc, ls, cnt);
         if (r != 1) break;
-        const DevInstance &in = h.instances[hit.inst];
-        const ShadeTri &st = h.shade[in.shade_base + hit.tri];
-        f3 p0{st.p[0], st.p[1], st.p[2]}, p1{st.p[3], st.p[4], st.p[5]}, p2{st.p[6], st.p[7], st.p[8]};
-        float xs = fabsf(p0.x * hit.b0) + fabsf(p1.x * hit.b1) + fabsf(p2.x * hit.b2);
-        float ys = fabsf(p0.y * hit.b0) + fabsf(p1.y * hit.b1) + fabsf(p2.y * hit.b2);
-        float zs = fabsf(p0.z * hit.b0) + fabsf(p1.z * hit.b1) + fabsf(p2.z * hit.b2);
-        f3 p = p0 * hit.b0 + p1 * hit.b1 + p2 * hit.b2;
-        f3 perr = error_gamma(7) * f3{xs, ys, zs};
-        f3 ng = cross(p1 - p0, p2 - p0);
-        p = xf_position(in.o2w, p, &perr);
-        ng = normalize(xf_transpose_vector(in.w2o, ng));
+        f3 p, perr, ng;
+        hit_geometry(hs, hit.inst, hit.tri, hit.b0, hit.b1, hit.b2, p, perr, ng);
         if (std::find(seen.begin(), seen.end(), hit.inst) == seen.end()) seen.push_back(hit.inst);
         q.o = offset_ray_origin(p, perr, q.d, ng);
         q.last_inst = hit.inst;
@@ -166,6 +157,11 @@ static void camera_media(const DevScene &hs, const HostScene &h, const ymrt_scen
         f3 po = xf_point(in.w2o, c.o), pd = xf_dir(in.w2o, c.d);
         RayPre pos = make_pre(po, pd), neg = make_pre(po, -pd);
         bool hp = false, hn = false;
+        if (in.flags & kInstSphere) {
+            SphereTest st;
+            hp = sphere_test(po, pd, in.radius, kTMin, kTMax, st);
+            hn = sphere_test(po, -pd, in.radius, kTMin, kTMax, st);
+        }
         for (uint32_t k = 0; k < in.tri_count && !(hp && hn); ++k) {
             const ShadeTri &st = h.shade[in.shade_base + k];
             f3 p0{st.p[0], st.p[1], st.p[2]}, p1{st.p[3], st.p[4], st.p[5]}, p2{st.p[6], st.p[7], st.p[8]};
@@ -274,33 +270,46 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
     }
     // ------------------------------------------------------------- instances
     std::vector<BuildPrim> iprims;
-    uint32_t has_boundaries = 0;
+    uint32_t has_boundaries = 0, has_spheres = 0;
     for (uint32_t i = 0; i < d->instance_count; ++i) {
         const ymrt_instance_desc &id = d->instances[i];
-        YM_REQUIRE(id.mesh < d->mesh_count, "instance mesh index out of range");
+        YM_REQUIRE(id.geometry_type == YMRT_GEOMETRY_MESH || id.geometry_type == YMRT_GEOMETRY_SPHERE,
+                   "instance geometry_type must be MESH or SPHERE");
+        const bool sphere = id.geometry_type == YMRT_GEOMETRY_SPHERE;
+        YM_REQUIRE(sphere ? id.geometry < d->sphere_count : id.geometry < d->mesh_count,
+                   "instance geometry index out of range");
         YM_REQUIRE(id.material < d->material_count, "instance material index out of range");
         YM_REQUIRE(id.inner_volume < 0 || (uint32_t)id.inner_volume < d->volume_count,
                    "instance inner_volume out of range");
         DevInstance in{};
         in.w2o = affine_from_colmajor(id.world_to_object);
         in.o2w = affine_from_colmajor(id.object_to_world);
-        in.blas_root = blas_root[id.mesh];
-        in.mesh = id.mesh;
+        in.blas_root = sphere ? kSentinel : blas_root[id.geometry];
+        in.mesh = id.geometry;
         in.material = id.material;
         in.inner_volume = id.inner_volume;
-        in.flags = (id.treat_as_boundary ? kInstBoundary : 0u) | mesh_flags[id.mesh];
-        in.shade_base = shade_base[id.mesh];
-        in.tri_count = d->meshes[id.mesh].triangle_count;
+        in.flags = (id.treat_as_boundary ? kInstBoundary : 0u) | (sphere ? kInstSphere : mesh_flags[id.geometry]);
+        in.shade_base = sphere ? 0u : shade_base[id.geometry];
+        in.tri_count = sphere ? 0u : d->meshes[id.geometry].triangle_count;
+        in.radius = sphere ? d->spheres[id.geometry].radius : 0.0f;
         has_boundaries |= id.treat_as_boundary ? 1u : 0u;
+        has_spheres |= sphere ? 1u : 0u;
         total_tris += in.tri_count;
         h.instances.push_back(in);
-        if (in.tri_count == 0) continue;
-        // world bounds of the mesh's corners
+        if (!sphere && in.tri_count == 0) continue;
+        // world bounds of the geometry's object-space box
         Aabb ob;
-        const ymrt_mesh_desc &m = d->meshes[id.mesh];
-        for (uint32_t k = 0; k < m.position_count; ++k) {
-            const float pk[3] = {m.positions[3 * k], m.positions[3 * k + 1], m.positions[3 * k + 2]};
-            ob.grow(pk);
+        if (sphere) {
+            const float r = std::fabs(in.radius);
+            const float lo[3] = {-r, -r, -r}, hi[3] = {r, r, r};
+            ob.grow(lo);
+            ob.grow(hi);
+        } else {
+            const ymrt_mesh_desc &m = d->meshes[id.geometry];
+            for (uint32_t k = 0; k < m.position_count; ++k) {
+                const float pk[3] = {m.positions[3 * k], m.positions[3 * k + 1], m.positions[3 * k + 2]};
+                ob.grow(pk);
+            }
         }
         Aabb wb;
         for (int c = 0; c < 8; ++c) {
@@ -362,6 +371,7 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
     ds.instance_count = (uint32_t)iprims.size() ? d->instance_count : 0;
     ds.material_count = d->material_count;
     ds.has_boundaries = has_boundaries;
+    ds.has_spheres = has_spheres;
     // ---------------------------------------------------------- textures / grids
     std::vector<DevTexture> texs;
     std::vector<uint32_t> texels;
@@ -444,6 +454,16 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
         if (lm.type != YMRT_MATERIAL_LIGHT) continue;
         const float irr = (lm.light_color[0] + lm.light_color[1] + lm.light_color[2]) * lm.intensity * kPi;
         const DevInstance &in = h.instances[i];
+        if (in.flags & kInstSphere) {
+            // ellipsoid area estimate of LoadShapeLight with the instance's axis
+            // scales (lengths of the object-to-world columns)
+            const float sx = sphere_axis_scale(in.o2w, 0), sy = sphere_axis_scale(in.o2w, 1),
+                        sz = sphere_axis_scale(in.o2w, 2);
+            const float a = in.radius * sx, b = in.radius * sy, c = in.radius * sz;
+            const float area = (4.0f / 3.0f) * kPi * (a * b + b * c + c * a);
+            lights.push_back(DevLight{i, kEmpty, irr * area, area});
+            continue;
+        }
         for (uint32_t t = 0; t < in.tri_count; ++t) {
             const ShadeTri &st = h.shade[in.shade_base + t];
             f3 p0 = xf_position(in.o2w, f3{st.p[0], st.p[1], st.p[2]}, nullptr);
@@ -461,6 +481,20 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
     }
     float trcp = safe_rcp(total);
     for (size_t i = 0; i < cdf.size(); ++i) cdf[i] = total == 0.0f ? 0.0f : cdf[i] * trcp;
+    // ------------------------------------------------------------ distant lights
+    std::vector<DevDistant> distant;
+    YM_REQUIRE(d->distant_light_count == 0 || d->distant_lights != nullptr, "distant_lights is NULL");
+    for (uint32_t i = 0; i < d->distant_light_count; ++i) {
+        const ymrt_distant_light_desc &dl = d->distant_lights[i];
+        DevDistant dd;
+        dd.color = f3{dl.light_color[0], dl.light_color[1], dl.light_color[2]};
+        dd.intensity = dl.intensity;
+        // DistantLight() (src/Light.h:335-339): glm::cos(glm::radians(theta_max))
+        dd.cos_max = cosf(dl.theta_max * 0.01745329251994329576923690768489f);
+        dd.o2w = affine_from_colmajor(dl.object_to_world);
+        dd.w2o = affine_from_colmajor(dl.world_to_object);
+        distant.push_back(dd);
+    }
     // ------------------------------------------------- Halton permutation table
     std::vector<uint32_t> primes, prime_sums(1001, 0);
     for (uint32_t c = 2; primes.size() < 1000; ++c) {
@@ -493,6 +527,7 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
         hs.nodes = h.nodes.data();
         hs.tris = h.tris.data();
         hs.instances = h.instances.data();
+        hs.shade = h.shade.data();
         camera_media(hs, h, d, ds.camera, sc->max_stack);
         sc->info.camera_medium_records = (uint32_t)ds.camera.record_count;
     }
@@ -514,6 +549,12 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
     if ((rc = upload(sc, table, &ds.halton))) return rc;
     if ((rc = upload(sc, primes, &ds.primes))) return rc;
     if ((rc = upload(sc, prime_sums, &ds.prime_sums))) return rc;
+    if ((rc = upload(sc, distant, &ds.distant))) return rc;
+    ds.distant_count = (uint32_t)distant.size();
+    // ceil(2^64 / p): the high 64 bits of n * magic are n / p for every n < 2^32
+    std::vector<uint64_t> magic(1000);
+    for (int i = 0; i < 1000; ++i) magic[i] = ~0ull / primes[i] + 1ull;
+    if ((rc = upload(sc, magic, &ds.prime_magic))) return rc;
     ds.texture_count = d->texture_count;
     ds.grid_count = d->grid_count;
     ds.volume_count = d->volume_count;

@@ -80,8 +80,10 @@ def _add(sa, geom):
 
 
 # -------------------------------------------------------------- Cornell box
-def cornell_box(width=256, height=256, halton_seed=0):
-    """Unit Cornell box [0,1]^3 open at z=1 (the camera side)."""
+def cornell_box(width=256, height=256, halton_seed=0, analytic_sphere=False, sphere_light=False):
+    """Unit Cornell box [0,1]^3 open at z=1 (the camera side).  analytic_sphere
+    puts an analytic sphere where the glass UV sphere is; sphere_light replaces
+    the ceiling quad light with a sphere light of radius 0.08."""
     sa = abi.SceneArrays()
     sa.halton_seed = halton_seed
     white = sa.add_material(abi.default_material(diffuse_albedo=(0.7, 0.7, 0.7), specular_weight=0.0))
@@ -104,9 +106,16 @@ def cornell_box(width=256, height=256, halton_seed=0):
         sa.add_instance(_add(sa, geom), mat)
     sa.add_instance(_add(sa, box((0.55, 0.0, 0.45), (0.85, 0.3, 0.75))), white)
     sa.add_instance(_add(sa, box((0.15, 0.0, 0.15), (0.45, 0.6, 0.45))), metal)
-    sa.add_instance(_add(sa, uv_sphere((0.65, 0.45, 0.55), 0.15)), glass)
-    lq = quad((0.35, 0.999, 0.35), (0.65, 0.999, 0.35), (0.65, 0.999, 0.65), (0.35, 0.999, 0.65))
-    sa.add_instance(_add(sa, lq), light)
+    if analytic_sphere:
+        sa.add_instance(sa.add_sphere(0.15), glass, _translate(0.65, 0.45, 0.55))
+    else:
+        sa.add_instance(_add(sa, uv_sphere((0.65, 0.45, 0.55), 0.15)), glass)
+    if sphere_light:
+        m = _translate(0.5, 0.85, 0.5)
+        sa.add_instance(sa.add_sphere(0.08), sa.add_material(abi.light_material((1.0, 1.0, 1.0), 30.0)), m)
+    else:
+        lq = quad((0.35, 0.999, 0.35), (0.65, 0.999, 0.35), (0.65, 0.999, 0.65), (0.35, 0.999, 0.65))
+        sa.add_instance(_add(sa, lq), light)
     sa.set_camera((0.5, 0.5, 2.2), (0.5, 0.5, 0.0), 40.0, width / height)
     return sa
 
@@ -129,6 +138,13 @@ def cornell_media(width=64, height=64, grid=True, halton_seed=0, sigma=2.0):
     mat = sa.add_material(abi.default_material(ior_n=1.0))
     sa.add_instance(_add(sa, box(tuple(lo), tuple(hi))), mat, inner_volume=vol, boundary=True)
     return sa
+
+
+def _translate(x, y, z, scale=1.0):
+    m = np.eye(4, dtype=np.float32)
+    m[:3, :3] *= scale
+    m[:3, 3] = (x, y, z)
+    return m
 
 
 # --------------------------------------------------------------- value noise
@@ -196,7 +212,11 @@ def bounds(sa):
     lo = np.full(3, np.inf)
     hi = np.full(3, -np.inf)
     for it in sa.instances:
-        p = sa.meshes[it["mesh"]]["pos"].astype(np.float64)
+        if it["geometry_type"] == abi.GEOMETRY_SPHERE:
+            r = sa.spheres[it["mesh"]]
+            p = np.array([[x, y, z] for x in (-r, r) for y in (-r, r) for z in (-r, r)], np.float64)
+        else:
+            p = sa.meshes[it["mesh"]]["pos"].astype(np.float64)
         ph = np.c_[p, np.ones(len(p))] @ it["o2w"].astype(np.float64).T
         lo = np.minimum(lo, ph[:, :3].min(0))
         hi = np.maximum(hi, ph[:, :3].max(0))
