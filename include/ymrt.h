@@ -107,6 +107,8 @@ typedef struct ymrt_material_desc {
     float coat_roughness_x;    uint32_t coat_roughness_x_tex;
     float coat_roughness_y;    uint32_t coat_roughness_y_tex;
     float coat_ior;
+    uint32_t normal_mapping_tex;     /* tangent-space normal map (rgb*2-1)    */
+    uint32_t bump_mapping_tex;       /* height map (x)                        */
     /* LightMtl */
     float light_color[3];
     float intensity;
@@ -117,15 +119,39 @@ typedef struct ymrt_material_desc {
 #define YMRT_COLOR_SRGB 0
 #define YMRT_COLOR_LINEAR 1
 
-/* Image texture: 8-bit texels, decoded like LoadPng8 (src/FileIO.cpp:17-45):
- * value/255 then sRGB->linear when color_space == YMRT_COLOR_SRGB.  Row 0 is
- * the bottom row (the loader flips vertically).  channels is 1, 3 or 4. */
+/* Texture node (Texture, src/Texture.h:16-720).  IMAGE: 8-bit texels decoded
+ * like LoadPng8 (src/FileIO.cpp:17-45): value/255, then sRGB->linear when
+ * color_space == YMRT_COLOR_SRGB; row 0 is the bottom row; channels 1..4.
+ * CONSTANT_*: value.  The procedural nodes (TextureEval.cuh:238-404) blend
+ * two inputs child[0] ("black") and child[1] ("white"), each another texture
+ * node or YMRT_INVALID_INDEX for black; a graph may nest 8 deep. */
+#define YMRT_TEXTURE_IMAGE 0u
+#define YMRT_TEXTURE_CONSTANT_FLOAT 1u
+#define YMRT_TEXTURE_CONSTANT_RGB 2u
+#define YMRT_TEXTURE_CHECKERBOARD 3u
+#define YMRT_TEXTURE_NOISE 4u
+#define YMRT_TEXTURE_FBM 5u
+#define YMRT_TEXTURE_TURBULENCE 6u
+#define YMRT_TEXTURE_MARBLE 7u
+#define YMRT_TEXTURE_WOOD 8u
+#define YMRT_TEXTURE_POLKA_DOT 9u
+#define YMRT_TEXTURE_WAVE 10u
+
 typedef struct ymrt_texture_desc {
     uint32_t width, height, channels;
     const uint8_t *texels;           /* width * height * channels             */
     int32_t wrap_mode;
     int32_t color_space;
     float scale_u, scale_v, offset_u, offset_v;
+    uint32_t type;                   /* YMRT_TEXTURE_*                        */
+    float value[3];                  /* CONSTANT_FLOAT uses value[0]         */
+    uint32_t child[2];               /* procedural inputs: black, white       */
+    float frequency, lacunarity, gain, amplitude, offset, variation, radius;
+    int32_t layer_count;
+    int32_t normalized;              /* NOISE: remap [-1,1] to [0,1]          */
+    int32_t turbulence_axes;         /* MARBLE: bit 0 x, bit 1 y, bit 2 z     */
+    float frequency1, lacunarity1, gain1;   /* WAVE: second fBm (the first uses */
+    int32_t layer_count1;                    /* frequency, lacunarity, gain)     */
 } ymrt_texture_desc;
 
 /* Density grid for heterogeneous media (no counterpart in YMRT, whose
@@ -300,6 +326,56 @@ int ymrt_halton_values(ymrt_scene *scene, const uint64_t *indices, uint32_t n,
 int ymrt_halton_enumerate(uint32_t width, uint32_t height, const uint32_t *px,
                           const uint32_t *py, const uint32_t *sample, uint32_t n,
                           uint64_t *out, void *stream);
+
+/* ----------------------------------------------------------- .ymtmp files
+ * Scene files in YMRT's text format, parsed as SceneParser::parsing_scene
+ * (src/core/YMParser.cpp:2330-2433) with the block readers 971-2231 and the
+ * SceneModule containers they fill.  The result owns a ymrt_scene_desc for
+ * ymrt_scene_create (set camera.aspect_ratio and halton_seed first: the file
+ * has neither).  A block that a reader rejects (missing name, undeclared or
+ * duplicate name, wrong index counts, missing inputs) is skipped and listed
+ * in the warnings, as the reference prints and skips it; a statement that is
+ * not Begin(Type) / End(Type) / attribute=value, a nested or unclosed block
+ * or an attribute that the block type does not know fails the whole load.
+ * ImageTexture reads 8-bit PNG from <dir>/texture_<stem>_<ext>/.  Not
+ * supported (YMRT_ERR_UNSUPPORTED): Volume density_texture_index (media take
+ * voxel grids here). */
+typedef struct ymrt_ymtmp ymrt_ymtmp;
+
+#define YMRT_YMTMP_TRANSFORM 0u
+#define YMRT_YMTMP_MESH 1u
+#define YMRT_YMTMP_SPHERE 2u
+#define YMRT_YMTMP_MATERIAL 3u
+#define YMRT_YMTMP_TEXTURE 4u
+#define YMRT_YMTMP_INSTANCE 5u
+#define YMRT_YMTMP_DISTANT_LIGHT 6u
+#define YMRT_YMTMP_VOLUME 7u
+
+/* SceneParser::parsing_scene: path must end in ".ymtmp". */
+int ymrt_ymtmp_load(const char *path, ymrt_ymtmp **out);
+/* The same from text in memory; scene_path (may be NULL) locates textures. */
+int ymrt_ymtmp_parse(const char *text, size_t length, const char *scene_path,
+                     ymrt_ymtmp **out);
+/* The loaded scene; valid until ymrt_ymtmp_free. */
+const ymrt_scene_desc *ymrt_ymtmp_scene(const ymrt_ymtmp *file);
+uint32_t ymrt_ymtmp_warning_count(const ymrt_ymtmp *file);
+const char *ymrt_ymtmp_warning(const ymrt_ymtmp *file, uint32_t i);
+/* The name -> (kind, index) map of load_scene_asset (YMParser.cpp:2284);
+ * index is YMRT_INVALID_INDEX for an ImageTexture whose file failed to load. */
+int ymrt_ymtmp_find(const ymrt_ymtmp *file, const char *name, uint32_t *kind,
+                    uint32_t *index);
+/* SceneModule::transforms / i_transforms[index] (InitTransformMatrix,
+ * src/Transform.h:36-60), column-major; either output may be NULL. */
+int ymrt_ymtmp_transform(const ymrt_ymtmp *file, uint32_t index,
+                         float *object_to_world, float *world_to_object);
+int ymrt_ymtmp_free(ymrt_ymtmp *file);
+
+/* Texture graph rooted at `texture` evaluated at n points (TextureEval,
+ * src/TextureEval.cuh:415-481, with the node functions 162-404):
+ * coords[i*8 .. i*8+7] = (s, t, p_world xyz, p_object xyz), out[i*3 .. i*3+2]
+ * = rgb.  Device pointers. */
+int ymrt_texture_eval(ymrt_scene *scene, uint32_t texture, const float *coords,
+                      uint32_t n, float *out, void *stream);
 
 #ifdef __cplusplus
 }

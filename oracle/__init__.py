@@ -74,6 +74,13 @@ def lib():
             L.or_srgb_to_linear.argtypes = [C.c_float]
             L.or_texture_eval.restype = V3
             L.or_texture_eval.argtypes = [vp, C.c_uint32, C.c_float, C.c_float]
+            L.or_texture_graph.restype = V3
+            L.or_texture_graph.argtypes = [vp, C.c_uint32, TexCoord]
+            L.or_probe_uv_differentials.argtypes = [vp, vp, vp, vp, vp, vp, C.c_uint32, C.c_uint32,
+                                                    C.c_float, C.c_float, vp]
+            L.or_glm_transform.argtypes = [vp, vp, vp, vp, vp, vp]
+            L.or_perlin.restype = C.c_float
+            L.or_perlin.argtypes = [V3]
             L.or_accumulate.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_int]
             L.or_tonemap.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_float, C.c_float]
             L.or_probe_ggx_d.restype = C.c_float
@@ -90,6 +97,10 @@ def lib():
 
 class V3(C.Structure):
     _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
+class TexCoord(C.Structure):
+    _fields_ = [("s", C.c_float), ("t", C.c_float), ("p_world", V3), ("p_object", V3)]
 
 
 def v3(a):
@@ -169,6 +180,24 @@ class Scene:
         r = lib().or_texture_eval(self.h, tex, u, v)
         return np.array([r.x, r.y, r.z], np.float32)
 
+    def uv_differentials(self, hp, sn, ng, dpdu, dpdv, width, height, sx=0.25, sy=0.25):
+        """(du/dx, dv/dx, du/dy, dv/dy) of the scene camera at hit point hp."""
+        a = [np.ascontiguousarray(x, np.float32) for x in (hp, sn, ng, dpdu, dpdv)]
+        out = np.zeros(4, np.float32)
+        lib().or_probe_uv_differentials(self.h, *[_p(x) for x in a], width, height, sx, sy, _p(out))
+        return out
+
+    def texture_graph(self, tex, coords):
+        """Texture graph rooted at ``tex`` at each row of coords (n, 8):
+        (s, t, p_world xyz, p_object xyz); returns (n, 3) float32."""
+        coords = np.asarray(coords, np.float32).reshape(-1, 8)
+        out = np.zeros((len(coords), 3), np.float32)
+        for i, c in enumerate(coords):
+            tc = TexCoord(float(c[0]), float(c[1]), v3(c[2:5]), v3(c[5:8]))
+            r = lib().or_texture_graph(self.h, tex, tc)
+            out[i] = (r.x, r.y, r.z)
+        return out
+
 
 def halton_table(seed):
     n = lib().or_halton_table_size()
@@ -191,6 +220,12 @@ def mt19937(seed, n):
     m = MT()
     lib().or_mt_seed(C.byref(m), seed)
     return np.array([lib().or_mt_next(C.byref(m)) for _ in range(n)], np.uint64)
+
+
+def perlin(p):
+    """Perlin noise of each point of p (n, 3)."""
+    p = np.asarray(p, np.float32).reshape(-1, 3)
+    return np.array([lib().or_perlin(v3(x)) for x in p], np.float32)
 
 
 def pcg4d(v):

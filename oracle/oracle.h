@@ -112,7 +112,13 @@ typedef struct {
     int wrap, color_space;
     float su, sv, ou, ov;
     float *texels;          /* width*height*channels decoded floats */
+    const ymrt_texture_desc *desc;   /* type, constants, inputs, parameters */
 } or_texture;
+
+typedef struct {
+    v2 st;
+    v3 p_world, p_object;
+} or_tex_coord;
 
 typedef struct {
     uint32_t nx, ny, nz;
@@ -221,6 +227,9 @@ int or_bsdf_sample(const ymrt_material_desc *m, v3 wo, float ray_ior, float ex_i
 float or_hg_eval(float g, v3 wo, v3 wi, float *pdf);
 int or_hg_sample(float g, float u0, float u1, v3 wo, v3 *wi, float *pdf);
 v3 or_texture_eval(const or_scene *sc, uint32_t tex, float u, float v);
+/* TextureEval over the texture graph rooted at tex */
+v3 or_texture_graph(const or_scene *sc, uint32_t tex, or_tex_coord tc);
+float or_perlin(v3 p);
 float or_srgb_to_linear(float s);
 
 void or_accumulate(const float *noise, float *accum, uint32_t w, uint32_t h, int frame);

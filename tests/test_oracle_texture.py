@@ -1,0 +1,320 @@
+"""Oracle pins for the texture graph (src/TextureEval.cuh), the camera's
+texture-coordinate differentials (src/Camera.h:56-140) and normal / bump
+mapping (src/BSDF.cuh:694-807).
+
+Perlin noise is pinned against what Perlin's "Improving Noise" (SIGGRAPH
+2002) fixes: zero at lattice points, the gradient at a lattice point is the
+published gradient picked by the textbook hash P[P[P[x]+y]+z] over the
+permutation table, the quintic fade along lattice edges, continuity across
+cell faces.  The composite nodes are checked against sums of the noise
+primitive, and normal / bump mapping against the closed-form irradiance of a
+tilted Lambert surface under a distant light.
+"""
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from ymrt_amd import abi, scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+# The 12 cube-edge gradients of Perlin (2002) padded to 16 with (1,1,0),
+# (-1,1,0), (0,-1,1), (0,-1,-1), indexed by hash & 15.
+GRAD = np.array([(1, 1, 0), (-1, 1, 0), (1, -1, 0), (-1, -1, 0), (1, 0, 1), (-1, 0, 1), (1, 0, -1), (-1, 0, -1),
+                 (0, 1, 1), (0, -1, 1), (0, 1, -1), (0, -1, -1), (1, 1, 0), (-1, 1, 0), (0, -1, 1), (0, -1, -1)],
+                np.float64)
+
+
+def _perm():
+    txt = open(os.path.join(ROOT, "ymrt_amd", "csrc", "perlin_perm.inc")).read()
+    body = re.sub(r"/\*.*?\*/", " ", txt, flags=re.S)
+    return np.array([int(x, 0) for x in body.split(",") if x.strip()], np.int64)
+
+
+def _hash(P, x, y, z):
+    x, y, z = x & 255, y & 255, z & 255
+    return P[(P[(P[x] + y) & 255] + z) & 255]
+
+
+def test_permutation_table():
+    """The 256-entry table of src/TextureEval.cuh:19-53 (its 512 entries are
+    these 256 twice) is a permutation; first and last entries as listed there."""
+    P = _perm()
+    assert len(P) == 256
+    assert sorted(P.tolist()) == list(range(256))
+    assert P[:4].tolist() == [172, 142, 240, 182] and P[-2:].tolist() == [170, 202]
+
+
+def test_perlin_zero_on_lattice():
+    rng = np.random.default_rng(1)
+    pts = rng.integers(-300, 300, size=(500, 3)).astype(np.float32)
+    assert np.all(oracle.perlin(pts) == 0.0)
+
+
+def test_perlin_gradient_at_lattice_points():
+    """noise(c + e) = g(hash(c)) . e + O(|e|^3) near lattice point c (|e| =
+    0.01: the other corners enter with weight fade(0.01) ~ 1e-5, and float
+    rounding of fade near 1 adds ~1e-6)."""
+    P = _perm()
+    rng = np.random.default_rng(2)
+    c = rng.integers(-40, 40, size=(400, 3))
+    d = rng.normal(size=(400, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    p = (c + 1e-2 * d).astype(np.float32)
+    e = p.astype(np.float64) - c
+    g = GRAD[[_hash(P, *ci) & 15 for ci in c]]
+    expect = (g * e).sum(1)
+    got = oracle.perlin(p).astype(np.float64)
+    assert np.allclose(got, expect, rtol=1e-3, atol=5e-5), np.abs(got - expect).max()
+    assert np.abs(expect).mean() > 3e-3
+
+
+def test_perlin_edge_closed_form():
+    """Along a lattice edge (y, z integer) only the two end gradients' x parts
+    act: n = g0x t + fade(t) (g1x (t - 1) - g0x t), fade = 6t^5 - 15t^4 + 10t^3."""
+    P = _perm()
+    rng = np.random.default_rng(3)
+    ijk = rng.integers(-50, 50, size=(300, 3))
+    t = rng.uniform(0.01, 0.99, 300).astype(np.float32)
+    p = np.stack([ijk[:, 0] + t, ijk[:, 1], ijk[:, 2]], 1).astype(np.float32)
+    t64 = p[:, 0].astype(np.float64) - ijk[:, 0]
+    g0 = GRAD[[_hash(P, i, j, k) & 15 for i, j, k in ijk]][:, 0]
+    g1 = GRAD[[_hash(P, i + 1, j, k) & 15 for i, j, k in ijk]][:, 0]
+    fade = t64 ** 3 * (t64 * (t64 * 6 - 15) + 10)
+    expect = g0 * t64 + fade * (g1 * (t64 - 1) - g0 * t64)
+    assert np.allclose(oracle.perlin(p), expect, atol=2e-6)
+
+
+def test_perlin_continuous_across_faces():
+    rng = np.random.default_rng(4)
+    for axis in range(3):
+        q = rng.uniform(-30, 30, size=(300, 3))
+        q[:, axis] = np.round(q[:, axis])
+        lo, hi = q.copy(), q.copy()
+        lo[:, axis] -= 2e-4
+        hi[:, axis] += 2e-4
+        diff = np.abs(oracle.perlin(lo) - oracle.perlin(hi))
+        assert diff.max() < 2e-3, (axis, diff.max())
+
+
+def test_perlin_bounded():
+    rng = np.random.default_rng(5)
+    v = oracle.perlin(rng.uniform(-100, 100, size=(5000, 3)))
+    assert np.abs(v).max() < 1.1 and v.std() > 0.1
+
+
+# ------------------------------------------------------------ graph nodes
+def _coords(n, seed=6):
+    rng = np.random.default_rng(seed)
+    c = np.zeros((n, 8), np.float32)
+    c[:, 0:2] = rng.uniform(-2, 3, size=(n, 2))
+    c[:, 2:5] = rng.uniform(-5, 5, size=(n, 3))
+    c[:, 5:8] = rng.uniform(-5, 5, size=(n, 3))
+    return c
+
+
+def _fbm(p, lac, gain, layers, amp, off, turb=False):
+    f32 = np.float32
+    r = np.zeros(len(p), f32)
+    amp, off, freq = f32(amp), f32(off), f32(1.0)
+    for _ in range(layers):
+        nz = oracle.perlin(freq * p)
+        r = r + (amp * (np.abs(nz) if turb else nz) + off)
+        amp, off, freq = f32(amp * f32(gain)), f32(off * f32(gain)), f32(freq * f32(lac))
+    return r
+
+
+@pytest.fixture(scope="module")
+def graph():
+    sa, n = scenes.texture_node_set()
+    sc = oracle.Scene(sa)
+    yield sc, n
+    sc.close()
+
+
+def test_constant_and_checker_nodes(graph):
+    sc, n = graph
+    c = _coords(200)
+    assert np.all(sc.texture_graph(n["float"], c) == np.float32(0.375))
+    assert np.all(sc.texture_graph(n["rgb"], c) == np.array([0.25, 0.5, 0.75], np.float32))
+    # frequency pi: sign of sin(pi x) sin(pi y) sin(pi z) picks white where negative
+    pts = np.zeros((3, 8), np.float32)
+    pts[:, 5:8] = [(0.5, 0.5, 0.5), (1.5, 0.5, 0.5), (1.5, 1.5, 0.5)]
+    got = sc.texture_graph(n["checker"], pts)
+    assert np.allclose(got[0], (0.25, 0.5, 0.75)) and np.allclose(got[1], 0.375) and np.allclose(got[2], (0.25, 0.5, 0.75))
+
+
+def test_noise_fbm_turbulence_nodes(graph):
+    sc, n = graph
+    c = _coords(300)
+    po = c[:, 5:8]
+    f32 = np.float32
+    nz = oracle.perlin(po * f32(3.0))
+    assert np.allclose(sc.texture_graph(n["noise"], c)[:, 0], nz, atol=1e-6)
+    assert np.allclose(sc.texture_graph(n["noise_n"], c)[:, 0], nz * 0.5 + 0.5, atol=1e-6)
+    p = f32(2.5) * po
+    assert np.allclose(sc.texture_graph(n["fbm"], c)[:, 1], _fbm(p, 2.1, 0.45, 5, 0.9, 0.05), atol=2e-6)
+    assert np.allclose(sc.texture_graph(n["turb"], c)[:, 2], _fbm(p, 2.1, 0.45, 5, 0.9, 0.05, True), atol=2e-6)
+
+
+def test_marble_wood_wave_nodes(graph):
+    sc, n = graph
+    c = _coords(300)
+    po = c[:, 5:8]
+    f32 = np.float32
+    fbm = _fbm(f32(2.0) * po, 2.0, 0.5, 4, 1.0, 0.0)
+    marble = np.sin(po[:, 0] + po[:, 2] + f32(5.0) * fbm) * 0.5 + 0.5      # axes 5 = x and z
+    assert np.allclose(sc.texture_graph(n["marble"], c)[:, 0], marble, atol=1e-5)
+    w = oracle.perlin(po * f32(1.5)) * f32(7.0)
+    wood = w - np.floor(w)
+    got = sc.texture_graph(n["wood"], c)[:, 0]
+    ok = np.isclose(got, wood, atol=1e-5) | (np.minimum(wood, 1 - wood) < 1e-5)
+    assert ok.all()
+    f0 = _fbm(po * f32(4.0), 2.0, 0.5, 3, 1.0, 0.0)
+    f1 = _fbm(po * f32(1.5), 1.8, 0.7, 2, 1.0, 0.0)
+    assert np.allclose(sc.texture_graph(n["wave"], c)[:, 0], np.abs(f0) * f1, atol=1e-5)
+
+
+def test_polka_dot_node(graph):
+    """Cells whose centre noise is not positive are black; in the others the
+    white points lie within radius of one centre per cell."""
+    sc, n = graph
+    rng = np.random.default_rng(7)
+    c = np.zeros((4000, 8), np.float32)
+    c[:, 0:2] = rng.uniform(0, 4, size=(4000, 2))
+    got = sc.texture_graph(n["polka"], c)[:, 0]
+    assert set(np.unique(got).tolist()) <= {0.0, 1.0}
+    st = c[:, 0:2] * np.float32(5.0)
+    cell = np.floor(st)
+    centre = oracle.perlin(np.concatenate([cell + 0.5, np.zeros((len(c), 1), np.float32)], 1))
+    assert np.all(got[centre <= 0] == 0.0)
+    assert 0.02 < got.mean() < 0.5
+    # white points lie within the radius of the cell's unit square
+    d = st - cell
+    assert np.all(np.abs(d[got == 1.0] - 0.5) <= 0.5)
+
+
+def test_nested_graph_and_empty_input(graph):
+    """Each node reads its inputs' results: the nested checkerboard returns 0
+    (empty black input) or the inner fBm of (normalized noise, constant)."""
+    sc, n = graph
+    c = _coords(300)
+    po = c[:, 5:8]
+    f32 = np.float32
+    s = np.sin(po * f32(2.0))
+    white = (s[:, 0] * s[:, 1] * s[:, 2]) < 0
+    got = sc.texture_graph(n["nested"], c)
+    inner = sc.texture_graph(n["inner"], c)
+    assert np.all(got[~white] == 0.0)
+    assert np.array_equal(got[white], inner[white])
+    # inner: black = normalized noise (grey), white = (0.25, 0.5, 0.75)
+    r = _fbm(f32(2.5) * po, 2.1, 0.45, 5, 0.9, 0.05)
+    b = oracle.perlin(po * f32(3.0)) * 0.5 + 0.5
+    for ch, wv in enumerate((0.25, 0.5, 0.75)):
+        assert np.allclose(inner[:, ch], b + r * (wv - b), atol=1e-5)
+
+
+# ------------------------------------------------- differentials and maps
+def _floor_scene(mat, size=1.0):
+    sa = abi.SceneArrays()
+    s = size
+    sa.add_instance(scenes._add(sa, scenes.quad((-s, 0, -s), (-s, 0, s), (s, 0, s), (s, 0, -s))), mat(sa))
+    flip = np.diag([1.0, -1.0, -1.0, 1.0]).astype(np.float32)   # the light frame's -y points up
+    sa.add_distant_light((1.0, 1.0, 1.0), 1.0, 30.0, flip)
+    sa.set_camera((0.0, 0.5, 0.0001), (0.0, 0.0, 0.0), 0.2, 1.0)
+    return sa
+
+
+def test_uv_differentials_closed_form():
+    """Camera at distance d = 0.5 in front of the z = 0 wall, looking along -z:
+    the footprint of a pixel is 2 tan(fov/2) * aspect * d / W across and
+    2 tan(fov/2) * d / H down; with dpdu = (2,0,0), dpdv = (0,2,0) the uv
+    footprint is half of it, along u for x and along v for y."""
+    sa = _floor_scene(lambda s: s.add_material(abi.default_material()))
+    sa.set_camera((0.0, 0.0, 0.5), (0.0, 0.0, 0.0), 0.2, 2.0)
+    sc = oracle.Scene(sa)
+    W, H = 64, 32
+    z = np.array([0.0, 0.0, 1.0], np.float32)
+    duv = sc.uv_differentials((0, 0, 0), z, z, (2, 0, 0), (0, 2, 0), W, H, 0.25, 0.25).astype(np.float64)
+    iph = 2 * math.tan(math.radians(0.1))    # set_camera takes the field of view in degrees
+    fx = iph * 2.0 * 0.5 / W / 2
+    fy = iph * 0.5 / H / 2
+    assert abs(duv[0] - fx) < 1e-3 * fx and abs(duv[1]) < 1e-3 * fx
+    assert abs(duv[3] - fy) < 1e-3 * fy and abs(duv[2]) < 1e-3 * fy
+    # the differential rays are the local (0,0,-1) plus the WORLD right / up
+    # vectors (src/Camera.h:93-94); looking straight down, world up is the
+    # view axis and the y footprint vanishes
+    sa.set_camera((0.0, 0.5, 0.0001), (0.0, 0.0, 0.0), 0.2, 1.0)
+    sc.close()
+    sc = oracle.Scene(sa)
+    up = np.array([0.0, 1.0, 0.0], np.float32)
+    duv = sc.uv_differentials((0, 0, 0.0001), up, up, (0, 0, 2), (2, 0, 0), W, H, 0.25, 0.25)
+    sc.close()
+    assert duv[2] == 0.0 and duv[3] == 0.0 and abs(math.hypot(duv[0], duv[1]) - iph * 0.5 / W / 2) < 1e-3 * fx
+
+
+def _render_mean(sa, spp=512):
+    sc = oracle.Scene(sa)
+    rs = abi.render_settings(8, 8, spp, 1, sampler=abi.SAMPLER_PCG, shape_light=0, distant_light=1)
+    img = sc.render(rs, 1, threads=8)
+    sc.close()
+    return img
+
+
+def test_normal_map_tilt_closed_form():
+    """A constant normal map (0.5 + 0.5 sin a, 0.5, 0.5 + 0.5 cos a) tilts the
+    shading normal by a; under a distant light whose 30-degree cone is centred
+    on the geometric normal the Lambert estimate is the cone average
+    albedo / pi * Le * cos(a) * (1 + cos 30) / 2."""
+    a = math.radians(35.0)
+
+    def mat(sa):
+        nm = sa.add_texture_node(abi.TEXTURE_CONSTANT_RGB, value=(0.5 + 0.5 * math.sin(a), 0.5, 0.5 + 0.5 * math.cos(a)))
+        return sa.add_material(abi.default_material(diffuse_albedo=(0.5, 0.5, 0.5), specular_weight=0.0,
+                                                    normal_mapping_tex=nm))
+    got = float(_render_mean(_floor_scene(mat))[..., :3].mean())
+    expect = 0.5 / math.pi * math.cos(a) * (1 + math.cos(math.radians(30.0))) / 2
+    assert abs(got - expect) < 0.01 * expect, (got, expect)
+
+
+def test_flat_maps_leave_frame_unchanged():
+    """A normal map of (0.5, 0.5, 1) and a constant bump height reproduce the
+    unmapped frame sample for sample."""
+    base = lambda sa: sa.add_material(abi.default_material(diffuse_albedo=(0.5, 0.5, 0.5), roughness_x=0.3,
+                                                           roughness_y=0.3))
+
+    def nmap(sa):
+        t = sa.add_texture_node(abi.TEXTURE_CONSTANT_RGB, value=(0.5, 0.5, 1.0))
+        return sa.add_material(abi.default_material(diffuse_albedo=(0.5, 0.5, 0.5), roughness_x=0.3,
+                                                    roughness_y=0.3, normal_mapping_tex=t))
+
+    def bump(sa):
+        t = sa.add_texture_node(abi.TEXTURE_CONSTANT_FLOAT, value=(0.3, 0.0, 0.0))
+        return sa.add_material(abi.default_material(diffuse_albedo=(0.5, 0.5, 0.5), roughness_x=0.3,
+                                                    roughness_y=0.3, bump_mapping_tex=t))
+    ref = _render_mean(_floor_scene(base), spp=16)
+    for m in (nmap, bump):
+        img = _render_mean(_floor_scene(m), spp=16)
+        assert np.allclose(img, ref, rtol=1e-4, atol=1e-6), m.__name__
+
+
+def test_bump_ramp_closed_form():
+    """A height ramp h(u) = (256 u - 0.5) / 255 (bilinear over a 256-texel
+    linear ramp) on a floor with dpdu = (0,0,2), dpdv = (2,0,0) turns the
+    shading normal to normalize(cross((0, h', 2), (2, 0, 0))), a tilt of
+    atan(h' / 2); the distant-light estimate scales with its cosine."""
+    ramp = np.tile(np.arange(256, dtype=np.uint8)[None, :], (2, 1))
+
+    def mat(sa):
+        t = sa.add_texture(ramp, wrap=abi.WRAP_CLAMP, color_space=abi.COLOR_LINEAR)
+        return sa.add_material(abi.default_material(diffuse_albedo=(0.5, 0.5, 0.5), specular_weight=0.0,
+                                                    bump_mapping_tex=t))
+    got = float(_render_mean(_floor_scene(mat))[..., :3].mean())
+    hp = 256.0 / 255.0
+    cos_a = 2.0 / math.sqrt(4.0 + hp * hp)
+    expect = 0.5 / math.pi * cos_a * (1 + math.cos(math.radians(30.0))) / 2
+    assert abs(got - expect) < 0.01 * expect, (got, expect)

@@ -21,7 +21,9 @@ EXPORTS = [
     "ymrt_last_error", "ymrt_abi_version", "ymrt_context_create", "ymrt_context_destroy",
     "ymrt_scene_create", "ymrt_scene_destroy", "ymrt_scene_get_info", "ymrt_trace_closest",
     "ymrt_trace_any", "ymrt_trace_closest_counted", "ymrt_render_frame", "ymrt_accumulate",
-    "ymrt_tonemap", "ymrt_halton_values", "ymrt_halton_enumerate",
+    "ymrt_tonemap", "ymrt_halton_values", "ymrt_halton_enumerate", "ymrt_texture_eval",
+    "ymrt_ymtmp_load", "ymrt_ymtmp_parse", "ymrt_ymtmp_scene", "ymrt_ymtmp_warning_count",
+    "ymrt_ymtmp_warning", "ymrt_ymtmp_find", "ymrt_ymtmp_transform", "ymrt_ymtmp_free",
 ]
 
 _lib = None
@@ -60,9 +62,21 @@ def lib():
         L.ymrt_tonemap.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_float, C.c_float, vp]
         L.ymrt_halton_values.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp]
         L.ymrt_halton_enumerate.argtypes = [C.c_uint32, C.c_uint32, vp, vp, vp, C.c_uint32, vp, vp]
+        L.ymrt_texture_eval.argtypes = [vp, C.c_uint32, vp, C.c_uint32, vp, vp]
+        L.ymrt_ymtmp_load.argtypes = [C.c_char_p, C.POINTER(vp)]
+        L.ymrt_ymtmp_parse.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p, C.POINTER(vp)]
+        L.ymrt_ymtmp_scene.argtypes = [vp]
+        L.ymrt_ymtmp_warning_count.argtypes = [vp]
+        L.ymrt_ymtmp_warning.argtypes = [vp, C.c_uint32]
+        L.ymrt_ymtmp_find.argtypes = [vp, C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.ymrt_ymtmp_transform.argtypes = [vp, C.c_uint32, vp, vp]
+        L.ymrt_ymtmp_free.argtypes = [vp]
         for name in EXPORTS:
             if name not in ("ymrt_last_error", "ymrt_abi_version"):
                 getattr(L, name).restype = C.c_int
+        L.ymrt_ymtmp_scene.restype = C.POINTER(abi.SceneDesc)
+        L.ymrt_ymtmp_warning_count.restype = C.c_uint32
+        L.ymrt_ymtmp_warning.restype = C.c_char_p
         _lib = L
     return _lib
 
@@ -172,6 +186,17 @@ class Scene:
         _check(lib().ymrt_halton_values(self.h, _dev_ptr(idx), len(indices), dim0, dims, _dev_ptr(out), _stream()))
         return out.cpu().numpy()
 
+    # ------------------------------------------------------------ textures
+    def texture_eval(self, tex, coords):
+        """coords: (n, 8) float32 (s, t, p_world xyz, p_object xyz); returns
+        (n, 3) float32 rgb of the texture graph rooted at ``tex``."""
+        import torch
+        c = torch.from_numpy(np.ascontiguousarray(coords, np.float32).reshape(-1, 8).copy()).cuda()
+        n = c.shape[0]
+        out = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+        _check(lib().ymrt_texture_eval(self.h, tex, _dev_ptr(c), n, _dev_ptr(out), _stream()))
+        return out.cpu().numpy()
+
     # ------------------------------------------------------------ rendering
     def render_frame(self, settings, frame=1, noise=None, timing=False):
         import torch
@@ -181,6 +206,56 @@ class Scene:
         _check(lib().ymrt_render_frame(self.h, C.byref(settings), frame, _dev_ptr(noise), _stream(),
                                        1 if timing else 0, C.byref(stats)))
         return noise, stats
+
+
+class YmtmpFile:
+    """A .ymtmp scene file loaded by the library's parser (ymrt_ymtmp_load);
+    needs no GPU.  ``arrays()`` copies the scene into an abi.SceneArrays."""
+
+    KINDS = ("transform", "mesh", "sphere", "material", "texture", "instance", "distant_light", "volume")
+
+    def __init__(self, path=None, text=None, scene_path=None):
+        self.h = C.c_void_p()
+        if text is not None:
+            b = text.encode() if isinstance(text, str) else bytes(text)
+            sp = scene_path.encode() if scene_path else None
+            _check(lib().ymrt_ymtmp_parse(b, len(b), sp, C.byref(self.h)))
+        else:
+            _check(lib().ymrt_ymtmp_load(os.fspath(path).encode(), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().ymrt_ymtmp_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def warnings(self):
+        n = lib().ymrt_ymtmp_warning_count(self.h)
+        return [lib().ymrt_ymtmp_warning(self.h, i).decode() for i in range(n)]
+
+    def find(self, name):
+        """(kind, index) of a named asset; kind is one of KINDS."""
+        k, i = C.c_uint32(), C.c_uint32()
+        _check(lib().ymrt_ymtmp_find(self.h, name.encode(), C.byref(k), C.byref(i)))
+        return self.KINDS[k.value], i.value
+
+    def transform(self, index):
+        """(object_to_world, world_to_object) as row-major 4x4 float32."""
+        a, b = np.zeros(16, np.float32), np.zeros(16, np.float32)
+        _check(lib().ymrt_ymtmp_transform(self.h, index, a.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p)))
+        return a.reshape(4, 4).T.copy(), b.reshape(4, 4).T.copy()
+
+    def desc(self):
+        return lib().ymrt_ymtmp_scene(self.h).contents
+
+    def arrays(self):
+        return abi.arrays_from_desc(self.desc())
 
 
 def halton_enumerate(width, height, px, py, sample):

@@ -65,16 +65,26 @@ class MaterialDesc(C.Structure):
         ("coat_roughness_x", C.c_float), ("coat_roughness_x_tex", C.c_uint32),
         ("coat_roughness_y", C.c_float), ("coat_roughness_y_tex", C.c_uint32),
         ("coat_ior", C.c_float),
+        ("normal_mapping_tex", C.c_uint32), ("bump_mapping_tex", C.c_uint32),
         ("light_color", C.c_float * 3), ("intensity", C.c_float),
     ]
+
+
+TEXTURE_IMAGE, TEXTURE_CONSTANT_FLOAT, TEXTURE_CONSTANT_RGB, TEXTURE_CHECKERBOARD = 0, 1, 2, 3
+TEXTURE_NOISE, TEXTURE_FBM, TEXTURE_TURBULENCE, TEXTURE_MARBLE = 4, 5, 6, 7
+TEXTURE_WOOD, TEXTURE_POLKA_DOT, TEXTURE_WAVE = 8, 9, 10
 
 
 class TextureDesc(C.Structure):
     _fields_ = [
         ("width", C.c_uint32), ("height", C.c_uint32), ("channels", C.c_uint32),
-        ("texels", u8p), ("wrap_mode", C.c_int32), ("color_space", C.c_int32),
-        ("scale_u", C.c_float), ("scale_v", C.c_float),
-        ("offset_u", C.c_float), ("offset_v", C.c_float),
+        ("texels", C.POINTER(C.c_uint8)), ("wrap_mode", C.c_int32), ("color_space", C.c_int32),
+        ("scale_u", C.c_float), ("scale_v", C.c_float), ("offset_u", C.c_float), ("offset_v", C.c_float),
+        ("type", C.c_uint32), ("value", C.c_float * 3), ("child", C.c_uint32 * 2),
+        ("frequency", C.c_float), ("lacunarity", C.c_float), ("gain", C.c_float), ("amplitude", C.c_float),
+        ("offset", C.c_float), ("variation", C.c_float), ("radius", C.c_float),
+        ("layer_count", C.c_int32), ("normalized", C.c_int32), ("turbulence_axes", C.c_int32),
+        ("frequency1", C.c_float), ("lacunarity1", C.c_float), ("gain1", C.c_float), ("layer_count1", C.c_int32),
     ]
 
 
@@ -281,6 +291,15 @@ class SceneArrays:
         self.textures.append(dict(texels=t, wrap=wrap, cs=color_space, scale=scale, offset=offset))
         return len(self.textures) - 1
 
+    def add_texture_node(self, type, child=(None, None), value=(0.0, 0.0, 0.0), **params):
+        """Constant or procedural texture node; child holds the indices of the
+        black and white inputs (None: black).  params: frequency, lacunarity,
+        gain, amplitude, offset, variation, radius, layer_count, normalized,
+        turbulence_axes, frequency1, lacunarity1, gain1, layer_count1."""
+        self.textures.append(dict(type=type, child=tuple(INVALID if c is None else int(c) for c in child),
+                                  value=tuple(float(v) for v in value), params=params))
+        return len(self.textures) - 1
+
     def add_grid(self, density):
         d = np.ascontiguousarray(density, np.float32)
         assert d.ndim == 3  # (nz, ny, nx)
@@ -292,13 +311,17 @@ class SceneArrays:
         self.volumes.append(dict(sigma_t=sigma_t, albedo=albedo, g=g, grid=grid, w2o=w2o))
         return len(self.volumes) - 1
 
-    def add_distant_light(self, color, intensity=1.0, theta_max_deg=5.0, object_to_world=None):
+    def add_distant_light(self, color, intensity=1.0, theta_max_deg=5.0, object_to_world=None,
+                          world_to_object=None):
         """Distant light shining along -y of its frame (object_to_world)."""
         o2w = np.eye(4, dtype=np.float32) if object_to_world is None else np.asarray(object_to_world, np.float32)
         o2w = o2w.copy()
         o2w[3, :] = (0.0, 0.0, 0.0, 1.0)
-        w2o = np.linalg.inv(o2w.astype(np.float64)).astype(np.float32)
-        w2o[3, :] = (0.0, 0.0, 0.0, 1.0)
+        if world_to_object is None:
+            w2o = np.linalg.inv(o2w.astype(np.float64)).astype(np.float32)
+            w2o[3, :] = (0.0, 0.0, 0.0, 1.0)
+        else:
+            w2o = np.array(world_to_object, np.float32)
         self.distant_lights.append(dict(color=tuple(float(c) for c in color), intensity=float(intensity),
                                         theta=float(theta_max_deg), o2w=o2w, w2o=w2o))
         return len(self.distant_lights) - 1
@@ -345,12 +368,22 @@ class SceneArrays:
         texs = (TextureDesc * max(1, len(self.textures)))()
         for i, t in enumerate(self.textures):
             d = texs[i]
-            h, w, c = t["texels"].shape
-            d.width, d.height, d.channels = w, h, c
-            d.texels = _ptr(t["texels"], C.c_uint8)
-            d.wrap_mode, d.color_space = t["wrap"], t["cs"]
-            d.scale_u, d.scale_v = t["scale"]
-            d.offset_u, d.offset_v = t["offset"]
+            d.child[:] = [INVALID, INVALID]
+            d.amplitude, d.gain, d.lacunarity = 1.0, 0.5, 2.0
+            if "texels" in t:
+                h, w, c = t["texels"].shape
+                d.type = TEXTURE_IMAGE
+                d.width, d.height, d.channels = w, h, c
+                d.texels = _ptr(t["texels"], C.c_uint8)
+                d.wrap_mode, d.color_space = t["wrap"], t["cs"]
+                d.scale_u, d.scale_v = t["scale"]
+                d.offset_u, d.offset_v = t["offset"]
+                continue
+            d.type = t["type"]
+            d.value[:] = list(t["value"])
+            d.child[:] = list(t["child"])
+            for k, v in t["params"].items():
+                setattr(d, k, v)
         grids = (GridDesc * max(1, len(self.grids)))()
         for i, g in enumerate(self.grids):
             grids[i].nz, grids[i].ny, grids[i].nx = g.shape
@@ -409,3 +442,68 @@ def render_settings(width, height, spp, depth, sampler=SAMPLER_HALTON, shape_lig
     rs.exposure = 1.0
     rs.path_capacity = path_capacity
     return rs
+
+
+def _np(ptr, n, dtype):
+    if n == 0 or not ptr:
+        return np.zeros(0, dtype)
+    ct = {np.float32: C.c_float, np.uint32: C.c_uint32, np.uint8: C.c_uint8}[dtype]
+    return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), shape=(n,)).astype(dtype, copy=True)
+
+
+def _mat(cm):
+    """column-major 16 floats -> row-major 4x4"""
+    return np.array(list(cm), np.float32).reshape(4, 4).T.copy()
+
+
+def arrays_from_desc(sd):
+    """A SceneArrays holding copies of everything a SceneDesc points to."""
+    sa = SceneArrays()
+    for i in range(sd.mesh_count):
+        m = sd.meshes[i]
+        nt = m.triangle_count
+        pos = _np(m.positions, 3 * m.position_count, np.float32)
+        pidx = _np(m.position_indices, 3 * nt, np.uint32)
+        kw = {}
+        if m.normals and m.normal_indices:
+            kw.update(nrm=_np(m.normals, 3 * m.normal_count, np.float32),
+                      nidx=_np(m.normal_indices, 3 * nt, np.uint32))
+        if m.texcoords and m.texcoord_indices:
+            kw.update(uv=_np(m.texcoords, 2 * m.texcoord_count, np.float32),
+                      tidx=_np(m.texcoord_indices, 3 * nt, np.uint32))
+        sa.add_mesh(pos, pidx, **kw)
+    for i in range(sd.sphere_count):
+        sa.add_sphere(sd.spheres[i].radius)
+    for i in range(sd.material_count):
+        m = MaterialDesc()
+        C.memmove(C.byref(m), C.byref(sd.materials[i]), C.sizeof(MaterialDesc))
+        sa.add_material(m)
+    for i in range(sd.texture_count):
+        t = sd.textures[i]
+        if t.type == TEXTURE_IMAGE:
+            tex = _np(t.texels, t.width * t.height * t.channels, np.uint8).reshape(t.height, t.width, t.channels)
+            sa.add_texture(tex, wrap=t.wrap_mode, color_space=t.color_space, scale=(t.scale_u, t.scale_v),
+                           offset=(t.offset_u, t.offset_v))
+        else:
+            params = {k: getattr(t, k) for k in ("frequency", "lacunarity", "gain", "amplitude", "offset", "variation",
+                                                 "radius", "layer_count", "normalized", "turbulence_axes",
+                                                 "frequency1", "lacunarity1", "gain1", "layer_count1")}
+            sa.add_texture_node(t.type, tuple(None if c == INVALID else c for c in t.child), tuple(t.value),
+                                **params)
+    for i in range(sd.volume_count):
+        v = sd.volumes[i]
+        sa.add_volume(tuple(v.sigma_t), tuple(v.albedo), v.g, grid=v.density_grid,
+                      world_to_object=_mat(v.world_to_object))
+    for i in range(sd.instance_count):
+        d = sd.instances[i]
+        geom = SphereRef(d.geometry) if d.geometry_type == GEOMETRY_SPHERE else d.geometry
+        sa.add_instance(geom, d.material, _mat(d.object_to_world), _mat(d.world_to_object),
+                        inner_volume=d.inner_volume, boundary=bool(d.treat_as_boundary))
+    for i in range(sd.distant_light_count):
+        d = sd.distant_lights[i]
+        sa.add_distant_light(tuple(d.light_color), d.intensity, d.theta_max, _mat(d.object_to_world),
+                             _mat(d.world_to_object))
+    c = sd.camera
+    sa.camera = dict(position=tuple(c.position), w=tuple(c.w), fov=float(c.fov), aspect=float(c.aspect_ratio))
+    sa.halton_seed = sd.halton_seed
+    return sa

@@ -180,6 +180,16 @@ int ymrt_halton_values(ymrt_scene *sc, const uint64_t *indices, uint32_t n, uint
     return YMRT_OK;
 }
 
+int ymrt_texture_eval(ymrt_scene *sc, uint32_t texture, const float *coords, uint32_t n, float *out, void *stream)
+{
+    YM_REQUIRE(sc != nullptr, "scene is NULL");
+    YM_REQUIRE(n == 0 || (coords && out), "NULL buffer");
+    int rc = launch_texture_eval(sc->dev, texture, coords, n, out, (hipStream_t)stream);
+    if (rc) return rc;
+    YM_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return YMRT_OK;
+}
+
 int ymrt_halton_enumerate(uint32_t width, uint32_t height, const uint32_t *px, const uint32_t *py,
                           const uint32_t *sample, uint32_t n, uint64_t *out, void *stream)
 {

@@ -59,7 +59,7 @@ struct alignas(16) ShadeTri {      // 96 B, by original triangle index
 };
 static_assert(sizeof(ShadeTri) == 96, "ShadeTri size");
 
-struct DevMaterial {               // ymrt_material_desc, 128 B
+struct DevMaterial {               // ymrt_material_desc, 136 B
     uint32_t type;
     float diffuse_albedo[3]; uint32_t diffuse_albedo_tex;
     float specular_albedo[3]; uint32_t specular_albedo_tex;
@@ -75,6 +75,8 @@ struct DevMaterial {               // ymrt_material_desc, 128 B
     float coat_roughness_x; uint32_t coat_roughness_x_tex;
     float coat_roughness_y; uint32_t coat_roughness_y_tex;
     float coat_ior;
+    uint32_t normal_mapping_tex;
+    uint32_t bump_mapping_tex;
     float light_color[3];
     float intensity;
 };
@@ -84,7 +86,18 @@ struct DevTexture {
     int32_t wrap, color_space;
     float su, sv, ou, ov;
     uint64_t offset;               // first texel (uint32 RGBA8 words) in the texel pool
+    uint32_t type;                 // YMRT_TEXTURE_*
+    float value[3];
+    float frequency, lacunarity, gain, amplitude, offset_f, variation, radius;
+    int32_t layers, normalized, axes;
+    float frequency1, lacunarity1, gain1;
+    int32_t layers1;
+    uint32_t prog_start, prog_len; // post-order program of the graph rooted here
 };
+
+// Texture program codes: a texture index, or kTexBlack for an unconnected input.
+constexpr uint32_t kTexBlack = 0xFFFFFFFEu;
+constexpr int kTexStack = 8;       // TextureEval's texture_stack_size
 
 struct DevGrid {
     uint32_t nx, ny, nz;
@@ -134,6 +147,7 @@ struct DevScene {
     const ShadeTri *shade;
     const DevMaterial *materials;
     const DevTexture *textures;
+    const uint32_t *tex_prog;      // texture programs (codes)
     const uint32_t *texels;
     const float *tex_lut;          // 2 x 256 decode tables (sRGB, linear)
     const DevGrid *grids;
@@ -153,6 +167,7 @@ struct DevScene {
     uint32_t distant_count;
     uint32_t has_boundaries;       // any instance with treat_as_boundary
     uint32_t has_spheres;          // any sphere instance
+    uint32_t has_graph;            // procedural texture nodes or normal / bump maps
     DevCamera camera;
 };
 

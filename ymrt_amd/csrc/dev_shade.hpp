@@ -185,9 +185,8 @@ __device__ __forceinline__ f3 texel(const DevScene &sc, const DevTexture &t, int
     return r;
 }
 
-__device__ __forceinline__ f3 texture_eval(const DevScene &sc, uint32_t ti, f2 uv)
+__device__ __forceinline__ f3 image_eval(const DevScene &sc, const DevTexture &t, f2 uv)
 {
-    const DevTexture t = sc.textures[ti];
     float tu = uv.x * t.su + t.ou;
     if (tu < 0.0f) tu = gmax(0.0f, tu + fabsf(floorf(tu)));
     float tv = uv.y * t.sv + t.ov;
@@ -201,6 +200,203 @@ __device__ __forceinline__ f3 texture_eval(const DevScene &sc, uint32_t ti, f2 u
     r = r + (fx * fy) * texel(sc, t, x0 + 1, y0 + 1);
     return r;
 }
+
+// Perlin noise (PerlinNoiseEval, src/TextureEval.cuh:55-131): Ken Perlin's
+// improved noise over the reference's lattice permutation and its 16
+// gradient cases.
+__constant__ uint8_t kPerlinPerm[256] = {
+#include "perlin_perm.inc"
+};
+
+__device__ __forceinline__ int perlin_hash(int x, int y, int z)
+{
+    return kPerlinPerm[(kPerlinPerm[(kPerlinPerm[x] + y) & 255] + z) & 255];
+}
+
+__device__ __forceinline__ float perlin_grad(int h, float x, float y, float z)
+{
+    switch (h & 15) {
+    case 0: return x + y;
+    case 1: return -x + y;
+    case 2: return x - y;
+    case 3: return -x - y;
+    case 4: return x + z;
+    case 5: return -x + z;
+    case 6: return x - z;
+    case 7: return -x - z;
+    case 8: return y + z;
+    case 9: return -y + z;
+    case 10: return y - z;
+    case 11: return -y - z;
+    case 12: return y + x;
+    case 13: return -x + y;
+    case 14: return -y + z;
+    default: return -y - z;
+    }
+}
+
+__device__ __forceinline__ float perlin_fade(float t) { return t * t * t * (t * (t * 6.0f - 15.0f) + 10.0f); }
+
+__device__ __noinline__ float perlin_noise(f3 p)
+{
+    int x0 = (int)floorf(p.x), y0 = (int)floorf(p.y), z0 = (int)floorf(p.z);
+    const float tx = p.x - (float)x0, ty = p.y - (float)y0, tz = p.z - (float)z0;
+    const int x1 = (x0 + 1) & 255, y1 = (y0 + 1) & 255, z1 = (z0 + 1) & 255;
+    x0 &= 255;
+    y0 &= 255;
+    z0 &= 255;
+    const float v000 = perlin_grad(perlin_hash(x0, y0, z0), tx, ty, tz);
+    const float v100 = perlin_grad(perlin_hash(x1, y0, z0), tx - 1, ty, tz);
+    const float v010 = perlin_grad(perlin_hash(x0, y1, z0), tx, ty - 1, tz);
+    const float v001 = perlin_grad(perlin_hash(x0, y0, z1), tx, ty, tz - 1);
+    const float v110 = perlin_grad(perlin_hash(x1, y1, z0), tx - 1, ty - 1, tz);
+    const float v011 = perlin_grad(perlin_hash(x0, y1, z1), tx, ty - 1, tz - 1);
+    const float v101 = perlin_grad(perlin_hash(x1, y0, z1), tx - 1, ty, tz - 1);
+    const float v111 = perlin_grad(perlin_hash(x1, y1, z1), tx - 1, ty - 1, tz - 1);
+    const float ix = perlin_fade(tx), iy = perlin_fade(ty), iz = perlin_fade(tz);
+    const float v00 = v000 + ix * (v100 - v000);
+    const float v01 = v001 + ix * (v101 - v001);
+    const float v10 = v010 + ix * (v110 - v010);
+    const float v11 = v011 + ix * (v111 - v011);
+    const float v0 = v00 + iy * (v10 - v00);
+    const float v1 = v01 + iy * (v11 - v01);
+    return v0 + iz * (v1 - v0);
+}
+
+// PerlinNoiseFBM / PerlinNoiseTurbulence (src/TextureEval.cuh:132-160)
+__device__ __forceinline__ float perlin_layers(f3 p, float lacunarity, float gain, int layers, float amplitude,
+                                               float offset, bool turbulence)
+{
+    float r = 0.0f, frequency = 1.0f;
+    for (int i = 0; i < layers; ++i) {
+        const float n = perlin_noise(frequency * p);
+        r += amplitude * (turbulence ? fabsf(n) : n) + offset;
+        amplitude *= gain;
+        offset *= gain;
+        frequency *= lacunarity;
+    }
+    return r;
+}
+
+struct TexCoord {
+    f2 st;
+    f3 p_world, p_object;
+};
+
+// One node of a texture graph given its inputs (EvalTexResult and the EvalTex*
+// functions, src/TextureEval.cuh:162-404).
+__device__ __forceinline__ f3 texture_node(const DevScene &sc, const DevTexture &t, f3 black, f3 white,
+                                           const TexCoord &tc)
+{
+    const f3 po = tc.p_object;
+    switch (t.type) {
+    case YMRT_TEXTURE_IMAGE: return image_eval(sc, t, tc.st);
+    case YMRT_TEXTURE_CONSTANT_FLOAT: return s3(t.value[0]);
+    case YMRT_TEXTURE_CONSTANT_RGB: return f3{t.value[0], t.value[1], t.value[2]};
+    case YMRT_TEXTURE_CHECKERBOARD: {
+        const float x = sinf(po.x * t.frequency), y = sinf(po.y * t.frequency), z = sinf(po.z * t.frequency);
+        return x * y * z < 0.0f ? white : black;
+    }
+    case YMRT_TEXTURE_NOISE: {
+        float n = perlin_noise(po * t.frequency);
+        if (t.normalized) n = n * 0.5f + 0.5f;
+        return black + n * (white - black);
+    }
+    case YMRT_TEXTURE_FBM:
+    case YMRT_TEXTURE_TURBULENCE: {
+        const float r = perlin_layers(t.frequency * po, t.lacunarity, t.gain, t.layers, t.amplitude, t.offset_f,
+                                      t.type == YMRT_TEXTURE_TURBULENCE);
+        return black + r * (white - black);
+    }
+    case YMRT_TEXTURE_MARBLE: {
+        const float fbm = perlin_layers(t.frequency * po, t.lacunarity, t.gain, t.layers, 1.0f, 0.0f, false);
+        float turb = 0.0f;
+        turb += (t.axes & 1) ? po.x : 0.0f;
+        turb += (t.axes & 2) ? po.y : 0.0f;
+        turb += (t.axes & 4) ? po.z : 0.0f;
+        const float m = sinf(turb + t.variation * fbm) * 0.5f + 0.5f;
+        return black + m * (white - black);
+    }
+    case YMRT_TEXTURE_WOOD: {
+        float w = perlin_noise(po * t.frequency) * t.variation;
+        w = w - floorf(w);
+        return black + w * (white - black);
+    }
+    case YMRT_TEXTURE_POLKA_DOT: {
+        const f2 st{tc.st.x * t.frequency, tc.st.y * t.frequency};
+        const float fs = floorf(st.x), ft = floorf(st.y);
+        if (!(perlin_noise(f3{fs + 0.5f, ft + 0.5f, 0.0f}) > 0.0f)) return black;
+        auto hash = [](float f, float multi) {
+            const float x = f * multi;
+            return (x - floorf(x)) * 114.f;
+        };
+        const float max_offset = 0.5f - t.radius;
+        const float ds = st.x - fs, dt = st.y - ft;
+        const float sc0 = 0.5f + max_offset * perlin_noise(f3{hash(st.x, 1145.14f), hash(st.y, 9807.26f), 0.0f});
+        const float tc0 = 0.5f + max_offset * perlin_noise(f3{hash(st.x, 4444.44f), hash(st.y, 2333.33f), 0.0f});
+        return sqr(ds - sc0) + sqr(dt - tc0) < t.radius * t.radius ? white : black;
+    }
+    default: {  // YMRT_TEXTURE_WAVE
+        const float f0 = perlin_layers(po * t.frequency, t.lacunarity, t.gain, t.layers, 1.0f, 0.0f, false);
+        const float f1 = perlin_layers(po * t.frequency1, t.lacunarity1, t.gain1, t.layers1, 1.0f, 0.0f, false);
+        const float w = fabsf(f0) * f1;
+        return black + w * (white - black);
+    }
+    }
+}
+
+// Texture graph evaluation (TextureEval, src/TextureEval.cuh:415-481): the
+// graph's post-order program runs on a value stack of kTexStack entries kept
+// per lane in LDS ([entry * 3 + channel][lane]); a one-node graph (an image
+// or a constant) is evaluated inline without the stack.
+template <bool kGraph = true>
+struct TexEval {
+    const DevScene &sc;
+    TexCoord tc;
+    float *stk;
+    int lane;
+
+    __device__ __forceinline__ void put(int i, f3 v)
+    {
+        stk[(i * 3 + 0) * 64 + lane] = v.x;
+        stk[(i * 3 + 1) * 64 + lane] = v.y;
+        stk[(i * 3 + 2) * 64 + lane] = v.z;
+    }
+    __device__ __forceinline__ f3 get(int i) const
+    {
+        return f3{stk[(i * 3 + 0) * 64 + lane], stk[(i * 3 + 1) * 64 + lane], stk[(i * 3 + 2) * 64 + lane]};
+    }
+    __device__ __forceinline__ f3 eval(uint32_t ti, const TexCoord &at)
+    {
+        const DevTexture &root = sc.textures[ti];
+        if (root.type == YMRT_TEXTURE_IMAGE) return image_eval(sc, root, at.st);
+        if (root.type == YMRT_TEXTURE_CONSTANT_FLOAT) return s3(root.value[0]);
+        if (root.type == YMRT_TEXTURE_CONSTANT_RGB) return f3{root.value[0], root.value[1], root.value[2]};
+        if constexpr (kGraph) return graph(root, at);
+        return s3(0.0f);  // kGraph is false only for scenes without procedural nodes
+    }
+    __device__ __noinline__ f3 graph(const DevTexture &root, const TexCoord &at)
+    {
+        int sp = 0;
+        for (uint32_t k = 0; k < root.prog_len; ++k) {
+            const uint32_t code = sc.tex_prog[root.prog_start + k];
+            f3 v = s3(0.0f);
+            if (code != kTexBlack) {
+                const DevTexture &t = sc.textures[code];
+                f3 black = s3(0.0f), white = s3(0.0f);
+                if (t.type >= YMRT_TEXTURE_CHECKERBOARD) {
+                    sp -= 2;
+                    black = get(sp);
+                    white = get(sp + 1);
+                }
+                v = texture_node(sc, t, black, white, at);
+            }
+            put(sp++, v);
+        }
+        return get(sp - 1);
+    }
+    __device__ __forceinline__ f3 eval(uint32_t ti) { return eval(ti, tc); }
+};
 
 // ---------------------------------------------------------------- BSDF lobes
 // src/BSDF.cuh:20-375
@@ -465,13 +661,58 @@ struct Bsdf {
     }
 };
 
-YM_HD void init_shading_space(Bsdf &bs, int hit_back, f3 ray_d, f3 ng, f3 sn_in, f3 dpdu, f3 dpdv)
+// Camera::TextureCoordinatesDifferential (src/Camera.h:56-140): (du/dx,
+// dv/dx, du/dy, dv/dy) of a world-space hit, from rays through pixel
+// neighbours offset by (sx, sy) pixels.
+__device__ __forceinline__ float4 camera_uv_differentials(const DevCamera &cam, f3 hp, f3 sn, f3 ng, f3 dpdu,
+                                                          f3 dpdv, uint32_t width, uint32_t height, float sx,
+                                                          float sy)
 {
-    f3 sn = sn_in;
-    if (dot(sn_in, ng) < 0.0f) sn = -sn_in;
-    f3 su = dpdu;
-    if (dot(cross(dpdu, dpdv), ng) < 0.0f) su = -dpdu;
-    su = su - sn * dot(su, sn);
+    f3 look = cam.position - hp;
+    const float dist = length(look);
+    look = look * safe_rcp(dist);
+    const f3 right = normalize(cam.u - look * dot(cam.u, look));
+    const f3 up = cross(look, right);
+    const float iph = 2.0f * cam.tan_half_fov;
+    const float yoff = iph * sy / float(height);
+    const float ipw = iph * cam.aspect;
+    const float xoff = ipw * sx / float(width);
+    auto min_abs = [](float f, float th) { return fabsf(f) < th ? (f > 0.0f ? th : -th) : f; };
+    const f3 dxd = normalize(f3{0.0f, 0.0f, -1.0f} + right * xoff);
+    const f3 dyd = normalize(f3{0.0f, 0.0f, -1.0f} + up * yoff);
+    const f3 lsn{dot(sn, right), dot(sn, up), dot(sn, look)};
+    const f3 lp{0.0f, 0.0f, -dist};
+    const float tx = dot(lsn, lp) / min_abs(dot(lsn, dxd), 1E-4f);
+    f3 dpdx = (dxd * tx - lp) * safe_rcp(sx);
+    const float ty = dot(lsn, lp) / min_abs(dot(lsn, dyd), 1E-4f);
+    f3 dpdy = (dyd * ty - lp) * safe_rcp(sy);
+    dpdx = dpdx.x * right + dpdx.y * up + dpdx.z * look;
+    dpdy = dpdy.x * right + dpdy.y * up + dpdy.z * look;
+    const f3 an = abs3(ng);
+    int d0, d1;
+    if (an.z > an.y && an.z > an.x) {
+        d0 = 0;
+        d1 = 1;
+    } else if (an.y > an.x) {
+        d0 = 0;
+        d1 = 2;
+    } else {
+        d0 = 1;
+        d1 = 2;
+    }
+    const float m00 = comp(dpdu, d0), m01 = comp(dpdv, d0), m10 = comp(dpdu, d1), m11 = comp(dpdv, d1);
+    const float det = m00 * m11 - m10 * m01;
+    if (fabsf(det) < 1E-7f) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float idet = 1.0f / det;
+    const float dudx = m11 * comp(dpdx, d0) - m01 * comp(dpdx, d1);
+    const float dvdx = -m10 * comp(dpdx, d0) + m00 * comp(dpdx, d1);
+    const float dudy = m11 * comp(dpdy, d0) - m01 * comp(dpdy, d1);
+    const float dvdy = -m10 * comp(dpdy, d0) + m00 * comp(dpdy, d1);
+    return make_float4(dudx * idet, dvdx * idet, dudy * idet, dvdy * idet);
+}
+
+YM_HD void shading_frame(Bsdf &bs, int hit_back, f3 sn, f3 su)
+{
     if (!hit_back) {
         bs.n = sn;
         bs.t = normalize(su);
@@ -481,6 +722,55 @@ YM_HD void init_shading_space(Bsdf &bs, int hit_back, f3 ray_d, f3 ng, f3 sn_in,
         bs.t = -normalize(su);
         bs.b = -cross(bs.n, bs.t);
     }
+}
+
+// MaterialBSDF::InitShadingSpace (src/BSDF.cuh:694-807): shading frame from
+// the interpolated normal and dpdu, perturbed by a normal map or a bump map
+// (finite differences of the height over the texture-coordinate footprint
+// duv), then flipped to face the incoming ray.  normal_tex / bump_tex are
+// kEmpty when the material has none.
+template <class TX>
+__device__ __forceinline__ void init_shading_space(Bsdf &bs, TX &tx, uint32_t normal_tex, uint32_t bump_tex,
+                                                   int hit_back, f3 ray_d, f3 ng, f3 hsn, f3 dpdu, f3 dpdv,
+                                                   const Affine &w2o, float4 duv)
+{
+    f3 sn = hsn;
+    if (dot(hsn, ng) < 0.0f) sn = -hsn;
+    f3 su = dpdu;
+    if (dot(cross(dpdu, dpdv), ng) < 0.0f) su = -dpdu;
+    f3 sv = dpdv;
+    su = su - sn * dot(su, sn);
+    sv = sv - sn * dot(sv, sn);
+    if (normal_tex != kEmpty) {
+        shading_frame(bs, hit_back, sn, su);
+        const f3 tn = normalize(tx.eval(normal_tex) * 2.0f - s3(1.0f));
+        bs.n = tn.x * bs.t + tn.y * bs.b + tn.z * bs.n;
+        bs.t = normalize(bs.t - bs.n * dot(bs.t, bs.n));
+        bs.b = hit_back ? -cross(bs.n, bs.t) : cross(bs.n, bs.t);
+    } else if (bump_tex != kEmpty) {
+        const f3 osu = xf_vector(w2o, su), osv = xf_vector(w2o, sv);
+        float du = gmax((fabsf(duv.x) + fabsf(duv.z)) * 0.5f, 0.0005f);
+        if (!(du > 0.0f)) du = 0.0005f;
+        TexCoord tu = tx.tc;
+        tu.st = f2{tx.tc.st.x + du, tx.tc.st.y};
+        tu.p_world = tx.tc.p_world + su * du;
+        tu.p_object = tx.tc.p_object + osu * du;
+        const float hu = tx.eval(bump_tex, tu).x;
+        float dv = gmax((fabsf(duv.y) + fabsf(duv.w)) * 0.5f, 0.0005f);
+        if (!(dv > 0.0f)) dv = 0.0005f;
+        TexCoord tvv = tx.tc;
+        tvv.st = f2{tx.tc.st.x, tx.tc.st.y + dv};
+        tvv.p_world = tx.tc.p_world + sv * dv;
+        tvv.p_object = tx.tc.p_object + osv * dv;
+        const float hv = tx.eval(bump_tex, tvv).x;
+        const float h0 = tx.eval(bump_tex).x;
+        su = su + (hu - h0) * safe_rcp(du) * sn;
+        sv = sv + (hv - h0) * safe_rcp(dv) * sn;
+        sn = normalize(cross(su, sv));
+        shading_frame(bs, hit_back, sn, su);
+    } else {
+        shading_frame(bs, hit_back, sn, su);
+    }
     if (dot(-ray_d, bs.n) < 0.0f) {
         bs.n = reflect3(bs.n, dot(bs.n, ng) > 0.0f ? ng : -ng);
         bs.t = normalize(bs.t - bs.n * dot(bs.t, bs.n));
@@ -488,21 +778,24 @@ YM_HD void init_shading_space(Bsdf &bs, int hit_back, f3 ray_d, f3 ng, f3 sn_in,
     }
 }
 
-__device__ __forceinline__ float tex_float(const DevScene &sc, uint32_t tex, float def, f2 uv)
+template <class TX>
+__device__ __forceinline__ float tex_float(TX &tx, uint32_t tex, float def)
 {
-    return tex != kEmpty ? texture_eval(sc, tex, uv).x : def;
+    return tex != kEmpty ? tx.eval(tex).x : def;
 }
-__device__ __forceinline__ f3 tex_color(const DevScene &sc, uint32_t tex, const float *def, f2 uv)
+template <class TX>
+__device__ __forceinline__ f3 tex_color(TX &tx, uint32_t tex, const float *def)
 {
-    return tex != kEmpty ? texture_eval(sc, tex, uv) : f3{def[0], def[1], def[2]};
+    return tex != kEmpty ? tx.eval(tex) : f3{def[0], def[1], def[2]};
 }
 
 // DefaultMtlBSDF::InitBSDFSettings (src/BSDF.cuh:453-618)
-__device__ __forceinline__ void init_default_bsdf(const DevScene &sc, Bsdf &b, const DevMaterial &m, f2 uv,
-                                                  f3 ray_d, int hit_back, float ray_ior, float ex_ior)
+template <class TX>
+__device__ __forceinline__ void init_default_bsdf(TX &tx, Bsdf &b, const DevMaterial &m, f3 ray_d,
+                                                  int hit_back, float ray_ior, float ex_ior)
 {
     const float ior_min = 0.0001f;
-    const float coat_w = tex_float(sc, m.coat_weight_tex, m.coat_weight, uv);
+    const float coat_w = tex_float(tx, m.coat_weight_tex, m.coat_weight);
     const float ior_in = gmax(ray_ior, ior_min);
     const float ior_coat = gmax(m.coat_ior, ior_min);
     const float ior_out = gmax(hit_back ? ex_ior : m.ior_n, ior_min);
@@ -511,17 +804,17 @@ __device__ __forceinline__ void init_default_bsdf(const DevScene &sc, Bsdf &b, c
     const float cos_in = gmax(dot(b.n, -ray_d), 0.0f);
     const float fr_ic = coat_w > 0.0f ? fresnel_dd(eta_ic, cos_in) : 1.0f;
     const float fr_io = fresnel_dd(eta_io, cos_in);
-    const float metal = tex_float(sc, m.metalness_tex, m.metalness, uv);
-    const float ax = sqr(gclamp(tex_float(sc, m.roughness_x_tex, m.roughness_x, uv), 0.001f, 0.999f));
-    const float ay = sqr(gclamp(tex_float(sc, m.roughness_y_tex, m.roughness_y, uv), 0.001f, 0.999f));
-    const float spec_w = tex_float(sc, m.specular_weight_tex, m.specular_weight, uv);
-    const float trans_w = tex_float(sc, m.transmission_weight_tex, m.transmission_weight, uv);
-    const f3 dif = tex_color(sc, m.diffuse_albedo_tex, m.diffuse_albedo, uv);
-    const f3 spe = tex_color(sc, m.specular_albedo_tex, m.specular_albedo, uv);
-    const float coat_th = tex_float(sc, m.coat_thickness_tex, m.coat_thickness, uv);
-    const float cax = sqr(gclamp(tex_float(sc, m.coat_roughness_x_tex, m.coat_roughness_x, uv), 0.001f, 0.999f));
-    const float cay = sqr(gclamp(tex_float(sc, m.coat_roughness_y_tex, m.coat_roughness_y, uv), 0.001f, 0.999f));
-    const f3 coat_alb = tex_color(sc, m.coat_albedo_tex, m.coat_albedo, uv);
+    const float metal = tex_float(tx, m.metalness_tex, m.metalness);
+    const float ax = sqr(gclamp(tex_float(tx, m.roughness_x_tex, m.roughness_x), 0.001f, 0.999f));
+    const float ay = sqr(gclamp(tex_float(tx, m.roughness_y_tex, m.roughness_y), 0.001f, 0.999f));
+    const float spec_w = tex_float(tx, m.specular_weight_tex, m.specular_weight);
+    const float trans_w = tex_float(tx, m.transmission_weight_tex, m.transmission_weight);
+    const f3 dif = tex_color(tx, m.diffuse_albedo_tex, m.diffuse_albedo);
+    const f3 spe = tex_color(tx, m.specular_albedo_tex, m.specular_albedo);
+    const float coat_th = tex_float(tx, m.coat_thickness_tex, m.coat_thickness);
+    const float cax = sqr(gclamp(tex_float(tx, m.coat_roughness_x_tex, m.coat_roughness_x), 0.001f, 0.999f));
+    const float cay = sqr(gclamp(tex_float(tx, m.coat_roughness_y_tex, m.coat_roughness_y), 0.001f, 0.999f));
+    const f3 coat_alb = tex_color(tx, m.coat_albedo_tex, m.coat_albedo);
     const f3 ca = max3(coat_alb, s3(1E-8f));
     const f3 sigma = (-f3{logf(ca.x), logf(ca.y), logf(ca.z)}) * coat_th;
     const float rc = safe_rcp(cos_in);

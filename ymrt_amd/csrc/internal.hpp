@@ -62,6 +62,10 @@ int launch_halton_enumerate(uint32_t w, uint32_t h, const uint32_t *px, const ui
                             uint32_t n, uint64_t *out, hipStream_t stream);
 int upload_sampler_tables(ymrt_scene *sc);
 
+// kernels_pt.hip
+int launch_texture_eval(const DevScene &sc, uint32_t tex, const float *coords, uint32_t n, float *out,
+                        hipStream_t stream);
+
 // render.cpp
 void destroy_render_buffers(ymrt_scene *sc);
 

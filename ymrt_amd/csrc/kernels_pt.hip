@@ -159,12 +159,17 @@ __device__ __forceinline__ uint32_t info_vertex(uint32_t i) { return i >> 16; }
 constexpr uint32_t kHitBack = 1u;
 constexpr uint32_t kHitVolume = 2u;
 
-// shadow flags: bits 0-1 active rays, 2-3 transmit per ray, 4 volume vertex,
-// 5 path ends after this shadow stage, 6-7 ray reached the light past solid
-// geometry, 8-15 vertex depth
-constexpr uint32_t kShVolume = 1u << 4;
-constexpr uint32_t kShTerminate = 1u << 5;
-constexpr uint32_t kShClear0 = 1u << 6;   // bits 6-7: shadow ray k is not occluded by solid geometry
+// shadow flags: bits 0-3 shadow ray k recorded (slots 0-1 shape light,
+// 2-3 distant light), 4-7 ray k crosses the surface, 8-11 ray k reached the
+// light past solid geometry, 12 volume vertex, 13 path ends after this
+// shadow stage, 16-23 vertex depth
+constexpr uint32_t kShActiveMask = 0xFu;
+constexpr uint32_t kShTransmit0 = 1u << 4;
+constexpr uint32_t kShClear0 = 1u << 8;
+constexpr uint32_t kShClearMask = 0xFu << 8;
+constexpr uint32_t kShVolume = 1u << 12;
+constexpr uint32_t kShTerminate = 1u << 13;
+constexpr uint32_t kShDepthShift = 16;
 
 __device__ __forceinline__ f3 light_clamp(const ymrt_render_settings &rs, f3 c, int depth)
 {
@@ -401,32 +406,40 @@ __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PtParams prm, Pa
 }
 
 // ------------------------------------------------------------ shade
-struct ShadowOut {
+// Records shadow ray k of path p (origin, direction, maximum distance, BSDF or
+// phase value, clamped Li, MIS weight) straight to its slot in HBM.
+struct ShadowEmit {
+    const PathState &ps;
+    uint32_t cap, p;
     uint32_t flags;
-    f3 o[2], d[2];
-    float tmax[2];
-    f3 f[2], li[2];
-    float mis[2];
+    __device__ __forceinline__ void emit(int k, f3 orig, f3 dir, float tmax, bool transmit, f3 f, f3 li, float mis)
+    {
+        const size_t o = (size_t)k * cap + p;
+        ps.sh_o[o] = f4(orig, tmax);
+        ps.sh_d[o] = f4(dir, 0.0f);
+        ps.sh_f[o] = f4(f, mis);
+        ps.sh_li[o] = f4(li, 0.0f);
+        flags |= (1u << k) | (transmit ? (kShTransmit0 << k) : 0u);
+    }
 };
 
-__device__ __forceinline__ void emit_shadow(f3 orig, f3 pos, f3 ldir, f3 lpos, f3 lerr, f3 lng, int k, bool transmit,
-                                            f3 f, f3 li, float mis, ShadowOut &so)
+// Distance to a shape-light sample, shortened by SHADOW_RAY_CLAMP
+// (src/Shading.cu:337).
+__device__ __forceinline__ float shadow_tmax(f3 pos, f3 ldir, f3 lpos, f3 lerr, f3 lng)
 {
-    so.o[k] = orig;
-    so.d[k] = ldir;
-    so.tmax[k] = length(offset_ray_origin(lpos, lerr, -ldir, lng) - pos) * kShadowRayClamp;
-    so.flags |= (1u << k) | (transmit ? (4u << k) : 0u);
-    so.f[k] = f;
-    so.li[k] = li;
-    so.mis[k] = mis;
+    return length(offset_ray_origin(lpos, lerr, -ldir, lng) - pos) * kShadowRayClamp;
 }
 
 // kVolume: volume scattering vertices (Ray_shading_path_tracing's volume branch,
-// src/Shading.cu:2070-2130, with EvalVolumeShapeLight 419-513); otherwise
-// surface vertices (2132-2302 with EvalShapeLight 294-417).
-template <bool kVolume>
+// src/Shading.cu:2070-2130, with EvalVolumeDistantLight 202-290 and
+// EvalVolumeShapeLight 419-513); otherwise surface vertices (2132-2302 with
+// EvalDistantLight 83-198 and EvalShapeLight 294-417).
+// kGraph: the scene has procedural texture nodes or normal / bump maps (the
+// variant without them keeps the image-texture fast path only).
+template <bool kVolume, bool kGraph>
 __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
 {
+    __shared__ float tex_stk[(kVolume || !kGraph) ? 1 : kTexStack * 3 * kBlock];
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     uint32_t *cnt_next = prm.counters + (pass + 1) * kCtrStride;
     const uint32_t n = cnt[kVolume ? kCtrHitVol : kCtrHit];
@@ -449,8 +462,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
             f3 beta = xyz(ps.beta[p]);
             f3 L = xyz(ps.L[p]);
             Sampler smp = load_sampler(ps, p);
-            ShadowOut so;
-            so.flags = (uint32_t)depth << 8;
+            ShadowEmit so{ps, cap, p, (uint32_t)depth << kShDepthShift};
             bool alive = false;       // a next extension ray
             const f3 sh_beta = beta;
             float sel_pdf = 0.0f;
@@ -469,6 +481,34 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                 }
                 const f3 vpos = xyz(hb);
                 so.flags |= kShVolume;
+                if (rs.enable_distant_light && sc.distant_count > 0 && nv > 0) {
+                    const DevDistant &dl = sc.distant[pick_distant(sc.distant_count, sampler_next(sc, smp))];
+                    {
+                        f3 ldir = s3(0.0f), Li = s3(0.0f);
+                        float lpdf = 0.0f;
+                        const float u0 = sampler_next(sc, smp);
+                        const float u1 = sampler_next(sc, smp);
+                        distant_sample_li(dl, u0, u1, ldir, Li, lpdf);
+                        float wpdf = 0.0f;
+                        const float ph = mixed_phase_eval(vw, gs, nv, -ray_d, ldir, wpdf);
+                        if (ph > kMinColorEpsilon)
+                            so.emit(2, vpos, ldir, kTMax, false, s3(gmin(ph, kMaxColorClamp)),
+                                    min3(Li, s3(kMaxColorClamp)), power_heuristic(lpdf, wpdf));
+                    }
+                    {
+                        f3 pwi = s3(0.0f);
+                        float ppdf = 0.0f, pw = 0.0f;
+                        const float u0 = sampler_next(sc, smp);
+                        const float u1 = sampler_next(sc, smp);
+                        if (mixed_phase_sample(vw, gs, nv, u0, u1, -ray_d, pw, pwi, ppdf)) {
+                            float lpdf = 0.0f;
+                            const f3 Li = distant_eval_li(dl, pwi, lpdf);
+                            if (maxcomp(Li) > kMinColorEpsilon)
+                                so.emit(3, vpos, pwi, kTMax, false, s3(gmin(pw, kMaxColorClamp)),
+                                        min3(Li, s3(kMaxColorClamp)), power_heuristic(ppdf, lpdf));
+                        }
+                    }
+                }
                 if (rs.enable_shape_light && sc.light_count > 0 && nv > 0) {
                     const int li = sample_light_power(sc, sampler_next(sc, smp), sel_pdf);
                     const DevLight light = sc.lights[li];
@@ -481,8 +521,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                         float wpdf = 0.0f;
                         const float ph = mixed_phase_eval(vw, gs, nv, -ray_d, ldir, wpdf);
                         if (ph > kMinColorEpsilon && valid)
-                            emit_shadow(vpos, vpos, ldir, lpos, lerr, lng, 0, false, s3(gmin(ph, kMaxColorClamp)),
-                                        min3(Li, s3(kMaxColorClamp)), power_heuristic(lpdf, wpdf), so);
+                            so.emit(0, vpos, ldir, shadow_tmax(vpos, ldir, lpos, lerr, lng), false,
+                                    s3(gmin(ph, kMaxColorClamp)), min3(Li, s3(kMaxColorClamp)),
+                                    power_heuristic(lpdf, wpdf));
                     }
                     {
                         f3 pwi = s3(0.0f);
@@ -494,8 +535,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                             float lpdf = 0.0f;
                             const f3 Li = light_eval_li(sc, light, vpos, pwi, lpos, lerr, lng, lpdf);
                             if (maxcomp(Li) > kMinColorEpsilon)
-                                emit_shadow(vpos, vpos, pwi, lpos, lerr, lng, 1, false, s3(gmin(pw, kMaxColorClamp)),
-                                            min3(Li, s3(kMaxColorClamp)), power_heuristic(ppdf, lpdf), so);
+                                so.emit(1, vpos, pwi, shadow_tmax(vpos, pwi, lpos, lerr, lng), false,
+                                        s3(gmin(pw, kMaxColorClamp)), min3(Li, s3(kMaxColorClamp)),
+                                        power_heuristic(ppdf, lpdf));
                         }
                     }
                 }
@@ -609,17 +651,67 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                         dpdu = (dv02 * dp01 - dv01 * dp02) * idet;
                         dpdv = (-du02 * dp01 + du01 * dp02) * idet;
                     }
+                    const f3 pos_obj = pos;
                     pos = xf_position(in.o2w, pos, &perr);
                     sn = normalize(xf_transpose_vector(in.w2o, sn));
                     ng = normalize(xf_transpose_vector(in.w2o, ng));
                     dpdu = xf_vector(in.o2w, dpdu);
                     dpdv = xf_vector(in.o2w, dpdv);
+                    const bool is_default = mtype == YMRT_MATERIAL_DEFAULT;
+                    const uint32_t normal_tex = kGraph && is_default ? m.normal_mapping_tex : kEmpty;
+                    const uint32_t bump_tex = kGraph && is_default && normal_tex == kEmpty ? m.bump_mapping_tex : kEmpty;
+                    TexEval<kGraph> tx{sc, TexCoord{uv, pos, pos_obj}, tex_stk, (int)lane_id()};
+                    float4 duv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    if (bump_tex != kEmpty)
+                        duv = camera_uv_differentials(sc.camera, pos, sn, ng, dpdu, dpdv, rs.width, rs.height, 0.25f,
+                                                      0.25f);
                     Bsdf bs;
-                    init_shading_space(bs, back, ray_d, ng, sn, dpdu, dpdv);
+                    init_shading_space(bs, tx, normal_tex, bump_tex, back, ray_d, ng, sn, dpdu, dpdv, in.w2o, duv);
                     bs.mask = 0;
-                    if (mtype == YMRT_MATERIAL_DEFAULT)
-                        init_default_bsdf(sc, bs, m, uv, ray_d, back, t.ray_ior(), t.ex_ior(inst));
+                    if (is_default) init_default_bsdf(tx, bs, m, ray_d, back, t.ray_ior(), t.ex_ior(inst));
                     const f3 wo = bs.to_local(-ray_d);
+                    if (rs.enable_distant_light && sc.distant_count > 0) {
+                        const DevDistant &dl = sc.distant[pick_distant(sc.distant_count, sampler_next(sc, smp))];
+                        {
+                            f3 ldir = s3(0.0f), Li = s3(0.0f);
+                            float lpdf = 0.0f;
+                            const float u0 = sampler_next(sc, smp);
+                            const float u1 = sampler_next(sc, smp);
+                            distant_sample_li(dl, u0, u1, ldir, Li, lpdf);
+                            const f3 lwi = bs.to_local(ldir);
+                            float wpdf = 0.0f;
+                            const f3 f = bsdf_eval(bs, wo, lwi, wpdf);
+                            if (maxcomp(f) > kMinColorEpsilon) {
+                                const bool transmit = dot(-ray_d, ng) * dot(ldir, ng) < 0.0f;
+                                const bool invalid = (!same_hemi(wo, lwi) && !transmit) || (same_hemi(wo, lwi) && transmit);
+                                if (!invalid)
+                                    so.emit(2, offset_ray_origin(pos, perr, ldir, ng), ldir, kTMax, transmit,
+                                            min3(f, s3(kMaxColorClamp)), min3(Li, s3(kMaxColorClamp)),
+                                            power_heuristic(lpdf, wpdf));
+                            }
+                        }
+                        {
+                            f3 bw = s3(0.0f), bwi = s3(0.0f);
+                            float bpdf = 0.0f;
+                            const float u0 = sampler_next(sc, smp);
+                            const float u1 = sampler_next(sc, smp);
+                            const float u2 = sampler_next(sc, smp);
+                            if (bsdf_sample(bs, wo, bw, bwi, bpdf, u0, u1, u2)) {
+                                const f3 ldir = bs.to_world(bwi);
+                                float lpdf = 0.0f;
+                                const f3 Li = distant_eval_li(dl, ldir, lpdf);
+                                if (maxcomp(Li) > kMinColorEpsilon) {
+                                    const bool transmit = dot(-ray_d, ng) * dot(ldir, ng) < 0.0f;
+                                    const bool invalid =
+                                        (!same_hemi(wo, bwi) && !transmit) || (same_hemi(wo, bwi) && transmit);
+                                    if (!invalid)
+                                        so.emit(3, offset_ray_origin(pos, perr, ldir, ng), ldir, kTMax, transmit,
+                                                min3(bw, s3(kMaxColorClamp)), min3(Li, s3(kMaxColorClamp)),
+                                                power_heuristic(bpdf, lpdf));
+                                }
+                            }
+                        }
+                    }
                     if (rs.enable_shape_light && sc.light_count > 0) {
                         const int li = sample_light_power(sc, sampler_next(sc, smp), sel_pdf);
                         const DevLight light = sc.lights[li];
@@ -636,9 +728,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                                 const bool transmit = dot(-ray_d, ng) * dot(ldir, ng) < 0.0f;
                                 const bool invalid = (!same_hemi(wo, lwi) && !transmit) || (same_hemi(wo, lwi) && transmit);
                                 if (!invalid)
-                                    emit_shadow(offset_ray_origin(pos, perr, ldir, ng), pos, ldir, lpos, lerr, lng, 0,
-                                                transmit, min3(f, s3(kMaxColorClamp)), min3(Li, s3(kMaxColorClamp)),
-                                                power_heuristic(lpdf, wpdf), so);
+                                    so.emit(0, offset_ray_origin(pos, perr, ldir, ng), ldir,
+                                            shadow_tmax(pos, ldir, lpos, lerr, lng), transmit,
+                                            min3(f, s3(kMaxColorClamp)), min3(Li, s3(kMaxColorClamp)),
+                                            power_heuristic(lpdf, wpdf));
                             }
                         }
                         {
@@ -657,9 +750,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                                     const bool invalid =
                                         (!same_hemi(wo, bwi) && !transmit) || (same_hemi(wo, bwi) && transmit);
                                     if (!invalid)
-                                        emit_shadow(offset_ray_origin(pos, perr, ldir, ng), pos, ldir, lpos, lerr, lng,
-                                                    1, transmit, min3(bw, s3(kMaxColorClamp)),
-                                                    min3(Li, s3(kMaxColorClamp)), power_heuristic(bpdf, lpdf), so);
+                                        so.emit(1, offset_ray_origin(pos, perr, ldir, ng), ldir,
+                                                shadow_tmax(pos, ldir, lpos, lerr, lng), transmit,
+                                                min3(bw, s3(kMaxColorClamp)), min3(Li, s3(kMaxColorClamp)),
+                                                power_heuristic(bpdf, lpdf));
                                 }
                             }
                         }
@@ -696,7 +790,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                 }
             }
             // ---------------------------------------------- write back
-            const bool nee = (so.flags & 3u) != 0;
+            const bool nee = (so.flags & kShActiveMask) != 0;
             store_sampler(ps, p, smp);
             ps.L[p] = f4(L, 0.0f);
             if (nee) {
@@ -704,14 +798,6 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                 if (!alive) so.flags |= kShTerminate;
                 ps.sh_flags[p] = so.flags;
                 ps.sh_c[p] = f4(sh_beta, sel_pdf);
-#pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    if (!(so.flags & (1u << k))) continue;
-                    ps.sh_o[(size_t)k * cap + p] = f4(so.o[k], so.tmax[k]);
-                    ps.sh_d[(size_t)k * cap + p] = f4(so.d[k], 0.0f);
-                    ps.sh_f[(size_t)k * cap + p] = f4(so.f[k], so.mis[k]);
-                    ps.sh_li[(size_t)k * cap + p] = f4(so.li[k], 0.0f);
-                }
             } else if (!alive) {
                 ps.out[p] = f4(L, 0.0f);
             }
@@ -730,30 +816,37 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
 }
 
 // ------------------------------------------------------------ shadow
-// NEE contribution of a path's shadow rays (EvalShapeLight's tail,
-// src/Shading.cu:337-416): sum f * Li * mis * transmittance over the rays that
-// reached the light, divide by the light-selection pdf, clamp, deposit.
+// NEE contribution of a path's shadow rays: per light kind, the sum of
+// mis * tr * f * Li over the rays that reached the light, divided by the
+// light-selection pdf, clamped and deposited, distant light first
+// (EvalDistantLight, src/Shading.cu:83-198, then EvalShapeLight 294-417, as
+// the shading stage adds them at 2217-2240).  with_tr: the transmittance of
+// each ray is in sh_tr (paths in media), otherwise 1.
 __device__ __forceinline__ void shadow_finish(const PtParams &prm, const PathState &ps, uint32_t p, uint32_t fl,
-                                              f3 tr0, f3 tr1)
+                                              uint32_t distant_count, bool with_tr)
 {
     const uint32_t cap = prm.cap;
+    f3 shape = s3(0.0f), dist = s3(0.0f);
+    static_for<0, kShadowSlots>([&](auto K) {
+        if ((fl & (1u << K())) && (fl & (kShClear0 << K()))) {
+            const size_t o = (size_t)K() * cap + p;
+            const float4 f4v = ps.sh_f[o];
+            const float4 li4 = ps.sh_li[o];
+            const f3 tr = with_tr ? xyz(ps.sh_tr[o]) : s3(1.0f);
+            const f3 c = ((f4v.w * tr) * xyz(f4v)) * xyz(li4);
+            if (K() < 2) shape = shape + c;
+            else dist = dist + c;
+        }
+    });
     const float4 c4 = ps.sh_c[p];
-    f3 direct = s3(0.0f);
-    if ((fl & 1u) && (fl & kShClear0)) {
-        const float4 f4v = ps.sh_f[p];
-        const float4 li4 = ps.sh_li[p];
-        direct = direct + ((f4v.w * tr0) * xyz(f4v)) * xyz(li4);
-    }
-    if ((fl & 2u) && (fl & (kShClear0 << 1))) {
-        const float4 f4v = ps.sh_f[(size_t)cap + p];
-        const float4 li4 = ps.sh_li[(size_t)cap + p];
-        direct = direct + ((f4v.w * tr1) * xyz(f4v)) * xyz(li4);
-    }
+    const f3 beta = xyz(c4);
+    const int depth = (int)((fl >> kShDepthShift) & 0xFFu);
+    f3 L = xyz(ps.L[p]);
+    if (distant_count > 0) L = L + light_clamp(prm.rs, beta * (dist / (1.0f / (float)distant_count)), depth);
     // EvalShapeLight returns direct * SafeRcp(select pdf); the volume variant
     // divides when the pdf is positive.
-    const f3 sel = (fl & kShVolume) ? (c4.w > 0.0f ? direct / c4.w : s3(0.0f)) : direct * safe_rcp(c4.w);
-    f3 L = xyz(ps.L[p]);
-    L = L + light_clamp(prm.rs, xyz(c4) * sel, (int)((fl >> 8) & 0xFFu));
+    const f3 sel = (fl & kShVolume) ? (c4.w > 0.0f ? shape / c4.w : s3(0.0f)) : shape * safe_rcp(c4.w);
+    L = L + light_clamp(prm.rs, beta * sel, depth);
     ps.L[p] = f4(L, 0.0f);
     if (fl & kShTerminate) ps.out[p] = f4(L, 0.0f);
 }
@@ -779,8 +872,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         {
             p = ps.q_shadow[i];
             uint32_t fl = ps.sh_flags[p];
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
+#pragma unroll 1
+            for (int k = 0; k < kShadowSlots; ++k) {
                 if (!(fl & (1u << k))) continue;
                 const float4 o4 = ps.sh_o[(size_t)k * cap + p];
                 const float4 d4 = ps.sh_d[(size_t)k * cap + p];
@@ -798,10 +891,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 if (r < 0) atomicOr(prm.error, 1u);
                 if (r == 0) fl |= kShClear0 << k;
             }
-            const bool walk = kMedia && (fl & (kShClear0 | (kShClear0 << 1)));
+            const bool walk = kMedia && (fl & kShClearMask);
             out[0] = walk;
             if (walk) ps.sh_flags[p] = fl;
-            else shadow_finish(prm, ps, p, fl, s3(1.0f), s3(1.0f));
+            else shadow_finish(prm, ps, p, fl, sc.distant_count, false);
         }
     });
     wave_add64(prm.stats + kStatShadow, nsh);
@@ -850,15 +943,16 @@ __global__ __launch_bounds__(kBlock) void k_transmit(DevScene sc, PtParams prm, 
                 rng.key[3] = (uint32_t)prm.frame;
                 rng.ctr = 0;
             }
-            f3 trk[2] = {s3(1.0f), s3(1.0f)};
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
+            // distant-light rays first, as the shading stage evaluates them
+#pragma unroll 1
+            for (int j = 0; j < kShadowSlots; ++j) {
+                const int k = (j + 2) & 3;
                 if (!(fl & (1u << k)) || !(fl & (kShClear0 << k))) continue;
                 const float4 o4 = ps.sh_o[(size_t)k * cap + p];
                 const float4 d4 = ps.sh_d[(size_t)k * cap + p];
                 f3 tr = s3(1.0f);
                 Transfer t = t0;
-                if (fl & (4u << k)) boundary_bunch(sc, t, nb0);
+                if (fl & (kShTransmit0 << k)) boundary_bunch(sc, t, nb0);
                 const bool volv = (fl & kShVolume) != 0;
                 Query tq;
                 tq.o = xyz(o4);
@@ -904,9 +998,9 @@ __global__ __launch_bounds__(kBlock) void k_transmit(DevScene sc, PtParams prm, 
                     tq.last_tri = bh.tri;
                     if (++guard > 64) break;
                 }
-                trk[k] = tr;
+                ps.sh_tr[(size_t)k * cap + p] = f4(tr, 0.0f);
             }
-            shadow_finish(prm, ps, p, fl, trk[0], trk[1]);
+            shadow_finish(prm, ps, p, fl, sc.distant_count, true);
         }
     });
     wave_add64(prm.stats + kStatTransmittance, ntr);
@@ -1050,9 +1144,14 @@ int launch_shade(const DevScene &sc, const PtParams &prm, const PathState &ps, u
                  hipStream_t s)
 {
     if (volume)
-        hipLaunchKernelGGL(k_shade<true>, persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps, pass);
+        hipLaunchKernelGGL((k_shade<true, false>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps,
+                           pass);
+    else if (sc.has_graph)
+        hipLaunchKernelGGL((k_shade<false, true>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps,
+                           pass);
     else
-        hipLaunchKernelGGL(k_shade<false>, persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps, pass);
+        hipLaunchKernelGGL((k_shade<false, false>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps,
+                           pass);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }
@@ -1088,6 +1187,32 @@ int launch_resolve(const PtParams &prm, const PathState &ps, float *noise, hipSt
 {
     const uint32_t pix = prm.chunk_paths / (uint32_t)prm.rs.spp;
     hipLaunchKernelGGL(k_resolve, dim3((pix + 255) / 256), dim3(256), 0, s, prm, ps, reinterpret_cast<float4 *>(noise));
+    YM_HIP(hipGetLastError());
+    return YMRT_OK;
+}
+
+// Texture graph at n given coordinates: coords[i*8 + 0..7] = (s, t,
+// p_world.xyz, p_object.xyz), out[i*3 + 0..2] = the texture's rgb.
+__global__ __launch_bounds__(kBlock) void k_texture_eval(DevScene sc, uint32_t tex, const float *coords, uint32_t n,
+                                                         float *out)
+{
+    __shared__ float tex_stk[kTexStack * 3 * kBlock];
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float *c = coords + (size_t)i * 8;
+    TexEval<true> tx{sc, TexCoord{f2{c[0], c[1]}, f3{c[2], c[3], c[4]}, f3{c[5], c[6], c[7]}}, tex_stk,
+                     (int)threadIdx.x};
+    const f3 v = tx.eval(tex);
+    out[(size_t)i * 3 + 0] = v.x;
+    out[(size_t)i * 3 + 1] = v.y;
+    out[(size_t)i * 3 + 2] = v.z;
+}
+
+int launch_texture_eval(const DevScene &sc, uint32_t tex, const float *coords, uint32_t n, float *out, hipStream_t s)
+{
+    YM_REQUIRE(tex < sc.texture_count, "texture index out of range");
+    if (n == 0) return YMRT_OK;
+    hipLaunchKernelGGL(k_texture_eval, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, sc, tex, coords, n, out);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }

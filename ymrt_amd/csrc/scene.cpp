@@ -8,6 +8,7 @@
 // Camera::InitCameraRayTransfer (src/Camera.cpp:15-78), the editor->render
 // camera copy with the image aspect ratio, and the UPLOAD_TO_GPU calls.
 #include <chrono>
+#include <functional>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -375,11 +376,34 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
     // ---------------------------------------------------------- textures / grids
     std::vector<DevTexture> texs;
     std::vector<uint32_t> texels;
+    YM_REQUIRE(d->texture_count == 0 || d->textures != nullptr, "textures is NULL");
     for (uint32_t i = 0; i < d->texture_count; ++i) {
         const ymrt_texture_desc &t = d->textures[i];
+        YM_REQUIRE(t.type <= YMRT_TEXTURE_WAVE, "unknown texture type");
+        DevTexture dt{};
+        dt.type = t.type;
+        for (int k = 0; k < 3; ++k) dt.value[k] = t.value[k];
+        dt.frequency = t.frequency;
+        dt.lacunarity = t.lacunarity;
+        dt.gain = t.gain;
+        dt.amplitude = t.amplitude;
+        dt.offset_f = t.offset;
+        dt.variation = t.variation;
+        dt.radius = t.radius;
+        dt.layers = t.layer_count;
+        dt.normalized = t.normalized;
+        dt.axes = t.turbulence_axes;
+        dt.frequency1 = t.frequency1;
+        dt.lacunarity1 = t.lacunarity1;
+        dt.gain1 = t.gain1;
+        dt.layers1 = t.layer_count1;
+        if (t.type != YMRT_TEXTURE_IMAGE) {
+            dt.offset = texels.size();
+            texs.push_back(dt);
+            continue;
+        }
         YM_REQUIRE(t.width > 0 && t.height > 0 && t.texels, "empty texture");
         YM_REQUIRE(t.channels >= 1 && t.channels <= 4, "texture channels must be 1..4");
-        DevTexture dt;
         dt.width = t.width;
         dt.height = t.height;
         dt.channels = t.channels;
@@ -390,7 +414,7 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
         dt.ou = t.offset_u;
         dt.ov = t.offset_v;
         dt.offset = texels.size();
-        size_t n = (size_t)t.width * t.height;
+        const size_t n = (size_t)t.width * t.height;
         for (size_t k = 0; k < n; ++k) {
             uint32_t w = 0;
             for (uint32_t c = 0; c < t.channels; ++c) w |= (uint32_t)t.texels[k * t.channels + c] << (8 * c);
@@ -398,14 +422,57 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
         }
         texs.push_back(dt);
     }
+    // Texture programs: each graph flattened in post order (inputs black then
+    // white before their node) for a value stack of at most kTexStack entries,
+    // the bound of TextureEval's explicit stack (src/TextureEval.cuh:415-481).
+    std::vector<uint32_t> tex_prog;
+    for (uint32_t i = 0; i < d->texture_count; ++i) {
+        const uint32_t start = (uint32_t)tex_prog.size();
+        int depth = 0, max_depth = 0;
+        bool ok = true;
+        std::function<void(uint32_t, int)> emit = [&](uint32_t ti, int level) {
+            if (!ok) return;
+            if (level > kTexStack) {
+                ok = false;
+                return;
+            }
+            if (ti == kEmpty) {
+                tex_prog.push_back(kTexBlack);
+                max_depth = std::max(max_depth, ++depth);
+                return;
+            }
+            const ymrt_texture_desc &t = d->textures[ti];
+            if (t.type >= YMRT_TEXTURE_CHECKERBOARD) {
+                for (int c = 0; c < 2; ++c) {
+                    if (t.child[c] != kEmpty && t.child[c] >= d->texture_count) {
+                        ok = false;
+                        return;
+                    }
+                    emit(t.child[c], level + 1);
+                }
+                depth -= 2;
+            }
+            tex_prog.push_back(ti);
+            max_depth = std::max(max_depth, ++depth);
+        };
+        emit(i, 1);
+        YM_REQUIRE(ok, "texture graph has an input out of range, a cycle or more than 8 levels");
+        YM_REQUIRE(max_depth <= kTexStack, "texture graph needs more than 8 stack entries");
+        texs[i].prog_start = start;
+        texs[i].prog_len = (uint32_t)tex_prog.size() - start;
+    }
     for (uint32_t i = 0; i < d->material_count; ++i) {
         const DevMaterial &m = h.materials[i];
         const uint32_t ids[] = {m.diffuse_albedo_tex, m.specular_albedo_tex, m.roughness_x_tex, m.roughness_y_tex,
                                 m.specular_weight_tex, m.metalness_tex, m.transmission_weight_tex,
                                 m.coat_albedo_tex, m.coat_weight_tex, m.coat_thickness_tex,
-                                m.coat_roughness_x_tex, m.coat_roughness_y_tex};
+                                m.coat_roughness_x_tex, m.coat_roughness_y_tex, m.normal_mapping_tex,
+                                m.bump_mapping_tex};
         for (uint32_t t : ids) YM_REQUIRE(t == kEmpty || t < d->texture_count, "material texture index out of range");
+        if (m.normal_mapping_tex != kEmpty || m.bump_mapping_tex != kEmpty) ds.has_graph = 1;
     }
+    for (const DevTexture &t : texs)
+        if (t.type >= YMRT_TEXTURE_CHECKERBOARD) ds.has_graph = 1;
     std::vector<float> lut(512);
     for (int i = 0; i < 256; ++i) {
         float v = (float)i * (1.0f / 255.0f);
@@ -539,6 +606,7 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
     if ((rc = upload(sc, h.shade, &ds.shade))) return rc;
     if ((rc = upload(sc, h.materials, &ds.materials))) return rc;
     if ((rc = upload(sc, texs, &ds.textures))) return rc;
+    if ((rc = upload(sc, tex_prog, &ds.tex_prog))) return rc;
     if ((rc = upload(sc, texels, &ds.texels))) return rc;
     if ((rc = upload(sc, lut, &ds.tex_lut))) return rc;
     if ((rc = upload(sc, grids, &ds.grids))) return rc;

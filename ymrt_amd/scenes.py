@@ -120,6 +120,114 @@ def cornell_box(width=256, height=256, halton_seed=0, analytic_sphere=False, sph
     return sa
 
 
+def normal_map_texels(n=64, waves=4, slope=0.35):
+    """Tangent-space normal map of n x n texels (linear RGB8): a bump field of
+    `waves` sine periods per side, n = normalize(-slope cos, -slope cos, 1)."""
+    u = (np.arange(n) + 0.5) / n
+    gx = -slope * np.cos(2 * np.pi * waves * u)[None, :] * np.ones((n, 1))
+    gy = -slope * np.cos(2 * np.pi * waves * u)[:, None] * np.ones((1, n))
+    v = np.stack([gx, gy, np.ones((n, n))], -1)
+    v /= np.linalg.norm(v, axis=-1, keepdims=True)
+    return np.round((v * 0.5 + 0.5) * 255.0).astype(np.uint8)
+
+
+def texture_node_set():
+    """A Cornell box plus one texture graph of every node type, for evaluating
+    textures directly; returns (arrays, {name: texture index})."""
+    sa = cornell_box(8, 8)
+    T = abi
+    c = lambda *rgb: sa.add_texture_node(T.TEXTURE_CONSTANT_RGB, value=rgb)
+    n = {}
+    n["black"], n["white"] = c(0.0, 0.0, 0.0), c(1.0, 1.0, 1.0)
+    bw = (n["black"], n["white"])
+    n["float"] = sa.add_texture_node(T.TEXTURE_CONSTANT_FLOAT, value=(0.375, 0.0, 0.0))
+    n["rgb"] = c(0.25, 0.5, 0.75)
+    n["checker"] = sa.add_texture_node(T.TEXTURE_CHECKERBOARD, (n["rgb"], n["float"]), frequency=np.pi)
+    n["noise"] = sa.add_texture_node(T.TEXTURE_NOISE, bw, frequency=3.0)
+    n["noise_n"] = sa.add_texture_node(T.TEXTURE_NOISE, bw, frequency=3.0, normalized=1)
+    fb = dict(frequency=2.5, lacunarity=2.1, gain=0.45, layer_count=5, amplitude=0.9, offset=0.05)
+    n["fbm"] = sa.add_texture_node(T.TEXTURE_FBM, bw, **fb)
+    n["turb"] = sa.add_texture_node(T.TEXTURE_TURBULENCE, bw, **fb)
+    n["marble"] = sa.add_texture_node(T.TEXTURE_MARBLE, bw, frequency=2.0, lacunarity=2.0, gain=0.5,
+                                      layer_count=4, variation=5.0, turbulence_axes=5)
+    n["wood"] = sa.add_texture_node(T.TEXTURE_WOOD, bw, frequency=1.5, variation=7.0)
+    n["polka"] = sa.add_texture_node(T.TEXTURE_POLKA_DOT, bw, frequency=5.0, radius=0.3)
+    n["wave"] = sa.add_texture_node(T.TEXTURE_WAVE, bw, frequency=4.0, lacunarity=2.0, gain=0.5, layer_count=3,
+                                    frequency1=1.5, lacunarity1=1.8, gain1=0.7, layer_count1=2)
+    # nested: a checkerboard whose black input is empty and whose white input
+    # is an fBm node over a noise node and a constant
+    n["inner"] = sa.add_texture_node(T.TEXTURE_FBM, (n["noise_n"], n["rgb"]), **fb)
+    n["nested"] = sa.add_texture_node(T.TEXTURE_CHECKERBOARD, (None, n["inner"]), frequency=2.0)
+    tex = np.zeros((4, 4, 3), np.uint8)
+    tex[..., 0] = np.arange(16).reshape(4, 4) * 16
+    n["image"] = sa.add_texture(tex, color_space=T.COLOR_LINEAR, scale=(2.0, 3.0), offset=(0.1, -0.2))
+    n["image_in_graph"] = sa.add_texture_node(T.TEXTURE_MARBLE, (n["image"], n["fbm"]), frequency=1.0,
+                                              lacunarity=2.0, gain=0.5, layer_count=2, variation=2.0,
+                                              turbulence_axes=2)
+    return sa, n
+
+
+def textured_box(width=64, height=64, halton_seed=0):
+    """Cornell box whose materials are driven by texture graphs: a checkerboard
+    floor whose white input is an fBm node, a marble wall, a wood-textured
+    roughness, polka dots on the ceiling (surface uv), turbulence with a clear
+    coat, a wave-textured analytic sphere, a normal-mapped box (8-bit linear
+    image) and a bump-mapped back wall (height from a noise node)."""
+    T = abi
+    sa = abi.SceneArrays()
+    sa.halton_seed = halton_seed
+    c = lambda *rgb: sa.add_texture_node(T.TEXTURE_CONSTANT_RGB, value=rgb)
+    f = lambda v: sa.add_texture_node(T.TEXTURE_CONSTANT_FLOAT, value=(v, 0.0, 0.0))
+    fbm = sa.add_texture_node(T.TEXTURE_FBM, (c(0.5, 0.45, 0.3), c(0.9, 0.85, 0.7)), frequency=9.0,
+                              lacunarity=2.0, gain=0.5, layer_count=4, amplitude=0.8, offset=0.1)
+    checker = sa.add_texture_node(T.TEXTURE_CHECKERBOARD, (c(0.15, 0.15, 0.2), fbm), frequency=18.0)
+    marble = sa.add_texture_node(T.TEXTURE_MARBLE, (c(0.6, 0.1, 0.1), c(0.9, 0.85, 0.8)), frequency=3.0,
+                                 lacunarity=2.0, gain=0.5, layer_count=4, variation=6.0, turbulence_axes=3)
+    wood = sa.add_texture_node(T.TEXTURE_WOOD, (f(0.1), f(0.7)), frequency=2.5, variation=6.0)
+    polka = sa.add_texture_node(T.TEXTURE_POLKA_DOT, (c(0.7, 0.7, 0.7), c(0.2, 0.3, 0.8)), frequency=6.0,
+                                radius=0.3)
+    turb = sa.add_texture_node(T.TEXTURE_TURBULENCE, (c(0.1, 0.1, 0.5), c(0.9, 0.8, 0.2)), frequency=7.0,
+                               lacunarity=2.1, gain=0.55, layer_count=3, amplitude=1.0, offset=0.0)
+    wave = sa.add_texture_node(T.TEXTURE_WAVE, (c(0.2, 0.6, 0.3), c(0.9, 0.9, 0.9)), frequency=12.0,
+                               lacunarity=2.0, gain=0.5, layer_count=3, frequency1=5.0, lacunarity1=1.9,
+                               gain1=0.6, layer_count1=2)
+    bump = sa.add_texture_node(T.TEXTURE_NOISE, (None, f(0.02)), frequency=25.0, normalized=1)
+    nmap = sa.add_texture(normal_map_texels(), color_space=T.COLOR_LINEAR)
+    floor = sa.add_material(abi.default_material(diffuse_albedo_tex=checker, specular_weight=0.0))
+    ceil = sa.add_material(abi.default_material(diffuse_albedo_tex=polka, specular_weight=0.0))
+    back = sa.add_material(abi.default_material(diffuse_albedo=(0.7, 0.7, 0.7), specular_weight=0.0,
+                                                bump_mapping_tex=bump))
+    left = sa.add_material(abi.default_material(diffuse_albedo_tex=marble, roughness_x=0.4, roughness_y=0.4))
+    right = sa.add_material(abi.default_material(diffuse_albedo=(0.1, 0.6, 0.15), roughness_x_tex=wood,
+                                                 roughness_y_tex=wood))
+    coated = sa.add_material(abi.default_material(diffuse_albedo_tex=turb, specular_weight=0.0, coat_weight=1.0,
+                                                  coat_roughness_x=0.1, coat_roughness_y=0.1))
+    metal = sa.add_material(abi.default_material(diffuse_albedo=(0.9, 0.9, 0.9), specular_albedo=(1.0, 1.0, 1.0),
+                                                 roughness_x=0.25, roughness_y=0.25, metalness=1.0,
+                                                 normal_mapping_tex=nmap))
+    ball = sa.add_material(abi.default_material(diffuse_albedo_tex=wave, specular_weight=0.5))
+    light = sa.add_material(abi.light_material((1.0, 1.0, 1.0), 15.0))
+    # the floor's object space is lifted by 0.05 so that p_object.y is not a
+    # zero of the checkerboard's sin(y * frequency)
+    sa.add_instance(_add(sa, quad((0, 0.05, 0), (0, 0.05, 1), (1, 0.05, 1), (1, 0.05, 0))), floor,
+                    _translate(0.0, -0.05, 0.0))
+    walls = [
+        (quad((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)), ceil),
+        (quad((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0)), back),
+        (quad((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1)), left),
+        (quad((1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0)), right),
+    ]
+    for geom, mat in walls:
+        sa.add_instance(_add(sa, geom), mat)
+    sa.add_instance(_add(sa, box((0.55, 0.0, 0.45), (0.85, 0.3, 0.75))), coated)
+    sa.add_instance(_add(sa, box((0.15, 0.0, 0.15), (0.45, 0.6, 0.45))), metal)
+    sa.add_instance(sa.add_sphere(0.15), ball, _translate(0.65, 0.45, 0.55))
+    lq = quad((0.35, 0.999, 0.35), (0.65, 0.999, 0.35), (0.65, 0.999, 0.65), (0.35, 0.999, 0.65))
+    sa.add_instance(_add(sa, lq), light)
+    sa.set_camera((0.5, 0.5, 2.2), (0.5, 0.5, 0.0), 40.0, width / height)
+    return sa
+
+
 def cornell_media(width=64, height=64, grid=True, halton_seed=0, sigma=2.0):
     """Cornell box with a participating medium in the box [0.2,0.8] x [0.05,0.65]
     x [0.2,0.8] (a boundary instance): an 8^3 voxel density grid in [0, 1]
@@ -145,6 +253,38 @@ def _translate(x, y, z, scale=1.0):
     m[:3, :3] *= scale
     m[:3, 3] = (x, y, z)
     return m
+
+
+def outdoor(width=64, height=48, fog=False, halton_seed=0):
+    """Ground plane with boxes and analytic spheres under a distant light (cone
+    half-angle 5 degrees, from above at an angle) and one small quad light;
+    fog adds a homogeneous medium box that shadow rays must cross."""
+    sa = abi.SceneArrays()
+    sa.halton_seed = halton_seed
+    ground = sa.add_material(abi.default_material(diffuse_albedo=(0.6, 0.6, 0.55), specular_weight=0.0))
+    red = sa.add_material(abi.default_material(diffuse_albedo=(0.7, 0.2, 0.15), roughness_x=0.3, roughness_y=0.3))
+    metal = sa.add_material(abi.default_material(diffuse_albedo=(0.9, 0.8, 0.5), specular_albedo=(1.0, 1.0, 1.0),
+                                                 roughness_x=0.2, roughness_y=0.2, metalness=1.0))
+    glass = sa.add_material(abi.default_material(diffuse_albedo=(1.0, 1.0, 1.0), roughness_x=0.05,
+                                                 roughness_y=0.05, transmission_weight=1.0, ior_n=1.5,
+                                                 ior_priority=1))
+    sa.add_instance(_add(sa, quad((-20, 0, -20), (-20, 0, 20), (20, 0, 20), (20, 0, -20))), ground)
+    sa.add_instance(_add(sa, box((-1.5, 0.0, -1.0), (-0.5, 1.2, 0.0))), red)
+    sa.add_instance(sa.add_sphere(0.6), metal, _translate(0.8, 0.6, -0.3))
+    sa.add_instance(sa.add_sphere(0.4), glass, _translate(0.0, 0.4, 1.0))
+    lq = quad((2.0, 2.5, -1.0), (2.5, 2.5, -1.0), (2.5, 2.5, -0.5), (2.0, 2.5, -0.5))
+    sa.add_instance(_add(sa, lq), sa.add_material(abi.light_material((1.0, 0.9, 0.8), 20.0)))
+    # light frame: -y of the frame points to the light (up and to the side)
+    ang = np.radians(35.0)
+    rot = np.array([[1, 0, 0, 0], [0, -np.cos(ang), np.sin(ang), 0], [0, -np.sin(ang), -np.cos(ang), 0],
+                    [0, 0, 0, 1]], np.float32)
+    sa.add_distant_light((1.0, 0.95, 0.9), 3.0, 5.0, rot)
+    if fog:
+        vol = sa.add_volume((0.3, 0.3, 0.3), (0.9, 0.9, 0.9), 0.2)
+        sa.add_instance(_add(sa, box((-2.0, 0.01, -1.5), (2.0, 1.8, 2.0))),
+                        sa.add_material(abi.default_material(ior_n=1.0)), inner_volume=vol, boundary=True)
+    sa.set_camera((0.0, 1.6, 5.0), (0.0, 0.5, 0.0), 45.0, width / height)
+    return sa
 
 
 # --------------------------------------------------------------- value noise
