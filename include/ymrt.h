@@ -308,6 +308,21 @@ int ymrt_render_frame(ymrt_scene *scene, const ymrt_render_settings *settings,
                       int frame_index, float *noise_rgba, void *stream,
                       int timing, ymrt_frame_stats *stats);
 
+/* ymrt_render_frame plus an instance-id image: instance_ids (width*height
+ * uint32, device, may be NULL) receives the index of the instance that
+ * sample 0's camera ray hits first, YMRT_INVALID_INDEX on a miss -- the
+ * picking buffer RayTraceEditorView fills (src/Shading.cu:1442-1547), where
+ * the id written last among the samples wins. */
+int ymrt_render_frame_aov(ymrt_scene *scene, const ymrt_render_settings *settings,
+                          int frame_index, float *noise_rgba, uint32_t *instance_ids,
+                          void *stream, int timing, ymrt_frame_stats *stats);
+
+/* 8-bit RGBA PNG of a host RGBA32F image as SavePngImage writes it
+ * (src/FileIO.cpp:167-194): the extension becomes ".png", parent folders
+ * are created, channels are clamp(v, 0, 1) * 255 + 0.5 truncated, row 0 of
+ * the image is the bottom row of the file. */
+int ymrt_save_png(const char *path, const float *rgba, uint32_t width, uint32_t height);
+
 /* Running mean over frames (Accumulate_image_path_tracing). */
 int ymrt_accumulate(const float *noise_rgba, float *accum_rgba, uint32_t width,
                     uint32_t height, int frame_index, int max_frames, void *stream);

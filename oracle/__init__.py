@@ -79,6 +79,8 @@ def lib():
             L.or_probe_uv_differentials.argtypes = [vp, vp, vp, vp, vp, vp, C.c_uint32, C.c_uint32,
                                                     C.c_float, C.c_float, vp]
             L.or_glm_transform.argtypes = [vp, vp, vp, vp, vp, vp]
+            L.or_first_hits.argtypes = [vp, C.POINTER(abi.RenderSettings), C.c_int, vp]
+            L.or_primary_rays.argtypes = [vp, C.POINTER(abi.RenderSettings), C.c_int, C.c_uint32, vp]
             L.or_perlin.restype = C.c_float
             L.or_perlin.argtypes = [V3]
             L.or_accumulate.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_int]
@@ -175,6 +177,18 @@ class Scene:
                 t.join()
         self.last_counts = counts.sum(axis=0)
         return out
+
+    def primary_rays(self, settings, frame=1, sample=0):
+        """Camera rays (H*W records of abi.RAY_DTYPE) of one sample per pixel."""
+        rays = np.zeros(settings.height * settings.width, abi.RAY_DTYPE)
+        lib().or_primary_rays(self.h, C.byref(settings), frame, sample, _p(rays))
+        return rays
+
+    def first_hits(self, settings, frame=1):
+        """(H, W) int32 instance ids of sample 0's camera rays, -1 on a miss."""
+        ids = np.zeros((settings.height, settings.width), np.uint32)
+        lib().or_first_hits(self.h, C.byref(settings), frame, _p(ids))
+        return ids.view(np.int32)
 
     def texture_eval(self, tex, u, v):
         r = lib().or_texture_eval(self.h, tex, u, v)

@@ -1982,10 +1982,10 @@ static void fetch_shading(const or_scene *sc, const or_hit *h, v3 *p, v3 *perr, 
 }
 
 /* ------------------------------------------------------------- path tracer */
-v3 or_trace_path(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t px, uint32_t py,
-                 uint32_t sample, uint64_t *counts)
+/* ray generation: src/Shading.cu:1634-1725 */
+static or_ray primary_ray(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t px, uint32_t py,
+                          uint32_t sample, or_sampler *sp)
 {
-    /* ray generation: src/Shading.cu:1634-1725 */
     or_sampler s;
     memset(&s, 0, sizeof(s));
     v2 off;
@@ -2005,13 +2005,50 @@ v3 or_trace_path(const or_scene *sc, const ymrt_render_settings *rs, int frame, 
         off.x = or_sampler_next(&s);
         off.y = or_sampler_next(&s);
     }
+    *sp = s;
+    return camera_ray(sc, ((float)px + off.x) / (float)rs->width, ((float)py + off.y) / (float)rs->height);
+}
+
+/* the camera ray of one sample of every pixel, as ymrt_ray records */
+void or_primary_rays(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t sample, ymrt_ray *out)
+{
+    for (uint32_t y = 0; y < rs->height; ++y)
+        for (uint32_t x = 0; x < rs->width; ++x) {
+            or_sampler s;
+            or_ray r = primary_ray(sc, rs, frame, x, y, sample, &s);
+            ymrt_ray *o = &out[(size_t)y * rs->width + x];
+            o->ox = r.o.x; o->oy = r.o.y; o->oz = r.o.z; o->tmin = OR_TMIN;
+            o->dx = r.d.x; o->dy = r.d.y; o->dz = r.d.z; o->tmax = OR_TMAX;
+        }
+}
+
+/* instance hit first by sample 0's camera ray of every pixel (the picking
+ * buffer of RayTraceEditorView, src/Shading.cu:1545-1547) */
+void or_first_hits(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t *ids)
+{
+    for (uint32_t y = 0; y < rs->height; ++y)
+        for (uint32_t x = 0; x < rs->width; ++x) {
+            or_sampler s;
+            or_ray ray = primary_ray(sc, rs, frame, x, y, 0, &s);
+            or_hit h;
+            or_nearby nb[OR_MAX_NEARBY];
+            int nc = 0;
+            ids[(size_t)y * rs->width + x] = or_trace_ray(sc, &ray, &h, &nc, nb) ? h.inst : OR_EMPTY;
+        }
+}
+
+v3 or_trace_path(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t px, uint32_t py,
+                 uint32_t sample, uint64_t *counts)
+{
+    or_sampler s;
+    uint32_t pixel_index = py * rs->width + px;
+    or_ray ray = primary_ray(sc, rs, frame, px, py, sample, &s);
     rng_ctx rng;
     rng.s = &s;
     rng.hash_key[0] = pixel_index; rng.hash_key[1] = 0; rng.hash_key[2] = sample; rng.hash_key[3] = (uint32_t)frame;
     rng.hash_ctr = 0;
     path_ctx c = {sc, rs, &s, &rng, counts};
 
-    or_ray ray = camera_ray(sc, ((float)px + off.x) / (float)rs->width, ((float)py + off.y) / (float)rs->height);
     v3 L = s3(0.0f), beta = s3(1.0f);
     int depth = 1, never_scatter = 1;
     or_transfer t = sc->camera_transfer;

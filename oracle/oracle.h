@@ -9,24 +9,38 @@
  * Scene input is the same POD descriptor the product's C ABI takes
  * (include/ymrt.h), so both sides see identical float inputs.
  *
- * What pins each part (tests/test_oracle_*.py) without trusting the oracle:
+ * What pins each part (tests/test_oracle_*.py, tests/test_ymtmp.py) without
+ * trusting the oracle:
  *   primes, prime sums, MT19937, permutation table, radical inverses (exact
  *   rationals), Halton enumerator (brute force), PCG4d (published algorithm);
  *   triangle test (float64), BVH closest/any hit (brute force over all
- *   triangles), watertightness; sRGB decode (IEC 61966-2-1), bilinear image
- *   lookup (ramps), Fresnel (textbook complex-index formulas), edge tint
- *   (reflectivity identity), GGX D (normalisation), every DefaultMtl lobe
- *   (sample/eval consistency, sampling histogram), Henyey-Greenstein (moments,
- *   sampled density), homogeneous free flight (closed form), grid free flight
- *   and transmittance (constant grid == homogeneous), surface NEE with MIS
- *   (analytic irradiance of a square emitter), accumulate and ACES tonemap.
+ *   triangles), watertightness, analytic sphere (closed-form roots); sRGB
+ *   decode (IEC 61966-2-1), bilinear image lookup (ramps), Fresnel (textbook
+ *   complex-index formulas), edge tint (reflectivity identity), GGX D
+ *   (normalisation), every DefaultMtl lobe (sample/eval consistency, sampling
+ *   histogram), Henyey-Greenstein (moments, sampled density), homogeneous
+ *   free flight (closed form), grid free flight and transmittance (constant
+ *   grid == homogeneous), surface NEE with MIS (analytic irradiance of a
+ *   square emitter, a sphere emitter, a distant-light cone), accumulate and
+ *   ACES tonemap; Perlin noise (zero on the lattice, lattice gradients of
+ *   Perlin 2002 through the textbook hash, quintic fade along edges,
+ *   continuity), fBm / turbulence / marble / wood / wave / checkerboard /
+ *   polka-dot nodes (sums of the noise primitive, sign patterns), texture
+ *   graph order (nested graphs, empty inputs), camera texture-coordinate
+ *   differentials (pixel footprint), normal and bump maps (closed-form
+ *   irradiance of a tilted Lambert surface; flat maps change nothing);
+ *   .ymtmp transforms (glm from the reference tree, bit for bit, in
+ *   oracle/_ref), the .ymtmp fixture (values read off the file).
  *
  * Parity unpinned (checked only against the HIP path, no independent truth):
  *   indirect_light_clamp, Russian roulette and the environment background
  *   (all off in the measured workloads); next-event estimation at volume
- *   vertices (EvalVolumeShapeLight) and the mixture of overlapping media;
- *   the medium-record stack (RayTransfer priorities, BoundaryTransitionBunch,
- *   camera medium records, nested-dielectric pass-through).
+ *   vertices (EvalVolumeShapeLight, EvalVolumeDistantLight) and the mixture
+ *   of overlapping media; the medium-record stack (RayTransfer priorities,
+ *   BoundaryTransitionBunch, camera medium records, nested-dielectric
+ *   pass-through); the polka-dot centre jitter (only cell membership is
+ *   pinned); the .ymtmp statement rules beyond the cases tests/test_ymtmp.py
+ *   lists.
  */
 #ifndef ORACLE_H
 #define ORACLE_H
@@ -227,6 +241,8 @@ int or_bsdf_sample(const ymrt_material_desc *m, v3 wo, float ray_ior, float ex_i
 float or_hg_eval(float g, v3 wo, v3 wi, float *pdf);
 int or_hg_sample(float g, float u0, float u1, v3 wo, v3 *wi, float *pdf);
 v3 or_texture_eval(const or_scene *sc, uint32_t tex, float u, float v);
+void or_first_hits(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t *ids);
+void or_primary_rays(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t sample, ymrt_ray *out);
 /* TextureEval over the texture graph rooted at tex */
 v3 or_texture_graph(const or_scene *sc, uint32_t tex, or_tex_coord tc);
 float or_perlin(v3 p);

@@ -216,3 +216,19 @@ def test_ymtmp_cornell_fixture_parity(gpu_ctx):
     assert float(o[..., :3].mean()) > 0.01
     assert same >= 0.90
     assert err <= 0.01
+
+
+def test_instance_id_aov(gpu_ctx):
+    """ymrt_render_frame_aov: the id image is sample 0's camera-ray instance,
+    as the oracle traces it, and the noise image equals ymrt_render_frame's."""
+    sa = scenes.cornell_box(48, 40, analytic_sphere=True)
+    rs = abi.render_settings(48, 40, 4, 4)
+    sc = ymrt_amd.Scene(gpu_ctx, sa)
+    img, ids, _ = sc.render_frame_aov(rs, 2)
+    ref_img = sc.render_frame(rs, 2)[0]
+    assert torch.equal(img, ref_img)
+    o = oracle.Scene(sa).first_hits(rs, 2)
+    g = ids.cpu().numpy()
+    assert (g == o).mean() >= 0.999, (g != o).sum()
+    assert set(np.unique(g).tolist()) <= set(range(len(sa.instances))) | {-1}
+    assert len(np.unique(g)) >= 6

@@ -24,6 +24,7 @@ EXPORTS = [
     "ymrt_tonemap", "ymrt_halton_values", "ymrt_halton_enumerate", "ymrt_texture_eval",
     "ymrt_ymtmp_load", "ymrt_ymtmp_parse", "ymrt_ymtmp_scene", "ymrt_ymtmp_warning_count",
     "ymrt_ymtmp_warning", "ymrt_ymtmp_find", "ymrt_ymtmp_transform", "ymrt_ymtmp_free",
+    "ymrt_render_frame_aov", "ymrt_save_png",
 ]
 
 _lib = None
@@ -71,6 +72,9 @@ def lib():
         L.ymrt_ymtmp_find.argtypes = [vp, C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.ymrt_ymtmp_transform.argtypes = [vp, C.c_uint32, vp, vp]
         L.ymrt_ymtmp_free.argtypes = [vp]
+        L.ymrt_render_frame_aov.argtypes = [vp, C.POINTER(abi.RenderSettings), C.c_int, vp, vp, vp, C.c_int,
+                                            C.POINTER(abi.FrameStats)]
+        L.ymrt_save_png.argtypes = [C.c_char_p, vp, C.c_uint32, C.c_uint32]
         for name in EXPORTS:
             if name not in ("ymrt_last_error", "ymrt_abi_version"):
                 getattr(L, name).restype = C.c_int
@@ -207,6 +211,16 @@ class Scene:
                                        1 if timing else 0, C.byref(stats)))
         return noise, stats
 
+    def render_frame_aov(self, settings, frame=1):
+        """(noise image, int32 instance-id image, stats); ids are -1 on a miss."""
+        import torch
+        noise = torch.empty((settings.height, settings.width, 4), dtype=torch.float32, device="cuda")
+        ids = torch.empty((settings.height, settings.width), dtype=torch.int32, device="cuda")
+        stats = abi.FrameStats()
+        _check(lib().ymrt_render_frame_aov(self.h, C.byref(settings), frame, _dev_ptr(noise), _dev_ptr(ids),
+                                           _stream(), 0, C.byref(stats)))
+        return noise, ids, stats
+
 
 class YmtmpFile:
     """A .ymtmp scene file loaded by the library's parser (ymrt_ymtmp_load);
@@ -266,6 +280,13 @@ def halton_enumerate(width, height, px, py, sample):
     _check(lib().ymrt_halton_enumerate(width, height, _dev_ptr(t[0]), _dev_ptr(t[1]), _dev_ptr(t[2]), n,
                                        _dev_ptr(out), _stream()))
     return out.cpu().numpy().view(np.uint64)
+
+
+def save_png(path, rgba):
+    """SavePngImage of a host (H, W, 4) float32 image (row 0 at the bottom)."""
+    a = np.ascontiguousarray(rgba, np.float32)
+    h, w = a.shape[:2]
+    _check(lib().ymrt_save_png(os.fspath(path).encode(), a.ctypes.data_as(C.c_void_p), w, h))
 
 
 def accumulate(noise, accum, frame, max_frames=-1):

@@ -302,6 +302,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere 
             rays += 1;
             cam_rays += (info_depth(info) == 1 && (info & kInfoNeverScatter)) ? 1u : 0u;
             const bool hit = r == 1;
+            if (pass == 0 && prm.aov) ps.first_hit[p] = hit ? h.inst : kEmpty;
             const bool in_medium = kMedia && (ps.xhdr[p] & 0xFFu) != 0;
             if (!hit && !in_medium) {
                 f3 Lv = xyz(ps.L[p]);
@@ -1044,6 +1045,9 @@ __global__ void k_resolve(PtParams prm, PathState ps, float4 *noise)
         a3 += inv;
     }
     noise[(size_t)py * prm.rs.width + px] = make_float4(a0, a1, a2, a3);
+    // the id image holds sample 0's camera-ray instance (RayTraceEditorView,
+    // src/Shading.cu:1545-1547, keeps whichever sample wrote last)
+    if (prm.aov) prm.aov[(size_t)py * prm.rs.width + px] = ps.first_hit[(size_t)k * spp];
 }
 
 // ------------------------------------------------------------ post

@@ -64,6 +64,7 @@ struct PathState {
     uint32_t *queue[2];
     uint32_t *q_hit, *q_hit_vol, *q_medium, *q_shadow, *q_transmit;
     uint32_t *spill;    // traversal stack spill [kMaxStackDepth - kStackWindow][kSpillThreads]
+    uint32_t *first_hit;  // instance hit by the camera ray (written when an id image is requested)
 };
 
 // Shadow rays per path vertex: slots 0-1 shape light (light sample, BSDF or
@@ -81,6 +82,7 @@ struct PtParams {
     uint32_t *counters;
     unsigned long long *stats;
     uint32_t *error;
+    uint32_t *aov;           // per-pixel instance ids of the camera rays, or null
 };
 
 }  // namespace ym

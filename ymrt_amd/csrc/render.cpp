@@ -101,6 +101,7 @@ static int ensure_buffers(ymrt_scene *sc, uint32_t cap)
     rc |= alloc(rb, &ps.q_shadow, n);
     rc |= alloc(rb, &ps.q_transmit, n);
     rc |= alloc(rb, &ps.spill, (size_t)(kMaxStackDepth - kStackWindow) * kSpillThreads);
+    rc |= alloc(rb, &ps.first_hit, n);
     rc |= alloc(rb, &rb->counters, (size_t)(kMaxPasses + 2) * kCtrStride);
     rc |= alloc(rb, &rb->stats, kStatCount);
     if (rc) {
@@ -120,8 +121,8 @@ static hipEvent_t next_event(RenderBuffers *rb)
     return rb->events[rb->event_next++];
 }
 
-int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, float *noise, hipStream_t stream,
-                 int timing, ymrt_frame_stats *stats)
+int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, float *noise, uint32_t *instance_ids,
+                 hipStream_t stream, int timing, ymrt_frame_stats *stats)
 {
     YM_REQUIRE(rs->width > 0 && rs->height > 0, "image size must be positive");
     YM_REQUIRE(rs->spp >= 1, "spp must be >= 1");
@@ -148,6 +149,7 @@ int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, floa
     prm.counters = rb->counters;
     prm.stats = rb->stats;
     prm.error = sc->d_error;
+    prm.aov = instance_ids;
     const uint64_t total_paths = (uint64_t)prm.tiles_x * tiles_y * 256ull * spp;
     // The coincident-surface list only changes results where medium records
     // change: boundaries, dielectrics, priorities.
