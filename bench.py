@@ -15,6 +15,7 @@ the same frame.
 """
 import argparse
 import json
+import re
 import os
 import sys
 import time
@@ -51,10 +52,44 @@ def parse():
     ap.add_argument("--sampler", choices=["halton", "pcg"], default="halton")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before rendering, copy MIB MiB device-to-device once (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
+
+
+def adjoint_timing(ctx, frames=2):
+    """Forward and path-replay adjoint (Lambert albedo gradients) of the
+    Cornell box at 512x512, 16 spp, depth 6, timed with HIP events on the
+    render stream; the adjoint frame includes its forward rendering."""
+    import numpy as np
+    import torch
+
+    import ymrt_amd
+    from ymrt_amd import abi, scenes
+    sa = scenes.cornell_box(512, 512)
+    sc = ymrt_amd.Scene(ctx, sa)
+    rs = abi.render_settings(512, 512, 16, 6)
+    adj = torch.from_numpy(np.random.default_rng(5).uniform(0.5, 1.5, (512, 512, 3)).astype(np.float32)).cuda()
+    sc.render_frame(rs, 1)
+    sc.render_adjoint(rs, adj, 1)
+    torch.cuda.synchronize()
+    t = {}
+    for name in ("forward", "adjoint"):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for f in range(frames):
+            if name == "forward":
+                sc.render_frame(rs, 2 + f)
+            else:
+                sc.render_adjoint(rs, adj, 2 + f)
+        b.record()
+        torch.cuda.synchronize()
+        t[name] = a.elapsed_time(b) / frames
+    return {"workload": "Cornell box 512x512 16spp depth 6, gradients of 3 Lambert albedos (rgb)",
+            "forward_ms": round(t["forward"], 3), "adjoint_ms": round(t["adjoint"], 3),
+            "adjoint_over_forward": round(t["adjoint"] / t["forward"], 3)}
 
 
 def cpu_baseline(sa, rs_full, budget_s):
@@ -88,6 +123,21 @@ def cpu_baseline(sa, rs_full, budget_s):
         "sample": "oracle, the full %dx%d frame at %d spp (%d samples, %d rays), %.1f s on %d threads"
                   % (w, h, spp, samples, rays, dt, threads),
     }
+
+
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "k_extend_interior_1080p_pmc_traffic.md")
+
+
+def measured_traffic(kernel, ext_rays, launches):
+    """HBM bytes per k_extend launch: the calibrated FETCH_SIZE + WRITE_SIZE
+    per extension ray of the committed rocprofv3 counter runs (same scene and
+    kernels) times this run's extension rays per launch; None without it."""
+    if kernel != "k_extend" or not launches or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    m = re.search(r"traffic bytes per extension ray \(k_extend, calibrated fetch \+ write\): ([0-9.]+)",
+                  open(TRAFFIC_PROFILE).read())
+    return int(float(m.group(1)) * ext_rays / launches) if m else None
 
 
 def main():
@@ -198,7 +248,8 @@ def main():
         "rays_per_step": int(rays / args.steps),
         "stage_ms_per_step": {k: round(v, 2) for k, v in per_step.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(dom, ext_rays, dom_launch),
                      "bytes_per_launch": int(dom_bytes / max(dom_launch, 1)),
                      "avg_launch_ms": round(dom_ms / max(dom_launch, 1), 4),
                      "launches": dom_launch},
@@ -211,6 +262,8 @@ def main():
                       "nodes_per_transmittance_ray": round(visits["transmit_nodes"] / max(tr_rays, 1), 2)},
         "wall_s": round(wall, 2),
     }
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_timing(ctx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sa, rs, args.cpu_seconds)
     print(json.dumps(out), flush=True)

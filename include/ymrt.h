@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define YMRT_ABI_VERSION 2
+#define YMRT_ABI_VERSION 3
 
 /* status codes */
 #define YMRT_OK 0
@@ -258,6 +258,8 @@ typedef struct ymrt_frame_stats {
     float ms_transmit;               /* transmittance of shadow rays in media */
     uint32_t launches_transmit;
     uint64_t transmit_nodes, transmit_tris;   /* boundary walks of those rays */
+    float ms_adjoint;                /* gradient accumulation of ymrt_render_adjoint */
+    uint32_t reserved;
 } ymrt_frame_stats;
 
 /* ---------------------------------------------------------------- entry points */
@@ -316,6 +318,25 @@ int ymrt_render_frame(ymrt_scene *scene, const ymrt_render_settings *settings,
 int ymrt_render_frame_aov(ymrt_scene *scene, const ymrt_render_settings *settings,
                           int frame_index, float *noise_rgba, uint32_t *instance_ids,
                           void *stream, int timing, ymrt_frame_stats *stats);
+
+/* Path-replay adjoint of one frame for the albedo of Lambert-only DefaultMtl
+ * materials (specular, metalness, transmission and coat weights 0, no
+ * texture on those or on the albedo): renders the frame like
+ * ymrt_render_frame and adds to grad_albedo[m*3 + c] (device, material_count
+ * x 3 floats, not cleared) the derivative of sum_pixels adjoint_rgb . image
+ * with respect to diffuse_albedo[c] of material m, with the frame's own
+ * samples: every path term after a vertex on m is linear in that albedo
+ * (cosine sampling and MIS do not depend on it), so the derivative of a path
+ * is (L_path - L_before_vertex) / albedo summed over its vertices on m.
+ * Channels with albedo 0 and other materials get 0.  Needs Russian roulette
+ * and the indirect clamp off.  No counterpart in the reference, whose
+ * render loop has no derivatives; it is the adjoint its differentiable
+ * rendering workload asks for.  Gradients are summed with float atomics, so
+ * their last bits vary between runs. */
+int ymrt_render_adjoint(ymrt_scene *scene, const ymrt_render_settings *settings,
+                        int frame_index, float *noise_rgba, const float *adjoint_rgb,
+                        float *grad_albedo, void *stream, int timing,
+                        ymrt_frame_stats *stats);
 
 /* 8-bit RGBA PNG of a host RGBA32F image as SavePngImage writes it
  * (src/FileIO.cpp:167-194): the extension becomes ".png", parent folders

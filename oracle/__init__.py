@@ -81,6 +81,10 @@ def lib():
             L.or_glm_transform.argtypes = [vp, vp, vp, vp, vp, vp]
             L.or_first_hits.argtypes = [vp, C.POINTER(abi.RenderSettings), C.c_int, vp]
             L.or_primary_rays.argtypes = [vp, C.POINTER(abi.RenderSettings), C.c_int, C.c_uint32, vp]
+            L.or_render_adjoint.argtypes = [vp, C.POINTER(abi.RenderSettings), C.c_int, C.c_uint32, C.c_uint32,
+                                            vp, vp, vp]
+            L.or_lambert_only.restype = C.c_int
+            L.or_lambert_only.argtypes = [C.POINTER(abi.MaterialDesc)]
             L.or_perlin.restype = C.c_float
             L.or_perlin.argtypes = [V3]
             L.or_accumulate.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_int]
@@ -178,6 +182,28 @@ class Scene:
         self.last_counts = counts.sum(axis=0)
         return out
 
+    def render_adjoint(self, settings, adj, frame=1, threads=1):
+        """(noise image, gradient) of the path-replay adjoint for per-material
+        Lambert albedos: adj is dLoss/dImage (H, W, 3); the gradient is
+        (material_count, 3) float64, zero for materials that are not
+        Lambert-only."""
+        h, w = settings.height, settings.width
+        adj = np.ascontiguousarray(adj, np.float32).reshape(h, w, 3)
+        out = np.zeros((h, w, 4), np.float32)
+        nm = len(self.arrays.materials)
+        grads = np.zeros((max(1, threads), max(nm, 1), 3), np.float64)
+
+        def work(k, a, b):
+            lib().or_render_adjoint(self.h, C.byref(settings), frame, a, b, _p(adj), _p(grads[k]), _p(out))
+
+        bounds = np.linspace(0, h, max(1, threads) + 1).astype(int)
+        ts = [threading.Thread(target=work, args=(k, bounds[k], bounds[k + 1])) for k in range(max(1, threads))]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        return out, grads.sum(axis=0)[:nm]
+
     def primary_rays(self, settings, frame=1, sample=0):
         """Camera rays (H*W records of abi.RAY_DTYPE) of one sample per pixel."""
         rays = np.zeros(settings.height * settings.width, abi.RAY_DTYPE)
@@ -234,6 +260,10 @@ def mt19937(seed, n):
     m = MT()
     lib().or_mt_seed(C.byref(m), seed)
     return np.array([lib().or_mt_next(C.byref(m)) for _ in range(n)], np.uint64)
+
+
+def lambert_only(material):
+    return bool(lib().or_lambert_only(C.byref(material)))
 
 
 def perlin(p):

@@ -2037,8 +2037,73 @@ void or_first_hits(const or_scene *sc, const ymrt_render_settings *rs, int frame
         }
 }
 
+/* A DefaultMtl whose only lobe is Lambert with an untextured albedo: every
+ * path term after a vertex on it is linear in that albedo (cosine sampling,
+ * one lobe, MIS pdfs without the albedo), so d(term)/d(albedo) = term/albedo
+ * per channel. */
+int or_lambert_only(const ymrt_material_desc *m)
+{
+    return m->type == YMRT_MATERIAL_DEFAULT && m->specular_weight == 0.0f && m->metalness == 0.0f &&
+           m->transmission_weight == 0.0f && m->coat_weight == 0.0f && m->diffuse_albedo_tex == OR_EMPTY &&
+           m->specular_weight_tex == OR_EMPTY && m->metalness_tex == OR_EMPTY &&
+           m->transmission_weight_tex == OR_EMPTY && m->coat_weight_tex == OR_EMPTY;
+}
+
+typedef struct {
+    uint32_t n;
+    uint32_t m[OR_MAX_RAY_DEPTH];
+    v3 Lb[OR_MAX_RAY_DEPTH];   /* radiance of the path before the vertex */
+} adj_rec;
+
+static v3 trace_path(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t px, uint32_t py,
+                     uint32_t sample, uint64_t *counts, adj_rec *rec);
+
 v3 or_trace_path(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t px, uint32_t py,
                  uint32_t sample, uint64_t *counts)
+{
+    return trace_path(sc, rs, frame, px, py, sample, counts, NULL);
+}
+
+/* Path-replay adjoint of the frame for per-material Lambert albedos: with
+ * adj the per-pixel derivative of a loss with respect to the noise image
+ * (rgb), grad[m*3+c] += sum over paths of adj * (L_path - L_before_vertex) /
+ * albedo / spp over the path's vertices on Lambert-only material m.  Also
+ * writes the frame like or_render_rows. */
+void or_render_adjoint(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t row0, uint32_t row1,
+                       const float *adj, double *grad, float *out)
+{
+    float inv_spp = 1.0f / (float)(rs->spp > 1 ? rs->spp : 1);
+    adj_rec rec;
+    for (uint32_t y = row0; y < row1 && y < rs->height; ++y) {
+        for (uint32_t x = 0; x < rs->width; ++x) {
+            size_t pix = (size_t)y * rs->width + x;
+            float acc[4] = {0, 0, 0, 0};
+            for (int smp = 0; smp < rs->spp; ++smp) {
+                rec.n = 0;
+                v3 L = trace_path(sc, rs, frame, x, y, (uint32_t)smp, NULL, &rec);
+                acc[0] += L.x * inv_spp;
+                acc[1] += L.y * inv_spp;
+                acc[2] += L.z * inv_spp;
+                acc[3] += inv_spp;
+                for (uint32_t i = 0; i < rec.n; ++i) {
+                    const float *alb = sc->materials[rec.m[i]].diffuse_albedo;
+                    float d[3] = {L.x - rec.Lb[i].x, L.y - rec.Lb[i].y, L.z - rec.Lb[i].z};
+                    for (int c = 0; c < 3; ++c)
+                        if (alb[c] > 0.0f)
+                            grad[rec.m[i] * 3 + c] += (double)adj[pix * 3 + c] * (double)d[c] / (double)alb[c] *
+                                                      (double)inv_spp;
+                }
+            }
+            if (out) {
+                float *o = out + 4 * pix;
+                o[0] = acc[0]; o[1] = acc[1]; o[2] = acc[2]; o[3] = acc[3];
+            }
+        }
+    }
+}
+
+static v3 trace_path(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t px, uint32_t py,
+                     uint32_t sample, uint64_t *counts, adj_rec *rec)
 {
     or_sampler s;
     uint32_t pixel_index = py * rs->width + px;
@@ -2137,6 +2202,11 @@ v3 or_trace_path(const or_scene *sc, const ymrt_render_settings *rs, int frame, 
                            in->world_to_object, tc, duv);
         b.is_default = m->type == YMRT_MATERIAL_DEFAULT;
         if (b.is_default) init_default_bsdf(sc, &b, m, tc, ray.d, h.back, tr_ray_ior(&t), tr_ex_ior(&t, h.inst));
+        if (rec && rec->n < OR_MAX_RAY_DEPTH && or_lambert_only(m)) {
+            rec->m[rec->n] = in->material;
+            rec->Lb[rec->n] = L;
+            ++rec->n;
+        }
         if (rs->enable_distant_light)
             L = add3(L, light_clamp(rs, mul3(beta, eval_distant_light(&c, &b, ray.d, &h, &t, nc, nb, p, perr, ng)), depth));
         if (rs->enable_shape_light)

@@ -242,6 +242,9 @@ float or_hg_eval(float g, v3 wo, v3 wi, float *pdf);
 int or_hg_sample(float g, float u0, float u1, v3 wo, v3 *wi, float *pdf);
 v3 or_texture_eval(const or_scene *sc, uint32_t tex, float u, float v);
 void or_first_hits(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t *ids);
+int or_lambert_only(const ymrt_material_desc *m);
+void or_render_adjoint(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t row0, uint32_t row1,
+                       const float *adj, double *grad, float *out);
 void or_primary_rays(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t sample, ymrt_ray *out);
 /* TextureEval over the texture graph rooted at tex */
 v3 or_texture_graph(const or_scene *sc, uint32_t tex, or_tex_coord tc);

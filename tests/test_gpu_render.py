@@ -232,3 +232,23 @@ def test_instance_id_aov(gpu_ctx):
     assert (g == o).mean() >= 0.999, (g != o).sum()
     assert set(np.unique(g).tolist()) <= set(range(len(sa.instances))) | {-1}
     assert len(np.unique(g)) >= 6
+
+
+def test_adjoint_albedo_gradient(gpu_ctx):
+    """ymrt_render_adjoint against the oracle's path-replay adjoint (itself
+    pinned by central finite differences, tests/test_oracle_adjoint.py) on the
+    Cornell box: three Lambert-only wall materials get gradients, the metal,
+    glass and light materials none; the frame equals ymrt_render_frame's."""
+    sa = scenes.cornell_box(48, 48)
+    rs = abi.render_settings(48, 48, 8, 6)
+    rng = np.random.default_rng(5)
+    adj = rng.uniform(0.5, 1.5, (48, 48, 3)).astype(np.float32)
+    sc = ymrt_amd.Scene(gpu_ctx, sa)
+    img, grad, stats = sc.render_adjoint(rs, torch.from_numpy(adj).cuda(), 1, timing=True)
+    assert torch.equal(img, sc.render_frame(rs, 1)[0])
+    g = grad.cpu().numpy().astype(np.float64)
+    _, og = oracle.Scene(sa).render_adjoint(rs, adj, 1, threads=8)
+    print("adjoint gpu", g[:3].tolist(), "oracle", og[:3].tolist(), "ms", stats.ms_adjoint)
+    assert np.all(g[3:] == 0.0)
+    assert np.all(og[:3] > 0)
+    assert np.allclose(g[:3], og[:3], rtol=2e-3), (g, og)

@@ -24,7 +24,7 @@ EXPORTS = [
     "ymrt_tonemap", "ymrt_halton_values", "ymrt_halton_enumerate", "ymrt_texture_eval",
     "ymrt_ymtmp_load", "ymrt_ymtmp_parse", "ymrt_ymtmp_scene", "ymrt_ymtmp_warning_count",
     "ymrt_ymtmp_warning", "ymrt_ymtmp_find", "ymrt_ymtmp_transform", "ymrt_ymtmp_free",
-    "ymrt_render_frame_aov", "ymrt_save_png",
+    "ymrt_render_frame_aov", "ymrt_save_png", "ymrt_render_adjoint",
 ]
 
 _lib = None
@@ -75,6 +75,8 @@ def lib():
         L.ymrt_render_frame_aov.argtypes = [vp, C.POINTER(abi.RenderSettings), C.c_int, vp, vp, vp, C.c_int,
                                             C.POINTER(abi.FrameStats)]
         L.ymrt_save_png.argtypes = [C.c_char_p, vp, C.c_uint32, C.c_uint32]
+        L.ymrt_render_adjoint.argtypes = [vp, C.POINTER(abi.RenderSettings), C.c_int, vp, vp, vp, vp, C.c_int,
+                                          C.POINTER(abi.FrameStats)]
         for name in EXPORTS:
             if name not in ("ymrt_last_error", "ymrt_abi_version"):
                 getattr(L, name).restype = C.c_int
@@ -210,6 +212,21 @@ class Scene:
         _check(lib().ymrt_render_frame(self.h, C.byref(settings), frame, _dev_ptr(noise), _stream(),
                                        1 if timing else 0, C.byref(stats)))
         return noise, stats
+
+    def render_adjoint(self, settings, adjoint, frame=1, grad=None, timing=False):
+        """(noise image, gradient (material_count, 3) cuda tensor, stats) of
+        ymrt_render_adjoint; adjoint is dLoss/dImage, a (H, W, 3) cuda tensor."""
+        import torch
+        h, w = settings.height, settings.width
+        adjoint = adjoint.contiguous()
+        assert adjoint.shape == (h, w, 3) and adjoint.dtype == torch.float32
+        noise = torch.empty((h, w, 4), dtype=torch.float32, device="cuda")
+        if grad is None:
+            grad = torch.zeros((max(len(self.arrays.materials), 1), 3), dtype=torch.float32, device="cuda")
+        stats = abi.FrameStats()
+        _check(lib().ymrt_render_adjoint(self.h, C.byref(settings), frame, _dev_ptr(noise), _dev_ptr(adjoint),
+                                         _dev_ptr(grad), _stream(), 1 if timing else 0, C.byref(stats)))
+        return noise, grad, stats
 
     def render_frame_aov(self, settings, frame=1):
         """(noise image, int32 instance-id image, stats); ids are -1 on a miss."""

@@ -162,6 +162,7 @@ class FrameStats(C.Structure):
         ("shadow_nodes", C.c_uint64), ("shadow_tris", C.c_uint64),
         ("ms_transmit", C.c_float), ("launches_transmit", C.c_uint32),
         ("transmit_nodes", C.c_uint64), ("transmit_tris", C.c_uint64),
+        ("ms_adjoint", C.c_float), ("reserved", C.c_uint32),
     ]
 
 

@@ -12,7 +12,8 @@ void set_error(const std::string &msg) { g_last_error = msg; }
 
 int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d);                       // scene.cpp
 int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, float *noise, uint32_t *instance_ids,
-                 hipStream_t stream, int timing, ymrt_frame_stats *stats);      // render.cpp
+                 const float *adj_img, float *adj_grad, hipStream_t stream, int timing,
+                 ymrt_frame_stats *stats);                                      // render.cpp
 int accumulate(const float *noise, float *accum, uint32_t w, uint32_t h, int frame, hipStream_t s);
 int tonemap(float *rgba, uint32_t w, uint32_t h, float gamma, float exposure, hipStream_t s);
 
@@ -144,7 +145,26 @@ int ymrt_render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame_
     YM_REQUIRE(sc != nullptr && rs != nullptr && noise_rgba != nullptr, "NULL argument");
     YM_REQUIRE(frame_index >= 1, "frame_index starts at 1");
     try {
-        int rc = render_frame(sc, rs, frame_index, noise_rgba, nullptr, (hipStream_t)stream, timing, stats);
+        int rc = render_frame(sc, rs, frame_index, noise_rgba, nullptr, nullptr, nullptr, (hipStream_t)stream, timing,
+                              stats);
+        if (rc) return rc;
+    } catch (const std::exception &e) {
+        set_error(std::string("render failed: ") + e.what());
+        return YMRT_ERR_INTERNAL;
+    }
+    return check_stack_flag(sc);
+}
+
+int ymrt_render_adjoint(ymrt_scene *sc, const ymrt_render_settings *rs, int frame_index, float *noise_rgba,
+                        const float *adjoint_rgb, float *grad_albedo, void *stream, int timing,
+                        ymrt_frame_stats *stats)
+{
+    YM_REQUIRE(sc != nullptr && rs != nullptr && noise_rgba != nullptr, "NULL argument");
+    YM_REQUIRE(adjoint_rgb != nullptr && grad_albedo != nullptr, "NULL adjoint buffer");
+    YM_REQUIRE(frame_index >= 1, "frame_index starts at 1");
+    try {
+        int rc = render_frame(sc, rs, frame_index, noise_rgba, nullptr, adjoint_rgb, grad_albedo, (hipStream_t)stream,
+                              timing, stats);
         if (rc) return rc;
     } catch (const std::exception &e) {
         set_error(std::string("render failed: ") + e.what());
@@ -159,7 +179,8 @@ int ymrt_render_frame_aov(ymrt_scene *sc, const ymrt_render_settings *rs, int fr
     YM_REQUIRE(sc != nullptr && rs != nullptr && noise_rgba != nullptr, "NULL argument");
     YM_REQUIRE(frame_index >= 1, "frame_index starts at 1");
     try {
-        int rc = render_frame(sc, rs, frame_index, noise_rgba, instance_ids, (hipStream_t)stream, timing, stats);
+        int rc = render_frame(sc, rs, frame_index, noise_rgba, instance_ids, nullptr, nullptr, (hipStream_t)stream,
+                              timing, stats);
         if (rc) return rc;
     } catch (const std::exception &e) {
         set_error(std::string("render failed: ") + e.what());

@@ -1187,6 +1187,9 @@ __device__ __forceinline__ bool sample_heterogeneous(const DevScene &sc, int n, 
 
 // EvalTr (src/VolumeEval.cuh:253-418): transmittance of one segment inside
 // the current media.
+// kGrid: the scene has voxel-grid media; without them only the homogeneous
+// closed form is compiled.
+template <bool kGrid>
 __device__ __forceinline__ f3 eval_tr(const DevScene &sc, f3 ro_w, f3 rd_w, const Transfer &t, float tmax,
                                       HashRng &rng)
 {
@@ -1194,6 +1197,13 @@ __device__ __forceinline__ f3 eval_tr(const DevScene &sc, f3 ro_w, f3 rd_w, cons
     int vi[kMaxBoundaryRecord];
     const int n = t.volumes(vi, sc.volume_count);
     if (n == 0) return s3(1.0f);
+    if constexpr (!kGrid) {
+        f3 st = s3(0.0f);
+#pragma unroll
+        for (int i = 0; i < kMaxBoundaryRecord; ++i)
+            if (i < n) st = st + vsig(sc.volumes[vi[i]]);
+        return f3{expf(-st.x * tmax), expf(-st.y * tmax), expf(-st.z * tmax)};
+    }
     bool het = false;
 #pragma unroll
     for (int i = 0; i < kMaxBoundaryRecord; ++i)

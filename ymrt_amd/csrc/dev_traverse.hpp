@@ -242,33 +242,56 @@ struct TraceCounters {
         }                                             \
     } while (0)
 
-// Returns 1 on hit, 0 on miss, -1 on stack overflow (the ray is abandoned).
+// Resumable traversal: init() then step() until it returns something other
+// than kTravContinue.  One step visits one wide node, one top-level leaf
+// (an instance entry or a sphere) or one bottom-level leaf, then pops the
+// next entry, so a caller can interleave rays (a lane that finishes can
+// start another ray while the rest of its wave traverses).
 // kSphere: the scene has sphere instances (their test is compiled in only
 // then; it costs registers in the traversal loop).
-template <int kMode, bool kAny, bool kNearby, bool kCount, bool kSphere, int kStack, typename LS>
-YM_HD int traverse(const DevScene &sc, const Query &q, Hit &hit, int &nb_count, LS &ls, TraceCounters &cnt)
-{
-    const uint32_t root = kMode == kModeAll ? sc.tlas_root
-                          : kMode == kModeSkipBoundary ? sc.tlas_root_solid
-                                                       : sc.tlas_root_boundary;
-    if (sc.instance_count == 0 || root == kSentinel) return 0;
-    const bool skip_enabled = q.last_inst != kEmpty && q.last_tri != kEmpty;
-    RayPre pre = make_pre(q.o, q.d);
-    float tbest = q.tmax;
-    bool found = false;
-    bool in_blas = false;
-    uint32_t cur_inst = kEmpty;
-    // nearby-list state (TraceRay, src/ShadingUtilities.cuh:337-394)
-    bool inst_hit = false, ex_hit = false;
-    float ex_t = kTMax;
-    int ex_back = 0;
-    float tub = 0.0f, tlb = 0.0f;
-    int nbn = 0;
-    int sp = 0;
-    uint32_t node = root;
+constexpr int kTravContinue = 0, kTravHit = 1, kTravMiss = 2, kTravOverflow = -1;
+
+template <int kMode, bool kAny, bool kNearby, bool kCount, bool kSphere, int kStack>
+struct Traversal {
+    Query q;
+    RayPre pre;
+    Hit hit;
+    float tbest;
+    uint32_t node, cur_inst;
+    int sp, nb_count, nbn, ex_back;
+    bool found, in_blas, skip_enabled, inst_hit, ex_hit;
+    float ex_t, tub, tlb;
+
+    YM_HD int init(const DevScene &sc, const Query &query)
+    {
+        const uint32_t root = kMode == kModeAll ? sc.tlas_root
+                              : kMode == kModeSkipBoundary ? sc.tlas_root_solid
+                                                           : sc.tlas_root_boundary;
+        q = query;
+        found = false;
+        if (sc.instance_count == 0 || root == kSentinel) return kTravMiss;
+        skip_enabled = q.last_inst != kEmpty && q.last_tri != kEmpty;
+        pre = make_pre(q.o, q.d);
+        tbest = q.tmax;
+        in_blas = false;
+        cur_inst = kEmpty;
+        inst_hit = false;
+        ex_hit = false;
+        ex_t = kTMax;
+        ex_back = 0;
+        tub = 0.0f;
+        tlb = 0.0f;
+        nbn = 0;
+        sp = 0;
+        node = root;
+        return kTravContinue;
+    }
+
     // nearby-list update when the traversal leaves an instance (TraceRay's
     // per-instance block, src/ShadingUtilities.cuh:357-393)
-    auto nearby_exit = [&]() {
+    template <typename LS>
+    YM_HD void nearby_exit(LS &ls)
+    {
         if (inst_hit) {
             tub = next_float_up(next_float_up(tbest));
             tlb = next_float_down(next_float_down(tbest));
@@ -298,9 +321,11 @@ YM_HD int traverse(const DevScene &sc, const Query &q, Hit &hit, int &nb_count, 
             }
         }
         nb_count = nbn;
-    };
+    }
 
-    for (;;) {
+    template <typename LS>
+    YM_HD int step(const DevScene &sc, LS &ls, TraceCounters &cnt)
+    {
         if (!(node & kLeafBit)) {
             const WNode *n = sc.nodes + node;
             const float4 lx = *reinterpret_cast<const float4 *>(n->lo_x);
@@ -337,12 +362,12 @@ YM_HD int traverse(const DevScene &sc, const Query &q, Hit &hit, int &nb_count, 
             YM_SORT2(k1, v1, k2, v2);
             if (k0 != INFINITY) {
                 int need = (k1 != INFINITY) + (k2 != INFINITY) + (k3 != INFINITY);
-                if (sp + need > kStack) return -1;
+                if (sp + need > kStack) return kTravOverflow;
                 if (k3 != INFINITY) ls.push(sp++, v3);
                 if (k2 != INFINITY) ls.push(sp++, v2);
                 if (k1 != INFINITY) ls.push(sp++, v1);
                 node = v0;
-                continue;
+                return kTravContinue;
             }
         } else if (!in_blas) {
             // top-level leaf: one instance
@@ -367,7 +392,7 @@ YM_HD int traverse(const DevScene &sc, const Query &q, Hit &hit, int &nb_count, 
                         hit.b2 = st.p.z;
                         hit.back = st.back;
                     }
-                    if (kAny) return 1;
+                    if (kAny) return kTravHit;
                 }
                 if (kNearby) {
                     cur_inst = ii;
@@ -375,10 +400,10 @@ YM_HD int traverse(const DevScene &sc, const Query &q, Hit &hit, int &nb_count, 
                     ex_hit = st.ex_hit;
                     ex_t = st.ex_t;
                     ex_back = st.ex_back;
-                    nearby_exit();
+                    nearby_exit(ls);
                 }
             } else if (take) {
-                if (sp + 1 > kStack) return -1;
+                if (sp + 1 > kStack) return kTravOverflow;
                 ls.push(sp++, kSentinel);
                 pre = make_pre(xf_point(in.w2o, q.o), xf_dir(in.w2o, q.d));
                 in_blas = true;
@@ -387,7 +412,7 @@ YM_HD int traverse(const DevScene &sc, const Query &q, Hit &hit, int &nb_count, 
                 ex_hit = false;
                 ex_t = kTMax;
                 node = in.blas_root;
-                continue;
+                return kTravContinue;
             }
         } else {
             const uint32_t first = leaf_first(node), count = leaf_count(node);
@@ -418,22 +443,35 @@ YM_HD int traverse(const DevScene &sc, const Query &q, Hit &hit, int &nb_count, 
                     hit.back = tri_back(q0, q1, q2, pre.d);
                 }
                 inst_hit = true;
-                if (kAny) return 1;
+                if (kAny) return kTravHit;
             }
         }
         // pop
         for (;;) {
-            if (sp == 0) {
-                return found ? 1 : 0;
-            }
+            if (sp == 0) return found ? kTravHit : kTravMiss;
             node = ls.pop(--sp);
             if (node != kSentinel) break;
             // leaving an instance: back to world space
             pre = make_pre(q.o, q.d);
             in_blas = false;
-            if (kNearby) nearby_exit();
+            if (kNearby) nearby_exit(ls);
         }
+        return kTravContinue;
     }
+};
+
+// Returns 1 on hit, 0 on miss, -1 on stack overflow (the ray is abandoned).
+template <int kMode, bool kAny, bool kNearby, bool kCount, bool kSphere, int kStack, typename LS>
+YM_HD int traverse(const DevScene &sc, const Query &q, Hit &hit, int &nb_count, LS &ls, TraceCounters &cnt)
+{
+    Traversal<kMode, kAny, kNearby, kCount, kSphere, kStack> tv;
+    tv.hit = hit;
+    tv.nb_count = nb_count;
+    int r = tv.init(sc, q);
+    while (r == kTravContinue) r = tv.step(sc, ls, cnt);
+    hit = tv.hit;
+    nb_count = tv.nb_count;
+    return r == kTravHit ? 1 : (r == kTravMiss ? 0 : -1);
 }
 
 #undef YM_SORT2

@@ -49,6 +49,7 @@ struct ymrt_scene {
     bool needs_nearby = false;    // boundaries, dielectrics or ior priorities present
     ym::RenderBuffers *render = nullptr;
     uint32_t *d_error = nullptr;  // device flag: traversal stack overflow
+    const uint32_t *adj_mat = nullptr;  // per material: 1 when Lambert-only (adjoint)
 };
 
 namespace ym {

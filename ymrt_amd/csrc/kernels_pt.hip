@@ -86,6 +86,31 @@ __device__ __forceinline__ void run_items(uint32_t *fetch, uint32_t n, uint32_t 
     }
 }
 
+// Sorts the first n of kN slot ids in LDS ascending (bitonic network over kN,
+// the tail padded with ~0u) with one wave; a batch published in slot order
+// lets the next stage's lanes read neighbouring path-state entries together.
+template <uint32_t kN>
+__device__ __forceinline__ void wave_sort_lds(uint32_t *s, uint32_t n)
+{
+    for (uint32_t j = n + lane_id(); j < kN; j += kBlock) s[j] = ~0u;
+    __syncthreads();
+#pragma unroll 1
+    for (uint32_t k = 2; k <= kN; k <<= 1) {
+#pragma unroll 1
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t t = lane_id(); t < kN / 2; t += kBlock) {
+                const uint32_t i = 2u * t - (t & (j - 1u)), l = i + j;
+                const uint32_t a = s[i], b = s[l];
+                if ((a > b) == ((i & k) == 0u)) {
+                    s[i] = b;
+                    s[l] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 __device__ __forceinline__ void wave_add64(unsigned long long *counter, uint32_t v)
 {
     // sum v over the wave, one atomic
@@ -234,6 +259,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(DevScene sc, PtParams prm, Pa
                 ps.L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 store_sampler(ps, p, smp);
                 ps.info[p] = 1u | kInfoNeverScatter;
+                if (prm.adj_mat) ps.adj_n[p] = 0;
                 Transfer t;
                 t.count = cam.record_count;
                 t.max_idx = cam.max_idx;
@@ -250,35 +276,84 @@ __global__ __launch_bounds__(kBlock) void k_raygen(DevScene sc, PtParams prm, Pa
 // hand their hit or miss to k_medium, which samples free flight; the others
 // resolve misses here and queue surface hits for shading.  Visited wide nodes
 // and tested triangles are summed per wave into the frame statistics.
+//
+// Lane refill: each lane keeps its own traversal; once kRefillLanes lanes of
+// the wave have finished, their results are written and they start new rays
+// from the wave's fetched range while the others stay where they are in their
+// trees.  Ray lengths vary by orders of magnitude, so a wave that waited for
+// its slowest ray would run most of its steps with few lanes active.
+constexpr int kRefillLanes = 16;
+constexpr int kLaneIdle = 3;  // no ray (beside kTravContinue/Hit/Miss/Overflow)
+constexpr uint32_t kStageCap = kSub * kBlock;
+
 template <bool kNearby, bool kMedia, bool kSphere, int kStack>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere ? 1 : 4))) void k_extend(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
 {
+    constexpr int kQ = kMedia ? 2 : 1;
     __shared__ uint32_t stack[kStackWindow * kBlock];
     __shared__ uint32_t nbp[(kNearby ? kNearbyCap : 1) * kBlock];
     __shared__ float nbt[(kNearby ? kNearbyCap : 1) * kBlock];
     __shared__ int nbb[(kNearby ? kNearbyCap : 1) * kBlock];
+    __shared__ uint32_t stage[kQ * kStageCap];
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     const uint32_t *qin = ps.queue[pass & 1];
     const uint32_t n = cnt[kCtrActive];
     LaneScratch<kBlock, kStackWindow> ls{stack, nbp, nbt, nbb, (int)lane_id(), ps.spill, kSpillThreads,
                                          blockIdx.x * kBlock + threadIdx.x};
     const ymrt_render_settings &rs = prm.rs;
+    const uint64_t below = (1ull << lane_id()) - 1ull;
     uint32_t rays = 0, cam_rays = 0;
     TraceCounters tc{0, 0};
     uint32_t *const ctrs[2] = {cnt + kCtrHit, cnt + kCtrMedium};
     uint32_t *const qs[2] = {ps.q_hit, ps.q_medium};
-    run_items<kMedia ? 2 : 1>(cnt + kCtrWorkExtend, n, ctrs, qs, [&](uint32_t i, uint32_t &p, bool(&out)[kMedia ? 2 : 1]) {
-        {
-            p = qin[i];
-            uint32_t info = ps.info[p];
-            if (info & kInfoPendingBunch) {
-                Transfer t = load_transfer(ps, p, prm.cap);
-                const NearbyList nb = load_nearby(ps, p, prm.cap, ps.hit_a[p].w);
-                boundary_bunch(sc, t, nb);
-                store_transfer(ps, p, prm.cap, t);
-                info &= ~kInfoPendingBunch;
-                ps.info[p] = info;
+    uint32_t filled[kQ] = {};
+    // publish staged survivors of queue Q: one atomic, coalesced copy
+    auto flush = [&](auto Q) {
+        __syncthreads();
+        if (filled[Q()]) {
+            wave_sort_lds<kStageCap>(stage + Q() * kStageCap, filled[Q()]);
+            uint32_t dst = 0;
+            if (lane_id() == 0) dst = atomicAdd(ctrs[Q()], filled[Q()]);
+            dst = __shfl(dst, 0);
+            for (uint32_t j = lane_id(); j < filled[Q()]; j += kBlock)
+                qs[Q()][dst + j] = stage[Q() * kStageCap + j];
+            filled[Q()] = 0;
+        }
+        __syncthreads();
+    };
+
+    Traversal<kModeAll, false, kNearby, true, kSphere, kStack> tv;
+    uint32_t p = 0;
+    int r = kLaneIdle;
+    uint32_t cur = 0, end = 0;  // the wave's fetched range [cur, end)
+    bool more = true;           // the work counter may hold items
+    for (;;) {
+        // refill idle lanes from the fetched range, fetching as it runs dry
+        bool fresh = false;
+        uint64_t idle = __ballot(r == kLaneIdle);
+        while (idle && more) {
+            if (cur >= end) {
+                const uint32_t base = wave_fetch(cnt + kCtrWorkExtend);
+                if (base >= n) {
+                    more = false;
+                    break;
+                }
+                cur = base;
+                end = base + (uint32_t)(kBlock * kSub) < n ? base + (uint32_t)(kBlock * kSub) : n;
             }
+            const uint32_t take = min((uint32_t)__popcll(idle), end - cur);
+            if (r == kLaneIdle) {
+                const uint32_t rank = (uint32_t)__popcll(idle & below);
+                if (rank < take) {
+                    p = qin[cur + rank];
+                    r = kTravContinue;
+                    fresh = true;
+                }
+            }
+            cur += take;
+            idle = __ballot(r == kLaneIdle);
+        }
+        if (fresh) {
             const float4 o4 = ps.ray_o[p], d4 = ps.ray_d[p];
             const uint2 last = ps.last[p];
             Query q;
@@ -288,32 +363,53 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere 
             q.tmax = kTMax;
             q.last_inst = last.x;
             q.last_tri = last.y;
-            Hit h;
-            h.back = 0;
-            h.inst = kEmpty;
-            h.tri = 0;
-            h.b0 = h.b1 = h.b2 = 0.0f;
-            int nbc = 0;
-            int r = traverse<kModeAll, false, kNearby, true, kSphere, kStack>(sc, q, h, nbc, ls, tc);
-            if (r < 0) {
-                atomicOr(prm.error, 1u);
-                r = 0;
+            tv.hit.back = 0;
+            tv.hit.inst = kEmpty;
+            tv.hit.tri = 0;
+            tv.hit.b0 = tv.hit.b1 = tv.hit.b2 = 0.0f;
+            tv.nb_count = 0;
+            r = tv.init(sc, q);
+        }
+        // step until kRefillLanes lanes are done (all of them once the
+        // counter is drained: nothing could refill them)
+        const uint32_t want = more ? (uint32_t)kRefillLanes : (uint32_t)kBlock;
+        for (;;) {
+            const uint64_t busy = __ballot(r == kTravContinue);
+            if (busy == 0 || (uint32_t)(kBlock - __popcll(busy)) >= want) break;
+            if (r == kTravContinue) r = tv.step(sc, ls, tc);
+        }
+        bool out[kQ] = {};
+        if (r != kTravContinue && r != kLaneIdle) {
+            if (r == kTravOverflow) atomicOr(prm.error, 1u);
+            const bool hit = r == kTravHit;
+            r = kLaneIdle;
+            uint32_t info = ps.info[p];
+            if (info & kInfoPendingBunch) {
+                // the boundary bunch of the previous vertex reads that
+                // vertex's nearby list, which the writes below replace
+                Transfer t = load_transfer(ps, p, prm.cap);
+                const NearbyList nb = load_nearby(ps, p, prm.cap, ps.hit_a[p].w);
+                boundary_bunch(sc, t, nb);
+                store_transfer(ps, p, prm.cap, t);
+                info &= ~kInfoPendingBunch;
+                ps.info[p] = info;
             }
             rays += 1;
             cam_rays += (info_depth(info) == 1 && (info & kInfoNeverScatter)) ? 1u : 0u;
-            const bool hit = r == 1;
+            const Hit &h = tv.hit;
             if (pass == 0 && prm.aov) ps.first_hit[p] = hit ? h.inst : kEmpty;
             const bool in_medium = kMedia && (ps.xhdr[p] & 0xFFu) != 0;
             if (!hit && !in_medium) {
                 f3 Lv = xyz(ps.L[p]);
                 if (rs.enable_env_light)
-                    Lv = Lv + light_clamp(rs, xyz(ps.beta[p]) * background(q.d), info_depth(info));
+                    Lv = Lv + light_clamp(rs, xyz(ps.beta[p]) * background(tv.q.d), info_depth(info));
                 ps.out[p] = f4(Lv, 0.0f);
             } else {
                 out[0] = !in_medium;
                 if constexpr (kMedia) out[1] = in_medium;
                 uint32_t flags = h.back ? kHitBack : 0u;
                 bool listed = false;
+                const int nbc = tv.nb_count;
                 if constexpr (kNearby) {
                     if (hit) {
                         listed = true;
@@ -336,7 +432,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere 
                 ps.hit_b[p] = make_float4(h.b0, h.b1, h.b2, 0.0f);
             }
         }
-    });
+        static_for<0, kQ>([&](auto Q) {
+            const uint64_t m = __ballot(out[Q()]);
+            if (out[Q()]) stage[Q() * kStageCap + filled[Q()] + (uint32_t)__popcll(m & below)] = p;
+            filled[Q()] += (uint32_t)__popcll(m);
+            if (filled[Q()] > kStageCap - kBlock) flush(Q);
+        });
+        if (!more && __ballot(r != kLaneIdle) == 0) break;
+    }
+    static_for<0, kQ>([&](auto Q) { flush(Q); });
     wave_add64(prm.stats + kStatExtension, rays);
     wave_add64(prm.stats + kStatCamera, cam_rays);
     wave_add64(prm.stats + kStatExtendNodes, tc.nodes);
@@ -408,7 +512,11 @@ __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PtParams prm, Pa
 
 // ------------------------------------------------------------ shade
 // Records shadow ray k of path p (origin, direction, maximum distance, BSDF or
-// phase value, clamped Li, MIS weight) straight to its slot in HBM.
+// phase value, clamped Li, MIS weight) straight to its slot in HBM.  The
+// direction is renormalised as the reference's Ray constructor does
+// (src/Ray.h:28-29): a light-sample direction is already unit length up to
+// rounding, and those last bits decide self-hits of rays that leave a surface
+// with no spawn offset.
 struct ShadowEmit {
     const PathState &ps;
     uint32_t cap, p;
@@ -417,7 +525,7 @@ struct ShadowEmit {
     {
         const size_t o = (size_t)k * cap + p;
         ps.sh_o[o] = f4(orig, tmax);
-        ps.sh_d[o] = f4(dir, 0.0f);
+        ps.sh_d[o] = f4(normalize(dir), 0.0f);
         ps.sh_f[o] = f4(f, mis);
         ps.sh_li[o] = f4(li, 0.0f);
         flags |= (1u << k) | (transmit ? (kShTransmit0 << k) : 0u);
@@ -670,6 +778,13 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                     init_shading_space(bs, tx, normal_tex, bump_tex, back, ray_d, ng, sn, dpdu, dpdv, in.w2o, duv);
                     bs.mask = 0;
                     if (is_default) init_default_bsdf(tx, bs, m, ray_d, back, t.ray_ior(), t.ex_ior(inst));
+                    if (prm.adj_mat && prm.adj_mat[in.material]) {
+                        // every later term of the path carries this vertex's albedo once
+                        const uint32_t k = ps.adj_n[p];
+                        ps.adj_rec[(size_t)k * cap + p] =
+                            make_uint4(in.material, __float_as_uint(L.x), __float_as_uint(L.y), __float_as_uint(L.z));
+                        ps.adj_n[p] = k + 1;
+                    }
                     const f3 wo = bs.to_local(-ray_d);
                     if (rs.enable_distant_light && sc.distant_count > 0) {
                         const DevDistant &dl = sc.distant[pick_distant(sc.distant_count, sampler_next(sc, smp))];
@@ -907,7 +1022,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 // (src/VolumeEval.cuh:420-468) walks the boundary instances along the ray,
 // updating the medium records at each crossing and multiplying the segment
 // transmittances (EvalTr, 253-418).
-template <bool kSphere, int kStack>
+template <bool kSphere, int kStack, bool kGrid>
 __global__ __launch_bounds__(kBlock) void k_transmit(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
 {
     __shared__ uint32_t stack[kStackWindow * kBlock];
@@ -927,9 +1042,7 @@ __global__ __launch_bounds__(kBlock) void k_transmit(DevScene sc, PtParams prm, 
             slot = p;
             const uint32_t fl = ps.sh_flags[p];
             const uint32_t info = ps.info[p];
-            const Transfer t0 = load_transfer(ps, p, cap);
             const uint4 ha = ps.hit_a[p];
-            const NearbyList nb0 = load_nearby(ps, p, cap, ha.w);
             HashRng rng;
             {
                 const uint32_t gp = (uint32_t)(prm.chunk_base + p);
@@ -952,8 +1065,10 @@ __global__ __launch_bounds__(kBlock) void k_transmit(DevScene sc, PtParams prm, 
                 const float4 o4 = ps.sh_o[(size_t)k * cap + p];
                 const float4 d4 = ps.sh_d[(size_t)k * cap + p];
                 f3 tr = s3(1.0f);
-                Transfer t = t0;
-                if (fl & (kShTransmit0 << k)) boundary_bunch(sc, t, nb0);
+                // the vertex's records and nearby list are reloaded per ray
+                // rather than held across the loop (fewer live registers)
+                Transfer t = load_transfer(ps, p, cap);
+                if (fl & (kShTransmit0 << k)) boundary_bunch(sc, t, load_nearby(ps, p, cap, ha.w));
                 const bool volv = (fl & kShVolume) != 0;
                 Query tq;
                 tq.o = xyz(o4);
@@ -978,10 +1093,10 @@ __global__ __launch_bounds__(kBlock) void k_transmit(DevScene sc, PtParams prm, 
                         break;
                     }
                     if (br == 0) {
-                        tr = tr * eval_tr(sc, tq.o, tq.d, t, gmax(maxd - traced, 0.0f), rng);
+                        tr = tr * eval_tr<kGrid>(sc, tq.o, tq.d, t, gmax(maxd - traced, 0.0f), rng);
                         break;
                     }
-                    tr = tr * eval_tr(sc, tq.o, tq.d, t, bh.t, rng);
+                    tr = tr * eval_tr<kGrid>(sc, tq.o, tq.d, t, bh.t, rng);
                     traced += bh.t;
                     f3 bp, bperr, bng;
                     hit_geometry(sc, bh.inst, bh.tri, bh.b0, bh.b1, bh.b2, bp, bperr, bng);
@@ -993,7 +1108,9 @@ __global__ __launch_bounds__(kBlock) void k_transmit(DevScene sc, PtParams prm, 
                         bnb.back[J()] = nbb[J() * kBlock + lane_id()];
                     });
                     boundary_bunch(sc, t, bnb);
+                    // the next segment is a new Ray, direction renormalised
                     tq.o = offset_ray_origin(bp, bperr, tq.d, bng);
+                    tq.d = normalize(tq.d);
                     tq.tmax = kTMax;
                     tq.last_inst = bh.inst;
                     tq.last_tri = bh.tri;
@@ -1170,12 +1287,18 @@ int launch_shadow(const DevScene &sc, const PtParams &prm, const PathState &ps, 
 
 int launch_transmit(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
 {
-    if (sc.has_spheres)
-        hipLaunchKernelGGL((k_transmit<true, kMaxStackDepth>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s,
-                           sc, prm, ps, pass);
-    else
-        hipLaunchKernelGGL((k_transmit<false, kMaxStackDepth>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s,
-                           sc, prm, ps, pass);
+    const dim3 g = persistent_grid(prm.chunk_paths), b(kBlock);
+    if (sc.grid_count > 0) {
+        if (sc.has_spheres)
+            hipLaunchKernelGGL((k_transmit<true, kMaxStackDepth, true>), g, b, 0, s, sc, prm, ps, pass);
+        else
+            hipLaunchKernelGGL((k_transmit<false, kMaxStackDepth, true>), g, b, 0, s, sc, prm, ps, pass);
+    } else {
+        if (sc.has_spheres)
+            hipLaunchKernelGGL((k_transmit<true, kMaxStackDepth, false>), g, b, 0, s, sc, prm, ps, pass);
+        else
+            hipLaunchKernelGGL((k_transmit<false, kMaxStackDepth, false>), g, b, 0, s, sc, prm, ps, pass);
+    }
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }
@@ -1191,6 +1314,59 @@ int launch_resolve(const PtParams &prm, const PathState &ps, float *noise, hipSt
 {
     const uint32_t pix = prm.chunk_paths / (uint32_t)prm.rs.spp;
     hipLaunchKernelGGL(k_resolve, dim3((pix + 255) / 256), dim3(256), 0, s, prm, ps, reinterpret_cast<float4 *>(noise));
+    YM_HIP(hipGetLastError());
+    return YMRT_OK;
+}
+
+// Path-replay adjoint for Lambert-only albedos: for each path of the chunk,
+// d(loss)/d(albedo_m) += adj(pixel) * (L_path - L_before_vertex) / albedo_m
+// / spp over its recorded vertices.  Sums go through LDS per block, then one
+// global atomic per block and material channel (or straight to global memory
+// when the materials do not fit the LDS table).
+constexpr uint32_t kAdjLdsMaterials = 1024;
+
+__global__ __launch_bounds__(256) void k_adjoint(DevScene sc, PtParams prm, PathState ps)
+{
+    __shared__ float acc[kAdjLdsMaterials * 3];
+    const bool lds = sc.material_count <= kAdjLdsMaterials;
+    if (lds)
+        for (uint32_t i = threadIdx.x; i < sc.material_count * 3; i += blockDim.x) acc[i] = 0.0f;
+    __syncthreads();
+    const uint32_t spp = (uint32_t)prm.rs.spp;
+    const float inv = 1.0f / (float)(spp > 1 ? spp : 1);
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < prm.chunk_paths; p += gridDim.x * blockDim.x) {
+        const uint64_t vpix = prm.chunk_base / spp + p / spp;
+        const uint32_t tile = (uint32_t)(vpix >> 8), within = (uint32_t)(vpix & 255u);
+        const uint32_t ty = tile / prm.tiles_x, tx = tile - ty * prm.tiles_x;
+        const uint32_t px = tx * 16u + (within & 15u), py = ty * 16u + (within >> 4);
+        if (px >= prm.rs.width || py >= prm.rs.height) continue;
+        const size_t pix = (size_t)py * prm.rs.width + px;
+        const f3 a{prm.adj_img[pix * 3 + 0] * inv, prm.adj_img[pix * 3 + 1] * inv, prm.adj_img[pix * 3 + 2] * inv};
+        const f3 Lt = xyz(ps.out[p]);
+        const uint32_t n = ps.adj_n[p];
+        for (uint32_t k = 0; k < n; ++k) {
+            const uint4 r = ps.adj_rec[(size_t)k * prm.cap + p];
+            const float *alb = sc.materials[r.x].diffuse_albedo;
+            const f3 d = Lt - f3{__uint_as_float(r.y), __uint_as_float(r.z), __uint_as_float(r.w)};
+            const float g[3] = {d.x * a.x, d.y * a.y, d.z * a.z};
+            for (int c = 0; c < 3; ++c) {
+                if (!(alb[c] > 0.0f)) continue;
+                const float v = g[c] / alb[c];
+                if (lds) atomicAdd(&acc[r.x * 3 + c], v);
+                else atomicAdd(&prm.adj_grad[r.x * 3 + c], v);
+            }
+        }
+    }
+    if (!lds) return;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < sc.material_count * 3; i += blockDim.x)
+        if (acc[i] != 0.0f) atomicAdd(&prm.adj_grad[i], acc[i]);
+}
+
+int launch_adjoint(const DevScene &sc, const PtParams &prm, const PathState &ps, hipStream_t s)
+{
+    const uint32_t blocks = std::min<uint32_t>((prm.chunk_paths + 255) / 256, 1024u);
+    hipLaunchKernelGGL(k_adjoint, dim3(blocks), dim3(256), 0, s, sc, prm, ps);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }

@@ -65,6 +65,8 @@ struct PathState {
     uint32_t *q_hit, *q_hit_vol, *q_medium, *q_shadow, *q_transmit;
     uint32_t *spill;    // traversal stack spill [kMaxStackDepth - kStackWindow][kSpillThreads]
     uint32_t *first_hit;  // instance hit by the camera ray (written when an id image is requested)
+    uint4 *adj_rec;       // adjoint: [ray_depth][cap] (material, radiance before the vertex)
+    uint32_t *adj_n;      // adjoint: records per path
 };
 
 // Shadow rays per path vertex: slots 0-1 shape light (light sample, BSDF or
@@ -83,6 +85,9 @@ struct PtParams {
     unsigned long long *stats;
     uint32_t *error;
     uint32_t *aov;           // per-pixel instance ids of the camera rays, or null
+    const uint32_t *adj_mat; // adjoint: 1 per Lambert-only material, or null (no adjoint)
+    const float *adj_img;    // adjoint: dLoss/dImage, width*height*3
+    float *adj_grad;         // adjoint: gradient, material_count*3 (accumulated)
 };
 
 }  // namespace ym

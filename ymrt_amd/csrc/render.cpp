@@ -28,11 +28,13 @@ int launch_shadow(const DevScene &sc, const PtParams &prm, const PathState &ps, 
 int launch_transmit(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s);
 int launch_flush(const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s);
 int launch_resolve(const PtParams &prm, const PathState &ps, float *noise, hipStream_t s);
+int launch_adjoint(const DevScene &sc, const PtParams &prm, const PathState &ps, hipStream_t s);
 
 constexpr uint32_t kDefaultCapacity = 1u << 22;
 
 struct RenderBuffers {
     uint32_t cap = 0;
+    uint32_t adj_depth = 0;  // vertex records allocated per path for the adjoint
     PathState ps{};
     std::vector<void *> allocs;
     uint32_t *counters = nullptr;
@@ -121,8 +123,18 @@ static hipEvent_t next_event(RenderBuffers *rb)
     return rb->events[rb->event_next++];
 }
 
+static int ensure_adjoint(RenderBuffers *rb, uint32_t depth)
+{
+    if (rb->adj_depth >= depth) return YMRT_OK;
+    int rc = alloc(rb, &rb->ps.adj_rec, (size_t)depth * rb->cap);
+    rc |= alloc(rb, &rb->ps.adj_n, rb->cap);
+    if (rc) return rc;
+    rb->adj_depth = depth;
+    return YMRT_OK;
+}
+
 int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, float *noise, uint32_t *instance_ids,
-                 hipStream_t stream, int timing, ymrt_frame_stats *stats)
+                 const float *adj_img, float *adj_grad, hipStream_t stream, int timing, ymrt_frame_stats *stats)
 {
     YM_REQUIRE(rs->width > 0 && rs->height > 0, "image size must be positive");
     YM_REQUIRE(rs->spp >= 1, "spp must be >= 1");
@@ -150,6 +162,15 @@ int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, floa
     prm.stats = rb->stats;
     prm.error = sc->d_error;
     prm.aov = instance_ids;
+    if (adj_img) {
+        YM_REQUIRE(adj_grad != nullptr, "adjoint needs a gradient buffer");
+        YM_REQUIRE(!rs->enable_russian_roulette, "the adjoint needs Russian roulette off");
+        YM_REQUIRE(!rs->enable_indirect_clamp, "the adjoint needs the indirect clamp off");
+        if ((rc = ensure_adjoint(rb, (uint32_t)rs->ray_depth))) return rc;
+        prm.adj_mat = sc->adj_mat;
+        prm.adj_img = adj_img;
+        prm.adj_grad = adj_grad;
+    }
     const uint64_t total_paths = (uint64_t)prm.tiles_x * tiles_y * 256ull * spp;
     // The coincident-surface list only changes results where medium records
     // change: boundaries, dielectrics, priorities.
@@ -224,6 +245,7 @@ int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, floa
         }
         passes_total += pass;
         if ((rc = timed(4, [&] { return launch_resolve(prm, rb->ps, noise, stream); }))) return rc;
+        if (prm.adj_mat && (rc = timed(7, [&] { return launch_adjoint(sc->dev, prm, rb->ps, stream); }))) return rc;
     }
     if (timing) YM_HIP(hipEventRecord(frame_b, stream));
     if (stats) {
@@ -245,8 +267,8 @@ int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, floa
         stats->passes = passes_total;
         stats->chunks = chunks;
         if (timing) {
-            float ms[7] = {0, 0, 0, 0, 0, 0, 0};
-            uint32_t launches[7] = {0, 0, 0, 0, 0, 0, 0};
+            float ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            uint32_t launches[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             for (const Mark &m : marks) {
                 float t = 0.0f;
                 YM_HIP(hipEventElapsedTime(&t, m.a, m.b));
@@ -262,6 +284,7 @@ int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, floa
             stats->launches_medium = launches[5];
             stats->ms_transmit = ms[6];
             stats->launches_transmit = launches[6];
+            stats->ms_adjoint = ms[7];
             stats->launches_extend = launches[1];
             stats->launches_shade = launches[2];
             stats->launches_shadow = launches[3];

@@ -605,6 +605,22 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
     if ((rc = upload(sc, h.instances, &ds.instances))) return rc;
     if ((rc = upload(sc, h.shade, &ds.shade))) return rc;
     if ((rc = upload(sc, h.materials, &ds.materials))) return rc;
+    {
+        // Lambert-only DefaultMtl (the adjoint's differentiable materials):
+        // its one lobe is the untextured albedo, so sampling and MIS do not
+        // depend on it
+        std::vector<uint32_t> lam(h.materials.size());
+        for (size_t i = 0; i < h.materials.size(); ++i) {
+            const DevMaterial &m = h.materials[i];
+            lam[i] = m.type == YMRT_MATERIAL_DEFAULT && m.specular_weight == 0.0f && m.metalness == 0.0f &&
+                     m.transmission_weight == 0.0f && m.coat_weight == 0.0f && m.diffuse_albedo_tex == kEmpty &&
+                     m.specular_weight_tex == kEmpty && m.metalness_tex == kEmpty &&
+                     m.transmission_weight_tex == kEmpty && m.coat_weight_tex == kEmpty;
+        }
+        const uint32_t *d = nullptr;
+        if ((rc = upload(sc, lam, &d))) return rc;
+        sc->adj_mat = d;
+    }
     if ((rc = upload(sc, texs, &ds.textures))) return rc;
     if ((rc = upload(sc, tex_prog, &ds.tex_prog))) return rc;
     if ((rc = upload(sc, texels, &ds.texels))) return rc;
