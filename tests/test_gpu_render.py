@@ -108,7 +108,7 @@ def test_interior_fog_parity(gpu_ctx):
     assert stats.transmittance_rays > 0
     # every fog segment runs expf/logf, whose last bits differ between the GPU
     # and glibc, so fewer pixels agree to 1e-4 while the image error stays low
-    assert same >= 0.70
+    assert same >= 0.80
     assert err <= 0.01
 
 
@@ -121,7 +121,7 @@ def test_cornell_medium_parity(gpu_ctx, grid):
     g, o, stats = render_both(gpu_ctx, sa, rs)
     assert np.isfinite(g).all()
     same, err = _media_report("cornell medium grid=%d" % grid, g, o, stats)
-    assert same >= 0.50
+    assert same >= 0.95
     assert err <= 0.02
 
 
@@ -152,7 +152,10 @@ def test_outdoor_distant_light_parity(gpu_ctx, fog):
     assert np.isfinite(g).all()
     same, err = close_frac(g, o), rel_rmse(g, o)
     print("outdoor fog=%d: close pixels %.4f  rel rmse %.5f" % (fog, same, err))
-    assert same >= (0.70 if fog else 0.90)
+    # the ground plane lies at y = 0, where the spawn offset is zero: shadow
+    # rays that graze it hit or miss it on their last bits, so the fog case
+    # (expf/logf last bits as above) agrees on fewer pixels
+    assert same >= (0.90 if fog else 0.95)
     assert err <= 0.01
 
 

@@ -111,6 +111,96 @@ __device__ __forceinline__ void wave_sort_lds(uint32_t *s, uint32_t n)
     }
 }
 
+// Output queues of a lane-refill kernel: survivors are staged in LDS as lanes
+// finish and published kStageCap at a time (sorted by slot, one atomic per
+// queue).  filled[] is wave-uniform, so every call sits in uniform control flow.
+constexpr uint32_t kStageCap = kSub * kBlock;
+template <int kQ>
+struct StagedQueues {
+    static constexpr int kN = kQ > 0 ? kQ : 1;
+    uint32_t *stage;  // LDS [kN][kStageCap]
+    uint32_t *const *ctrs;
+    uint32_t *const *qs;
+    uint32_t filled[kN];
+
+    template <class QC>
+    __device__ __forceinline__ void flush(QC)
+    {
+        constexpr int Q = QC::value;
+        __syncthreads();
+        if (filled[Q]) {
+            wave_sort_lds<kStageCap>(stage + Q * kStageCap, filled[Q]);
+            uint32_t dst = 0;
+            if (lane_id() == 0) dst = atomicAdd(ctrs[Q], filled[Q]);
+            dst = __shfl(dst, 0);
+            for (uint32_t j = lane_id(); j < filled[Q]; j += kBlock) qs[Q][dst + j] = stage[Q * kStageCap + j];
+            filled[Q] = 0;
+        }
+        __syncthreads();
+    }
+    __device__ __forceinline__ void append(const bool (&out)[kN], uint32_t p)
+    {
+        const uint64_t below = (1ull << lane_id()) - 1ull;
+        static_for<0, kQ>([&](auto Q) {
+            const uint64_t m = __ballot(out[Q()]);
+            if (out[Q()]) stage[Q() * kStageCap + filled[Q()] + (uint32_t)__popcll(m & below)] = p;
+            filled[Q()] += (uint32_t)__popcll(m);
+            if (filled[Q()] > kStageCap - kBlock) flush(Q);
+        });
+    }
+    __device__ __forceinline__ void flush_all()
+    {
+        static_for<0, kQ>([&](auto Q) { flush(Q); });
+    }
+};
+
+// Lane refill: a kernel that keeps one traversal per lane lets finished lanes
+// take new items while the others stay where they are in their trees, once
+// kRefillLanes lanes are done.  Ray lengths vary by orders of magnitude, so a
+// wave that waited for its slowest ray would run most of its steps with few
+// lanes active.  refill() hands the wave's idle lanes the next items of its
+// fetched range [cur, end), fetching kSub * 64 more as the range runs dry; it
+// returns true on the lanes that took item *item.
+constexpr int kRefillLanes = 16;
+constexpr int kLaneIdle = 3;  // no ray (beside kTravContinue/Hit/Miss/Overflow)
+struct RefillRange {
+    uint32_t *work;
+    uint32_t n;
+    uint32_t cur, end;
+    bool more;  // the work counter may hold items
+    __device__ __forceinline__ bool refill(bool idle_lane, uint32_t &item)
+    {
+        const uint64_t below = (1ull << lane_id()) - 1ull;
+        bool took = false;
+        uint64_t idle = __ballot(idle_lane);
+        while (idle && more) {
+            if (cur >= end) {
+                const uint32_t base = wave_fetch(work);
+                if (base >= n) {
+                    more = false;
+                    break;
+                }
+                cur = base;
+                end = base + (uint32_t)(kBlock * kSub) < n ? base + (uint32_t)(kBlock * kSub) : n;
+            }
+            const uint32_t take = min((uint32_t)__popcll(idle), end - cur);
+            if (idle_lane && !took) {
+                const uint32_t rank = (uint32_t)__popcll(idle & below);
+                if (rank < take) {
+                    item = cur + rank;
+                    took = true;
+                }
+            }
+            cur += take;
+            idle = __ballot(idle_lane && !took);
+        }
+        return took;
+    }
+    // lanes to wait for before the next refill: all of them once the counter
+    // is drained, since nothing could refill them
+    __device__ __forceinline__ uint32_t want() const { return more ? (uint32_t)kRefillLanes : (uint32_t)kBlock; }
+};
+
 __device__ __forceinline__ void wave_add64(unsigned long long *counter, uint32_t v)
 {
     // sum v over the wave, one atomic
@@ -276,15 +366,8 @@ __global__ __launch_bounds__(kBlock) void k_raygen(DevScene sc, PtParams prm, Pa
 // hand their hit or miss to k_medium, which samples free flight; the others
 // resolve misses here and queue surface hits for shading.  Visited wide nodes
 // and tested triangles are summed per wave into the frame statistics.
-//
-// Lane refill: each lane keeps its own traversal; once kRefillLanes lanes of
-// the wave have finished, their results are written and they start new rays
-// from the wave's fetched range while the others stay where they are in their
-// trees.  Ray lengths vary by orders of magnitude, so a wave that waited for
-// its slowest ray would run most of its steps with few lanes active.
-constexpr int kRefillLanes = 16;
-constexpr int kLaneIdle = 3;  // no ray (beside kTravContinue/Hit/Miss/Overflow)
-constexpr uint32_t kStageCap = kSub * kBlock;
+// Each lane keeps its own traversal and is refilled as it finishes
+// (RefillRange).
 
 template <bool kNearby, bool kMedia, bool kSphere, int kStack>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere ? 1 : 4))) void k_extend(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
@@ -297,63 +380,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere 
     __shared__ uint32_t stage[kQ * kStageCap];
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     const uint32_t *qin = ps.queue[pass & 1];
-    const uint32_t n = cnt[kCtrActive];
     LaneScratch<kBlock, kStackWindow> ls{stack, nbp, nbt, nbb, (int)lane_id(), ps.spill, kSpillThreads,
                                          blockIdx.x * kBlock + threadIdx.x};
     const ymrt_render_settings &rs = prm.rs;
-    const uint64_t below = (1ull << lane_id()) - 1ull;
     uint32_t rays = 0, cam_rays = 0;
     TraceCounters tc{0, 0};
     uint32_t *const ctrs[2] = {cnt + kCtrHit, cnt + kCtrMedium};
     uint32_t *const qs[2] = {ps.q_hit, ps.q_medium};
-    uint32_t filled[kQ] = {};
-    // publish staged survivors of queue Q: one atomic, coalesced copy
-    auto flush = [&](auto Q) {
-        __syncthreads();
-        if (filled[Q()]) {
-            wave_sort_lds<kStageCap>(stage + Q() * kStageCap, filled[Q()]);
-            uint32_t dst = 0;
-            if (lane_id() == 0) dst = atomicAdd(ctrs[Q()], filled[Q()]);
-            dst = __shfl(dst, 0);
-            for (uint32_t j = lane_id(); j < filled[Q()]; j += kBlock)
-                qs[Q()][dst + j] = stage[Q() * kStageCap + j];
-            filled[Q()] = 0;
-        }
-        __syncthreads();
-    };
+    StagedQueues<kQ> sq{stage, ctrs, qs, {}};
+    RefillRange rr{cnt + kCtrWorkExtend, cnt[kCtrActive], 0u, 0u, true};
 
     Traversal<kModeAll, false, kNearby, true, kSphere, kStack> tv;
     uint32_t p = 0;
     int r = kLaneIdle;
-    uint32_t cur = 0, end = 0;  // the wave's fetched range [cur, end)
-    bool more = true;           // the work counter may hold items
     for (;;) {
-        // refill idle lanes from the fetched range, fetching as it runs dry
-        bool fresh = false;
-        uint64_t idle = __ballot(r == kLaneIdle);
-        while (idle && more) {
-            if (cur >= end) {
-                const uint32_t base = wave_fetch(cnt + kCtrWorkExtend);
-                if (base >= n) {
-                    more = false;
-                    break;
-                }
-                cur = base;
-                end = base + (uint32_t)(kBlock * kSub) < n ? base + (uint32_t)(kBlock * kSub) : n;
-            }
-            const uint32_t take = min((uint32_t)__popcll(idle), end - cur);
-            if (r == kLaneIdle) {
-                const uint32_t rank = (uint32_t)__popcll(idle & below);
-                if (rank < take) {
-                    p = qin[cur + rank];
-                    r = kTravContinue;
-                    fresh = true;
-                }
-            }
-            cur += take;
-            idle = __ballot(r == kLaneIdle);
-        }
-        if (fresh) {
+        uint32_t item = 0;
+        if (rr.more && rr.refill(r == kLaneIdle, item)) {
+            p = qin[item];
             const float4 o4 = ps.ray_o[p], d4 = ps.ray_d[p];
             const uint2 last = ps.last[p];
             Query q;
@@ -370,9 +413,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere 
             tv.nb_count = 0;
             r = tv.init(sc, q);
         }
-        // step until kRefillLanes lanes are done (all of them once the
-        // counter is drained: nothing could refill them)
-        const uint32_t want = more ? (uint32_t)kRefillLanes : (uint32_t)kBlock;
+        const uint32_t want = rr.want();
         for (;;) {
             const uint64_t busy = __ballot(r == kTravContinue);
             if (busy == 0 || (uint32_t)(kBlock - __popcll(busy)) >= want) break;
@@ -432,15 +473,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere 
                 ps.hit_b[p] = make_float4(h.b0, h.b1, h.b2, 0.0f);
             }
         }
-        static_for<0, kQ>([&](auto Q) {
-            const uint64_t m = __ballot(out[Q()]);
-            if (out[Q()]) stage[Q() * kStageCap + filled[Q()] + (uint32_t)__popcll(m & below)] = p;
-            filled[Q()] += (uint32_t)__popcll(m);
-            if (filled[Q()] > kStageCap - kBlock) flush(Q);
-        });
-        if (!more && __ballot(r != kLaneIdle) == 0) break;
+        sq.append(out, p);
+        if (!rr.more && __ballot(r != kLaneIdle) == 0) break;
     }
-    static_for<0, kQ>([&](auto Q) { flush(Q); });
+    sq.flush_all();
     wave_add64(prm.stats + kStatExtension, rays);
     wave_add64(prm.stats + kStatCamera, cam_rays);
     wave_add64(prm.stats + kStatExtendNodes, tc.nodes);
@@ -974,9 +1010,10 @@ __device__ __forceinline__ void shadow_finish(const PtParams &prm, const PathSta
 template <bool kMedia, bool kSphere, int kStack>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_shadow(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
 {
+    constexpr int kQ = kMedia ? 1 : 0;
     __shared__ uint32_t stack[kStackWindow * kBlock];
+    __shared__ uint32_t stage[(kQ > 0 ? kQ : 1) * kStageCap];
     uint32_t *cnt = prm.counters + pass * kCtrStride;
-    const uint32_t n = cnt[kCtrShadow];
     const uint32_t cap = prm.cap;
     LaneScratch<kBlock, kStackWindow> ls{stack, nullptr, nullptr, nullptr, (int)lane_id(), ps.spill, kSpillThreads,
                                          blockIdx.x * kBlock + threadIdx.x};
@@ -984,35 +1021,63 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     TraceCounters tc{0, 0};
     uint32_t *const ctrs[1] = {cnt + kCtrTransmit};
     uint32_t *const qs[1] = {ps.q_transmit};
-    run_items<kMedia ? 1 : 0>(cnt + kCtrWorkShadow, n, ctrs, qs, [&](uint32_t i, uint32_t &p, bool(&out)[1]) {
-        {
-            p = ps.q_shadow[i];
-            uint32_t fl = ps.sh_flags[p];
-#pragma unroll 1
-            for (int k = 0; k < kShadowSlots; ++k) {
-                if (!(fl & (1u << k))) continue;
-                const float4 o4 = ps.sh_o[(size_t)k * cap + p];
-                const float4 d4 = ps.sh_d[(size_t)k * cap + p];
-                Query q;
-                q.o = xyz(o4);
-                q.d = xyz(d4);
-                q.tmin = kTMin;
-                q.tmax = o4.w;
-                q.last_inst = kEmpty;
-                q.last_tri = kEmpty;
-                Hit h;
-                int nbc = 0;
-                const int r = traverse<kModeSkipBoundary, true, false, true, kSphere, kStack>(sc, q, h, nbc, ls, tc);
-                ++nsh;
-                if (r < 0) atomicOr(prm.error, 1u);
-                if (r == 0) fl |= kShClear0 << k;
-            }
-            const bool walk = kMedia && (fl & kShClearMask);
-            out[0] = walk;
-            if (walk) ps.sh_flags[p] = fl;
-            else shadow_finish(prm, ps, p, fl, sc.distant_count, false);
+    StagedQueues<kQ> sq{stage, ctrs, qs, {}};
+    RefillRange rr{cnt + kCtrWorkShadow, cnt[kCtrShadow], 0u, 0u, true};
+
+    // one lane walks its path's recorded rays (slots k) in order
+    Traversal<kModeSkipBoundary, true, false, true, kSphere, kStack> tv;
+    uint32_t p = 0, fl = 0;
+    int k = 0;
+    int r = kLaneIdle;
+    auto start_ray = [&]() {
+        const float4 o4 = ps.sh_o[(size_t)k * cap + p];
+        const float4 d4 = ps.sh_d[(size_t)k * cap + p];
+        Query q;
+        q.o = xyz(o4);
+        q.d = xyz(d4);
+        q.tmin = kTMin;
+        q.tmax = o4.w;
+        q.last_inst = kEmpty;
+        q.last_tri = kEmpty;
+        r = tv.init(sc, q);
+    };
+    for (;;) {
+        uint32_t item = 0;
+        if (rr.more && rr.refill(r == kLaneIdle, item)) {
+            p = ps.q_shadow[item];
+            fl = ps.sh_flags[p];
+            k = __builtin_ctz(fl & kShActiveMask);  // a queued path records at least one ray
+            start_ray();
         }
-    });
+        const uint32_t want = rr.want();
+        for (;;) {
+            const uint64_t busy = __ballot(r == kTravContinue);
+            if (busy == 0 || (uint32_t)(kBlock - __popcll(busy)) >= want) break;
+            if (r == kTravContinue) r = tv.step(sc, ls, tc);
+        }
+        bool out[1] = {false};
+        // finished rays: note the result, start the path's next ray or
+        // finish the path (a ray can end in init, hence the loop)
+        while (r != kTravContinue && r != kLaneIdle) {
+            ++nsh;
+            if (r == kTravOverflow) atomicOr(prm.error, 1u);
+            if (r == kTravMiss) fl |= kShClear0 << k;
+            const uint32_t rest = fl & kShActiveMask & ~((2u << k) - 1u);
+            if (rest) {
+                k = __builtin_ctz(rest);
+                start_ray();
+            } else {
+                r = kLaneIdle;
+                const bool walk = kMedia && (fl & kShClearMask);
+                out[0] = walk;
+                if (walk) ps.sh_flags[p] = fl;
+                else shadow_finish(prm, ps, p, fl, sc.distant_count, false);
+            }
+        }
+        if constexpr (kQ > 0) sq.append(out, p);
+        if (!rr.more && __ballot(r != kLaneIdle) == 0) break;
+    }
+    sq.flush_all();
     wave_add64(prm.stats + kStatShadow, nsh);
     wave_add64(prm.stats + kStatShadowNodes, tc.nodes);
     wave_add64(prm.stats + kStatShadowTris, tc.tris);
