@@ -289,6 +289,15 @@ typedef struct ymrt_scene_info {
 } ymrt_scene_info;
 int ymrt_scene_get_info(const ymrt_scene *scene, ymrt_scene_info *info);
 
+/* Replaces materials [first, first + count) of a built scene in HBM
+ * (SceneModule::UpdateMaterial, src/core/SceneModule.cpp:654-663, which the
+ * editor calls after changing a material).  Synchronous on stream.  As in
+ * the reference, the shape-light CDF and the camera's medium records keep
+ * the values the scene was built with; a material may not change its type
+ * (YMRT_ERR_INVALID_ARGUMENT). */
+int ymrt_scene_update_materials(ymrt_scene *scene, uint32_t first, uint32_t count,
+                                const ymrt_material_desc *materials, void *stream);
+
 /* Closest hit (TraceRay) for n device-resident rays. */
 int ymrt_trace_closest(ymrt_scene *scene, const ymrt_ray *rays, uint32_t n,
                        ymrt_hit *hits, void *stream);

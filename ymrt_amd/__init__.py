@@ -24,7 +24,7 @@ EXPORTS = [
     "ymrt_tonemap", "ymrt_halton_values", "ymrt_halton_enumerate", "ymrt_texture_eval",
     "ymrt_ymtmp_load", "ymrt_ymtmp_parse", "ymrt_ymtmp_scene", "ymrt_ymtmp_warning_count",
     "ymrt_ymtmp_warning", "ymrt_ymtmp_find", "ymrt_ymtmp_transform", "ymrt_ymtmp_free",
-    "ymrt_render_frame_aov", "ymrt_save_png", "ymrt_render_adjoint",
+    "ymrt_render_frame_aov", "ymrt_save_png", "ymrt_render_adjoint", "ymrt_scene_update_materials",
 ]
 
 _lib = None
@@ -64,6 +64,7 @@ def lib():
         L.ymrt_halton_values.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp]
         L.ymrt_halton_enumerate.argtypes = [C.c_uint32, C.c_uint32, vp, vp, vp, C.c_uint32, vp, vp]
         L.ymrt_texture_eval.argtypes = [vp, C.c_uint32, vp, C.c_uint32, vp, vp]
+        L.ymrt_scene_update_materials.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(abi.MaterialDesc), vp]
         L.ymrt_ymtmp_load.argtypes = [C.c_char_p, C.POINTER(vp)]
         L.ymrt_ymtmp_parse.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p, C.POINTER(vp)]
         L.ymrt_ymtmp_scene.argtypes = [vp]
@@ -145,6 +146,15 @@ class Scene:
         i = abi.SceneInfo()
         _check(lib().ymrt_scene_get_info(self.h, C.byref(i)))
         return i
+
+    def update_materials(self, first, materials):
+        """ymrt_scene_update_materials: materials (abi.MaterialDesc records)
+        replace [first, first + len(materials)); self.arrays follows."""
+        n = len(materials)
+        buf = (abi.MaterialDesc * max(1, n))(*materials)
+        _check(lib().ymrt_scene_update_materials(self.h, first, n, buf, _stream()))
+        for i, m in enumerate(materials):
+            self.arrays.materials[first + i] = m
 
     # ----------------------------------------------------------- ray queries
     @staticmethod

@@ -11,6 +11,8 @@ static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
 
 int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d);                       // scene.cpp
+int update_materials(ymrt_scene *sc, uint32_t first, uint32_t count, const ymrt_material_desc *m,
+                     hipStream_t s);  // scene.cpp
 int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, float *noise, uint32_t *instance_ids,
                  const float *adj_img, float *adj_grad, hipStream_t stream, int timing,
                  ymrt_frame_stats *stats);                                      // render.cpp
@@ -214,6 +216,13 @@ int ymrt_halton_values(ymrt_scene *sc, const uint64_t *indices, uint32_t n, uint
     if (rc) return rc;
     YM_HIP(hipStreamSynchronize((hipStream_t)stream));
     return YMRT_OK;
+}
+
+int ymrt_scene_update_materials(ymrt_scene *sc, uint32_t first, uint32_t count,
+                                const ymrt_material_desc *materials, void *stream)
+{
+    YM_REQUIRE(sc != nullptr, "scene is NULL");
+    return update_materials(sc, first, count, materials, (hipStream_t)stream);
 }
 
 int ymrt_texture_eval(ymrt_scene *sc, uint32_t texture, const float *coords, uint32_t n, float *out, void *stream)

@@ -50,6 +50,11 @@ struct ymrt_scene {
     ym::RenderBuffers *render = nullptr;
     uint32_t *d_error = nullptr;  // device flag: traversal stack overflow
     const uint32_t *adj_mat = nullptr;  // per material: 1 when Lambert-only (adjoint)
+    // host state that ymrt_scene_update_materials re-derives flags from
+    std::vector<ym::DevMaterial> materials;
+    uint32_t texture_count = 0;
+    bool has_boundaries = false;
+    bool procedural_textures = false;
 };
 
 namespace ym {

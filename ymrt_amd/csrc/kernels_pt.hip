@@ -160,9 +160,11 @@ struct StagedQueues {
 // wave that waited for its slowest ray would run most of its steps with few
 // lanes active.  refill() hands the wave's idle lanes the next items of its
 // fetched range [cur, end), fetching kSub * 64 more as the range runs dry; it
-// returns true on the lanes that took item *item.
-constexpr int kRefillLanes = 16;
+// returns true on the lanes that took item *item.  Refilling at 24 (extend)
+// and 16 (shadow) idle lanes measured fastest on the interior frame (8 and 32
+// were 1-3 % slower).
 constexpr int kLaneIdle = 3;  // no ray (beside kTravContinue/Hit/Miss/Overflow)
+template <int kRefillLanes>
 struct RefillRange {
     uint32_t *work;
     uint32_t n;
@@ -388,7 +390,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere 
     uint32_t *const ctrs[2] = {cnt + kCtrHit, cnt + kCtrMedium};
     uint32_t *const qs[2] = {ps.q_hit, ps.q_medium};
     StagedQueues<kQ> sq{stage, ctrs, qs, {}};
-    RefillRange rr{cnt + kCtrWorkExtend, cnt[kCtrActive], 0u, 0u, true};
+    RefillRange<24> rr{cnt + kCtrWorkExtend, cnt[kCtrActive], 0u, 0u, true};
 
     Traversal<kModeAll, false, kNearby, true, kSphere, kStack> tv;
     uint32_t p = 0;
@@ -1022,7 +1024,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     uint32_t *const ctrs[1] = {cnt + kCtrTransmit};
     uint32_t *const qs[1] = {ps.q_transmit};
     StagedQueues<kQ> sq{stage, ctrs, qs, {}};
-    RefillRange rr{cnt + kCtrWorkShadow, cnt[kCtrShadow], 0u, 0u, true};
+    RefillRange<16> rr{cnt + kCtrWorkShadow, cnt[kCtrShadow], 0u, 0u, true};
 
     // one lane walks its path's recorded rays (slots k) in order
     Traversal<kModeSkipBoundary, true, false, true, kSphere, kStack> tv;

@@ -62,6 +62,49 @@ struct Transfer {
     }
 };
 
+// Closest-hit traversals keep the nearby-hit list when medium boundaries,
+// dielectrics or IOR priorities can change a path's medium records.
+bool needs_nearby(bool has_boundaries, const std::vector<DevMaterial> &materials)
+{
+    bool r = has_boundaries;
+    for (const DevMaterial &m : materials)
+        if (m.type == YMRT_MATERIAL_DEFAULT &&
+            (m.transmission_weight > 0.0f || m.transmission_weight_tex != kEmpty || m.ior_priority != 0))
+            r = true;
+    return r;
+}
+
+// Lambert-only DefaultMtl (the adjoint's differentiable materials): its one
+// lobe is the untextured albedo, so sampling and MIS do not depend on it.
+uint32_t lambert_only(const DevMaterial &m)
+{
+    return m.type == YMRT_MATERIAL_DEFAULT && m.specular_weight == 0.0f && m.metalness == 0.0f &&
+           m.transmission_weight == 0.0f && m.coat_weight == 0.0f && m.diffuse_albedo_tex == kEmpty &&
+           m.specular_weight_tex == kEmpty && m.metalness_tex == kEmpty && m.transmission_weight_tex == kEmpty &&
+           m.coat_weight_tex == kEmpty;
+}
+
+// Texture slots of a material, all of which must name a texture or be empty.
+bool material_textures_valid(const DevMaterial &m, uint32_t texture_count)
+{
+    const uint32_t ids[] = {m.diffuse_albedo_tex, m.specular_albedo_tex, m.roughness_x_tex, m.roughness_y_tex,
+                            m.specular_weight_tex, m.metalness_tex, m.transmission_weight_tex,
+                            m.coat_albedo_tex, m.coat_weight_tex, m.coat_thickness_tex,
+                            m.coat_roughness_x_tex, m.coat_roughness_y_tex, m.normal_mapping_tex,
+                            m.bump_mapping_tex};
+    for (uint32_t t : ids)
+        if (t != kEmpty && t >= texture_count) return false;
+    return true;
+}
+
+bool uses_texture_graph(bool procedural_textures, const std::vector<DevMaterial> &materials)
+{
+    bool r = procedural_textures;
+    for (const DevMaterial &m : materials)
+        if (m.normal_mapping_tex != kEmpty || m.bump_mapping_tex != kEmpty) r = true;
+    return r;
+}
+
 float fetch_ior(const DevMaterial &m, uint32_t *prio)
 {
     if (m.type == YMRT_MATERIAL_DEFAULT) {
@@ -356,11 +399,8 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
     const uint32_t tlas_root_solid = build_tlas(1);
     const uint32_t tlas_root_boundary = build_tlas(2);
     sc->max_stack = 3 * max_blas_depth + 3 * tlas_depth + 2;
-    sc->needs_nearby = has_boundaries != 0;
-    for (const DevMaterial &m : h.materials)
-        if (m.type == YMRT_MATERIAL_DEFAULT &&
-            (m.transmission_weight > 0.0f || m.transmission_weight_tex != kEmpty || m.ior_priority != 0))
-            sc->needs_nearby = true;
+    sc->has_boundaries = has_boundaries != 0;
+    sc->needs_nearby = needs_nearby(sc->has_boundaries, h.materials);
     sc->info.triangle_count = total_tris;
     sc->info.node_count = h.nodes.size();
 
@@ -461,18 +501,12 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
         texs[i].prog_start = start;
         texs[i].prog_len = (uint32_t)tex_prog.size() - start;
     }
-    for (uint32_t i = 0; i < d->material_count; ++i) {
-        const DevMaterial &m = h.materials[i];
-        const uint32_t ids[] = {m.diffuse_albedo_tex, m.specular_albedo_tex, m.roughness_x_tex, m.roughness_y_tex,
-                                m.specular_weight_tex, m.metalness_tex, m.transmission_weight_tex,
-                                m.coat_albedo_tex, m.coat_weight_tex, m.coat_thickness_tex,
-                                m.coat_roughness_x_tex, m.coat_roughness_y_tex, m.normal_mapping_tex,
-                                m.bump_mapping_tex};
-        for (uint32_t t : ids) YM_REQUIRE(t == kEmpty || t < d->texture_count, "material texture index out of range");
-        if (m.normal_mapping_tex != kEmpty || m.bump_mapping_tex != kEmpty) ds.has_graph = 1;
-    }
+    for (uint32_t i = 0; i < d->material_count; ++i)
+        YM_REQUIRE(material_textures_valid(h.materials[i], d->texture_count), "material texture index out of range");
+    sc->procedural_textures = false;
     for (const DevTexture &t : texs)
-        if (t.type >= YMRT_TEXTURE_CHECKERBOARD) ds.has_graph = 1;
+        if (t.type >= YMRT_TEXTURE_CHECKERBOARD) sc->procedural_textures = true;
+    ds.has_graph = uses_texture_graph(sc->procedural_textures, h.materials) ? 1 : 0;
     std::vector<float> lut(512);
     for (int i = 0; i < 256; ++i) {
         float v = (float)i * (1.0f / 255.0f);
@@ -606,21 +640,14 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
     if ((rc = upload(sc, h.shade, &ds.shade))) return rc;
     if ((rc = upload(sc, h.materials, &ds.materials))) return rc;
     {
-        // Lambert-only DefaultMtl (the adjoint's differentiable materials):
-        // its one lobe is the untextured albedo, so sampling and MIS do not
-        // depend on it
         std::vector<uint32_t> lam(h.materials.size());
-        for (size_t i = 0; i < h.materials.size(); ++i) {
-            const DevMaterial &m = h.materials[i];
-            lam[i] = m.type == YMRT_MATERIAL_DEFAULT && m.specular_weight == 0.0f && m.metalness == 0.0f &&
-                     m.transmission_weight == 0.0f && m.coat_weight == 0.0f && m.diffuse_albedo_tex == kEmpty &&
-                     m.specular_weight_tex == kEmpty && m.metalness_tex == kEmpty &&
-                     m.transmission_weight_tex == kEmpty && m.coat_weight_tex == kEmpty;
-        }
+        for (size_t i = 0; i < h.materials.size(); ++i) lam[i] = lambert_only(h.materials[i]);
         const uint32_t *d = nullptr;
         if ((rc = upload(sc, lam, &d))) return rc;
         sc->adj_mat = d;
     }
+    sc->materials = h.materials;
+    sc->texture_count = d->texture_count;
     if ((rc = upload(sc, texs, &ds.textures))) return rc;
     if ((rc = upload(sc, tex_prog, &ds.tex_prog))) return rc;
     if ((rc = upload(sc, texels, &ds.texels))) return rc;
@@ -655,6 +682,38 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
     auto t1 = std::chrono::high_resolution_clock::now();
     sc->info.build_ms = (float)std::chrono::duration<double, std::milli>(t1 - t0).count();
     sc->info.max_stack = sc->max_stack;
+    return YMRT_OK;
+}
+
+// SceneModule::UpdateMaterial (src/core/SceneModule.cpp:654-663) for a range:
+// the new records replace the old ones in HBM, and the flags derived from
+// materials (nearby-hit list, texture-graph shading, Lambert-only) follow.
+// Like the reference, the shape-light CDF and the camera's medium records
+// keep what the scene was built with, so a material keeps its type here.
+int update_materials(ymrt_scene *sc, uint32_t first, uint32_t count, const ymrt_material_desc *m, hipStream_t s)
+{
+    YM_REQUIRE(count == 0 || m != nullptr, "materials is NULL");
+    YM_REQUIRE(first <= sc->materials.size() && count <= sc->materials.size() - first,
+               "material range out of bounds");
+    if (count == 0) return YMRT_OK;
+    std::vector<DevMaterial> next = sc->materials;
+    for (uint32_t i = 0; i < count; ++i) {
+        DevMaterial dm;
+        std::memcpy(&dm, &m[i], sizeof(dm));
+        YM_REQUIRE(dm.type == next[first + i].type, "a material update cannot change the material type");
+        YM_REQUIRE(material_textures_valid(dm, sc->texture_count), "material texture index out of range");
+        next[first + i] = dm;
+    }
+    std::vector<uint32_t> lam(count);
+    for (uint32_t i = 0; i < count; ++i) lam[i] = lambert_only(next[first + i]);
+    YM_HIP(hipMemcpyAsync(const_cast<DevMaterial *>(sc->dev.materials) + first, next.data() + first,
+                          sizeof(DevMaterial) * count, hipMemcpyHostToDevice, s));
+    YM_HIP(hipMemcpyAsync(const_cast<uint32_t *>(sc->adj_mat) + first, lam.data(), sizeof(uint32_t) * count,
+                          hipMemcpyHostToDevice, s));
+    YM_HIP(hipStreamSynchronize(s));
+    sc->materials.swap(next);
+    sc->needs_nearby = needs_nearby(sc->has_boundaries, sc->materials);
+    sc->dev.has_graph = uses_texture_graph(sc->procedural_textures, sc->materials) ? 1 : 0;
     return YMRT_OK;
 }
 
