@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define YMRT_ABI_VERSION 3
+#define YMRT_ABI_VERSION 4
 
 /* status codes */
 #define YMRT_OK 0
