@@ -234,12 +234,18 @@ struct TraceCounters {
     uint32_t nodes, tris;
 };
 
+// compare-exchange as selects (no branch, no exec-mask changes)
 #define YM_SORT2(ka, va, kb, vb)                      \
     do {                                              \
-        if (kb < ka) {                                \
-            float tk = ka; ka = kb; kb = tk;          \
-            uint32_t tv = va; va = vb; vb = tv;       \
-        }                                             \
+        const bool sw_ = kb < ka;                     \
+        const float lo_ = sw_ ? kb : ka;              \
+        const float hi_ = sw_ ? ka : kb;              \
+        const uint32_t vlo_ = sw_ ? vb : va;          \
+        const uint32_t vhi_ = sw_ ? va : vb;          \
+        ka = lo_;                                     \
+        kb = hi_;                                     \
+        va = vlo_;                                    \
+        vb = vhi_;                                    \
     } while (0)
 
 // Resumable traversal: init() then step() until it returns something other
@@ -346,7 +352,7 @@ struct Traversal {
         float tn = fmaxf(fmaxf(fminf(a0, a1), fminf(b0, b1)), fminf(c0, c1));   \
         float tf = fminf(fminf(fmaxf(a0, a1), fmaxf(b0, b1)), fmaxf(c0, c1));   \
         tf *= kRobust;                                                          \
-        bool h = (C != kSentinel) && (tf > tn) && (tf > 0.0f) && (tn <= tbest); \
+        bool h = (C != kSentinel) & (tf > tn) & (tf > 0.0f) & (tn <= tbest);    \
         K = h ? (tn < 0.0f ? 0.0f : tn) : INFINITY;                             \
     }
             YM_SLAB(k0, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ch.x)
