@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
+    ap.add_argument("--no-batch", action="store_true", help="skip the multi-scene batch timing")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before rendering, copy MIB MiB device-to-device once (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -90,6 +91,48 @@ def adjoint_timing(ctx, frames=2):
     return {"workload": "Cornell box 512x512 16spp depth 6, gradients of 3 Lambert albedos (rgb)",
             "forward_ms": round(t["forward"], 3), "adjoint_ms": round(t["adjoint"], 3),
             "adjoint_over_forward": round(t["adjoint"] / t["forward"], 3)}
+
+
+def batch_timing(ctx, count=16, seed=11):
+    """A batch of small scenes (the multi-scene workload at reduced size):
+    procedural interiors with 1-40 instances (6.5 k-260 k triangles,
+    log-uniform), 256^2 or 512^2, 4 or 16 spp, depth 6, fog on half of them.
+    The same frames are rendered one after another with ymrt_render_frame
+    and in one ymrt_render_batch call; wall times include every frame."""
+    import time
+
+    import numpy as np
+    import torch
+
+    import ymrt_amd
+    from ymrt_amd import abi, scenes
+    rng = np.random.default_rng(seed)
+    jobs, tris = [], 0
+    for i in range(count):
+        inst = int(round(np.exp(rng.uniform(0.0, np.log(40.0)))))
+        res = int(rng.choice([256, 512]))
+        spp = int(rng.choice([4, 16]))
+        fog = bool(rng.integers(0, 2))
+        sa = scenes.interior(res, res, tex_size=256, instances=inst, halton_seed=i, fog=fog)
+        sc = ymrt_amd.Scene(ctx, sa)
+        tris += sc.info().triangle_count
+        jobs.append((sc, abi.render_settings(res, res, spp, 6, volume=int(fog)), 1))
+    ctx.render_batch(jobs)  # warm-up: path-state buffers of every scene
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for sc, rs, frame in jobs:
+        sc.render_frame(rs, frame + 1)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    out = ctx.render_batch([(sc, rs, frame + 1) for sc, rs, frame in jobs])
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    rays = sum(st.camera_rays + st.extension_rays + st.shadow_rays + st.transmittance_rays for _, st in out)
+    samples = sum(rs.width * rs.height * rs.spp for _, rs, _ in jobs)
+    return {"workload": "%d interiors, 256^2/512^2, 4/16 spp, depth 6, %d triangles in all" % (count, tris),
+            "sequential_ms": round((t1 - t0) * 1e3, 2), "batch_ms": round((t2 - t1) * 1e3, 2),
+            "speedup": round((t1 - t0) / (t2 - t1), 3), "batch_mrays_per_s": round(rays / (t2 - t1) / 1e6, 2),
+            "batch_samples_per_s": round(samples / (t2 - t1), 1)}
 
 
 def cpu_baseline(sa, rs_full, budget_s):
@@ -264,6 +307,8 @@ def main():
     }
     if not args.no_adjoint:
         out["adjoint"] = adjoint_timing(ctx)
+    if not args.no_batch:
+        out["batch"] = batch_timing(ctx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sa, rs, args.cpu_seconds)
     print(json.dumps(out), flush=True)

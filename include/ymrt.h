@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define YMRT_ABI_VERSION 4
+#define YMRT_ABI_VERSION 5
 
 /* status codes */
 #define YMRT_OK 0
@@ -346,6 +346,27 @@ int ymrt_render_adjoint(ymrt_scene *scene, const ymrt_render_settings *settings,
                         int frame_index, float *noise_rgba, const float *adjoint_rgb,
                         float *grad_albedo, void *stream, int timing,
                         ymrt_frame_stats *stats);
+
+/* One frame of one scene in a batch. */
+typedef struct ymrt_batch_item {
+    ymrt_scene *scene;
+    const ymrt_render_settings *settings;
+    int frame_index;
+    int timing;                      /* as ymrt_render_frame                  */
+    float *noise_rgba;               /* device, width*height float4           */
+    ymrt_frame_stats *stats;         /* may be NULL                           */
+} ymrt_batch_item;
+
+/* Renders every item as ymrt_render_frame would, bit for bit, with up to
+ * max_streams host threads (0: 4), each driving a HIP stream of its own, so
+ * the frames of small scenes overlap on the device.  The frames of one scene
+ * run in item order on one stream; scenes go out longest first (pixels x
+ * spp x depth x log2 triangles).  YMRT drives its editor and path-tracing
+ * views the same way, one render thread and stream each
+ * (src/core/RenderModule.cpp:55-66, 95-143).  Waits for `stream` first and
+ * returns when every frame is done; on failure the message names the item. */
+int ymrt_render_batch(ymrt_context *ctx, const ymrt_batch_item *items, uint32_t n,
+                      uint32_t max_streams, void *stream);
 
 /* 8-bit RGBA PNG of a host RGBA32F image as SavePngImage writes it
  * (src/FileIO.cpp:167-194): the extension becomes ".png", parent folders

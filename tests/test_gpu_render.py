@@ -311,3 +311,29 @@ def test_autograd_albedo_matches_finite_differences(gpu_ctx):
     print("autograd", g.tolist(), "fd", fd.tolist())
     assert np.all(fd > 0)
     assert np.allclose(g, fd, rtol=5e-3), (g, fd)
+
+
+def test_render_batch_equals_single_frames(gpu_ctx):
+    """ymrt_render_batch: frames of several scenes (one scene twice, with two
+    frame indices) on several streams equal ymrt_render_frame's bit for bit;
+    a bad item fails the call with its index in the message."""
+    jobs = []
+    cb = ymrt_amd.Scene(gpu_ctx, scenes.cornell_box(40, 32))
+    od = ymrt_amd.Scene(gpu_ctx, scenes.outdoor(48, 32))
+    tb = ymrt_amd.Scene(gpu_ctx, scenes.textured_box(32, 32))
+    it = ymrt_amd.Scene(gpu_ctx, scenes.interior(64, 40, tex_size=64, instances=6))
+    jobs = [(cb, abi.render_settings(40, 32, 4, 5), 1),
+            (od, abi.render_settings(48, 32, 4, 5, distant_light=1), 1),
+            (tb, abi.render_settings(32, 32, 2, 4), 3),
+            (it, abi.render_settings(64, 40, 4, 6, volume=1), 1),
+            (cb, abi.render_settings(40, 32, 4, 5), 2)]
+    out = gpu_ctx.render_batch(jobs, max_streams=3)
+    for (sc, rs, frame), (img, st) in zip(jobs, out):
+        ref, ref_st = sc.render_frame(rs, frame)
+        assert torch.equal(img, ref)
+        assert st.extension_rays == ref_st.extension_rays > 0
+    assert not torch.equal(out[0][0], out[4][0])
+    bad = list(jobs)
+    bad[2] = (tb, abi.render_settings(32, 32, 2, 4), 0)
+    with pytest.raises(ymrt_amd.YmrtError, match="batch item 2"):
+        gpu_ctx.render_batch(bad)

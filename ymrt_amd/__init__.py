@@ -25,6 +25,7 @@ EXPORTS = [
     "ymrt_ymtmp_load", "ymrt_ymtmp_parse", "ymrt_ymtmp_scene", "ymrt_ymtmp_warning_count",
     "ymrt_ymtmp_warning", "ymrt_ymtmp_find", "ymrt_ymtmp_transform", "ymrt_ymtmp_free",
     "ymrt_render_frame_aov", "ymrt_save_png", "ymrt_render_adjoint", "ymrt_scene_update_materials",
+    "ymrt_render_batch",
 ]
 
 _lib = None
@@ -65,6 +66,7 @@ def lib():
         L.ymrt_halton_enumerate.argtypes = [C.c_uint32, C.c_uint32, vp, vp, vp, C.c_uint32, vp, vp]
         L.ymrt_texture_eval.argtypes = [vp, C.c_uint32, vp, C.c_uint32, vp, vp]
         L.ymrt_scene_update_materials.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(abi.MaterialDesc), vp]
+        L.ymrt_render_batch.argtypes = [vp, C.POINTER(abi.BatchItem), C.c_uint32, C.c_uint32, vp]
         L.ymrt_ymtmp_load.argtypes = [C.c_char_p, C.POINTER(vp)]
         L.ymrt_ymtmp_parse.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p, C.POINTER(vp)]
         L.ymrt_ymtmp_scene.argtypes = [vp]
@@ -119,6 +121,27 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+    def render_batch(self, jobs, max_streams=0):
+        """ymrt_render_batch over jobs = [(scene, settings, frame), ...]:
+        returns one (H, W, 4) cuda noise image and FrameStats per job."""
+        import torch
+        n = len(jobs)
+        items = (abi.BatchItem * max(1, n))()
+        keep = []
+        for i, (scene, settings, frame) in enumerate(jobs):
+            img = torch.empty((settings.height, settings.width, 4), dtype=torch.float32, device="cuda")
+            st = abi.FrameStats()
+            keep.append((img, st, settings))
+            items[i].scene = scene.h.value
+            items[i].settings = C.pointer(settings)
+            items[i].frame_index = frame
+            items[i].timing = 0
+            items[i].noise_rgba = img.data_ptr()
+            items[i].stats = C.pointer(st)
+        _check(lib().ymrt_render_batch(self.h, items, n, max_streams, _stream()))
+        return [(img, st) for img, st, _ in keep]
 
 
 class Scene:

@@ -166,6 +166,14 @@ class FrameStats(C.Structure):
     ]
 
 
+class BatchItem(C.Structure):
+    """ymrt_batch_item: one frame of one scene for ymrt_render_batch."""
+    _fields_ = [
+        ("scene", C.c_void_p), ("settings", C.POINTER(RenderSettings)), ("frame_index", C.c_int),
+        ("timing", C.c_int), ("noise_rgba", C.c_void_p), ("stats", C.POINTER(FrameStats)),
+    ]
+
+
 class SceneInfo(C.Structure):
     _fields_ = [
         ("triangle_count", C.c_uint64), ("node_count", C.c_uint64),

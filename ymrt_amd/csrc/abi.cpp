@@ -13,6 +13,8 @@ void set_error(const std::string &msg) { g_last_error = msg; }
 int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d);                       // scene.cpp
 int update_materials(ymrt_scene *sc, uint32_t first, uint32_t count, const ymrt_material_desc *m,
                      hipStream_t s);  // scene.cpp
+int render_batch(ymrt_context *ctx, const ymrt_batch_item *items, uint32_t n, uint32_t max_streams,
+                 hipStream_t stream);  // batch.cpp
 int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, float *noise, uint32_t *instance_ids,
                  const float *adj_img, float *adj_grad, hipStream_t stream, int timing,
                  ymrt_frame_stats *stats);                                      // render.cpp
@@ -216,6 +218,12 @@ int ymrt_halton_values(ymrt_scene *sc, const uint64_t *indices, uint32_t n, uint
     if (rc) return rc;
     YM_HIP(hipStreamSynchronize((hipStream_t)stream));
     return YMRT_OK;
+}
+
+int ymrt_render_batch(ymrt_context *ctx, const ymrt_batch_item *items, uint32_t n, uint32_t max_streams,
+                      void *stream)
+{
+    return render_batch(ctx, items, n, max_streams, (hipStream_t)stream);
 }
 
 int ymrt_scene_update_materials(ymrt_scene *sc, uint32_t first, uint32_t count,

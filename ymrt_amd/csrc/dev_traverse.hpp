@@ -204,7 +204,17 @@ struct LaneScratch {
     uint32_t *spill;        // global [kMaxSpill][spill_stride], unused when kWindow == 0
     uint32_t spill_stride, spill_index;
 
+    static constexpr int kWindowSize = kWindow;
     YM_HD int nbi(int i) const { return i * kBlock + lane; }
+    // Three entries at depths sp, sp+1, sp+2 with no spill bookkeeping: only
+    // for sp + 3 <= kWindow (or the full stack when kWindow == 0), where those
+    // depths are free slots of their own.
+    YM_HD void put3(int sp, uint32_t a, uint32_t b, uint32_t c)
+    {
+        stack[sp * kBlock + lane] = a;
+        stack[(sp + 1) * kBlock + lane] = b;
+        stack[(sp + 2) * kBlock + lane] = c;
+    }
     YM_HD void push(int sp, uint32_t v)
     {
         if constexpr (kWindow == 0) {
@@ -367,11 +377,21 @@ struct Traversal {
             YM_SORT2(k1, v1, k3, v3);
             YM_SORT2(k1, v1, k2, v2);
             if (k0 != INFINITY) {
-                int need = (k1 != INFINITY) + (k2 != INFINITY) + (k3 != INFINITY);
+                const int need = (k1 != INFINITY) + (k2 != INFINITY) + (k3 != INFINITY);
                 if (sp + need > kStack) return kTravOverflow;
-                if (k3 != INFINITY) ls.push(sp++, v3);
-                if (k2 != INFINITY) ls.push(sp++, v2);
-                if (k1 != INFINITY) ls.push(sp++, v1);
+                constexpr int kFast = LS::kWindowSize > 0 ? LS::kWindowSize : kStack;
+                if (sp + 3 <= kFast) {
+                    // missed children sort last: the hit ones go far to near,
+                    // the slots past them are written but stay above the top
+                    const uint32_t s0 = need == 3 ? v3 : (need == 2 ? v2 : v1);
+                    const uint32_t s1 = need == 3 ? v2 : v1;
+                    ls.put3(sp, s0, s1, v1);
+                    sp += need;
+                } else {
+                    if (k3 != INFINITY) ls.push(sp++, v3);
+                    if (k2 != INFINITY) ls.push(sp++, v2);
+                    if (k1 != INFINITY) ls.push(sp++, v1);
+                }
                 node = v0;
                 return kTravContinue;
             }

@@ -755,8 +755,16 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                 const uint32_t mtype = m.type;
                 uint32_t prio;
                 fetch_ior(m, &prio);
-                const Transfer t = load_transfer(ps, p, cap);
-                const bool pass_through = (t.count != 0 && prio < t.get(t.max_idx).priority) || (iflags & kInstBoundary);
+                // the medium records give the pass-through test and the two
+                // IORs; nothing later needs the records themselves
+                float ray_ior, ex_ior;
+                bool pass_through;
+                {
+                    const Transfer t = load_transfer(ps, p, cap);
+                    pass_through = (t.count != 0 && prio < t.get(t.max_idx).priority) || (iflags & kInstBoundary);
+                    ray_ior = t.ray_ior();
+                    ex_ior = t.ex_ior(inst);
+                }
                 if (pass_through || mtype == YMRT_MATERIAL_LIGHT) {
                     pos = xf_position(in.o2w, pos, &perr);
                     ng = normalize(xf_transpose_vector(in.w2o, ng));
@@ -831,7 +839,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, Pat
                     Bsdf bs;
                     init_shading_space(bs, tx, normal_tex, bump_tex, back, ray_d, ng, sn, dpdu, dpdv, in.w2o, duv);
                     bs.mask = 0;
-                    if (is_default) init_default_bsdf(tx, bs, m, ray_d, back, t.ray_ior(), t.ex_ior(inst));
+                    if (is_default) init_default_bsdf(tx, bs, m, ray_d, back, ray_ior, ex_ior);
                     if (prm.adj_mat && prm.adj_mat[in.material]) {
                         // every later term of the path carries this vertex's albedo once
                         const uint32_t k = ps.adj_n[p];
