@@ -324,6 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(DevScene sc, PtParams prm, Pa
     uint32_t *const qs[1] = {ps.queue[0]};
     run_items<1>(cnt + kCtrWorkRaygen, prm.chunk_paths, ctrs, qs, [&](uint32_t p, uint32_t &slot, bool(&out)[1]) {
         slot = p;
+        ps.flag[p] = 0;
         {
             const uint64_t gp = prm.chunk_base + p;
             const uint32_t spp = (uint32_t)prm.rs.spp;
@@ -390,12 +391,10 @@ __global__ __launch_bounds__(kBlock) void k_raygen(DevScene sc, PtParams prm, Pa
 template <bool kNearby, bool kMedia, bool kSphere, int kStack>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere ? 1 : 4))) void k_extend(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
 {
-    constexpr int kQ = kMedia ? 2 : 1;
     __shared__ uint32_t stack[kStackWindow * kBlock];
     __shared__ uint32_t nbp[(kNearby ? kNearbyCap : 1) * kBlock];
     __shared__ float nbt[(kNearby ? kNearbyCap : 1) * kBlock];
     __shared__ int nbb[(kNearby ? kNearbyCap : 1) * kBlock];
-    __shared__ uint32_t stage[kQ * kStageCap];
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     const uint32_t *qin = ps.queue[pass & 1];
     LaneScratch<kBlock, kStackWindow> ls{stack, nbp, nbt, nbb, (int)lane_id(), ps.spill, kSpillThreads,
@@ -403,9 +402,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere 
     const ymrt_render_settings &rs = prm.rs;
     uint32_t rays = 0, cam_rays = 0;
     TraceCounters tc{0, 0};
-    uint32_t *const ctrs[2] = {cnt + kCtrHit, cnt + kCtrMedium};
-    uint32_t *const qs[2] = {ps.q_hit, ps.q_medium};
-    StagedQueues<kQ> sq{stage, ctrs, qs, {}};
     RefillRange<24> rr{cnt + kCtrWorkExtend, cnt[kCtrActive], 0u, 0u, true};
 
     Traversal<kModeAll, false, kNearby, true, kSphere, kStack> tv;
@@ -437,7 +433,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere 
             if (busy == 0 || (uint32_t)(kBlock - __popcll(busy)) >= want) break;
             wave_step(tv, r, sc, ls, tc);
         }
-        bool out[kQ] = {};
         if (r != kTravContinue && r != kLaneIdle) {
             if (r == kTravOverflow) atomicOr(prm.error, 1u);
             const bool hit = r == kTravHit;
@@ -464,8 +459,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere 
                     Lv = Lv + light_clamp(rs, xyz(ps.beta[p]) * background(tv.q.d), info_depth(info));
                 ps.out[p] = f4(Lv, 0.0f);
             } else {
-                out[0] = !in_medium;
-                if constexpr (kMedia) out[1] = in_medium;
+                ps.flag[p] = (pass << 2) | (in_medium ? kFlagMedium : kFlagHit);
                 uint32_t flags = h.back ? kHitBack : 0u;
                 bool listed = false;
                 const int nbc = tv.nb_count;
@@ -491,14 +485,127 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere 
                 ps.hit_b[p] = make_float4(h.b0, h.b1, h.b2, 0.0f);
             }
         }
-        sq.append(out, p);
         if (!rr.more && __ballot(r != kLaneIdle) == 0) break;
     }
-    sq.flush_all();
     wave_add64(prm.stats + kStatExtension, rays);
     wave_add64(prm.stats + kStatCamera, cam_rays);
     wave_add64(prm.stats + kStatExtendNodes, tc.nodes);
     wave_add64(prm.stats + kStatExtendTris, tc.tris);
+}
+
+// ------------------------------------------------------------ compaction
+// Gathers the paths flagged for the next stage of this pass (kShade: surface
+// and volume shading; otherwise free flight) in slot order: the traversal
+// kernels finish paths in no fixed order, and read in slot order the next
+// stage's lanes (and neighbouring waves) load neighbouring path-state
+// entries together.  A scan of every slot's flag (4 B) costs less than the
+// scattered reads it saves.  No atomics: k_compact_count sums the matches of
+// each block of kCompactSpan slots, k_compact_scatter writes a block's
+// matches at the sum of the counts before it, and the block holding the last
+// slot sets the queue lengths.
+template <bool kShade>
+__device__ __forceinline__ void compact_match(uint32_t f, uint32_t tag, bool (&m)[kShade ? 2 : 1])
+{
+    if constexpr (kShade) {
+        m[0] = f == (tag | kFlagHit);
+        m[1] = f == (tag | kFlagVolume);
+    } else {
+        m[0] = f == (tag | kFlagMedium);
+    }
+}
+
+template <bool kShade>
+__global__ __launch_bounds__(kCompactThreads) void k_compact_count(PtParams prm, PathState ps, uint32_t pass)
+{
+    constexpr int kQ = kShade ? 2 : 1;
+    constexpr uint32_t kWaves = kCompactThreads / kBlock;
+    __shared__ uint32_t red[kQ][kWaves];
+    const uint32_t n = prm.chunk_paths;
+    const uint32_t b = blockIdx.x, first = b * kCompactSpan;
+    if (first >= n) return;
+    const uint32_t last = min(first + kCompactSpan, n);
+    const uint32_t tag = pass << 2;
+    uint32_t c[kQ] = {};
+    for (uint32_t i = first + threadIdx.x; i < last; i += kCompactThreads) {
+        bool m[kQ];
+        compact_match<kShade>(ps.flag[i], tag, m);
+        static_for<0, kQ>([&](auto Q) { c[Q()] += m[Q()] ? 1u : 0u; });
+    }
+    static_for<0, kQ>([&](auto Q) {
+        uint32_t v = c[Q()];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane_id() == 0) red[Q()][threadIdx.x / kBlock] = v;
+    });
+    __syncthreads();
+    if (threadIdx.x < (uint32_t)kQ) {
+        uint32_t v = 0;
+        for (uint32_t w = 0; w < kWaves; ++w) v += red[threadIdx.x][w];
+        ps.ccount[threadIdx.x * (prm.cap / kCompactSpan + 1) + b] = v;
+    }
+}
+
+template <bool kShade>
+__global__ __launch_bounds__(kCompactThreads) void k_compact_scatter(PtParams prm, PathState ps, uint32_t pass)
+{
+    constexpr int kQ = kShade ? 2 : 1;
+    constexpr uint32_t kWaves = kCompactThreads / kBlock;
+    __shared__ uint32_t red[kQ][kWaves];
+    __shared__ uint32_t base[kQ];
+    uint32_t *cnt = prm.counters + pass * kCtrStride;
+    const uint32_t n = prm.chunk_paths;
+    const uint32_t b = blockIdx.x, first = b * kCompactSpan;
+    if (first >= n) return;
+    const uint32_t last = min(first + kCompactSpan, n);
+    const uint32_t stride = prm.cap / kCompactSpan + 1;
+    uint32_t *const qs[2] = {kShade ? ps.q_hit : ps.q_medium, ps.q_hit_vol};
+    const uint32_t ctr[2] = {kShade ? kCtrHit : kCtrMedium, kCtrHitVol};
+    const uint32_t tag = pass << 2;
+    const uint32_t wave = threadIdx.x / kBlock;
+    const uint64_t below = (1ull << lane_id()) - 1ull;
+    // this block's offsets: the counts of the blocks before it
+    static_for<0, kQ>([&](auto Q) {
+        uint32_t v = 0;
+        for (uint32_t j = threadIdx.x; j < b; j += kCompactThreads) v += ps.ccount[Q() * stride + j];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane_id() == 0) red[Q()][wave] = v;
+    });
+    __syncthreads();
+    if (threadIdx.x < (uint32_t)kQ) {
+        uint32_t v = 0;
+        for (uint32_t w = 0; w < kWaves; ++w) v += red[threadIdx.x][w];
+        base[threadIdx.x] = v;
+    }
+    __syncthreads();
+    uint32_t off[kQ];
+    static_for<0, kQ>([&](auto Q) { off[Q()] = base[Q()]; });
+    for (uint32_t t0 = first; t0 < last; t0 += kCompactThreads) {
+        const uint32_t i = t0 + threadIdx.x;
+        uint32_t p = 0;
+        bool m[kQ] = {};
+        if (i < last) {
+            p = i;
+            compact_match<kShade>(ps.flag[p], tag, m);
+        }
+        uint32_t rank[kQ];
+        __syncthreads();  // red[] of the previous tile has been read
+        static_for<0, kQ>([&](auto Q) {
+            const uint64_t mask = __ballot(m[Q()]);
+            rank[Q()] = (uint32_t)__popcll(mask & below);
+            if (lane_id() == 0) red[Q()][wave] = (uint32_t)__popcll(mask);
+        });
+        __syncthreads();
+        static_for<0, kQ>([&](auto Q) {
+            uint32_t before = 0, total = 0;
+            for (uint32_t w = 0; w < kWaves; ++w) {
+                const uint32_t v = red[Q()][w];
+                before += w < wave ? v : 0u;
+                total += v;
+            }
+            if (m[Q()]) qs[Q()][off[Q()] + before + rank[Q()]] = p;
+            off[Q()] += total;
+        });
+    }
+    if (last == n && threadIdx.x < (uint32_t)kQ) cnt[ctr[threadIdx.x]] = off[threadIdx.x];
 }
 
 // ------------------------------------------------------------ medium
@@ -514,9 +621,7 @@ __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PtParams prm, Pa
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     const uint32_t n = cnt[kCtrMedium];
     const ymrt_render_settings &rs = prm.rs;
-    uint32_t *const ctrs[2] = {cnt + kCtrHit, cnt + kCtrHitVol};
-    uint32_t *const qs[2] = {ps.q_hit, ps.q_hit_vol};
-    run_items<2>(cnt + kCtrWorkMedium, n, ctrs, qs, [&](uint32_t i, uint32_t &p, bool(&out)[2]) {
+    run_items<0>(cnt + kCtrWorkMedium, n, nullptr, nullptr, [&](uint32_t i, uint32_t &p, bool(&)[1]) {
         {
             p = ps.q_medium[i];
             const uint4 ha = ps.hit_a[p];
@@ -549,12 +654,12 @@ __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PtParams prm, Pa
                 store_sampler(ps, p, smp);
             }
             if (vol_hit) {
-                out[1] = true;
+                ps.flag[p] = (pass << 2) | kFlagVolume;
                 const f3 vpos = o + d * dist;
                 ps.hit_a[p] = make_uint4(ha.x, ha.y, ha.z, ha.w | kHitVolume);
                 ps.hit_b[p] = make_float4(vpos.x, vpos.y, vpos.z, 0.0f);
             } else if (hit) {
-                out[0] = true;
+                ps.flag[p] = (pass << 2) | kFlagHit;
             } else {
                 f3 Lv = xyz(ps.L[p]);
                 if (rs.enable_env_light) Lv = Lv + light_clamp(rs, beta * background(d), info_depth(ps.info[p]));
@@ -1348,6 +1453,20 @@ int launch_medium(const DevScene &sc, const PtParams &prm, const PathState &ps, 
         hipLaunchKernelGGL(k_medium<true>, persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps, pass);
     else
         hipLaunchKernelGGL(k_medium<false>, persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps, pass);
+    YM_HIP(hipGetLastError());
+    return YMRT_OK;
+}
+
+int launch_compact(const PtParams &prm, const PathState &ps, uint32_t pass, bool shade, hipStream_t s)
+{
+    const dim3 g((prm.chunk_paths + kCompactSpan - 1) / kCompactSpan), b(kCompactThreads);
+    if (shade) {
+        hipLaunchKernelGGL(k_compact_count<true>, g, b, 0, s, prm, ps, pass);
+        hipLaunchKernelGGL(k_compact_scatter<true>, g, b, 0, s, prm, ps, pass);
+    } else {
+        hipLaunchKernelGGL(k_compact_count<false>, g, b, 0, s, prm, ps, pass);
+        hipLaunchKernelGGL(k_compact_scatter<false>, g, b, 0, s, prm, ps, pass);
+    }
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }

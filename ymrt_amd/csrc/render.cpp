@@ -21,6 +21,7 @@ int launch_raygen(const DevScene &sc, const PtParams &prm, const PathState &ps, 
 int launch_extend(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool nearby,
                   bool media, uint32_t stack, hipStream_t s);
 int launch_medium(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s);
+int launch_compact(const PtParams &prm, const PathState &ps, uint32_t pass, bool shade, hipStream_t s);
 int launch_shade(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool volume,
                  hipStream_t s);
 int launch_shadow(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool media,
@@ -100,6 +101,8 @@ static int ensure_buffers(ymrt_scene *sc, uint32_t cap)
     rc |= alloc(rb, &ps.q_hit, n);
     rc |= alloc(rb, &ps.q_hit_vol, n);
     rc |= alloc(rb, &ps.q_medium, n);
+    rc |= alloc(rb, &ps.flag, n);
+    rc |= alloc(rb, &ps.ccount, 2 * ((size_t)n / kCompactSpan + 1));
     rc |= alloc(rb, &ps.q_shadow, n);
     rc |= alloc(rb, &ps.q_transmit, n);
     rc |= alloc(rb, &ps.spill, (size_t)(kMaxStackDepth - kStackWindow) * kSpillThreads);
@@ -220,8 +223,12 @@ int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, floa
                      return launch_extend(sc->dev, prm, rb->ps, pass, nearby, media, stack, stream);
                  })))
                 return rc;
-            if (media && (rc = timed(5, [&] { return launch_medium(sc->dev, prm, rb->ps, pass, stream); })))
+            if (media && (rc = timed(5, [&] {
+                              int r = launch_compact(prm, rb->ps, pass, false, stream);
+                              return r ? r : launch_medium(sc->dev, prm, rb->ps, pass, stream);
+                          })))
                 return rc;
+            if ((rc = timed(2, [&] { return launch_compact(prm, rb->ps, pass, true, stream); }))) return rc;
             if ((rc = timed(2, [&] { return launch_shade(sc->dev, prm, rb->ps, pass, false, stream); }))) return rc;
             if (media && (rc = timed(2, [&] { return launch_shade(sc->dev, prm, rb->ps, pass, true, stream); })))
                 return rc;
