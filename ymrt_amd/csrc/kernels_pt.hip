@@ -703,9 +703,13 @@ __device__ __forceinline__ float shadow_tmax(f3 pos, f3 ldir, f3 lpos, f3 lerr, 
 // EvalVolumeShapeLight 419-513); otherwise surface vertices (2132-2302 with
 // EvalDistantLight 83-198 and EvalShapeLight 294-417).
 // kGraph: the scene has procedural texture nodes or normal / bump maps (the
-// variant without them keeps the image-texture fast path only).
+// variant without them keeps the image-texture fast path only).  The
+// variants without the graph evaluator are held to 168 VGPRs (3 waves per
+// SIMD, a few spilled registers): the shading stage waits on dependent
+// loads, and the third wave hides more of them than the spills cost (6 %
+// faster on the interior frame; 4 waves spill too much).
 template <bool kVolume, bool kGraph>
-__global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGraph ? 2 : 3))) void k_shade(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
 {
     __shared__ float tex_stk[(kVolume || !kGraph) ? 1 : kTexStack * 3 * kBlock];
     uint32_t *cnt = prm.counters + pass * kCtrStride;
