@@ -193,9 +193,16 @@ struct Hit {
 // [depth - kWindow][thread], and popping restores it.  Most traversals never
 // leave the window, so the spill area costs nothing but its allocation while
 // the small window lets more waves share a CU's LDS.
-template <int kBlock, int kWindow = 0>
+//
+// With kPreSlots, the world-space ray setup (reciprocal direction, shear
+// constants and axis) is parked in LDS [kPreSlots][kBlock] when a traversal
+// enters an instance and read back when it leaves, instead of being
+// recomputed with six divisions.
+constexpr int kPreSlots = 7;
+template <int kBlock, int kWindow = 0, bool kPreLds = false>
 struct LaneScratch {
     static constexpr int kLanes = kBlock;
+    static constexpr bool kParkPre = kPreLds;
     uint32_t *stack;        // [kWindow or kStack][kBlock]
     uint32_t *nb_prim;      // [kNearbyCap][kBlock]
     float *nb_t;
@@ -203,6 +210,35 @@ struct LaneScratch {
     int lane;
     uint32_t *spill;        // global [kMaxSpill][spill_stride], unused when kWindow == 0
     uint32_t spill_stride, spill_index;
+    float *pre_park = nullptr;  // LDS [kPreSlots][kBlock] when kPreLds
+
+    YM_HD void park_pre(const RayPre &r)
+    {
+        float *q = pre_park + lane;
+        q[0 * kBlock] = r.inv.x;
+        q[1 * kBlock] = r.inv.y;
+        q[2 * kBlock] = r.inv.z;
+        q[3 * kBlock] = r.sx;
+        q[4 * kBlock] = r.sy;
+        q[5 * kBlock] = r.sz;
+        q[6 * kBlock] = __builtin_bit_cast(float, r.kz);
+    }
+    // the parked setup of the ray (o, d)
+    YM_HD RayPre unpark_pre(f3 o, f3 d) const
+    {
+        const float *q = pre_park + lane;
+        RayPre r;
+        r.o = o;
+        r.d = d;
+        r.inv = f3{q[0 * kBlock], q[1 * kBlock], q[2 * kBlock]};
+        r.sx = q[3 * kBlock];
+        r.sy = q[4 * kBlock];
+        r.sz = q[5 * kBlock];
+        r.kz = __builtin_bit_cast(int, q[6 * kBlock]);
+        r.kx = r.kz + 1 == 3 ? 0 : r.kz + 1;
+        r.ky = r.kx + 1 == 3 ? 0 : r.kx + 1;
+        return r;
+    }
 
     static constexpr int kWindowSize = kWindow;
     YM_HD int nbi(int i) const { return i * kBlock + lane; }
@@ -431,6 +467,7 @@ struct Traversal {
             } else if (take) {
                 if (sp + 1 > kStack) return kTravOverflow;
                 ls.push(sp++, kSentinel);
+                if constexpr (LS::kParkPre) ls.park_pre(pre);
                 pre = make_pre(xf_point(in.w2o, q.o), xf_dir(in.w2o, q.d));
                 in_blas = true;
                 cur_inst = ii;
@@ -478,7 +515,10 @@ struct Traversal {
             node = ls.pop(--sp);
             if (node != kSentinel) break;
             // leaving an instance: back to world space
-            pre = make_pre(q.o, q.d);
+            if constexpr (LS::kParkPre)
+                pre = ls.unpark_pre(q.o, q.d);
+            else
+                pre = make_pre(q.o, q.d);
             in_blas = false;
             if (kNearby) nearby_exit(ls);
         }

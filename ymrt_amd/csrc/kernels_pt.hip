@@ -395,10 +395,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere 
     __shared__ uint32_t nbp[(kNearby ? kNearbyCap : 1) * kBlock];
     __shared__ float nbt[(kNearby ? kNearbyCap : 1) * kBlock];
     __shared__ int nbb[(kNearby ? kNearbyCap : 1) * kBlock];
+    __shared__ float pre_park[kPreSlots * kBlock];
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     const uint32_t *qin = ps.queue[pass & 1];
-    LaneScratch<kBlock, kStackWindow> ls{stack, nbp, nbt, nbb, (int)lane_id(), ps.spill, kSpillThreads,
-                                         blockIdx.x * kBlock + threadIdx.x};
+    LaneScratch<kBlock, kStackWindow, true> ls{stack, nbp, nbt, nbb, (int)lane_id(), ps.spill, kSpillThreads,
+                                               blockIdx.x * kBlock + threadIdx.x, pre_park};
     const ymrt_render_settings &rs = prm.rs;
     uint32_t rays = 0, cam_rays = 0;
     TraceCounters tc{0, 0};
@@ -1148,10 +1149,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     constexpr int kQ = kMedia ? 1 : 0;
     __shared__ uint32_t stack[kStackWindow * kBlock];
     __shared__ uint32_t stage[(kQ > 0 ? kQ : 1) * kStageCap];
+    __shared__ float pre_park[kPreSlots * kBlock];
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     const uint32_t cap = prm.cap;
-    LaneScratch<kBlock, kStackWindow> ls{stack, nullptr, nullptr, nullptr, (int)lane_id(), ps.spill, kSpillThreads,
-                                         blockIdx.x * kBlock + threadIdx.x};
+    LaneScratch<kBlock, kStackWindow, true> ls{stack, nullptr, nullptr, nullptr, (int)lane_id(), ps.spill,
+                                               kSpillThreads, blockIdx.x * kBlock + threadIdx.x, pre_park};
     uint32_t nsh = 0;
     TraceCounters tc{0, 0};
     uint32_t *const ctrs[1] = {cnt + kCtrTransmit};
