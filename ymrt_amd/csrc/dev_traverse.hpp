@@ -375,10 +375,39 @@ struct Traversal {
         nb_count = nbn;
     }
 
+    // leaving an instance: back to world space
     template <typename LS>
-    YM_HD int step(const DevScene &sc, LS &ls, TraceCounters &cnt)
+    YM_HD void leave_instance(LS &ls)
     {
-        if (!(node & kLeafBit)) {
+        if constexpr (LS::kParkPre)
+            pre = ls.unpark_pre(q.o, q.d);
+        else
+            pre = make_pre(q.o, q.d);
+        in_blas = false;
+        if (kNearby) nearby_exit(ls);
+    }
+
+    // Pops the next entry.  kThrough: a sentinel is handled on the spot
+    // (leave the instance, pop again); otherwise it is left in node for the
+    // next step.
+    template <bool kThrough, typename LS>
+    YM_HD int pop(LS &ls)
+    {
+        for (;;) {
+            if (sp == 0) return found ? kTravHit : kTravMiss;
+            node = ls.pop(--sp);
+            if (!kThrough || node != kSentinel) return kTravContinue;
+            leave_instance(ls);
+        }
+    }
+
+    // One wide-node visit (node is an inner node): pushes the far children
+    // that are hit and moves to the nearest, or pops.  Without kThrough it
+    // never changes the ray setup, so a loop of visits keeps it in place.
+    template <bool kThrough, typename LS>
+    YM_HD int visit_node(const DevScene &sc, LS &ls, TraceCounters &cnt)
+    {
+        {
             const WNode *n = sc.nodes + node;
             const float4 lx = *reinterpret_cast<const float4 *>(n->lo_x);
             const float4 hx = *reinterpret_cast<const float4 *>(n->hi_x);
@@ -431,6 +460,19 @@ struct Traversal {
                 node = v0;
                 return kTravContinue;
             }
+        }
+        return pop<kThrough>(ls);
+    }
+
+    template <typename LS>
+    YM_HD int step(const DevScene &sc, LS &ls, TraceCounters &cnt)
+    {
+        if (node == kSentinel) {
+            leave_instance(ls);
+            return pop<true>(ls);
+        }
+        if (!(node & kLeafBit)) {
+            return visit_node<true>(sc, ls, cnt);
         } else if (!in_blas) {
             // top-level leaf: one instance
             const uint32_t ii = leaf_first(node);
@@ -509,20 +551,7 @@ struct Traversal {
                 if (kAny) return kTravHit;
             }
         }
-        // pop
-        for (;;) {
-            if (sp == 0) return found ? kTravHit : kTravMiss;
-            node = ls.pop(--sp);
-            if (node != kSentinel) break;
-            // leaving an instance: back to world space
-            if constexpr (LS::kParkPre)
-                pre = ls.unpark_pre(q.o, q.d);
-            else
-                pre = make_pre(q.o, q.d);
-            in_blas = false;
-            if (kNearby) nearby_exit(ls);
-        }
-        return kTravContinue;
+        return pop<true>(ls);
     }
 };
 

@@ -203,20 +203,32 @@ struct RefillRange {
     __device__ __forceinline__ uint32_t want() const { return more ? (uint32_t)kRefillLanes : (uint32_t)kBlock; }
 };
 
-// One traversal step of the wave's continuing lanes (r == kTravContinue).
+// Traversal steps of the wave's continuing lanes (r == kTravContinue).
 // Node visits and leaf tests are two long code paths; while at least
-// kInnerFirst lanes are at wide nodes, lanes at leaves wait, so each path
-// runs with a fuller wave instead of both running under partial masks.
-// Every lane steps when fewer are at nodes, so no lane waits for long.  24
-// measured best on the interior frame (8: slower than stepping every lane;
-// 12, 16, 32: less gain).
+// kInnerFirst lanes are at wide nodes, only those lanes step, so each path
+// runs with a fuller wave instead of both running under partial masks; with
+// fewer at nodes every lane steps, so no lane waits for long.  24 measured
+// best on the interior frame (8: slower than stepping every lane; 12, 16,
+// 32: less gain).  kLoop: the node visits run as a loop of their own until
+// fewer than kInnerFirst lanes are at nodes (the ray setup then stays in
+// place, which saves register copies per visit); faster for closest-hit
+// rays, slower for shadow rays, which end at their first hit.
 constexpr int kInnerFirst = 24;
-template <class TV, class LS>
+template <bool kLoop, class TV, class LS>
 __device__ __forceinline__ void wave_step(TV &tv, int &r, const DevScene &sc, LS &ls, TraceCounters &tc)
 {
-    const bool leaf = (tv.node & kLeafBit) != 0;
-    const uint64_t inner = __ballot(r == kTravContinue && !leaf);
-    if (r == kTravContinue && (!leaf || __popcll(inner) < kInnerFirst)) r = tv.step(sc, ls, tc);
+    if constexpr (kLoop) {
+        for (;;) {
+            const bool at_node = r == kTravContinue && !(tv.node & kLeafBit);
+            if (__popcll(__ballot(at_node)) < kInnerFirst) break;
+            if (at_node) r = tv.template visit_node<false>(sc, ls, tc);
+        }
+        if (r == kTravContinue) r = tv.step(sc, ls, tc);
+    } else {
+        const bool leaf = (tv.node & kLeafBit) != 0;
+        const uint64_t inner = __ballot(r == kTravContinue && !leaf);
+        if (r == kTravContinue && (!leaf || __popcll(inner) < kInnerFirst)) r = tv.step(sc, ls, tc);
+    }
 }
 
 __device__ __forceinline__ void wave_add64(unsigned long long *counter, uint32_t v)
@@ -432,7 +444,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere 
         for (;;) {
             const uint64_t busy = __ballot(r == kTravContinue);
             if (busy == 0 || (uint32_t)(kBlock - __popcll(busy)) >= want) break;
-            wave_step(tv, r, sc, ls, tc);
+            wave_step<true>(tv, r, sc, ls, tc);
         }
         if (r != kTravContinue && r != kLaneIdle) {
             if (r == kTravOverflow) atomicOr(prm.error, 1u);
@@ -1190,7 +1202,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         for (;;) {
             const uint64_t busy = __ballot(r == kTravContinue);
             if (busy == 0 || (uint32_t)(kBlock - __popcll(busy)) >= want) break;
-            wave_step(tv, r, sc, ls, tc);
+            wave_step<false>(tv, r, sc, ls, tc);
         }
         bool out[1] = {false};
         // finished rays: note the result, start the path's next ray or
