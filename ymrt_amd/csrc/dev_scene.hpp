@@ -168,6 +168,7 @@ struct DevScene {
     uint32_t has_boundaries;       // any instance with treat_as_boundary
     uint32_t has_spheres;          // any sphere instance
     uint32_t has_graph;            // procedural texture nodes or normal / bump maps
+    const uint32_t *mat_diffuse;   // per material: 1 when its BSDF is the diffuse lobe alone
     DevCamera camera;
 };
 

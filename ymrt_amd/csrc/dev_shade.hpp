@@ -789,12 +789,34 @@ __device__ __forceinline__ f3 tex_color(TX &tx, uint32_t tex, const float *def)
     return tex != kEmpty ? tx.eval(tex) : f3{def[0], def[1], def[2]};
 }
 
-// DefaultMtlBSDF::InitBSDFSettings (src/BSDF.cuh:453-618)
-template <class TX>
+// DefaultMtlBSDF::InitBSDFSettings (src/BSDF.cuh:453-618).  kDiffuse: the
+// material is diffuse-only (zero coat, metal, specular and transmission
+// weights, a coat that cannot overflow), where the general weighting leaves
+// the diffuse lobe alone with the albedo as its colour: tint, (1 - metal),
+// (1 - F * 0) and (1 - 0) are all exactly 1.
+template <bool kDiffuse = false, class TX>
 __device__ __forceinline__ void init_default_bsdf(TX &tx, Bsdf &b, const DevMaterial &m, f3 ray_d,
                                                   int hit_back, float ray_ior, float ex_ior)
 {
     const float ior_min = 0.0001f;
+    if constexpr (kDiffuse) {
+        const f3 dif = tex_color(tx, m.diffuse_albedo_tex, m.diffuse_albedo);
+        b.ni = gmax(ray_ior, ior_min);
+        b.ax = b.ay = b.cax = b.cay = 0.0f;
+        b.ior_coat = b.ior_out = 1.0f;
+        b.metal_n = b.metal_k = s3(0.0f);
+#pragma unroll
+        for (int i = 0; i < kSlots; ++i) {
+            b.w[i] = 0.0f;
+            b.col[i] = s3(0.0f);
+        }
+        b.mask = 1u << kSlotDiffuse;
+        b.col[kSlotDiffuse] = dif;
+        const float dif_lum = luminance(dif);
+        const float wsum = 0.0f + dif_lum;
+        b.w[kSlotDiffuse] = wsum > 0.0f ? dif_lum * safe_rcp(wsum) : 1.0f;
+        return;
+    }
     const float coat_w = tex_float(tx, m.coat_weight_tex, m.coat_weight);
     const float ior_in = gmax(ray_ior, ior_min);
     const float ior_coat = gmax(m.coat_ior, ior_min);
@@ -890,14 +912,17 @@ __device__ __forceinline__ void init_default_bsdf(TX &tx, Bsdf &b, const DevMate
     }
 }
 
-// DefaultMtlBSDF::EvalWi (src/BSDF.cuh:619-632)
+// DefaultMtlBSDF::EvalWi (src/BSDF.cuh:619-632).  kFixed: the lobe mask
+// known at compile time (0: b.mask).
+template <uint32_t kFixed = 0>
 YM_HD f3 bsdf_eval(const Bsdf &b, f3 wo, f3 wi, float &pdf)
 {
+    const uint32_t mask = kFixed ? kFixed : b.mask;
     f3 f = s3(0.0f);
     float p = 0.0f;
 #pragma unroll
     for (int i = 0; i < kSlots; ++i) {
-        if (!(b.mask & (1u << i))) continue;
+        if (!(mask & (1u << i))) continue;
         float lp = 0.0f;
         f = f + lobe_eval(b.lobe(i), wo, wi, lp);
         p += b.w[i] * lp;
@@ -906,16 +931,18 @@ YM_HD f3 bsdf_eval(const Bsdf &b, f3 wo, f3 wi, float &pdf)
     return f;
 }
 
-// DefaultMtlBSDF::SampleWi (src/BSDF.cuh:633-679)
+// DefaultMtlBSDF::SampleWi (src/BSDF.cuh:633-679); kFixed as for bsdf_eval.
+template <uint32_t kFixed = 0>
 __device__ __forceinline__ bool bsdf_sample(const Bsdf &b, f3 wo, f3 &weight, f3 &wi, float &pdf, float u0,
                                             float u1, float u2)
 {
-    if (!b.mask) return false;
+    const uint32_t mask = kFixed ? kFixed : b.mask;
+    if (!mask) return false;
     float cdf = 0.0f;
     int sel = -1, last = -1;
 #pragma unroll
     for (int i = 0; i < kSlots; ++i) {
-        if (!(b.mask & (1u << i))) continue;
+        if (!(mask & (1u << i))) continue;
         last = i;
         if (sel < 0) {
             if (u0 < cdf + b.w[i]) sel = i;
@@ -939,7 +966,7 @@ __device__ __forceinline__ bool bsdf_sample(const Bsdf &b, f3 wo, f3 &weight, f3
         }
 #pragma unroll
     for (int i = 0; i < kSlots; ++i) {
-        if (!(b.mask & (1u << i)) || i == sel) continue;
+        if (!(mask & (1u << i)) || i == sel) continue;
         float p = 0.0f;
         mf = mf + lobe_eval(b.lobe(i), wo, wi, p);
         mp += b.w[i] * p;

@@ -472,7 +472,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere 
                     Lv = Lv + light_clamp(rs, xyz(ps.beta[p]) * background(tv.q.d), info_depth(info));
                 ps.out[p] = f4(Lv, 0.0f);
             } else {
-                ps.flag[p] = (pass << 2) | (in_medium ? kFlagMedium : kFlagHit);
+                ps.flag[p] = (pass << kFlagBits) |
+                             (in_medium ? kFlagMedium
+                                        : (sc.mat_diffuse[sc.instances[h.inst].material] ? kFlagDiffuse : kFlagHit));
                 uint32_t flags = h.back ? kHitBack : 0u;
                 bool listed = false;
                 const int nbc = tv.nb_count;
@@ -517,11 +519,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere 
 // matches at the sum of the counts before it, and the block holding the last
 // slot sets the queue lengths.
 template <bool kShade>
-__device__ __forceinline__ void compact_match(uint32_t f, uint32_t tag, bool (&m)[kShade ? 2 : 1])
+__device__ __forceinline__ void compact_match(uint32_t f, uint32_t tag, bool (&m)[kShade ? 3 : 1])
 {
     if constexpr (kShade) {
         m[0] = f == (tag | kFlagHit);
         m[1] = f == (tag | kFlagVolume);
+        m[2] = f == (tag | kFlagDiffuse);
     } else {
         m[0] = f == (tag | kFlagMedium);
     }
@@ -530,14 +533,14 @@ __device__ __forceinline__ void compact_match(uint32_t f, uint32_t tag, bool (&m
 template <bool kShade>
 __global__ __launch_bounds__(kCompactThreads) void k_compact_count(PtParams prm, PathState ps, uint32_t pass)
 {
-    constexpr int kQ = kShade ? 2 : 1;
+    constexpr int kQ = kShade ? 3 : 1;
     constexpr uint32_t kWaves = kCompactThreads / kBlock;
     __shared__ uint32_t red[kQ][kWaves];
     const uint32_t n = prm.chunk_paths;
     const uint32_t b = blockIdx.x, first = b * kCompactSpan;
     if (first >= n) return;
     const uint32_t last = min(first + kCompactSpan, n);
-    const uint32_t tag = pass << 2;
+    const uint32_t tag = pass << kFlagBits;
     uint32_t c[kQ] = {};
     for (uint32_t i = first + threadIdx.x; i < last; i += kCompactThreads) {
         bool m[kQ];
@@ -560,7 +563,7 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_count(PtParams prm,
 template <bool kShade>
 __global__ __launch_bounds__(kCompactThreads) void k_compact_scatter(PtParams prm, PathState ps, uint32_t pass)
 {
-    constexpr int kQ = kShade ? 2 : 1;
+    constexpr int kQ = kShade ? 3 : 1;
     constexpr uint32_t kWaves = kCompactThreads / kBlock;
     __shared__ uint32_t red[kQ][kWaves];
     __shared__ uint32_t base[kQ];
@@ -570,9 +573,9 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_scatter(PtParams pr
     if (first >= n) return;
     const uint32_t last = min(first + kCompactSpan, n);
     const uint32_t stride = prm.cap / kCompactSpan + 1;
-    uint32_t *const qs[2] = {kShade ? ps.q_hit : ps.q_medium, ps.q_hit_vol};
-    const uint32_t ctr[2] = {kShade ? kCtrHit : kCtrMedium, kCtrHitVol};
-    const uint32_t tag = pass << 2;
+    uint32_t *const qs[3] = {kShade ? ps.q_hit : ps.q_medium, ps.q_hit_vol, ps.q_hit_diffuse};
+    const uint32_t ctr[3] = {kShade ? kCtrHit : kCtrMedium, kCtrHitVol, kCtrHitDiffuse};
+    const uint32_t tag = pass << kFlagBits;
     const uint32_t wave = threadIdx.x / kBlock;
     const uint64_t below = (1ull << lane_id()) - 1ull;
     // this block's offsets: the counts of the blocks before it
@@ -667,12 +670,12 @@ __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PtParams prm, Pa
                 store_sampler(ps, p, smp);
             }
             if (vol_hit) {
-                ps.flag[p] = (pass << 2) | kFlagVolume;
+                ps.flag[p] = (pass << kFlagBits) | kFlagVolume;
                 const f3 vpos = o + d * dist;
                 ps.hit_a[p] = make_uint4(ha.x, ha.y, ha.z, ha.w | kHitVolume);
                 ps.hit_b[p] = make_float4(vpos.x, vpos.y, vpos.z, 0.0f);
             } else if (hit) {
-                ps.flag[p] = (pass << 2) | kFlagHit;
+                ps.flag[p] = (pass << kFlagBits) | (sc.mat_diffuse[sc.instances[ha.y].material] ? kFlagDiffuse : kFlagHit);
             } else {
                 f3 Lv = xyz(ps.L[p]);
                 if (rs.enable_env_light) Lv = Lv + light_clamp(rs, beta * background(d), info_depth(ps.info[p]));
@@ -716,24 +719,28 @@ __device__ __forceinline__ float shadow_tmax(f3 pos, f3 ldir, f3 lpos, f3 lerr, 
 // EvalVolumeShapeLight 419-513); otherwise surface vertices (2132-2302 with
 // EvalDistantLight 83-198 and EvalShapeLight 294-417).
 // kGraph: the scene has procedural texture nodes or normal / bump maps (the
-// variant without them keeps the image-texture fast path only).  The
+// variant without them keeps the image-texture fast path only).  kDiffuse:
+// the hits on diffuse-only materials (their own queue), shaded with the
+// diffuse lobe alone, which gives the general variant's floats with far
+// fewer live registers.  The
 // variants without the graph evaluator are held to 168 VGPRs (3 waves per
 // SIMD, a few spilled registers): the shading stage waits on dependent
 // loads, and the third wave hides more of them than the spills cost (6 %
 // faster on the interior frame; 4 waves spill too much).
-template <bool kVolume, bool kGraph>
+template <bool kVolume, bool kGraph, bool kDiffuse = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGraph ? 2 : 3))) void k_shade(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
 {
     __shared__ float tex_stk[(kVolume || !kGraph) ? 1 : kTexStack * 3 * kBlock];
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     uint32_t *cnt_next = prm.counters + (pass + 1) * kCtrStride;
-    const uint32_t n = cnt[kVolume ? kCtrHitVol : kCtrHit];
-    const uint32_t *q = kVolume ? ps.q_hit_vol : ps.q_hit;
+    const uint32_t n = cnt[kVolume ? kCtrHitVol : kDiffuse ? kCtrHitDiffuse : kCtrHit];
+    const uint32_t *q = kVolume ? ps.q_hit_vol : kDiffuse ? ps.q_hit_diffuse : ps.q_hit;
     const ymrt_render_settings &rs = prm.rs;
     const uint32_t cap = prm.cap;
     uint32_t *const ctrs[2] = {cnt + kCtrShadow, cnt_next + kCtrActive};
     uint32_t *const qs[2] = {ps.q_shadow, ps.queue[(pass + 1) & 1]};
-    run_items<2>(cnt + (kVolume ? kCtrWorkShadeVol : kCtrWorkShade), n, ctrs, qs,
+    constexpr uint32_t kLobes = kDiffuse ? 1u << kSlotDiffuse : 0u;
+    run_items<2>(cnt + (kVolume ? kCtrWorkShadeVol : kDiffuse ? kCtrWorkShadeDiffuse : kCtrWorkShade), n, ctrs, qs,
                  [&](uint32_t i, uint32_t &p, bool(&out)[2]) {
         {
             p = q[i];
@@ -961,7 +968,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGraph ?
                     Bsdf bs;
                     init_shading_space(bs, tx, normal_tex, bump_tex, back, ray_d, ng, sn, dpdu, dpdv, in.w2o, duv);
                     bs.mask = 0;
-                    if (is_default) init_default_bsdf(tx, bs, m, ray_d, back, ray_ior, ex_ior);
+                    if (is_default) init_default_bsdf<kDiffuse>(tx, bs, m, ray_d, back, ray_ior, ex_ior);
                     if (prm.adj_mat && prm.adj_mat[in.material]) {
                         // every later term of the path carries this vertex's albedo once
                         const uint32_t k = ps.adj_n[p];
@@ -980,7 +987,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGraph ?
                             distant_sample_li(dl, u0, u1, ldir, Li, lpdf);
                             const f3 lwi = bs.to_local(ldir);
                             float wpdf = 0.0f;
-                            const f3 f = bsdf_eval(bs, wo, lwi, wpdf);
+                            const f3 f = bsdf_eval<kLobes>(bs, wo, lwi, wpdf);
                             if (maxcomp(f) > kMinColorEpsilon) {
                                 const bool transmit = dot(-ray_d, ng) * dot(ldir, ng) < 0.0f;
                                 const bool invalid = (!same_hemi(wo, lwi) && !transmit) || (same_hemi(wo, lwi) && transmit);
@@ -996,7 +1003,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGraph ?
                             const float u0 = sampler_next(sc, smp);
                             const float u1 = sampler_next(sc, smp);
                             const float u2 = sampler_next(sc, smp);
-                            if (bsdf_sample(bs, wo, bw, bwi, bpdf, u0, u1, u2)) {
+                            if (bsdf_sample<kLobes>(bs, wo, bw, bwi, bpdf, u0, u1, u2)) {
                                 const f3 ldir = bs.to_world(bwi);
                                 float lpdf = 0.0f;
                                 const f3 Li = distant_eval_li(dl, ldir, lpdf);
@@ -1023,7 +1030,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGraph ?
                             const bool valid = light_sample_li(sc, light, pos, u0, u1, ldir, lpos, lerr, lng, Li, lpdf);
                             const f3 lwi = bs.to_local(ldir);
                             float wpdf = 0.0f;
-                            const f3 f = bsdf_eval(bs, wo, lwi, wpdf);
+                            const f3 f = bsdf_eval<kLobes>(bs, wo, lwi, wpdf);
                             if (maxcomp(f) > kMinColorEpsilon && valid) {
                                 const bool transmit = dot(-ray_d, ng) * dot(ldir, ng) < 0.0f;
                                 const bool invalid = (!same_hemi(wo, lwi) && !transmit) || (same_hemi(wo, lwi) && transmit);
@@ -1040,7 +1047,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGraph ?
                             const float u0 = sampler_next(sc, smp);
                             const float u1 = sampler_next(sc, smp);
                             const float u2 = sampler_next(sc, smp);
-                            if (bsdf_sample(bs, wo, bw, bwi, bpdf, u0, u1, u2)) {
+                            if (bsdf_sample<kLobes>(bs, wo, bw, bwi, bpdf, u0, u1, u2)) {
                                 const f3 ldir = bs.to_world(bwi);
                                 f3 lpos = s3(0.0f), lerr = s3(0.0f), lng = s3(0.0f);
                                 float lpdf = 0.0f;
@@ -1063,7 +1070,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGraph ?
                     const float u0 = sampler_next(sc, smp);
                     const float u1 = sampler_next(sc, smp);
                     const float u2 = sampler_next(sc, smp);
-                    if (bsdf_sample(bs, wo, bw, wi, pdf, u0, u1, u2)) {
+                    if (bsdf_sample<kLobes>(bs, wo, bw, wi, pdf, u0, u1, u2)) {
                         beta = beta * bw;
                         const float rr = gmin(gmax(beta.x, gmax(beta.y, beta.z)), 1.0f);
                         bool cont = true;
@@ -1495,12 +1502,17 @@ int launch_shade(const DevScene &sc, const PtParams &prm, const PathState &ps, u
     if (volume)
         hipLaunchKernelGGL((k_shade<true, false>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps,
                            pass);
-    else if (sc.has_graph)
+    else if (sc.has_graph) {
         hipLaunchKernelGGL((k_shade<false, true>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps,
                            pass);
-    else
+        hipLaunchKernelGGL((k_shade<false, true, true>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm,
+                           ps, pass);
+    } else {
         hipLaunchKernelGGL((k_shade<false, false>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps,
                            pass);
+        hipLaunchKernelGGL((k_shade<false, false, true>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc,
+                           prm, ps, pass);
+    }
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }

@@ -21,6 +21,8 @@ constexpr uint32_t kCtrMedium = 10;        // paths inside media queued for free
 constexpr uint32_t kCtrWorkMedium = 11;
 constexpr uint32_t kCtrTransmit = 12;      // paths whose unoccluded shadow rays cross media
 constexpr uint32_t kCtrWorkTransmit = 13;
+constexpr uint32_t kCtrHitDiffuse = 14;    // surface hits on diffuse-only materials
+constexpr uint32_t kCtrWorkShadeDiffuse = 15;
 constexpr uint32_t kCtrStride = 16;
 
 // Compaction: blocks of kCompactThreads threads, kCompactSpan queue entries
@@ -28,13 +30,15 @@ constexpr uint32_t kCtrStride = 16;
 constexpr uint32_t kCompactThreads = 256;
 constexpr uint32_t kCompactSpan = 4096;
 
-// Stage flags: flag[slot] = (pass << 2) | kind names the stage a path goes
-// to next in that pass; the compaction kernels gather each kind into its
-// queue in the order of the pass's extension queue.  Flags of earlier passes are stale and ignored;
+// Stage flags: flag[slot] = (pass << kFlagBits) | kind names the stage a
+// path goes to next in that pass; the compaction kernels gather each kind
+// into its queue in slot order.  Flags of earlier passes are stale and ignored;
 // raygen clears every slot of a chunk.
+constexpr uint32_t kFlagBits = 3;
 constexpr uint32_t kFlagHit = 1;      // surface shading
 constexpr uint32_t kFlagMedium = 2;   // free flight
 constexpr uint32_t kFlagVolume = 3;   // volume-vertex shading
+constexpr uint32_t kFlagDiffuse = 4;  // surface shading, diffuse-only material
 constexpr uint32_t kMaxPasses = 4 * kMaxRayDepth;
 
 // Path-tracing traversals keep the top kStackWindow stack entries per lane in
@@ -75,10 +79,10 @@ struct PathState {
     uint32_t *sh_flags;
     float4 *out;        // final radiance per path
     uint32_t *queue[2];
-    uint32_t *q_hit, *q_hit_vol, *q_medium, *q_shadow, *q_transmit;
+    uint32_t *q_hit, *q_hit_vol, *q_hit_diffuse, *q_medium, *q_shadow, *q_transmit;
     uint32_t *spill;    // traversal stack spill [kMaxStackDepth - kStackWindow][kSpillThreads]
     uint32_t *flag;       // stage flag per slot (kFlag*)
-    uint32_t *ccount;     // compaction: matches per block, [2][cap / kCompactSpan + 1]
+    uint32_t *ccount;     // compaction: matches per block, [3][cap / kCompactSpan + 1]
     uint32_t *first_hit;  // instance hit by the camera ray (written when an id image is requested)
     uint4 *adj_rec;       // adjoint: [ray_depth][cap] (material, radiance before the vertex)
     uint32_t *adj_n;      // adjoint: records per path

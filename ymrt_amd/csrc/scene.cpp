@@ -84,6 +84,23 @@ uint32_t lambert_only(const DevMaterial &m)
            m.coat_weight_tex == kEmpty;
 }
 
+// A DefaultMtl whose coat, metal, specular and transmission weights are 0
+// (untextured), with no normal or bump map and a coat that cannot overflow
+// (albedo in [0, 1], finite thickness): InitBSDFSettings then leaves the
+// diffuse lobe alone, with the albedo itself as its colour, bit for bit, so
+// a shading variant with only that lobe gives the same floats.
+uint32_t diffuse_only(const DevMaterial &m)
+{
+    bool ok = m.type == YMRT_MATERIAL_DEFAULT && m.coat_weight == 0.0f && m.coat_weight_tex == kEmpty &&
+              m.metalness == 0.0f && m.metalness_tex == kEmpty && m.specular_weight == 0.0f &&
+              m.specular_weight_tex == kEmpty && m.transmission_weight == 0.0f &&
+              m.transmission_weight_tex == kEmpty && m.coat_albedo_tex == kEmpty &&
+              m.coat_thickness_tex == kEmpty && m.normal_mapping_tex == kEmpty && m.bump_mapping_tex == kEmpty &&
+              std::isfinite(m.coat_thickness) && m.coat_thickness >= 0.0f && std::isfinite(m.ior_n);
+    for (int c = 0; c < 3; ++c) ok = ok && m.coat_albedo[c] >= 0.0f && m.coat_albedo[c] <= 1.0f;
+    return ok ? 1u : 0u;
+}
+
 // Texture slots of a material, all of which must name a texture or be empty.
 bool material_textures_valid(const DevMaterial &m, uint32_t texture_count)
 {
@@ -645,6 +662,9 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
         const uint32_t *d = nullptr;
         if ((rc = upload(sc, lam, &d))) return rc;
         sc->adj_mat = d;
+        std::vector<uint32_t> dif(h.materials.size());
+        for (size_t i = 0; i < h.materials.size(); ++i) dif[i] = diffuse_only(h.materials[i]);
+        if ((rc = upload(sc, dif, &ds.mat_diffuse))) return rc;
     }
     sc->materials = h.materials;
     sc->texture_count = d->texture_count;
@@ -704,11 +724,16 @@ int update_materials(ymrt_scene *sc, uint32_t first, uint32_t count, const ymrt_
         YM_REQUIRE(material_textures_valid(dm, sc->texture_count), "material texture index out of range");
         next[first + i] = dm;
     }
-    std::vector<uint32_t> lam(count);
-    for (uint32_t i = 0; i < count; ++i) lam[i] = lambert_only(next[first + i]);
+    std::vector<uint32_t> lam(count), dif(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        lam[i] = lambert_only(next[first + i]);
+        dif[i] = diffuse_only(next[first + i]);
+    }
     YM_HIP(hipMemcpyAsync(const_cast<DevMaterial *>(sc->dev.materials) + first, next.data() + first,
                           sizeof(DevMaterial) * count, hipMemcpyHostToDevice, s));
     YM_HIP(hipMemcpyAsync(const_cast<uint32_t *>(sc->adj_mat) + first, lam.data(), sizeof(uint32_t) * count,
+                          hipMemcpyHostToDevice, s));
+    YM_HIP(hipMemcpyAsync(const_cast<uint32_t *>(sc->dev.mat_diffuse) + first, dif.data(), sizeof(uint32_t) * count,
                           hipMemcpyHostToDevice, s));
     YM_HIP(hipStreamSynchronize(s));
     sc->materials.swap(next);
