@@ -54,6 +54,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
     ap.add_argument("--no-batch", action="store_true", help="skip the multi-scene batch timing")
+    ap.add_argument("--capacity", type=int, default=0, help="paths in flight per chunk (0: library default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before rendering, copy MIB MiB device-to-device once (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -204,7 +205,8 @@ def main():
     sc = ymrt_amd.Scene(ctx, sa)
     info = sc.info()
     rs = abi.render_settings(args.width, args.height, args.spp, args.depth, volume=1,
-                             sampler=abi.SAMPLER_HALTON if args.sampler == "halton" else abi.SAMPLER_PCG)
+                             sampler=abi.SAMPLER_HALTON if args.sampler == "halton" else abi.SAMPLER_PCG,
+                             path_capacity=args.capacity)
     noise = torch.empty((args.height, args.width, 4), dtype=torch.float32, device="cuda")
     print("scene: %d tris, %d wide nodes, %.1f MB in HBM, build %.0f ms (gen %.1f s); stack %d"
           % (info.triangle_count, info.node_count, info.device_bytes / 1e6, info.build_ms, gen_s, info.max_stack),

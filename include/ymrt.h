@@ -234,7 +234,9 @@ typedef struct ymrt_render_settings {
     float indirect_limit;
     float gamma;
     float exposure;
-    uint32_t path_capacity;          /* paths in flight; 0 = library default  */
+    uint32_t path_capacity;          /* paths in flight per chunk; 0: up to 64 M,
+                                        within a quarter of free device memory
+                                        (about 720 B per path)                */
 } ymrt_render_settings;
 
 typedef struct ymrt_frame_stats {

@@ -337,3 +337,26 @@ def test_render_batch_equals_single_frames(gpu_ctx):
     bad[2] = (tb, abi.render_settings(32, 32, 2, 4), 0)
     with pytest.raises(ymrt_amd.YmrtError, match="batch item 2"):
         gpu_ctx.render_batch(bad)
+
+
+def test_diffuse_only_shading_equals_general(gpu_ctx):
+    """Hits on diffuse-only materials shade in a variant with the diffuse lobe
+    alone; it must give the general variant's floats.  The same Cornell box
+    with the walls' coat thickness read from a constant texture (the same
+    value, which keeps the walls out of the diffuse-only class) renders the
+    same image bit for bit."""
+    from ymrt_amd import abi as T
+    sa = scenes.cornell_box(48, 40)
+    sb = scenes.cornell_box(48, 40)
+    one = sb.add_texture_node(T.TEXTURE_CONSTANT_FLOAT, value=(1.0, 0.0, 0.0))
+    walls = 0
+    for m in sb.materials:
+        if m.type == T.MATERIAL_DEFAULT and m.specular_weight == 0.0 and m.metalness == 0.0:
+            assert m.coat_thickness == 1.0
+            m.coat_thickness_tex = one
+            walls += 1
+    assert walls >= 3
+    rs = abi.render_settings(48, 40, 8, 6)
+    a = ymrt_amd.Scene(gpu_ctx, sa).render_frame(rs, 1)[0]
+    b = ymrt_amd.Scene(gpu_ctx, sb).render_frame(rs, 1)[0]
+    assert torch.equal(a, b)
