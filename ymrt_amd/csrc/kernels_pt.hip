@@ -86,74 +86,6 @@ __device__ __forceinline__ void run_items(uint32_t *fetch, uint32_t n, uint32_t 
     }
 }
 
-// Sorts the first n of kN slot ids in LDS ascending (bitonic network over kN,
-// the tail padded with ~0u) with one wave; a batch published in slot order
-// lets the next stage's lanes read neighbouring path-state entries together.
-template <uint32_t kN>
-__device__ __forceinline__ void wave_sort_lds(uint32_t *s, uint32_t n)
-{
-    for (uint32_t j = n + lane_id(); j < kN; j += kBlock) s[j] = ~0u;
-    __syncthreads();
-#pragma unroll 1
-    for (uint32_t k = 2; k <= kN; k <<= 1) {
-#pragma unroll 1
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t t = lane_id(); t < kN / 2; t += kBlock) {
-                const uint32_t i = 2u * t - (t & (j - 1u)), l = i + j;
-                const uint32_t a = s[i], b = s[l];
-                if ((a > b) == ((i & k) == 0u)) {
-                    s[i] = b;
-                    s[l] = a;
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
-
-// Output queues of a lane-refill kernel: survivors are staged in LDS as lanes
-// finish and published kStageCap at a time (sorted by slot, one atomic per
-// queue).  filled[] is wave-uniform, so every call sits in uniform control flow.
-constexpr uint32_t kStageCap = kSub * kBlock;
-template <int kQ>
-struct StagedQueues {
-    static constexpr int kN = kQ > 0 ? kQ : 1;
-    uint32_t *stage;  // LDS [kN][kStageCap]
-    uint32_t *const *ctrs;
-    uint32_t *const *qs;
-    uint32_t filled[kN];
-
-    template <class QC>
-    __device__ __forceinline__ void flush(QC)
-    {
-        constexpr int Q = QC::value;
-        __syncthreads();
-        if (filled[Q]) {
-            wave_sort_lds<kStageCap>(stage + Q * kStageCap, filled[Q]);
-            uint32_t dst = 0;
-            if (lane_id() == 0) dst = atomicAdd(ctrs[Q], filled[Q]);
-            dst = __shfl(dst, 0);
-            for (uint32_t j = lane_id(); j < filled[Q]; j += kBlock) qs[Q][dst + j] = stage[Q * kStageCap + j];
-            filled[Q] = 0;
-        }
-        __syncthreads();
-    }
-    __device__ __forceinline__ void append(const bool (&out)[kN], uint32_t p)
-    {
-        const uint64_t below = (1ull << lane_id()) - 1ull;
-        static_for<0, kQ>([&](auto Q) {
-            const uint64_t m = __ballot(out[Q()]);
-            if (out[Q()]) stage[Q() * kStageCap + filled[Q()] + (uint32_t)__popcll(m & below)] = p;
-            filled[Q()] += (uint32_t)__popcll(m);
-            if (filled[Q()] > kStageCap - kBlock) flush(Q);
-        });
-    }
-    __device__ __forceinline__ void flush_all()
-    {
-        static_for<0, kQ>([&](auto Q) { flush(Q); });
-    }
-};
-
 // Lane refill: a kernel that keeps one traversal per lane lets finished lanes
 // take new items while the others stay where they are in their trees, once
 // kRefillLanes lanes are done.  Ray lengths vary by orders of magnitude, so a
@@ -518,22 +450,27 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere 
 // each block of kCompactSpan slots, k_compact_scatter writes a block's
 // matches at the sum of the counts before it, and the block holding the last
 // slot sets the queue lengths.
-template <bool kShade>
-__device__ __forceinline__ void compact_match(uint32_t f, uint32_t tag, bool (&m)[kShade ? 3 : 1])
-{
-    if constexpr (kShade) {
-        m[0] = f == (tag | kFlagHit);
-        m[1] = f == (tag | kFlagVolume);
-        m[2] = f == (tag | kFlagDiffuse);
-    } else {
-        m[0] = f == (tag | kFlagMedium);
+// kMode: kCompactMedium, kCompactShade (three queues) or kCompactTransmit
+enum : int { kCompactMedium = 0, kCompactShade = 1, kCompactTransmit = 2 };
+template <int kMode>
+struct Compact {
+    static constexpr int kQ = kMode == kCompactShade ? 3 : 1;
+    __device__ static void match(uint32_t f, uint32_t tag, bool (&m)[kQ])
+    {
+        if constexpr (kMode == kCompactShade) {
+            m[0] = f == (tag | kFlagHit);
+            m[1] = f == (tag | kFlagVolume);
+            m[2] = f == (tag | kFlagDiffuse);
+        } else {
+            m[0] = f == (tag | (kMode == kCompactMedium ? kFlagMedium : kFlagTransmit));
+        }
     }
-}
+};
 
-template <bool kShade>
+template <int kMode>
 __global__ __launch_bounds__(kCompactThreads) void k_compact_count(PtParams prm, PathState ps, uint32_t pass)
 {
-    constexpr int kQ = kShade ? 3 : 1;
+    constexpr int kQ = Compact<kMode>::kQ;
     constexpr uint32_t kWaves = kCompactThreads / kBlock;
     __shared__ uint32_t red[kQ][kWaves];
     const uint32_t n = prm.chunk_paths;
@@ -544,7 +481,7 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_count(PtParams prm,
     uint32_t c[kQ] = {};
     for (uint32_t i = first + threadIdx.x; i < last; i += kCompactThreads) {
         bool m[kQ];
-        compact_match<kShade>(ps.flag[i], tag, m);
+        Compact<kMode>::match(ps.flag[i], tag, m);
         static_for<0, kQ>([&](auto Q) { c[Q()] += m[Q()] ? 1u : 0u; });
     }
     static_for<0, kQ>([&](auto Q) {
@@ -560,10 +497,10 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_count(PtParams prm,
     }
 }
 
-template <bool kShade>
+template <int kMode>
 __global__ __launch_bounds__(kCompactThreads) void k_compact_scatter(PtParams prm, PathState ps, uint32_t pass)
 {
-    constexpr int kQ = kShade ? 3 : 1;
+    constexpr int kQ = Compact<kMode>::kQ;
     constexpr uint32_t kWaves = kCompactThreads / kBlock;
     __shared__ uint32_t red[kQ][kWaves];
     __shared__ uint32_t base[kQ];
@@ -573,8 +510,12 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_scatter(PtParams pr
     if (first >= n) return;
     const uint32_t last = min(first + kCompactSpan, n);
     const uint32_t stride = prm.cap / kCompactSpan + 1;
-    uint32_t *const qs[3] = {kShade ? ps.q_hit : ps.q_medium, ps.q_hit_vol, ps.q_hit_diffuse};
-    const uint32_t ctr[3] = {kShade ? kCtrHit : kCtrMedium, kCtrHitVol, kCtrHitDiffuse};
+    uint32_t *const qs[3] = {kMode == kCompactShade    ? ps.q_hit
+                             : kMode == kCompactMedium ? ps.q_medium
+                                                       : ps.q_transmit,
+                             ps.q_hit_vol, ps.q_hit_diffuse};
+    const uint32_t ctr[3] = {kMode == kCompactShade ? kCtrHit : kMode == kCompactMedium ? kCtrMedium : kCtrTransmit,
+                             kCtrHitVol, kCtrHitDiffuse};
     const uint32_t tag = pass << kFlagBits;
     const uint32_t wave = threadIdx.x / kBlock;
     const uint64_t below = (1ull << lane_id()) - 1ull;
@@ -600,7 +541,7 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_scatter(PtParams pr
         bool m[kQ] = {};
         if (i < last) {
             p = i;
-            compact_match<kShade>(ps.flag[p], tag, m);
+            Compact<kMode>::match(ps.flag[p], tag, m);
         }
         uint32_t rank[kQ];
         __syncthreads();  // red[] of the previous tile has been read
@@ -1165,9 +1106,7 @@ __device__ __forceinline__ void shadow_finish(const PtParams &prm, const PathSta
 template <bool kMedia, bool kSphere, int kStack>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_shadow(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
 {
-    constexpr int kQ = kMedia ? 1 : 0;
     __shared__ uint32_t stack[kStackWindow * kBlock];
-    __shared__ uint32_t stage[(kQ > 0 ? kQ : 1) * kStageCap];
     __shared__ float pre_park[kPreSlots * kBlock];
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     const uint32_t cap = prm.cap;
@@ -1175,9 +1114,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                                                kSpillThreads, blockIdx.x * kBlock + threadIdx.x, pre_park};
     uint32_t nsh = 0;
     TraceCounters tc{0, 0};
-    uint32_t *const ctrs[1] = {cnt + kCtrTransmit};
-    uint32_t *const qs[1] = {ps.q_transmit};
-    StagedQueues<kQ> sq{stage, ctrs, qs, {}};
     RefillRange<16> rr{cnt + kCtrWorkShadow, cnt[kCtrShadow], 0u, 0u, true};
 
     // one lane walks its path's recorded rays (slots k) in order
@@ -1211,7 +1147,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             if (busy == 0 || (uint32_t)(kBlock - __popcll(busy)) >= want) break;
             wave_step<false>(tv, r, sc, ls, tc);
         }
-        bool out[1] = {false};
         // finished rays: note the result, start the path's next ray or
         // finish the path (a ray can end in init, hence the loop)
         while (r != kTravContinue && r != kLaneIdle) {
@@ -1225,15 +1160,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             } else {
                 r = kLaneIdle;
                 const bool walk = kMedia && (fl & kShClearMask);
-                out[0] = walk;
-                if (walk) ps.sh_flags[p] = fl;
-                else shadow_finish(prm, ps, p, fl, sc.distant_count, false);
+                if (walk) {
+                    ps.sh_flags[p] = fl;
+                    ps.flag[p] = (pass << kFlagBits) | kFlagTransmit;
+                } else {
+                    shadow_finish(prm, ps, p, fl, sc.distant_count, false);
+                }
             }
         }
-        if constexpr (kQ > 0) sq.append(out, p);
         if (!rr.more && __ballot(r != kLaneIdle) == 0) break;
     }
-    sq.flush_all();
     wave_add64(prm.stats + kStatShadow, nsh);
     wave_add64(prm.stats + kStatShadowNodes, tc.nodes);
     wave_add64(prm.stats + kStatShadowTris, tc.tris);
@@ -1482,15 +1418,18 @@ int launch_medium(const DevScene &sc, const PtParams &prm, const PathState &ps, 
     return YMRT_OK;
 }
 
-int launch_compact(const PtParams &prm, const PathState &ps, uint32_t pass, bool shade, hipStream_t s)
+int launch_compact(const PtParams &prm, const PathState &ps, uint32_t pass, int mode, hipStream_t s)
 {
     const dim3 g((prm.chunk_paths + kCompactSpan - 1) / kCompactSpan), b(kCompactThreads);
-    if (shade) {
-        hipLaunchKernelGGL(k_compact_count<true>, g, b, 0, s, prm, ps, pass);
-        hipLaunchKernelGGL(k_compact_scatter<true>, g, b, 0, s, prm, ps, pass);
+    if (mode == kCompactShade) {
+        hipLaunchKernelGGL(k_compact_count<kCompactShade>, g, b, 0, s, prm, ps, pass);
+        hipLaunchKernelGGL(k_compact_scatter<kCompactShade>, g, b, 0, s, prm, ps, pass);
+    } else if (mode == kCompactMedium) {
+        hipLaunchKernelGGL(k_compact_count<kCompactMedium>, g, b, 0, s, prm, ps, pass);
+        hipLaunchKernelGGL(k_compact_scatter<kCompactMedium>, g, b, 0, s, prm, ps, pass);
     } else {
-        hipLaunchKernelGGL(k_compact_count<false>, g, b, 0, s, prm, ps, pass);
-        hipLaunchKernelGGL(k_compact_scatter<false>, g, b, 0, s, prm, ps, pass);
+        hipLaunchKernelGGL(k_compact_count<kCompactTransmit>, g, b, 0, s, prm, ps, pass);
+        hipLaunchKernelGGL(k_compact_scatter<kCompactTransmit>, g, b, 0, s, prm, ps, pass);
     }
     YM_HIP(hipGetLastError());
     return YMRT_OK;

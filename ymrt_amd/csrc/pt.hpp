@@ -39,6 +39,7 @@ constexpr uint32_t kFlagHit = 1;      // surface shading
 constexpr uint32_t kFlagMedium = 2;   // free flight
 constexpr uint32_t kFlagVolume = 3;   // volume-vertex shading
 constexpr uint32_t kFlagDiffuse = 4;  // surface shading, diffuse-only material
+constexpr uint32_t kFlagTransmit = 5; // transmittance of unoccluded shadow rays
 constexpr uint32_t kMaxPasses = 4 * kMaxRayDepth;
 
 // Path-tracing traversals keep the top kStackWindow stack entries per lane in

@@ -21,7 +21,8 @@ int launch_raygen(const DevScene &sc, const PtParams &prm, const PathState &ps, 
 int launch_extend(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool nearby,
                   bool media, uint32_t stack, hipStream_t s);
 int launch_medium(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s);
-int launch_compact(const PtParams &prm, const PathState &ps, uint32_t pass, bool shade, hipStream_t s);
+// mode: 0 free flight, 1 shading, 2 transmittance (kCompact* in kernels_pt.hip)
+int launch_compact(const PtParams &prm, const PathState &ps, uint32_t pass, int mode, hipStream_t s);
 int launch_shade(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool volume,
                  hipStream_t s);
 int launch_shadow(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool media,
@@ -253,17 +254,20 @@ int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, floa
                  })))
                 return rc;
             if (media && (rc = timed(5, [&] {
-                              int r = launch_compact(prm, rb->ps, pass, false, stream);
+                              int r = launch_compact(prm, rb->ps, pass, 0, stream);
                               return r ? r : launch_medium(sc->dev, prm, rb->ps, pass, stream);
                           })))
                 return rc;
-            if ((rc = timed(2, [&] { return launch_compact(prm, rb->ps, pass, true, stream); }))) return rc;
+            if ((rc = timed(2, [&] { return launch_compact(prm, rb->ps, pass, 1, stream); }))) return rc;
             if ((rc = timed(2, [&] { return launch_shade(sc->dev, prm, rb->ps, pass, false, stream); }))) return rc;
             if (media && (rc = timed(2, [&] { return launch_shade(sc->dev, prm, rb->ps, pass, true, stream); })))
                 return rc;
             if ((rc = timed(3, [&] { return launch_shadow(sc->dev, prm, rb->ps, pass, media, stack, stream); })))
                 return rc;
-            if (media && (rc = timed(6, [&] { return launch_transmit(sc->dev, prm, rb->ps, pass, stream); })))
+            if (media && (rc = timed(6, [&] {
+                              int r = launch_compact(prm, rb->ps, pass, 2, stream);
+                              return r ? r : launch_transmit(sc->dev, prm, rb->ps, pass, stream);
+                          })))
                 return rc;
             ++pass;
             if (pass == next_check || pass >= kMaxPasses) {
