@@ -139,14 +139,14 @@ struct RefillRange {
 // Node visits and leaf tests are two long code paths; while at least
 // kInnerFirst lanes are at wide nodes, only those lanes step, so each path
 // runs with a fuller wave instead of both running under partial masks; with
-// fewer at nodes every lane steps, so no lane waits for long.  24 measured
-// best on the interior frame (8: slower than stepping every lane; 12, 16,
-// 32: less gain).  kLoop: the node visits run as a loop of their own until
-// fewer than kInnerFirst lanes are at nodes (the ray setup then stays in
-// place, which saves register copies per visit); faster for closest-hit
-// rays, slower for shadow rays, which end at their first hit.
-constexpr int kInnerFirst = 24;
-template <bool kLoop, class TV, class LS>
+// fewer at nodes every lane steps, so no lane waits for long.  kLoop: the
+// node visits run as a loop of their own until fewer than kInnerFirst lanes
+// are at nodes (the ray setup then stays in place, which saves register
+// copies per visit); faster for closest-hit rays, slower for shadow rays,
+// which end at their first hit.  Measured best on the interior frame:
+// kInnerFirst 16 with the loop (closest hit), 24 without (shadow rays); 8
+// and 32 were slower for both.
+template <int kInnerFirst, bool kLoop, class TV, class LS>
 __device__ __forceinline__ void wave_step(TV &tv, int &r, const DevScene &sc, LS &ls, TraceCounters &tc)
 {
     if constexpr (kLoop) {
@@ -376,7 +376,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere 
         for (;;) {
             const uint64_t busy = __ballot(r == kTravContinue);
             if (busy == 0 || (uint32_t)(kBlock - __popcll(busy)) >= want) break;
-            wave_step<true>(tv, r, sc, ls, tc);
+            wave_step<16, true>(tv, r, sc, ls, tc);
         }
         if (r != kTravContinue && r != kLaneIdle) {
             if (r == kTravOverflow) atomicOr(prm.error, 1u);
@@ -1145,7 +1145,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         for (;;) {
             const uint64_t busy = __ballot(r == kTravContinue);
             if (busy == 0 || (uint32_t)(kBlock - __popcll(busy)) >= want) break;
-            wave_step<false>(tv, r, sc, ls, tc);
+            wave_step<24, false>(tv, r, sc, ls, tc);
         }
         // finished rays: note the result, start the path's next ray or
         // finish the path (a ray can end in init, hence the loop)
