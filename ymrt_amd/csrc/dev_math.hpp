@@ -89,6 +89,62 @@ YM_HD float sqr(float x) { return x * x; }
 
 // src/MathCommon.h:103-106
 YM_HD float safe_rcp(float f) { return fabsf(f) < 1E-30f ? (f > 0.0f ? 1E30f : -1E30f) : (1.0f / f); }
+
+// Transcendentals of float arguments, evaluated in double precision and
+// rounded once: the correctly rounded float result except when the double
+// result lies within one of its own ulps of a float rounding boundary.  The
+// oracle's glibc float functions are within about half an ulp as well, so
+// both sides round to the same float nearly always; the GPU's own float
+// versions are up to a few ulps off, and the last bits of a free-flight
+// distance, a transmittance or a sampled direction decide later branches.
+YM_HD float ym_expf(float x) { return (float)exp((double)x); }
+YM_HD float ym_logf(float x) { return (float)log((double)x); }
+YM_HD float ym_sinf(float x) { return (float)sin((double)x); }
+YM_HD float ym_cosf(float x) { return (float)cos((double)x); }
+// sin and cos of an angle |x| <= 64 (the sampled azimuths, [0, 2 pi)) in
+// double precision with few registers, each rounded once to float: Cody-Waite
+// reduction by pi/2 with fdlibm's two-part pi/2 (pio2_1 has 33 significant
+// bits, so k * pio2_1 is exact for |k| < 2^20), then the Taylor series of sin
+// and cos on |r| <= pi/4 up to r^19 and r^18, whose truncation errors stay
+// below 1e-19.  The general double sin/cos carry a reduction for huge
+// arguments that costs the shading kernels spilled registers.
+YM_HD void ym_sincosf(float x, float &s, float &c)
+{
+    constexpr double kPio2Hi = 1.57079632673412561417e+00;  // fdlibm pio2_1
+    constexpr double kPio2Lo = 6.07710050650619224932e-11;  // fdlibm pio2_1t
+    const double xd = (double)x;
+    const double k = rint(xd * 0.63661977236758134308);      // 2 / pi
+    const double r = (xd - k * kPio2Hi) - k * kPio2Lo;
+    const double r2 = r * r;
+    double ps = 1.0 / 121645100408832000.0;                  // 1/19!
+    ps = ps * -r2 + 1.0 / 355687428096000.0;                 // 1/17!
+    ps = ps * -r2 + 1.0 / 1307674368000.0;                   // 1/15!
+    ps = ps * -r2 + 1.0 / 6227020800.0;                      // 1/13!
+    ps = ps * -r2 + 1.0 / 39916800.0;                        // 1/11!
+    ps = ps * -r2 + 1.0 / 362880.0;                          // 1/9!
+    ps = ps * -r2 + 1.0 / 5040.0;                            // 1/7!
+    ps = ps * -r2 + 1.0 / 120.0;                             // 1/5!
+    ps = ps * -r2 + 1.0 / 6.0;                               // 1/3!
+    const double sr = r - r * r2 * ps;
+    double pc = 1.0 / 6402373705728000.0;                    // 1/18!
+    pc = pc * -r2 + 1.0 / 20922789888000.0;                  // 1/16!
+    pc = pc * -r2 + 1.0 / 87178291200.0;                     // 1/14!
+    pc = pc * -r2 + 1.0 / 479001600.0;                       // 1/12!
+    pc = pc * -r2 + 1.0 / 3628800.0;                         // 1/10!
+    pc = pc * -r2 + 1.0 / 40320.0;                           // 1/8!
+    pc = pc * -r2 + 1.0 / 720.0;                             // 1/6!
+    pc = pc * -r2 + 1.0 / 24.0;                              // 1/4!
+    pc = pc * -r2 + 0.5;                                     // 1/2!
+    const double cr = 1.0 - r2 * pc;
+    const int q = (int)k & 3;
+    const double sd = q == 0 ? sr : q == 1 ? cr : q == 2 ? -sr : -cr;
+    const double cd = q == 0 ? cr : q == 1 ? -sr : q == 2 ? -cr : sr;
+    s = (float)sd;
+    c = (float)cd;
+}
+YM_HD float ym_powf(float x, float y) { return (float)pow((double)x, (double)y); }
+YM_HD float ym_acosf(float x) { return (float)acos((double)x); }
+YM_HD float ym_atan2f(float y, float x) { return (float)atan2((double)y, (double)x); }
 // src/MathCommon.h:151-154
 YM_HD constexpr float error_gamma(int n)
 {

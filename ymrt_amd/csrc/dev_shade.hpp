@@ -294,7 +294,7 @@ __device__ __forceinline__ f3 texture_node(const DevScene &sc, const DevTexture 
     case YMRT_TEXTURE_CONSTANT_FLOAT: return s3(t.value[0]);
     case YMRT_TEXTURE_CONSTANT_RGB: return f3{t.value[0], t.value[1], t.value[2]};
     case YMRT_TEXTURE_CHECKERBOARD: {
-        const float x = sinf(po.x * t.frequency), y = sinf(po.y * t.frequency), z = sinf(po.z * t.frequency);
+        const float x = ym_sinf(po.x * t.frequency), y = ym_sinf(po.y * t.frequency), z = ym_sinf(po.z * t.frequency);
         return x * y * z < 0.0f ? white : black;
     }
     case YMRT_TEXTURE_NOISE: {
@@ -314,7 +314,7 @@ __device__ __forceinline__ f3 texture_node(const DevScene &sc, const DevTexture 
         turb += (t.axes & 1) ? po.x : 0.0f;
         turb += (t.axes & 2) ? po.y : 0.0f;
         turb += (t.axes & 4) ? po.z : 0.0f;
-        const float m = sinf(turb + t.variation * fbm) * 0.5f + 0.5f;
+        const float m = ym_sinf(turb + t.variation * fbm) * 0.5f + 0.5f;
         return black + m * (white - black);
     }
     case YMRT_TEXTURE_WOOD: {
@@ -477,8 +477,10 @@ __device__ __forceinline__ f3 ggx_sample_wh(f3 wo, float u0, float u1, float ax,
     f3 T2 = cross(wh, T1);
     float r = sqrtf(gmax(u0, 0.0f));
     float phi = 2.0f * kPi * u1;
-    float t1 = r * cosf(phi);
-    float t2 = r * sinf(phi);
+    float sp, cp;
+    ym_sincosf(phi, sp, cp);
+    float t1 = r * cp;
+    float t2 = r * sp;
     float s = (float)(0.5 * (1.0 + (double)wh.z));
     t2 = (float)((1.0 - (double)s) * sqrt(1.0 - (double)(t1 * t1)) + (double)s * (double)t2);
     f3 Nh = t1 * T1 + t2 * T2 + sqrtf(gmax(0.0f, 1.0f - t1 * t1 - t2 * t2)) * wh;
@@ -488,7 +490,9 @@ __device__ __forceinline__ f3 sample_cosine(float u0, float u1)
 {
     float r = sqrtf(u0);
     float phi = u1 * kTwoPi;
-    float x = r * cosf(phi), y = r * sinf(phi);
+    float sp, cp;
+    ym_sincosf(phi, sp, cp);
+    float x = r * cp, y = r * sp;
     float r2 = x * x + y * y;
     return f3{x, y, r2 < 1.0f ? sqrtf(1.0f - r2) : 0.0f};
 }
@@ -838,10 +842,10 @@ __device__ __forceinline__ void init_default_bsdf(TX &tx, Bsdf &b, const DevMate
     const float cay = sqr(gclamp(tex_float(tx, m.coat_roughness_y_tex, m.coat_roughness_y), 0.001f, 0.999f));
     const f3 coat_alb = tex_color(tx, m.coat_albedo_tex, m.coat_albedo);
     const f3 ca = max3(coat_alb, s3(1E-8f));
-    const f3 sigma = (-f3{logf(ca.x), logf(ca.y), logf(ca.z)}) * coat_th;
+    const f3 sigma = (-f3{ym_logf(ca.x), ym_logf(ca.y), ym_logf(ca.z)}) * coat_th;
     const float rc = safe_rcp(cos_in);
     const f3 ns = -sigma;
-    const f3 coat_tr{expf(ns.x * rc), expf(ns.y * rc), expf(ns.z * rc)};
+    const f3 coat_tr{ym_expf(ns.x * rc), ym_expf(ns.y * rc), ym_expf(ns.z * rc)};
     b.ni = ior_in;
     b.ax = ax;
     b.ay = ay;
@@ -998,7 +1002,9 @@ __device__ __forceinline__ void hg_sample(float g, float u0, float u1, f3 wo, f3
     float phi = 2.f * kPi * u1;
     f3 w = wo, u, v;
     orthogonal_basis(w, u, v);
-    wi = u * s * cosf(phi) + v * s * sinf(phi) + w * c;
+    float sp, cp;
+    ym_sincosf(phi, sp, cp);
+    wi = u * s * cp + v * s * sp + w * c;
     hg_eval(g, wo, wi, pdf);
 }
 // SampleMixedPhases (src/VolumeEval.cuh:470-514); loops over the six fixed
@@ -1083,7 +1089,7 @@ __device__ __forceinline__ bool vol_is_het(const DevScene &sc, int v)
 
 __device__ __forceinline__ f3 vsig(const DevVolume &v) { return f3{v.sigma_t[0], v.sigma_t[1], v.sigma_t[2]}; }
 __device__ __forceinline__ f3 valb(const DevVolume &v) { return f3{v.albedo[0], v.albedo[1], v.albedo[2]}; }
-YM_HD f3 exp3(f3 a) { return f3{expf(a.x), expf(a.y), expf(a.z)}; }
+YM_HD f3 exp3(f3 a) { return f3{ym_expf(a.x), ym_expf(a.y), ym_expf(a.z)}; }
 
 // Random source for the shadow-ray transmittance of grid media: PCG hash of
 // (pixel, vertex, sample, frame) and a per-vertex counter.
@@ -1128,10 +1134,10 @@ __device__ __forceinline__ bool sample_homogeneous(const DevScene &sc, int n, co
         u0 = (u0 - cw.x - cw.y) * safe_rcp(cw.z);
     }
     const float sc_ = comp(st, ch);
-    const float fpl = sc_ > 0.0f ? (-logf(1.0f - u0) / sc_) : kFloatMax;
+    const float fpl = sc_ > 0.0f ? (-ym_logf(1.0f - u0) / sc_) : kFloatMax;
     const bool scatter = fpl < tmax;
     const float m = gmin(tmax, fpl);
-    const f3 tr{expf(-st.x * m), expf(-st.y * m), expf(-st.z * m)};
+    const f3 tr{ym_expf(-st.x * m), ym_expf(-st.y * m), ym_expf(-st.z * m)};
     const float pdf = scatter ? (cw.x * st.x * tr.x + cw.y * st.y * tr.y + cw.z * st.z * tr.z)
                               : (cw.x * tr.x + cw.y * tr.y + cw.z * tr.z);
     if (!(fabsf(pdf) > 0.0f)) return false;
@@ -1172,10 +1178,10 @@ __device__ __forceinline__ bool sample_heterogeneous(const DevScene &sc, int n, 
     int it = 0;
     for (;;) {
         if (it >= kMaxFreePathLength) break;
-        const float fpl = -logf(1.0f - sampler_next(sc, s)) / comp(sm, ch);
+        const float fpl = -ym_logf(1.0f - sampler_next(sc, s)) / comp(sm, ch);
         if (acc + fpl > tmax) {
             const float dd = tmax - acc;
-            const f3 ptr{expf(-sm.x * dd), expf(-sm.y * dd), expf(-sm.z * dd)};
+            const f3 ptr{ym_expf(-sm.x * dd), ym_expf(-sm.y * dd), ym_expf(-sm.z * dd)};
             tr = tr * (ptr * nf);
             pdf = pdf * (ptr * nf);
             acc += fpl;
@@ -1197,7 +1203,7 @@ __device__ __forceinline__ bool sample_heterogeneous(const DevScene &sc, int n, 
         rp.y = gmin(sm.y > 0.0f ? st.y * smr.y : 0.0f, 1.0f);
         rp.z = gmin(sm.z > 0.0f ? st.z * smr.z : 0.0f, 1.0f);
         const f3 fp = s3(1.0f) - rp;
-        const f3 ptr{expf(-sm.x * fpl), expf(-sm.y * fpl), expf(-sm.z * fpl)};
+        const f3 ptr{ym_expf(-sm.x * fpl), ym_expf(-sm.y * fpl), ym_expf(-sm.z * fpl)};
         if (sampler_next(sc, s) < comp(rp, ch)) {
             tr = tr * ((ptr * ss) * nf);
             pdf = pdf * (((ptr * sm) * rp) * nf);
@@ -1229,7 +1235,7 @@ __device__ __forceinline__ f3 eval_tr(const DevScene &sc, f3 ro_w, f3 rd_w, cons
 #pragma unroll
         for (int i = 0; i < kMaxBoundaryRecord; ++i)
             if (i < n) st = st + vsig(sc.volumes[vi[i]]);
-        return f3{expf(-st.x * tmax), expf(-st.y * tmax), expf(-st.z * tmax)};
+        return f3{ym_expf(-st.x * tmax), ym_expf(-st.y * tmax), ym_expf(-st.z * tmax)};
     }
     bool het = false;
 #pragma unroll
@@ -1240,7 +1246,7 @@ __device__ __forceinline__ f3 eval_tr(const DevScene &sc, f3 ro_w, f3 rd_w, cons
 #pragma unroll
         for (int i = 0; i < kMaxBoundaryRecord; ++i)
             if (i < n) st = st + vsig(sc.volumes[vi[i]]);
-        return f3{expf(-st.x * tmax), expf(-st.y * tmax), expf(-st.z * tmax)};
+        return f3{ym_expf(-st.x * tmax), ym_expf(-st.y * tmax), ym_expf(-st.z * tmax)};
     }
     f3 ro[kMaxBoundaryRecord], rd[kMaxBoundaryRecord];
     f3 sm = s3(0.0f);
@@ -1268,10 +1274,10 @@ __device__ __forceinline__ f3 eval_tr(const DevScene &sc, f3 ro_w, f3 rd_w, cons
     int it = 0;
     for (;;) {
         if (it >= kMaxFreePathLength) break;
-        const float fpl = -logf(1.0f - rng.next()) / comp(sm, ch);
+        const float fpl = -ym_logf(1.0f - rng.next()) / comp(sm, ch);
         if (acc + fpl > tmax) {
             const float dd = tmax - acc;
-            const f3 ptr{expf(-sm.x * dd), expf(-sm.y * dd), expf(-sm.z * dd)};
+            const f3 ptr{ym_expf(-sm.x * dd), ym_expf(-sm.y * dd), ym_expf(-sm.z * dd)};
             tr = tr * (ptr * nf);
             pdf = pdf * (ptr * nf);
             break;
@@ -1291,7 +1297,7 @@ __device__ __forceinline__ f3 eval_tr(const DevScene &sc, f3 ro_w, f3 rd_w, cons
         rp.y = gmin(sm.y > 0.0f ? st.y * smr.y : 0.0f, 1.0f);
         rp.z = gmin(sm.z > 0.0f ? st.z * smr.z : 0.0f, 1.0f);
         const f3 fp = s3(1.0f) - rp;
-        const f3 ptr{expf(-sm.x * fpl), expf(-sm.y * fpl), expf(-sm.z * fpl)};
+        const f3 ptr{ym_expf(-sm.x * fpl), ym_expf(-sm.y * fpl), ym_expf(-sm.z * fpl)};
         tr = tr * (((ptr * sm) * fp) * nf);
         pdf = pdf * ((ptr * sm) * nf);
         if (it + 1 > 32) {
@@ -1368,7 +1374,9 @@ YM_HD f3 sample_cone(float u0, float u1, float cos_theta_max)
     const float ct = (1.0f - u0) + u0 * cos_theta_max;
     const float st = sqrtf(gmax(0.0f, 1.0f - ct * ct));
     const float phi = u1 * 2.0f * kPi;
-    return f3{cosf(phi) * st, sinf(phi) * st, ct};
+    float sp, cp;
+    ym_sincosf(phi, sp, cp);
+    return f3{cp * st, sp * st, ct};
 }
 
 YM_HD f3 light_emission(const DevScene &sc, const DevInstance &in)

@@ -248,6 +248,10 @@ constexpr uint32_t kShVolume = 1u << 12;
 constexpr uint32_t kShTerminate = 1u << 13;
 constexpr uint32_t kShDepthShift = 16;
 
+// indirect_light_clamp (src/Shading.cu:61-81).  Its powers use the GPU's
+// float powf: the double-precision form (ym_powf) would raise the registers
+// of every kernel that deposits radiance, for a setting that is off in the
+// measured workloads.
 __device__ __forceinline__ f3 light_clamp(const ymrt_render_settings &rs, f3 c, int depth)
 {
     if (!(rs.enable_indirect_clamp && depth > 1)) return c;
@@ -858,8 +862,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGraph ?
                     f2 uv;
                     if (is_sphere) {
                         sn = ng;
-                        const float theta = acosf(ng.y);
-                        float phi = atan2f(ng.z, ng.x);
+                        const float theta = ym_acosf(ng.y);
+                        float phi = ym_atan2f(ng.z, ng.x);
                         if (phi < 0.0f) phi += kTwoPi;
                         uv = f2{0.5f * phi * kInvPi, theta * kInvPi};
                         float sin_theta = 0.0f;
@@ -1349,7 +1353,7 @@ __global__ void k_tonemap(float4 *img, uint32_t n, float gamma, float exposure)
     for (int k = 0; k < 3; ++k) {
         const float x = c[k] * exposure;
         const float y = (x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f);
-        c[k] = powf(y, 1.0f / gamma);
+        c[k] = ym_powf(y, 1.0f / gamma);
     }
     img[i] = make_float4(c[0], c[1], c[2], 1.0f);
 }
