@@ -50,7 +50,7 @@ def test_cornell_halton_parity(gpu_ctx):
     err = rel_rmse(g, o)
     print("close pixels %.4f  rel rmse %.5f  stats ext %d shadow %d" %
           (same, err, stats.extension_rays, stats.shadow_rays))
-    assert same >= 0.95
+    assert same >= 0.99
     assert err <= 0.01
 
 
@@ -58,8 +58,9 @@ def test_cornell_pcg_parity(gpu_ctx):
     sa = scenes.cornell_box(48, 32)
     rs = abi.render_settings(48, 32, 4, 6, sampler=abi.SAMPLER_PCG)
     g, o, _ = render_both(gpu_ctx, sa, rs, frame=3)
-    assert close_frac(g, o) >= 0.90
-    assert rel_rmse(g, o) <= 0.02
+    print("pcg cornell: close pixels %.4f  rel rmse %.5f" % (close_frac(g, o), rel_rmse(g, o)))
+    assert close_frac(g, o) >= 0.99
+    assert rel_rmse(g, o) <= 0.01
 
 
 def test_deterministic_frames(gpu_ctx):
@@ -106,9 +107,10 @@ def test_interior_fog_parity(gpu_ctx):
     assert np.isfinite(g).all()
     same, err = _media_report("interior fog", g, o, stats)
     assert stats.transmittance_rays > 0
-    # every fog segment runs expf/logf, whose last bits differ between the GPU
-    # and glibc, so fewer pixels agree to 1e-4 while the image error stays low
-    assert same >= 0.80
+    # every fog segment runs exp/log; the GPU's (double, rounded once) and
+    # glibc's float functions differ in rare last bits, which move a few
+    # free-flight distances and so a few more pixels than elsewhere
+    assert same >= 0.95
     assert err <= 0.01
 
 
@@ -121,7 +123,7 @@ def test_cornell_medium_parity(gpu_ctx, grid):
     g, o, stats = render_both(gpu_ctx, sa, rs)
     assert np.isfinite(g).all()
     same, err = _media_report("cornell medium grid=%d" % grid, g, o, stats)
-    assert same >= 0.95
+    assert same >= 0.99
     assert err <= 0.02
 
 
@@ -137,7 +139,7 @@ def test_cornell_sphere_parity(gpu_ctx, kw):
     assert np.isfinite(g).all()
     same, err = close_frac(g, o), rel_rmse(g, o)
     print("cornell %s: close pixels %.4f  rel rmse %.5f" % (kw, same, err))
-    assert same >= 0.90
+    assert same >= 0.99
     assert err <= 0.01
 
 
@@ -153,9 +155,9 @@ def test_outdoor_distant_light_parity(gpu_ctx, fog):
     same, err = close_frac(g, o), rel_rmse(g, o)
     print("outdoor fog=%d: close pixels %.4f  rel rmse %.5f" % (fog, same, err))
     # the ground plane lies at y = 0, where the spawn offset is zero: shadow
-    # rays that graze it hit or miss it on their last bits, so the fog case
-    # (expf/logf last bits as above) agrees on fewer pixels
-    assert same >= (0.90 if fog else 0.95)
+    # rays that graze it hit or miss it on their last bits, so any last-bit
+    # difference upstream (the fog's exp/log as above) shows in more pixels
+    assert same >= (0.98 if fog else 0.99)
     assert err <= 0.01
 
 
@@ -194,7 +196,7 @@ def test_textured_box_parity(gpu_ctx):
     assert np.isfinite(g).all()
     same, err = close_frac(g, o), rel_rmse(g, o)
     print("textured box: close pixels %.4f  rel rmse %.5f" % (same, err))
-    assert same >= 0.90
+    assert same >= 0.99
     assert err <= 0.01
 
 
@@ -217,7 +219,7 @@ def test_ymtmp_cornell_fixture_parity(gpu_ctx):
     same, err = close_frac(g, o), rel_rmse(g, o)
     print("ymtmp cornell: close pixels %.4f  rel rmse %.5f  mean %.4f" % (same, err, float(o[..., :3].mean())))
     assert float(o[..., :3].mean()) > 0.01
-    assert same >= 0.90
+    assert same >= 0.99
     assert err <= 0.01
 
 
