@@ -54,6 +54,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
     ap.add_argument("--no-batch", action="store_true", help="skip the multi-scene batch timing")
+    ap.add_argument("--no-traversal", action="store_true", help="skip the 2 M-triangle traversal timing")
     ap.add_argument("--capacity", type=int, default=0, help="paths in flight per chunk (0: library default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before rendering, copy MIB MiB device-to-device once (the known-size kernel that "
@@ -134,6 +135,48 @@ def batch_timing(ctx, count=16, seed=11):
             "sequential_ms": round((t1 - t0) * 1e3, 2), "batch_ms": round((t2 - t1) * 1e3, 2),
             "speedup": round((t1 - t0) / (t2 - t1), 3), "batch_mrays_per_s": round(rays / (t2 - t1) / 1e6, 2),
             "batch_samples_per_s": round(samples / (t2 - t1), 1)}
+
+
+def traversal_timing(ctx, reps=3):
+    """The traversal workload: a 2 M-triangle heightfield (1001^2 vertices)
+    and 4 M rays (origins uniform in its box, directions uniform on the
+    sphere; any-hit segments to uniform points), through ymrt_trace_closest
+    and ymrt_trace_any, timed with HIP events on the stream.  Algorithmic
+    bytes per closest-hit ray: 32 B ray + 8 B hit + 128 B per visited wide
+    node + 48 B per tested triangle (the counted variant gives the visits)."""
+    import torch
+
+    import ymrt_amd
+    from ymrt_amd import scenes
+    sa = scenes.heightfield(1001)
+    sc = ymrt_amd.Scene(ctx, sa)
+    closest, shadow = scenes.stress_rays(sa, 1 << 22)
+    rc = ymrt_amd.Scene.rays_to_device(closest)
+    rsh = ymrt_amd.Scene.rays_to_device(shadow)
+    n = rc.shape[0]
+    _, visits = sc.trace_closest(rc, counted=True)
+    v = visits.double().sum(0).cpu().numpy()
+    sc.trace_closest(rc)
+    sc.trace_any(rsh)
+    torch.cuda.synchronize()
+    t = {}
+    for name, fn in (("closest", lambda: sc.trace_closest(rc)), ("any", lambda: sc.trace_any(rsh))):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        t[name] = a.elapsed_time(b) / reps
+    nodes, tris = v[0] / n, v[1] / n
+    per_ray = 40.0 + nodes * 128.0 + tris * 48.0
+    achieved = per_ray * n / (t["closest"] / 1e3) / 1e9
+    return {"workload": "heightfield %d triangles, %d rays" % (sc.info().triangle_count, n),
+            "closest_ms": round(t["closest"], 3), "any_ms": round(t["any"], 3),
+            "closest_mrays_per_s": round(n / t["closest"] / 1e3, 1), "any_mrays_per_s": round(n / t["any"] / 1e3, 1),
+            "nodes_per_ray": round(nodes, 2), "tris_per_ray": round(tris, 2),
+            "closest_gb_per_s_algorithmic": round(achieved, 1), "frac_of_8tb": round(achieved / 8000.0, 4),
+            "compulsory_frac": round(40.0 * n / (t["closest"] / 1e3) / 8e12, 4)}
 
 
 def cpu_baseline(sa, rs_full, budget_s):
@@ -311,6 +354,8 @@ def main():
         out["adjoint"] = adjoint_timing(ctx)
     if not args.no_batch:
         out["batch"] = batch_timing(ctx)
+    if not args.no_traversal:
+        out["traversal_c3"] = traversal_timing(ctx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sa, rs, args.cpu_seconds)
     print(json.dumps(out), flush=True)

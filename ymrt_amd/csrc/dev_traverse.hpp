@@ -555,6 +555,34 @@ struct Traversal {
     }
 };
 
+// Traversal steps of the wave's continuing lanes (r == kTravContinue).
+// Node visits and leaf tests are two long code paths; while at least
+// kInnerFirst lanes are at wide nodes, only those lanes step, so each path
+// runs with a fuller wave instead of both running under partial masks; with
+// fewer at nodes every lane steps, so no lane waits for long.  kLoop: the
+// node visits run as a loop of their own until fewer than kInnerFirst lanes
+// are at nodes (the ray setup then stays in place, which saves register
+// copies per visit); faster for closest-hit rays, slower for shadow rays,
+// which end at their first hit.  Measured best on the interior frame:
+// kInnerFirst 16 with the loop (closest hit), 24 without (shadow rays); 8
+// and 32 were slower for both.
+template <int kInnerFirst, bool kLoop, class TV, class LS>
+__device__ __forceinline__ void wave_step(TV &tv, int &r, const DevScene &sc, LS &ls, TraceCounters &tc)
+{
+    if constexpr (kLoop) {
+        for (;;) {
+            const bool at_node = r == kTravContinue && !(tv.node & kLeafBit);
+            if (__popcll(__ballot(at_node)) < kInnerFirst) break;
+            if (at_node) r = tv.template visit_node<false>(sc, ls, tc);
+        }
+        if (r == kTravContinue) r = tv.step(sc, ls, tc);
+    } else {
+        const bool leaf = (tv.node & kLeafBit) != 0;
+        const uint64_t inner = __ballot(r == kTravContinue && !leaf);
+        if (r == kTravContinue && (!leaf || __popcll(inner) < kInnerFirst)) r = tv.step(sc, ls, tc);
+    }
+}
+
 // Returns 1 on hit, 0 on miss, -1 on stack overflow (the ray is abandoned).
 template <int kMode, bool kAny, bool kNearby, bool kCount, bool kSphere, int kStack, typename LS>
 YM_HD int traverse(const DevScene &sc, const Query &q, Hit &hit, int &nb_count, LS &ls, TraceCounters &cnt)
