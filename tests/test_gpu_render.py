@@ -362,3 +362,34 @@ def test_diffuse_only_shading_equals_general(gpu_ctx):
     a = ymrt_amd.Scene(gpu_ctx, sa).render_frame(rs, 1)[0]
     b = ymrt_amd.Scene(gpu_ctx, sb).render_frame(rs, 1)[0]
     assert torch.equal(a, b)
+
+
+@pytest.mark.gpu
+def test_metal_only_shading_equals_general(gpu_ctx):
+    """Hits on metal-only materials (metalness 1, no coat) shade in a variant
+    with the metal and diffuse lobes fixed at compile time; it must give the
+    general variant's floats.  The walls of a Cornell box made rough metal,
+    once plain and once with the metalness read from a constant texture of
+    the same value (which keeps them out of the metal class), render the same
+    image bit for bit."""
+    from ymrt_amd import abi as T
+    sa = scenes.cornell_box(48, 40)
+    sb = scenes.cornell_box(48, 40)
+    one = sb.add_texture_node(T.TEXTURE_CONSTANT_FLOAT, value=(1.0, 0.0, 0.0))
+    walls = 0
+    for s in (sa, sb):
+        for m in s.materials:
+            if m.type == T.MATERIAL_DEFAULT and m.specular_weight == 0.0 and m.metalness == 0.0:
+                m.metalness = 1.0
+                m.roughness_x = m.roughness_y = 0.35
+                m.specular_albedo = (0.9, 0.6, 0.3)
+                m.coat_weight = 0.0
+                if s is sb:
+                    m.metalness_tex = one
+                    walls += 1
+    assert walls >= 3
+    rs = abi.render_settings(48, 40, 8, 6)
+    a = ymrt_amd.Scene(gpu_ctx, sa).render_frame(rs, 1)[0]
+    b = ymrt_amd.Scene(gpu_ctx, sb).render_frame(rs, 1)[0]
+    assert torch.isfinite(a).all() and a[..., :3].abs().sum() > 0
+    assert torch.equal(a, b)

@@ -15,6 +15,10 @@
 
 namespace ym {
 
+// Shading classes of a material (DevScene::mat_class): the lobes its
+// DefaultMtl weighting leaves, known at compile time in a shading variant.
+constexpr uint32_t kMatGeneral = 0, kMatDiffuse = 1, kMatMetal = 2;
+
 // Child encoding in WNode::child and on the traversal stack (32 bits):
 //   bit 31 clear        internal node index
 //   bit 31 set          leaf: bits 30..3 = first primitive, bits 2..0 = count-1
@@ -168,7 +172,7 @@ struct DevScene {
     uint32_t has_boundaries;       // any instance with treat_as_boundary
     uint32_t has_spheres;          // any sphere instance
     uint32_t has_graph;            // procedural texture nodes or normal / bump maps
-    const uint32_t *mat_diffuse;   // per material: 1 when its BSDF is the diffuse lobe alone
+    const uint32_t *mat_class;     // per material: kMatDiffuse, kMatMetal or 0 (shading variant)
     DevCamera camera;
 };
 

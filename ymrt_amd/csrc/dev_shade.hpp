@@ -793,17 +793,23 @@ __device__ __forceinline__ f3 tex_color(TX &tx, uint32_t tex, const float *def)
     return tex != kEmpty ? tx.eval(tex) : f3{def[0], def[1], def[2]};
 }
 
-// DefaultMtlBSDF::InitBSDFSettings (src/BSDF.cuh:453-618).  kDiffuse: the
-// material is diffuse-only (zero coat, metal, specular and transmission
-// weights, a coat that cannot overflow), where the general weighting leaves
-// the diffuse lobe alone with the albedo as its colour: tint, (1 - metal),
-// (1 - F * 0) and (1 - 0) are all exactly 1.
-template <bool kDiffuse = false, class TX>
+// DefaultMtlBSDF::InitBSDFSettings (src/BSDF.cuh:453-618).  kClass names a
+// material class whose weights the general code below fixes exactly:
+//  kMatDiffuse: zero coat, metal, specular and transmission weights and a
+//   coat that cannot overflow; the diffuse lobe is left alone with the albedo
+//   as its colour (tint, (1 - metal), (1 - F * 0) and (1 - 0) are all 1).
+//  kMatMetal: metalness 1, zero coat weight and a finite coat transmittance;
+//   tint is exactly 1, so the metal lobe has colour (1, 1, 1), and every
+//   product with (1 - metal) = 0 is a signed zero or NaN, which the "> 0"
+//   tests of the specular and transmission lobes reject.  The diffuse lobe
+//   stays, weighted as in the general code.
+template <uint32_t kClass = kMatGeneral, class TX>
 __device__ __forceinline__ void init_default_bsdf(TX &tx, Bsdf &b, const DevMaterial &m, f3 ray_d,
                                                   int hit_back, float ray_ior, float ex_ior)
 {
     const float ior_min = 0.0001f;
-    if constexpr (kDiffuse) {
+    constexpr bool kNoCoat = kClass == kMatMetal;
+    if constexpr (kClass == kMatDiffuse) {
         const f3 dif = tex_color(tx, m.diffuse_albedo_tex, m.diffuse_albedo);
         b.ni = gmax(ray_ior, ior_min);
         b.ax = b.ay = b.cax = b.cay = 0.0f;
@@ -821,16 +827,16 @@ __device__ __forceinline__ void init_default_bsdf(TX &tx, Bsdf &b, const DevMate
         b.w[kSlotDiffuse] = wsum > 0.0f ? dif_lum * safe_rcp(wsum) : 1.0f;
         return;
     }
-    const float coat_w = tex_float(tx, m.coat_weight_tex, m.coat_weight);
+    const float coat_w = kNoCoat ? 0.0f : tex_float(tx, m.coat_weight_tex, m.coat_weight);
     const float ior_in = gmax(ray_ior, ior_min);
     const float ior_coat = gmax(m.coat_ior, ior_min);
     const float ior_out = gmax(hit_back ? ex_ior : m.ior_n, ior_min);
     const float eta_ic = ior_coat * safe_rcp(ior_in);
     const float eta_io = ior_out * safe_rcp(ior_in);
     const float cos_in = gmax(dot(b.n, -ray_d), 0.0f);
-    const float fr_ic = coat_w > 0.0f ? fresnel_dd(eta_ic, cos_in) : 1.0f;
+    const float fr_ic = !kNoCoat && coat_w > 0.0f ? fresnel_dd(eta_ic, cos_in) : 1.0f;
     const float fr_io = fresnel_dd(eta_io, cos_in);
-    const float metal = tex_float(tx, m.metalness_tex, m.metalness);
+    const float metal = kNoCoat ? 1.0f : tex_float(tx, m.metalness_tex, m.metalness);
     const float ax = sqr(gclamp(tex_float(tx, m.roughness_x_tex, m.roughness_x), 0.001f, 0.999f));
     const float ay = sqr(gclamp(tex_float(tx, m.roughness_y_tex, m.roughness_y), 0.001f, 0.999f));
     const float spec_w = tex_float(tx, m.specular_weight_tex, m.specular_weight);
@@ -841,11 +847,16 @@ __device__ __forceinline__ void init_default_bsdf(TX &tx, Bsdf &b, const DevMate
     const float cax = sqr(gclamp(tex_float(tx, m.coat_roughness_x_tex, m.coat_roughness_x), 0.001f, 0.999f));
     const float cay = sqr(gclamp(tex_float(tx, m.coat_roughness_y_tex, m.coat_roughness_y), 0.001f, 0.999f));
     const f3 coat_alb = tex_color(tx, m.coat_albedo_tex, m.coat_albedo);
-    const f3 ca = max3(coat_alb, s3(1E-8f));
-    const f3 sigma = (-f3{ym_logf(ca.x), ym_logf(ca.y), ym_logf(ca.z)}) * coat_th;
-    const float rc = safe_rcp(cos_in);
-    const f3 ns = -sigma;
-    const f3 coat_tr{ym_expf(ns.x * rc), ym_expf(ns.y * rc), ym_expf(ns.z * rc)};
+    // coat transmittance; multiplied by the coat weight (0 for kNoCoat, where
+    // it is finite, so the product is 0 whatever its value)
+    f3 coat_tr = s3(0.0f);
+    if constexpr (!kNoCoat) {
+        const f3 ca = max3(coat_alb, s3(1E-8f));
+        const f3 sigma = (-f3{ym_logf(ca.x), ym_logf(ca.y), ym_logf(ca.z)}) * coat_th;
+        const float rc = safe_rcp(cos_in);
+        const f3 ns = -sigma;
+        coat_tr = f3{ym_expf(ns.x * rc), ym_expf(ns.y * rc), ym_expf(ns.z * rc)};
+    }
     b.ni = ior_in;
     b.ax = ax;
     b.ay = ay;
@@ -864,7 +875,7 @@ __device__ __forceinline__ void init_default_bsdf(TX &tx, Bsdf &b, const DevMate
     }
     const f3 coat_color = s3(coat_w * fr_ic);
     const float coat_lum = luminance(coat_color);
-    if (coat_lum > 0.0f) {
+    if (!kNoCoat && coat_lum > 0.0f) {
         b.mask |= 1u << kSlotCoat;
         b.col[kSlotCoat] = coat_color;
         b.w[kSlotCoat] = coat_lum;
@@ -883,7 +894,7 @@ __device__ __forceinline__ void init_default_bsdf(TX &tx, Bsdf &b, const DevMate
     }
     const f3 spec_c = tint * (1.0f - metal) * fr_io * spec_w * spe;
     const float spec_lum = luminance(spec_c);
-    if (spec_lum > 0.0f) {
+    if (!kNoCoat && spec_lum > 0.0f) {
         b.mask |= 1u << kSlotSpec;
         b.col[kSlotSpec] = spec_c;
         b.w[kSlotSpec] = spec_lum;
@@ -891,7 +902,7 @@ __device__ __forceinline__ void init_default_bsdf(TX &tx, Bsdf &b, const DevMate
     }
     const f3 trans_c = tint * (1.0f - metal) * (1.0f - fr_io * spec_w) * trans_w * spe;
     const float trans_lum = luminance(trans_c);
-    if (trans_lum > 0.0f) {
+    if (!kNoCoat && trans_lum > 0.0f) {
         b.mask |= 1u << kSlotTrans;
         b.col[kSlotTrans] = trans_c;
         b.w[kSlotTrans] = trans_lum;

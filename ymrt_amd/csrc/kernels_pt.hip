@@ -204,6 +204,12 @@ constexpr uint32_t kInfoPendingBunch = 1u << 10;
 __device__ __forceinline__ int info_depth(uint32_t i) { return (int)(i & 0xFFu); }
 __device__ __forceinline__ uint32_t info_vertex(uint32_t i) { return i >> 16; }
 
+// Stage flag kind of a surface hit on a material of shading class cls.
+__device__ __forceinline__ uint32_t shade_flag(uint32_t cls)
+{
+    return cls == kMatDiffuse ? kFlagDiffuse : cls == kMatMetal ? kFlagMetal : kFlagHit;
+}
+
 // hit flags
 constexpr uint32_t kHitBack = 1u;
 constexpr uint32_t kHitVolume = 2u;
@@ -381,8 +387,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere 
                 ps.out[p] = f4(Lv, 0.0f);
             } else {
                 ps.flag[p] = (pass << kFlagBits) |
-                             (in_medium ? kFlagMedium
-                                        : (sc.mat_diffuse[sc.instances[h.inst].material] ? kFlagDiffuse : kFlagHit));
+                             (in_medium ? kFlagMedium : shade_flag(sc.mat_class[sc.instances[h.inst].material]));
                 uint32_t flags = h.back ? kHitBack : 0u;
                 bool listed = false;
                 const int nbc = tv.nb_count;
@@ -426,17 +431,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere 
 // each block of kCompactSpan slots, k_compact_scatter writes a block's
 // matches at the sum of the counts before it, and the block holding the last
 // slot sets the queue lengths.
-// kMode: kCompactMedium, kCompactShade (three queues) or kCompactTransmit
+// kMode: kCompactMedium, kCompactShade (four queues) or kCompactTransmit
 enum : int { kCompactMedium = 0, kCompactShade = 1, kCompactTransmit = 2 };
 template <int kMode>
 struct Compact {
-    static constexpr int kQ = kMode == kCompactShade ? 3 : 1;
+    static constexpr int kQ = kMode == kCompactShade ? 4 : 1;
     __device__ static void match(uint32_t f, uint32_t tag, bool (&m)[kQ])
     {
         if constexpr (kMode == kCompactShade) {
             m[0] = f == (tag | kFlagHit);
             m[1] = f == (tag | kFlagVolume);
             m[2] = f == (tag | kFlagDiffuse);
+            m[3] = f == (tag | kFlagMetal);
         } else {
             m[0] = f == (tag | (kMode == kCompactMedium ? kFlagMedium : kFlagTransmit));
         }
@@ -486,12 +492,12 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_scatter(PtParams pr
     if (first >= n) return;
     const uint32_t last = min(first + kCompactSpan, n);
     const uint32_t stride = prm.cap / kCompactSpan + 1;
-    uint32_t *const qs[3] = {kMode == kCompactShade    ? ps.q_hit
+    uint32_t *const qs[4] = {kMode == kCompactShade    ? ps.q_hit
                              : kMode == kCompactMedium ? ps.q_medium
                                                        : ps.q_transmit,
-                             ps.q_hit_vol, ps.q_hit_diffuse};
-    const uint32_t ctr[3] = {kMode == kCompactShade ? kCtrHit : kMode == kCompactMedium ? kCtrMedium : kCtrTransmit,
-                             kCtrHitVol, kCtrHitDiffuse};
+                             ps.q_hit_vol, ps.q_hit_diffuse, ps.q_hit_metal};
+    const uint32_t ctr[4] = {kMode == kCompactShade ? kCtrHit : kMode == kCompactMedium ? kCtrMedium : kCtrTransmit,
+                             kCtrHitVol, kCtrHitDiffuse, kCtrHitMetal};
     const uint32_t tag = pass << kFlagBits;
     const uint32_t wave = threadIdx.x / kBlock;
     const uint64_t below = (1ull << lane_id()) - 1ull;
@@ -592,7 +598,7 @@ __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PtParams prm, Pa
                 ps.hit_a[p] = make_uint4(ha.x, ha.y, ha.z, ha.w | kHitVolume);
                 ps.hit_b[p] = make_float4(vpos.x, vpos.y, vpos.z, 0.0f);
             } else if (hit) {
-                ps.flag[p] = (pass << kFlagBits) | (sc.mat_diffuse[sc.instances[ha.y].material] ? kFlagDiffuse : kFlagHit);
+                ps.flag[p] = (pass << kFlagBits) | shade_flag(sc.mat_class[sc.instances[ha.y].material]);
             } else {
                 f3 Lv = xyz(ps.L[p]);
                 if (rs.enable_env_light) Lv = Lv + light_clamp(rs, beta * background(d), info_depth(ps.info[p]));
@@ -636,28 +642,35 @@ __device__ __forceinline__ float shadow_tmax(f3 pos, f3 ldir, f3 lpos, f3 lerr, 
 // EvalVolumeShapeLight 419-513); otherwise surface vertices (2132-2302 with
 // EvalDistantLight 83-198 and EvalShapeLight 294-417).
 // kGraph: the scene has procedural texture nodes or normal / bump maps (the
-// variant without them keeps the image-texture fast path only).  kDiffuse:
-// the hits on diffuse-only materials (their own queue), shaded with the
-// diffuse lobe alone, which gives the general variant's floats with far
-// fewer live registers.  The
+// variant without them keeps the image-texture fast path only).  kClass:
+// kMatDiffuse or kMatMetal shades the hits on materials of that class (their
+// own queue) with the lobes the class leaves fixed at compile time, which
+// gives the general variant's floats with fewer live registers.  The
 // variants without the graph evaluator are held to 168 VGPRs (3 waves per
 // SIMD, a few spilled registers): the shading stage waits on dependent
 // loads, and the third wave hides more of them than the spills cost (6 %
 // faster on the interior frame; 4 waves spill too much).
-template <bool kVolume, bool kGraph, bool kDiffuse = false>
+template <bool kVolume, bool kGraph, uint32_t kClass = kMatGeneral>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGraph ? 2 : 3))) void k_shade(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
 {
     __shared__ float tex_stk[(kVolume || !kGraph) ? 1 : kTexStack * 3 * kBlock];
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     uint32_t *cnt_next = prm.counters + (pass + 1) * kCtrStride;
-    const uint32_t n = cnt[kVolume ? kCtrHitVol : kDiffuse ? kCtrHitDiffuse : kCtrHit];
-    const uint32_t *q = kVolume ? ps.q_hit_vol : kDiffuse ? ps.q_hit_diffuse : ps.q_hit;
+    constexpr bool kDiffuse = kClass == kMatDiffuse, kMetal = kClass == kMatMetal;
+    const uint32_t n = cnt[kVolume ? kCtrHitVol : kDiffuse ? kCtrHitDiffuse : kMetal ? kCtrHitMetal : kCtrHit];
+    const uint32_t *q = kVolume ? ps.q_hit_vol : kDiffuse ? ps.q_hit_diffuse : kMetal ? ps.q_hit_metal : ps.q_hit;
     const ymrt_render_settings &rs = prm.rs;
     const uint32_t cap = prm.cap;
     uint32_t *const ctrs[2] = {cnt + kCtrShadow, cnt_next + kCtrActive};
     uint32_t *const qs[2] = {ps.q_shadow, ps.queue[(pass + 1) & 1]};
-    constexpr uint32_t kLobes = kDiffuse ? 1u << kSlotDiffuse : 0u;
-    run_items<2>(cnt + (kVolume ? kCtrWorkShadeVol : kDiffuse ? kCtrWorkShadeDiffuse : kCtrWorkShade), n, ctrs, qs,
+    constexpr uint32_t kLobes = kDiffuse ? 1u << kSlotDiffuse
+                                : kMetal ? (1u << kSlotMetal) | (1u << kSlotDiffuse)
+                                         : 0u;
+    run_items<2>(cnt + (kVolume     ? kCtrWorkShadeVol
+                        : kDiffuse ? kCtrWorkShadeDiffuse
+                        : kMetal   ? kCtrWorkShadeMetal
+                                   : kCtrWorkShade),
+                 n, ctrs, qs,
                  [&](uint32_t i, uint32_t &p, bool(&out)[2]) {
         {
             p = q[i];
@@ -885,7 +898,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGraph ?
                     Bsdf bs;
                     init_shading_space(bs, tx, normal_tex, bump_tex, back, ray_d, ng, sn, dpdu, dpdv, in.w2o, duv);
                     bs.mask = 0;
-                    if (is_default) init_default_bsdf<kDiffuse>(tx, bs, m, ray_d, back, ray_ior, ex_ior);
+                    if (is_default) init_default_bsdf<kClass>(tx, bs, m, ray_d, back, ray_ior, ex_ior);
                     if (prm.adj_mat && prm.adj_mat[in.material]) {
                         // every later term of the path carries this vertex's albedo once
                         const uint32_t k = ps.adj_n[p];
@@ -1414,19 +1427,17 @@ int launch_compact(const PtParams &prm, const PathState &ps, uint32_t pass, int 
 int launch_shade(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool volume,
                  hipStream_t s)
 {
+    const dim3 g = persistent_grid(prm.chunk_paths), b(kBlock);
     if (volume)
-        hipLaunchKernelGGL((k_shade<true, false>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps,
-                           pass);
+        hipLaunchKernelGGL((k_shade<true, false>), g, b, 0, s, sc, prm, ps, pass);
     else if (sc.has_graph) {
-        hipLaunchKernelGGL((k_shade<false, true>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps,
-                           pass);
-        hipLaunchKernelGGL((k_shade<false, true, true>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm,
-                           ps, pass);
+        hipLaunchKernelGGL((k_shade<false, true>), g, b, 0, s, sc, prm, ps, pass);
+        hipLaunchKernelGGL((k_shade<false, true, kMatDiffuse>), g, b, 0, s, sc, prm, ps, pass);
+        hipLaunchKernelGGL((k_shade<false, true, kMatMetal>), g, b, 0, s, sc, prm, ps, pass);
     } else {
-        hipLaunchKernelGGL((k_shade<false, false>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps,
-                           pass);
-        hipLaunchKernelGGL((k_shade<false, false, true>), persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc,
-                           prm, ps, pass);
+        hipLaunchKernelGGL((k_shade<false, false>), g, b, 0, s, sc, prm, ps, pass);
+        hipLaunchKernelGGL((k_shade<false, false, kMatDiffuse>), g, b, 0, s, sc, prm, ps, pass);
+        hipLaunchKernelGGL((k_shade<false, false, kMatMetal>), g, b, 0, s, sc, prm, ps, pass);
     }
     YM_HIP(hipGetLastError());
     return YMRT_OK;

@@ -23,7 +23,9 @@ constexpr uint32_t kCtrTransmit = 12;      // paths whose unoccluded shadow rays
 constexpr uint32_t kCtrWorkTransmit = 13;
 constexpr uint32_t kCtrHitDiffuse = 14;    // surface hits on diffuse-only materials
 constexpr uint32_t kCtrWorkShadeDiffuse = 15;
-constexpr uint32_t kCtrStride = 16;
+constexpr uint32_t kCtrHitMetal = 16;      // surface hits on metal-only materials
+constexpr uint32_t kCtrWorkShadeMetal = 17;
+constexpr uint32_t kCtrStride = 24;
 
 // Compaction: blocks of kCompactThreads threads, kCompactSpan queue entries
 // each.
@@ -40,6 +42,7 @@ constexpr uint32_t kFlagMedium = 2;   // free flight
 constexpr uint32_t kFlagVolume = 3;   // volume-vertex shading
 constexpr uint32_t kFlagDiffuse = 4;  // surface shading, diffuse-only material
 constexpr uint32_t kFlagTransmit = 5; // transmittance of unoccluded shadow rays
+constexpr uint32_t kFlagMetal = 6;    // surface shading, metal-only material
 constexpr uint32_t kMaxPasses = 4 * kMaxRayDepth;
 
 // Path-tracing traversals keep the top kStackWindow stack entries per lane in
@@ -80,10 +83,10 @@ struct PathState {
     uint32_t *sh_flags;
     float4 *out;        // final radiance per path
     uint32_t *queue[2];
-    uint32_t *q_hit, *q_hit_vol, *q_hit_diffuse, *q_medium, *q_shadow, *q_transmit;
+    uint32_t *q_hit, *q_hit_vol, *q_hit_diffuse, *q_hit_metal, *q_medium, *q_shadow, *q_transmit;
     uint32_t *spill;    // traversal stack spill [kMaxStackDepth - kStackWindow][kSpillThreads]
     uint32_t *flag;       // stage flag per slot (kFlag*)
-    uint32_t *ccount;     // compaction: matches per block, [3][cap / kCompactSpan + 1]
+    uint32_t *ccount;     // compaction: matches per block, [4][cap / kCompactSpan + 1]
     uint32_t *first_hit;  // instance hit by the camera ray (written when an id image is requested)
     uint4 *adj_rec;       // adjoint: [ray_depth][cap] (material, radiance before the vertex)
     uint32_t *adj_n;      // adjoint: records per path

@@ -110,9 +110,10 @@ static int ensure_buffers(ymrt_scene *sc, uint32_t cap)
     rc |= alloc(rb, &ps.q_hit, n);
     rc |= alloc(rb, &ps.q_hit_vol, n);
     rc |= alloc(rb, &ps.q_hit_diffuse, n);
+    rc |= alloc(rb, &ps.q_hit_metal, n);
     rc |= alloc(rb, &ps.q_medium, n);
     rc |= alloc(rb, &ps.flag, n);
-    rc |= alloc(rb, &ps.ccount, 3 * ((size_t)n / kCompactSpan + 1));
+    rc |= alloc(rb, &ps.ccount, 4 * ((size_t)n / kCompactSpan + 1));
     rc |= alloc(rb, &ps.q_shadow, n);
     rc |= alloc(rb, &ps.q_transmit, n);
     rc |= alloc(rb, &ps.spill, (size_t)(kMaxStackDepth - kStackWindow) * kSpillThreads);

@@ -101,6 +101,29 @@ uint32_t diffuse_only(const DevMaterial &m)
     return ok ? 1u : 0u;
 }
 
+// A DefaultMtl with metalness exactly 1 (untextured), zero coat weight, a
+// coat whose transmittance stays finite (albedo in [0, 1], thickness in
+// [0, 1e6]: -log(1e-8) * 1e6 / cos >= -1e37), finite untextured specular and transmission
+// weights and no normal or bump map: InitBSDFSettings then leaves the metal
+// lobe with colour (1, 1, 1) and a diffuse lobe of weight 0 (the
+// (1 - metalness) factor), bit for bit.
+uint32_t metal_only(const DevMaterial &m)
+{
+    bool ok = m.type == YMRT_MATERIAL_DEFAULT && m.metalness == 1.0f && m.metalness_tex == kEmpty &&
+              m.coat_weight == 0.0f && m.coat_weight_tex == kEmpty && m.coat_albedo_tex == kEmpty &&
+              m.coat_thickness_tex == kEmpty && m.specular_weight_tex == kEmpty &&
+              m.transmission_weight_tex == kEmpty && m.normal_mapping_tex == kEmpty && m.bump_mapping_tex == kEmpty &&
+              m.coat_thickness >= 0.0f && m.coat_thickness <= 1e6f && std::isfinite(m.ior_n) &&
+              std::isfinite(m.specular_weight) && std::isfinite(m.transmission_weight);
+    for (int c = 0; c < 3; ++c) ok = ok && m.coat_albedo[c] >= 0.0f && m.coat_albedo[c] <= 1.0f;
+    return ok ? 1u : 0u;
+}
+
+uint32_t shading_class(const DevMaterial &m)
+{
+    return diffuse_only(m) ? kMatDiffuse : metal_only(m) ? kMatMetal : kMatGeneral;
+}
+
 // Texture slots of a material, all of which must name a texture or be empty.
 bool material_textures_valid(const DevMaterial &m, uint32_t texture_count)
 {
@@ -663,8 +686,8 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
         if ((rc = upload(sc, lam, &d))) return rc;
         sc->adj_mat = d;
         std::vector<uint32_t> dif(h.materials.size());
-        for (size_t i = 0; i < h.materials.size(); ++i) dif[i] = diffuse_only(h.materials[i]);
-        if ((rc = upload(sc, dif, &ds.mat_diffuse))) return rc;
+        for (size_t i = 0; i < h.materials.size(); ++i) dif[i] = shading_class(h.materials[i]);
+        if ((rc = upload(sc, dif, &ds.mat_class))) return rc;
     }
     sc->materials = h.materials;
     sc->texture_count = d->texture_count;
@@ -727,13 +750,13 @@ int update_materials(ymrt_scene *sc, uint32_t first, uint32_t count, const ymrt_
     std::vector<uint32_t> lam(count), dif(count);
     for (uint32_t i = 0; i < count; ++i) {
         lam[i] = lambert_only(next[first + i]);
-        dif[i] = diffuse_only(next[first + i]);
+        dif[i] = shading_class(next[first + i]);
     }
     YM_HIP(hipMemcpyAsync(const_cast<DevMaterial *>(sc->dev.materials) + first, next.data() + first,
                           sizeof(DevMaterial) * count, hipMemcpyHostToDevice, s));
     YM_HIP(hipMemcpyAsync(const_cast<uint32_t *>(sc->adj_mat) + first, lam.data(), sizeof(uint32_t) * count,
                           hipMemcpyHostToDevice, s));
-    YM_HIP(hipMemcpyAsync(const_cast<uint32_t *>(sc->dev.mat_diffuse) + first, dif.data(), sizeof(uint32_t) * count,
+    YM_HIP(hipMemcpyAsync(const_cast<uint32_t *>(sc->dev.mat_class) + first, dif.data(), sizeof(uint32_t) * count,
                           hipMemcpyHostToDevice, s));
     YM_HIP(hipStreamSynchronize(s));
     sc->materials.swap(next);
