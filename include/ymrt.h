@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define YMRT_ABI_VERSION 5
+#define YMRT_ABI_VERSION 6
 
 /* status codes */
 #define YMRT_OK 0
@@ -237,6 +237,10 @@ typedef struct ymrt_render_settings {
     uint32_t path_capacity;          /* paths in flight per chunk; 0: up to 64 M,
                                         within a quarter of free device memory
                                         (about 720 B per path)                */
+    uint32_t persistent_waves;       /* waves of each persistent pass kernel;
+                                        0: 4096, the most the traversal spill
+                                        area holds (also the maximum).  Like
+                                        path_capacity it changes no pixel.    */
 } ymrt_render_settings;
 
 typedef struct ymrt_frame_stats {
@@ -320,6 +324,26 @@ int ymrt_trace_closest_counted(ymrt_scene *scene, const ymrt_ray *rays, uint32_t
 int ymrt_render_frame(ymrt_scene *scene, const ymrt_render_settings *settings,
                       int frame_index, float *noise_rgba, void *stream,
                       int timing, ymrt_frame_stats *stats);
+
+/* Autotuner for the two launch settings that change no pixel, path_capacity
+ * and persistent_waves: renders trial frames of the caller's settings
+ * (frame_index, into an image of its own) and times them with HIP events --
+ * capacities {library default, 1/2 and 1/4 of the frame's paths} at the
+ * default waves, then waves {4096, 3072, 2048, 1024} at the best capacity,
+ * each the faster of two frames after one warm-up.  The fastest pair goes to
+ * result; the caller copies it into its settings.  No counterpart in the
+ * reference, which launches with fixed sizes; it is the autotuner its
+ * batched-wavefront plan asks for. */
+typedef struct ymrt_tune_result {
+    uint32_t path_capacity;          /* 0: the library default was fastest   */
+    uint32_t persistent_waves;       /* 0: the default (4096) was fastest    */
+    float ms;                        /* frame time of that pair              */
+    float ms_default;                /* frame time with both at 0            */
+    uint32_t trials;                 /* frames rendered, warm-up included    */
+} ymrt_tune_result;
+
+int ymrt_autotune(ymrt_scene *scene, const ymrt_render_settings *settings,
+                  int frame_index, void *stream, ymrt_tune_result *result);
 
 /* ymrt_render_frame plus an instance-id image: instance_ids (width*height
  * uint32, device, may be NULL) receives the index of the instance that

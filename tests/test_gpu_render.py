@@ -393,3 +393,30 @@ def test_metal_only_shading_equals_general(gpu_ctx):
     b = ymrt_amd.Scene(gpu_ctx, sb).render_frame(rs, 1)[0]
     assert torch.isfinite(a).all() and a[..., :3].abs().sum() > 0
     assert torch.equal(a, b)
+
+
+@pytest.mark.gpu
+def test_autotune_and_launch_settings_keep_pixels(gpu_ctx):
+    """path_capacity and persistent_waves change timing, never a pixel; the
+    autotuner returns one of its candidates for each and a positive time."""
+    sa = scenes.interior(64, 48, tex_size=32, instances=4)
+    rs = abi.render_settings(64, 48, 8, 6)
+    sc = ymrt_amd.Scene(gpu_ctx, sa)
+    ref = sc.render_frame(rs, 2)[0].cpu().numpy()
+    paths = 4 * 3 * 256 * 8
+    for cap, waves in [(4096, 0), (0, 1024), (paths // 4, 64), (0, 1)]:
+        rs2 = abi.render_settings(64, 48, 8, 6, path_capacity=cap, persistent_waves=waves)
+        img = sc.render_frame(rs2, 2)[0].cpu().numpy()
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), (cap, waves)
+    with pytest.raises(ymrt_amd.YmrtError):
+        sc.render_frame(abi.render_settings(64, 48, 8, 6, persistent_waves=4097), 2)
+    res = sc.autotune(rs, 2)
+    print("autotune: capacity %d waves %d  %.3f ms (default %.3f ms), %d trials" %
+          (res.path_capacity, res.persistent_waves, res.ms, res.ms_default, res.trials))
+    assert res.path_capacity in (0, paths // 2, paths // 4)
+    assert res.persistent_waves in (0, 3072, 2048, 1024)
+    assert 0.0 < res.ms <= res.ms_default
+    assert res.trials == 1 + 2 * 3 + 2 * 3
+    rs3 = abi.render_settings(64, 48, 8, 6, path_capacity=res.path_capacity, persistent_waves=res.persistent_waves)
+    img = sc.render_frame(rs3, 2)[0].cpu().numpy()
+    assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))

@@ -25,7 +25,7 @@ EXPORTS = [
     "ymrt_ymtmp_load", "ymrt_ymtmp_parse", "ymrt_ymtmp_scene", "ymrt_ymtmp_warning_count",
     "ymrt_ymtmp_warning", "ymrt_ymtmp_find", "ymrt_ymtmp_transform", "ymrt_ymtmp_free",
     "ymrt_render_frame_aov", "ymrt_save_png", "ymrt_render_adjoint", "ymrt_scene_update_materials",
-    "ymrt_render_batch",
+    "ymrt_render_batch", "ymrt_autotune",
 ]
 
 _lib = None
@@ -67,6 +67,7 @@ def lib():
         L.ymrt_texture_eval.argtypes = [vp, C.c_uint32, vp, C.c_uint32, vp, vp]
         L.ymrt_scene_update_materials.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(abi.MaterialDesc), vp]
         L.ymrt_render_batch.argtypes = [vp, C.POINTER(abi.BatchItem), C.c_uint32, C.c_uint32, vp]
+        L.ymrt_autotune.argtypes = [vp, C.POINTER(abi.RenderSettings), C.c_int, vp, C.POINTER(abi.TuneResult)]
         L.ymrt_ymtmp_load.argtypes = [C.c_char_p, C.POINTER(vp)]
         L.ymrt_ymtmp_parse.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p, C.POINTER(vp)]
         L.ymrt_ymtmp_scene.argtypes = [vp]
@@ -260,6 +261,13 @@ class Scene:
         _check(lib().ymrt_render_adjoint(self.h, C.byref(settings), frame, _dev_ptr(noise), _dev_ptr(adjoint),
                                          _dev_ptr(grad), _stream(), 1 if timing else 0, C.byref(stats)))
         return noise, grad, stats
+
+    def autotune(self, settings, frame=1):
+        """abi.TuneResult of ymrt_autotune: the fastest path_capacity and
+        persistent_waves for these settings (0: the library default)."""
+        res = abi.TuneResult()
+        _check(lib().ymrt_autotune(self.h, C.byref(settings), frame, _stream(), C.byref(res)))
+        return res
 
     def render_frame_aov(self, settings, frame=1):
         """(noise image, int32 instance-id image, stats); ids are -1 on a miss."""

@@ -145,6 +145,7 @@ class RenderSettings(C.Structure):
         ("enable_env_light", C.c_int32), ("enable_russian_roulette", C.c_int32),
         ("enable_indirect_clamp", C.c_int32), ("indirect_limit", C.c_float),
         ("gamma", C.c_float), ("exposure", C.c_float), ("path_capacity", C.c_uint32),
+        ("persistent_waves", C.c_uint32),
     ]
 
 
@@ -436,7 +437,7 @@ class SceneArrays:
 
 
 def render_settings(width, height, spp, depth, sampler=SAMPLER_HALTON, shape_light=1,
-                    volume=0, env_light=0, rr=0, path_capacity=0, distant_light=0):
+                    volume=0, env_light=0, rr=0, path_capacity=0, distant_light=0, persistent_waves=0):
     rs = RenderSettings()
     rs.width, rs.height, rs.spp, rs.ray_depth = width, height, spp, depth
     rs.sampler_type = sampler
@@ -450,7 +451,14 @@ def render_settings(width, height, spp, depth, sampler=SAMPLER_HALTON, shape_lig
     rs.gamma = 2.2
     rs.exposure = 1.0
     rs.path_capacity = path_capacity
+    rs.persistent_waves = persistent_waves
     return rs
+
+
+class TuneResult(C.Structure):
+    """ymrt_tune_result."""
+    _fields_ = [("path_capacity", C.c_uint32), ("persistent_waves", C.c_uint32), ("ms", C.c_float),
+                ("ms_default", C.c_float), ("trials", C.c_uint32)]
 
 
 def _np(ptr, n, dtype):

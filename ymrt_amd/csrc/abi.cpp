@@ -15,6 +15,8 @@ int update_materials(ymrt_scene *sc, uint32_t first, uint32_t count, const ymrt_
                      hipStream_t s);  // scene.cpp
 int render_batch(ymrt_context *ctx, const ymrt_batch_item *items, uint32_t n, uint32_t max_streams,
                  hipStream_t stream);  // batch.cpp
+int autotune(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, hipStream_t stream,
+             ymrt_tune_result *out);  // tune.cpp
 int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, float *noise, uint32_t *instance_ids,
                  const float *adj_img, float *adj_grad, hipStream_t stream, int timing,
                  ymrt_frame_stats *stats);                                      // render.cpp
@@ -224,6 +226,13 @@ int ymrt_render_batch(ymrt_context *ctx, const ymrt_batch_item *items, uint32_t 
                       void *stream)
 {
     return render_batch(ctx, items, n, max_streams, (hipStream_t)stream);
+}
+
+int ymrt_autotune(ymrt_scene *sc, const ymrt_render_settings *rs, int frame_index, void *stream,
+                  ymrt_tune_result *result)
+{
+    const int rc = autotune(sc, rs, frame_index, (hipStream_t)stream, result);
+    return rc ? rc : check_stack_flag(sc);
 }
 
 int ymrt_scene_update_materials(ymrt_scene *sc, uint32_t first, uint32_t count,

@@ -1344,18 +1344,23 @@ __global__ void k_tonemap(float4 *img, uint32_t n, float gamma, float exposure)
 }
 
 // ------------------------------------------------------------ launchers
-static dim3 persistent_grid(uint32_t items)
+// Persistent pass kernels: by default enough waves to fill 256 CUs several
+// times over (the traversal spill area holds kSpillThreads threads), or the
+// caller's persistent_waves; never more than the items need.  Every pass
+// kernel fetches work dynamically or strides over the grid, so the count
+// changes timing only.
+static dim3 persistent_grid(const PtParams &prm)
 {
-    // enough waves to fill 256 CUs several times over, never more than items
-    // need; the traversal spill area holds kSpillThreads threads
-    uint32_t waves = (items + kBlock - 1) / kBlock;
-    constexpr uint32_t cap = kSpillThreads / kBlock;
+    const uint32_t waves = (prm.chunk_paths + kBlock - 1) / kBlock;
+    const uint32_t w = prm.rs.persistent_waves;
+    static_assert(kMaxPersistentWaves * kBlock == kSpillThreads, "one spill row per lane");
+    const uint32_t cap = w && w < kMaxPersistentWaves ? w : kMaxPersistentWaves;
     return dim3(waves < cap ? (waves ? waves : 1) : cap);
 }
 
 int launch_raygen(const DevScene &sc, const PtParams &prm, const PathState &ps, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_raygen, persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps);
+    hipLaunchKernelGGL(k_raygen, persistent_grid(prm), dim3(kBlock), 0, s, sc, prm, ps);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }
@@ -1363,7 +1368,7 @@ int launch_raygen(const DevScene &sc, const PtParams &prm, const PathState &ps, 
 template <bool kNearby, bool kMedia, bool kSphere>
 static int launch_extend_t(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_extend<kNearby, kMedia, kSphere, kMaxStackDepth>), persistent_grid(prm.chunk_paths),
+    hipLaunchKernelGGL((k_extend<kNearby, kMedia, kSphere, kMaxStackDepth>), persistent_grid(prm),
                        dim3(kBlock), 0, s, sc, prm, ps, pass);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
@@ -1372,7 +1377,7 @@ static int launch_extend_t(const DevScene &sc, const PtParams &prm, const PathSt
 template <bool kMedia, bool kSphere>
 static int launch_shadow_t(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_shadow<kMedia, kSphere, kMaxStackDepth>), persistent_grid(prm.chunk_paths), dim3(kBlock),
+    hipLaunchKernelGGL((k_shadow<kMedia, kSphere, kMaxStackDepth>), persistent_grid(prm), dim3(kBlock),
                        0, s, sc, prm, ps, pass);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
@@ -1400,9 +1405,9 @@ int launch_extend(const DevScene &sc, const PtParams &prm, const PathState &ps, 
 int launch_medium(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
 {
     if (sc.grid_count > 0)
-        hipLaunchKernelGGL(k_medium<true>, persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps, pass);
+        hipLaunchKernelGGL(k_medium<true>, persistent_grid(prm), dim3(kBlock), 0, s, sc, prm, ps, pass);
     else
-        hipLaunchKernelGGL(k_medium<false>, persistent_grid(prm.chunk_paths), dim3(kBlock), 0, s, sc, prm, ps, pass);
+        hipLaunchKernelGGL(k_medium<false>, persistent_grid(prm), dim3(kBlock), 0, s, sc, prm, ps, pass);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }
@@ -1427,7 +1432,7 @@ int launch_compact(const PtParams &prm, const PathState &ps, uint32_t pass, int 
 int launch_shade(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool volume,
                  hipStream_t s)
 {
-    const dim3 g = persistent_grid(prm.chunk_paths), b(kBlock);
+    const dim3 g = persistent_grid(prm), b(kBlock);
     if (volume)
         hipLaunchKernelGGL((k_shade<true, false>), g, b, 0, s, sc, prm, ps, pass);
     else if (sc.has_graph) {
@@ -1453,7 +1458,7 @@ int launch_shadow(const DevScene &sc, const PtParams &prm, const PathState &ps, 
 
 int launch_transmit(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
 {
-    const dim3 g = persistent_grid(prm.chunk_paths), b(kBlock);
+    const dim3 g = persistent_grid(prm), b(kBlock);
     if (sc.grid_count > 0) {
         if (sc.has_spheres)
             hipLaunchKernelGGL((k_transmit<true, kMaxStackDepth, true>), g, b, 0, s, sc, prm, ps, pass);

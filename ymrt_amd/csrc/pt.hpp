@@ -51,6 +51,8 @@ constexpr uint32_t kMaxPasses = 4 * kMaxRayDepth;
 constexpr int kStackWindow = 16;
 constexpr int kMaxStackDepth = 64;
 constexpr uint32_t kSpillThreads = 256u * 16u * 64u;
+// Most waves (64 lanes) of a persistent pass kernel: one spill row per lane.
+constexpr uint32_t kMaxPersistentWaves = kSpillThreads / 64u;
 
 // Frame statistics (device, 64-bit)
 constexpr uint32_t kStatCamera = 0;

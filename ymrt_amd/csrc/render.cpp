@@ -175,6 +175,8 @@ int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, floa
         cap = std::max<uint64_t>(cap, spp);
     }
     YM_REQUIRE(cap >= spp, "path_capacity must hold one pixel's samples");
+    YM_REQUIRE(rs->persistent_waves <= kMaxPersistentWaves,
+               "persistent_waves exceeds " + std::to_string(kMaxPersistentWaves) + " (the traversal spill area)");
     // equal chunks of whole pixels, none larger than the capacity
     const uint64_t per_cap = (cap / spp) * spp;
     const uint64_t nchunks = (total_paths + per_cap - 1) / per_cap;
