@@ -237,9 +237,10 @@ typedef struct ymrt_render_settings {
     uint32_t path_capacity;          /* paths in flight per chunk; 0: up to 64 M,
                                         within a quarter of free device memory
                                         (about 720 B per path)                */
-    uint32_t persistent_waves;       /* waves of each persistent pass kernel;
-                                        0: 4096, the most the traversal spill
-                                        area holds (also the maximum).  Like
+    uint32_t persistent_waves;       /* waves of each persistent pass kernel,
+                                        at most 4096 (the traversal spill
+                                        area); 0: one per 256 paths of the
+                                        chunk, within [1024, 4096].  Like
                                         path_capacity it changes no pixel.    */
 } ymrt_render_settings;
 
@@ -329,14 +330,14 @@ int ymrt_render_frame(ymrt_scene *scene, const ymrt_render_settings *settings,
  * and persistent_waves: renders trial frames of the caller's settings
  * (frame_index, into an image of its own) and times them with HIP events --
  * capacities {library default, 1/2 and 1/4 of the frame's paths} at the
- * default waves, then waves {4096, 3072, 2048, 1024} at the best capacity,
+ * default waves, then waves {4096, 2048, 1024} at the best capacity,
  * each the faster of two frames after one warm-up.  The fastest pair goes to
  * result; the caller copies it into its settings.  No counterpart in the
  * reference, which launches with fixed sizes; it is the autotuner its
  * batched-wavefront plan asks for. */
 typedef struct ymrt_tune_result {
     uint32_t path_capacity;          /* 0: the library default was fastest   */
-    uint32_t persistent_waves;       /* 0: the default (4096) was fastest    */
+    uint32_t persistent_waves;       /* 0: the default was fastest           */
     float ms;                        /* frame time of that pair              */
     float ms_default;                /* frame time with both at 0            */
     uint32_t trials;                 /* frames rendered, warm-up included    */

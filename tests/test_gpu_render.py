@@ -414,7 +414,7 @@ def test_autotune_and_launch_settings_keep_pixels(gpu_ctx):
     print("autotune: capacity %d waves %d  %.3f ms (default %.3f ms), %d trials" %
           (res.path_capacity, res.persistent_waves, res.ms, res.ms_default, res.trials))
     assert res.path_capacity in (0, paths // 2, paths // 4)
-    assert res.persistent_waves in (0, 3072, 2048, 1024)
+    assert res.persistent_waves in (0, 4096, 2048, 1024)
     assert 0.0 < res.ms <= res.ms_default
     assert res.trials == 1 + 2 * 3 + 2 * 3
     rs3 = abi.render_settings(64, 48, 8, 6, path_capacity=res.path_capacity, persistent_waves=res.persistent_waves)

@@ -71,7 +71,7 @@ int autotune(ymrt_scene *sc, const ymrt_render_settings *rs_in, int frame, hipSt
                 res.path_capacity = caps[i];
             }
         }
-        for (uint32_t w : {3072u, 2048u, 1024u}) {
+        for (uint32_t w : {4096u, 2048u, 1024u}) {
             if (rc) break;
             float ms;
             rc = trial(res.path_capacity, w, ms);
