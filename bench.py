@@ -26,9 +26,9 @@ sys.path.insert(0, ROOT)
 METRIC = "Mrays/s & 1080p samples/s vs plain; adjoint/forward time at fixed RMSE"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per item of the traversal kernels ("Roofline
-# accounting").  extend, per closest-hit segment: queue id 4 + info 4 + ray 32
-# + last hit 8 + medium header 4 read; hit record 32 + nearby entry 8 + queue
+# Algorithmic bytes per item of the traversal kernels.  extend, per
+# closest-hit segment: queue id 4 + info 4 + ray 32 + last hit 8 + medium
+# header 4 read; hit record 32 + nearby entry 8 + queue
 # id 4 written.  Each visited wide node costs 128 B and each tested triangle
 # 48 B; the kernel counts both on the device in every frame.
 EXTEND_STATE_BYTES = 4 + 4 + 32 + 8 + 4 + 32 + 8 + 4
@@ -230,7 +230,7 @@ def measured_traffic(kernel, ext_rays, launches):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; --gpus %d is not supported (the frame would be split "
+        print("bench.py: the path tracer runs on one GPU; --gpus %d is not supported (the frame would be split "
               "by pixel tiles across GPUs with one reduction of the accumulation buffer)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
