@@ -238,9 +238,9 @@ typedef struct ymrt_render_settings {
                                         within a quarter of free device memory
                                         (about 720 B per path)                */
     uint32_t persistent_waves;       /* waves of each persistent pass kernel,
-                                        at most 4096 (the traversal spill
+                                        at most 5120 (the traversal spill
                                         area); 0: one per 256 paths of the
-                                        chunk, within [1024, 4096].  Like
+                                        chunk, within [1024, 5120].  Like
                                         path_capacity it changes no pixel.    */
 } ymrt_render_settings;
 

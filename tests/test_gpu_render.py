@@ -409,7 +409,7 @@ def test_autotune_and_launch_settings_keep_pixels(gpu_ctx):
         img = sc.render_frame(rs2, 2)[0].cpu().numpy()
         assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), (cap, waves)
     with pytest.raises(ymrt_amd.YmrtError):
-        sc.render_frame(abi.render_settings(64, 48, 8, 6, persistent_waves=4097), 2)
+        sc.render_frame(abi.render_settings(64, 48, 8, 6, persistent_waves=5121), 2)
     res = sc.autotune(rs, 2)
     print("autotune: capacity %d waves %d  %.3f ms (default %.3f ms), %d trials" %
           (res.path_capacity, res.persistent_waves, res.ms, res.ms_default, res.trials))

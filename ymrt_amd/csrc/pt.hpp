@@ -47,10 +47,12 @@ constexpr uint32_t kMaxPasses = 4 * kMaxRayDepth;
 
 // Path-tracing traversals keep the top kStackWindow stack entries per lane in
 // LDS and spill deeper ones to a global area of (kMaxStackDepth - kStackWindow)
-// entries for each of kSpillThreads threads (the largest persistent grid).
+// entries for each of kSpillThreads threads (the largest persistent grid:
+// 5 waves on each SIMD of 256 CUs, the most the shadow kernel's registers
+// allow; 63 MB).
 constexpr int kStackWindow = 16;
 constexpr int kMaxStackDepth = 64;
-constexpr uint32_t kSpillThreads = 256u * 16u * 64u;
+constexpr uint32_t kSpillThreads = 256u * 4u * 5u * 64u;
 // Most waves (64 lanes) of a persistent pass kernel: one spill row per lane.
 constexpr uint32_t kMaxPersistentWaves = kSpillThreads / 64u;
 
