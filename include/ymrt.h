@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define YMRT_ABI_VERSION 6
+#define YMRT_ABI_VERSION 7
 
 /* status codes */
 #define YMRT_OK 0
@@ -155,19 +155,28 @@ typedef struct ymrt_texture_desc {
 } ymrt_texture_desc;
 
 /* Density grid for heterogeneous media (no counterpart in YMRT, whose
- * heterogeneous density comes only from procedural textures).  Voxel-constant
- * density over the unit cube [0,1]^3 of the volume's object space. */
+ * heterogeneous density comes only from its texture graph; see
+ * ymrt_volume_desc.density_texture).  Voxel-constant density over the unit
+ * cube [0,1]^3 of the volume's object space. */
 typedef struct ymrt_grid_desc {
     uint32_t nx, ny, nz;
     const float *density;            /* nx*ny*nz, x fastest                   */
 } ymrt_grid_desc;
 
+/* Volume (src/Volume.h:9-18).  The density is homogeneous (1), a voxel grid
+ * or, as YMRT's density_texture_idx, the x channel of a texture graph
+ * evaluated at (st = 0, world point, object point), clamped at 0, under the
+ * majorant Texture::GetUpperBound (src/Texture.h:1181-1251) -- the tracker
+ * of SampleTrHeterogeneous / EvalTrHeterogeneous (src/VolumeEval.cuh:80-418).
+ * At most one of density_grid and density_texture is >= 0. */
 typedef struct ymrt_volume_desc {
     float sigma_t[3];
     float albedo[3];
     float g;                         /* Henyey-Greenstein asymmetry           */
-    int32_t density_grid;            /* -1: homogeneous                       */
-    float world_to_object[16];       /* maps world to the grid's unit cube    */
+    int32_t density_grid;            /* -1: no grid                           */
+    float world_to_object[16];       /* the volume's object space (the grid's
+                                        unit cube; the texture's object point) */
+    int32_t density_texture;         /* -1: no density texture                */
 } ymrt_volume_desc;
 
 /* DistantLight (src/Light.h:324-386): radiance light_color * intensity from
@@ -430,9 +439,8 @@ int ymrt_halton_enumerate(uint32_t width, uint32_t height, const uint32_t *px,
  * in the warnings, as the reference prints and skips it; a statement that is
  * not Begin(Type) / End(Type) / attribute=value, a nested or unclosed block
  * or an attribute that the block type does not know fails the whole load.
- * ImageTexture reads 8-bit PNG from <dir>/texture_<stem>_<ext>/.  Not
- * supported (YMRT_ERR_UNSUPPORTED): Volume density_texture_index (media take
- * voxel grids here). */
+ * ImageTexture reads 8-bit PNG from <dir>/texture_<stem>_<ext>/.  A Volume's
+ * density_texture_index becomes ymrt_volume_desc.density_texture. */
 typedef struct ymrt_ymtmp ymrt_ymtmp;
 
 #define YMRT_YMTMP_TRANSFORM 0u

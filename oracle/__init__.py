@@ -95,6 +95,7 @@ def lib():
             L.or_probe_fresnel_dd.argtypes = [C.c_float, C.c_float]
             L.or_probe_fresnel_dc.argtypes = [vp, vp, C.c_float, vp]
             L.or_probe_edge_tint.argtypes = [vp, vp, vp, vp]
+            L.or_probe_light_clamp.argtypes = [C.POINTER(abi.RenderSettings), vp, C.c_int, vp]
             L.or_probe_free_flight.restype = C.c_int
             L.or_probe_free_flight.argtypes = [vp, vp, C.c_float, C.c_float, vp, vp]
             _lib = L
@@ -354,3 +355,11 @@ def free_flight(sigma_t, albedo, tmax, u):
     w = np.zeros(3, np.float32)
     r = lib().or_probe_free_flight(_p(s), _p(a), tmax, u, _p(dist), _p(w))
     return bool(r), float(dist[0]), w
+
+
+def light_clamp(settings, c, depth):
+    """indirect_light_clamp (src/Shading.cu:61-81) of one rgb contribution."""
+    cc = np.ascontiguousarray(c, np.float32)
+    out = np.zeros(3, np.float32)
+    lib().or_probe_light_clamp(C.byref(settings), _p(cc), depth, _p(out))
+    return out

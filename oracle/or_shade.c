@@ -1113,10 +1113,56 @@ static float grid_density(const or_scene *sc, int gi, v3 p)
     return g->density[((size_t)iz * g->ny + iy) * g->nx + ix];
 }
 
+/* heterogeneous: a voxel grid, or a density texture as the reference's
+ * density_texture_idx (src/VolumeEval.cuh:231-238) */
 static int vol_is_het(const or_scene *sc, int v)
 {
     int gi = sc->volumes[v].density_grid;
-    return gi >= 0 && (uint32_t)gi < sc->grid_count;
+    int ti = sc->volumes[v].density_texture;
+    return (gi >= 0 && (uint32_t)gi < sc->grid_count) || (ti >= 0 && (uint32_t)ti < sc->texture_count);
+}
+
+/* Texture::GetUpperBound (src/Texture.h:1181-1251) of the graph's root */
+static float texture_upper_bound(const ymrt_texture_desc *t)
+{
+    switch (t->type) {
+    case YMRT_TEXTURE_IMAGE: return 0.0f;
+    case YMRT_TEXTURE_CONSTANT_FLOAT: return t->value[0];
+    case YMRT_TEXTURE_CONSTANT_RGB: return fmaxr(t->value[0], fmaxr(t->value[1], t->value[2]));
+    case YMRT_TEXTURE_FBM:
+    case YMRT_TEXTURE_TURBULENCE: {
+        float result = 0.0f, amplitude = t->amplitude, offset = t->offset;
+        for (int i = 0; i < t->layer_count; ++i) {
+            result += amplitude + offset;
+            amplitude *= t->gain;
+            offset *= t->gain;
+        }
+        return result;
+    }
+    case YMRT_TEXTURE_WAVE: return safe_rcp(1.0f - t->gain) * safe_rcp(1.0f - t->gain1);
+    default: return 1.0f;  /* checkerboard, noise, marble, wood, polka dot */
+    }
+}
+
+/* majorant of a heterogeneous volume's density */
+static float vol_upper_bound(const or_scene *sc, const ymrt_volume_desc *v)
+{
+    if (v->density_texture >= 0) return texture_upper_bound(sc->textures[v->density_texture].desc);
+    return sc->grids[v->density_grid].max_density;
+}
+
+/* density at a tracking step, max(., 0): the texture's x at (st = 0, world
+ * point, object point) as src/VolumeEval.cuh:165-176, or the grid voxel */
+static float vol_density(const or_scene *sc, const ymrt_volume_desc *v, v3 pw, v3 po)
+{
+    if (v->density_texture >= 0) {
+        or_tex_coord tc;
+        tc.st = mk2(0.0f, 0.0f);
+        tc.p_world = pw;
+        tc.p_object = po;
+        return fmaxr(or_texture_graph(sc, (uint32_t)v->density_texture, tc).x, 0.0f);
+    }
+    return fmaxr(grid_density(sc, v->density_grid, po), 0.0f);
 }
 
 static v3 vsig(const ymrt_volume_desc *v) { return mk3(v->sigma_t[0], v->sigma_t[1], v->sigma_t[2]); }
@@ -1156,7 +1202,7 @@ static int sample_homogeneous(const or_scene *sc, int n, const int *vi, float tm
     return scatter;
 }
 
-/* src/VolumeEval.cuh:80-214, densities from the voxel grid */
+/* src/VolumeEval.cuh:80-214, densities from the voxel grid or a texture graph */
 static int sample_heterogeneous(const or_scene *sc, int n, const int *vi, const or_ray *ray, float tmax,
                                 v3 *trw, float *dist, float *vw, or_sampler *s)
 {
@@ -1170,7 +1216,7 @@ static int sample_heterogeneous(const or_scene *sc, int n, const int *vi, const 
         if (vol_is_het(sc, vi[i])) {
             ro[i] = m4_point(v->world_to_object, ray->o);
             rd[i] = m4_dir(v->world_to_object, ray->d);
-            ub = sc->grids[v->density_grid].max_density;
+            ub = vol_upper_bound(sc, v);
         }
         sm = add3(sm, scl3(vsig(v), ub));
     }
@@ -1200,7 +1246,7 @@ static int sample_heterogeneous(const or_scene *sc, int n, const int *vi, const 
         for (int i = 0; i < n; ++i) {
             const ymrt_volume_desc *v = &sc->volumes[vi[i]];
             float dens = 1.0f;
-            if (vol_is_het(sc, vi[i])) dens = fmaxr(grid_density(sc, v->density_grid, add3(ro[i], scl3(rd[i], acc))), 0.0f);
+            if (vol_is_het(sc, vi[i])) dens = vol_density(sc, v, add3(ray->o, scl3(ray->d, acc)), add3(ro[i], scl3(rd[i], acc)));
             st = add3(st, scl3(vsig(v), dens));
             ss = add3(ss, scl3(mul3(vsig(v), valb(v)), dens));
         }
@@ -1258,7 +1304,7 @@ static v3 eval_tr(const or_scene *sc, const or_ray *ray, const or_transfer *t, f
         if (vol_is_het(sc, vi[i])) {
             ro[i] = m4_point(v->world_to_object, ray->o);
             rd[i] = m4_dir(v->world_to_object, ray->d);
-            ub = sc->grids[v->density_grid].max_density;
+            ub = vol_upper_bound(sc, v);
         }
         sm = add3(sm, scl3(vsig(v), ub));
     }
@@ -1287,7 +1333,7 @@ static v3 eval_tr(const or_scene *sc, const or_ray *ray, const or_transfer *t, f
         for (int i = 0; i < n; ++i) {
             const ymrt_volume_desc *v = &sc->volumes[vi[i]];
             float dens = 1.0f;
-            if (vol_is_het(sc, vi[i])) dens = fmaxr(grid_density(sc, v->density_grid, add3(ro[i], scl3(rd[i], acc))), 0.0f);
+            if (vol_is_het(sc, vi[i])) dens = vol_density(sc, v, add3(ray->o, scl3(ray->d, acc)), add3(ro[i], scl3(rd[i], acc)));
             st = add3(st, scl3(vsig(v), dens));
         }
         v3 rp;
@@ -2329,4 +2375,10 @@ void or_probe_uv_differentials(const or_scene *sc, const float *hp, const float 
     camera_uv_differentials(sc, mk3(hp[0], hp[1], hp[2]), mk3(sn[0], sn[1], sn[2]), mk3(ng[0], ng[1], ng[2]),
                             mk3(dpdu[0], dpdu[1], dpdu[2]), mk3(dpdv[0], dpdv[1], dpdv[2]), width, height, sx, sy,
                             out);
+}
+/* indirect_light_clamp of one contribution (src/Shading.cu:61-81) */
+void or_probe_light_clamp(const ymrt_render_settings *rs, const float *c, int depth, float *out)
+{
+    v3 r = light_clamp(rs, mk3(c[0], c[1], c[2]), depth);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
 }

@@ -500,9 +500,9 @@ def _read(L, ty, stmts):
                 dens = b.ref(v, ("texture",))
         if not name or xf is None:
             b.skip("requires name and transform")
-        if dens is not None:
-            raise ParseError("density textures are not supported")
-        sa.add_volume(sig, alb, g, grid=-1, world_to_object=L.transforms[xf][1].reshape(4, 4).T)
+        tex = -1 if dens is None or dens[1] is None or dens[1] == abi.INVALID else int(dens[1])
+        sa.add_volume(sig, alb, g, grid=-1, world_to_object=L.transforms[xf][1].reshape(4, 4).T,
+                      density_texture=tex)
         L.names[name] = ("volume", len(sa.volumes) - 1)
         return
     if ty == "ImageTexture":

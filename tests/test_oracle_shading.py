@@ -442,3 +442,89 @@ def test_direct_light_distant_cone():
     expect = 0.5 / math.pi * (1.0 + math.cos(math.radians(30.0))) / 2.0
     got = float(img[..., :3].mean())
     assert abs(got - expect) < 0.01 * expect, (got, expect)
+
+
+def _sphere_cells(m, wo, nt, nph, sub, valid=None, **kw):
+    """Per cell of (cos theta in [-1, 1], phi) the eval pdf and the eval f
+    integrated over the cell (midpoint rule on sub x sub points), over the
+    directions wi where valid(wi) holds (all when valid is None)."""
+    pdf_cells = np.zeros((nt, nph))
+    f_cells = np.zeros((nt, nph, 3))
+    dct, dph = 2.0 / nt, 2 * np.pi / nph
+    for it in range(nt):
+        for ip in range(nph):
+            acc, accf = 0.0, np.zeros(3)
+            for a in range(sub):
+                ct = 1.0 - (it + (a + 0.5) / sub) * dct
+                st = math.sqrt(max(0.0, 1 - ct * ct))
+                for b in range(sub):
+                    p = (ip + (b + 0.5) / sub) * dph
+                    wi = (st * math.cos(p), st * math.sin(p), ct)
+                    if valid is not None and not valid(wi):
+                        continue
+                    f, pdf = oracle.bsdf_eval(m, wo, wi, **kw)
+                    acc += pdf
+                    accf += f
+            pdf_cells[it, ip] = acc / sub / sub * dct * dph
+            f_cells[it, ip] = accf / sub / sub * dct * dph
+    return pdf_cells, f_cells
+
+
+@pytest.mark.parametrize("rough", [0.3, 0.6])
+def test_ggx_transmission_follows_pdf_and_conserves(rough):
+    """Rough dielectric with transmission (MicrofacetTransmission,
+    src/BSDF.cuh:285-375, in the DefaultMtl mixture): sampled directions over
+    the whole sphere follow the eval pdf (chi-square over cells on both sides,
+    so a wrong half-vector Jacobian in the BTDF pdf fails); the mean sampled
+    weight f/pdf equals the eval f integrated over the sphere (the directional
+    albedo), and that albedo is at most 1 (no energy gain).
+
+    The reference's transmission Eval and PDF test only that the half-vector
+    h = -(ni wo + nt wi) lies in wo's hemisphere, not that it faces wo
+    (pbrt's Dot(wo, h) * Dot(wi, h) < 0), so they give density and value to
+    transmitted directions whose h has wo.h < 0, which Sample never draws
+    (it requires wo.h >= 0).  The oracle keeps that; the checks integrate over
+    the directions the sampler can reach (reflected ones, and transmitted
+    ones with wo.h >= 0), and the unreachable region is where the full-sphere
+    pdf integrates above 1."""
+    m = abi.default_material(diffuse_albedo=(1.0, 1.0, 1.0), specular_albedo=(1.0, 1.0, 1.0),
+                             roughness_x=rough, roughness_y=rough, transmission_weight=1.0, ior_n=1.5)
+    wo = np.array([0.6, 0.0, 0.8])
+    rng = np.random.RandomState(12)
+    nt, nph = 16, 8
+    counts = np.zeros((nt, nph))
+    wsum = np.zeros(3)
+    n = 60000
+    for _ in range(n):
+        ok, w, wi, pdf = oracle.bsdf_sample(m, wo, rng.rand(3))
+        if not ok:
+            continue
+        wsum += w
+        ct = min(max(wi[2], -1.0), 1.0)
+        it = min(int((1.0 - ct) / 2.0 * nt), nt - 1)
+        ip = int((math.atan2(wi[1], wi[0]) % (2 * np.pi)) / (2 * np.pi) * nph) % nph
+        counts[it, ip] += 1
+    def reachable(wi):
+        if wi[2] * wo[2] > 0:
+            return True
+        h = -(1.0 * wo + 1.5 * np.asarray(wi))
+        if h[2] * wo[2] < 0:           # zero in the eval's own hemisphere test
+            return True
+        return float(np.dot(wo, h)) >= 0.0
+
+    pdf_cells, f_cells = _sphere_cells(m, wo, nt, nph, 10, valid=reachable)
+    full_pdf, _ = _sphere_cells(m, wo, nt, nph, 10)
+    expect = pdf_cells * n
+    # transmitted directions are a real share of the samples
+    assert counts[nt // 2:].sum() > 0.3 * n
+    chi2 = float((((counts - expect) ** 2) / np.maximum(expect, 1.0)).sum())
+    # 128 cells: chi2 has mean 128 and sd 16 when the samples follow the pdf
+    assert chi2 < 200, chi2
+    albedo_quad = f_cells.sum(axis=(0, 1))
+    albedo_mc = wsum / n
+    assert np.allclose(albedo_mc, albedo_quad, rtol=0.02), (albedo_mc, albedo_quad)
+    assert np.all(albedo_quad <= 1.0 + 1e-3), albedo_quad
+    # the reachable pdf mass is all drawn (but for rare failed samples); the
+    # unreachable region holds the excess of the full-sphere integral
+    assert abs(pdf_cells.sum() - counts.sum() / n) < 0.01, (pdf_cells.sum(), counts.sum() / n)
+    assert full_pdf.sum() >= pdf_cells.sum()

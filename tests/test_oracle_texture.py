@@ -49,6 +49,19 @@ def test_permutation_table():
     assert P[:4].tolist() == [172, 142, 240, 182] and P[-2:].tolist() == [170, 202]
 
 
+def test_permutation_table_equals_reference_fixture():
+    """Every entry of the table the device code and the oracle include
+    (ymrt_amd/csrc/perlin_perm.inc) equals the reference's, as read from
+    src/TextureEval.cuh by tests/golden/make_perlin_fixture.py into
+    tests/golden/perlin_perm.json (512 entries: the permutation twice)."""
+    import json
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "perlin_perm.json")))
+    ref = np.array(fx["table"], np.int64)
+    assert fx["count"] == 512 and ref.shape == (512,)
+    assert np.array_equal(ref[:256], ref[256:])
+    assert np.array_equal(_perm(), ref[:256])
+
+
 def test_perlin_zero_on_lattice():
     rng = np.random.default_rng(1)
     pts = rng.integers(-300, 300, size=(500, 3)).astype(np.float32)
@@ -318,3 +331,31 @@ def test_bump_ramp_closed_form():
     cos_a = 2.0 / math.sqrt(4.0 + hp * hp)
     expect = 0.5 / math.pi * cos_a * (1 + math.cos(math.radians(30.0))) / 2
     assert abs(got - expect) < 0.01 * expect, (got, expect)
+
+
+def test_polka_dot_centred_disc_at_radius_half():
+    """At radius 0.5 the centre jitter's amplitude (0.5 - radius) is zero, so
+    the dot of a cell with positive centre noise is exactly the disc of
+    radius 0.5 about the cell centre (EvalTexPerlinNoisePolkaDot,
+    src/TextureEval.cuh:322-346); the jitter's hash of the point itself
+    (not of the cell) is what the general case adds."""
+    sa = abi.SceneArrays()
+    black = sa.add_texture_node(abi.TEXTURE_CONSTANT_FLOAT, value=(0.0, 0.0, 0.0))
+    white = sa.add_texture_node(abi.TEXTURE_CONSTANT_FLOAT, value=(1.0, 0.0, 0.0))
+    dot = sa.add_texture_node(abi.TEXTURE_POLKA_DOT, (black, white), frequency=3.0, radius=0.5)
+    sc = oracle.Scene(sa)
+    rng = np.random.default_rng(11)
+    c = np.zeros((6000, 8), np.float32)
+    c[:, 0:2] = rng.uniform(0, 5, size=(6000, 2))
+    got = sc.texture_graph(dot, c)[:, 0]
+    sc.close()
+    st = c[:, 0:2] * np.float32(3.0)
+    cell = np.floor(st)
+    centre = oracle.perlin(np.concatenate([cell + 0.5, np.zeros((len(c), 1), np.float32)], 1))
+    d = (st - cell).astype(np.float64) - 0.5
+    r2 = (d * d).sum(axis=1)
+    inside = (centre > 0) & (r2 < 0.25)
+    edge = np.abs(r2 - 0.25) < 1e-5
+    assert np.all((got == 1.0) == inside | edge & (got == 1.0))
+    assert np.all(got[~edge] == inside[~edge].astype(np.float32))
+    assert 0.1 < inside.mean() < 0.6

@@ -295,10 +295,21 @@ Begin(Camera); from = <1,2,3>; look = <1,2,2>; fov = 0; End(Camera);
     assert sa.camera["fov"] == float(np.float32(np.float32(0.01) * np.float32(0.017453292519943295)))
 
 
-def test_density_texture_volume_is_unsupported():
-    text = BASE + 'Begin(Volume); name = "v"; transform_index = "t"; density_texture_index = "c1"; End(Volume);'
-    with pytest.raises(ymrt_amd.YmrtError, match="densities from texture graphs"):
-        _product_text(text)
+def test_density_texture_volume():
+    """A Volume's density_texture_index names the texture whose x channel is
+    its density (src/core/YMParser.cpp:2210, Volume::density_texture_idx);
+    without it the volume is homogeneous.  Both readers agree."""
+    text = (BASE + 'Begin(Volume); name = "v"; transform_index = "t"; sigma_t = <0.5,0.5,0.5>; '
+            'density_texture_index = "c1"; End(Volume);\n'
+            'Begin(Volume); name = "w"; transform_index = "t"; End(Volume);')
+    f = _product_text(text)
+    sd = f.desc()
+    assert sd.volume_count == 2
+    c1 = f.find("c1")[1]
+    assert sd.volumes[0].density_texture == c1 and sd.volumes[0].density_grid == -1
+    assert sd.volumes[1].density_texture == -1
+    o = oracle.ymtmp.load_text(text)
+    assert [v["density_texture"] for v in o.arrays.volumes] == [c1, -1]
 
 
 def test_file_type_and_missing_file(tmp_path):

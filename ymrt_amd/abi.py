@@ -96,6 +96,7 @@ class VolumeDesc(C.Structure):
     _fields_ = [
         ("sigma_t", C.c_float * 3), ("albedo", C.c_float * 3), ("g", C.c_float),
         ("density_grid", C.c_int32), ("world_to_object", C.c_float * 16),
+        ("density_texture", C.c_int32),
     ]
 
 
@@ -316,9 +317,10 @@ class SceneArrays:
         self.grids.append(d)
         return len(self.grids) - 1
 
-    def add_volume(self, sigma_t, albedo, g, grid=-1, world_to_object=None):
+    def add_volume(self, sigma_t, albedo, g, grid=-1, world_to_object=None, density_texture=-1):
         w2o = np.eye(4, dtype=np.float32) if world_to_object is None else np.asarray(world_to_object, np.float32)
-        self.volumes.append(dict(sigma_t=sigma_t, albedo=albedo, g=g, grid=grid, w2o=w2o))
+        self.volumes.append(dict(sigma_t=sigma_t, albedo=albedo, g=g, grid=grid, w2o=w2o,
+                                 density_texture=int(density_texture)))
         return len(self.volumes) - 1
 
     def add_distant_light(self, color, intensity=1.0, theta_max_deg=5.0, object_to_world=None,
@@ -405,6 +407,7 @@ class SceneArrays:
             d.albedo[:] = [float(x) for x in v["albedo"]]
             d.g = v["g"]
             d.density_grid = v["grid"]
+            d.density_texture = v.get("density_texture", -1)
             d.world_to_object[:] = v["w2o"].T.reshape(-1).tolist()
         sd = SceneDesc()
         sd.mesh_count, sd.meshes = len(self.meshes), md
@@ -510,7 +513,7 @@ def arrays_from_desc(sd):
     for i in range(sd.volume_count):
         v = sd.volumes[i]
         sa.add_volume(tuple(v.sigma_t), tuple(v.albedo), v.g, grid=v.density_grid,
-                      world_to_object=_mat(v.world_to_object))
+                      world_to_object=_mat(v.world_to_object), density_texture=v.density_texture)
     for i in range(sd.instance_count):
         d = sd.instances[i]
         geom = SphereRef(d.geometry) if d.geometry_type == GEOMETRY_SPHERE else d.geometry

@@ -23,7 +23,15 @@ int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, floa
 int accumulate(const float *noise, float *accum, uint32_t w, uint32_t h, int frame, hipStream_t s);
 int tonemap(float *rgba, uint32_t w, uint32_t h, float gamma, float exposure, hipStream_t s);
 
-static int check_stack_flag(ymrt_scene *sc)
+// The overflow flag describes one call: cleared on the call's stream before
+// its kernels, read after the stream has drained.
+int clear_stack_flag(ymrt_scene *sc, hipStream_t stream)
+{
+    YM_HIP(hipMemsetAsync(sc->d_error, 0, sizeof(uint32_t), stream));
+    return YMRT_OK;
+}
+
+int check_stack_flag(ymrt_scene *sc)
 {
     uint32_t flag = 0;
     YM_HIP(hipMemcpy(&flag, sc->d_error, sizeof(flag), hipMemcpyDeviceToHost));
@@ -118,7 +126,9 @@ int ymrt_trace_closest(ymrt_scene *sc, const ymrt_ray *rays, uint32_t n, ymrt_hi
 {
     YM_REQUIRE(sc != nullptr, "scene is NULL");
     YM_REQUIRE(n == 0 || (rays && hits), "NULL buffer");
-    int rc = launch_trace(sc, rays, n, hits, nullptr, nullptr, 0, (hipStream_t)stream);
+    int rc = clear_stack_flag(sc, (hipStream_t)stream);
+    if (rc) return rc;
+    rc = launch_trace(sc, rays, n, hits, nullptr, nullptr, 0, (hipStream_t)stream);
     if (rc) return rc;
     YM_HIP(hipStreamSynchronize((hipStream_t)stream));
     return check_stack_flag(sc);
@@ -128,7 +138,9 @@ int ymrt_trace_any(ymrt_scene *sc, const ymrt_ray *rays, uint32_t n, uint32_t *o
 {
     YM_REQUIRE(sc != nullptr, "scene is NULL");
     YM_REQUIRE(n == 0 || (rays && occluded), "NULL buffer");
-    int rc = launch_trace(sc, rays, n, nullptr, occluded, nullptr, 1, (hipStream_t)stream);
+    int rc = clear_stack_flag(sc, (hipStream_t)stream);
+    if (rc) return rc;
+    rc = launch_trace(sc, rays, n, nullptr, occluded, nullptr, 1, (hipStream_t)stream);
     if (rc) return rc;
     YM_HIP(hipStreamSynchronize((hipStream_t)stream));
     return check_stack_flag(sc);
@@ -139,7 +151,9 @@ int ymrt_trace_closest_counted(ymrt_scene *sc, const ymrt_ray *rays, uint32_t n,
 {
     YM_REQUIRE(sc != nullptr, "scene is NULL");
     YM_REQUIRE(n == 0 || (rays && hits && visits), "NULL buffer");
-    int rc = launch_trace(sc, rays, n, hits, nullptr, visits, 0, (hipStream_t)stream);
+    int rc = clear_stack_flag(sc, (hipStream_t)stream);
+    if (rc) return rc;
+    rc = launch_trace(sc, rays, n, hits, nullptr, visits, 0, (hipStream_t)stream);
     if (rc) return rc;
     YM_HIP(hipStreamSynchronize((hipStream_t)stream));
     return check_stack_flag(sc);

@@ -6,8 +6,10 @@
 // frames: a few host threads, each with a stream of its own (default
 // priority), take scenes longest first and render their frames with the
 // single-frame path, so the small launches and per-chunk counter reads of one
-// frame overlap with another frame's kernels.  Frames of one scene share that
-// scene's path-state buffers and run in item order on one stream.
+// frame overlap with another frame's kernels.  Frames of one scene run in
+// item order on one stream.  Each thread owns one set of path-state buffers
+// for every frame it renders (grown as frames need, freed at the end), so
+// the memory a batch holds does not grow with its scene count.
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -20,10 +22,6 @@
 #include "internal.hpp"
 
 namespace ym {
-
-int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, float *noise, uint32_t *instance_ids,
-                 const float *adj_img, float *adj_grad, hipStream_t stream, int timing,
-                 ymrt_frame_stats *stats);  // render.cpp
 
 constexpr uint32_t kDefaultBatchStreams = 4;
 
@@ -99,15 +97,21 @@ int render_batch(ymrt_context *ctx, const ymrt_batch_item *items, uint32_t n, ui
             fail(YMRT_ERR_DEVICE, "hipStreamCreateWithFlags failed in a batch thread");
             return;
         }
+        RenderBuffers *buffers = nullptr;
         for (;;) {
             const size_t k = next.fetch_add(1);
             if (k >= order.size() || failed) break;
+            // the scene's overflow flag covers all of its frames in this batch
+            if (int rc = clear_stack_flag(items[groups[order[k]].front()].scene, s)) {
+                fail(rc, std::string("batch: ") + ymrt_last_error());
+                break;
+            }
             for (uint32_t i : groups[order[k]]) {
                 const ymrt_batch_item &it = items[i];
                 int rc;
                 try {
-                    rc = render_frame(it.scene, it.settings, it.frame_index, it.noise_rgba, nullptr, nullptr, nullptr,
-                                      s, it.timing, it.stats);
+                    rc = render_frame_with(it.scene, buffers, it.settings, it.frame_index, it.noise_rgba, nullptr,
+                                           nullptr, nullptr, s, it.timing, it.stats);
                 } catch (const std::exception &e) {
                     set_error(std::string("render failed: ") + e.what());
                     rc = YMRT_ERR_INTERNAL;
@@ -119,6 +123,7 @@ int render_batch(ymrt_context *ctx, const ymrt_batch_item *items, uint32_t n, ui
             }
         }
         if (hipStreamSynchronize(s) != hipSuccess) fail(YMRT_ERR_DEVICE, "batch stream failed");
+        free_render_buffers(buffers);
         if (hipStreamDestroy(s) != hipSuccess) fail(YMRT_ERR_DEVICE, "hipStreamDestroy failed in a batch thread");
     };
     std::vector<std::thread> pool;
@@ -128,6 +133,16 @@ int render_batch(ymrt_context *ctx, const ymrt_batch_item *items, uint32_t n, ui
     if (err_rc != YMRT_OK) {
         set_error(err_msg);
         return err_rc;
+    }
+    // a traversal overflow in any scene's frames fails the batch, as it fails
+    // a single frame
+    for (size_t g = 0; g < groups.size(); ++g) {
+        const uint32_t i = groups[g].front();
+        const int rc = check_stack_flag(items[i].scene);
+        if (rc) {
+            set_error("batch item " + std::to_string(i) + ": " + ymrt_last_error());
+            return rc;
+        }
     }
     return YMRT_OK;
 }

@@ -113,8 +113,10 @@ struct DevVolume {
     float sigma_t[3];
     float albedo[3];
     float g;
-    int32_t grid;
+    int32_t grid;                  // voxel-grid density, or -1
     Affine w2o;
+    int32_t tex;                   // texture-graph density (x channel), or -1
+    float tex_bound;               // its majorant, Texture::GetUpperBound
 };
 
 struct DevLight {
@@ -172,6 +174,7 @@ struct DevScene {
     uint32_t has_boundaries;       // any instance with treat_as_boundary
     uint32_t has_spheres;          // any sphere instance
     uint32_t has_graph;            // procedural texture nodes or normal / bump maps
+    uint32_t has_tex_media;        // any volume whose density is a texture graph
     const uint32_t *mat_class;     // per material: kMatDiffuse, kMatMetal or 0 (shading variant)
     DevCamera camera;
 };

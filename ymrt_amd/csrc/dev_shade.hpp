@@ -1092,10 +1092,37 @@ __device__ __forceinline__ float grid_density(const DevScene &sc, int gi, f3 p)
     return sc.voxels[g.offset + ((uint64_t)iz * g.ny + iy) * g.nx + ix];
 }
 
+// Heterogeneous media: a voxel grid, or a texture graph as the reference's
+// density_texture_idx (SampleVolumeScattering's test, src/VolumeEval.cuh:231-238).
 __device__ __forceinline__ bool vol_is_het(const DevScene &sc, int v)
 {
-    const int gi = sc.volumes[v].grid;
-    return gi >= 0 && (uint32_t)gi < sc.grid_count;
+    const DevVolume &dv = sc.volumes[v];
+    return (dv.grid >= 0 && (uint32_t)dv.grid < sc.grid_count) ||
+           (dv.tex >= 0 && (uint32_t)dv.tex < sc.texture_count);
+}
+
+// Majorant of a heterogeneous volume's density: Texture::GetUpperBound
+// (src/Texture.h:1181-1251, computed on the host) or the grid's maximum.
+__device__ __forceinline__ float vol_bound(const DevScene &sc, const DevVolume &v)
+{
+    return v.tex >= 0 ? v.tex_bound : sc.grids[v.grid].max_density;
+}
+
+// Density of a heterogeneous volume at a tracking step, max(., 0): the x
+// channel of its texture graph at (st = 0, world point, object point) as
+// src/VolumeEval.cuh:165-176 evaluates it, or the grid's voxel at the object
+// point.  kTex: the scene has texture-density media (the graph evaluator and
+// its LDS stack, tex_stk, are compiled in).
+template <bool kTex>
+__device__ __forceinline__ float vol_density(const DevScene &sc, const DevVolume &v, f3 pw, f3 po, float *tex_stk)
+{
+    if constexpr (kTex) {
+        if (v.tex >= 0) {
+            TexEval<true> tx{sc, TexCoord{f2{0.0f, 0.0f}, pw, po}, tex_stk, (int)__lane_id()};
+            return gmax(tx.eval((uint32_t)v.tex).x, 0.0f);
+        }
+    }
+    return gmax(grid_density(sc, v.grid, po), 0.0f);
 }
 
 __device__ __forceinline__ f3 vsig(const DevVolume &v) { return f3{v.sigma_t[0], v.sigma_t[1], v.sigma_t[2]}; }
@@ -1157,9 +1184,10 @@ __device__ __forceinline__ bool sample_homogeneous(const DevScene &sc, int n, co
     return scatter;
 }
 
-// src/VolumeEval.cuh:80-214 with voxel-grid densities
+// src/VolumeEval.cuh:80-214 with voxel-grid or texture densities
+template <bool kTex>
 __device__ __forceinline__ bool sample_heterogeneous(const DevScene &sc, int n, const int *vi, f3 ro_w, f3 rd_w,
-                                                     float tmax, f3 &trw, float &dist, Sampler &s)
+                                                     float tmax, f3 &trw, float &dist, Sampler &s, float *tex_stk)
 {
     trw = s3(1.0f);
     dist = kFloatMax;
@@ -1173,7 +1201,7 @@ __device__ __forceinline__ bool sample_heterogeneous(const DevScene &sc, int n, 
         if (vol_is_het(sc, vi[i])) {
             ro[i] = xf_point(v.w2o, ro_w);
             rd[i] = xf_dir(v.w2o, rd_w);
-            ub = sc.grids[v.grid].max_density;
+            ub = vol_bound(sc, v);
         }
         sm = sm + vsig(v) * ub;
     }
@@ -1205,7 +1233,7 @@ __device__ __forceinline__ bool sample_heterogeneous(const DevScene &sc, int n, 
             if (i >= n) continue;
             const DevVolume &v = sc.volumes[vi[i]];
             float dens = 1.0f;
-            if (vol_is_het(sc, vi[i])) dens = gmax(grid_density(sc, v.grid, ro[i] + rd[i] * acc), 0.0f);
+            if (vol_is_het(sc, vi[i])) dens = vol_density<kTex>(sc, v, ro_w + rd_w * acc, ro[i] + rd[i] * acc, tex_stk);
             st = st + vsig(v) * dens;
             ss = ss + (vsig(v) * valb(v)) * dens;
         }
@@ -1231,17 +1259,18 @@ __device__ __forceinline__ bool sample_heterogeneous(const DevScene &sc, int n, 
 
 // EvalTr (src/VolumeEval.cuh:253-418): transmittance of one segment inside
 // the current media.
-// kGrid: the scene has voxel-grid media; without them only the homogeneous
-// closed form is compiled.
-template <bool kGrid>
+// kHet: 0 every medium is homogeneous (only the closed form is compiled),
+// 1 voxel-grid media, 2 texture-density media as well (tex_stk: the texture
+// graph's LDS stack).
+template <int kHet>
 __device__ __forceinline__ f3 eval_tr(const DevScene &sc, f3 ro_w, f3 rd_w, const Transfer &t, float tmax,
-                                      HashRng &rng)
+                                      HashRng &rng, float *tex_stk)
 {
     if (sc.volume_count == 0) return s3(1.0f);
     int vi[kMaxBoundaryRecord];
     const int n = t.volumes(vi, sc.volume_count);
     if (n == 0) return s3(1.0f);
-    if constexpr (!kGrid) {
+    if constexpr (kHet == 0) {
         f3 st = s3(0.0f);
 #pragma unroll
         for (int i = 0; i < kMaxBoundaryRecord; ++i)
@@ -1269,7 +1298,7 @@ __device__ __forceinline__ f3 eval_tr(const DevScene &sc, f3 ro_w, f3 rd_w, cons
         if (vol_is_het(sc, vi[i])) {
             ro[i] = xf_point(v.w2o, ro_w);
             rd[i] = xf_dir(v.w2o, rd_w);
-            ub = sc.grids[v.grid].max_density;
+            ub = vol_bound(sc, v);
         }
         sm = sm + vsig(v) * ub;
     }
@@ -1300,7 +1329,8 @@ __device__ __forceinline__ f3 eval_tr(const DevScene &sc, f3 ro_w, f3 rd_w, cons
             if (i >= n) continue;
             const DevVolume &v = sc.volumes[vi[i]];
             float dens = 1.0f;
-            if (vol_is_het(sc, vi[i])) dens = gmax(grid_density(sc, v.grid, ro[i] + rd[i] * acc), 0.0f);
+            if (vol_is_het(sc, vi[i]))
+                dens = vol_density<kHet == 2>(sc, v, ro_w + rd_w * acc, ro[i] + rd[i] * acc, tex_stk);
             st = st + vsig(v) * dens;
         }
         f3 rp;

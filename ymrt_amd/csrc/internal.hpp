@@ -74,5 +74,16 @@ int launch_texture_eval(const DevScene &sc, uint32_t tex, const float *coords, u
 
 // render.cpp
 void destroy_render_buffers(ymrt_scene *sc);
+void free_render_buffers(RenderBuffers *rb);
+// One frame with the path-state buffers in `buffers` (allocated or grown as
+// the frame needs; the scene's own for the single-frame entry points).
+int render_frame_with(ymrt_scene *sc, RenderBuffers *&buffers, const ymrt_render_settings *rs, int frame,
+                      float *noise, uint32_t *instance_ids, const float *adj_img, float *adj_grad,
+                      hipStream_t stream, int timing, ymrt_frame_stats *stats);
+
+// abi.cpp: the traversal-overflow flag of a scene, cleared before a call's
+// kernels and read once its stream has drained
+int clear_stack_flag(ymrt_scene *sc, hipStream_t stream);
+int check_stack_flag(ymrt_scene *sc);
 
 }  // namespace ym

@@ -324,7 +324,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere 
     __shared__ float pre_park[kPreSlots * kBlock];
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     const uint32_t *qin = ps.queue[pass & 1];
-    LaneScratch<kBlock, kStackWindow, true> ls{stack, nbp, nbt, nbb, (int)lane_id(), ps.spill, kSpillThreads,
+    LaneScratch<kBlock, kStackWindow, true> ls{stack, nbp, nbt, nbb, (int)lane_id(), ps.spill, ps.spill_threads,
                                                blockIdx.x * kBlock + threadIdx.x, pre_park};
     const ymrt_render_settings &rs = prm.rs;
     uint32_t rays = 0, cam_rays = 0;
@@ -552,11 +552,14 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_scatter(PtParams pr
 // (SampleVolumeScattering, src/VolumeEval.cuh:217-251, called from
 // src/Shading.cu:1800-1840): scattering vertices go to the volume shading
 // queue, surviving rays to surface shading or, on a miss, to the environment.
-// kGrid: the scene has voxel-grid media (delta tracking); otherwise every
-// medium is homogeneous and the tracker is compiled out.
-template <bool kGrid>
+// kHet: 1 the scene has voxel-grid media (delta tracking), 2 texture-density
+// media as well (the texture graph evaluator and its LDS stack are compiled
+// in); 0 every medium is homogeneous and the tracker is compiled out.
+template <int kHet>
 __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
 {
+    constexpr bool kGrid = kHet > 0;
+    __shared__ float tex_stk[kHet == 2 ? kTexStack * 3 * kBlock : 1];
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     const uint32_t n = cnt[kCtrMedium];
     const ymrt_render_settings &rs = prm.rs;
@@ -583,7 +586,7 @@ __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PtParams prm, Pa
                 f3 trw;
                 const float tmax = hit ? __uint_as_float(ha.x) : kTMax;
                 if constexpr (kGrid) {
-                    vol_hit = het ? sample_heterogeneous(sc, nv, vi, o, d, tmax, trw, dist, smp)
+                    vol_hit = het ? sample_heterogeneous<kHet == 2>(sc, nv, vi, o, d, tmax, trw, dist, smp, tex_stk)
                                   : sample_homogeneous(sc, nv, vi, tmax, trw, dist, smp);
                 } else {
                     vol_hit = sample_homogeneous(sc, nv, vi, tmax, trw, dist, smp);
@@ -1100,7 +1103,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     const uint32_t cap = prm.cap;
     LaneScratch<kBlock, kStackWindow, true> ls{stack, nullptr, nullptr, nullptr, (int)lane_id(), ps.spill,
-                                               kSpillThreads, blockIdx.x * kBlock + threadIdx.x, pre_park};
+                                               ps.spill_threads, blockIdx.x * kBlock + threadIdx.x, pre_park};
     uint32_t nsh = 0;
     TraceCounters tc{0, 0};
     RefillRange<16> rr{cnt + kCtrWorkShadow, cnt[kCtrShadow], 0u, 0u, true};
@@ -1168,9 +1171,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 // (src/VolumeEval.cuh:420-468) walks the boundary instances along the ray,
 // updating the medium records at each crossing and multiplying the segment
 // transmittances (EvalTr, 253-418).
-template <bool kSphere, int kStack, bool kGrid>
+// kHet as k_medium's.
+template <bool kSphere, int kStack, int kHet>
 __global__ __launch_bounds__(kBlock) void k_transmit(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
 {
+    __shared__ float tex_stk[kHet == 2 ? kTexStack * 3 * kBlock : 1];
     __shared__ uint32_t stack[kStackWindow * kBlock];
     __shared__ uint32_t nbp[kNearbyCap * kBlock];
     __shared__ float nbt[kNearbyCap * kBlock];
@@ -1178,7 +1183,7 @@ __global__ __launch_bounds__(kBlock) void k_transmit(DevScene sc, PtParams prm, 
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     const uint32_t n = cnt[kCtrTransmit];
     const uint32_t cap = prm.cap;
-    LaneScratch<kBlock, kStackWindow> ls{stack, nbp, nbt, nbb, (int)lane_id(), ps.spill, kSpillThreads,
+    LaneScratch<kBlock, kStackWindow> ls{stack, nbp, nbt, nbb, (int)lane_id(), ps.spill, ps.spill_threads,
                                          blockIdx.x * kBlock + threadIdx.x};
     uint32_t ntr = 0;
     TraceCounters tc{0, 0};
@@ -1191,10 +1196,12 @@ __global__ __launch_bounds__(kBlock) void k_transmit(DevScene sc, PtParams prm, 
             const uint4 ha = ps.hit_a[p];
             HashRng rng;
             {
-                const uint32_t gp = (uint32_t)(prm.chunk_base + p);
+                // 64-bit like k_raygen: a frame may hold more than 2^32 path slots
+                const uint64_t gp = prm.chunk_base + p;
                 const uint32_t spp = (uint32_t)prm.rs.spp;
-                const uint32_t vpix = gp / spp, s = gp - vpix * spp;
-                const uint32_t tile = vpix >> 8, within = vpix & 255u;
+                const uint64_t vpix = gp / spp;
+                const uint32_t s = (uint32_t)(gp - vpix * spp);
+                const uint32_t tile = (uint32_t)(vpix >> 8), within = (uint32_t)(vpix & 255u);
                 const uint32_t ty = tile / prm.tiles_x, tx = tile - ty * prm.tiles_x;
                 const uint32_t px = tx * 16u + (within & 15u), py = ty * 16u + (within >> 4);
                 rng.key[0] = py * prm.rs.width + px;
@@ -1239,10 +1246,10 @@ __global__ __launch_bounds__(kBlock) void k_transmit(DevScene sc, PtParams prm, 
                         break;
                     }
                     if (br == 0) {
-                        tr = tr * eval_tr<kGrid>(sc, tq.o, tq.d, t, gmax(maxd - traced, 0.0f), rng);
+                        tr = tr * eval_tr<kHet>(sc, tq.o, tq.d, t, gmax(maxd - traced, 0.0f), rng, tex_stk);
                         break;
                     }
-                    tr = tr * eval_tr<kGrid>(sc, tq.o, tq.d, t, bh.t, rng);
+                    tr = tr * eval_tr<kHet>(sc, tq.o, tq.d, t, bh.t, rng, tex_stk);
                     traced += bh.t;
                     f3 bp, bperr, bng;
                     hit_geometry(sc, bh.inst, bh.tri, bh.b0, bh.b1, bh.b2, bp, bperr, bng);
@@ -1352,21 +1359,22 @@ __global__ void k_tonemap(float4 *img, uint32_t n, float gamma, float exposure)
 // path per lane); never more waves than the items need.  Every pass kernel
 // fetches work dynamically or strides over the grid, so the count changes
 // timing only.
-static dim3 persistent_grid(const PtParams &prm)
+static dim3 persistent_grid(const PtParams &prm, const PathState &ps)
 {
     const uint32_t waves = (prm.chunk_paths + kBlock - 1) / kBlock;
     const uint32_t w = prm.rs.persistent_waves;
     static_assert(kMaxPersistentWaves * kBlock == kSpillThreads, "one spill row per lane");
+    const uint32_t most = ps.spill_threads / kBlock;  // one spill row per lane
     uint32_t cap = (prm.chunk_paths + 255u) / 256u;
     cap = cap < 1024u ? 1024u : cap;
     if (w) cap = w;
-    cap = cap < kMaxPersistentWaves ? cap : kMaxPersistentWaves;
+    cap = cap < most ? cap : most;
     return dim3(waves < cap ? (waves ? waves : 1) : cap);
 }
 
 int launch_raygen(const DevScene &sc, const PtParams &prm, const PathState &ps, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_raygen, persistent_grid(prm), dim3(kBlock), 0, s, sc, prm, ps);
+    hipLaunchKernelGGL(k_raygen, persistent_grid(prm, ps), dim3(kBlock), 0, s, sc, prm, ps);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }
@@ -1374,7 +1382,7 @@ int launch_raygen(const DevScene &sc, const PtParams &prm, const PathState &ps, 
 template <bool kNearby, bool kMedia, bool kSphere>
 static int launch_extend_t(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_extend<kNearby, kMedia, kSphere, kMaxStackDepth>), persistent_grid(prm),
+    hipLaunchKernelGGL((k_extend<kNearby, kMedia, kSphere, kMaxStackDepth>), persistent_grid(prm, ps),
                        dim3(kBlock), 0, s, sc, prm, ps, pass);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
@@ -1383,7 +1391,7 @@ static int launch_extend_t(const DevScene &sc, const PtParams &prm, const PathSt
 template <bool kMedia, bool kSphere>
 static int launch_shadow_t(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_shadow<kMedia, kSphere, kMaxStackDepth>), persistent_grid(prm), dim3(kBlock),
+    hipLaunchKernelGGL((k_shadow<kMedia, kSphere, kMaxStackDepth>), persistent_grid(prm, ps), dim3(kBlock),
                        0, s, sc, prm, ps, pass);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
@@ -1408,12 +1416,18 @@ int launch_extend(const DevScene &sc, const PtParams &prm, const PathState &ps, 
                  : launch_extend_m<false, false>(sc, prm, ps, pass, s);
 }
 
+// 0 homogeneous media only, 1 voxel grids, 2 texture densities too
+static int het_mode(const DevScene &sc) { return sc.has_tex_media ? 2 : sc.grid_count > 0 ? 1 : 0; }
+
 int launch_medium(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
 {
-    if (sc.grid_count > 0)
-        hipLaunchKernelGGL(k_medium<true>, persistent_grid(prm), dim3(kBlock), 0, s, sc, prm, ps, pass);
+    const int het = het_mode(sc);
+    if (het == 2)
+        hipLaunchKernelGGL(k_medium<2>, persistent_grid(prm, ps), dim3(kBlock), 0, s, sc, prm, ps, pass);
+    else if (het == 1)
+        hipLaunchKernelGGL(k_medium<1>, persistent_grid(prm, ps), dim3(kBlock), 0, s, sc, prm, ps, pass);
     else
-        hipLaunchKernelGGL(k_medium<false>, persistent_grid(prm), dim3(kBlock), 0, s, sc, prm, ps, pass);
+        hipLaunchKernelGGL(k_medium<0>, persistent_grid(prm, ps), dim3(kBlock), 0, s, sc, prm, ps, pass);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }
@@ -1438,7 +1452,7 @@ int launch_compact(const PtParams &prm, const PathState &ps, uint32_t pass, int 
 int launch_shade(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool volume,
                  hipStream_t s)
 {
-    const dim3 g = persistent_grid(prm), b(kBlock);
+    const dim3 g = persistent_grid(prm, ps), b(kBlock);
     if (volume)
         hipLaunchKernelGGL((k_shade<true, false>), g, b, 0, s, sc, prm, ps, pass);
     else if (sc.has_graph) {
@@ -1464,17 +1478,16 @@ int launch_shadow(const DevScene &sc, const PtParams &prm, const PathState &ps, 
 
 int launch_transmit(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
 {
-    const dim3 g = persistent_grid(prm), b(kBlock);
-    if (sc.grid_count > 0) {
-        if (sc.has_spheres)
-            hipLaunchKernelGGL((k_transmit<true, kMaxStackDepth, true>), g, b, 0, s, sc, prm, ps, pass);
-        else
-            hipLaunchKernelGGL((k_transmit<false, kMaxStackDepth, true>), g, b, 0, s, sc, prm, ps, pass);
+    const dim3 g = persistent_grid(prm, ps), b(kBlock);
+    const int het = het_mode(sc);
+    if (sc.has_spheres) {
+        if (het == 2) hipLaunchKernelGGL((k_transmit<true, kMaxStackDepth, 2>), g, b, 0, s, sc, prm, ps, pass);
+        else if (het == 1) hipLaunchKernelGGL((k_transmit<true, kMaxStackDepth, 1>), g, b, 0, s, sc, prm, ps, pass);
+        else hipLaunchKernelGGL((k_transmit<true, kMaxStackDepth, 0>), g, b, 0, s, sc, prm, ps, pass);
     } else {
-        if (sc.has_spheres)
-            hipLaunchKernelGGL((k_transmit<true, kMaxStackDepth, false>), g, b, 0, s, sc, prm, ps, pass);
-        else
-            hipLaunchKernelGGL((k_transmit<false, kMaxStackDepth, false>), g, b, 0, s, sc, prm, ps, pass);
+        if (het == 2) hipLaunchKernelGGL((k_transmit<false, kMaxStackDepth, 2>), g, b, 0, s, sc, prm, ps, pass);
+        else if (het == 1) hipLaunchKernelGGL((k_transmit<false, kMaxStackDepth, 1>), g, b, 0, s, sc, prm, ps, pass);
+        else hipLaunchKernelGGL((k_transmit<false, kMaxStackDepth, 0>), g, b, 0, s, sc, prm, ps, pass);
     }
     YM_HIP(hipGetLastError());
     return YMRT_OK;

@@ -88,7 +88,8 @@ struct PathState {
     float4 *out;        // final radiance per path
     uint32_t *queue[2];
     uint32_t *q_hit, *q_hit_vol, *q_hit_diffuse, *q_hit_metal, *q_medium, *q_shadow, *q_transmit;
-    uint32_t *spill;    // traversal stack spill [kMaxStackDepth - kStackWindow][kSpillThreads]
+    uint32_t *spill;    // traversal stack spill [kMaxStackDepth - kStackWindow][spill_threads]
+    uint32_t spill_threads;  // spill rows: the most lanes a persistent pass kernel may launch
     uint32_t *flag;       // stage flag per slot (kFlag*)
     uint32_t *ccount;     // compaction: matches per block, [4][cap / kCompactSpan + 1]
     uint32_t *first_hit;  // instance hit by the camera ray (written when an id image is requested)

@@ -11,6 +11,8 @@
 // pass-throughs need more passes.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <type_traits>
 
 #include "internal.hpp"
 #include "pt.hpp"
@@ -65,6 +67,12 @@ void destroy_render_buffers(ymrt_scene *sc)
     sc->render = nullptr;
 }
 
+void free_render_buffers(RenderBuffers *rb) { delete rb; }
+
+// Capacity choices read the free device memory and then allocate; threads
+// of a batch would otherwise all see the same free memory.
+static std::mutex g_alloc_mu;
+
 template <typename T>
 static int alloc(RenderBuffers *rb, T **p, size_t count)
 {
@@ -75,56 +83,132 @@ static int alloc(RenderBuffers *rb, T **p, size_t count)
     return YMRT_OK;
 }
 
-static int ensure_buffers(ymrt_scene *sc, uint32_t cap)
+static int run_frame(ymrt_scene *sc, RenderBuffers *rb, uint32_t chunk_paths_max, const ymrt_render_settings *rs,
+                     int frame, float *noise, uint32_t *instance_ids, const float *adj_img, float *adj_grad,
+                     hipStream_t stream, int timing, ymrt_frame_stats *stats);
+
+static int check_settings(const ymrt_render_settings *rs)
 {
-    if (sc->render && sc->render->cap >= cap) return YMRT_OK;
-    destroy_render_buffers(sc);
-    RenderBuffers *rb = new RenderBuffers;
-    sc->render = rb;
-    rb->cap = cap;
-    PathState &ps = rb->ps;
+    YM_REQUIRE(rs->width > 0 && rs->height > 0, "image size must be positive");
+    YM_REQUIRE(rs->spp >= 1, "spp must be >= 1");
+    YM_REQUIRE(rs->ray_depth >= 1 && rs->ray_depth <= kMaxRayDepth, "ray_depth must be in 1..64");
+    YM_REQUIRE(rs->sampler_type == YMRT_SAMPLER_HALTON || rs->sampler_type == YMRT_SAMPLER_PCG,
+               "sampler_type must be PCG (0) or HALTON (1)");
+    return YMRT_OK;
+}
+
+// Lays out the path state of n paths (and the pass counters and statistics)
+// through get(pointer, count), which allocates or carves each array.
+template <typename Get>
+static int layout_buffers(PathState &ps, uint32_t *&counters, unsigned long long *&stats, size_t n,
+                          uint32_t spill_threads, Get &&get)
+{
     int rc = 0;
-    const size_t n = cap;
-    rc |= alloc(rb, &ps.ray_o, n);
-    rc |= alloc(rb, &ps.ray_d, n);
-    rc |= alloc(rb, &ps.last, n);
-    rc |= alloc(rb, &ps.beta, n);
-    rc |= alloc(rb, &ps.L, n);
-    rc |= alloc(rb, &ps.smp, n);
-    rc |= alloc(rb, &ps.info, n);
-    rc |= alloc(rb, &ps.xhdr, n);
-    rc |= alloc(rb, &ps.xrec, n * kMaxBoundaryRecord);
-    rc |= alloc(rb, &ps.hit_a, n);
-    rc |= alloc(rb, &ps.hit_b, n);
-    rc |= alloc(rb, &ps.nb, n * 4);
-    rc |= alloc(rb, &ps.sh_o, n * kShadowSlots);
-    rc |= alloc(rb, &ps.sh_d, n * kShadowSlots);
-    rc |= alloc(rb, &ps.sh_f, n * kShadowSlots);
-    rc |= alloc(rb, &ps.sh_li, n * kShadowSlots);
-    rc |= alloc(rb, &ps.sh_tr, n * kShadowSlots);
-    rc |= alloc(rb, &ps.sh_c, n);
-    rc |= alloc(rb, &ps.sh_flags, n);
-    rc |= alloc(rb, &ps.out, n);
-    rc |= alloc(rb, &ps.queue[0], n);
-    rc |= alloc(rb, &ps.queue[1], n);
-    rc |= alloc(rb, &ps.q_hit, n);
-    rc |= alloc(rb, &ps.q_hit_vol, n);
-    rc |= alloc(rb, &ps.q_hit_diffuse, n);
-    rc |= alloc(rb, &ps.q_hit_metal, n);
-    rc |= alloc(rb, &ps.q_medium, n);
-    rc |= alloc(rb, &ps.flag, n);
-    rc |= alloc(rb, &ps.ccount, 4 * ((size_t)n / kCompactSpan + 1));
-    rc |= alloc(rb, &ps.q_shadow, n);
-    rc |= alloc(rb, &ps.q_transmit, n);
-    rc |= alloc(rb, &ps.spill, (size_t)(kMaxStackDepth - kStackWindow) * kSpillThreads);
-    rc |= alloc(rb, &ps.first_hit, n);
-    rc |= alloc(rb, &rb->counters, (size_t)(kMaxPasses + 2) * kCtrStride);
-    rc |= alloc(rb, &rb->stats, kStatCount);
+    rc |= get(ps.ray_o, n);
+    rc |= get(ps.ray_d, n);
+    rc |= get(ps.last, n);
+    rc |= get(ps.beta, n);
+    rc |= get(ps.L, n);
+    rc |= get(ps.smp, n);
+    rc |= get(ps.info, n);
+    rc |= get(ps.xhdr, n);
+    rc |= get(ps.xrec, n * kMaxBoundaryRecord);
+    rc |= get(ps.hit_a, n);
+    rc |= get(ps.hit_b, n);
+    rc |= get(ps.nb, n * 4);
+    rc |= get(ps.sh_o, n * kShadowSlots);
+    rc |= get(ps.sh_d, n * kShadowSlots);
+    rc |= get(ps.sh_f, n * kShadowSlots);
+    rc |= get(ps.sh_li, n * kShadowSlots);
+    rc |= get(ps.sh_tr, n * kShadowSlots);
+    rc |= get(ps.sh_c, n);
+    rc |= get(ps.sh_flags, n);
+    rc |= get(ps.out, n);
+    rc |= get(ps.queue[0], n);
+    rc |= get(ps.queue[1], n);
+    rc |= get(ps.q_hit, n);
+    rc |= get(ps.q_hit_vol, n);
+    rc |= get(ps.q_hit_diffuse, n);
+    rc |= get(ps.q_hit_metal, n);
+    rc |= get(ps.q_medium, n);
+    rc |= get(ps.flag, n);
+    rc |= get(ps.ccount, 4 * (n / kCompactSpan + 1));
+    rc |= get(ps.q_shadow, n);
+    rc |= get(ps.q_transmit, n);
+    rc |= get(ps.spill, (size_t)(kMaxStackDepth - kStackWindow) * spill_threads);
+    rc |= get(ps.first_hit, n);
+    rc |= get(counters, (size_t)(kMaxPasses + 2) * kCtrStride);
+    rc |= get(stats, (size_t)kStatCount);
+    ps.spill_threads = spill_threads;
+    return rc;
+}
+
+static int ensure_buffers(RenderBuffers *&slot, uint32_t cap)
+{
+    if (slot && slot->cap >= cap) return YMRT_OK;
+    delete slot;
+    RenderBuffers *rb = new RenderBuffers;
+    slot = rb;
+    rb->cap = cap;
+    const int rc = layout_buffers(rb->ps, rb->counters, rb->stats, cap, kSpillThreads,
+                                  [&](auto *&p, size_t count) { return alloc(rb, &p, count); });
     if (rc) {
-        destroy_render_buffers(sc);
+        delete slot;
+        slot = nullptr;
         return rc;
     }
     return YMRT_OK;
+}
+
+// Arrays carved out of fixed device memory start on 256-B boundaries.
+static size_t carve_round(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t fixed_buffer_bytes(uint32_t cap, uint32_t spill_threads)
+{
+    PathState ps{};
+    uint32_t *counters = nullptr;
+    unsigned long long *stats = nullptr;
+    size_t total = 0;
+    layout_buffers(ps, counters, stats, cap, spill_threads, [&](auto *&p, size_t count) {
+        total += carve_round(sizeof(*p) * std::max<size_t>(count, 1));
+        p = nullptr;
+        return 0;
+    });
+    return total;
+}
+
+// One frame whose path state is carved out of `memory` (device, `bytes`
+// long) instead of allocated: chunks of up to cap paths, persistent kernels
+// of up to spill_threads / 64 waves.  Allocates no device memory.
+int render_frame_fixed(ymrt_scene *sc, void *memory, size_t bytes, uint32_t cap, uint32_t spill_threads,
+                       const ymrt_render_settings *rs, int frame, float *noise, hipStream_t stream,
+                       ymrt_frame_stats *stats)
+{
+    if (int r = check_settings(rs)) return r;
+    YM_REQUIRE(memory != nullptr, "no fixed device memory");
+    YM_REQUIRE(fixed_buffer_bytes(cap, spill_threads) <= bytes, "fixed device memory too small for the capacity");
+    YM_REQUIRE(rs->path_capacity <= cap,
+               "path_capacity exceeds the fixed capacity of " + std::to_string(cap) + " paths");
+    const uint32_t spp = (uint32_t)rs->spp;
+    YM_REQUIRE(spp <= cap, "spp exceeds the fixed capacity of " + std::to_string(cap) + " paths");
+    RenderBuffers view;  // owns nothing: no allocations, no events
+    view.cap = cap;
+    char *base = static_cast<char *>(memory);
+    size_t off = 0;
+    layout_buffers(view.ps, view.counters, view.stats, cap, spill_threads, [&](auto *&p, size_t count) {
+        p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(base + off);
+        off += carve_round(sizeof(*p) * std::max<size_t>(count, 1));
+        return 0;
+    });
+    const uint64_t total_paths = (uint64_t)((rs->width + 15) / 16) * ((rs->height + 15) / 16) * 256ull * spp;
+    const uint64_t want = rs->path_capacity ? rs->path_capacity : cap;
+    const uint64_t per_cap = (want / spp) * spp;
+    const uint64_t nchunks = (total_paths + per_cap - 1) / per_cap;
+    const uint32_t chunk_paths_max =
+        (uint32_t)std::min<uint64_t>(per_cap, (total_paths / nchunks + spp - 1) / spp * spp);
+    YM_HIP(hipSetDevice(sc->ctx->device));
+    if (int r = clear_stack_flag(sc, stream)) return r;
+    return run_frame(sc, &view, chunk_paths_max, rs, frame, noise, nullptr, nullptr, nullptr, stream, 0, stats);
 }
 
 static hipEvent_t next_event(RenderBuffers *rb)
@@ -147,30 +231,28 @@ static int ensure_adjoint(RenderBuffers *rb, uint32_t depth)
     return YMRT_OK;
 }
 
-int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, float *noise, uint32_t *instance_ids,
-                 const float *adj_img, float *adj_grad, hipStream_t stream, int timing, ymrt_frame_stats *stats)
+int render_frame_with(ymrt_scene *sc, RenderBuffers *&buffers, const ymrt_render_settings *rs, int frame,
+                      float *noise, uint32_t *instance_ids, const float *adj_img, float *adj_grad,
+                      hipStream_t stream, int timing, ymrt_frame_stats *stats)
 {
-    YM_REQUIRE(rs->width > 0 && rs->height > 0, "image size must be positive");
-    YM_REQUIRE(rs->spp >= 1, "spp must be >= 1");
-    YM_REQUIRE(rs->ray_depth >= 1 && rs->ray_depth <= kMaxRayDepth, "ray_depth must be in 1..64");
-    YM_REQUIRE(rs->sampler_type == YMRT_SAMPLER_HALTON || rs->sampler_type == YMRT_SAMPLER_PCG,
-               "sampler_type must be PCG (0) or HALTON (1)");
+    if (int r = check_settings(rs)) return r;
     YM_HIP(hipSetDevice(sc->ctx->device));
     const uint32_t spp = (uint32_t)rs->spp;
     const uint64_t total_paths = (uint64_t)((rs->width + 15) / 16) * ((rs->height + 15) / 16) * 256ull * spp;
+    std::unique_lock<std::mutex> alloc_lock(g_alloc_mu);
     uint64_t cap = rs->path_capacity;
     if (cap == 0) {
         const uint64_t want = std::min<uint64_t>(kDefaultCapacity, total_paths);
-        if (sc->render && sc->render->cap >= want) {
+        if (buffers && buffers->cap >= want) {
             cap = want;
         } else {
             // grow within a quarter of the free memory (counting the buffers
-            // this scene would give back), never below what it already holds
+            // this call would give back), never below what it already holds
             size_t free_b = 0, total_b = 0;
             YM_HIP(hipMemGetInfo(&free_b, &total_b));
-            const uint64_t held = sc->render ? (uint64_t)sc->render->cap * kPathBytes : 0;
+            const uint64_t held = buffers ? (uint64_t)buffers->cap * kPathBytes : 0;
             cap = std::min<uint64_t>(want, (free_b + held) / 4 / kPathBytes);
-            if (sc->render) cap = std::max<uint64_t>(cap, sc->render->cap);
+            if (buffers) cap = std::max<uint64_t>(cap, buffers->cap);
         }
         cap = std::max<uint64_t>(cap, spp);
     }
@@ -182,10 +264,27 @@ int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, floa
     const uint64_t nchunks = (total_paths + per_cap - 1) / per_cap;
     const uint32_t chunk_paths_max =
         (uint32_t)std::min<uint64_t>(per_cap, (total_paths / nchunks + spp - 1) / spp * spp);
-    int rc = ensure_buffers(sc, chunk_paths_max);
+    int rc = ensure_buffers(buffers, chunk_paths_max);
     if (rc) return rc;
-    RenderBuffers *rb = sc->render;
+    alloc_lock.unlock();
+    return run_frame(sc, buffers, chunk_paths_max, rs, frame, noise, instance_ids, adj_img, adj_grad, stream, timing,
+                     stats);
+}
 
+// The frame itself, in chunks of up to chunk_paths_max paths in rb's path
+// state (rb->cap >= chunk_paths_max, a multiple of spp).
+static int run_frame(ymrt_scene *sc, RenderBuffers *rb, uint32_t chunk_paths_max, const ymrt_render_settings *rs,
+                     int frame, float *noise, uint32_t *instance_ids, const float *adj_img, float *adj_grad,
+                     hipStream_t stream, int timing, ymrt_frame_stats *stats)
+{
+    const uint32_t spp = (uint32_t)rs->spp;
+    const uint64_t total_paths = (uint64_t)((rs->width + 15) / 16) * ((rs->height + 15) / 16) * 256ull * spp;
+    YM_REQUIRE(chunk_paths_max >= spp && chunk_paths_max % spp == 0 && chunk_paths_max <= rb->cap,
+               "chunk size must be whole pixels within the path capacity");
+    YM_REQUIRE(rs->persistent_waves <= rb->ps.spill_threads / 64u,
+               "persistent_waves exceeds " + std::to_string(rb->ps.spill_threads / 64u) +
+                   " (the traversal spill area)");
+    int rc = YMRT_OK;
     PtParams prm;
     std::memset(&prm, 0, sizeof(prm));
     prm.rs = *rs;
@@ -339,6 +438,16 @@ int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, floa
         YM_HIP(hipStreamSynchronize(stream));
     }
     return YMRT_OK;
+}
+
+int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, float *noise, uint32_t *instance_ids,
+                 const float *adj_img, float *adj_grad, hipStream_t stream, int timing, ymrt_frame_stats *stats)
+{
+    YM_HIP(hipSetDevice(sc->ctx->device));
+    const int rc = clear_stack_flag(sc, stream);
+    if (rc) return rc;
+    return render_frame_with(sc, sc->render, rs, frame, noise, instance_ids, adj_img, adj_grad, stream, timing,
+                             stats);
 }
 
 }  // namespace ym

@@ -166,6 +166,39 @@ struct HostScene {
     std::vector<DevMaterial> materials;
 };
 
+// Texture::GetUpperBound (src/Texture.h:1181-1251): the majorant of a
+// density texture, per node type of the graph's root (images 0, so a medium
+// whose only density is an image never scatters; fBm and turbulence the sum
+// of amplitude + offset over their layers, in float as written there).
+static float texture_upper_bound(const ymrt_texture_desc &t)
+{
+    switch (t.type) {
+    case YMRT_TEXTURE_IMAGE:
+        return 0.0f;
+    case YMRT_TEXTURE_CONSTANT_FLOAT:
+        return t.value[0];
+    case YMRT_TEXTURE_CONSTANT_RGB:
+        return std::max(t.value[0], std::max(t.value[1], t.value[2]));
+    case YMRT_TEXTURE_FBM:
+    case YMRT_TEXTURE_TURBULENCE: {
+        float r = 0.0f, amp = t.amplitude, off = t.offset;
+        for (int i = 0; i < t.layer_count; ++i) {
+            r += amp + off;
+            amp *= t.gain;
+            off *= t.gain;
+        }
+        return r;
+    }
+    case YMRT_TEXTURE_WAVE: {
+        // SafeRcp (src/MathCommon.h:103-106)
+        auto safe_rcp = [](float f) { return std::fabs(f) < 1E-30f ? (f > 0.0f ? 1E30f : -1E30f) : 1.0f / f; };
+        return safe_rcp(1.0f - t.gain) * safe_rcp(1.0f - t.gain1);
+    }
+    default:  // checkerboard, noise, marble, wood, polka dot
+        return 1.0f;
+    }
+}
+
 static void camera_setup(const ymrt_scene_desc *d, DevCamera &cam)
 {
     // Camera::SetDir / MakeCameraBasis (src/Camera.h:28-32, 311-322)
@@ -574,10 +607,17 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
         grids.push_back(dg);
     }
     std::vector<DevVolume> vols;
+    bool tex_media = false;
     for (uint32_t i = 0; i < d->volume_count; ++i) {
         const ymrt_volume_desc &v = d->volumes[i];
         YM_REQUIRE(v.density_grid < 0 || (uint32_t)v.density_grid < d->grid_count, "volume grid out of range");
+        YM_REQUIRE(v.density_texture < 0 || (uint32_t)v.density_texture < d->texture_count,
+                   "volume density texture out of range");
+        YM_REQUIRE(v.density_grid < 0 || v.density_texture < 0, "a volume takes a density grid or a density texture");
         DevVolume dv;
+        dv.tex = v.density_texture;
+        dv.tex_bound = v.density_texture >= 0 ? texture_upper_bound(d->textures[v.density_texture]) : 0.0f;
+        tex_media = tex_media || v.density_texture >= 0;
         for (int k = 0; k < 3; ++k) {
             dv.sigma_t[k] = v.sigma_t[k];
             dv.albedo[k] = v.albedo[k];
@@ -711,6 +751,7 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
     if ((rc = upload(sc, magic, &ds.prime_magic))) return rc;
     ds.texture_count = d->texture_count;
     ds.grid_count = d->grid_count;
+    ds.has_tex_media = tex_media ? 1 : 0;
     ds.volume_count = d->volume_count;
     ds.light_count = (uint32_t)lights.size();
     sc->info.shape_light_count = ds.light_count;

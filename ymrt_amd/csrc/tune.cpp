@@ -15,10 +15,6 @@
 
 namespace ym {
 
-int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, float *noise, uint32_t *instance_ids,
-                 const float *adj_img, float *adj_grad, hipStream_t stream, int timing,
-                 ymrt_frame_stats *stats);  // render.cpp
-
 constexpr int kTrialRepeats = 2;
 
 int autotune(ymrt_scene *sc, const ymrt_render_settings *rs_in, int frame, hipStream_t stream,
@@ -29,6 +25,9 @@ int autotune(ymrt_scene *sc, const ymrt_render_settings *rs_in, int frame, hipSt
     YM_REQUIRE(rs_in->spp >= 1 && rs_in->width >= 1 && rs_in->height >= 1, "empty frame");
     const uint64_t spp = (uint64_t)rs_in->spp;
     const uint64_t total = (uint64_t)((rs_in->width + 15) / 16) * ((rs_in->height + 15) / 16) * 256ull * spp;
+    YM_HIP(hipSetDevice(sc->ctx->device));
+    // an overflow in any trial frame fails the tuning
+    if (int r = clear_stack_flag(sc, stream)) return r;
     float *img = nullptr;
     YM_HIP(hipMalloc(&img, sizeof(float) * 4 * rs_in->width * rs_in->height));
     ymrt_tune_result res{};
@@ -42,7 +41,7 @@ int autotune(ymrt_scene *sc, const ymrt_render_settings *rs_in, int frame, hipSt
         ms = -1.0f;
         for (int k = 0; k < kTrialRepeats; ++k) {
             ymrt_frame_stats st{};
-            const int r = render_frame(sc, &rs, frame, img, nullptr, nullptr, nullptr, stream, 1, &st);
+            const int r = render_frame_with(sc, sc->render, &rs, frame, img, nullptr, nullptr, nullptr, stream, 1, &st);
             ++res.trials;
             if (r == YMRT_ERR_OUT_OF_MEMORY && cap != 0) return YMRT_OK;
             if (r) return r;
@@ -54,7 +53,7 @@ int autotune(ymrt_scene *sc, const ymrt_render_settings *rs_in, int frame, hipSt
         ymrt_render_settings warm = *rs_in;
         warm.path_capacity = 0;
         warm.persistent_waves = 0;
-        rc = render_frame(sc, &warm, frame, img, nullptr, nullptr, nullptr, stream, 0, nullptr);
+        rc = render_frame_with(sc, sc->render, &warm, frame, img, nullptr, nullptr, nullptr, stream, 0, nullptr);
         ++res.trials;
         std::vector<uint32_t> caps{0};
         for (uint64_t div : {2ull, 4ull}) {

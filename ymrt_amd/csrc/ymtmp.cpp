@@ -1223,10 +1223,9 @@ bool read_volume(Block &b)
         b.err = "Volume \"" + name + "\" requires a transform";
         return false;
     }
-    if (density != YMRT_INVALID_INDEX)
-        throw std::runtime_error("Volume \"" + name +
-                                 "\": densities from texture graphs are not supported (volumes take a voxel grid)");
     ymrt_volume_desc v{};
+    // the texture's index, or none (-1) for an image whose file failed to load
+    v.density_texture = density == YMRT_INVALID_INDEX ? -1 : (int32_t)density;
     for (int k = 0; k < 3; ++k) {
         v.sigma_t[k] = sigma[k];
         v.albedo[k] = albedo[k];

@@ -228,16 +228,26 @@ def textured_box(width=64, height=64, halton_seed=0):
     return sa
 
 
-def cornell_media(width=64, height=64, grid=True, halton_seed=0, sigma=2.0):
+def cornell_media(width=64, height=64, grid=True, halton_seed=0, sigma=2.0, density_texture=False):
     """Cornell box with a participating medium in the box [0.2,0.8] x [0.05,0.65]
     x [0.2,0.8] (a boundary instance): an 8^3 voxel density grid in [0, 1]
-    scaled by sigma when grid is True, else homogeneous sigma * 0.5."""
+    scaled by sigma when grid is True, else homogeneous sigma * 0.5.
+    density_texture: the density is instead an fBm texture graph (3 layers,
+    frequency 4 over the unit cube of the box) under its upper bound, as
+    YMRT's density_texture_idx volumes."""
     sa = cornell_box(width, height, halton_seed)
     lo, hi = np.array((0.2, 0.05, 0.2)), np.array((0.8, 0.65, 0.8))
     w2o = np.eye(4, dtype=np.float32)
     w2o[:3, :3] = np.diag(1.0 / (hi - lo))
     w2o[:3, 3] = -lo / (hi - lo)
-    if grid:
+    if density_texture:
+        black = sa.add_texture_node(abi.TEXTURE_CONSTANT_FLOAT, value=(0.0, 0.0, 0.0))
+        white = sa.add_texture_node(abi.TEXTURE_CONSTANT_FLOAT, value=(1.0, 0.0, 0.0))
+        fbm = sa.add_texture_node(abi.TEXTURE_FBM, (black, white), frequency=4.0, lacunarity=2.0, gain=0.5,
+                                  amplitude=0.6, offset=0.3, layer_count=3)
+        vol = sa.add_volume((sigma, sigma, sigma), (0.9, 0.7, 0.5), 0.2, world_to_object=w2o,
+                            density_texture=fbm)
+    elif grid:
         rng = np.random.RandomState(5)
         g = sa.add_grid(rng.rand(8, 8, 8).astype(np.float32))
         vol = sa.add_volume((sigma, sigma, sigma), (0.9, 0.7, 0.5), 0.2, grid=g, world_to_object=w2o)
