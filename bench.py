@@ -95,46 +95,61 @@ def adjoint_timing(ctx, frames=2):
             "adjoint_over_forward": round(t["adjoint"] / t["forward"], 3)}
 
 
-def batch_timing(ctx, count=16, seed=11):
-    """A batch of small scenes (the multi-scene workload at reduced size):
-    procedural interiors with 1-40 instances (6.5 k-260 k triangles,
-    log-uniform), 256^2 or 512^2, 4 or 16 spp, depth 6, fog on half of them.
-    The same frames are rendered one after another with ymrt_render_frame
-    and in one ymrt_render_batch call; wall times include every frame."""
-    import time
-
-    import numpy as np
+def batch_timing(ctx, count=64, seed=11):
+    """The multi-scene workload as configured: 64 scenes (Cornell box with a
+    displaced floor grid, triangle counts log-uniform in [1k, 500k]),
+    resolution uniform in {256^2, 512^2, 1024^2}, spp uniform in {4, 16, 64},
+    depth 6.  Plain: ymrt_render_frame one frame after another, each timed
+    with HIP events after a warm-up frame of its own scene, its path state
+    released afterwards (64 scenes' buffers would not fit at once).
+    Batched: one ymrt_render_batch call after a warm-up call.  Every batched
+    image is compared with the plain one bit for bit (equal RMSE)."""
     import torch
 
     import ymrt_amd
     from ymrt_amd import abi, scenes
-    rng = np.random.default_rng(seed)
+    work = scenes.batch_workload(count, seed)
     jobs, tris = [], 0
-    for i in range(count):
-        inst = int(round(np.exp(rng.uniform(0.0, np.log(40.0)))))
-        res = int(rng.choice([256, 512]))
-        spp = int(rng.choice([4, 16]))
-        fog = bool(rng.integers(0, 2))
-        sa = scenes.interior(res, res, tex_size=256, instances=inst, halton_seed=i, fog=fog)
-        sc = ymrt_amd.Scene(ctx, sa)
+    t0 = time.time()
+    for i, (t, res, spp, depth) in enumerate(work):
+        sc = ymrt_amd.Scene(ctx, scenes.batch_scene(t, res, res, i))
         tris += sc.info().triangle_count
-        jobs.append((sc, abi.render_settings(res, res, spp, 6, volume=int(fog)), 1))
-    ctx.render_batch(jobs)  # warm-up: path-state buffers of every scene
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
+        jobs.append((sc, abi.render_settings(res, res, spp, depth), 2))
+    gen_s = time.time() - t0
+    plain_ms, plain_rays, plain_imgs = 0.0, 0, []
     for sc, rs, frame in jobs:
-        sc.render_frame(rs, frame + 1)
+        sc.render_frame(rs, 1)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        img, st = sc.render_frame(rs, frame)
+        b.record()
+        torch.cuda.synchronize()
+        plain_ms += a.elapsed_time(b)
+        plain_rays += st.extension_rays + st.shadow_rays + st.transmittance_rays
+        plain_imgs.append(img)
+        sc.release_buffers()
+    ctx.render_batch(jobs)  # warm-up: the context's round buffers
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    out = ctx.render_batch([(sc, rs, frame + 1) for sc, rs, frame in jobs])
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    out = ctx.render_batch(jobs)
+    b.record()
     torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    rays = sum(st.camera_rays + st.extension_rays + st.shadow_rays + st.transmittance_rays for _, st in out)
+    batch_ms = a.elapsed_time(b)
+    rays = sum(st.extension_rays + st.shadow_rays + st.transmittance_rays for _, st in out)
+    same = all(torch.equal(img, ref) for (img, _), ref in zip(out, plain_imgs))
     samples = sum(rs.width * rs.height * rs.spp for _, rs, _ in jobs)
-    return {"workload": "%d interiors, 256^2/512^2, 4/16 spp, depth 6, %d triangles in all" % (count, tris),
-            "sequential_ms": round((t1 - t0) * 1e3, 2), "batch_ms": round((t2 - t1) * 1e3, 2),
-            "speedup": round((t1 - t0) / (t2 - t1), 3), "batch_mrays_per_s": round(rays / (t2 - t1) / 1e6, 2),
-            "batch_samples_per_s": round(samples / (t2 - t1), 1)}
+    for sc, _, _ in jobs:
+        sc.close()
+    return {"workload": "%d scenes, %d triangles in all (1k-500k each, log-uniform), 256^2/512^2/1024^2, 4/16/64 "
+                        "spp, depth 6, %d samples" % (count, tris, samples),
+            "plain_ms": round(plain_ms, 2), "batch_ms": round(batch_ms, 2),
+            "speedup": round(plain_ms / batch_ms, 3),
+            "plain_mrays_per_s": round(plain_rays / (plain_ms / 1e3) / 1e6, 1),
+            "batch_mrays_per_s": round(rays / (batch_ms / 1e3) / 1e6, 1),
+            "batch_samples_per_s": round(samples / (batch_ms / 1e3), 1),
+            "images_bit_identical": bool(same), "scene_gen_s": round(gen_s, 1)}
 
 
 def traversal_timing(ctx, reps=3):
@@ -210,6 +225,46 @@ def cpu_baseline(sa, rs_full, budget_s):
         "sample": "oracle, the full %dx%d frame at %d spp (%d samples, %d rays), %.1f s on %d threads"
                   % (w, h, spp, samples, rays, dt, threads),
     }
+
+
+def cornell_cpu_workload(ctx, budget_s):
+    """The CPU workload as configured: Cornell box 256x256, 16 spp, depth 8,
+    Halton (permutation seed 0), RR off.  The oracle builds its BVH on the
+    host (timed) and renders on the host's cores; when the frame would take
+    more than budget_s it renders the first rows only and the rate is taken
+    over those.  The GPU renders the same frame for comparison."""
+    import torch
+
+    import oracle
+    import ymrt_amd
+    from ymrt_amd import abi, scenes
+    threads = max(1, min(16, os.cpu_count() or 1))
+    sa = scenes.cornell_box(256, 256, halton_seed=0)
+    rs = abi.render_settings(256, 256, 16, 8, sampler=abi.SAMPLER_HALTON)
+    t0 = time.time()
+    osc = oracle.Scene(sa)
+    build_ms = (time.time() - t0) * 1e3
+    t0 = time.time()
+    osc.render(rs, 1, rows=(0, 16), threads=threads)
+    probe = time.time() - t0
+    rows = 256 if probe * 16 <= budget_s else max(16, int(16 * budget_s / max(probe, 1e-3)) // 16 * 16)
+    t0 = time.time()
+    osc.render(rs, 1, rows=(0, rows), threads=threads)
+    dt = time.time() - t0
+    c = osc.last_counts
+    cpu_rays = int(c[1] + c[2] + c[3])
+    osc.close()
+    sc = ymrt_amd.Scene(ctx, sa)
+    sc.render_frame(rs, 1)
+    torch.cuda.synchronize()
+    _, st = sc.render_frame(rs, 2, timing=True)
+    gpu_rays = st.extension_rays + st.shadow_rays + st.transmittance_rays
+    return {"workload": "Cornell box 256x256 16spp depth 8, Halton seed 0, %d triangles" % sc.info().triangle_count,
+            "cpu_kind": "port (oracle)", "cpu_threads": threads, "cpu_bvh_build_ms": round(build_ms, 1),
+            "cpu_rows": rows, "cpu_render_s": round(dt, 3), "cpu_render_s_full_frame": round(dt * 256 / rows, 3),
+            "cpu_mrays_per_s": round(cpu_rays / dt / 1e6, 3),
+            "gpu_bvh_build_ms": round(sc.info().build_ms, 1), "gpu_frame_ms": round(st.ms_total, 3),
+            "gpu_mrays_per_s": round(gpu_rays / (st.ms_total / 1e3) / 1e6, 1)}
 
 
 TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
@@ -314,6 +369,8 @@ def main():
     else:
         dom, dom_bytes, dom_ms, dom_launch = "k_extend", ext_bytes, stage["extend"], launches["extend"]
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9
+    traffic = measured_traffic(dom, ext_rays, dom_launch)
+    physical = traffic / (dom_ms / max(dom_launch, 1) / 1e3) / 1e9 if traffic else None
     per_step = {k: v / args.steps for k, v in stage.items()}
     out = {
         "metric": METRIC,
@@ -337,10 +394,20 @@ def main():
         "stage_ms_per_step": {k: round(v, 2) for k, v in per_step.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(dom, ext_rays, dom_launch),
+                     "achieved_kind": "algorithmic bytes: path state + 128 B per visited wide node + 48 B per "
+                                      "tested triangle, visits counted on the device in this run",
+                     "traffic": traffic,
+                     "traffic_source": "profiles/k_extend_interior_1080p_pmc_traffic.md: calibrated rocprofv3 "
+                                       "FETCH_SIZE + WRITE_SIZE per extension ray (8 spp run of this scene), "
+                                       "times this run's extension rays per launch" if traffic else None,
+                     "physical_frac": round(physical / HBM_PEAK_GBS, 4) if physical else None,
+                     "limiter": "not HBM: nodes and triangles come from L2/MALL (physical bytes are a sixth of "
+                                "the algorithmic ones); the kernel is bound by VALU issue under divergence and "
+                                "dependent-load latency (SQ counters in profiles/traversal_shading_sq_counters.md)",
                      "bytes_per_launch": int(dom_bytes / max(dom_launch, 1)),
                      "avg_launch_ms": round(dom_ms / max(dom_launch, 1), 4),
                      "launches": dom_launch},
+        "mrays_per_s_without_transmittance": round((rays - tr_rays) / (ms / 1e3) / 1e6, 2),
         "traversal": {"extend_rays": ext_rays // args.steps,
                       "nodes_per_extend_ray": round(visits["extend_nodes"] / max(ext_rays, 1), 2),
                       "tris_per_extend_ray": round(visits["extend_tris"] / max(ext_rays, 1), 2),
@@ -358,6 +425,7 @@ def main():
         out["traversal_c3"] = traversal_timing(ctx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sa, rs, args.cpu_seconds)
+        out["cornell_cpu_workload"] = cornell_cpu_workload(ctx, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 

@@ -152,6 +152,11 @@ typedef struct ymrt_texture_desc {
     int32_t turbulence_axes;         /* MARBLE: bit 0 x, bit 1 y, bit 2 z     */
     float frequency1, lacunarity1, gain1;   /* WAVE: second fBm (the first uses */
     int32_t layer_count1;                    /* frequency, lacunarity, gain)     */
+    /* IMAGE with float texels: width * height * channels linear values (the
+     * FP32 image tiles the reference decodes its files into,
+     * src/TextureManager.h:19-50); when set, texels and color_space are
+     * unused.  These are the texels ymrt_render_adjoint differentiates. */
+    const float *texels_f32;
 } ymrt_texture_desc;
 
 /* Density grid for heterogeneous media (no counterpart in YMRT, whose
@@ -314,6 +319,20 @@ int ymrt_scene_get_info(const ymrt_scene *scene, ymrt_scene_info *info);
 int ymrt_scene_update_materials(ymrt_scene *scene, uint32_t first, uint32_t count,
                                 const ymrt_material_desc *materials, void *stream);
 
+/* Releases the path-state buffers the scene's frames allocated (the next
+ * frame allocates them again); the scene itself stays.  For callers that
+ * render many scenes in turn and keep them resident: RenderModule frees its
+ * per-task device memory the same way (src/core/RenderModule.cpp:535-539,
+ * 640-653).  Waits for the device. */
+int ymrt_scene_release_buffers(ymrt_scene *scene);
+
+/* Replaces the texels of a float image texture (texels_f32 at scene
+ * creation) with `texels` (host, width * height * channels linear values),
+ * as SceneModule::UpdateTexture does for the editor; synchronous on stream.
+ * An optimisation loop over ymrt_render_adjoint's texture gradient writes
+ * its updated texels back this way. */
+int ymrt_scene_update_texture(ymrt_scene *scene, uint32_t texture, const float *texels, void *stream);
+
 /* Closest hit (TraceRay) for n device-resident rays. */
 int ymrt_trace_closest(ymrt_scene *scene, const ymrt_ray *rays, uint32_t n,
                        ymrt_hit *hits, void *stream);
@@ -388,21 +407,25 @@ typedef struct ymrt_batch_item {
     ymrt_scene *scene;
     const ymrt_render_settings *settings;
     int frame_index;
-    int timing;                      /* as ymrt_render_frame                  */
+    int timing;                      /* unused: frames share their launches   */
     float *noise_rgba;               /* device, width*height float4           */
     ymrt_frame_stats *stats;         /* may be NULL                           */
 } ymrt_batch_item;
 
-/* Renders every item as ymrt_render_frame would, bit for bit, with up to
- * max_streams host threads (0: 4), each driving a HIP stream of its own, so
- * the frames of small scenes overlap on the device.  The frames of one scene
- * run in item order on one stream; scenes go out longest first (pixels x
- * spp x depth x log2 triangles).  YMRT drives its editor and path-tracing
- * views the same way, one render thread and stream each
- * (src/core/RenderModule.cpp:55-66, 95-143).  Waits for `stream` first and
- * returns when every frame is done; on failure the message names the item. */
+/* Renders every item as ymrt_render_frame would, bit for bit, in shared
+ * launches: the frames are cut into chunks of whole pixels and packed into
+ * rounds of up to max_slots chunks (0 or > 64: 64) and up to 64 M paths (within
+ * a quarter of free device memory); one launch of each stage kernel serves
+ * every chunk of a round, so small frames fill the GPU together and share one
+ * tail of long paths.  Replaces YMRT's render threads, one stream and frame
+ * loop per view (src/core/RenderModule.cpp:55-66, 95-143, host loop 580-631).
+ * Runs on `stream` and returns when every frame is done; stats get the ray
+ * and traversal counts (no per-stage timing: stages are shared).  The
+ * round's buffers belong to the context (kept between calls, released by
+ * ymrt_context_destroy); calls on one context run one at a time.  On failure
+ * the message names the item. */
 int ymrt_render_batch(ymrt_context *ctx, const ymrt_batch_item *items, uint32_t n,
-                      uint32_t max_streams, void *stream);
+                      uint32_t max_slots, void *stream);
 
 /* 8-bit RGBA PNG of a host RGBA32F image as SavePngImage writes it
  * (src/FileIO.cpp:167-194): the extension becomes ".png", parent folders

@@ -82,7 +82,11 @@ def lib():
             L.or_first_hits.argtypes = [vp, C.POINTER(abi.RenderSettings), C.c_int, vp]
             L.or_primary_rays.argtypes = [vp, C.POINTER(abi.RenderSettings), C.c_int, C.c_uint32, vp]
             L.or_render_adjoint.argtypes = [vp, C.POINTER(abi.RenderSettings), C.c_int, C.c_uint32, C.c_uint32,
-                                            vp, vp, vp]
+                                            vp, C.POINTER(AdjTargets), vp, vp]
+            L.or_adjoint_prepare.restype = C.c_int
+            L.or_adjoint_prepare.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32, C.c_uint32, C.c_uint32,
+                                             C.POINTER(AdjTargets)]
+            L.or_adjoint_free.argtypes = [C.POINTER(AdjTargets)]
             L.or_lambert_only.restype = C.c_int
             L.or_lambert_only.argtypes = [C.POINTER(abi.MaterialDesc)]
             L.or_perlin.restype = C.c_float
@@ -100,6 +104,13 @@ def lib():
             L.or_probe_free_flight.argtypes = [vp, vp, C.c_float, C.c_float, vp, vp]
             _lib = L
     return _lib
+
+
+class AdjTargets(C.Structure):
+    """or_adj_targets (oracle.h)."""
+    _fields_ = [("alb_slot", C.c_void_p), ("rough_slot", C.c_void_p), ("n_alb", C.c_uint32),
+                ("n_rough", C.c_uint32), ("tex", C.c_uint32), ("vol", C.c_uint32), ("off_rough", C.c_size_t),
+                ("off_tex", C.c_size_t), ("off_vox", C.c_size_t), ("n_params", C.c_size_t)]
 
 
 class V3(C.Structure):
@@ -183,27 +194,59 @@ class Scene:
         self.last_counts = counts.sum(axis=0)
         return out
 
-    def render_adjoint(self, settings, adj, frame=1, threads=1):
-        """(noise image, gradient) of the path-replay adjoint for per-material
-        Lambert albedos: adj is dLoss/dImage (H, W, 3); the gradient is
-        (material_count, 3) float64, zero for materials that are not
-        Lambert-only."""
+    def render_adjoint(self, settings, adj, frame=1, threads=1, albedo=None, roughness=(), texture=None,
+                       volume=None):
+        """(noise image, gradient) of the path-replay adjoint: adj is
+        dLoss/dImage (H, W, 3).  Targets: albedo, a list of Lambert-only
+        materials (default: every Lambert-only material); roughness, a list of
+        metal-only materials; texture, an image texture used as the diffuse
+        albedo of Lambert materials; volume, a volume with a gray grid
+        density.  The gradient is a dict of float64 arrays: "albedo"
+        (len(albedo), 3), "roughness" (len(roughness),), "texture" (H, W, C),
+        "density" (nz, ny, nx).  Raises ValueError naming a target the adjoint
+        cannot differentiate."""
         h, w = settings.height, settings.width
         adj = np.ascontiguousarray(adj, np.float32).reshape(h, w, 3)
         out = np.zeros((h, w, 4), np.float32)
-        nm = len(self.arrays.materials)
-        grads = np.zeros((max(1, threads), max(nm, 1), 3), np.float64)
+        if albedo is None:
+            albedo = [i for i, m in enumerate(self.arrays.materials) if lambert_only(m)]
+        alb = np.ascontiguousarray(albedo, np.uint32)
+        rough = np.ascontiguousarray(roughness, np.uint32)
+        INV = 0xFFFFFFFF
+        tg = AdjTargets()
+        bad = lib().or_adjoint_prepare(self.h, _p(alb) if len(alb) else None, len(alb),
+                                       _p(rough) if len(rough) else None, len(rough),
+                                       INV if texture is None else texture, INV if volume is None else volume,
+                                       C.byref(tg))
+        try:
+            if bad:
+                names = (["albedo material %d" % m for m in alb] + ["roughness material %d" % m for m in rough]
+                         + ["texture %s" % texture, "volume %s" % volume])
+                raise ValueError("the adjoint cannot differentiate " + names[bad - 1])
+            grads = np.zeros((max(1, threads), max(tg.n_params, 1)), np.float64)
 
-        def work(k, a, b):
-            lib().or_render_adjoint(self.h, C.byref(settings), frame, a, b, _p(adj), _p(grads[k]), _p(out))
+            def work(k, a, b):
+                lib().or_render_adjoint(self.h, C.byref(settings), frame, a, b, _p(adj), C.byref(tg),
+                                        _p(grads[k]), _p(out))
 
-        bounds = np.linspace(0, h, max(1, threads) + 1).astype(int)
-        ts = [threading.Thread(target=work, args=(k, bounds[k], bounds[k + 1])) for k in range(max(1, threads))]
-        for t in ts:
-            t.start()
-        for t in ts:
-            t.join()
-        return out, grads.sum(axis=0)[:nm]
+            bounds = np.linspace(0, h, max(1, threads) + 1).astype(int)
+            ts = [threading.Thread(target=work, args=(k, bounds[k], bounds[k + 1]))
+                  for k in range(max(1, threads))]
+            for t in ts:
+                t.start()
+            for t in ts:
+                t.join()
+            g = grads.sum(axis=0)
+            res = {"albedo": g[:tg.off_rough].reshape(-1, 3), "roughness": g[tg.off_rough:tg.off_tex]}
+            if texture is not None:
+                td = self.arrays.textures[texture]["texels"]
+                res["texture"] = g[tg.off_tex:tg.off_vox].reshape(td.shape[0], td.shape[1], -1)
+            if volume is not None:
+                gd = self.arrays.grids[self.arrays.volumes[volume]["grid"]]
+                res["density"] = g[tg.off_vox:tg.n_params].reshape(gd.shape)
+            return out, res
+        finally:
+            lib().or_adjoint_free(C.byref(tg))
 
     def primary_rays(self, settings, frame=1, sample=0):
         """Camera rays (H*W records of abi.RAY_DTYPE) of one sample per pixel."""

@@ -245,8 +245,14 @@ void or_shading_init(or_scene *sc, const ymrt_scene_desc *d)
         t->su = td->scale_u; t->sv = td->scale_v; t->ou = td->offset_u; t->ov = td->offset_v;
         size_t n = (size_t)td->width * td->height * td->channels;
         t->texels = (float *)malloc(sizeof(float) * (n ? n : 1));
-        const float *l = lut[td->color_space == YMRT_COLOR_SRGB ? 0 : 1];
-        for (size_t i = 0; i < n; ++i) t->texels[i] = l[td->texels[i]];
+        if (td->texels_f32) {
+            /* linear float texels are taken as they are (no colour-space
+             * conversion) */
+            for (size_t i = 0; i < n; ++i) t->texels[i] = td->texels_f32[i];
+        } else {
+            const float *l = lut[td->color_space == YMRT_COLOR_SRGB ? 0 : 1];
+            for (size_t i = 0; i < n; ++i) t->texels[i] = l[td->texels[i]];
+        }
     }
     sc->grid_count = d->grid_count;
     sc->grids = (or_grid *)calloc(d->grid_count ? d->grid_count : 1, sizeof(or_grid));
@@ -1094,6 +1100,57 @@ typedef struct {
     uint32_t hash_ctr;
 } rng_ctx;
 
+/* Path-replay adjoint state of one path (or_render_adjoint).  Gradient
+ * terms come in two kinds: a "downstream" event (param, w, mark) adds
+ * adj . (L_path - mark) * w to the parameter once the path's radiance is
+ * known -- the factor it stands for multiplies every deposit after the mark
+ * -- and a direct term adds adj . deposit * w for one shadow ray's deposit
+ * (the BSDF and the transmittance along that ray).  Sampling pdfs and MIS
+ * weights are held fixed (detached): the estimator differentiates the
+ * integrand only, which is unbiased for the derivative of the expected
+ * image. */
+#define OR_ADJ_RAY_EVENTS 1024
+typedef struct { uint32_t param; v3 w; v3 mark; } adj_down;
+typedef struct { uint32_t param; float w; } adj_ev;
+typedef struct {
+    const or_adj_targets *tg;
+    adj_down *down;
+    uint32_t nd, cap;
+    adj_ev ray_ev[OR_ADJ_RAY_EVENTS];
+    uint32_t nray;
+    int recording;      /* shadow-ray transmittance events are wanted */
+    double *grad;
+    v3 adj;             /* the pixel's dLoss/dImage / spp */
+    v3 beta;            /* throughput at the current vertex */
+    v3 L;               /* radiance so far (mark of free-flight events) */
+    int rough_param;    /* the surface vertex's roughness parameter, or -1 */
+    float rough_r;      /* and its roughness */
+} or_adj_path;
+
+static void adj_add_down(or_adj_path *a, uint32_t param, v3 w, v3 mark)
+{
+    if (a->nd == a->cap) {
+        a->cap = a->cap ? 2 * a->cap : 256;
+        a->down = (adj_down *)realloc(a->down, sizeof(adj_down) * a->cap);
+    }
+    adj_down e = {param, w, mark};
+    a->down[a->nd++] = e;
+}
+
+/* the voxel of a grid holding object point p, or -1 outside the unit cube
+ * (grid_density's cell) */
+static int grid_voxel(const or_scene *sc, int gi, v3 p)
+{
+    if (gi < 0 || (uint32_t)gi >= sc->grid_count) return -1;
+    const or_grid *g = &sc->grids[gi];
+    if (!(p.x >= 0.0f && p.x < 1.0f && p.y >= 0.0f && p.y < 1.0f && p.z >= 0.0f && p.z < 1.0f)) return -1;
+    uint32_t ix = (uint32_t)(p.x * (float)g->nx), iy = (uint32_t)(p.y * (float)g->ny), iz = (uint32_t)(p.z * (float)g->nz);
+    if (ix >= g->nx) ix = g->nx - 1;
+    if (iy >= g->ny) iy = g->ny - 1;
+    if (iz >= g->nz) iz = g->nz - 1;
+    return (int)(((size_t)iz * g->ny + iy) * g->nx + ix);
+}
+
 static float shadow_rand(rng_ctx *r)
 {
     uint32_t v[4] = {r->hash_key[0], r->hash_key[1] ^ (r->hash_ctr << 8), r->hash_key[2], r->hash_key[3]};
@@ -1204,7 +1261,7 @@ static int sample_homogeneous(const or_scene *sc, int n, const int *vi, float tm
 
 /* src/VolumeEval.cuh:80-214, densities from the voxel grid or a texture graph */
 static int sample_heterogeneous(const or_scene *sc, int n, const int *vi, const or_ray *ray, float tmax,
-                                v3 *trw, float *dist, float *vw, or_sampler *s)
+                                v3 *trw, float *dist, float *vw, or_sampler *s, or_adj_path *adj)
 {
     *trw = s3(1.0f);
     *dist = OR_FLOAT_MAX;
@@ -1243,12 +1300,19 @@ static int sample_heterogeneous(const or_scene *sc, int n, const int *vi, const 
         }
         acc += fpl;
         v3 st = s3(0.0f), ss = s3(0.0f);
+        int tvox = -1;
+        float tsig = 0.0f;
         for (int i = 0; i < n; ++i) {
             const ymrt_volume_desc *v = &sc->volumes[vi[i]];
             float dens = 1.0f;
-            if (vol_is_het(sc, vi[i])) dens = vol_density(sc, v, add3(ray->o, scl3(ray->d, acc)), add3(ro[i], scl3(rd[i], acc)));
+            v3 po = add3(ro[i], scl3(rd[i], acc));
+            if (vol_is_het(sc, vi[i])) dens = vol_density(sc, v, add3(ray->o, scl3(ray->d, acc)), po);
             st = add3(st, scl3(vsig(v), dens));
             ss = add3(ss, scl3(mul3(vsig(v), valb(v)), dens));
+            if (adj && (uint32_t)vi[i] == adj->tg->vol) {
+                tvox = grid_voxel(sc, v->density_grid, po);
+                tsig = v->sigma_t[0];
+            }
         }
         v3 rp;
         rp.x = fminr(sm.x > 0.0f ? st.x * smr.x : 0.0f, 1.0f);
@@ -1259,8 +1323,14 @@ static int sample_heterogeneous(const or_scene *sc, int n, const int *vi, const 
         if (or_sampler_next(s) < comp3(rp, ch)) {
             tr = mul3(tr, mul3(mul3(ptr, ss), nf));
             pdf = mul3(pdf, mul3(mul3(mul3(ptr, sm), rp), nf));
+            /* a real collision: probability sigma(x) / sigma_m (gray media) */
+            if (tvox >= 0 && st.x > 0.0f)
+                adj_add_down(adj, (uint32_t)(adj->tg->off_vox + (size_t)tvox), s3(tsig / st.x), adj->L);
             break;
         }
+        /* a null collision: probability 1 - sigma(x) / sigma_m */
+        if (tvox >= 0 && sm.x - st.x > 0.0f)
+            adj_add_down(adj, (uint32_t)(adj->tg->off_vox + (size_t)tvox), s3(-tsig / (sm.x - st.x)), adj->L);
         tr = mul3(tr, mul3(mul3(mul3(ptr, sm), fp), nf));
         pdf = mul3(pdf, mul3(mul3(mul3(ptr, sm), fp), nf));
         ++it;
@@ -1273,17 +1343,18 @@ static int sample_heterogeneous(const or_scene *sc, int n, const int *vi, const 
 }
 
 static int sample_volume(const or_scene *sc, const or_ray *ray, int n, const int *vi, float tmax,
-                         v3 *trw, float *dist, float *vw, or_sampler *s)
+                         v3 *trw, float *dist, float *vw, or_sampler *s, or_adj_path *adj)
 {
     if (sc->volume_count == 0 || n == 0) return 0;
     int het = 0;
     for (int i = 0; i < n; ++i) if (vol_is_het(sc, vi[i])) { het = 1; break; }
-    if (het) return sample_heterogeneous(sc, n, vi, ray, tmax, trw, dist, vw, s);
+    if (het) return sample_heterogeneous(sc, n, vi, ray, tmax, trw, dist, vw, s, adj);
     return sample_homogeneous(sc, n, vi, tmax, trw, dist, vw, s);
 }
 
 /* src/VolumeEval.cuh:253-385 */
-static v3 eval_tr(const or_scene *sc, const or_ray *ray, const or_transfer *t, float tmax, rng_ctx *rng)
+static v3 eval_tr(const or_scene *sc, const or_ray *ray, const or_transfer *t, float tmax, rng_ctx *rng,
+                  or_adj_path *adj)
 {
     if (sc->volume_count == 0) return s3(1.0f);
     int vi[OR_MAX_BOUNDARY_RECORD];
@@ -1330,11 +1401,23 @@ static v3 eval_tr(const or_scene *sc, const or_ray *ray, const or_transfer *t, f
         }
         acc += fpl;
         v3 st = s3(0.0f);
+        int tvox = -1;
+        float tsig = 0.0f;
         for (int i = 0; i < n; ++i) {
             const ymrt_volume_desc *v = &sc->volumes[vi[i]];
             float dens = 1.0f;
-            if (vol_is_het(sc, vi[i])) dens = vol_density(sc, v, add3(ray->o, scl3(ray->d, acc)), add3(ro[i], scl3(rd[i], acc)));
+            v3 po = add3(ro[i], scl3(rd[i], acc));
+            if (vol_is_het(sc, vi[i])) dens = vol_density(sc, v, add3(ray->o, scl3(ray->d, acc)), po);
             st = add3(st, scl3(vsig(v), dens));
+            if (adj && adj->recording && (uint32_t)vi[i] == adj->tg->vol) {
+                tvox = grid_voxel(sc, v->density_grid, po);
+                tsig = v->sigma_t[0];
+            }
+        }
+        /* ratio tracking's factor 1 - sigma(x) / sigma_m (gray media) */
+        if (tvox >= 0 && sm.x - st.x > 0.0f && adj->nray < OR_ADJ_RAY_EVENTS) {
+            adj_ev e = {(uint32_t)(adj->tg->off_vox + (size_t)tvox), -tsig / (sm.x - st.x)};
+            adj->ray_ev[adj->nray++] = e;
         }
         v3 rp;
         rp.x = fminr(sm.x > 0.0f ? st.x * smr.x : 0.0f, 1.0f);
@@ -1356,7 +1439,8 @@ static v3 eval_tr(const or_scene *sc, const or_ray *ray, const or_transfer *t, f
 }
 
 /* src/VolumeEval.cuh:420-468 */
-static v3 trace_tr(const or_scene *sc, const or_ray *ray, or_transfer *t, rng_ctx *rng, uint64_t *counts)
+static v3 trace_tr(const or_scene *sc, const or_ray *ray, or_transfer *t, rng_ctx *rng, uint64_t *counts,
+                   or_adj_path *adj)
 {
     if (sc->volume_count == 0) return s3(1.0f);
     v3 tr = s3(1.0f);
@@ -1372,10 +1456,10 @@ static v3 trace_tr(const or_scene *sc, const or_ray *ray, or_transfer *t, rng_ct
         if (counts) counts[3]++;
         int hb = or_trace_volume(sc, &r, &h, &nc, nb);
         if (!hb) {
-            tr = mul3(tr, eval_tr(sc, &r, t, fmaxr(maxd - traced, 0.0f), rng));
+            tr = mul3(tr, eval_tr(sc, &r, t, fmaxr(maxd - traced, 0.0f), rng, adj));
             break;
         }
-        tr = mul3(tr, eval_tr(sc, &r, t, h.t, rng));
+        tr = mul3(tr, eval_tr(sc, &r, t, h.t, rng, adj));
         traced += h.t;
         v3 p, perr, ng;
         or_fetch_geometry_normal(sc, &h, &p, &perr, &ng);
@@ -1624,7 +1708,22 @@ typedef struct {
     or_sampler *s;
     rng_ctx *rng;
     uint64_t *counts;
+    or_adj_path *adj;
 } path_ctx;
+
+/* one shadow ray's deposit, beta * term * scale: its roughness term (g, at a
+ * surface vertex on a target material) and the transmittance events
+ * recorded along it */
+static void adj_nee(path_ctx *c, v3 term, float scale, int rough_param, float g)
+{
+    or_adj_path *a = c->adj;
+    if (!a) return;
+    v3 dep = sclf3(scale, mul3(a->beta, term));
+    double d = (double)a->adj.x * dep.x + (double)a->adj.y * dep.y + (double)a->adj.z * dep.z;
+    if (rough_param >= 0) a->grad[rough_param] += d * (double)g;
+    for (uint32_t i = 0; i < a->nray; ++i) a->grad[a->ray_ev[i].param] += d * (double)a->ray_ev[i].w;
+    a->nray = 0;
+}
 
 /* DistantLight::SampleLi (src/Light.h:361-385) */
 static int distant_sample_li(const or_scene *sc, uint32_t i, float u0, float u1, v3 *wi, v3 *color, float *pdf)
@@ -1666,6 +1765,41 @@ static uint32_t pick_distant(const or_scene *sc, float u)
     return (uint32_t)k;
 }
 
+/* d ln f / d r of a metal-only material's GGX lobe at (wo, wi) (local),
+ * f = albedo F D G2 / (4 wo.z) with alpha = clamp(r, 0.001, 0.999)^2:
+ * d ln D / d alpha + d ln G2 / d alpha, times 2 r (0 where the clamp holds).
+ * Isotropic GGX: D = 1 / (pi a^2 (s / a^2 + hz^2)^2) with s = hx^2 + hy^2,
+ * Lambda(w) = (sqrt(1 + a^2 t^2) - 1) / 2 with t^2 = (wx^2 + wy^2) / wz^2. */
+static float rough_dlog(float r, v3 wo, v3 wi)
+{
+    if (!(r > 0.001f && r < 0.999f) || !(wo.z > 0.0f && wi.z > 0.0f)) return 0.0f;
+    double a = (double)r * r;
+    double hx = (double)wo.x + wi.x, hy = (double)wo.y + wi.y, hz = (double)wo.z + wi.z;
+    double hl = sqrt(hx * hx + hy * hy + hz * hz);
+    if (hl < 1e-12) return 0.0f;
+    hx /= hl; hy /= hl; hz /= hl;
+    double sh = hx * hx + hy * hy, q = sh / (a * a) + hz * hz;
+    double dlnd = -2.0 / a + 4.0 * sh / (a * a * a * q);
+    double lam[2] = {0.0, 0.0}, dlam[2] = {0.0, 0.0};
+    v3 ws[2] = {wo, wi};
+    for (int k = 0; k < 2; ++k) {
+        double wz = ws[k].z;
+        if (fabs(wz) < 1e-8) continue;
+        double t2 = ((double)ws[k].x * ws[k].x + (double)ws[k].y * ws[k].y) / (wz * wz);
+        double root = sqrt(1.0 + a * a * t2);
+        lam[k] = 0.5 * (root - 1.0);
+        dlam[k] = 0.5 * a * t2 / root;
+    }
+    double dlng = -(dlam[0] + dlam[1]) / (1.0 + lam[0] + lam[1]);
+    return (float)((dlnd + dlng) * 2.0 * r);
+}
+
+static int adj_rough_param(const path_ctx *c) { return c->adj ? c->adj->rough_param : -1; }
+static float adj_rough_g(const path_ctx *c, v3 wo, v3 wi)
+{
+    return c->adj && c->adj->rough_param >= 0 ? rough_dlog(c->adj->rough_r, wo, wi) : 0.0f;
+}
+
 /* EvalDistantLight, src/Shading.cu:83-198 */
 static v3 eval_distant_light(path_ctx *c, mbsdf *b, v3 ray_d, const or_hit *h, const or_transfer *t,
                              int nc, const or_nearby *nb, v3 pos, v3 perr, v3 ng)
@@ -1698,10 +1832,12 @@ static v3 eval_distant_light(path_ctx *c, mbsdf *b, v3 ray_d, const or_hit *h, c
                         or_transfer st = *t;
                         if (transmit) boundary_bunch(sc, &st, nb, nc);
                         sr = make_ray(so, ldir, h->inst, h->tri);
-                        tr = trace_tr(sc, &sr, &st, c->rng, c->counts);
+                        tr = trace_tr(sc, &sr, &st, c->rng, c->counts, c->adj);
                     }
                     float mis = power_heuristic(lpdf, wpdf);
-                    direct = add3(direct, mul3(mul3(sclf3(mis, tr), clampv(f)), clampv(Li)));
+                    v3 term = mul3(mul3(sclf3(mis, tr), clampv(f)), clampv(Li));
+                    direct = add3(direct, term);
+                    adj_nee(c, term, 1.0f / sel_pdf, adj_rough_param(c), adj_rough_g(c, wo, lwi));
                 }
             }
         }
@@ -1729,10 +1865,12 @@ static v3 eval_distant_light(path_ctx *c, mbsdf *b, v3 ray_d, const or_hit *h, c
                             or_transfer st = *t;
                             if (transmit) boundary_bunch(sc, &st, nb, nc);
                             sr = make_ray(so, ldir, h->inst, h->tri);
-                            tr = trace_tr(sc, &sr, &st, c->rng, c->counts);
+                            tr = trace_tr(sc, &sr, &st, c->rng, c->counts, c->adj);
                         }
                         float mis = power_heuristic(bpdf, lpdf);
-                        direct = add3(direct, mul3(mul3(sclf3(mis, tr), clampv(bw)), clampv(Li)));
+                        v3 term = mul3(mul3(sclf3(mis, tr), clampv(bw)), clampv(Li));
+                        direct = add3(direct, term);
+                        adj_nee(c, term, 1.0f / sel_pdf, adj_rough_param(c), adj_rough_g(c, wo, bwi));
                     }
                 }
             }
@@ -1777,10 +1915,12 @@ static v3 eval_shape_light(path_ctx *c, mbsdf *b, v3 ray_d, const or_hit *h, con
                         if (transmit) boundary_bunch(sc, &st, nb, nc);
                         sr = make_ray(so, ldir, h->inst, h->tri);
                         sr.t = maxd;
-                        tr = trace_tr(sc, &sr, &st, c->rng, c->counts);
+                        tr = trace_tr(sc, &sr, &st, c->rng, c->counts, c->adj);
                     }
                     float mis = power_heuristic(lpdf, wpdf);
-                    direct = add3(direct, mul3(mul3(sclf3(mis, tr), clampv(f)), clampv(Li)));
+                    v3 term = mul3(mul3(sclf3(mis, tr), clampv(f)), clampv(Li));
+                    direct = add3(direct, term);
+                    adj_nee(c, term, safe_rcp(sel_pdf), adj_rough_param(c), adj_rough_g(c, wo, lwi));
                 }
             }
         }
@@ -1812,10 +1952,12 @@ static v3 eval_shape_light(path_ctx *c, mbsdf *b, v3 ray_d, const or_hit *h, con
                             if (transmit) boundary_bunch(sc, &st, nb, nc);
                             sr = make_ray(so, ldir, h->inst, h->tri);
                             sr.t = maxd;
-                            tr = trace_tr(sc, &sr, &st, c->rng, c->counts);
+                            tr = trace_tr(sc, &sr, &st, c->rng, c->counts, c->adj);
                         }
                         float mis = power_heuristic(bpdf, lpdf);
-                        direct = add3(direct, mul3(mul3(sclf3(mis, tr), clampv(bw)), clampv(Li)));
+                        v3 term = mul3(mul3(sclf3(mis, tr), clampv(bw)), clampv(Li));
+                        direct = add3(direct, term);
+                        adj_nee(c, term, safe_rcp(sel_pdf), adj_rough_param(c), adj_rough_g(c, wo, bwi));
                     }
                 }
             }
@@ -1847,9 +1989,11 @@ static v3 eval_volume_distant(path_ctx *c, int n, const float *gs, const float *
             if (!or_trace_shadow(sc, &sr)) {
                 or_transfer st = *t;
                 sr = make_ray(vpos, ldir, OR_EMPTY, OR_EMPTY);
-                v3 tr = trace_tr(sc, &sr, &st, c->rng, c->counts);
+                v3 tr = trace_tr(sc, &sr, &st, c->rng, c->counts, c->adj);
                 float mis = power_heuristic(lpdf, wpdf);
-                direct = add3(direct, mul3(scl3(sclf3(mis, tr), fminr(ph, OR_MAX_COLOR_CLAMP)), clampv(Li)));
+                v3 term = mul3(scl3(sclf3(mis, tr), fminr(ph, OR_MAX_COLOR_CLAMP)), clampv(Li));
+                direct = add3(direct, term);
+                adj_nee(c, term, sel_pdf > 0.0f ? 1.0f / sel_pdf : 0.0f, -1, 0.0f);
             }
         }
     }
@@ -1867,9 +2011,11 @@ static v3 eval_volume_distant(path_ctx *c, int n, const float *gs, const float *
                 if (!or_trace_shadow(sc, &sr)) {
                     or_transfer st = *t;
                     sr = make_ray(vpos, pwi, OR_EMPTY, OR_EMPTY);
-                    v3 tr = trace_tr(sc, &sr, &st, c->rng, c->counts);
+                    v3 tr = trace_tr(sc, &sr, &st, c->rng, c->counts, c->adj);
                     float mis = power_heuristic(ppdf, lpdf);
-                    direct = add3(direct, mul3(scl3(sclf3(mis, tr), fminr(pw, OR_MAX_COLOR_CLAMP)), clampv(Li)));
+                    v3 term = mul3(scl3(sclf3(mis, tr), fminr(pw, OR_MAX_COLOR_CLAMP)), clampv(Li));
+                    direct = add3(direct, term);
+                    adj_nee(c, term, sel_pdf > 0.0f ? 1.0f / sel_pdf : 0.0f, -1, 0.0f);
                 }
             }
         }
@@ -1904,9 +2050,11 @@ static v3 eval_volume_light(path_ctx *c, int n, const float *gs, const float *vw
                 or_transfer st = *t;
                 sr = make_ray(vpos, ldir, OR_EMPTY, OR_EMPTY);
                 sr.t = maxd;
-                v3 tr = trace_tr(sc, &sr, &st, c->rng, c->counts);
+                v3 tr = trace_tr(sc, &sr, &st, c->rng, c->counts, c->adj);
                 float mis = power_heuristic(lpdf, wpdf);
-                direct = add3(direct, mul3(scl3(sclf3(mis, tr), fminr(ph, OR_MAX_COLOR_CLAMP)), clampv(Li)));
+                v3 term = mul3(scl3(sclf3(mis, tr), fminr(ph, OR_MAX_COLOR_CLAMP)), clampv(Li));
+                direct = add3(direct, term);
+                adj_nee(c, term, sel_pdf > 0.0f ? 1.0f / sel_pdf : 0.0f, -1, 0.0f);
             }
         }
     }
@@ -1929,9 +2077,11 @@ static v3 eval_volume_light(path_ctx *c, int n, const float *gs, const float *vw
                     or_transfer st = *t;
                     sr = make_ray(vpos, ldir, OR_EMPTY, OR_EMPTY);
                     sr.t = maxd;
-                    v3 tr = trace_tr(sc, &sr, &st, c->rng, c->counts);
+                    v3 tr = trace_tr(sc, &sr, &st, c->rng, c->counts, c->adj);
                     float mis = power_heuristic(ppdf, lpdf);
-                    direct = add3(direct, mul3(scl3(sclf3(mis, tr), fminr(pw, OR_MAX_COLOR_CLAMP)), clampv(Li)));
+                    v3 term = mul3(scl3(sclf3(mis, tr), fminr(pw, OR_MAX_COLOR_CLAMP)), clampv(Li));
+                    direct = add3(direct, term);
+                    adj_nee(c, term, sel_pdf > 0.0f ? 1.0f / sel_pdf : 0.0f, -1, 0.0f);
                 }
             }
         }
@@ -2095,14 +2245,110 @@ int or_lambert_only(const ymrt_material_desc *m)
            m->transmission_weight_tex == OR_EMPTY && m->coat_weight_tex == OR_EMPTY;
 }
 
-typedef struct {
-    uint32_t n;
-    uint32_t m[OR_MAX_RAY_DEPTH];
-    v3 Lb[OR_MAX_RAY_DEPTH];   /* radiance of the path before the vertex */
-} adj_rec;
+/* A Lambert-only material but for a diffuse albedo texture. */
+static int lambert_textured(const ymrt_material_desc *m)
+{
+    return m->type == YMRT_MATERIAL_DEFAULT && m->specular_weight == 0.0f && m->metalness == 0.0f &&
+           m->transmission_weight == 0.0f && m->coat_weight == 0.0f && m->diffuse_albedo_tex != OR_EMPTY &&
+           m->specular_weight_tex == OR_EMPTY && m->metalness_tex == OR_EMPTY &&
+           m->transmission_weight_tex == OR_EMPTY && m->coat_weight_tex == OR_EMPTY;
+}
+
+/* Metalness 1 without coat: the BSDF is the GGX conductor lobe alone (the
+ * diffuse, specular and transmission lobes carry 1 - metalness); isotropic
+ * untextured roughness. */
+static int metal_only(const ymrt_material_desc *m)
+{
+    return m->type == YMRT_MATERIAL_DEFAULT && m->metalness == 1.0f && m->metalness_tex == OR_EMPTY &&
+           m->coat_weight == 0.0f && m->coat_weight_tex == OR_EMPTY && m->roughness_x == m->roughness_y &&
+           m->roughness_x_tex == OR_EMPTY && m->roughness_y_tex == OR_EMPTY;
+}
+
+int or_adjoint_prepare(const or_scene *sc, const uint32_t *alb, uint32_t n_alb, const uint32_t *rough,
+                       uint32_t n_rough, uint32_t tex, uint32_t vol, or_adj_targets *tg)
+{
+    memset(tg, 0, sizeof(*tg));
+    uint32_t nm = sc->material_count;
+    tg->alb_slot = (int32_t *)malloc(sizeof(int32_t) * (nm ? nm : 1));
+    tg->rough_slot = (int32_t *)malloc(sizeof(int32_t) * (nm ? nm : 1));
+    for (uint32_t i = 0; i < nm; ++i) tg->alb_slot[i] = tg->rough_slot[i] = -1;
+    tg->n_alb = n_alb;
+    tg->n_rough = n_rough;
+    tg->tex = tex;
+    tg->vol = vol;
+    int pos = 1;
+    for (uint32_t i = 0; i < n_alb; ++i, ++pos) {
+        if (alb[i] >= nm || !or_lambert_only(&sc->materials[alb[i]])) return pos;
+        tg->alb_slot[alb[i]] = (int32_t)i;
+    }
+    for (uint32_t i = 0; i < n_rough; ++i, ++pos) {
+        if (rough[i] >= nm || !metal_only(&sc->materials[rough[i]])) return pos;
+        tg->rough_slot[rough[i]] = (int32_t)i;
+    }
+    tg->off_rough = (size_t)n_alb * 3;
+    tg->off_tex = tg->off_rough + n_rough;
+    size_t ntex = 0;
+    if (tex != OR_EMPTY) {
+        if (tex >= sc->texture_count || sc->textures[tex].desc->type != YMRT_TEXTURE_IMAGE) return pos;
+        for (uint32_t m = 0; m < nm; ++m) {
+            const ymrt_material_desc *md = &sc->materials[m];
+            if (md->diffuse_albedo_tex == tex && !lambert_textured(md)) return pos;
+        }
+        ntex = (size_t)sc->textures[tex].width * sc->textures[tex].height * sc->textures[tex].channels;
+    }
+    ++pos;
+    tg->off_vox = tg->off_tex + ntex;
+    size_t nvox = 0;
+    if (vol != OR_EMPTY) {
+        if (vol >= sc->volume_count) return pos;
+        const ymrt_volume_desc *v = &sc->volumes[vol];
+        if (!(v->density_grid >= 0 && (uint32_t)v->density_grid < sc->grid_count) || v->density_texture >= 0 ||
+            v->sigma_t[0] != v->sigma_t[1] || v->sigma_t[0] != v->sigma_t[2])
+            return pos;
+        const or_grid *g = &sc->grids[v->density_grid];
+        nvox = (size_t)g->nx * g->ny * g->nz;
+    }
+    tg->n_params = tg->off_vox + nvox;
+    return 0;
+}
+
+void or_adjoint_free(or_adj_targets *tg)
+{
+    free(tg->alb_slot);
+    free(tg->rough_slot);
+    tg->alb_slot = tg->rough_slot = NULL;
+}
+
+/* The four bilinear taps of or_texture_eval at (u, v): texel offsets (of
+ * the first channel) and weights. */
+static void texture_taps(const or_texture *t, float u, float v, size_t *off, float *w)
+{
+    float tu = u * t->su + t->ou;
+    if (tu < 0.0f) tu = fmaxr(0.0f, tu + fabsf(floorf(tu)));
+    float tv = v * t->sv + t->ov;
+    if (tv < 0.0f) tv = fmaxr(0.0f, tv + fabsf(floorf(tv)));
+    float px = tu * (float)t->width, py = tv * (float)t->height;
+    float cx = px - 0.5f, cy = py - 0.5f;
+    int x0 = (int)floorf(cx), y0 = (int)floorf(cy);
+    float fx = cx - (float)x0, fy = cy - (float)y0;
+    int xs[4] = {x0, x0 + 1, x0, x0 + 1}, ys[4] = {y0, y0, y0 + 1, y0 + 1};
+    w[0] = (1.0f - fx) * (1.0f - fy); w[1] = fx * (1.0f - fy); w[2] = (1.0f - fx) * fy; w[3] = fx * fy;
+    int wd = (int)t->width, ht = (int)t->height;
+    for (int k = 0; k < 4; ++k) {
+        int x = xs[k], y = ys[k];
+        if (t->wrap == YMRT_WRAP_CLAMP) {
+            x = x < 0 ? 0 : (x > wd - 1 ? wd - 1 : x);
+            y = y < 0 ? 0 : (y > ht - 1 ? ht - 1 : y);
+        } else {
+            x = ((x % wd) + wd) % wd;
+            y = ((y % ht) + ht) % ht;
+        }
+        off[k] = ((size_t)y * wd + x) * t->channels;
+    }
+}
 
 static v3 trace_path(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t px, uint32_t py,
-                     uint32_t sample, uint64_t *counts, adj_rec *rec);
+                     uint32_t sample, uint64_t *counts, or_adj_path *adj);
 
 v3 or_trace_path(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t px, uint32_t py,
                  uint32_t sample, uint64_t *counts)
@@ -2110,34 +2356,37 @@ v3 or_trace_path(const or_scene *sc, const ymrt_render_settings *rs, int frame, 
     return trace_path(sc, rs, frame, px, py, sample, counts, NULL);
 }
 
-/* Path-replay adjoint of the frame for per-material Lambert albedos: with
- * adj the per-pixel derivative of a loss with respect to the noise image
- * (rgb), grad[m*3+c] += sum over paths of adj * (L_path - L_before_vertex) /
- * albedo / spp over the path's vertices on Lambert-only material m.  Also
- * writes the frame like or_render_rows. */
+/* Path-replay adjoint of the frame: with adj the per-pixel derivative of a
+ * loss with respect to the noise image (rgb), adds to grad (laid out by
+ * or_adjoint_prepare) the derivative of sum_pixels adj . image over the
+ * frame's paths, and writes the frame like or_render_rows.  Each path is
+ * traced once, recording its gradient events (see or_adj_path); the
+ * downstream events are resolved with the path's radiance at its end.
+ * Needs Russian roulette and the indirect clamp off. */
 void or_render_adjoint(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t row0, uint32_t row1,
-                       const float *adj, double *grad, float *out)
+                       const float *adj, const or_adj_targets *tg, double *grad, float *out)
 {
     float inv_spp = 1.0f / (float)(rs->spp > 1 ? rs->spp : 1);
-    adj_rec rec;
+    or_adj_path *a = (or_adj_path *)calloc(1, sizeof(or_adj_path));
+    a->tg = tg;
+    a->grad = grad;
     for (uint32_t y = row0; y < row1 && y < rs->height; ++y) {
         for (uint32_t x = 0; x < rs->width; ++x) {
             size_t pix = (size_t)y * rs->width + x;
             float acc[4] = {0, 0, 0, 0};
             for (int smp = 0; smp < rs->spp; ++smp) {
-                rec.n = 0;
-                v3 L = trace_path(sc, rs, frame, x, y, (uint32_t)smp, NULL, &rec);
+                a->nd = 0;
+                a->nray = 0;
+                a->adj = mk3(adj[pix * 3 + 0] * inv_spp, adj[pix * 3 + 1] * inv_spp, adj[pix * 3 + 2] * inv_spp);
+                v3 L = trace_path(sc, rs, frame, x, y, (uint32_t)smp, NULL, a);
                 acc[0] += L.x * inv_spp;
                 acc[1] += L.y * inv_spp;
                 acc[2] += L.z * inv_spp;
                 acc[3] += inv_spp;
-                for (uint32_t i = 0; i < rec.n; ++i) {
-                    const float *alb = sc->materials[rec.m[i]].diffuse_albedo;
-                    float d[3] = {L.x - rec.Lb[i].x, L.y - rec.Lb[i].y, L.z - rec.Lb[i].z};
-                    for (int c = 0; c < 3; ++c)
-                        if (alb[c] > 0.0f)
-                            grad[rec.m[i] * 3 + c] += (double)adj[pix * 3 + c] * (double)d[c] / (double)alb[c] *
-                                                      (double)inv_spp;
+                for (uint32_t i = 0; i < a->nd; ++i) {
+                    const adj_down *e = &a->down[i];
+                    v3 d = mul3(sub3(L, e->mark), e->w);
+                    grad[e->param] += (double)a->adj.x * d.x + (double)a->adj.y * d.y + (double)a->adj.z * d.z;
                 }
             }
             if (out) {
@@ -2146,10 +2395,12 @@ void or_render_adjoint(const or_scene *sc, const ymrt_render_settings *rs, int f
             }
         }
     }
+    free(a->down);
+    free(a);
 }
 
 static v3 trace_path(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t px, uint32_t py,
-                     uint32_t sample, uint64_t *counts, adj_rec *rec)
+                     uint32_t sample, uint64_t *counts, or_adj_path *adj)
 {
     or_sampler s;
     uint32_t pixel_index = py * rs->width + px;
@@ -2158,7 +2409,7 @@ static v3 trace_path(const or_scene *sc, const ymrt_render_settings *rs, int fra
     rng.s = &s;
     rng.hash_key[0] = pixel_index; rng.hash_key[1] = 0; rng.hash_key[2] = sample; rng.hash_key[3] = (uint32_t)frame;
     rng.hash_ctr = 0;
-    path_ctx c = {sc, rs, &s, &rng, counts};
+    path_ctx c = {sc, rs, &s, &rng, counts, adj};
 
     v3 L = s3(0.0f), beta = s3(1.0f);
     int depth = 1, never_scatter = 1;
@@ -2184,7 +2435,8 @@ static v3 trace_path(const or_scene *sc, const ymrt_render_settings *rs, int fra
             float dist = 0.0f;
             nv = tr_volumes(&t, vi, sc->volume_count);
             for (int i = 0; i < nv; ++i) gs[i] = sc->volumes[vi[i]].g;
-            int hv = sample_volume(sc, &ray, nv, vi, hit ? h.t : OR_TMAX, &trw, &dist, vw, &s);
+            if (adj) adj->L = L;
+            int hv = sample_volume(sc, &ray, nv, vi, hit ? h.t : OR_TMAX, &trw, &dist, vw, &s, adj);
             hit = hit || hv;
             beta = mul3(beta, trw);
             if (hv) { h.volume = 1; h.bary = add3(ray.o, scl3(ray.d, dist)); }
@@ -2196,6 +2448,12 @@ static v3 trace_path(const or_scene *sc, const ymrt_render_settings *rs, int fra
         /* ---- shading stage */
         rng.hash_key[1] = ++vertex;
         rng.hash_ctr = 0;
+        if (adj) {
+            adj->beta = beta;
+            adj->rough_param = -1;
+            adj->nray = 0;
+            adj->recording = adj->tg->vol != OR_EMPTY;
+        }
         if (rs->enable_volume_scattering && h.volume) {
             v3 vpos = h.bary;
             if (rs->enable_distant_light)
@@ -2248,10 +2506,48 @@ static v3 trace_path(const or_scene *sc, const ymrt_render_settings *rs, int fra
                            in->world_to_object, tc, duv);
         b.is_default = m->type == YMRT_MATERIAL_DEFAULT;
         if (b.is_default) init_default_bsdf(sc, &b, m, tc, ray.d, h.back, tr_ray_ior(&t), tr_ex_ior(&t, h.inst));
-        if (rec && rec->n < OR_MAX_RAY_DEPTH && or_lambert_only(m)) {
-            rec->m[rec->n] = in->material;
-            rec->Lb[rec->n] = L;
-            ++rec->n;
+        if (adj) {
+            /* a Lambert vertex multiplies everything after it by the albedo:
+             * d/d albedo_c = (L_path - L_before) / albedo_c per channel */
+            int32_t as = adj->tg->alb_slot[in->material];
+            if (as >= 0) {
+                for (int ch = 0; ch < 3; ++ch) {
+                    float al = m->diffuse_albedo[ch];
+                    if (!(al > 0.0f)) continue;
+                    v3 w = s3(0.0f);
+                    setc3(&w, ch, 1.0f / al);
+                    adj_add_down(adj, (uint32_t)(as * 3 + ch), w, L);
+                }
+            }
+            /* the same with the albedo a bilinear blend of four texels */
+            if (adj->tg->tex != OR_EMPTY && m->diffuse_albedo_tex == adj->tg->tex && lambert_textured(m)) {
+                const or_texture *tx = &sc->textures[adj->tg->tex];
+                v3 al = or_texture_graph(sc, m->diffuse_albedo_tex, tc);
+                size_t off[4];
+                float tw[4];
+                texture_taps(tx, uv.x, uv.y, off, tw);
+                for (int k = 0; k < 4; ++k) {
+                    if (!(tw[k] > 0.0f)) continue;
+                    if (tx->channels == 1) {
+                        v3 w = mk3(al.x > 0.0f ? tw[k] / al.x : 0.0f, al.y > 0.0f ? tw[k] / al.y : 0.0f,
+                                   al.z > 0.0f ? tw[k] / al.z : 0.0f);
+                        adj_add_down(adj, (uint32_t)(adj->tg->off_tex + off[k]), w, L);
+                    } else {
+                        for (int ch = 0; ch < 3; ++ch) {
+                            float a_ = comp3(al, ch);
+                            if (!(a_ > 0.0f)) continue;
+                            v3 w = s3(0.0f);
+                            setc3(&w, ch, tw[k] / a_);
+                            adj_add_down(adj, (uint32_t)(adj->tg->off_tex + off[k] + (size_t)ch), w, L);
+                        }
+                    }
+                }
+            }
+            int32_t rsl = adj->tg->rough_slot[in->material];
+            if (rsl >= 0) {
+                adj->rough_param = (int)(adj->tg->off_rough + (size_t)rsl);
+                adj->rough_r = m->roughness_x;
+            }
         }
         if (rs->enable_distant_light)
             L = add3(L, light_clamp(rs, mul3(beta, eval_distant_light(&c, &b, ray.d, &h, &t, nc, nb, p, perr, ng)), depth));
@@ -2263,6 +2559,10 @@ static v3 trace_path(const or_scene *sc, const ymrt_render_settings *rs, int fra
         float u1 = or_sampler_next(&s);
         float u2 = or_sampler_next(&s);
         if (!bsdf_sample(&b, wo, &bw, &wi, &pdf, u0, u1, u2)) return L;
+        /* the sampled direction's BSDF value scales everything after this
+         * vertex's next-event deposits (pdf held fixed) */
+        if (adj && adj->rough_param >= 0)
+            adj_add_down(adj, (uint32_t)adj->rough_param, s3(rough_dlog(adj->rough_r, wo, wi)), L);
         beta = mul3(beta, bw);
         float rr = fminr(fmaxr(beta.x, fmaxr(beta.y, beta.z)), 1.0f);
         if (depth + 1 > rs->ray_depth) {

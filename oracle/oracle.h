@@ -243,8 +243,31 @@ int or_hg_sample(float g, float u0, float u1, v3 wo, v3 *wi, float *pdf);
 v3 or_texture_eval(const or_scene *sc, uint32_t tex, float u, float v);
 void or_first_hits(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t *ids);
 int or_lambert_only(const ymrt_material_desc *m);
+
+/* Parameters of the path-replay adjoint (or_render_adjoint); the gradient is
+ * one float64 vector: albedo (3 per listed material), roughness (1 per listed
+ * material), the texels of the target texture (width * height * channels of
+ * decoded linear values), the voxels of the target volume's grid. */
+typedef struct {
+    int32_t *alb_slot;          /* per material: position in the albedo list or -1 */
+    int32_t *rough_slot;        /* per material: position in the roughness list or -1 */
+    uint32_t n_alb, n_rough;
+    uint32_t tex;               /* target texture or 0xFFFFFFFF */
+    uint32_t vol;               /* target volume or 0xFFFFFFFF */
+    size_t off_rough, off_tex, off_vox, n_params;
+} or_adj_targets;
+
+/* Checks the targets against the scene (albedo: Lambert-only materials,
+ * constant albedo; texture: an image used only as the diffuse albedo of
+ * Lambert-only materials; roughness: metal-only materials, isotropic, no
+ * textures; volume: a grid density with gray sigma_t) and lays out the
+ * gradient vector.  0 on success, else the position of the first bad target
+ * + 1 in the order albedo list, roughness list, texture, volume. */
+int or_adjoint_prepare(const or_scene *sc, const uint32_t *alb, uint32_t n_alb, const uint32_t *rough,
+                       uint32_t n_rough, uint32_t tex, uint32_t vol, or_adj_targets *tg);
+void or_adjoint_free(or_adj_targets *tg);
 void or_render_adjoint(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t row0, uint32_t row1,
-                       const float *adj, double *grad, float *out);
+                       const float *adj, const or_adj_targets *tg, double *grad, float *out);
 void or_primary_rays(const or_scene *sc, const ymrt_render_settings *rs, int frame, uint32_t sample, ymrt_ray *out);
 /* TextureEval over the texture graph rooted at tex */
 v3 or_texture_graph(const or_scene *sc, uint32_t tex, or_tex_coord tc);

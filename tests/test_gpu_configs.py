@@ -19,6 +19,7 @@ in scenes with media.
 """
 import numpy as np
 import pytest
+import torch
 
 import oracle
 import ymrt_amd
@@ -132,3 +133,24 @@ def test_texture_density_medium_parity(gpu_ctx, sigma):
     assert stats.transmittance_rays > 0
     assert same >= 0.98
     assert err <= 0.02
+
+
+def test_batch_workload_subset_equals_single_frames(gpu_ctx):
+    """Twelve scenes of the multi-scene workload (triangle counts and the
+    mix of resolutions and sample counts as configured, sizes divided by 8
+    and 4) in shared launches: every image equals ymrt_render_frame's bit
+    for bit, also when the round holds only five chunks."""
+    jobs = []
+    for i, (tris, res, spp, depth) in enumerate(scenes.batch_workload(12, seed=11)):
+        r = res // 8
+        sc = ymrt_amd.Scene(gpu_ctx, scenes.batch_scene(min(tris, 60000), r, r, i))
+        jobs.append((sc, abi.render_settings(r, r, max(1, spp // 4), depth), 1 + i % 3))
+    out = gpu_ctx.render_batch(jobs)
+    out5 = gpu_ctx.render_batch(jobs, max_slots=5)
+    for (sc, rs, frame), (img, st), (img5, _) in zip(jobs, out, out5):
+        ref, ref_st = sc.render_frame(rs, frame)
+        assert torch.equal(img, ref)
+        assert torch.equal(img5, ref)
+        assert st.extension_rays == ref_st.extension_rays and st.shadow_rays == ref_st.shadow_rays
+    for sc, _, _ in jobs:
+        sc.close()

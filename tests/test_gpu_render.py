@@ -252,7 +252,7 @@ def test_adjoint_albedo_gradient(gpu_ctx):
     img, grad, stats = sc.render_adjoint(rs, torch.from_numpy(adj).cuda(), 1, timing=True)
     assert torch.equal(img, sc.render_frame(rs, 1)[0])
     g = grad.cpu().numpy().astype(np.float64)
-    _, og = oracle.Scene(sa).render_adjoint(rs, adj, 1, threads=8)
+    og = oracle.Scene(sa).render_adjoint(rs, adj, 1, threads=8)[1]["albedo"]   # materials 0-2
     print("adjoint gpu", g[:3].tolist(), "oracle", og[:3].tolist(), "ms", stats.ms_adjoint)
     assert np.all(g[3:] == 0.0)
     assert np.all(og[:3] > 0)
@@ -317,8 +317,9 @@ def test_autograd_albedo_matches_finite_differences(gpu_ctx):
 
 def test_render_batch_equals_single_frames(gpu_ctx):
     """ymrt_render_batch: frames of several scenes (one scene twice, with two
-    frame indices) on several streams equal ymrt_render_frame's bit for bit;
-    a bad item fails the call with its index in the message."""
+    frame indices) in shared launches equal ymrt_render_frame's bit for bit,
+    packed three chunks to a round and all in one round; a bad item fails
+    the call with its index in the message."""
     jobs = []
     cb = ymrt_amd.Scene(gpu_ctx, scenes.cornell_box(40, 32))
     od = ymrt_amd.Scene(gpu_ctx, scenes.outdoor(48, 32))
@@ -329,11 +330,14 @@ def test_render_batch_equals_single_frames(gpu_ctx):
             (tb, abi.render_settings(32, 32, 2, 4), 3),
             (it, abi.render_settings(64, 40, 4, 6, volume=1), 1),
             (cb, abi.render_settings(40, 32, 4, 5), 2)]
-    out = gpu_ctx.render_batch(jobs, max_streams=3)
-    for (sc, rs, frame), (img, st) in zip(jobs, out):
+    out = gpu_ctx.render_batch(jobs, max_slots=3)
+    out_one = gpu_ctx.render_batch(jobs)
+    for (sc, rs, frame), (img, st), (img1, st1) in zip(jobs, out, out_one):
         ref, ref_st = sc.render_frame(rs, frame)
         assert torch.equal(img, ref)
+        assert torch.equal(img1, ref)
         assert st.extension_rays == ref_st.extension_rays > 0
+        assert st1.shadow_rays == ref_st.shadow_rays
     assert not torch.equal(out[0][0], out[4][0])
     bad = list(jobs)
     bad[2] = (tb, abi.render_settings(32, 32, 2, 4), 0)

@@ -25,7 +25,7 @@ EXPORTS = [
     "ymrt_ymtmp_load", "ymrt_ymtmp_parse", "ymrt_ymtmp_scene", "ymrt_ymtmp_warning_count",
     "ymrt_ymtmp_warning", "ymrt_ymtmp_find", "ymrt_ymtmp_transform", "ymrt_ymtmp_free",
     "ymrt_render_frame_aov", "ymrt_save_png", "ymrt_render_adjoint", "ymrt_scene_update_materials",
-    "ymrt_render_batch", "ymrt_autotune",
+    "ymrt_render_batch", "ymrt_autotune", "ymrt_scene_release_buffers", "ymrt_scene_update_texture",
 ]
 
 _lib = None
@@ -55,6 +55,8 @@ def lib():
         L.ymrt_scene_create.argtypes = [vp, C.POINTER(abi.SceneDesc), C.POINTER(vp)]
         L.ymrt_scene_destroy.argtypes = [vp]
         L.ymrt_scene_get_info.argtypes = [vp, C.POINTER(abi.SceneInfo)]
+        L.ymrt_scene_release_buffers.argtypes = [vp]
+        L.ymrt_scene_update_texture.argtypes = [vp, C.c_uint32, vp, vp]
         L.ymrt_trace_closest.argtypes = [vp, vp, C.c_uint32, vp, vp]
         L.ymrt_trace_any.argtypes = [vp, vp, C.c_uint32, vp, vp]
         L.ymrt_trace_closest_counted.argtypes = [vp, vp, C.c_uint32, vp, vp, vp]
@@ -124,9 +126,10 @@ class Context:
             pass
 
 
-    def render_batch(self, jobs, max_streams=0):
+    def render_batch(self, jobs, max_slots=0):
         """ymrt_render_batch over jobs = [(scene, settings, frame), ...]:
-        returns one (H, W, 4) cuda noise image and FrameStats per job."""
+        returns one (H, W, 4) cuda noise image and FrameStats per job;
+        max_slots bounds the frame chunks per shared launch (0: 64)."""
         import torch
         n = len(jobs)
         items = (abi.BatchItem * max(1, n))()
@@ -141,7 +144,7 @@ class Context:
             items[i].timing = 0
             items[i].noise_rgba = img.data_ptr()
             items[i].stats = C.pointer(st)
-        _check(lib().ymrt_render_batch(self.h, items, n, max_streams, _stream()))
+        _check(lib().ymrt_render_batch(self.h, items, n, max_slots, _stream()))
         return [(img, st) for img, st, _ in keep]
 
 
@@ -170,6 +173,15 @@ class Scene:
         i = abi.SceneInfo()
         _check(lib().ymrt_scene_get_info(self.h, C.byref(i)))
         return i
+
+    def update_texture(self, texture, texels):
+        """ymrt_scene_update_texture: new linear values of a float image texture."""
+        t = np.ascontiguousarray(texels, np.float32)
+        _check(lib().ymrt_scene_update_texture(self.h, texture, t.ctypes.data_as(C.c_void_p), _stream()))
+
+    def release_buffers(self):
+        """ymrt_scene_release_buffers: free the path state of this scene's frames."""
+        _check(lib().ymrt_scene_release_buffers(self.h))
 
     def update_materials(self, first, materials):
         """ymrt_scene_update_materials: materials (abi.MaterialDesc records)

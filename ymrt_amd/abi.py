@@ -85,6 +85,7 @@ class TextureDesc(C.Structure):
         ("offset", C.c_float), ("variation", C.c_float), ("radius", C.c_float),
         ("layer_count", C.c_int32), ("normalized", C.c_int32), ("turbulence_axes", C.c_int32),
         ("frequency1", C.c_float), ("lacunarity1", C.c_float), ("gain1", C.c_float), ("layer_count1", C.c_int32),
+        ("texels_f32", C.POINTER(C.c_float)),
     ]
 
 
@@ -296,6 +297,14 @@ class SceneArrays:
 
     def add_texture(self, texels, wrap=WRAP_REPEAT, color_space=COLOR_SRGB,
                     scale=(1.0, 1.0), offset=(0.0, 0.0)):
+        """Image texture: uint8 texels (decoded by color_space), or float32
+        texels taken as linear values (a float texture, color_space unused)."""
+        if np.asarray(texels).dtype == np.float32:
+            t = np.ascontiguousarray(texels, np.float32)
+            if t.ndim == 2:
+                t = t[:, :, None]
+            self.textures.append(dict(texels=t, wrap=wrap, cs=COLOR_LINEAR, scale=scale, offset=offset))
+            return len(self.textures) - 1
         t = np.ascontiguousarray(texels, np.uint8)
         if t.ndim == 2:
             t = t[:, :, None]
@@ -386,7 +395,10 @@ class SceneArrays:
                 h, w, c = t["texels"].shape
                 d.type = TEXTURE_IMAGE
                 d.width, d.height, d.channels = w, h, c
-                d.texels = _ptr(t["texels"], C.c_uint8)
+                if t["texels"].dtype == np.float32:
+                    d.texels_f32 = _ptr(t["texels"], C.c_float)
+                else:
+                    d.texels = _ptr(t["texels"], C.c_uint8)
                 d.wrap_mode, d.color_space = t["wrap"], t["cs"]
                 d.scale_u, d.scale_v = t["scale"]
                 d.offset_u, d.offset_v = t["offset"]
@@ -500,7 +512,11 @@ def arrays_from_desc(sd):
         sa.add_material(m)
     for i in range(sd.texture_count):
         t = sd.textures[i]
-        if t.type == TEXTURE_IMAGE:
+        if t.type == TEXTURE_IMAGE and t.texels_f32:
+            tex = _np(t.texels_f32, t.width * t.height * t.channels, np.float32).reshape(t.height, t.width,
+                                                                                           t.channels)
+            sa.add_texture(tex, wrap=t.wrap_mode, scale=(t.scale_u, t.scale_v), offset=(t.offset_u, t.offset_v))
+        elif t.type == TEXTURE_IMAGE:
             tex = _np(t.texels, t.width * t.height * t.channels, np.uint8).reshape(t.height, t.width, t.channels)
             sa.add_texture(tex, wrap=t.wrap_mode, color_space=t.color_space, scale=(t.scale_u, t.scale_v),
                            offset=(t.offset_u, t.offset_v))

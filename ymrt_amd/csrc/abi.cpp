@@ -13,7 +13,8 @@ void set_error(const std::string &msg) { g_last_error = msg; }
 int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d);                       // scene.cpp
 int update_materials(ymrt_scene *sc, uint32_t first, uint32_t count, const ymrt_material_desc *m,
                      hipStream_t s);  // scene.cpp
-int render_batch(ymrt_context *ctx, const ymrt_batch_item *items, uint32_t n, uint32_t max_streams,
+int update_texture(ymrt_scene *sc, uint32_t tex, const float *texels, hipStream_t s);  // scene.cpp
+int render_batch(ymrt_context *ctx, const ymrt_batch_item *items, uint32_t n, uint32_t max_slots,
                  hipStream_t stream);  // batch.cpp
 int autotune(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, hipStream_t stream,
              ymrt_tune_result *out);  // tune.cpp
@@ -112,6 +113,15 @@ int ymrt_scene_destroy(ymrt_scene *sc)
     destroy_render_buffers(sc);
     for (void *p : sc->allocations) (void)hipFree(p);
     delete sc;
+    return YMRT_OK;
+}
+
+int ymrt_scene_release_buffers(ymrt_scene *sc)
+{
+    YM_REQUIRE(sc != nullptr, "scene is NULL");
+    YM_HIP(hipSetDevice(sc->ctx->device));
+    YM_HIP(hipDeviceSynchronize());
+    destroy_render_buffers(sc);
     return YMRT_OK;
 }
 
@@ -236,10 +246,15 @@ int ymrt_halton_values(ymrt_scene *sc, const uint64_t *indices, uint32_t n, uint
     return YMRT_OK;
 }
 
-int ymrt_render_batch(ymrt_context *ctx, const ymrt_batch_item *items, uint32_t n, uint32_t max_streams,
+int ymrt_render_batch(ymrt_context *ctx, const ymrt_batch_item *items, uint32_t n, uint32_t max_slots,
                       void *stream)
 {
-    return render_batch(ctx, items, n, max_streams, (hipStream_t)stream);
+    try {
+        return render_batch(ctx, items, n, max_slots, (hipStream_t)stream);
+    } catch (const std::exception &e) {
+        set_error(std::string("batch failed: ") + e.what());
+        return YMRT_ERR_INTERNAL;
+    }
 }
 
 int ymrt_autotune(ymrt_scene *sc, const ymrt_render_settings *rs, int frame_index, void *stream,
@@ -254,6 +269,13 @@ int ymrt_scene_update_materials(ymrt_scene *sc, uint32_t first, uint32_t count,
 {
     YM_REQUIRE(sc != nullptr, "scene is NULL");
     return update_materials(sc, first, count, materials, (hipStream_t)stream);
+}
+
+int ymrt_scene_update_texture(ymrt_scene *sc, uint32_t texture, const float *texels, void *stream)
+{
+    YM_REQUIRE(sc != nullptr, "scene is NULL");
+    YM_HIP(hipSetDevice(sc->ctx->device));
+    return update_texture(sc, texture, texels, (hipStream_t)stream);
 }
 
 int ymrt_texture_eval(ymrt_scene *sc, uint32_t texture, const float *coords, uint32_t n, float *out, void *stream)

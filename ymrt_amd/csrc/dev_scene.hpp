@@ -97,6 +97,7 @@ struct DevTexture {
     float frequency1, lacunarity1, gain1;
     int32_t layers1;
     uint32_t prog_start, prog_len; // post-order program of the graph rooted here
+    uint32_t f32;                  // IMAGE: texels are linear floats (offset into texels_f32)
 };
 
 // Texture program codes: a texture index, or kTexBlack for an unconnected input.
@@ -155,6 +156,7 @@ struct DevScene {
     const DevTexture *textures;
     const uint32_t *tex_prog;      // texture programs (codes)
     const uint32_t *texels;
+    const float *texels_f32;       // float image textures, channels per texel
     const float *tex_lut;          // 2 x 256 decode tables (sRGB, linear)
     const DevGrid *grids;
     const float *voxels;

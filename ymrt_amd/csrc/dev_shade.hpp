@@ -175,6 +175,11 @@ __device__ __forceinline__ f3 texel(const DevScene &sc, const DevTexture &t, int
         x = ((x % w) + w) % w;
         y = ((y % h) + h) % h;
     }
+    if (t.f32) {
+        const float *p = sc.texels_f32 + t.offset + ((uint64_t)y * w + x) * t.channels;
+        if (t.channels == 1) return s3(p[0]);
+        return f3{p[0], t.channels > 1 ? p[1] : 0.0f, t.channels > 2 ? p[2] : 0.0f};
+    }
     const uint32_t word = sc.texels[t.offset + (uint64_t)y * w + x];
     const float *lut = sc.tex_lut + (t.color_space == YMRT_COLOR_SRGB ? 0 : 256);
     if (t.channels == 1) return s3(lut[word & 0xFFu]);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -32,12 +33,17 @@ void set_error(const std::string &msg);
         }                                                                                              \
     } while (0)
 
-struct RenderBuffers;  // render.cpp
+struct RenderBuffers;  // buffers.hpp
+struct BatchMemory;    // batch.cpp
+void free_batch_memory(BatchMemory *m);
 
 }  // namespace ym
 
 struct ymrt_context {
     int device = 0;
+    ym::BatchMemory *batch = nullptr;  // ymrt_render_batch's buffers
+    std::mutex batch_mu;
+    ~ymrt_context() { ym::free_batch_memory(batch); }
 };
 
 struct ymrt_scene {
@@ -52,6 +58,7 @@ struct ymrt_scene {
     const uint32_t *adj_mat = nullptr;  // per material: 1 when Lambert-only (adjoint)
     // host state that ymrt_scene_update_materials re-derives flags from
     std::vector<ym::DevMaterial> materials;
+    std::vector<ym::DevTexture> textures;  // texture records (float texel updates)
     uint32_t texture_count = 0;
     bool has_boundaries = false;
     bool procedural_textures = false;

@@ -143,6 +143,56 @@ __device__ __forceinline__ void wave_add64(unsigned long long *counter, uint32_t
     if (lane_id() == 0 && s) atomicAdd(counter, (unsigned long long)s);
 }
 
+// ------------------------------------------------------------ frame slots
+// Multi-frame launches (ymrt_render_batch): one launch of a pass kernel serves
+// every frame chunk of a round, each chunk a slot with its own scene,
+// settings, counters and part of the path state.  A wave starts at its home
+// slot -- the slot whose share of the launch's items holds the wave's index,
+// so waves spread over the slots in proportion to their work -- drains it
+// with the single-frame body, then visits the others in turn and helps with
+// what is left there.  The slot index is wave-uniform, so the slot's fields
+// come from scalar loads as kernel arguments do.
+__device__ __forceinline__ uint32_t pass_counter(const FrameSlot &f, uint32_t pass, uint32_t k)
+{
+    return f.prm.counters[pass * kCtrStride + k];
+}
+
+// the work counter has passed the item count: nothing left to fetch
+__device__ __forceinline__ bool work_drained(const uint32_t *work, uint32_t n)
+{
+    return __hip_atomic_load(work, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n;
+}
+__device__ __forceinline__ bool pass_drained(const FrameSlot &f, uint32_t pass, uint32_t work, uint32_t count)
+{
+    const uint32_t *c = f.prm.counters + pass * kCtrStride;
+    return work_drained(c + work, c[count]);
+}
+
+template <typename Count, typename Drained, typename Body>
+__device__ __forceinline__ void for_slots(const FrameSlot *__restrict__ fs, uint32_t nfs, Count count,
+                                          Drained drained, Body body)
+{
+    uint32_t total = 0;
+    for (uint32_t s = 0; s < nfs; ++s) total += count(fs[s]);
+    if (total == 0) return;
+    const uint32_t target = (uint32_t)(((uint64_t)blockIdx.x * total) / gridDim.x);
+    uint32_t home = 0;
+    for (uint32_t s = 0, acc = 0; s < nfs; ++s) {
+        acc += count(fs[s]);
+        if (acc > target) {
+            home = s;
+            break;
+        }
+    }
+    for (uint32_t j = 0; j < nfs; ++j) {
+        uint32_t s = home + j;
+        s = __builtin_amdgcn_readfirstlane(s >= nfs ? s - nfs : s);
+        const FrameSlot &f = fs[s];
+        if (count(f) == 0 || drained(f)) continue;
+        body(f);
+    }
+}
+
 // ------------------------------------------------------------ state access
 __device__ __forceinline__ f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
 __device__ __forceinline__ float4 f4(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
@@ -243,7 +293,7 @@ __device__ __forceinline__ f3 light_clamp(const ymrt_render_settings &rs, f3 c, 
 }
 
 // ------------------------------------------------------------ raygen
-__global__ __launch_bounds__(kBlock) void k_raygen(DevScene sc, PtParams prm, PathState ps)
+__device__ __forceinline__ void raygen_body(const DevScene &sc, const PtParams &prm, const PathState &ps)
 {
     uint32_t *cnt = prm.counters;  // pass 0 slots
     uint32_t *const ctrs[1] = {cnt + kCtrActive};
@@ -305,6 +355,17 @@ __global__ __launch_bounds__(kBlock) void k_raygen(DevScene sc, PtParams prm, Pa
     });
 }
 
+
+__global__ __launch_bounds__(kBlock) void k_raygen_m(const FrameSlot *__restrict__ fs, uint32_t nfs)
+{
+    for_slots(
+        fs, nfs, [](const FrameSlot &f) { return f.prm.chunk_paths; },
+        [](const FrameSlot &f) { return work_drained(f.prm.counters + kCtrWorkRaygen, f.prm.chunk_paths); },
+        [](const FrameSlot &f) { raygen_body(f.sc, f.prm, f.ps); });
+}
+
+
+
 // ------------------------------------------------------------ extend
 // Closest hit for every queued path (Ray_trace_path_tracing,
 // src/Shading.cu:1765-1901).  With kMedia, paths that carry medium records
@@ -315,7 +376,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(DevScene sc, PtParams prm, Pa
 // (RefillRange).
 
 template <bool kNearby, bool kMedia, bool kSphere, int kStack>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere ? 1 : 4))) void k_extend(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
+__device__ __forceinline__ void extend_body(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass)
 {
     __shared__ uint32_t stack[kStackWindow * kBlock];
     __shared__ uint32_t nbp[(kNearby ? kNearbyCap : 1) * kBlock];
@@ -379,7 +440,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere 
             cam_rays += (info_depth(info) == 1 && (info & kInfoNeverScatter)) ? 1u : 0u;
             const Hit &h = tv.hit;
             if (pass == 0 && prm.aov) ps.first_hit[p] = hit ? h.inst : kEmpty;
-            const bool in_medium = kMedia && (ps.xhdr[p] & 0xFFu) != 0;
+            const bool in_medium = kMedia && prm.media && (ps.xhdr[p] & 0xFFu) != 0;
             if (!hit && !in_medium) {
                 f3 Lv = xyz(ps.L[p]);
                 if (rs.enable_env_light)
@@ -421,6 +482,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere 
     wave_add64(prm.stats + kStatExtendTris, tc.tris);
 }
 
+
+template <bool kNearby, bool kMedia, bool kSphere, int kStack>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSphere ? 1 : 4))) void k_extend_m(const FrameSlot *__restrict__ fs, uint32_t nfs, uint32_t pass)
+{
+    for_slots(
+        fs, nfs, [&](const FrameSlot &f) { return pass_counter(f, pass, kCtrActive); },
+        [&](const FrameSlot &f) { return pass_drained(f, pass, kCtrWorkExtend, kCtrActive); },
+        [&](const FrameSlot &f) { extend_body<kNearby, kMedia, kSphere, kStack>(f.sc, f.prm, f.ps, pass); });
+}
+
+
+
 // ------------------------------------------------------------ compaction
 // Gathers the paths flagged for the next stage of this pass (kShade: surface
 // and volume shading; otherwise free flight) in slot order: the traversal
@@ -450,13 +523,13 @@ struct Compact {
 };
 
 template <int kMode>
-__global__ __launch_bounds__(kCompactThreads) void k_compact_count(PtParams prm, PathState ps, uint32_t pass)
+__device__ __forceinline__ void compact_count_body(const PtParams &prm, const PathState &ps, uint32_t pass, uint32_t b)
 {
     constexpr int kQ = Compact<kMode>::kQ;
     constexpr uint32_t kWaves = kCompactThreads / kBlock;
     __shared__ uint32_t red[kQ][kWaves];
     const uint32_t n = prm.chunk_paths;
-    const uint32_t b = blockIdx.x, first = b * kCompactSpan;
+    const uint32_t first = b * kCompactSpan;
     if (first >= n) return;
     const uint32_t last = min(first + kCompactSpan, n);
     const uint32_t tag = pass << kFlagBits;
@@ -475,12 +548,22 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_count(PtParams prm,
     if (threadIdx.x < (uint32_t)kQ) {
         uint32_t v = 0;
         for (uint32_t w = 0; w < kWaves; ++w) v += red[threadIdx.x][w];
-        ps.ccount[threadIdx.x * (prm.cap / kCompactSpan + 1) + b] = v;
+        ps.ccount[threadIdx.x * prm.ccount_stride + b] = v;
     }
 }
 
+
+// one row of the grid (blockIdx.y) per slot
 template <int kMode>
-__global__ __launch_bounds__(kCompactThreads) void k_compact_scatter(PtParams prm, PathState ps, uint32_t pass)
+__global__ __launch_bounds__(kCompactThreads) void k_compact_count_m(const FrameSlot *__restrict__ fs, uint32_t pass)
+{
+    const FrameSlot &f = fs[blockIdx.y];
+    compact_count_body<kMode>(f.prm, f.ps, pass, blockIdx.x);
+}
+
+template <int kMode>
+__device__ __forceinline__ void compact_scatter_body(const PtParams &prm, const PathState &ps, uint32_t pass,
+                                                     uint32_t b)
 {
     constexpr int kQ = Compact<kMode>::kQ;
     constexpr uint32_t kWaves = kCompactThreads / kBlock;
@@ -488,10 +571,10 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_scatter(PtParams pr
     __shared__ uint32_t base[kQ];
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     const uint32_t n = prm.chunk_paths;
-    const uint32_t b = blockIdx.x, first = b * kCompactSpan;
+    const uint32_t first = b * kCompactSpan;
     if (first >= n) return;
     const uint32_t last = min(first + kCompactSpan, n);
-    const uint32_t stride = prm.cap / kCompactSpan + 1;
+    const uint32_t stride = prm.ccount_stride;
     uint32_t *const qs[4] = {kMode == kCompactShade    ? ps.q_hit
                              : kMode == kCompactMedium ? ps.q_medium
                                                        : ps.q_transmit,
@@ -547,6 +630,15 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_scatter(PtParams pr
     if (last == n && threadIdx.x < (uint32_t)kQ) cnt[ctr[threadIdx.x]] = off[threadIdx.x];
 }
 
+
+// one row of the grid (blockIdx.y) per slot
+template <int kMode>
+__global__ __launch_bounds__(kCompactThreads) void k_compact_scatter_m(const FrameSlot *__restrict__ fs, uint32_t pass)
+{
+    const FrameSlot &f = fs[blockIdx.y];
+    compact_scatter_body<kMode>(f.prm, f.ps, pass, blockIdx.x);
+}
+
 // ------------------------------------------------------------ medium
 // Free flight through the media of the path's top-priority records
 // (SampleVolumeScattering, src/VolumeEval.cuh:217-251, called from
@@ -556,7 +648,7 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_scatter(PtParams pr
 // media as well (the texture graph evaluator and its LDS stack are compiled
 // in); 0 every medium is homogeneous and the tracker is compiled out.
 template <int kHet>
-__global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
+__device__ __forceinline__ void medium_body(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass)
 {
     constexpr bool kGrid = kHet > 0;
     __shared__ float tex_stk[kHet == 2 ? kTexStack * 3 * kBlock : 1];
@@ -611,6 +703,18 @@ __global__ __launch_bounds__(kBlock) void k_medium(DevScene sc, PtParams prm, Pa
     });
 }
 
+
+template <int kHet>
+__global__ __launch_bounds__(kBlock) void k_medium_m(const FrameSlot *__restrict__ fs, uint32_t nfs, uint32_t pass)
+{
+    for_slots(
+        fs, nfs, [&](const FrameSlot &f) { return pass_counter(f, pass, kCtrMedium); },
+        [&](const FrameSlot &f) { return pass_drained(f, pass, kCtrWorkMedium, kCtrMedium); },
+        [&](const FrameSlot &f) { medium_body<kHet>(f.sc, f.prm, f.ps, pass); });
+}
+
+
+
 // ------------------------------------------------------------ shade
 // Records shadow ray k of path p (origin, direction, maximum distance, BSDF or
 // phase value, clamped Li, MIS weight) straight to its slot in HBM.  The
@@ -654,7 +758,7 @@ __device__ __forceinline__ float shadow_tmax(f3 pos, f3 ldir, f3 lpos, f3 lerr, 
 // loads, and the third wave hides more of them than the spills cost (6 %
 // faster on the interior frame; 4 waves spill too much).
 template <bool kVolume, bool kGraph, uint32_t kClass = kMatGeneral>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGraph ? 2 : 3))) void k_shade(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
+__device__ __forceinline__ void shade_body(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass)
 {
     __shared__ float tex_stk[(kVolume || !kGraph) ? 1 : kTexStack * 3 * kBlock];
     uint32_t *cnt = prm.counters + pass * kCtrStride;
@@ -1055,6 +1159,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGraph ?
     });
 }
 
+
+template <bool kVolume, bool kGraph, uint32_t kClass = kMatGeneral>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGraph ? 2 : 3))) void k_shade_m(const FrameSlot *__restrict__ fs, uint32_t nfs, uint32_t pass)
+{
+    constexpr uint32_t kQueue = kVolume ? kCtrHitVol : kClass == kMatDiffuse ? kCtrHitDiffuse
+                                : kClass == kMatMetal ? kCtrHitMetal : kCtrHit;
+    constexpr uint32_t kWork = kVolume ? kCtrWorkShadeVol : kClass == kMatDiffuse ? kCtrWorkShadeDiffuse
+                               : kClass == kMatMetal ? kCtrWorkShadeMetal : kCtrWorkShade;
+    for_slots(
+        fs, nfs, [&](const FrameSlot &f) { return pass_counter(f, pass, kQueue); },
+        [&](const FrameSlot &f) { return pass_drained(f, pass, kWork, kQueue); },
+        [&](const FrameSlot &f) { shade_body<kVolume, kGraph, kClass>(f.sc, f.prm, f.ps, pass); });
+}
+
+
+
 // ------------------------------------------------------------ shadow
 // NEE contribution of a path's shadow rays: per light kind, the sum of
 // mis * tr * f * Li over the rays that reached the light, divided by the
@@ -1096,7 +1216,7 @@ __device__ __forceinline__ void shadow_finish(const PtParams &prm, const PathSta
 // paths with an unoccluded ray go on to k_transmit for the transmittance along
 // it; the others are finished here.
 template <bool kMedia, bool kSphere, int kStack>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_shadow(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
+__device__ __forceinline__ void shadow_body(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass)
 {
     __shared__ uint32_t stack[kStackWindow * kBlock];
     __shared__ float pre_park[kPreSlots * kBlock];
@@ -1151,7 +1271,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 start_ray();
             } else {
                 r = kLaneIdle;
-                const bool walk = kMedia && (fl & kShClearMask);
+                const bool walk = kMedia && prm.media && (fl & kShClearMask);
                 if (walk) {
                     ps.sh_flags[p] = fl;
                     ps.flag[p] = (pass << kFlagBits) | kFlagTransmit;
@@ -1167,13 +1287,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     wave_add64(prm.stats + kStatShadowTris, tc.tris);
 }
 
+
+template <bool kMedia, bool kSphere, int kStack>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_shadow_m(const FrameSlot *__restrict__ fs, uint32_t nfs, uint32_t pass)
+{
+    for_slots(
+        fs, nfs, [&](const FrameSlot &f) { return pass_counter(f, pass, kCtrShadow); },
+        [&](const FrameSlot &f) { return pass_drained(f, pass, kCtrWorkShadow, kCtrShadow); },
+        [&](const FrameSlot &f) { shadow_body<kMedia, kSphere, kStack>(f.sc, f.prm, f.ps, pass); });
+}
+
+
+
 // Transmittance of the unoccluded shadow rays of paths in media: TraceTr
 // (src/VolumeEval.cuh:420-468) walks the boundary instances along the ray,
 // updating the medium records at each crossing and multiplying the segment
 // transmittances (EvalTr, 253-418).
 // kHet as k_medium's.
 template <bool kSphere, int kStack, int kHet>
-__global__ __launch_bounds__(kBlock) void k_transmit(DevScene sc, PtParams prm, PathState ps, uint32_t pass)
+__device__ __forceinline__ void transmit_body(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass)
 {
     __shared__ float tex_stk[kHet == 2 ? kTexStack * 3 * kBlock : 1];
     __shared__ uint32_t stack[kStackWindow * kBlock];
@@ -1279,9 +1411,21 @@ __global__ __launch_bounds__(kBlock) void k_transmit(DevScene sc, PtParams prm, 
     wave_add64(prm.stats + kStatTransmitTris, tc.tris);
 }
 
+
+template <bool kSphere, int kStack, int kHet>
+__global__ __launch_bounds__(kBlock) void k_transmit_m(const FrameSlot *__restrict__ fs, uint32_t nfs, uint32_t pass)
+{
+    for_slots(
+        fs, nfs, [&](const FrameSlot &f) { return pass_counter(f, pass, kCtrTransmit); },
+        [&](const FrameSlot &f) { return pass_drained(f, pass, kCtrWorkTransmit, kCtrTransmit); },
+        [&](const FrameSlot &f) { transmit_body<kSphere, kStack, kHet>(f.sc, f.prm, f.ps, pass); });
+}
+
+
+
 // ------------------------------------------------------------ flush
 // Paths still queued after the last pass end with the radiance they carry.
-__global__ __launch_bounds__(kBlock) void k_flush(PtParams prm, PathState ps, uint32_t pass)
+__device__ __forceinline__ void flush_body(const PtParams &prm, const PathState &ps, uint32_t pass)
 {
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     const uint32_t n = cnt[kCtrActive];
@@ -1292,8 +1436,18 @@ __global__ __launch_bounds__(kBlock) void k_flush(PtParams prm, PathState ps, ui
     }
 }
 
+
+// one row of the grid (blockIdx.y) per slot
+__global__ __launch_bounds__(kBlock) void k_flush_m(const FrameSlot *__restrict__ fs, uint32_t nfs, uint32_t pass)
+{
+    const FrameSlot &f = fs[blockIdx.y];
+    flush_body(f.prm, f.ps, pass);
+}
+
+
+
 // ------------------------------------------------------------ resolve
-__global__ void k_resolve(PtParams prm, PathState ps, float4 *noise)
+__device__ __forceinline__ void resolve_body(const PtParams &prm, const PathState &ps, float4 *noise)
 {
     const uint32_t spp = (uint32_t)prm.rs.spp;
     const uint32_t pix_in_chunk = prm.chunk_paths / spp;
@@ -1319,6 +1473,16 @@ __global__ void k_resolve(PtParams prm, PathState ps, float4 *noise)
     // src/Shading.cu:1545-1547, keeps whichever sample wrote last)
     if (prm.aov) prm.aov[(size_t)py * prm.rs.width + px] = ps.first_hit[(size_t)k * spp];
 }
+
+
+// one row of the grid (blockIdx.y) per slot, each into its frame's image
+__global__ void k_resolve_m(const FrameSlot *__restrict__ fs)
+{
+    const FrameSlot &f = fs[blockIdx.y];
+    resolve_body(f.prm, f.ps, f.prm.noise);
+}
+
+
 
 // ------------------------------------------------------------ post
 __global__ void k_accumulate(const float4 *noise, float4 *accum, uint32_t n, int frame)
@@ -1352,158 +1516,166 @@ __global__ void k_tonemap(float4 *img, uint32_t n, float gamma, float exposure)
 
 // ------------------------------------------------------------ launchers
 // Persistent pass kernels: the caller's persistent_waves, or by default one
-// wave per 256 paths of the chunk, within [1024, kMaxPersistentWaves] (the
+// wave per 256 paths of the launch (all slots), within [1024, kMaxPersistentWaves] (the
 // traversal spill area holds kSpillThreads threads): large chunks fill 256
 // CUs several times over, and small frames launch fewer waves, which drain
 // sooner (256x256 at 4 spp: 7.8 ms with 1024 waves against 9.9 ms with one
 // path per lane); never more waves than the items need.  Every pass kernel
 // fetches work dynamically or strides over the grid, so the count changes
 // timing only.
-static dim3 persistent_grid(const PtParams &prm, const PathState &ps)
+uint32_t persistent_waves(uint64_t paths, uint32_t requested, uint32_t spill_threads)
 {
-    const uint32_t waves = (prm.chunk_paths + kBlock - 1) / kBlock;
-    const uint32_t w = prm.rs.persistent_waves;
     static_assert(kMaxPersistentWaves * kBlock == kSpillThreads, "one spill row per lane");
-    const uint32_t most = ps.spill_threads / kBlock;  // one spill row per lane
-    uint32_t cap = (prm.chunk_paths + 255u) / 256u;
+    const uint64_t waves = (paths + kBlock - 1) / kBlock;
+    const uint32_t most = spill_threads / kBlock;  // one spill row per lane
+    uint64_t cap = (paths + 255u) / 256u;
     cap = cap < 1024u ? 1024u : cap;
-    if (w) cap = w;
+    if (requested) cap = requested;
     cap = cap < most ? cap : most;
-    return dim3(waves < cap ? (waves ? waves : 1) : cap);
+    return (uint32_t)(waves < cap ? (waves ? waves : 1) : cap);
 }
 
-int launch_raygen(const DevScene &sc, const PtParams &prm, const PathState &ps, hipStream_t s)
+// ------------------------------------------------------------ multi-frame launchers
+// One launch per stage for every slot of a round (ymrt_render_batch).  The
+// variant is the union of the slots' scene features (each feature's code is
+// inert where a scene lacks it); waves: the persistent grid shared by all
+// slots; blocks: the most compaction / resolve blocks any slot needs.
+__global__ __launch_bounds__(kBlock) void k_active_m(const FrameSlot *__restrict__ fs, uint32_t nfs, uint32_t pass,
+                                                     uint32_t *out)
 {
-    hipLaunchKernelGGL(k_raygen, persistent_grid(prm, ps), dim3(kBlock), 0, s, sc, prm, ps);
+    uint32_t v = 0;
+    for (uint32_t s = threadIdx.x; s < nfs; s += kBlock) v += fs[s].prm.counters[pass * kCtrStride + kCtrActive];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (threadIdx.x == 0) *out = v;
+}
+
+int launch_multi_active(const FrameSlot *fs, uint32_t nfs, uint32_t pass, uint32_t *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_active_m, dim3(1), dim3(kBlock), 0, s, fs, nfs, pass, out);
+    YM_HIP(hipGetLastError());
+    return YMRT_OK;
+}
+
+int launch_multi_raygen(const FrameSlot *fs, uint32_t nfs, uint32_t waves, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_raygen_m, dim3(waves), dim3(kBlock), 0, s, fs, nfs);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }
 
 template <bool kNearby, bool kMedia, bool kSphere>
-static int launch_extend_t(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
+static void extend_m(const FrameSlot *fs, uint32_t nfs, uint32_t waves, uint32_t pass, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_extend<kNearby, kMedia, kSphere, kMaxStackDepth>), persistent_grid(prm, ps),
-                       dim3(kBlock), 0, s, sc, prm, ps, pass);
+    hipLaunchKernelGGL((k_extend_m<kNearby, kMedia, kSphere, kMaxStackDepth>), dim3(waves), dim3(kBlock), 0, s, fs,
+                       nfs, pass);
+}
+
+int launch_multi_extend(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v, uint32_t waves, uint32_t pass,
+                        hipStream_t s)
+{
+    const int k = (v.nearby ? 4 : 0) | (v.media ? 2 : 0) | (v.sphere ? 1 : 0);
+    switch (k) {
+    case 0: extend_m<false, false, false>(fs, nfs, waves, pass, s); break;
+    case 1: extend_m<false, false, true>(fs, nfs, waves, pass, s); break;
+    case 2: extend_m<false, true, false>(fs, nfs, waves, pass, s); break;
+    case 3: extend_m<false, true, true>(fs, nfs, waves, pass, s); break;
+    case 4: extend_m<true, false, false>(fs, nfs, waves, pass, s); break;
+    case 5: extend_m<true, false, true>(fs, nfs, waves, pass, s); break;
+    case 6: extend_m<true, true, false>(fs, nfs, waves, pass, s); break;
+    default: extend_m<true, true, true>(fs, nfs, waves, pass, s); break;
+    }
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }
 
-template <bool kMedia, bool kSphere>
-static int launch_shadow_t(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
+int launch_multi_medium(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v, uint32_t waves, uint32_t pass,
+                        hipStream_t s)
 {
-    hipLaunchKernelGGL((k_shadow<kMedia, kSphere, kMaxStackDepth>), persistent_grid(prm, ps), dim3(kBlock),
-                       0, s, sc, prm, ps, pass);
+    if (v.het == 2) hipLaunchKernelGGL(k_medium_m<2>, dim3(waves), dim3(kBlock), 0, s, fs, nfs, pass);
+    else if (v.het == 1) hipLaunchKernelGGL(k_medium_m<1>, dim3(waves), dim3(kBlock), 0, s, fs, nfs, pass);
+    else hipLaunchKernelGGL(k_medium_m<0>, dim3(waves), dim3(kBlock), 0, s, fs, nfs, pass);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }
 
-template <bool kNearby, bool kMedia>
-static int launch_extend_m(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
+int launch_multi_compact(const FrameSlot *fs, uint32_t nfs, uint32_t blocks, uint32_t pass, int mode, hipStream_t s)
 {
-    return sc.has_spheres ? launch_extend_t<kNearby, kMedia, true>(sc, prm, ps, pass, s)
-                          : launch_extend_t<kNearby, kMedia, false>(sc, prm, ps, pass, s);
-}
-
-// The window stack spills to a kMaxStackDepth-deep area, so one depth bound
-// serves every scene the host accepted.
-int launch_extend(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool nearby,
-                  bool media, uint32_t, hipStream_t s)
-{
-    if (nearby)
-        return media ? launch_extend_m<true, true>(sc, prm, ps, pass, s)
-                     : launch_extend_m<true, false>(sc, prm, ps, pass, s);
-    return media ? launch_extend_m<false, true>(sc, prm, ps, pass, s)
-                 : launch_extend_m<false, false>(sc, prm, ps, pass, s);
-}
-
-// 0 homogeneous media only, 1 voxel grids, 2 texture densities too
-static int het_mode(const DevScene &sc) { return sc.has_tex_media ? 2 : sc.grid_count > 0 ? 1 : 0; }
-
-int launch_medium(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
-{
-    const int het = het_mode(sc);
-    if (het == 2)
-        hipLaunchKernelGGL(k_medium<2>, persistent_grid(prm, ps), dim3(kBlock), 0, s, sc, prm, ps, pass);
-    else if (het == 1)
-        hipLaunchKernelGGL(k_medium<1>, persistent_grid(prm, ps), dim3(kBlock), 0, s, sc, prm, ps, pass);
-    else
-        hipLaunchKernelGGL(k_medium<0>, persistent_grid(prm, ps), dim3(kBlock), 0, s, sc, prm, ps, pass);
-    YM_HIP(hipGetLastError());
-    return YMRT_OK;
-}
-
-int launch_compact(const PtParams &prm, const PathState &ps, uint32_t pass, int mode, hipStream_t s)
-{
-    const dim3 g((prm.chunk_paths + kCompactSpan - 1) / kCompactSpan), b(kCompactThreads);
+    const dim3 g(blocks ? blocks : 1, nfs), b(kCompactThreads);
     if (mode == kCompactShade) {
-        hipLaunchKernelGGL(k_compact_count<kCompactShade>, g, b, 0, s, prm, ps, pass);
-        hipLaunchKernelGGL(k_compact_scatter<kCompactShade>, g, b, 0, s, prm, ps, pass);
+        hipLaunchKernelGGL(k_compact_count_m<kCompactShade>, g, b, 0, s, fs, pass);
+        hipLaunchKernelGGL(k_compact_scatter_m<kCompactShade>, g, b, 0, s, fs, pass);
     } else if (mode == kCompactMedium) {
-        hipLaunchKernelGGL(k_compact_count<kCompactMedium>, g, b, 0, s, prm, ps, pass);
-        hipLaunchKernelGGL(k_compact_scatter<kCompactMedium>, g, b, 0, s, prm, ps, pass);
+        hipLaunchKernelGGL(k_compact_count_m<kCompactMedium>, g, b, 0, s, fs, pass);
+        hipLaunchKernelGGL(k_compact_scatter_m<kCompactMedium>, g, b, 0, s, fs, pass);
     } else {
-        hipLaunchKernelGGL(k_compact_count<kCompactTransmit>, g, b, 0, s, prm, ps, pass);
-        hipLaunchKernelGGL(k_compact_scatter<kCompactTransmit>, g, b, 0, s, prm, ps, pass);
+        hipLaunchKernelGGL(k_compact_count_m<kCompactTransmit>, g, b, 0, s, fs, pass);
+        hipLaunchKernelGGL(k_compact_scatter_m<kCompactTransmit>, g, b, 0, s, fs, pass);
     }
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }
 
-int launch_shade(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool volume,
-                 hipStream_t s)
+template <bool kGraph>
+static void shade_m(const FrameSlot *fs, uint32_t nfs, uint32_t waves, uint32_t pass, hipStream_t s)
 {
-    const dim3 g = persistent_grid(prm, ps), b(kBlock);
-    if (volume)
-        hipLaunchKernelGGL((k_shade<true, false>), g, b, 0, s, sc, prm, ps, pass);
-    else if (sc.has_graph) {
-        hipLaunchKernelGGL((k_shade<false, true>), g, b, 0, s, sc, prm, ps, pass);
-        hipLaunchKernelGGL((k_shade<false, true, kMatDiffuse>), g, b, 0, s, sc, prm, ps, pass);
-        hipLaunchKernelGGL((k_shade<false, true, kMatMetal>), g, b, 0, s, sc, prm, ps, pass);
+    const dim3 g(waves), b(kBlock);
+    hipLaunchKernelGGL((k_shade_m<false, kGraph>), g, b, 0, s, fs, nfs, pass);
+    hipLaunchKernelGGL((k_shade_m<false, kGraph, kMatDiffuse>), g, b, 0, s, fs, nfs, pass);
+    hipLaunchKernelGGL((k_shade_m<false, kGraph, kMatMetal>), g, b, 0, s, fs, nfs, pass);
+}
+
+int launch_multi_shade(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v, bool volume, uint32_t waves,
+                       uint32_t pass, hipStream_t s)
+{
+    if (volume) hipLaunchKernelGGL((k_shade_m<true, false>), dim3(waves), dim3(kBlock), 0, s, fs, nfs, pass);
+    else if (v.graph) shade_m<true>(fs, nfs, waves, pass, s);
+    else shade_m<false>(fs, nfs, waves, pass, s);
+    YM_HIP(hipGetLastError());
+    return YMRT_OK;
+}
+
+int launch_multi_shadow(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v, uint32_t waves, uint32_t pass,
+                        hipStream_t s)
+{
+    const dim3 g(waves), b(kBlock);
+    if (v.sphere) {
+        if (v.media) hipLaunchKernelGGL((k_shadow_m<true, true, kMaxStackDepth>), g, b, 0, s, fs, nfs, pass);
+        else hipLaunchKernelGGL((k_shadow_m<false, true, kMaxStackDepth>), g, b, 0, s, fs, nfs, pass);
     } else {
-        hipLaunchKernelGGL((k_shade<false, false>), g, b, 0, s, sc, prm, ps, pass);
-        hipLaunchKernelGGL((k_shade<false, false, kMatDiffuse>), g, b, 0, s, sc, prm, ps, pass);
-        hipLaunchKernelGGL((k_shade<false, false, kMatMetal>), g, b, 0, s, sc, prm, ps, pass);
+        if (v.media) hipLaunchKernelGGL((k_shadow_m<true, false, kMaxStackDepth>), g, b, 0, s, fs, nfs, pass);
+        else hipLaunchKernelGGL((k_shadow_m<false, false, kMaxStackDepth>), g, b, 0, s, fs, nfs, pass);
     }
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }
 
-int launch_shadow(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool media,
-                  uint32_t, hipStream_t s)
+int launch_multi_transmit(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v, uint32_t waves, uint32_t pass,
+                          hipStream_t s)
 {
-    if (sc.has_spheres)
-        return media ? launch_shadow_t<true, true>(sc, prm, ps, pass, s) : launch_shadow_t<false, true>(sc, prm, ps, pass, s);
-    return media ? launch_shadow_t<true, false>(sc, prm, ps, pass, s) : launch_shadow_t<false, false>(sc, prm, ps, pass, s);
-}
-
-int launch_transmit(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
-{
-    const dim3 g = persistent_grid(prm, ps), b(kBlock);
-    const int het = het_mode(sc);
-    if (sc.has_spheres) {
-        if (het == 2) hipLaunchKernelGGL((k_transmit<true, kMaxStackDepth, 2>), g, b, 0, s, sc, prm, ps, pass);
-        else if (het == 1) hipLaunchKernelGGL((k_transmit<true, kMaxStackDepth, 1>), g, b, 0, s, sc, prm, ps, pass);
-        else hipLaunchKernelGGL((k_transmit<true, kMaxStackDepth, 0>), g, b, 0, s, sc, prm, ps, pass);
+    const dim3 g(waves), b(kBlock);
+    if (v.sphere) {
+        if (v.het == 2) hipLaunchKernelGGL((k_transmit_m<true, kMaxStackDepth, 2>), g, b, 0, s, fs, nfs, pass);
+        else if (v.het == 1) hipLaunchKernelGGL((k_transmit_m<true, kMaxStackDepth, 1>), g, b, 0, s, fs, nfs, pass);
+        else hipLaunchKernelGGL((k_transmit_m<true, kMaxStackDepth, 0>), g, b, 0, s, fs, nfs, pass);
     } else {
-        if (het == 2) hipLaunchKernelGGL((k_transmit<false, kMaxStackDepth, 2>), g, b, 0, s, sc, prm, ps, pass);
-        else if (het == 1) hipLaunchKernelGGL((k_transmit<false, kMaxStackDepth, 1>), g, b, 0, s, sc, prm, ps, pass);
-        else hipLaunchKernelGGL((k_transmit<false, kMaxStackDepth, 0>), g, b, 0, s, sc, prm, ps, pass);
+        if (v.het == 2) hipLaunchKernelGGL((k_transmit_m<false, kMaxStackDepth, 2>), g, b, 0, s, fs, nfs, pass);
+        else if (v.het == 1) hipLaunchKernelGGL((k_transmit_m<false, kMaxStackDepth, 1>), g, b, 0, s, fs, nfs, pass);
+        else hipLaunchKernelGGL((k_transmit_m<false, kMaxStackDepth, 0>), g, b, 0, s, fs, nfs, pass);
     }
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }
 
-int launch_flush(const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s)
+int launch_multi_flush(const FrameSlot *fs, uint32_t nfs, uint32_t pass, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_flush, dim3(1024), dim3(kBlock), 0, s, prm, ps, pass);
+    hipLaunchKernelGGL(k_flush_m, dim3(64, nfs), dim3(kBlock), 0, s, fs, nfs, pass);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }
 
-int launch_resolve(const PtParams &prm, const PathState &ps, float *noise, hipStream_t s)
+int launch_multi_resolve(const FrameSlot *fs, uint32_t nfs, uint32_t pixel_blocks, hipStream_t s)
 {
-    const uint32_t pix = prm.chunk_paths / (uint32_t)prm.rs.spp;
-    hipLaunchKernelGGL(k_resolve, dim3((pix + 255) / 256), dim3(256), 0, s, prm, ps, reinterpret_cast<float4 *>(noise));
+    hipLaunchKernelGGL(k_resolve_m, dim3(pixel_blocks ? pixel_blocks : 1, nfs), dim3(256), 0, s, fs);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }

@@ -116,6 +116,26 @@ struct PtParams {
     const uint32_t *adj_mat; // adjoint: 1 per Lambert-only material, or null (no adjoint)
     const float *adj_img;    // adjoint: dLoss/dImage, width*height*3
     float *adj_grad;         // adjoint: gradient, material_count*3 (accumulated)
+    uint32_t ccount_stride;  // entries per queue in ccount
+    uint32_t media;          // volume scattering on and the scene has volumes
+    float4 *noise;           // the frame's noise image (multi-frame resolve)
+};
+
+// Scene features a multi-frame launch compiles in: the union over its slots.
+struct MultiVariant {
+    bool nearby, media, sphere, graph;
+    int het;  // 0 homogeneous media, 1 voxel grids, 2 texture densities
+};
+
+// Most frame chunks one multi-frame launch serves.
+constexpr uint32_t kMaxSlots = 64;
+
+// One frame chunk of a multi-frame launch: scene, parameters and the slot's
+// part of the path state (device-resident table, read with scalar loads).
+struct FrameSlot {
+    DevScene sc;
+    PtParams prm;
+    PathState ps;
 };
 
 }  // namespace ym

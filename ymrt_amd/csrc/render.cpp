@@ -14,24 +14,28 @@
 #include <mutex>
 #include <type_traits>
 
+#include "buffers.hpp"
 #include "internal.hpp"
 #include "pt.hpp"
 
 namespace ym {
 
-int launch_raygen(const DevScene &sc, const PtParams &prm, const PathState &ps, hipStream_t s);
-int launch_extend(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool nearby,
-                  bool media, uint32_t stack, hipStream_t s);
-int launch_medium(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s);
-// mode: 0 free flight, 1 shading, 2 transmittance (kCompact* in kernels_pt.hip)
-int launch_compact(const PtParams &prm, const PathState &ps, uint32_t pass, int mode, hipStream_t s);
-int launch_shade(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool volume,
-                 hipStream_t s);
-int launch_shadow(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, bool media,
-                  uint32_t stack, hipStream_t s);
-int launch_transmit(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s);
-int launch_flush(const PtParams &prm, const PathState &ps, uint32_t pass, hipStream_t s);
-int launch_resolve(const PtParams &prm, const PathState &ps, float *noise, hipStream_t s);
+uint32_t persistent_waves(uint64_t paths, uint32_t requested, uint32_t spill_threads);
+int launch_multi_raygen(const FrameSlot *fs, uint32_t nfs, uint32_t waves, hipStream_t s);
+int launch_multi_extend(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v, uint32_t waves, uint32_t pass,
+                        hipStream_t s);
+int launch_multi_medium(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v, uint32_t waves, uint32_t pass,
+                        hipStream_t s);
+// mode: 0 free flight, 1 shading, 2 transmittance
+int launch_multi_compact(const FrameSlot *fs, uint32_t nfs, uint32_t blocks, uint32_t pass, int mode, hipStream_t s);
+int launch_multi_shade(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v, bool volume, uint32_t waves,
+                       uint32_t pass, hipStream_t s);
+int launch_multi_shadow(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v, uint32_t waves, uint32_t pass,
+                        hipStream_t s);
+int launch_multi_transmit(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v, uint32_t waves, uint32_t pass,
+                          hipStream_t s);
+int launch_multi_flush(const FrameSlot *fs, uint32_t nfs, uint32_t pass, hipStream_t s);
+int launch_multi_resolve(const FrameSlot *fs, uint32_t nfs, uint32_t pixel_blocks, hipStream_t s);
 int launch_adjoint(const DevScene &sc, const PtParams &prm, const PathState &ps, hipStream_t s);
 
 // Paths in flight per chunk when the settings leave it to the library.  Each
@@ -44,23 +48,6 @@ int launch_adjoint(const DevScene &sc, const PtParams &prm, const PathState &ps,
 constexpr uint32_t kDefaultCapacity = 1u << 26;
 constexpr uint64_t kPathBytes = 720;
 
-struct RenderBuffers {
-    uint32_t cap = 0;
-    uint32_t adj_depth = 0;  // vertex records allocated per path for the adjoint
-    PathState ps{};
-    std::vector<void *> allocs;
-    uint32_t *counters = nullptr;
-    unsigned long long *stats = nullptr;
-    std::vector<hipEvent_t> events;
-    size_t event_next = 0;
-
-    ~RenderBuffers()
-    {
-        for (void *p : allocs) (void)hipFree(p);
-        for (hipEvent_t e : events) (void)hipEventDestroy(e);
-    }
-};
-
 void destroy_render_buffers(ymrt_scene *sc)
 {
     delete sc->render;
@@ -72,16 +59,6 @@ void free_render_buffers(RenderBuffers *rb) { delete rb; }
 // Capacity choices read the free device memory and then allocate; threads
 // of a batch would otherwise all see the same free memory.
 static std::mutex g_alloc_mu;
-
-template <typename T>
-static int alloc(RenderBuffers *rb, T **p, size_t count)
-{
-    void *m = nullptr;
-    YM_HIP(hipMalloc(&m, sizeof(T) * std::max<size_t>(count, 1)));
-    rb->allocs.push_back(m);
-    *p = static_cast<T *>(m);
-    return YMRT_OK;
-}
 
 static int run_frame(ymrt_scene *sc, RenderBuffers *rb, uint32_t chunk_paths_max, const ymrt_render_settings *rs,
                      int frame, float *noise, uint32_t *instance_ids, const float *adj_img, float *adj_grad,
@@ -97,52 +74,6 @@ static int check_settings(const ymrt_render_settings *rs)
     return YMRT_OK;
 }
 
-// Lays out the path state of n paths (and the pass counters and statistics)
-// through get(pointer, count), which allocates or carves each array.
-template <typename Get>
-static int layout_buffers(PathState &ps, uint32_t *&counters, unsigned long long *&stats, size_t n,
-                          uint32_t spill_threads, Get &&get)
-{
-    int rc = 0;
-    rc |= get(ps.ray_o, n);
-    rc |= get(ps.ray_d, n);
-    rc |= get(ps.last, n);
-    rc |= get(ps.beta, n);
-    rc |= get(ps.L, n);
-    rc |= get(ps.smp, n);
-    rc |= get(ps.info, n);
-    rc |= get(ps.xhdr, n);
-    rc |= get(ps.xrec, n * kMaxBoundaryRecord);
-    rc |= get(ps.hit_a, n);
-    rc |= get(ps.hit_b, n);
-    rc |= get(ps.nb, n * 4);
-    rc |= get(ps.sh_o, n * kShadowSlots);
-    rc |= get(ps.sh_d, n * kShadowSlots);
-    rc |= get(ps.sh_f, n * kShadowSlots);
-    rc |= get(ps.sh_li, n * kShadowSlots);
-    rc |= get(ps.sh_tr, n * kShadowSlots);
-    rc |= get(ps.sh_c, n);
-    rc |= get(ps.sh_flags, n);
-    rc |= get(ps.out, n);
-    rc |= get(ps.queue[0], n);
-    rc |= get(ps.queue[1], n);
-    rc |= get(ps.q_hit, n);
-    rc |= get(ps.q_hit_vol, n);
-    rc |= get(ps.q_hit_diffuse, n);
-    rc |= get(ps.q_hit_metal, n);
-    rc |= get(ps.q_medium, n);
-    rc |= get(ps.flag, n);
-    rc |= get(ps.ccount, 4 * (n / kCompactSpan + 1));
-    rc |= get(ps.q_shadow, n);
-    rc |= get(ps.q_transmit, n);
-    rc |= get(ps.spill, (size_t)(kMaxStackDepth - kStackWindow) * spill_threads);
-    rc |= get(ps.first_hit, n);
-    rc |= get(counters, (size_t)(kMaxPasses + 2) * kCtrStride);
-    rc |= get(stats, (size_t)kStatCount);
-    ps.spill_threads = spill_threads;
-    return rc;
-}
-
 static int ensure_buffers(RenderBuffers *&slot, uint32_t cap)
 {
     if (slot && slot->cap >= cap) return YMRT_OK;
@@ -150,65 +81,14 @@ static int ensure_buffers(RenderBuffers *&slot, uint32_t cap)
     RenderBuffers *rb = new RenderBuffers;
     slot = rb;
     rb->cap = cap;
-    const int rc = layout_buffers(rb->ps, rb->counters, rb->stats, cap, kSpillThreads,
-                                  [&](auto *&p, size_t count) { return alloc(rb, &p, count); });
+    const int rc =
+        layout_buffers(*rb, cap, kSpillThreads, [&](auto *&p, size_t count) { return alloc(rb, &p, count); });
     if (rc) {
         delete slot;
         slot = nullptr;
         return rc;
     }
     return YMRT_OK;
-}
-
-// Arrays carved out of fixed device memory start on 256-B boundaries.
-static size_t carve_round(size_t b) { return (b + 255) & ~(size_t)255; }
-
-size_t fixed_buffer_bytes(uint32_t cap, uint32_t spill_threads)
-{
-    PathState ps{};
-    uint32_t *counters = nullptr;
-    unsigned long long *stats = nullptr;
-    size_t total = 0;
-    layout_buffers(ps, counters, stats, cap, spill_threads, [&](auto *&p, size_t count) {
-        total += carve_round(sizeof(*p) * std::max<size_t>(count, 1));
-        p = nullptr;
-        return 0;
-    });
-    return total;
-}
-
-// One frame whose path state is carved out of `memory` (device, `bytes`
-// long) instead of allocated: chunks of up to cap paths, persistent kernels
-// of up to spill_threads / 64 waves.  Allocates no device memory.
-int render_frame_fixed(ymrt_scene *sc, void *memory, size_t bytes, uint32_t cap, uint32_t spill_threads,
-                       const ymrt_render_settings *rs, int frame, float *noise, hipStream_t stream,
-                       ymrt_frame_stats *stats)
-{
-    if (int r = check_settings(rs)) return r;
-    YM_REQUIRE(memory != nullptr, "no fixed device memory");
-    YM_REQUIRE(fixed_buffer_bytes(cap, spill_threads) <= bytes, "fixed device memory too small for the capacity");
-    YM_REQUIRE(rs->path_capacity <= cap,
-               "path_capacity exceeds the fixed capacity of " + std::to_string(cap) + " paths");
-    const uint32_t spp = (uint32_t)rs->spp;
-    YM_REQUIRE(spp <= cap, "spp exceeds the fixed capacity of " + std::to_string(cap) + " paths");
-    RenderBuffers view;  // owns nothing: no allocations, no events
-    view.cap = cap;
-    char *base = static_cast<char *>(memory);
-    size_t off = 0;
-    layout_buffers(view.ps, view.counters, view.stats, cap, spill_threads, [&](auto *&p, size_t count) {
-        p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(base + off);
-        off += carve_round(sizeof(*p) * std::max<size_t>(count, 1));
-        return 0;
-    });
-    const uint64_t total_paths = (uint64_t)((rs->width + 15) / 16) * ((rs->height + 15) / 16) * 256ull * spp;
-    const uint64_t want = rs->path_capacity ? rs->path_capacity : cap;
-    const uint64_t per_cap = (want / spp) * spp;
-    const uint64_t nchunks = (total_paths + per_cap - 1) / per_cap;
-    const uint32_t chunk_paths_max =
-        (uint32_t)std::min<uint64_t>(per_cap, (total_paths / nchunks + spp - 1) / spp * spp);
-    YM_HIP(hipSetDevice(sc->ctx->device));
-    if (int r = clear_stack_flag(sc, stream)) return r;
-    return run_frame(sc, &view, chunk_paths_max, rs, frame, noise, nullptr, nullptr, nullptr, stream, 0, stats);
 }
 
 static hipEvent_t next_event(RenderBuffers *rb)
@@ -293,6 +173,7 @@ static int run_frame(ymrt_scene *sc, RenderBuffers *rb, uint32_t chunk_paths_max
     prm.tiles_x = (rs->width + 15) / 16;
     const uint32_t tiles_y = (rs->height + 15) / 16;
     prm.cap = rb->cap;
+    prm.ccount_stride = rb->cap / kCompactSpan + 1;
     prm.counters = rb->counters;
     prm.stats = rb->stats;
     prm.error = sc->d_error;
@@ -311,8 +192,19 @@ static int run_frame(ymrt_scene *sc, RenderBuffers *rb, uint32_t chunk_paths_max
     // change: boundaries, dielectrics, priorities.
     const bool nearby = sc->needs_nearby;
     const bool media = rs->enable_volume_scattering && sc->dev.volume_count > 0;
+    prm.media = media ? 1u : 0u;
     const uint32_t stack = sc->max_stack;
     YM_REQUIRE(stack <= (uint32_t)kMaxStackDepth, "scene BVH too deep for the path-tracing traversal stack");
+    prm.noise = reinterpret_cast<float4 *>(noise);
+    // the frame runs the multi-frame kernels with one slot: the chunk's scene,
+    // parameters and path state in a device table of one entry
+    const MultiVariant v{nearby, media, sc->dev.has_spheres != 0, sc->dev.has_graph != 0,
+                         media ? (sc->dev.has_tex_media ? 2 : sc->dev.grid_count > 0 ? 1 : 0) : 0};
+    FrameSlot slot;
+    std::memset(&slot, 0, sizeof(slot));
+    slot.sc = sc->dev;
+    slot.ps = rb->ps;
+    const FrameSlot *fs = rb->slots;
 
     YM_HIP(hipMemsetAsync(rb->stats, 0, sizeof(unsigned long long) * kStatCount, stream));
     rb->event_next = 0;
@@ -346,29 +238,31 @@ static int run_frame(ymrt_scene *sc, RenderBuffers *rb, uint32_t chunk_paths_max
         prm.chunk_base = base;
         prm.chunk_paths = (uint32_t)std::min<uint64_t>(chunk_paths_max, total_paths - base);
         ++chunks;
-        YM_HIP(hipMemsetAsync(rb->counters, 0, sizeof(uint32_t) * (kMaxPasses + 2) * kCtrStride, stream));
-        if ((rc = timed(0, [&] { return launch_raygen(sc->dev, prm, rb->ps, stream); }))) return rc;
+        // the previous chunk's kernels read the device slot until the stream drains
+        if (chunks > 1) YM_HIP(hipStreamSynchronize(stream));
+        slot.prm = prm;
+        YM_HIP(hipMemcpyAsync(rb->slots, &slot, sizeof(slot), hipMemcpyHostToDevice, stream));
+        YM_HIP(hipMemsetAsync(rb->counters, 0, sizeof(uint32_t) * kCountersPerSlot, stream));
+        const uint32_t waves = persistent_waves(prm.chunk_paths, rs->persistent_waves, rb->ps.spill_threads);
+        const uint32_t blocks = (prm.chunk_paths + kCompactSpan - 1) / kCompactSpan;
+        if ((rc = timed(0, [&] { return launch_multi_raygen(fs, 1, waves, stream); }))) return rc;
         uint32_t pass = 0;
         uint32_t next_check = (uint32_t)rs->ray_depth + 1;
         for (;;) {
-            if ((rc = timed(1, [&] {
-                     return launch_extend(sc->dev, prm, rb->ps, pass, nearby, media, stack, stream);
-                 })))
-                return rc;
+            if ((rc = timed(1, [&] { return launch_multi_extend(fs, 1, v, waves, pass, stream); }))) return rc;
             if (media && (rc = timed(5, [&] {
-                              int r = launch_compact(prm, rb->ps, pass, 0, stream);
-                              return r ? r : launch_medium(sc->dev, prm, rb->ps, pass, stream);
+                              int r = launch_multi_compact(fs, 1, blocks, pass, 0, stream);
+                              return r ? r : launch_multi_medium(fs, 1, v, waves, pass, stream);
                           })))
                 return rc;
-            if ((rc = timed(2, [&] { return launch_compact(prm, rb->ps, pass, 1, stream); }))) return rc;
-            if ((rc = timed(2, [&] { return launch_shade(sc->dev, prm, rb->ps, pass, false, stream); }))) return rc;
-            if (media && (rc = timed(2, [&] { return launch_shade(sc->dev, prm, rb->ps, pass, true, stream); })))
+            if ((rc = timed(2, [&] { return launch_multi_compact(fs, 1, blocks, pass, 1, stream); }))) return rc;
+            if ((rc = timed(2, [&] { return launch_multi_shade(fs, 1, v, false, waves, pass, stream); }))) return rc;
+            if (media && (rc = timed(2, [&] { return launch_multi_shade(fs, 1, v, true, waves, pass, stream); })))
                 return rc;
-            if ((rc = timed(3, [&] { return launch_shadow(sc->dev, prm, rb->ps, pass, media, stack, stream); })))
-                return rc;
+            if ((rc = timed(3, [&] { return launch_multi_shadow(fs, 1, v, waves, pass, stream); }))) return rc;
             if (media && (rc = timed(6, [&] {
-                              int r = launch_compact(prm, rb->ps, pass, 2, stream);
-                              return r ? r : launch_transmit(sc->dev, prm, rb->ps, pass, stream);
+                              int r = launch_multi_compact(fs, 1, blocks, pass, 2, stream);
+                              return r ? r : launch_multi_transmit(fs, 1, v, waves, pass, stream);
                           })))
                 return rc;
             ++pass;
@@ -379,14 +273,15 @@ static int run_frame(ymrt_scene *sc, RenderBuffers *rb, uint32_t chunk_paths_max
                 YM_HIP(hipStreamSynchronize(stream));
                 if (active == 0) break;
                 if (pass >= kMaxPasses) {
-                    if ((rc = launch_flush(prm, rb->ps, pass, stream))) return rc;
+                    if ((rc = launch_multi_flush(fs, 1, pass, stream))) return rc;
                     break;
                 }
                 next_check = pass + 2;
             }
         }
         passes_total += pass;
-        if ((rc = timed(4, [&] { return launch_resolve(prm, rb->ps, noise, stream); }))) return rc;
+        const uint32_t pixel_blocks = (prm.chunk_paths / spp + 255) / 256;
+        if ((rc = timed(4, [&] { return launch_multi_resolve(fs, 1, pixel_blocks, stream); }))) return rc;
         if (prm.adj_mat && (rc = timed(7, [&] { return launch_adjoint(sc->dev, prm, rb->ps, stream); }))) return rc;
     }
     if (timing) YM_HIP(hipEventRecord(frame_b, stream));

@@ -489,6 +489,7 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
     // ---------------------------------------------------------- textures / grids
     std::vector<DevTexture> texs;
     std::vector<uint32_t> texels;
+    std::vector<float> texels_f32;
     YM_REQUIRE(d->texture_count == 0 || d->textures != nullptr, "textures is NULL");
     for (uint32_t i = 0; i < d->texture_count; ++i) {
         const ymrt_texture_desc &t = d->textures[i];
@@ -515,7 +516,7 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
             texs.push_back(dt);
             continue;
         }
-        YM_REQUIRE(t.width > 0 && t.height > 0 && t.texels, "empty texture");
+        YM_REQUIRE(t.width > 0 && t.height > 0 && (t.texels || t.texels_f32), "empty texture");
         YM_REQUIRE(t.channels >= 1 && t.channels <= 4, "texture channels must be 1..4");
         dt.width = t.width;
         dt.height = t.height;
@@ -526,8 +527,15 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
         dt.sv = t.scale_v;
         dt.ou = t.offset_u;
         dt.ov = t.offset_v;
-        dt.offset = texels.size();
         const size_t n = (size_t)t.width * t.height;
+        if (t.texels_f32) {
+            dt.f32 = 1;
+            dt.offset = texels_f32.size();
+            texels_f32.insert(texels_f32.end(), t.texels_f32, t.texels_f32 + n * t.channels);
+            texs.push_back(dt);
+            continue;
+        }
+        dt.offset = texels.size();
         for (size_t k = 0; k < n; ++k) {
             uint32_t w = 0;
             for (uint32_t c = 0; c < t.channels; ++c) w |= (uint32_t)t.texels[k * t.channels + c] << (8 * c);
@@ -732,8 +740,10 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
     sc->materials = h.materials;
     sc->texture_count = d->texture_count;
     if ((rc = upload(sc, texs, &ds.textures))) return rc;
+    sc->textures = texs;
     if ((rc = upload(sc, tex_prog, &ds.tex_prog))) return rc;
     if ((rc = upload(sc, texels, &ds.texels))) return rc;
+    if ((rc = upload(sc, texels_f32, &ds.texels_f32))) return rc;
     if ((rc = upload(sc, lut, &ds.tex_lut))) return rc;
     if ((rc = upload(sc, grids, &ds.grids))) return rc;
     if ((rc = upload(sc, voxels, &ds.voxels))) return rc;
@@ -803,6 +813,19 @@ int update_materials(ymrt_scene *sc, uint32_t first, uint32_t count, const ymrt_
     sc->materials.swap(next);
     sc->needs_nearby = needs_nearby(sc->has_boundaries, sc->materials);
     sc->dev.has_graph = uses_texture_graph(sc->procedural_textures, sc->materials) ? 1 : 0;
+    return YMRT_OK;
+}
+
+int update_texture(ymrt_scene *sc, uint32_t tex, const float *texels, hipStream_t s)
+{
+    YM_REQUIRE(texels != nullptr, "texels is NULL");
+    YM_REQUIRE(tex < sc->textures.size(), "texture index out of range");
+    const DevTexture &t = sc->textures[tex];
+    YM_REQUIRE(t.type == YMRT_TEXTURE_IMAGE && t.f32, "only float image textures take texel updates");
+    const size_t n = (size_t)t.width * t.height * t.channels;
+    YM_HIP(hipMemcpyAsync(const_cast<float *>(sc->dev.texels_f32) + t.offset, texels, sizeof(float) * n,
+                          hipMemcpyHostToDevice, s));
+    YM_HIP(hipStreamSynchronize(s));
     return YMRT_OK;
 }
 

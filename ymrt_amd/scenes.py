@@ -539,3 +539,50 @@ def interior(width=1920, height=1080, tex_size=2048, instances=40, halton_seed=0
         sa.add_instance(_add(sa, fb), fogmat, inner_volume=vol, boundary=True)
     sa.set_camera((0.0, 1.7, 4.6), (0.0, 1.4, 0.0), 60.0, width / height)
     return sa
+
+
+# --------------------------------------------------------- multi-scene batch
+def batch_scene(triangles, width, height, seed):
+    """One scene of the multi-scene workload: the Cornell box (walls, two
+    boxes, glass sphere, quad light; 1k triangles) whose floor carries a
+    displaced grid of about `triangles` triangles in all (log-uniform over
+    the batch), glossy or diffuse by seed."""
+    sa = cornell_box(width, height, halton_seed=seed)
+    extra = max(0, int(triangles) - sa.triangle_count)
+    if extra >= 2:
+        n = max(2, int(round(np.sqrt(extra / 2.0))) + 1)          # n x n vertices, 2 (n-1)^2 triangles
+        rng = np.random.RandomState(seed)
+        g = np.linspace(0.02, 0.98, n)
+        X, Z = np.meshgrid(g, g)
+        ph = rng.uniform(0.0, 6.28, 4)
+        Y = 0.004 + 0.02 * (0.5 + 0.25 * np.sin(9.0 * X + ph[0]) * np.sin(7.0 * Z + ph[1])
+                            + 0.25 * np.sin(23.0 * X + ph[2]) * np.cos(19.0 * Z + ph[3]))
+        pos = np.stack([X, Y, Z], axis=-1).reshape(-1, 3).astype(np.float32)
+        k = np.arange(n - 1)
+        I, J = np.meshgrid(k, k)
+        v00 = (J * n + I).reshape(-1)
+        v10, v01 = v00 + 1, v00 + n
+        v11 = v01 + 1
+        idx = np.concatenate([np.stack([v00, v01, v11], 1), np.stack([v00, v11, v10], 1)]).astype(np.uint32)
+        if rng.rand() < 0.5:
+            mat = abi.default_material(diffuse_albedo=tuple(rng.uniform(0.3, 0.9, 3)), specular_weight=0.0)
+        else:
+            r = float(rng.uniform(0.1, 0.6))
+            mat = abi.default_material(diffuse_albedo=tuple(rng.uniform(0.5, 0.95, 3)), roughness_x=r,
+                                       roughness_y=r, metalness=1.0)
+        sa.add_instance(sa.add_mesh(pos, idx), sa.add_material(mat))
+    return sa
+
+
+def batch_workload(count=64, seed=11):
+    """The multi-scene workload: `count` scenes with triangle counts
+    log-uniform in [1k, 500k], resolution uniform in {256^2, 512^2, 1024^2},
+    spp uniform in {4, 16, 64}, depth 6.  Returns [(tris, res, spp, depth)]."""
+    rng = np.random.RandomState(seed)
+    out = []
+    for _ in range(count):
+        tris = int(round(np.exp(rng.uniform(np.log(1e3), np.log(5e5)))))
+        res = int(rng.choice([256, 512, 1024]))
+        spp = int(rng.choice([4, 16, 64]))
+        out.append((tris, res, spp, 6))
+    return out
