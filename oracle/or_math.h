@@ -66,6 +66,24 @@ static inline v3 mix3(v3 x, v3 y, float a) { return add3(x, sclf3(a, sub3(y, x))
 static inline v3 mixv3(v3 x, v3 y, v3 a) { return add3(x, mul3(a, sub3(y, x))); }
 static inline float sqrf(float x) { return x * x; }
 
+/* Transcendentals of float arguments evaluated in double precision and
+ * rounded once to float: the correctly rounded result except where the double
+ * value lies within one of its own ulps of a float rounding boundary.  CUDA's
+ * float functions that the reference calls are specified only to a few ulps;
+ * the oracle fixes them to the correctly rounded value, which is also what
+ * the GPU library computes, so both sides take the same branches on the last
+ * bits of a free-flight distance, a transmittance or a sampled direction.
+ * Host-side setup (camera field of view, light cones, transforms) and the
+ * indirect clamp's powers keep glibc's float functions, as the library's host
+ * code and its clamp keep float functions. */
+static inline float or_expf(float x) { return (float)exp((double)x); }
+static inline float or_logf(float x) { return (float)log((double)x); }
+static inline float or_sinf(float x) { return (float)sin((double)x); }
+static inline float or_cosf(float x) { return (float)cos((double)x); }
+static inline float or_powf(float x, float y) { return (float)pow((double)x, (double)y); }
+static inline float or_acosf(float x) { return (float)acos((double)x); }
+static inline float or_atan2f(float y, float x) { return (float)atan2((double)y, (double)x); }
+
 /* src/MathCommon.h:103-106 */
 static inline float safe_rcp(float f)
 {

@@ -470,8 +470,8 @@ static v3 ggx_sample_wh(v3 wo, float u0, float u1, float ax, float ay)
     v3 T2 = cross3(wh, T1);
     float r = sqrtf(fmaxr(u0, 0.0f));
     float phi = 2.0f * OR_PI * u1;
-    float t1 = r * cosf(phi);
-    float t2 = r * sinf(phi);
+    float t1 = r * or_cosf(phi);
+    float t2 = r * or_sinf(phi);
     float s = (float)(0.5 * (1.0 + (double)wh.z));
     t2 = (float)((1.0 - (double)s) * sqrt(1.0 - (double)(t1 * t1)) + (double)s * (double)t2);
     v3 Nh = add3(add3(sclf3(t1, T1), sclf3(t2, T2)), sclf3(sqrtf(fmaxr(0.0f, 1.0f - t1 * t1 - t2 * t2)), wh));
@@ -481,7 +481,7 @@ static v3 sample_cosine(float u0, float u1)
 {
     float r = sqrtf(u0);
     float phi = u1 * OR_TWO_PI;
-    float x = r * cosf(phi), y = r * sinf(phi);
+    float x = r * or_cosf(phi), y = r * or_sinf(phi);
     float r2 = x * x + y * y;
     return mk3(x, y, r2 < 1.0f ? sqrtf(1.0f - r2) : 0.0f);
 }
@@ -651,7 +651,7 @@ static v3 eval_node(const or_scene *sc, uint32_t ti, v3 black, v3 white, or_tex_
     case YMRT_TEXTURE_CONSTANT_FLOAT: return s3(t->value[0]);
     case YMRT_TEXTURE_CONSTANT_RGB: return mk3(t->value[0], t->value[1], t->value[2]);
     case YMRT_TEXTURE_CHECKERBOARD: {
-        float x = sinf(po.x * t->frequency), y = sinf(po.y * t->frequency), z = sinf(po.z * t->frequency);
+        float x = or_sinf(po.x * t->frequency), y = or_sinf(po.y * t->frequency), z = or_sinf(po.z * t->frequency);
         return x * y * z < 0.0f ? white : black;
     }
     case YMRT_TEXTURE_NOISE: {
@@ -671,7 +671,7 @@ static v3 eval_node(const or_scene *sc, uint32_t ti, v3 black, v3 white, or_tex_
         turb += (t->turbulence_axes & 1) ? po.x : 0.0f;
         turb += (t->turbulence_axes & 2) ? po.y : 0.0f;
         turb += (t->turbulence_axes & 4) ? po.z : 0.0f;
-        float m = sinf(turb + t->variation * fbm) * 0.5f + 0.5f;
+        float m = or_sinf(turb + t->variation * fbm) * 0.5f + 0.5f;
         return lerp_bw(black, white, m);
     }
     case YMRT_TEXTURE_WOOD: {
@@ -778,10 +778,10 @@ static void init_default_bsdf(const or_scene *sc, mbsdf *b, const ymrt_material_
     float cay = sqrf(clampf(tex_float(sc, m->coat_roughness_y_tex, m->coat_roughness_y, uv), 0.001f, 0.999f));
     v3 coat_alb = tex_color(sc, m->coat_albedo_tex, m->coat_albedo, uv);
     v3 ca = max3(coat_alb, s3(1E-8f));
-    v3 sigma = scl3(neg3(mk3(logf(ca.x), logf(ca.y), logf(ca.z))), coat_th);
+    v3 sigma = scl3(neg3(mk3(or_logf(ca.x), or_logf(ca.y), or_logf(ca.z))), coat_th);
     float rc = safe_rcp(cos_in);
     v3 ns = neg3(sigma);
-    v3 coat_tr = mk3(expf(ns.x * rc), expf(ns.y * rc), expf(ns.z * rc));
+    v3 coat_tr = mk3(or_expf(ns.x * rc), or_expf(ns.y * rc), or_expf(ns.z * rc));
     float wsum = 0.0f;
     b->count = 0;
     v3 coat_color = s3(coat_w * fr_ic);
@@ -1047,7 +1047,7 @@ int or_hg_sample(float g, float u0, float u1, v3 wo, v3 *wi, float *pdf)
     float phi = 2.f * OR_PI * u1;
     v3 w = wo, u, v;
     orthogonal_basis(&w, &u, &v);
-    *wi = add3(add3(scl3(scl3(u, s), cosf(phi)), scl3(scl3(v, s), sinf(phi))), scl3(w, c));
+    *wi = add3(add3(scl3(scl3(u, s), or_cosf(phi)), scl3(scl3(v, s), or_sinf(phi))), scl3(w, c));
     or_hg_eval(g, wo, *wi, pdf);
     return 1;
 }
@@ -1245,10 +1245,10 @@ static int sample_homogeneous(const or_scene *sc, int n, const int *vi, float tm
     else if (u0 < cw.x + cw.y) { ch = 1; u0 = (u0 - cw.x) * safe_rcp(cw.y); }
     else { ch = 2; u0 = (u0 - cw.x - cw.y) * safe_rcp(cw.z); }
     float sc_ = comp3(st, ch);
-    float fpl = sc_ > 0.0f ? (-logf(1.0f - u0) / sc_) : OR_FLOAT_MAX;
+    float fpl = sc_ > 0.0f ? (-or_logf(1.0f - u0) / sc_) : OR_FLOAT_MAX;
     int scatter = fpl < tmax;
     float m = fminr(tmax, fpl);
-    v3 tr = mk3(expf(-st.x * m), expf(-st.y * m), expf(-st.z * m));
+    v3 tr = mk3(or_expf(-st.x * m), or_expf(-st.y * m), or_expf(-st.z * m));
     float pdf = scatter ? (cw.x * st.x * tr.x + cw.y * st.y * tr.y + cw.z * st.z * tr.z)
                         : (cw.x * tr.x + cw.y * tr.y + cw.z * tr.z);
     if (!(fabsf(pdf) > 0.0f)) return 0;
@@ -1289,10 +1289,10 @@ static int sample_heterogeneous(const or_scene *sc, int n, const int *vi, const 
     int it = 0;
     for (;;) {
         if (it >= OR_MAX_FREE_PATH_LENGTH) break;
-        float fpl = -logf(1.0f - or_sampler_next(s)) / comp3(sm, ch);
+        float fpl = -or_logf(1.0f - or_sampler_next(s)) / comp3(sm, ch);
         if (acc + fpl > tmax) {
             float dd = tmax - acc;
-            v3 ptr = mk3(expf(-sm.x * dd), expf(-sm.y * dd), expf(-sm.z * dd));
+            v3 ptr = mk3(or_expf(-sm.x * dd), or_expf(-sm.y * dd), or_expf(-sm.z * dd));
             tr = mul3(tr, mul3(ptr, nf));
             pdf = mul3(pdf, mul3(ptr, nf));
             acc += fpl;
@@ -1319,7 +1319,7 @@ static int sample_heterogeneous(const or_scene *sc, int n, const int *vi, const 
         rp.y = fminr(sm.y > 0.0f ? st.y * smr.y : 0.0f, 1.0f);
         rp.z = fminr(sm.z > 0.0f ? st.z * smr.z : 0.0f, 1.0f);
         v3 fp = sub3(s3(1.0f), rp);
-        v3 ptr = mk3(expf(-sm.x * fpl), expf(-sm.y * fpl), expf(-sm.z * fpl));
+        v3 ptr = mk3(or_expf(-sm.x * fpl), or_expf(-sm.y * fpl), or_expf(-sm.z * fpl));
         if (or_sampler_next(s) < comp3(rp, ch)) {
             tr = mul3(tr, mul3(mul3(ptr, ss), nf));
             pdf = mul3(pdf, mul3(mul3(mul3(ptr, sm), rp), nf));
@@ -1365,7 +1365,7 @@ static v3 eval_tr(const or_scene *sc, const or_ray *ray, const or_transfer *t, f
     if (!het) {
         v3 st = s3(0.0f);
         for (int i = 0; i < n; ++i) st = add3(st, vsig(&sc->volumes[vi[i]]));
-        return mk3(expf(-st.x * tmax), expf(-st.y * tmax), expf(-st.z * tmax));
+        return mk3(or_expf(-st.x * tmax), or_expf(-st.y * tmax), or_expf(-st.z * tmax));
     }
     v3 ro[OR_MAX_BOUNDARY_RECORD], rd[OR_MAX_BOUNDARY_RECORD];
     v3 sm = s3(0.0f);
@@ -1391,10 +1391,10 @@ static v3 eval_tr(const or_scene *sc, const or_ray *ray, const or_transfer *t, f
     int it = 0;
     for (;;) {
         if (it >= OR_MAX_FREE_PATH_LENGTH) break;
-        float fpl = -logf(1.0f - shadow_rand(rng)) / comp3(sm, ch);
+        float fpl = -or_logf(1.0f - shadow_rand(rng)) / comp3(sm, ch);
         if (acc + fpl > tmax) {
             float dd = tmax - acc;
-            v3 ptr = mk3(expf(-sm.x * dd), expf(-sm.y * dd), expf(-sm.z * dd));
+            v3 ptr = mk3(or_expf(-sm.x * dd), or_expf(-sm.y * dd), or_expf(-sm.z * dd));
             tr = mul3(tr, mul3(ptr, nf));
             pdf = mul3(pdf, mul3(ptr, nf));
             break;
@@ -1424,7 +1424,7 @@ static v3 eval_tr(const or_scene *sc, const or_ray *ray, const or_transfer *t, f
         rp.y = fminr(sm.y > 0.0f ? st.y * smr.y : 0.0f, 1.0f);
         rp.z = fminr(sm.z > 0.0f ? st.z * smr.z : 0.0f, 1.0f);
         v3 fp = sub3(s3(1.0f), rp);
-        v3 ptr = mk3(expf(-sm.x * fpl), expf(-sm.y * fpl), expf(-sm.z * fpl));
+        v3 ptr = mk3(or_expf(-sm.x * fpl), or_expf(-sm.y * fpl), or_expf(-sm.z * fpl));
         tr = mul3(tr, mul3(mul3(mul3(ptr, sm), fp), nf));
         pdf = mul3(pdf, mul3(mul3(ptr, sm), nf));
         if (it + 1 > 32) {
@@ -1537,7 +1537,7 @@ static v3 sample_cone(float u0, float u1, float ctm)
     float ct = (1.0f - u0) + u0 * ctm;
     float st = sqrtf(fmaxr(0.0f, 1.0f - ct * ct));
     float phi = u1 * 2.0f * OR_PI;
-    return mk3(cosf(phi) * st, sinf(phi) * st, ct);
+    return mk3(or_cosf(phi) * st, or_sinf(phi) * st, ct);
 }
 
 static v3 inst_le(const or_scene *sc, const ymrt_instance_desc *in)
@@ -2101,6 +2101,7 @@ static v3 light_clamp(const ymrt_render_settings *rs, v3 c, int depth)
     float lum = luminance3(c);
     if (lum <= 1E-10f) return c;
     float lim = rs->indirect_limit;
+    /* glibc's float powf, as the library's float powf here */
     float pw = fminr(0.05f + 0.25f * powf(lum / lim, 1.5f), 1.0f);
     float sf = powf(lim / (lim + lum), pw);
     return scl3(c, sf);
@@ -2120,8 +2121,8 @@ static void fetch_shading(const or_scene *sc, const or_hit *h, v3 *p, v3 *perr, 
         *perr = sclf3(error_gamma(5), abs3(po));
         *sn = no;
         *ng = no;
-        float theta = acosf(no.y);
-        float phi = atan2f(no.z, no.x);
+        float theta = or_acosf(no.y);
+        float phi = or_atan2f(no.z, no.x);
         if (phi < 0.0f) phi += OR_TWO_PI;
         uv->x = 0.5f * phi * OR_INV_PI;
         uv->y = theta * OR_INV_PI;
@@ -2621,7 +2622,7 @@ void or_tonemap(float *rgba, uint32_t w, uint32_t h, float gamma, float exposure
         for (int c = 0; c < 3; ++c) {
             float x = p[c] * exposure;
             float y = (x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f);
-            p[c] = powf(y, 1.0f / gamma);
+            p[c] = or_powf(y, 1.0f / gamma);
         }
         p[3] = 1.0f;
     }

@@ -359,3 +359,22 @@ def test_polka_dot_centred_disc_at_radius_half():
     assert np.all((got == 1.0) == inside | edge & (got == 1.0))
     assert np.all(got[~edge] == inside[~edge].astype(np.float32))
     assert 0.1 < inside.mean() < 0.6
+
+
+def test_float_image_texture_constant_equals_material_colour():
+    """A linear float image texture holding one colour everywhere renders as
+    the material's own constant colour: texels are taken as they are, per
+    channel in order (no colour-space table, no channel swap)."""
+    rs = abi.render_settings(24, 24, 2, 4)
+    base = oracle.Scene(scenes.cornell_box(24, 24)).render(rs, 1, threads=4)
+    sa = scenes.cornell_box(24, 24)
+    t = np.empty((5, 7, 3), np.float32)
+    t[...] = (0.7, 0.1, 0.1)                 # the red wall's albedo
+    sa.materials[1].diffuse_albedo_tex = sa.add_texture(t)
+    img = oracle.Scene(sa).render(rs, 1, threads=4)
+    assert np.allclose(img[..., :3], base[..., :3], rtol=1e-5, atol=1e-7)
+    # a different colour does change the image
+    t[...] = (0.1, 0.1, 0.7)
+    sa.textures[-1]["texels"] = t
+    img2 = oracle.Scene(sa).render(rs, 1, threads=4)
+    assert np.abs(img2[..., :3] - base[..., :3]).max() > 1e-3

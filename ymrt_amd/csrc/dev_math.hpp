@@ -93,10 +93,9 @@ YM_HD float safe_rcp(float f) { return fabsf(f) < 1E-30f ? (f > 0.0f ? 1E30f : -
 // Transcendentals of float arguments, evaluated in double precision and
 // rounded once: the correctly rounded float result except when the double
 // result lies within one of its own ulps of a float rounding boundary.  The
-// oracle's glibc float functions are within about half an ulp as well, so
-// both sides round to the same float nearly always; the GPU's own float
-// versions are up to a few ulps off, and the last bits of a free-flight
-// distance, a transmittance or a sampled direction decide later branches.
+// GPU's own float versions are up to a few ulps off, and the last bits of a
+// free-flight distance, a transmittance or a sampled direction decide later
+// branches; a CPU evaluation of the same form rounds to the same float.
 YM_HD float ym_expf(float x) { return (float)exp((double)x); }
 YM_HD float ym_logf(float x) { return (float)log((double)x); }
 YM_HD float ym_sinf(float x) { return (float)sin((double)x); }
