@@ -354,6 +354,21 @@ int ymrt_render_frame(ymrt_scene *scene, const ymrt_render_settings *settings,
                       int frame_index, float *noise_rgba, void *stream,
                       int timing, ymrt_frame_stats *stats);
 
+/* ymrt_render_frame without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * path state lives in a fixed 64 MiB device array that the library holds on
+ * every GPU it runs on until the process ends, so a frame runs in chunks of at
+ * most 65536 paths with at most 1024 waves per pass kernel; the image is the
+ * one ymrt_render_frame gives, bit for bit.  Rejected with
+ * YMRT_ERR_INVALID_ARGUMENT: path_capacity > 65536, spp > 65536,
+ * persistent_waves > 1024.  No per-stage timing (stats gets the counts).
+ * Every call on one GPU shares that GPU's array, which keeps the last call's
+ * contents: calls that overlap on one GPU (other threads or streams) race. */
+int ymrt_render_frame_noalloc(ymrt_scene *scene, const ymrt_render_settings *settings,
+                              int frame_index, float *noise_rgba, void *stream,
+                              ymrt_frame_stats *stats);
+
 /* Autotuner for the two launch settings that change no pixel, path_capacity
  * and persistent_waves: renders trial frames of the caller's settings
  * (frame_index, into an image of its own) and times them with HIP events --

@@ -25,7 +25,7 @@ EXPORTS = [
     "ymrt_ymtmp_load", "ymrt_ymtmp_parse", "ymrt_ymtmp_scene", "ymrt_ymtmp_warning_count",
     "ymrt_ymtmp_warning", "ymrt_ymtmp_find", "ymrt_ymtmp_transform", "ymrt_ymtmp_free",
     "ymrt_render_frame_aov", "ymrt_save_png", "ymrt_render_adjoint", "ymrt_scene_update_materials",
-    "ymrt_render_batch", "ymrt_autotune", "ymrt_scene_release_buffers", "ymrt_scene_update_texture",
+    "ymrt_render_batch", "ymrt_autotune", "ymrt_scene_release_buffers", "ymrt_scene_update_texture", "ymrt_render_frame_noalloc",
 ]
 
 _lib = None
@@ -62,6 +62,8 @@ def lib():
         L.ymrt_trace_closest_counted.argtypes = [vp, vp, C.c_uint32, vp, vp, vp]
         L.ymrt_render_frame.argtypes = [vp, C.POINTER(abi.RenderSettings), C.c_int, vp, vp, C.c_int,
                                         C.POINTER(abi.FrameStats)]
+        L.ymrt_render_frame_noalloc.argtypes = [vp, C.POINTER(abi.RenderSettings), C.c_int, vp, vp,
+                                                C.POINTER(abi.FrameStats)]
         L.ymrt_accumulate.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_int, C.c_int, vp]
         L.ymrt_tonemap.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_float, C.c_float, vp]
         L.ymrt_halton_values.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp]
@@ -257,6 +259,17 @@ class Scene:
         stats = abi.FrameStats()
         _check(lib().ymrt_render_frame(self.h, C.byref(settings), frame, _dev_ptr(noise), _stream(),
                                        1 if timing else 0, C.byref(stats)))
+        return noise, stats
+
+    def render_frame_noalloc(self, settings, frame=1, noise=None):
+        """ymrt_render_frame_noalloc: the same image as render_frame, with the
+        path state in the library's fixed device memory (no allocation)."""
+        import torch
+        if noise is None:
+            noise = torch.empty((settings.height, settings.width, 4), dtype=torch.float32, device="cuda")
+        stats = abi.FrameStats()
+        _check(lib().ymrt_render_frame_noalloc(self.h, C.byref(settings), frame, _dev_ptr(noise), _stream(),
+                                               C.byref(stats)))
         return noise, stats
 
     def render_adjoint(self, settings, adjoint, frame=1, grad=None, timing=False):

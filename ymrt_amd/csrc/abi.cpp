@@ -21,6 +21,9 @@ int autotune(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, hipStrea
 int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, float *noise, uint32_t *instance_ids,
                  const float *adj_img, float *adj_grad, hipStream_t stream, int timing,
                  ymrt_frame_stats *stats);                                      // render.cpp
+int noalloc_setup(int device);  // noalloc.hip
+int render_frame_noalloc(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, float *noise,
+                         hipStream_t stream, ymrt_frame_stats *stats);  // noalloc.hip
 int accumulate(const float *noise, float *accum, uint32_t w, uint32_t h, int frame, hipStream_t s);
 int tonemap(float *rgba, uint32_t w, uint32_t h, float gamma, float exposure, hipStream_t s);
 
@@ -72,6 +75,11 @@ int ymrt_context_create(int device, ymrt_context **out)
         return YMRT_ERR_OUT_OF_MEMORY;
     }
     c->device = device;
+    // the fixed device memory of ymrt_render_frame_noalloc, once per device
+    if (int rc = noalloc_setup(device)) {
+        delete c;
+        return rc;
+    }
     *out = c;
     return YMRT_OK;
 }
@@ -177,6 +185,21 @@ int ymrt_render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame_
     try {
         int rc = render_frame(sc, rs, frame_index, noise_rgba, nullptr, nullptr, nullptr, (hipStream_t)stream, timing,
                               stats);
+        if (rc) return rc;
+    } catch (const std::exception &e) {
+        set_error(std::string("render failed: ") + e.what());
+        return YMRT_ERR_INTERNAL;
+    }
+    return check_stack_flag(sc);
+}
+
+int ymrt_render_frame_noalloc(ymrt_scene *sc, const ymrt_render_settings *rs, int frame_index, float *noise_rgba,
+                              void *stream, ymrt_frame_stats *stats)
+{
+    YM_REQUIRE(sc != nullptr && rs != nullptr && noise_rgba != nullptr, "NULL argument");
+    YM_REQUIRE(frame_index >= 1, "frame_index starts at 1");
+    try {
+        int rc = render_frame_noalloc(sc, rs, frame_index, noise_rgba, (hipStream_t)stream, stats);
         if (rc) return rc;
     } catch (const std::exception &e) {
         set_error(std::string("render failed: ") + e.what());

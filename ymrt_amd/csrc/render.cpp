@@ -91,6 +91,56 @@ static int ensure_buffers(RenderBuffers *&slot, uint32_t cap)
     return YMRT_OK;
 }
 
+// Arrays carved out of fixed device memory start on 256-B boundaries.
+static size_t carve_round(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t fixed_buffer_bytes(uint32_t cap, uint32_t spill_threads)
+{
+    RenderBuffers sizes;  // owns nothing: only the layout is computed
+    size_t total = 0;
+    layout_buffers(sizes, cap, spill_threads, [&](auto *&p, size_t count) {
+        total += carve_round(sizeof(*p) * std::max<size_t>(count, 1));
+        p = nullptr;
+        return 0;
+    });
+    return total;
+}
+
+// One frame whose path state is carved out of `memory` (device, `bytes`
+// long) instead of allocated: chunks of up to cap paths, persistent kernels
+// of up to spill_threads / 64 waves.  Allocates no device memory.
+int render_frame_fixed(ymrt_scene *sc, void *memory, size_t bytes, uint32_t cap, uint32_t spill_threads,
+                       const ymrt_render_settings *rs, int frame, float *noise, hipStream_t stream,
+                       ymrt_frame_stats *stats)
+{
+    if (int r = check_settings(rs)) return r;
+    YM_REQUIRE(memory != nullptr, "no fixed device memory");
+    YM_REQUIRE(fixed_buffer_bytes(cap, spill_threads) <= bytes, "fixed device memory too small for the capacity");
+    YM_REQUIRE(rs->path_capacity <= cap,
+               "path_capacity exceeds the fixed capacity of " + std::to_string(cap) + " paths");
+    const uint32_t spp = (uint32_t)rs->spp;
+    YM_REQUIRE(spp <= cap, "spp exceeds the fixed capacity of " + std::to_string(cap) + " paths");
+    RenderBuffers view;  // owns nothing: no allocations, no events
+    view.cap = cap;
+    char *base = static_cast<char *>(memory);
+    size_t off = 0;
+    layout_buffers(view, cap, spill_threads, [&](auto *&p, size_t count) {
+        p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(base + off);
+        off += carve_round(sizeof(*p) * std::max<size_t>(count, 1));
+        return 0;
+    });
+    const uint64_t total_paths = (uint64_t)((rs->width + 15) / 16) * ((rs->height + 15) / 16) * 256ull * spp;
+    const uint64_t want = rs->path_capacity ? rs->path_capacity : cap;
+    YM_REQUIRE(want >= spp, "path_capacity must hold one pixel's samples");
+    const uint64_t per_cap = (want / spp) * spp;
+    const uint64_t nchunks = (total_paths + per_cap - 1) / per_cap;
+    const uint32_t chunk_paths_max =
+        (uint32_t)std::min<uint64_t>(per_cap, (total_paths / nchunks + spp - 1) / spp * spp);
+    YM_HIP(hipSetDevice(sc->ctx->device));
+    if (int r = clear_stack_flag(sc, stream)) return r;
+    return run_frame(sc, &view, chunk_paths_max, rs, frame, noise, nullptr, nullptr, nullptr, stream, 0, stats);
+}
+
 static hipEvent_t next_event(RenderBuffers *rb)
 {
     if (rb->event_next == rb->events.size()) {
