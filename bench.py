@@ -63,22 +63,28 @@ def parse():
 
 
 def adjoint_timing(ctx, frames=2):
-    """Forward and path-replay adjoint (Lambert albedo gradients) of the
-    Cornell box at 512x512, 16 spp, depth 6, timed with HIP events on the
-    render stream; the adjoint frame includes its forward rendering."""
+    """The differentiable workload as configured: Cornell box at 512x512,
+    16 spp, depth 6 with a 256^2 FP32 RGB albedo texture on the floor, 8
+    metal spheres of their own GGX roughness and a 64^3 voxel-grid medium
+    (sigma_max 2.0).  Forward: ymrt_render_frame; adjoint: ymrt_render_adjoint
+    with every parameter a target (the forward frame plus the path replay),
+    both timed with HIP events on the render stream over `frames` frames
+    after a warm-up frame each."""
     import numpy as np
     import torch
 
     import ymrt_amd
     from ymrt_amd import abi, scenes
-    sa = scenes.cornell_box(512, 512)
+    sa, tg = scenes.cornell_inverse(512, 512, scenes.inverse_params())
     sc = ymrt_amd.Scene(ctx, sa)
-    rs = abi.render_settings(512, 512, 16, 6)
+    rs = abi.render_settings(512, 512, 16, 6, volume=1)
     adj = torch.from_numpy(np.random.default_rng(5).uniform(0.5, 1.5, (512, 512, 3)).astype(np.float32)).cuda()
+    targets = dict(albedo=tg["albedo"], roughness=tg["roughness"], texture=tg["texture"], volume=tg["volume"])
     sc.render_frame(rs, 1)
-    sc.render_adjoint(rs, adj, 1)
+    sc.render_adjoint(rs, adj, 1, **targets)
     torch.cuda.synchronize()
     t = {}
+    stats = None
     for name in ("forward", "adjoint"):
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         a.record()
@@ -86,13 +92,16 @@ def adjoint_timing(ctx, frames=2):
             if name == "forward":
                 sc.render_frame(rs, 2 + f)
             else:
-                sc.render_adjoint(rs, adj, 2 + f)
+                _, _, stats = sc.render_adjoint(rs, adj, 2 + f, **targets)
         b.record()
         torch.cuda.synchronize()
         t[name] = a.elapsed_time(b) / frames
-    return {"workload": "Cornell box 512x512 16spp depth 6, gradients of 3 Lambert albedos (rgb)",
+    sc.close()
+    return {"workload": "Cornell box 512x512 16spp depth 6 with a 64^3 grid medium (sigma_max 2); gradients of a "
+                        "256^2 RGB float albedo texture, 8 GGX roughnesses, 64^3 densities and 3 Lambert albedos",
             "forward_ms": round(t["forward"], 3), "adjoint_ms": round(t["adjoint"], 3),
-            "adjoint_over_forward": round(t["adjoint"] / t["forward"], 3)}
+            "adjoint_over_forward": round(t["adjoint"] / t["forward"], 3),
+            "paths": int(stats.paths) if stats is not None else None}
 
 
 def batch_timing(ctx, count=64, seed=11):

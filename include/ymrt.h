@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define YMRT_ABI_VERSION 7
+#define YMRT_ABI_VERSION 8
 
 /* status codes */
 #define YMRT_OK 0
@@ -279,7 +279,7 @@ typedef struct ymrt_frame_stats {
     float ms_transmit;               /* transmittance of shadow rays in media */
     uint32_t launches_transmit;
     uint64_t transmit_nodes, transmit_tris;   /* boundary walks of those rays */
-    float ms_adjoint;                /* gradient accumulation of ymrt_render_adjoint */
+    float ms_adjoint;                /* the replay pass of ymrt_render_adjoint    */
     uint32_t reserved;
 } ymrt_frame_stats;
 
@@ -332,6 +332,13 @@ int ymrt_scene_release_buffers(ymrt_scene *scene);
  * An optimisation loop over ymrt_render_adjoint's texture gradient writes
  * its updated texels back this way. */
 int ymrt_scene_update_texture(ymrt_scene *scene, uint32_t texture, const float *texels, void *stream);
+
+/* Replaces the densities of voxel grid `grid` with `density` (host, nz * ny
+ * * nx values, the layout of ymrt_grid_desc) and its majorant with their
+ * maximum; synchronous on stream.  No counterpart in the reference, which
+ * has no voxel grids; an optimisation loop over ymrt_render_adjoint's
+ * density gradient writes its updated grid back this way. */
+int ymrt_scene_update_grid(ymrt_scene *scene, uint32_t grid, const float *density, void *stream);
 
 /* Closest hit (TraceRay) for n device-resident rays. */
 int ymrt_trace_closest(ymrt_scene *scene, const ymrt_ray *rays, uint32_t n,
@@ -398,23 +405,56 @@ int ymrt_render_frame_aov(ymrt_scene *scene, const ymrt_render_settings *setting
                           int frame_index, float *noise_rgba, uint32_t *instance_ids,
                           void *stream, int timing, ymrt_frame_stats *stats);
 
-/* Path-replay adjoint of one frame for the albedo of Lambert-only DefaultMtl
- * materials (specular, metalness, transmission and coat weights 0, no
- * texture on those or on the albedo): renders the frame like
- * ymrt_render_frame and adds to grad_albedo[m*3 + c] (device, material_count
- * x 3 floats, not cleared) the derivative of sum_pixels adjoint_rgb . image
- * with respect to diffuse_albedo[c] of material m, with the frame's own
- * samples: every path term after a vertex on m is linear in that albedo
- * (cosine sampling and MIS do not depend on it), so the derivative of a path
- * is (L_path - L_before_vertex) / albedo summed over its vertices on m.
- * Channels with albedo 0 and other materials get 0.  Needs Russian roulette
- * and the indirect clamp off.  No counterpart in the reference, whose
+/* Parameters ymrt_render_adjoint differentiates, and their gradients
+ * (device arrays of floats the call adds to; the caller clears them):
+ *  - albedo_materials: Lambert-only DefaultMtl materials (specular,
+ *    metalness, transmission and coat weights 0, none of them or the diffuse
+ *    albedo textured); grad_albedo[k*3 + c] is d/d diffuse_albedo[c] of
+ *    albedo_materials[k];
+ *  - roughness_materials: metal-only DefaultMtl materials (metalness 1, no
+ *    coat, roughness_x == roughness_y, untextured); grad_roughness[k] is
+ *    d/d roughness of roughness_materials[k] (both axes move together);
+ *  - texture: an IMAGE texture used only as the diffuse albedo of Lambert
+ *    materials (otherwise as above); grad_texture has width * height *
+ *    channels entries, d/d the linear texel values (float texels, or the
+ *    decoded values of 8-bit ones);
+ *  - volume: a volume with a gray sigma_t and a voxel-grid density;
+ *    grad_density has nz * ny * nx entries, d/d each voxel's density.
+ * YMRT_INVALID_INDEX for texture / volume, or a count of 0, leaves that
+ * target out (its gradient pointer may be NULL). */
+typedef struct ymrt_adjoint_targets {
+    const uint32_t *albedo_materials;       /* host, albedo_count entries    */
+    uint32_t albedo_count;
+    const uint32_t *roughness_materials;    /* host, roughness_count entries */
+    uint32_t roughness_count;
+    uint32_t texture;
+    uint32_t volume;
+    float *grad_albedo;                     /* device, albedo_count * 3      */
+    float *grad_roughness;                  /* device, roughness_count       */
+    float *grad_texture;                    /* device, texels x channels     */
+    float *grad_density;                    /* device, voxels                */
+} ymrt_adjoint_targets;
+
+/* Path-replay adjoint of one frame: renders the frame like ymrt_render_frame
+ * into noise_rgba, then replays every path with the same samples (each
+ * random number is keyed by pixel, sample and vertex, so nothing is stored
+ * per vertex) and adds to the targets' gradients the derivative of
+ * sum_pixels adjoint_rgb . image (adjoint_rgb: device, width * height * 3)
+ * with the frame's own samples, sampling pdfs and MIS weights held fixed:
+ * a factor of a vertex or a free flight that scales every later deposit of
+ * the path adds adj . (L_path - L_before) * dln(factor), a factor of one
+ * shadow ray adds adj . deposit * dln(factor) (GGX roughness: d ln(D G2) /
+ * dr; voxel density: the delta tracker's collision probabilities sigma /
+ * sigma_m and 1 - sigma / sigma_m, and ratio tracking's 1 - sigma /
+ * sigma_m).  A target that is not differentiable as listed above is
+ * rejected with YMRT_ERR_INVALID_ARGUMENT naming it; Russian roulette and
+ * the indirect clamp must be off.  No counterpart in the reference, whose
  * render loop has no derivatives; it is the adjoint its differentiable
  * rendering workload asks for.  Gradients are summed with float atomics, so
  * their last bits vary between runs. */
 int ymrt_render_adjoint(ymrt_scene *scene, const ymrt_render_settings *settings,
                         int frame_index, float *noise_rgba, const float *adjoint_rgb,
-                        float *grad_albedo, void *stream, int timing,
+                        const ymrt_adjoint_targets *targets, void *stream, int timing,
                         ymrt_frame_stats *stats);
 
 /* One frame of one scene in a batch. */

@@ -2294,7 +2294,18 @@ int or_adjoint_prepare(const or_scene *sc, const uint32_t *alb, uint32_t n_alb, 
         for (uint32_t m = 0; m < nm; ++m) {
             const ymrt_material_desc *md = &sc->materials[m];
             if (md->diffuse_albedo_tex == tex && !lambert_textured(md)) return pos;
+            /* a texture read by any other material slot or a density is not
+             * a target either: its gradient would miss those uses */
+            const uint32_t other[] = {md->specular_albedo_tex, md->roughness_x_tex, md->roughness_y_tex,
+                                      md->specular_weight_tex, md->metalness_tex, md->transmission_weight_tex,
+                                      md->coat_albedo_tex, md->coat_weight_tex, md->coat_thickness_tex,
+                                      md->coat_roughness_x_tex, md->coat_roughness_y_tex,
+                                      md->normal_mapping_tex, md->bump_mapping_tex};
+            for (size_t k = 0; k < sizeof(other) / sizeof(other[0]); ++k)
+                if (other[k] == tex) return pos;
         }
+        for (uint32_t v = 0; v < sc->volume_count; ++v)
+            if (sc->volumes[v].density_texture == (int32_t)tex) return pos;
         ntex = (size_t)sc->textures[tex].width * sc->textures[tex].height * sc->textures[tex].channels;
     }
     ++pos;

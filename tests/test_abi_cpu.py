@@ -29,7 +29,7 @@ def test_library_exports_every_symbol():
 
 def test_abi_version_and_error_string():
     L = ymrt_amd.lib()
-    assert L.ymrt_abi_version() == 7
+    assert L.ymrt_abi_version() == 8
     assert isinstance(L.ymrt_last_error(), bytes)
 
 

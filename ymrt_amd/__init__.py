@@ -26,6 +26,7 @@ EXPORTS = [
     "ymrt_ymtmp_warning", "ymrt_ymtmp_find", "ymrt_ymtmp_transform", "ymrt_ymtmp_free",
     "ymrt_render_frame_aov", "ymrt_save_png", "ymrt_render_adjoint", "ymrt_scene_update_materials",
     "ymrt_render_batch", "ymrt_autotune", "ymrt_scene_release_buffers", "ymrt_scene_update_texture", "ymrt_render_frame_noalloc",
+    "ymrt_scene_update_grid",
 ]
 
 _lib = None
@@ -57,6 +58,7 @@ def lib():
         L.ymrt_scene_get_info.argtypes = [vp, C.POINTER(abi.SceneInfo)]
         L.ymrt_scene_release_buffers.argtypes = [vp]
         L.ymrt_scene_update_texture.argtypes = [vp, C.c_uint32, vp, vp]
+        L.ymrt_scene_update_grid.argtypes = [vp, C.c_uint32, vp, vp]
         L.ymrt_trace_closest.argtypes = [vp, vp, C.c_uint32, vp, vp]
         L.ymrt_trace_any.argtypes = [vp, vp, C.c_uint32, vp, vp]
         L.ymrt_trace_closest_counted.argtypes = [vp, vp, C.c_uint32, vp, vp, vp]
@@ -83,8 +85,8 @@ def lib():
         L.ymrt_render_frame_aov.argtypes = [vp, C.POINTER(abi.RenderSettings), C.c_int, vp, vp, vp, C.c_int,
                                             C.POINTER(abi.FrameStats)]
         L.ymrt_save_png.argtypes = [C.c_char_p, vp, C.c_uint32, C.c_uint32]
-        L.ymrt_render_adjoint.argtypes = [vp, C.POINTER(abi.RenderSettings), C.c_int, vp, vp, vp, vp, C.c_int,
-                                          C.POINTER(abi.FrameStats)]
+        L.ymrt_render_adjoint.argtypes = [vp, C.POINTER(abi.RenderSettings), C.c_int, vp, vp,
+                                          C.POINTER(abi.AdjointTargets), vp, C.c_int, C.POINTER(abi.FrameStats)]
         for name in EXPORTS:
             if name not in ("ymrt_last_error", "ymrt_abi_version"):
                 getattr(L, name).restype = C.c_int
@@ -179,7 +181,9 @@ class Scene:
     def update_texture(self, texture, texels):
         """ymrt_scene_update_texture: new linear values of a float image texture."""
         t = np.ascontiguousarray(texels, np.float32)
+        assert t.size == self.arrays.textures[texture]["texels"].size
         _check(lib().ymrt_scene_update_texture(self.h, texture, t.ctypes.data_as(C.c_void_p), _stream()))
+        self.arrays.textures[texture]["texels"] = t.reshape(self.arrays.textures[texture]["texels"].shape)
 
     def release_buffers(self):
         """ymrt_scene_release_buffers: free the path state of this scene's frames."""
@@ -272,20 +276,55 @@ class Scene:
                                                C.byref(stats)))
         return noise, stats
 
-    def render_adjoint(self, settings, adjoint, frame=1, grad=None, timing=False):
-        """(noise image, gradient (material_count, 3) cuda tensor, stats) of
-        ymrt_render_adjoint; adjoint is dLoss/dImage, a (H, W, 3) cuda tensor."""
+    def render_adjoint(self, settings, adjoint, frame=1, albedo=None, roughness=(), texture=None, volume=None,
+                       grads=None, timing=False):
+        """(noise image, gradients, stats) of ymrt_render_adjoint; adjoint is
+        dLoss/dImage, a (H, W, 3) cuda tensor.  Targets: albedo, a list of
+        Lambert-only materials (default: every Lambert-only material of the
+        scene); roughness, a list of metal-only materials; texture, an image
+        texture used as the albedo of Lambert materials; volume, a volume
+        with a gray sigma_t and a voxel-grid density.  The gradients are a
+        dict of float32 cuda tensors: "albedo" (len(albedo), 3), "roughness"
+        (len(roughness),), "texture" (H, W, C), "density" (nz, ny, nx);
+        grads (the same dict) is added to instead of fresh zeros."""
         import torch
         h, w = settings.height, settings.width
         adjoint = adjoint.contiguous()
         assert adjoint.shape == (h, w, 3) and adjoint.dtype == torch.float32
         noise = torch.empty((h, w, 4), dtype=torch.float32, device="cuda")
-        if grad is None:
-            grad = torch.zeros((max(len(self.arrays.materials), 1), 3), dtype=torch.float32, device="cuda")
+        if albedo is None:
+            albedo = [i for i, m in enumerate(self.arrays.materials) if abi.lambert_only(m)]
+        albedo, roughness = [int(m) for m in albedo], [int(m) for m in roughness]
+        if grads is None:
+            z = lambda *shape: torch.zeros(shape, dtype=torch.float32, device="cuda")
+            grads = {"albedo": z(len(albedo), 3), "roughness": z(len(roughness))}
+            if texture is not None:
+                grads["texture"] = z(*self.arrays.textures[texture]["texels"].shape)
+            if volume is not None:
+                g = self.arrays.volumes[volume]["grid"]
+                grads["density"] = z(*self.arrays.grids[g].shape)
+        alb = (C.c_uint32 * max(1, len(albedo)))(*albedo)
+        rgh = (C.c_uint32 * max(1, len(roughness)))(*roughness)
+        tg = abi.AdjointTargets()
+        tg.albedo_materials, tg.albedo_count = alb, len(albedo)
+        tg.roughness_materials, tg.roughness_count = rgh, len(roughness)
+        tg.texture = abi.INVALID if texture is None else int(texture)
+        tg.volume = abi.INVALID if volume is None else int(volume)
+        tg.grad_albedo = _dev_ptr(grads["albedo"]) if albedo else None
+        tg.grad_roughness = _dev_ptr(grads["roughness"]) if roughness else None
+        tg.grad_texture = _dev_ptr(grads["texture"]) if texture is not None else None
+        tg.grad_density = _dev_ptr(grads["density"]) if volume is not None else None
         stats = abi.FrameStats()
         _check(lib().ymrt_render_adjoint(self.h, C.byref(settings), frame, _dev_ptr(noise), _dev_ptr(adjoint),
-                                         _dev_ptr(grad), _stream(), 1 if timing else 0, C.byref(stats)))
-        return noise, grad, stats
+                                         C.byref(tg), _stream(), 1 if timing else 0, C.byref(stats)))
+        return noise, grads, stats
+
+    def update_grid(self, grid, density):
+        """ymrt_scene_update_grid: new densities (nz, ny, nx) of a voxel grid."""
+        d = np.ascontiguousarray(density, np.float32)
+        assert d.shape == self.arrays.grids[grid].shape
+        _check(lib().ymrt_scene_update_grid(self.h, grid, d.ctypes.data_as(C.c_void_p), _stream()))
+        self.arrays.grids[grid] = d
 
     def autotune(self, settings, frame=1):
         """abi.TuneResult of ymrt_autotune: the fastest path_capacity and

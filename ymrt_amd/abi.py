@@ -177,6 +177,36 @@ class BatchItem(C.Structure):
     ]
 
 
+class AdjointTargets(C.Structure):
+    """ymrt_adjoint_targets: the parameters ymrt_render_adjoint differentiates
+    (host material lists, texture and volume indices) and their device
+    gradient arrays."""
+    _fields_ = [
+        ("albedo_materials", C.POINTER(C.c_uint32)), ("albedo_count", C.c_uint32),
+        ("roughness_materials", C.POINTER(C.c_uint32)), ("roughness_count", C.c_uint32),
+        ("texture", C.c_uint32), ("volume", C.c_uint32),
+        ("grad_albedo", C.c_void_p), ("grad_roughness", C.c_void_p),
+        ("grad_texture", C.c_void_p), ("grad_density", C.c_void_p),
+    ]
+
+
+def lambert_only(m):
+    """A DefaultMtl whose one lobe is its untextured diffuse albedo (the
+    albedo targets ymrt_render_adjoint accepts)."""
+    return (m.type == MATERIAL_DEFAULT and m.specular_weight == 0.0 and m.metalness == 0.0
+            and m.transmission_weight == 0.0 and m.coat_weight == 0.0 and m.diffuse_albedo_tex == INVALID
+            and m.specular_weight_tex == INVALID and m.metalness_tex == INVALID
+            and m.transmission_weight_tex == INVALID and m.coat_weight_tex == INVALID)
+
+
+def metal_only(m):
+    """Metalness 1 without coat, isotropic untextured roughness (the roughness
+    targets ymrt_render_adjoint accepts)."""
+    return (m.type == MATERIAL_DEFAULT and m.metalness == 1.0 and m.metalness_tex == INVALID
+            and m.coat_weight == 0.0 and m.coat_weight_tex == INVALID and m.roughness_x == m.roughness_y
+            and m.roughness_x_tex == INVALID and m.roughness_y_tex == INVALID)
+
+
 class SceneInfo(C.Structure):
     _fields_ = [
         ("triangle_count", C.c_uint64), ("node_count", C.c_uint64),

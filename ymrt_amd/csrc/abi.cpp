@@ -3,6 +3,7 @@
 #include <new>
 
 #include "internal.hpp"
+#include "pt.hpp"
 
 namespace ym {
 
@@ -14,13 +15,13 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d);                      
 int update_materials(ymrt_scene *sc, uint32_t first, uint32_t count, const ymrt_material_desc *m,
                      hipStream_t s);  // scene.cpp
 int update_texture(ymrt_scene *sc, uint32_t tex, const float *texels, hipStream_t s);  // scene.cpp
+int update_grid(ymrt_scene *sc, uint32_t grid, const float *density, hipStream_t s);    // scene.cpp
 int render_batch(ymrt_context *ctx, const ymrt_batch_item *items, uint32_t n, uint32_t max_slots,
                  hipStream_t stream);  // batch.cpp
 int autotune(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, hipStream_t stream,
              ymrt_tune_result *out);  // tune.cpp
 int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, float *noise, uint32_t *instance_ids,
-                 const float *adj_img, float *adj_grad, hipStream_t stream, int timing,
-                 ymrt_frame_stats *stats);                                      // render.cpp
+                 const AdjParams *adj, hipStream_t stream, int timing, ymrt_frame_stats *stats);  // render.cpp
 int noalloc_setup(int device);  // noalloc.hip
 int render_frame_noalloc(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, float *noise,
                          hipStream_t stream, ymrt_frame_stats *stats);  // noalloc.hip
@@ -183,7 +184,7 @@ int ymrt_render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame_
     YM_REQUIRE(sc != nullptr && rs != nullptr && noise_rgba != nullptr, "NULL argument");
     YM_REQUIRE(frame_index >= 1, "frame_index starts at 1");
     try {
-        int rc = render_frame(sc, rs, frame_index, noise_rgba, nullptr, nullptr, nullptr, (hipStream_t)stream, timing,
+        int rc = render_frame(sc, rs, frame_index, noise_rgba, nullptr, nullptr, (hipStream_t)stream, timing,
                               stats);
         if (rc) return rc;
     } catch (const std::exception &e) {
@@ -209,15 +210,18 @@ int ymrt_render_frame_noalloc(ymrt_scene *sc, const ymrt_render_settings *rs, in
 }
 
 int ymrt_render_adjoint(ymrt_scene *sc, const ymrt_render_settings *rs, int frame_index, float *noise_rgba,
-                        const float *adjoint_rgb, float *grad_albedo, void *stream, int timing,
+                        const float *adjoint_rgb, const ymrt_adjoint_targets *targets, void *stream, int timing,
                         ymrt_frame_stats *stats)
 {
     YM_REQUIRE(sc != nullptr && rs != nullptr && noise_rgba != nullptr, "NULL argument");
-    YM_REQUIRE(adjoint_rgb != nullptr && grad_albedo != nullptr, "NULL adjoint buffer");
+    YM_REQUIRE(adjoint_rgb != nullptr && targets != nullptr, "NULL adjoint buffer or targets");
     YM_REQUIRE(frame_index >= 1, "frame_index starts at 1");
     try {
-        int rc = render_frame(sc, rs, frame_index, noise_rgba, nullptr, adjoint_rgb, grad_albedo, (hipStream_t)stream,
-                              timing, stats);
+        YM_HIP(hipSetDevice(sc->ctx->device));
+        AdjParams adj;
+        int rc = prepare_adjoint(sc, targets, adjoint_rgb, (hipStream_t)stream, &adj);
+        if (rc) return rc;
+        rc = render_frame(sc, rs, frame_index, noise_rgba, nullptr, &adj, (hipStream_t)stream, timing, stats);
         if (rc) return rc;
     } catch (const std::exception &e) {
         set_error(std::string("render failed: ") + e.what());
@@ -232,7 +236,7 @@ int ymrt_render_frame_aov(ymrt_scene *sc, const ymrt_render_settings *rs, int fr
     YM_REQUIRE(sc != nullptr && rs != nullptr && noise_rgba != nullptr, "NULL argument");
     YM_REQUIRE(frame_index >= 1, "frame_index starts at 1");
     try {
-        int rc = render_frame(sc, rs, frame_index, noise_rgba, instance_ids, nullptr, nullptr, (hipStream_t)stream,
+        int rc = render_frame(sc, rs, frame_index, noise_rgba, instance_ids, nullptr, (hipStream_t)stream,
                               timing, stats);
         if (rc) return rc;
     } catch (const std::exception &e) {
@@ -299,6 +303,13 @@ int ymrt_scene_update_texture(ymrt_scene *sc, uint32_t texture, const float *tex
     YM_REQUIRE(sc != nullptr, "scene is NULL");
     YM_HIP(hipSetDevice(sc->ctx->device));
     return update_texture(sc, texture, texels, (hipStream_t)stream);
+}
+
+int ymrt_scene_update_grid(ymrt_scene *sc, uint32_t grid, const float *density, void *stream)
+{
+    YM_REQUIRE(sc != nullptr, "scene is NULL");
+    YM_HIP(hipSetDevice(sc->ctx->device));
+    return update_grid(sc, grid, density, (hipStream_t)stream);
 }
 
 int ymrt_texture_eval(ymrt_scene *sc, uint32_t texture, const float *coords, uint32_t n, float *out, void *stream)

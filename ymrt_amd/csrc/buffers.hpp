@@ -14,7 +14,10 @@ constexpr size_t kCountersPerSlot = (size_t)(kMaxPasses + 2) * kCtrStride;
 
 struct RenderBuffers {
     uint32_t cap = 0;
-    uint32_t adj_depth = 0;  // vertex records allocated per path for the adjoint
+    // the adjoint's replay pass: its own pass counters, statistics and slot
+    uint32_t *adj_counters = nullptr;
+    unsigned long long *adj_stats = nullptr;
+    FrameSlot *adj_slot = nullptr;
     PathState ps{};
     std::vector<void *> allocs;
     uint32_t *counters = nullptr;

@@ -206,6 +206,37 @@ __device__ __forceinline__ f3 image_eval(const DevScene &sc, const DevTexture &t
     return r;
 }
 
+// The four bilinear taps image_eval blends at uv: the offset of each tap's
+// first channel in the texture's own texel order (row-major, channels
+// interleaved) and its weight.
+__device__ __forceinline__ void image_taps(const DevTexture &t, f2 uv, uint32_t (&off)[4], float (&w)[4])
+{
+    float tu = uv.x * t.su + t.ou;
+    if (tu < 0.0f) tu = gmax(0.0f, tu + fabsf(floorf(tu)));
+    float tv = uv.y * t.sv + t.ov;
+    if (tv < 0.0f) tv = gmax(0.0f, tv + fabsf(floorf(tv)));
+    const float cx = tu * (float)t.width - 0.5f, cy = tv * (float)t.height - 0.5f;
+    const int x0 = (int)floorf(cx), y0 = (int)floorf(cy);
+    const float fx = cx - (float)x0, fy = cy - (float)y0;
+    w[0] = (1.0f - fx) * (1.0f - fy);
+    w[1] = fx * (1.0f - fy);
+    w[2] = (1.0f - fx) * fy;
+    w[3] = fx * fy;
+    const int wd = (int)t.width, ht = (int)t.height;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int x = x0 + (k & 1), y = y0 + (k >> 1);
+        if (t.wrap == YMRT_WRAP_CLAMP) {
+            x = x < 0 ? 0 : (x > wd - 1 ? wd - 1 : x);
+            y = y < 0 ? 0 : (y > ht - 1 ? ht - 1 : y);
+        } else {
+            x = ((x % wd) + wd) % wd;
+            y = ((y % ht) + ht) % ht;
+        }
+        off[k] = ((uint32_t)y * t.width + (uint32_t)x) * t.channels;
+    }
+}
+
 // Perlin noise (PerlinNoiseEval, src/TextureEval.cuh:55-131): Ken Perlin's
 // improved noise over the reference's lattice permutation and its 16
 // gradient cases.
@@ -474,6 +505,41 @@ YM_HD float ggx_g2(f3 a, f3 b, float ax, float ay)
 {
     return 1.0f / (1.0f + ggx_lambda(a, ax, ay) + ggx_lambda(b, ax, ay));
 }
+// d ln f / d r of an isotropic GGX conductor lobe at (wo, wi) in the shading
+// frame, alpha = r^2 (InitBSDFSettings' clamp to (0.001, 0.999) inside: the
+// derivative is 0 where it holds).  f = F D G2 / (4 cos_o cos_i) depends on
+// alpha through D = 1 / (pi a^2 (s / a^2 + hz^2)^2), s = hx^2 + hy^2 of the
+// unit half vector, and G2 = 1 / (1 + Lambda(wo) + Lambda(wi)) with
+// Lambda(w) = (sqrt(1 + a^2 tan^2(w)) - 1) / 2; Fresnel does not depend on
+// it.  d ln D / da = -2 / a + 4 s / (a^3 q) with q = s / a^2 + hz^2, d Lambda
+// / da = a tan^2 / (2 sqrt(1 + a^2 tan^2)), d / dr = 2 r d / da.  Double
+// precision: it is summed over many paths.
+__device__ __forceinline__ float ggx_rough_dlog(float r, f3 wo, f3 wi)
+{
+    if (!(r > 0.001f && r < 0.999f) || !(wo.z > 0.0f && wi.z > 0.0f)) return 0.0f;
+    const double a = (double)r * r;
+    double hx = (double)wo.x + wi.x, hy = (double)wo.y + wi.y, hz = (double)wo.z + wi.z;
+    const double hl = sqrt(hx * hx + hy * hy + hz * hz);
+    if (hl < 1e-12) return 0.0f;
+    hx /= hl;
+    hy /= hl;
+    hz /= hl;
+    const double sh = hx * hx + hy * hy, q = sh / (a * a) + hz * hz;
+    const double dlnd = -2.0 / a + 4.0 * sh / (a * a * a * q);
+    double lam = 0.0, dlam = 0.0;
+    const f3 ws[2] = {wo, wi};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double wz = ws[k].z;
+        if (fabs(wz) < 1e-8) continue;
+        const double t2 = ((double)ws[k].x * ws[k].x + (double)ws[k].y * ws[k].y) / (wz * wz);
+        const double root = sqrt(1.0 + a * a * t2);
+        lam += 0.5 * (root - 1.0);
+        dlam += 0.5 * a * t2 / root;
+    }
+    return (float)((dlnd - dlam / (1.0 + lam)) * 2.0 * r);
+}
+
 __device__ __forceinline__ f3 ggx_sample_wh(f3 wo, float u0, float u1, float ax, float ay)
 {
     f3 wh = normalize(f3{ax * wo.x, ay * wo.y, wo.z});
@@ -1097,6 +1163,19 @@ __device__ __forceinline__ float grid_density(const DevScene &sc, int gi, f3 p)
     return sc.voxels[g.offset + ((uint64_t)iz * g.ny + iy) * g.nx + ix];
 }
 
+// The voxel of grid gi that grid_density reads at object point p, or -1
+// outside the unit cube.
+__device__ __forceinline__ int grid_voxel(const DevScene &sc, int gi, f3 p)
+{
+    const DevGrid g = sc.grids[gi];
+    if (!(p.x >= 0.0f && p.x < 1.0f && p.y >= 0.0f && p.y < 1.0f && p.z >= 0.0f && p.z < 1.0f)) return -1;
+    uint32_t ix = (uint32_t)(p.x * (float)g.nx), iy = (uint32_t)(p.y * (float)g.ny), iz = (uint32_t)(p.z * (float)g.nz);
+    if (ix >= g.nx) ix = g.nx - 1;
+    if (iy >= g.ny) iy = g.ny - 1;
+    if (iz >= g.nz) iz = g.nz - 1;
+    return (int)((iz * g.ny + iy) * g.nx + ix);
+}
+
 // Heterogeneous media: a voxel grid, or a texture graph as the reference's
 // density_texture_idx (SampleVolumeScattering's test, src/VolumeEval.cuh:231-238).
 __device__ __forceinline__ bool vol_is_het(const DevScene &sc, int v)
@@ -1148,6 +1227,17 @@ struct HashRng {
     }
 };
 
+// Gradient events of the delta and ratio trackers (the adjoint's replay
+// pass): note() sees each heterogeneous volume's object-space point at a
+// tentative collision, then real() or null() gets the collision's kind with
+// the summed sigma_t (channel x) there and the majorant.  NoTrackEvents
+// compiles to nothing.
+struct NoTrackEvents {
+    __device__ __forceinline__ void note(int, f3) {}
+    __device__ __forceinline__ void real(float, float) {}
+    __device__ __forceinline__ void null(float, float) {}
+};
+
 // src/VolumeEval.cuh:18-78
 __device__ __forceinline__ bool sample_homogeneous(const DevScene &sc, int n, const int *vi, float tmax, f3 &trw,
                                                    float &dist, Sampler &s)
@@ -1190,9 +1280,10 @@ __device__ __forceinline__ bool sample_homogeneous(const DevScene &sc, int n, co
 }
 
 // src/VolumeEval.cuh:80-214 with voxel-grid or texture densities
-template <bool kTex>
+template <bool kTex, typename Ev = NoTrackEvents>
 __device__ __forceinline__ bool sample_heterogeneous(const DevScene &sc, int n, const int *vi, f3 ro_w, f3 rd_w,
-                                                     float tmax, f3 &trw, float &dist, Sampler &s, float *tex_stk)
+                                                     float tmax, f3 &trw, float &dist, Sampler &s, float *tex_stk,
+                                                     Ev &&ev = Ev{})
 {
     trw = s3(1.0f);
     dist = kFloatMax;
@@ -1238,7 +1329,10 @@ __device__ __forceinline__ bool sample_heterogeneous(const DevScene &sc, int n, 
             if (i >= n) continue;
             const DevVolume &v = sc.volumes[vi[i]];
             float dens = 1.0f;
-            if (vol_is_het(sc, vi[i])) dens = vol_density<kTex>(sc, v, ro_w + rd_w * acc, ro[i] + rd[i] * acc, tex_stk);
+            if (vol_is_het(sc, vi[i])) {
+                dens = vol_density<kTex>(sc, v, ro_w + rd_w * acc, ro[i] + rd[i] * acc, tex_stk);
+                ev.note(vi[i], ro[i] + rd[i] * acc);
+            }
             st = st + vsig(v) * dens;
             ss = ss + (vsig(v) * valb(v)) * dens;
         }
@@ -1251,8 +1345,10 @@ __device__ __forceinline__ bool sample_heterogeneous(const DevScene &sc, int n, 
         if (sampler_next(sc, s) < comp(rp, ch)) {
             tr = tr * ((ptr * ss) * nf);
             pdf = pdf * (((ptr * sm) * rp) * nf);
+            ev.real(st.x, sm.x);
             break;
         }
+        ev.null(st.x, sm.x);
         tr = tr * (((ptr * sm) * fp) * nf);
         pdf = pdf * (((ptr * sm) * fp) * nf);
         ++it;
@@ -1267,9 +1363,9 @@ __device__ __forceinline__ bool sample_heterogeneous(const DevScene &sc, int n, 
 // kHet: 0 every medium is homogeneous (only the closed form is compiled),
 // 1 voxel-grid media, 2 texture-density media as well (tex_stk: the texture
 // graph's LDS stack).
-template <int kHet>
+template <int kHet, typename Ev = NoTrackEvents>
 __device__ __forceinline__ f3 eval_tr(const DevScene &sc, f3 ro_w, f3 rd_w, const Transfer &t, float tmax,
-                                      HashRng &rng, float *tex_stk)
+                                      HashRng &rng, float *tex_stk, Ev &&ev = Ev{})
 {
     if (sc.volume_count == 0) return s3(1.0f);
     int vi[kMaxBoundaryRecord];
@@ -1334,10 +1430,14 @@ __device__ __forceinline__ f3 eval_tr(const DevScene &sc, f3 ro_w, f3 rd_w, cons
             if (i >= n) continue;
             const DevVolume &v = sc.volumes[vi[i]];
             float dens = 1.0f;
-            if (vol_is_het(sc, vi[i]))
+            if (vol_is_het(sc, vi[i])) {
                 dens = vol_density<kHet == 2>(sc, v, ro_w + rd_w * acc, ro[i] + rd[i] * acc, tex_stk);
+                ev.note(vi[i], ro[i] + rd[i] * acc);
+            }
             st = st + vsig(v) * dens;
         }
+        // ratio tracking's factor 1 - sigma / sigma_m at this step
+        ev.null(st.x, sm.x);
         f3 rp;
         rp.x = gmin(sm.x > 0.0f ? st.x * smr.x : 0.0f, 1.0f);
         rp.y = gmin(sm.y > 0.0f ? st.y * smr.y : 0.0f, 1.0f);

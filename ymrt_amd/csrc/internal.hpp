@@ -34,6 +34,7 @@ void set_error(const std::string &msg);
     } while (0)
 
 struct RenderBuffers;  // buffers.hpp
+struct AdjParams;      // pt.hpp
 struct BatchMemory;    // batch.cpp
 void free_batch_memory(BatchMemory *m);
 
@@ -55,10 +56,12 @@ struct ymrt_scene {
     bool needs_nearby = false;    // boundaries, dielectrics or ior priorities present
     ym::RenderBuffers *render = nullptr;
     uint32_t *d_error = nullptr;  // device flag: traversal stack overflow
-    const uint32_t *adj_mat = nullptr;  // per material: 1 when Lambert-only (adjoint)
     // host state that ymrt_scene_update_materials re-derives flags from
     std::vector<ym::DevMaterial> materials;
     std::vector<ym::DevTexture> textures;  // texture records (float texel updates)
+    std::vector<ym::DevVolume> volumes;    // volume and grid records (adjoint targets)
+    std::vector<ym::DevGrid> grids;
+    int32_t *d_adj_slots = nullptr;        // adjoint: 2 x material_count target slots
     uint32_t texture_count = 0;
     bool has_boundaries = false;
     bool procedural_textures = false;
@@ -84,9 +87,14 @@ void destroy_render_buffers(ymrt_scene *sc);
 void free_render_buffers(RenderBuffers *rb);
 // One frame with the path-state buffers in `buffers` (allocated or grown as
 // the frame needs; the scene's own for the single-frame entry points).
+// adj: the adjoint's targets (a replay pass follows each chunk), or null.
 int render_frame_with(ymrt_scene *sc, RenderBuffers *&buffers, const ymrt_render_settings *rs, int frame,
-                      float *noise, uint32_t *instance_ids, const float *adj_img, float *adj_grad,
-                      hipStream_t stream, int timing, ymrt_frame_stats *stats);
+                      float *noise, uint32_t *instance_ids, const AdjParams *adj, hipStream_t stream, int timing,
+                      ymrt_frame_stats *stats);
+// adjoint.cpp: checks the targets of ymrt_render_adjoint against the scene and
+// fills the replay's parameters (uploading the per-material target slots)
+int prepare_adjoint(ymrt_scene *sc, const ymrt_adjoint_targets *tg, const float *adj_img, hipStream_t stream,
+                    AdjParams *out);
 
 // abi.cpp: the traversal-overflow flag of a scene, cleared before a call's
 // kernels and read once its stream has drained

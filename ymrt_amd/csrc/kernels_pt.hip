@@ -143,6 +143,122 @@ __device__ __forceinline__ void wave_add64(unsigned long long *counter, uint32_t
     if (lane_id() == 0 && s) atomicAdd(counter, (unsigned long long)s);
 }
 
+// ------------------------------------------------------------ adjoint replay
+// The replay pass of ymrt_render_adjoint runs the frame's paths a second time
+// with the same samples (every random number is keyed by pixel, sample and
+// vertex), so each path takes the same vertices and ps.out[p] already holds
+// its radiance L_path from the first pass.  A gradient event whose factor
+// multiplies every deposit after it adds adj . (L_path - L_now) * w; a term
+// of one shadow ray's deposit D adds adj . D * w; adj is the pixel's
+// dLoss/dImage over spp.  Sampling pdfs and MIS weights stay fixed.
+
+// Pixel adjoint / spp of path slot p (64-bit slot arithmetic as k_raygen).
+__device__ __forceinline__ f3 adj_pixel(const PtParams &prm, uint32_t p)
+{
+    const uint64_t gp = prm.chunk_base + p;
+    const uint32_t spp = (uint32_t)prm.rs.spp;
+    const uint64_t vpix = gp / spp;
+    const uint32_t tile = (uint32_t)(vpix >> 8), within = (uint32_t)(vpix & 255u);
+    const uint32_t ty = tile / prm.tiles_x, tx = tile - ty * prm.tiles_x;
+    const uint32_t px = tx * 16u + (within & 15u), py = ty * 16u + (within >> 4);
+    const float *a = prm.adj.img + ((size_t)py * prm.rs.width + px) * 3;
+    const float inv = 1.0f / (float)spp;
+    return f3{a[0] * inv, a[1] * inv, a[2] * inv};
+}
+
+// Gradient sums of one wave in a replay kernel: albedo rows and roughness
+// entries go to LDS (kAdjLds floats when they fit) and reach global memory
+// with one atomic per nonzero entry when the wave's work ends; texels and
+// voxels go straight to global memory (their targets are spread out).
+// kOn false: nothing is compiled.
+template <bool kOn>
+struct AdjSums {
+    float *lds;
+    const AdjParams *A;
+    __device__ __forceinline__ void begin()
+    {
+        if constexpr (kOn) {
+            if (A->lds) {
+                for (uint32_t i = lane_id(); i < kAdjLds; i += kBlock) lds[i] = 0.0f;
+                __syncthreads();
+            }
+        }
+    }
+    __device__ __forceinline__ void add(uint32_t i, float v)
+    {
+        if constexpr (kOn) {
+            if (v == 0.0f) return;
+            if (A->lds) atomicAdd(&lds[i], v);
+            else if (i < A->n_alb * 3u) unsafeAtomicAdd(&A->g_alb[i], v);
+            else unsafeAtomicAdd(&A->g_rough[i - A->n_alb * 3u], v);
+        }
+    }
+    __device__ __forceinline__ void albedo(uint32_t row, int c, float v) { add(row * 3u + (uint32_t)c, v); }
+    __device__ __forceinline__ void roughness(uint32_t entry, float v) { add(A->n_alb * 3u + entry, v); }
+    __device__ __forceinline__ void end()
+    {
+        if constexpr (kOn) {
+            if (A->lds) {
+                __syncthreads();
+                const uint32_t n = A->n_alb * 3u + A->n_rough;
+                for (uint32_t i = lane_id(); i < n; i += kBlock) {
+                    const float v = lds[i];
+                    if (v == 0.0f) continue;
+                    if (i < A->n_alb * 3u) unsafeAtomicAdd(&A->g_alb[i], v);
+                    else unsafeAtomicAdd(&A->g_rough[i - A->n_alb * 3u], v);
+                }
+            }
+        }
+    }
+};
+
+// Density events of the trackers for the target volume's voxel grid: a real
+// collision (probability sigma / sigma_m) adds scale * tsig / sigma, a null
+// collision or a ratio-tracking step (1 - sigma / sigma_m) adds scale *
+// -tsig / (sigma_m - sigma), tsig the target's sigma_t (gray); at most
+// `left` events.
+struct AdjTrackEvents {
+    const DevScene &sc;
+    float *g_vox;
+    uint32_t vol;
+    int grid;
+    float tsig, scale;
+    uint32_t left;
+    int vox;
+    __device__ __forceinline__ void note(int v, f3 po)
+    {
+        if ((uint32_t)v == vol) vox = grid_voxel(sc, grid, po);
+    }
+    __device__ __forceinline__ void real(float st, float sm)
+    {
+        (void)sm;
+        if (vox >= 0 && st > 0.0f && left) {
+            unsafeAtomicAdd(&g_vox[vox], scale * (tsig / st));
+            --left;
+        }
+        vox = -1;
+    }
+    __device__ __forceinline__ void null(float st, float sm)
+    {
+        if (vox >= 0 && sm - st > 0.0f && left) {
+            unsafeAtomicAdd(&g_vox[vox], scale * (-tsig / (sm - st)));
+            --left;
+        }
+        vox = -1;
+    }
+};
+
+__device__ __forceinline__ AdjTrackEvents adj_track(const DevScene &sc, const AdjParams &A, float scale,
+                                                    uint32_t limit)
+{
+    const DevVolume &v = sc.volumes[A.vol];
+    return AdjTrackEvents{sc, A.g_vox, A.vol, v.grid, v.sigma_t[0], scale, limit, -1};
+}
+
+// A shadow ray's transmittance records at most this many events; later ones
+// are dropped (the CPU restatement of the adjoint drops the same).
+constexpr uint32_t kAdjRayEvents = 1024;
+
 // ------------------------------------------------------------ frame slots
 // Multi-frame launches (ymrt_render_batch): one launch of a pass kernel serves
 // every frame chunk of a round, each chunk a slot with its own scene,
@@ -344,7 +460,6 @@ __device__ __forceinline__ void raygen_body(const DevScene &sc, const PtParams &
                 ps.L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 store_sampler(ps, p, smp);
                 ps.info[p] = 1u | kInfoNeverScatter;
-                if (prm.adj_mat) ps.adj_n[p] = 0;
                 Transfer t;
                 t.count = cam.record_count;
                 t.max_idx = cam.max_idx;
@@ -647,7 +762,9 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact_scatter_m(const Fra
 // kHet: 1 the scene has voxel-grid media (delta tracking), 2 texture-density
 // media as well (the texture graph evaluator and its LDS stack are compiled
 // in); 0 every medium is homogeneous and the tracker is compiled out.
-template <int kHet>
+// kAdj (the adjoint's replay): the delta tracker's collisions in the target
+// grid volume are density gradient events.
+template <int kHet, bool kAdj = false>
 __device__ __forceinline__ void medium_body(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass)
 {
     constexpr bool kGrid = kHet > 0;
@@ -677,7 +794,21 @@ __device__ __forceinline__ void medium_body(const DevScene &sc, const PtParams &
                 }
                 f3 trw;
                 const float tmax = hit ? __uint_as_float(ha.x) : kTMax;
-                if constexpr (kGrid) {
+                if constexpr (kGrid && kAdj) {
+                    // every later deposit of the path depends on this flight
+                    float scale = 0.0f;
+                    if (het && prm.adj.vol != kEmpty) {
+                        const f3 dl = adj_pixel(prm, p) * (xyz(ps.out[p]) - xyz(ps.L[p]));
+                        scale = dl.x + dl.y + dl.z;
+                    }
+                    if (scale != 0.0f) {
+                        AdjTrackEvents ev = adj_track(sc, prm.adj, scale, 0xFFFFFFFFu);
+                        vol_hit = sample_heterogeneous<kHet == 2>(sc, nv, vi, o, d, tmax, trw, dist, smp, tex_stk, ev);
+                    } else {
+                        vol_hit = het ? sample_heterogeneous<kHet == 2>(sc, nv, vi, o, d, tmax, trw, dist, smp, tex_stk)
+                                      : sample_homogeneous(sc, nv, vi, tmax, trw, dist, smp);
+                    }
+                } else if constexpr (kGrid) {
                     vol_hit = het ? sample_heterogeneous<kHet == 2>(sc, nv, vi, o, d, tmax, trw, dist, smp, tex_stk)
                                   : sample_homogeneous(sc, nv, vi, tmax, trw, dist, smp);
                 } else {
@@ -704,18 +835,48 @@ __device__ __forceinline__ void medium_body(const DevScene &sc, const PtParams &
 }
 
 
-template <int kHet>
+template <int kHet, bool kAdj = false>
 __global__ __launch_bounds__(kBlock) void k_medium_m(const FrameSlot *__restrict__ fs, uint32_t nfs, uint32_t pass)
 {
     for_slots(
         fs, nfs, [&](const FrameSlot &f) { return pass_counter(f, pass, kCtrMedium); },
         [&](const FrameSlot &f) { return pass_drained(f, pass, kCtrWorkMedium, kCtrMedium); },
-        [&](const FrameSlot &f) { medium_body<kHet>(f.sc, f.prm, f.ps, pass); });
+        [&](const FrameSlot &f) { medium_body<kHet, kAdj>(f.sc, f.prm, f.ps, pass); });
 }
 
 
 
 // ------------------------------------------------------------ shade
+// Adjoint replay, a surface vertex whose diffuse albedo is the target image
+// texture (every material using it is Lambert-only): the albedo is a bilinear
+// blend of four texels, so each tap's texel channel c gets dl_c * w / albedo_c
+// (a one-channel texture: the sum over the colour channels).
+__device__ __forceinline__ void adj_texels(const DevScene &sc, const AdjParams &A, f2 uv, f3 dl)
+{
+    const DevTexture &t = sc.textures[A.tex];
+    const f3 al = image_eval(sc, t, uv);
+    uint32_t off[4];
+    float w[4];
+    image_taps(t, uv, off, w);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (!(w[k] > 0.0f)) continue;
+        if (t.channels == 1) {
+            const float v = (al.x > 0.0f ? dl.x * (w[k] / al.x) : 0.0f) + (al.y > 0.0f ? dl.y * (w[k] / al.y) : 0.0f) +
+                            (al.z > 0.0f ? dl.z * (w[k] / al.z) : 0.0f);
+            if (v != 0.0f) unsafeAtomicAdd(&A.g_tex[off[k]], v);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float a = comp(al, c);
+                if (!(a > 0.0f)) continue;
+                const float v = comp(dl, c) * (w[k] / a);
+                if (v != 0.0f) unsafeAtomicAdd(&A.g_tex[off[k] + (uint32_t)c], v);
+            }
+        }
+    }
+}
+
 // Records shadow ray k of path p (origin, direction, maximum distance, BSDF or
 // phase value, clamped Li, MIS weight) straight to its slot in HBM.  The
 // direction is renormalised as the reference's Ray constructor does
@@ -726,13 +887,15 @@ struct ShadowEmit {
     const PathState &ps;
     uint32_t cap, p;
     uint32_t flags;
-    __device__ __forceinline__ void emit(int k, f3 orig, f3 dir, float tmax, bool transmit, f3 f, f3 li, float mis)
+    // g: the adjoint replay's d ln f / d r of the ray's BSDF value (0 otherwise)
+    __device__ __forceinline__ void emit(int k, f3 orig, f3 dir, float tmax, bool transmit, f3 f, f3 li, float mis,
+                                         float g = 0.0f)
     {
         const size_t o = (size_t)k * cap + p;
         ps.sh_o[o] = f4(orig, tmax);
         ps.sh_d[o] = f4(normalize(dir), 0.0f);
         ps.sh_f[o] = f4(f, mis);
-        ps.sh_li[o] = f4(li, 0.0f);
+        ps.sh_li[o] = f4(li, g);
         flags |= (1u << k) | (transmit ? (kShTransmit0 << k) : 0u);
     }
 };
@@ -757,10 +920,15 @@ __device__ __forceinline__ float shadow_tmax(f3 pos, f3 ldir, f3 lpos, f3 lerr, 
 // SIMD, a few spilled registers): the shading stage waits on dependent
 // loads, and the third wave hides more of them than the spills cost (6 %
 // faster on the interior frame; 4 waves spill too much).
-template <bool kVolume, bool kGraph, uint32_t kClass = kMatGeneral>
+// kAdj: the adjoint's replay pass (surface vertices): albedo, texel and
+// roughness gradient events.
+template <bool kVolume, bool kGraph, uint32_t kClass = kMatGeneral, bool kAdj = false>
 __device__ __forceinline__ void shade_body(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass)
 {
     __shared__ float tex_stk[(kVolume || !kGraph) ? 1 : kTexStack * 3 * kBlock];
+    __shared__ float adj_lds[kAdj ? kAdjLds : 1];
+    AdjSums<kAdj> sums{adj_lds, &prm.adj};
+    sums.begin();
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     uint32_t *cnt_next = prm.counters + (pass + 1) * kCtrStride;
     constexpr bool kDiffuse = kClass == kMatDiffuse, kMetal = kClass == kMatMetal;
@@ -1006,12 +1174,29 @@ __device__ __forceinline__ void shade_body(const DevScene &sc, const PtParams &p
                     init_shading_space(bs, tx, normal_tex, bump_tex, back, ray_d, ng, sn, dpdu, dpdv, in.w2o, duv);
                     bs.mask = 0;
                     if (is_default) init_default_bsdf<kClass>(tx, bs, m, ray_d, back, ray_ior, ex_ior);
-                    if (prm.adj_mat && prm.adj_mat[in.material]) {
-                        // every later term of the path carries this vertex's albedo once
-                        const uint32_t k = ps.adj_n[p];
-                        ps.adj_rec[(size_t)k * cap + p] =
-                            make_uint4(in.material, __float_as_uint(L.x), __float_as_uint(L.y), __float_as_uint(L.z));
-                        ps.adj_n[p] = k + 1;
+                    // adjoint replay: an albedo or texel factor of this vertex
+                    // scales every later deposit of the path (next-event ones
+                    // included); a target roughness scales each next-event ray
+                    // by its own BSDF value and the continuation by its own
+                    int32_t rough_entry = -1;
+                    float rough_r = 0.0f, adj_gc = 0.0f;
+                    f3 adj_dl = s3(0.0f);
+                    if constexpr (kAdj) {
+                        const AdjParams &A = prm.adj;
+                        const int32_t as = A.alb_slot[in.material];
+                        const bool tex_on = A.tex != kEmpty && m.diffuse_albedo_tex == A.tex;
+                        rough_entry = A.rough_slot[in.material];
+                        if (as >= 0 || tex_on || rough_entry >= 0)
+                            adj_dl = adj_pixel(prm, p) * (xyz(ps.out[p]) - L);
+                        if (as >= 0) {
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) {
+                                const float al = m.diffuse_albedo[c];
+                                if (al > 0.0f) sums.albedo((uint32_t)as, c, comp(adj_dl, c) / al);
+                            }
+                        }
+                        if (tex_on) adj_texels(sc, A, uv, adj_dl);
+                        if (rough_entry >= 0) rough_r = m.roughness_x;
                     }
                     const f3 wo = bs.to_local(-ray_d);
                     if (rs.enable_distant_light && sc.distant_count > 0) {
@@ -1031,7 +1216,8 @@ __device__ __forceinline__ void shade_body(const DevScene &sc, const PtParams &p
                                 if (!invalid)
                                     so.emit(2, offset_ray_origin(pos, perr, ldir, ng), ldir, kTMax, transmit,
                                             min3(f, s3(kMaxColorClamp)), min3(Li, s3(kMaxColorClamp)),
-                                            power_heuristic(lpdf, wpdf));
+                                            power_heuristic(lpdf, wpdf),
+                                            rough_entry >= 0 ? ggx_rough_dlog(rough_r, wo, lwi) : 0.0f);
                             }
                         }
                         {
@@ -1051,7 +1237,8 @@ __device__ __forceinline__ void shade_body(const DevScene &sc, const PtParams &p
                                     if (!invalid)
                                         so.emit(3, offset_ray_origin(pos, perr, ldir, ng), ldir, kTMax, transmit,
                                                 min3(bw, s3(kMaxColorClamp)), min3(Li, s3(kMaxColorClamp)),
-                                                power_heuristic(bpdf, lpdf));
+                                                power_heuristic(bpdf, lpdf),
+                                                rough_entry >= 0 ? ggx_rough_dlog(rough_r, wo, bwi) : 0.0f);
                                 }
                             }
                         }
@@ -1075,7 +1262,8 @@ __device__ __forceinline__ void shade_body(const DevScene &sc, const PtParams &p
                                     so.emit(0, offset_ray_origin(pos, perr, ldir, ng), ldir,
                                             shadow_tmax(pos, ldir, lpos, lerr, lng), transmit,
                                             min3(f, s3(kMaxColorClamp)), min3(Li, s3(kMaxColorClamp)),
-                                            power_heuristic(lpdf, wpdf));
+                                            power_heuristic(lpdf, wpdf),
+                                            rough_entry >= 0 ? ggx_rough_dlog(rough_r, wo, lwi) : 0.0f);
                             }
                         }
                         {
@@ -1097,7 +1285,8 @@ __device__ __forceinline__ void shade_body(const DevScene &sc, const PtParams &p
                                         so.emit(1, offset_ray_origin(pos, perr, ldir, ng), ldir,
                                                 shadow_tmax(pos, ldir, lpos, lerr, lng), transmit,
                                                 min3(bw, s3(kMaxColorClamp)), min3(Li, s3(kMaxColorClamp)),
-                                                power_heuristic(bpdf, lpdf));
+                                                power_heuristic(bpdf, lpdf),
+                                                rough_entry >= 0 ? ggx_rough_dlog(rough_r, wo, bwi) : 0.0f);
                                 }
                             }
                         }
@@ -1108,6 +1297,12 @@ __device__ __forceinline__ void shade_body(const DevScene &sc, const PtParams &p
                     const float u1 = sampler_next(sc, smp);
                     const float u2 = sampler_next(sc, smp);
                     if (bsdf_sample<kLobes>(bs, wo, bw, wi, pdf, u0, u1, u2)) {
+                        if constexpr (kAdj) {
+                            if (rough_entry >= 0) {
+                                adj_gc = ggx_rough_dlog(rough_r, wo, wi);
+                                sums.roughness((uint32_t)rough_entry, (adj_dl.x + adj_dl.y + adj_dl.z) * adj_gc);
+                            }
+                        }
                         beta = beta * bw;
                         const float rr = gmin(gmax(beta.x, gmax(beta.y, beta.z)), 1.0f);
                         bool cont = true;
@@ -1131,6 +1326,11 @@ __device__ __forceinline__ void shade_body(const DevScene &sc, const PtParams &p
                             }
                         }
                     }
+                    // the shadow stage subtracts the continuation's factor from
+                    // each next-event ray's own
+                    if constexpr (kAdj)
+                        ps.adj_g[p] = make_uint2(rough_entry >= 0 ? (uint32_t)rough_entry : kEmpty,
+                                                 __float_as_uint(adj_gc));
                 }
             }
             // ---------------------------------------------- write back
@@ -1157,10 +1357,11 @@ __device__ __forceinline__ void shade_body(const DevScene &sc, const PtParams &p
             ps.info[p] = info;
         }
     });
+    sums.end();
 }
 
 
-template <bool kVolume, bool kGraph, uint32_t kClass = kMatGeneral>
+template <bool kVolume, bool kGraph, uint32_t kClass = kMatGeneral, bool kAdj = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGraph ? 2 : 3))) void k_shade_m(const FrameSlot *__restrict__ fs, uint32_t nfs, uint32_t pass)
 {
     constexpr uint32_t kQueue = kVolume ? kCtrHitVol : kClass == kMatDiffuse ? kCtrHitDiffuse
@@ -1170,7 +1371,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGraph ?
     for_slots(
         fs, nfs, [&](const FrameSlot &f) { return pass_counter(f, pass, kQueue); },
         [&](const FrameSlot &f) { return pass_drained(f, pass, kWork, kQueue); },
-        [&](const FrameSlot &f) { shade_body<kVolume, kGraph, kClass>(f.sc, f.prm, f.ps, pass); });
+        [&](const FrameSlot &f) { shade_body<kVolume, kGraph, kClass, kAdj>(f.sc, f.prm, f.ps, pass); });
 }
 
 
@@ -1181,13 +1382,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGraph ?
 // light-selection pdf, clamped and deposited, distant light first
 // (EvalDistantLight, src/Shading.cu:83-198, then EvalShapeLight 294-417, as
 // the shading stage adds them at 2217-2240).  with_tr: the transmittance of
-// each ray is in sh_tr (paths in media), otherwise 1.
+// each ray is in sh_tr (paths in media), otherwise 1.  kAdj (the adjoint's
+// replay): at a surface vertex with a target roughness, each ray's deposit D
+// adds adj . D * (g_ray - g_continuation) to that roughness's gradient (the
+// continuation's event counted D with the continuation's factor).
+template <bool kAdj = false>
 __device__ __forceinline__ void shadow_finish(const PtParams &prm, const PathState &ps, uint32_t p, uint32_t fl,
-                                              uint32_t distant_count, bool with_tr)
+                                              uint32_t distant_count, bool with_tr, AdjSums<kAdj> *sums = nullptr)
 {
     const uint32_t cap = prm.cap;
     f3 shape = s3(0.0f), dist = s3(0.0f);
+    f3 rc[kShadowSlots];
+    float rg[kShadowSlots];
     static_for<0, kShadowSlots>([&](auto K) {
+        rc[K()] = s3(0.0f);
+        rg[K()] = 0.0f;
         if ((fl & (1u << K())) && (fl & (kShClear0 << K()))) {
             const size_t o = (size_t)K() * cap + p;
             const float4 f4v = ps.sh_f[o];
@@ -1196,10 +1405,30 @@ __device__ __forceinline__ void shadow_finish(const PtParams &prm, const PathSta
             const f3 c = ((f4v.w * tr) * xyz(f4v)) * xyz(li4);
             if (K() < 2) shape = shape + c;
             else dist = dist + c;
+            if constexpr (kAdj) {
+                rc[K()] = c;
+                rg[K()] = li4.w;
+            }
         }
     });
     const float4 c4 = ps.sh_c[p];
     const f3 beta = xyz(c4);
+    if constexpr (kAdj) {
+        if (!(fl & kShVolume)) {
+            const uint2 ag = ps.adj_g[p];
+            if (ag.x != kEmpty) {
+                const float gc = __uint_as_float(ag.y);
+                const f3 a = adj_pixel(prm, p);
+                float acc = 0.0f;
+                static_for<0, kShadowSlots>([&](auto K) {
+                    const float scale = K() < 2 ? safe_rcp(c4.w) : (float)distant_count;
+                    const f3 d = beta * (rc[K()] * scale);
+                    acc += dot(a, d) * (rg[K()] - gc);
+                });
+                sums->roughness(ag.x, acc);
+            }
+        }
+    }
     const int depth = (int)((fl >> kShDepthShift) & 0xFFu);
     f3 L = xyz(ps.L[p]);
     if (distant_count > 0) L = L + light_clamp(prm.rs, beta * (dist / (1.0f / (float)distant_count)), depth);
@@ -1215,11 +1444,14 @@ __device__ __forceinline__ void shadow_finish(const PtParams &prm, const PathSta
 // (BVHTraverseShadow, src/ShadingUtilities.cuh:563-635).  In scenes with media,
 // paths with an unoccluded ray go on to k_transmit for the transmittance along
 // it; the others are finished here.
-template <bool kMedia, bool kSphere, int kStack>
+template <bool kMedia, bool kSphere, int kStack, bool kAdj = false>
 __device__ __forceinline__ void shadow_body(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass)
 {
     __shared__ uint32_t stack[kStackWindow * kBlock];
     __shared__ float pre_park[kPreSlots * kBlock];
+    __shared__ float adj_lds[kAdj ? kAdjLds : 1];
+    AdjSums<kAdj> sums{adj_lds, &prm.adj};
+    sums.begin();
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     const uint32_t cap = prm.cap;
     LaneScratch<kBlock, kStackWindow, true> ls{stack, nullptr, nullptr, nullptr, (int)lane_id(), ps.spill,
@@ -1276,25 +1508,26 @@ __device__ __forceinline__ void shadow_body(const DevScene &sc, const PtParams &
                     ps.sh_flags[p] = fl;
                     ps.flag[p] = (pass << kFlagBits) | kFlagTransmit;
                 } else {
-                    shadow_finish(prm, ps, p, fl, sc.distant_count, false);
+                    shadow_finish<kAdj>(prm, ps, p, fl, sc.distant_count, false, &sums);
                 }
             }
         }
         if (!rr.more && __ballot(r != kLaneIdle) == 0) break;
     }
+    sums.end();
     wave_add64(prm.stats + kStatShadow, nsh);
     wave_add64(prm.stats + kStatShadowNodes, tc.nodes);
     wave_add64(prm.stats + kStatShadowTris, tc.tris);
 }
 
 
-template <bool kMedia, bool kSphere, int kStack>
+template <bool kMedia, bool kSphere, int kStack, bool kAdj = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_shadow_m(const FrameSlot *__restrict__ fs, uint32_t nfs, uint32_t pass)
 {
     for_slots(
         fs, nfs, [&](const FrameSlot &f) { return pass_counter(f, pass, kCtrShadow); },
         [&](const FrameSlot &f) { return pass_drained(f, pass, kCtrWorkShadow, kCtrShadow); },
-        [&](const FrameSlot &f) { shadow_body<kMedia, kSphere, kStack>(f.sc, f.prm, f.ps, pass); });
+        [&](const FrameSlot &f) { shadow_body<kMedia, kSphere, kStack, kAdj>(f.sc, f.prm, f.ps, pass); });
 }
 
 
@@ -1303,8 +1536,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 // (src/VolumeEval.cuh:420-468) walks the boundary instances along the ray,
 // updating the medium records at each crossing and multiplying the segment
 // transmittances (EvalTr, 253-418).
-// kHet as k_medium's.
-template <bool kSphere, int kStack, int kHet>
+// kHet as k_medium's.  kAdj (the adjoint's replay): the shadow_finish
+// roughness terms, and with a target grid volume each ray's ratio-tracking
+// events, found by walking the ray a second time with the same random
+// numbers once its deposit is known.
+template <bool kSphere, int kStack, int kHet, bool kAdj = false>
 __device__ __forceinline__ void transmit_body(const DevScene &sc, const PtParams &prm, const PathState &ps, uint32_t pass)
 {
     __shared__ float tex_stk[kHet == 2 ? kTexStack * 3 * kBlock : 1];
@@ -1319,6 +1555,9 @@ __device__ __forceinline__ void transmit_body(const DevScene &sc, const PtParams
                                          blockIdx.x * kBlock + threadIdx.x};
     uint32_t ntr = 0;
     TraceCounters tc{0, 0};
+    __shared__ float adj_lds[kAdj ? kAdjLds : 1];
+    AdjSums<kAdj> sums{adj_lds, &prm.adj};
+    sums.begin();
     run_items<0>(cnt + kCtrWorkTransmit, n, nullptr, nullptr, [&](uint32_t i, uint32_t &slot, bool(&)[1]) {
         {
             const uint32_t p = ps.q_transmit[i];
@@ -1347,78 +1586,105 @@ __device__ __forceinline__ void transmit_body(const DevScene &sc, const PtParams
             for (int j = 0; j < kShadowSlots; ++j) {
                 const int k = (j + 2) & 3;
                 if (!(fl & (1u << k)) || !(fl & (kShClear0 << k))) continue;
-                const float4 o4 = ps.sh_o[(size_t)k * cap + p];
-                const float4 d4 = ps.sh_d[(size_t)k * cap + p];
-                f3 tr = s3(1.0f);
-                // the vertex's records and nearby list are reloaded per ray
-                // rather than held across the loop (fewer live registers)
-                Transfer t = load_transfer(ps, p, cap);
-                if (fl & (kShTransmit0 << k)) boundary_bunch(sc, t, load_nearby(ps, p, cap, ha.w));
-                const bool volv = (fl & kShVolume) != 0;
-                Query tq;
-                tq.o = xyz(o4);
-                tq.d = xyz(d4);
-                tq.tmin = kTMin;
-                tq.tmax = o4.w;
-                tq.last_inst = volv ? kEmpty : ha.y;
-                tq.last_tri = volv ? kEmpty : ha.z;
-                const float maxd = o4.w;
-                float traced = kTMin;
-                int guard = 0;
-                while (traced < maxd) {
-                    Hit bh;
-                    bh.back = 0;
-                    int bnc = 0;
-                    int br = 0;
-                    ++ntr;
-                    if (sc.has_boundaries)
-                        br = traverse<kModeBoundaryOnly, false, true, true, kSphere, kStack>(sc, tq, bh, bnc, ls, tc);
-                    if (br < 0) {
-                        atomicOr(prm.error, 1u);
-                        break;
+                auto walk = [&](auto &&ev) -> f3 {
+                    const float4 o4 = ps.sh_o[(size_t)k * cap + p];
+                    const float4 d4 = ps.sh_d[(size_t)k * cap + p];
+                    f3 tr = s3(1.0f);
+                    // the vertex's records and nearby list are reloaded per ray
+                    // rather than held across the loop (fewer live registers)
+                    Transfer t = load_transfer(ps, p, cap);
+                    if (fl & (kShTransmit0 << k)) boundary_bunch(sc, t, load_nearby(ps, p, cap, ha.w));
+                    const bool volv = (fl & kShVolume) != 0;
+                    Query tq;
+                    tq.o = xyz(o4);
+                    tq.d = xyz(d4);
+                    tq.tmin = kTMin;
+                    tq.tmax = o4.w;
+                    tq.last_inst = volv ? kEmpty : ha.y;
+                    tq.last_tri = volv ? kEmpty : ha.z;
+                    const float maxd = o4.w;
+                    float traced = kTMin;
+                    int guard = 0;
+                    while (traced < maxd) {
+                        Hit bh;
+                        bh.back = 0;
+                        int bnc = 0;
+                        int br = 0;
+                        ++ntr;
+                        if (sc.has_boundaries)
+                            br = traverse<kModeBoundaryOnly, false, true, true, kSphere, kStack>(sc, tq, bh, bnc, ls, tc);
+                        if (br < 0) {
+                            atomicOr(prm.error, 1u);
+                            break;
+                        }
+                        if (br == 0) {
+                            tr = tr * eval_tr<kHet>(sc, tq.o, tq.d, t, gmax(maxd - traced, 0.0f), rng, tex_stk, ev);
+                            break;
+                        }
+                        tr = tr * eval_tr<kHet>(sc, tq.o, tq.d, t, bh.t, rng, tex_stk, ev);
+                        traced += bh.t;
+                        f3 bp, bperr, bng;
+                        hit_geometry(sc, bh.inst, bh.tri, bh.b0, bh.b1, bh.b2, bp, bperr, bng);
+                        NearbyList bnb;
+                        bnb.n = bnc;
+                        static_for<0, kNearbyMax>([&](auto J) {
+                            bnb.prim[J()] = nbp[J() * kBlock + lane_id()];
+                            bnb.t[J()] = nbt[J() * kBlock + lane_id()];
+                            bnb.back[J()] = nbb[J() * kBlock + lane_id()];
+                        });
+                        boundary_bunch(sc, t, bnb);
+                        // the next segment is a new Ray, direction renormalised
+                        tq.o = offset_ray_origin(bp, bperr, tq.d, bng);
+                        tq.d = normalize(tq.d);
+                        tq.tmax = kTMax;
+                        tq.last_inst = bh.inst;
+                        tq.last_tri = bh.tri;
+                        if (++guard > 64) break;
                     }
-                    if (br == 0) {
-                        tr = tr * eval_tr<kHet>(sc, tq.o, tq.d, t, gmax(maxd - traced, 0.0f), rng, tex_stk);
-                        break;
-                    }
-                    tr = tr * eval_tr<kHet>(sc, tq.o, tq.d, t, bh.t, rng, tex_stk);
-                    traced += bh.t;
-                    f3 bp, bperr, bng;
-                    hit_geometry(sc, bh.inst, bh.tri, bh.b0, bh.b1, bh.b2, bp, bperr, bng);
-                    NearbyList bnb;
-                    bnb.n = bnc;
-                    static_for<0, kNearbyMax>([&](auto J) {
-                        bnb.prim[J()] = nbp[J() * kBlock + lane_id()];
-                        bnb.t[J()] = nbt[J() * kBlock + lane_id()];
-                        bnb.back[J()] = nbb[J() * kBlock + lane_id()];
-                    });
-                    boundary_bunch(sc, t, bnb);
-                    // the next segment is a new Ray, direction renormalised
-                    tq.o = offset_ray_origin(bp, bperr, tq.d, bng);
-                    tq.d = normalize(tq.d);
-                    tq.tmax = kTMax;
-                    tq.last_inst = bh.inst;
-                    tq.last_tri = bh.tri;
-                    if (++guard > 64) break;
-                }
+                    return tr;
+                };
+                const uint32_t ctr0 = rng.ctr;
+                const f3 tr = walk(NoTrackEvents{});
                 ps.sh_tr[(size_t)k * cap + p] = f4(tr, 0.0f);
+                if constexpr (kAdj && kHet > 0) {
+                    if (prm.adj.vol != kEmpty) {
+                        // this ray's deposit, as shadow_finish sums it
+                        const size_t o = (size_t)k * cap + p;
+                        const float4 f4v = ps.sh_f[o];
+                        const float4 li4 = ps.sh_li[o];
+                        const float4 c4 = ps.sh_c[p];
+                        const f3 c = ((f4v.w * tr) * xyz(f4v)) * xyz(li4);
+                        const float scale = k >= 2 ? (float)sc.distant_count
+                                            : (fl & kShVolume) ? (c4.w > 0.0f ? 1.0f / c4.w : 0.0f)
+                                                               : safe_rcp(c4.w);
+                        const float d = dot(adj_pixel(prm, p), xyz(c4) * (c * scale));
+                        if (d != 0.0f) {
+                            const uint32_t ctr1 = rng.ctr;
+                            rng.ctr = ctr0;
+                            AdjTrackEvents ev = adj_track(sc, prm.adj, d, kAdjRayEvents);
+                            (void)walk(ev);
+                            rng.ctr = ctr1;
+                        }
+                    }
+                }
             }
-            shadow_finish(prm, ps, p, fl, sc.distant_count, true);
+            shadow_finish<kAdj>(prm, ps, p, fl, sc.distant_count, true, &sums);
         }
     });
+    sums.end();
     wave_add64(prm.stats + kStatTransmittance, ntr);
     wave_add64(prm.stats + kStatTransmitNodes, tc.nodes);
     wave_add64(prm.stats + kStatTransmitTris, tc.tris);
 }
 
 
-template <bool kSphere, int kStack, int kHet>
+template <bool kSphere, int kStack, int kHet, bool kAdj = false>
 __global__ __launch_bounds__(kBlock) void k_transmit_m(const FrameSlot *__restrict__ fs, uint32_t nfs, uint32_t pass)
 {
     for_slots(
         fs, nfs, [&](const FrameSlot &f) { return pass_counter(f, pass, kCtrTransmit); },
         [&](const FrameSlot &f) { return pass_drained(f, pass, kCtrWorkTransmit, kCtrTransmit); },
-        [&](const FrameSlot &f) { transmit_body<kSphere, kStack, kHet>(f.sc, f.prm, f.ps, pass); });
+        [&](const FrameSlot &f) { transmit_body<kSphere, kStack, kHet, kAdj>(f.sc, f.prm, f.ps, pass); });
 }
 
 
@@ -1591,7 +1857,9 @@ int launch_multi_extend(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v
 int launch_multi_medium(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v, uint32_t waves, uint32_t pass,
                         hipStream_t s)
 {
-    if (v.het == 2) hipLaunchKernelGGL(k_medium_m<2>, dim3(waves), dim3(kBlock), 0, s, fs, nfs, pass);
+    if (v.adj && v.het == 2) hipLaunchKernelGGL((k_medium_m<2, true>), dim3(waves), dim3(kBlock), 0, s, fs, nfs, pass);
+    else if (v.adj && v.het == 1) hipLaunchKernelGGL((k_medium_m<1, true>), dim3(waves), dim3(kBlock), 0, s, fs, nfs, pass);
+    else if (v.het == 2) hipLaunchKernelGGL(k_medium_m<2>, dim3(waves), dim3(kBlock), 0, s, fs, nfs, pass);
     else if (v.het == 1) hipLaunchKernelGGL(k_medium_m<1>, dim3(waves), dim3(kBlock), 0, s, fs, nfs, pass);
     else hipLaunchKernelGGL(k_medium_m<0>, dim3(waves), dim3(kBlock), 0, s, fs, nfs, pass);
     YM_HIP(hipGetLastError());
@@ -1615,21 +1883,22 @@ int launch_multi_compact(const FrameSlot *fs, uint32_t nfs, uint32_t blocks, uin
     return YMRT_OK;
 }
 
-template <bool kGraph>
+template <bool kGraph, bool kAdj>
 static void shade_m(const FrameSlot *fs, uint32_t nfs, uint32_t waves, uint32_t pass, hipStream_t s)
 {
     const dim3 g(waves), b(kBlock);
-    hipLaunchKernelGGL((k_shade_m<false, kGraph>), g, b, 0, s, fs, nfs, pass);
-    hipLaunchKernelGGL((k_shade_m<false, kGraph, kMatDiffuse>), g, b, 0, s, fs, nfs, pass);
-    hipLaunchKernelGGL((k_shade_m<false, kGraph, kMatMetal>), g, b, 0, s, fs, nfs, pass);
+    hipLaunchKernelGGL((k_shade_m<false, kGraph, kMatGeneral, kAdj>), g, b, 0, s, fs, nfs, pass);
+    hipLaunchKernelGGL((k_shade_m<false, kGraph, kMatDiffuse, kAdj>), g, b, 0, s, fs, nfs, pass);
+    hipLaunchKernelGGL((k_shade_m<false, kGraph, kMatMetal, kAdj>), g, b, 0, s, fs, nfs, pass);
 }
 
 int launch_multi_shade(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v, bool volume, uint32_t waves,
                        uint32_t pass, hipStream_t s)
 {
+    // volume vertices carry no adjoint events of their own
     if (volume) hipLaunchKernelGGL((k_shade_m<true, false>), dim3(waves), dim3(kBlock), 0, s, fs, nfs, pass);
-    else if (v.graph) shade_m<true>(fs, nfs, waves, pass, s);
-    else shade_m<false>(fs, nfs, waves, pass, s);
+    else if (v.graph) v.adj ? shade_m<true, true>(fs, nfs, waves, pass, s) : shade_m<true, false>(fs, nfs, waves, pass, s);
+    else v.adj ? shade_m<false, true>(fs, nfs, waves, pass, s) : shade_m<false, false>(fs, nfs, waves, pass, s);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }
@@ -1638,7 +1907,15 @@ int launch_multi_shadow(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v
                         hipStream_t s)
 {
     const dim3 g(waves), b(kBlock);
-    if (v.sphere) {
+    if (v.adj) {
+        if (v.sphere) {
+            if (v.media) hipLaunchKernelGGL((k_shadow_m<true, true, kMaxStackDepth, true>), g, b, 0, s, fs, nfs, pass);
+            else hipLaunchKernelGGL((k_shadow_m<false, true, kMaxStackDepth, true>), g, b, 0, s, fs, nfs, pass);
+        } else {
+            if (v.media) hipLaunchKernelGGL((k_shadow_m<true, false, kMaxStackDepth, true>), g, b, 0, s, fs, nfs, pass);
+            else hipLaunchKernelGGL((k_shadow_m<false, false, kMaxStackDepth, true>), g, b, 0, s, fs, nfs, pass);
+        }
+    } else if (v.sphere) {
         if (v.media) hipLaunchKernelGGL((k_shadow_m<true, true, kMaxStackDepth>), g, b, 0, s, fs, nfs, pass);
         else hipLaunchKernelGGL((k_shadow_m<false, true, kMaxStackDepth>), g, b, 0, s, fs, nfs, pass);
     } else {
@@ -1653,7 +1930,17 @@ int launch_multi_transmit(const FrameSlot *fs, uint32_t nfs, const MultiVariant 
                           hipStream_t s)
 {
     const dim3 g(waves), b(kBlock);
-    if (v.sphere) {
+    if (v.adj) {
+        if (v.sphere) {
+            if (v.het == 2) hipLaunchKernelGGL((k_transmit_m<true, kMaxStackDepth, 2, true>), g, b, 0, s, fs, nfs, pass);
+            else if (v.het == 1) hipLaunchKernelGGL((k_transmit_m<true, kMaxStackDepth, 1, true>), g, b, 0, s, fs, nfs, pass);
+            else hipLaunchKernelGGL((k_transmit_m<true, kMaxStackDepth, 0, true>), g, b, 0, s, fs, nfs, pass);
+        } else {
+            if (v.het == 2) hipLaunchKernelGGL((k_transmit_m<false, kMaxStackDepth, 2, true>), g, b, 0, s, fs, nfs, pass);
+            else if (v.het == 1) hipLaunchKernelGGL((k_transmit_m<false, kMaxStackDepth, 1, true>), g, b, 0, s, fs, nfs, pass);
+            else hipLaunchKernelGGL((k_transmit_m<false, kMaxStackDepth, 0, true>), g, b, 0, s, fs, nfs, pass);
+        }
+    } else if (v.sphere) {
         if (v.het == 2) hipLaunchKernelGGL((k_transmit_m<true, kMaxStackDepth, 2>), g, b, 0, s, fs, nfs, pass);
         else if (v.het == 1) hipLaunchKernelGGL((k_transmit_m<true, kMaxStackDepth, 1>), g, b, 0, s, fs, nfs, pass);
         else hipLaunchKernelGGL((k_transmit_m<true, kMaxStackDepth, 0>), g, b, 0, s, fs, nfs, pass);
@@ -1680,58 +1967,6 @@ int launch_multi_resolve(const FrameSlot *fs, uint32_t nfs, uint32_t pixel_block
     return YMRT_OK;
 }
 
-// Path-replay adjoint for Lambert-only albedos: for each path of the chunk,
-// d(loss)/d(albedo_m) += adj(pixel) * (L_path - L_before_vertex) / albedo_m
-// / spp over its recorded vertices.  Sums go through LDS per block, then one
-// global atomic per block and material channel (or straight to global memory
-// when the materials do not fit the LDS table).
-constexpr uint32_t kAdjLdsMaterials = 1024;
-
-__global__ __launch_bounds__(256) void k_adjoint(DevScene sc, PtParams prm, PathState ps)
-{
-    __shared__ float acc[kAdjLdsMaterials * 3];
-    const bool lds = sc.material_count <= kAdjLdsMaterials;
-    if (lds)
-        for (uint32_t i = threadIdx.x; i < sc.material_count * 3; i += blockDim.x) acc[i] = 0.0f;
-    __syncthreads();
-    const uint32_t spp = (uint32_t)prm.rs.spp;
-    const float inv = 1.0f / (float)(spp > 1 ? spp : 1);
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < prm.chunk_paths; p += gridDim.x * blockDim.x) {
-        const uint64_t vpix = prm.chunk_base / spp + p / spp;
-        const uint32_t tile = (uint32_t)(vpix >> 8), within = (uint32_t)(vpix & 255u);
-        const uint32_t ty = tile / prm.tiles_x, tx = tile - ty * prm.tiles_x;
-        const uint32_t px = tx * 16u + (within & 15u), py = ty * 16u + (within >> 4);
-        if (px >= prm.rs.width || py >= prm.rs.height) continue;
-        const size_t pix = (size_t)py * prm.rs.width + px;
-        const f3 a{prm.adj_img[pix * 3 + 0] * inv, prm.adj_img[pix * 3 + 1] * inv, prm.adj_img[pix * 3 + 2] * inv};
-        const f3 Lt = xyz(ps.out[p]);
-        const uint32_t n = ps.adj_n[p];
-        for (uint32_t k = 0; k < n; ++k) {
-            const uint4 r = ps.adj_rec[(size_t)k * prm.cap + p];
-            const float *alb = sc.materials[r.x].diffuse_albedo;
-            const f3 d = Lt - f3{__uint_as_float(r.y), __uint_as_float(r.z), __uint_as_float(r.w)};
-            const float g[3] = {d.x * a.x, d.y * a.y, d.z * a.z};
-            for (int c = 0; c < 3; ++c) {
-                if (!(alb[c] > 0.0f)) continue;
-                const float v = g[c] / alb[c];
-                if (lds) atomicAdd(&acc[r.x * 3 + c], v);
-                else atomicAdd(&prm.adj_grad[r.x * 3 + c], v);
-            }
-        }
-    }
-    if (!lds) return;
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < sc.material_count * 3; i += blockDim.x)
-        if (acc[i] != 0.0f) atomicAdd(&prm.adj_grad[i], acc[i]);
-}
-
-int launch_adjoint(const DevScene &sc, const PtParams &prm, const PathState &ps, hipStream_t s)
-{
-    const uint32_t blocks = std::min<uint32_t>((prm.chunk_paths + 255) / 256, 1024u);
-    hipLaunchKernelGGL(k_adjoint, dim3(blocks), dim3(256), 0, s, sc, prm, ps);
-    YM_HIP(hipGetLastError());
-    return YMRT_OK;
-}
 
 // Texture graph at n given coordinates: coords[i*8 + 0..7] = (s, t,
 // p_world.xyz, p_object.xyz), out[i*3 + 0..2] = the texture's rgb.

@@ -93,13 +93,28 @@ struct PathState {
     uint32_t *flag;       // stage flag per slot (kFlag*)
     uint32_t *ccount;     // compaction: matches per block, [4][cap / kCompactSpan + 1]
     uint32_t *first_hit;  // instance hit by the camera ray (written when an id image is requested)
-    uint4 *adj_rec;       // adjoint: [ray_depth][cap] (material, radiance before the vertex)
-    uint32_t *adj_n;      // adjoint: records per path
+    uint2 *adj_g;         // adjoint replay: (roughness entry or kEmpty, d ln f / d r of the
+                          // continuation direction) at the path's current surface vertex
 };
 
 // Shadow rays per path vertex: slots 0-1 shape light (light sample, BSDF or
 // phase sample), slots 2-3 distant light.
 constexpr int kShadowSlots = 4;
+
+// Targets of the path-replay adjoint (ymrt_render_adjoint) in the replay
+// pass: gradient events add to these device arrays.
+constexpr uint32_t kAdjLds = 256;  // albedo rows x 3 + roughness entries summed in LDS first
+
+struct AdjParams {
+    const float *img;          // dLoss/dImage, width * height * 3
+    const int32_t *alb_slot;   // per material: row of g_alb, or -1
+    const int32_t *rough_slot; // per material: entry of g_rough, or -1
+    float *g_alb, *g_rough, *g_tex, *g_vox;
+    uint32_t n_alb, n_rough;
+    uint32_t tex;              // target image texture, or kEmpty
+    uint32_t vol;              // target volume (gray sigma_t, voxel grid), or kEmpty
+    uint32_t lds;              // 1: albedo and roughness sums fit kAdjLds floats
+};
 
 struct PtParams {
     ymrt_render_settings rs;
@@ -113,9 +128,7 @@ struct PtParams {
     unsigned long long *stats;
     uint32_t *error;
     uint32_t *aov;           // per-pixel instance ids of the camera rays, or null
-    const uint32_t *adj_mat; // adjoint: 1 per Lambert-only material, or null (no adjoint)
-    const float *adj_img;    // adjoint: dLoss/dImage, width*height*3
-    float *adj_grad;         // adjoint: gradient, material_count*3 (accumulated)
+    AdjParams adj;           // the replay pass of ymrt_render_adjoint (zero otherwise)
     uint32_t ccount_stride;  // entries per queue in ccount
     uint32_t media;          // volume scattering on and the scene has volumes
     float4 *noise;           // the frame's noise image (multi-frame resolve)
@@ -125,6 +138,7 @@ struct PtParams {
 struct MultiVariant {
     bool nearby, media, sphere, graph;
     int het;  // 0 homogeneous media, 1 voxel grids, 2 texture densities
+    bool adj = false;  // the adjoint's replay pass: kernels that emit gradient events
 };
 
 // Most frame chunks one multi-frame launch serves.

@@ -36,7 +36,6 @@ int launch_multi_transmit(const FrameSlot *fs, uint32_t nfs, const MultiVariant 
                           hipStream_t s);
 int launch_multi_flush(const FrameSlot *fs, uint32_t nfs, uint32_t pass, hipStream_t s);
 int launch_multi_resolve(const FrameSlot *fs, uint32_t nfs, uint32_t pixel_blocks, hipStream_t s);
-int launch_adjoint(const DevScene &sc, const PtParams &prm, const PathState &ps, hipStream_t s);
 
 // Paths in flight per chunk when the settings leave it to the library.  Each
 // chunk ends in passes that hold only a frame's longest paths, where the GPU
@@ -61,8 +60,8 @@ void free_render_buffers(RenderBuffers *rb) { delete rb; }
 static std::mutex g_alloc_mu;
 
 static int run_frame(ymrt_scene *sc, RenderBuffers *rb, uint32_t chunk_paths_max, const ymrt_render_settings *rs,
-                     int frame, float *noise, uint32_t *instance_ids, const float *adj_img, float *adj_grad,
-                     hipStream_t stream, int timing, ymrt_frame_stats *stats);
+                     int frame, float *noise, uint32_t *instance_ids, const AdjParams *adj, hipStream_t stream,
+                     int timing, ymrt_frame_stats *stats);
 
 static int check_settings(const ymrt_render_settings *rs)
 {
@@ -138,7 +137,7 @@ int render_frame_fixed(ymrt_scene *sc, void *memory, size_t bytes, uint32_t cap,
         (uint32_t)std::min<uint64_t>(per_cap, (total_paths / nchunks + spp - 1) / spp * spp);
     YM_HIP(hipSetDevice(sc->ctx->device));
     if (int r = clear_stack_flag(sc, stream)) return r;
-    return run_frame(sc, &view, chunk_paths_max, rs, frame, noise, nullptr, nullptr, nullptr, stream, 0, stats);
+    return run_frame(sc, &view, chunk_paths_max, rs, frame, noise, nullptr, nullptr, stream, 0, stats);
 }
 
 static hipEvent_t next_event(RenderBuffers *rb)
@@ -151,19 +150,19 @@ static hipEvent_t next_event(RenderBuffers *rb)
     return rb->events[rb->event_next++];
 }
 
-static int ensure_adjoint(RenderBuffers *rb, uint32_t depth)
+static int ensure_adjoint(RenderBuffers *rb)
 {
-    if (rb->adj_depth >= depth) return YMRT_OK;
-    int rc = alloc(rb, &rb->ps.adj_rec, (size_t)depth * rb->cap);
-    rc |= alloc(rb, &rb->ps.adj_n, rb->cap);
-    if (rc) return rc;
-    rb->adj_depth = depth;
-    return YMRT_OK;
+    if (rb->adj_slot) return YMRT_OK;
+    int rc = alloc(rb, &rb->ps.adj_g, rb->cap);
+    rc |= alloc(rb, &rb->adj_counters, kCountersPerSlot);
+    rc |= alloc(rb, &rb->adj_stats, kStatCount);
+    rc |= alloc(rb, &rb->adj_slot, 1);
+    return rc;
 }
 
 int render_frame_with(ymrt_scene *sc, RenderBuffers *&buffers, const ymrt_render_settings *rs, int frame,
-                      float *noise, uint32_t *instance_ids, const float *adj_img, float *adj_grad,
-                      hipStream_t stream, int timing, ymrt_frame_stats *stats)
+                      float *noise, uint32_t *instance_ids, const AdjParams *adj, hipStream_t stream, int timing,
+                      ymrt_frame_stats *stats)
 {
     if (int r = check_settings(rs)) return r;
     YM_HIP(hipSetDevice(sc->ctx->device));
@@ -197,15 +196,14 @@ int render_frame_with(ymrt_scene *sc, RenderBuffers *&buffers, const ymrt_render
     int rc = ensure_buffers(buffers, chunk_paths_max);
     if (rc) return rc;
     alloc_lock.unlock();
-    return run_frame(sc, buffers, chunk_paths_max, rs, frame, noise, instance_ids, adj_img, adj_grad, stream, timing,
-                     stats);
+    return run_frame(sc, buffers, chunk_paths_max, rs, frame, noise, instance_ids, adj, stream, timing, stats);
 }
 
 // The frame itself, in chunks of up to chunk_paths_max paths in rb's path
 // state (rb->cap >= chunk_paths_max, a multiple of spp).
 static int run_frame(ymrt_scene *sc, RenderBuffers *rb, uint32_t chunk_paths_max, const ymrt_render_settings *rs,
-                     int frame, float *noise, uint32_t *instance_ids, const float *adj_img, float *adj_grad,
-                     hipStream_t stream, int timing, ymrt_frame_stats *stats)
+                     int frame, float *noise, uint32_t *instance_ids, const AdjParams *adj, hipStream_t stream,
+                     int timing, ymrt_frame_stats *stats)
 {
     const uint32_t spp = (uint32_t)rs->spp;
     const uint64_t total_paths = (uint64_t)((rs->width + 15) / 16) * ((rs->height + 15) / 16) * 256ull * spp;
@@ -228,14 +226,11 @@ static int run_frame(ymrt_scene *sc, RenderBuffers *rb, uint32_t chunk_paths_max
     prm.stats = rb->stats;
     prm.error = sc->d_error;
     prm.aov = instance_ids;
-    if (adj_img) {
-        YM_REQUIRE(adj_grad != nullptr, "adjoint needs a gradient buffer");
+    if (adj) {
         YM_REQUIRE(!rs->enable_russian_roulette, "the adjoint needs Russian roulette off");
         YM_REQUIRE(!rs->enable_indirect_clamp, "the adjoint needs the indirect clamp off");
-        if ((rc = ensure_adjoint(rb, (uint32_t)rs->ray_depth))) return rc;
-        prm.adj_mat = sc->adj_mat;
-        prm.adj_img = adj_img;
-        prm.adj_grad = adj_grad;
+        YM_REQUIRE(rb->allocs.size() > 0, "the adjoint needs allocated path state");
+        if ((rc = ensure_adjoint(rb))) return rc;
     }
     YM_REQUIRE(total_paths == (uint64_t)prm.tiles_x * tiles_y * 256ull * spp, "tile count");
     // The coincident-surface list only changes results where medium records
@@ -283,6 +278,54 @@ static int run_frame(ymrt_scene *sc, RenderBuffers *rb, uint32_t chunk_paths_max
         frame_b = next_event(rb);
         YM_HIP(hipEventRecord(frame_a, stream));
     }
+    // raygen, the bounce passes and the flush of one chunk over slot table f
+    // and its counters; the adjoint's replay runs them again with the
+    // event-emitting kernels (timed as one stage)
+    auto run_passes = [&](const FrameSlot *f, uint32_t *ctrs, const MultiVariant &vv, bool replay, uint32_t waves,
+                          uint32_t blocks, uint32_t &npass) -> int {
+        auto st = [&](int stage) { return replay ? 7 : stage; };
+        int r = 0;
+        if ((r = timed(st(0), [&] { return launch_multi_raygen(f, 1, waves, stream); }))) return r;
+        uint32_t pass = 0;
+        uint32_t next_check = (uint32_t)rs->ray_depth + 1;
+        for (;;) {
+            if ((r = timed(st(1), [&] { return launch_multi_extend(f, 1, vv, waves, pass, stream); }))) return r;
+            if (media && (r = timed(st(5), [&] {
+                              int q = launch_multi_compact(f, 1, blocks, pass, 0, stream);
+                              return q ? q : launch_multi_medium(f, 1, vv, waves, pass, stream);
+                          })))
+                return r;
+            if ((r = timed(st(2), [&] { return launch_multi_compact(f, 1, blocks, pass, 1, stream); }))) return r;
+            if ((r = timed(st(2), [&] { return launch_multi_shade(f, 1, vv, false, waves, pass, stream); }))) return r;
+            if (media && (r = timed(st(2), [&] { return launch_multi_shade(f, 1, vv, true, waves, pass, stream); })))
+                return r;
+            if ((r = timed(st(3), [&] { return launch_multi_shadow(f, 1, vv, waves, pass, stream); }))) return r;
+            if (media && (r = timed(st(6), [&] {
+                              int q = launch_multi_compact(f, 1, blocks, pass, 2, stream);
+                              return q ? q : launch_multi_transmit(f, 1, vv, waves, pass, stream);
+                          })))
+                return r;
+            ++pass;
+            if (pass == next_check || pass >= kMaxPasses) {
+                uint32_t active = 0;
+                YM_HIP(hipMemcpyAsync(&active, ctrs + pass * kCtrStride + kCtrActive, sizeof(uint32_t),
+                                      hipMemcpyDeviceToHost, stream));
+                YM_HIP(hipStreamSynchronize(stream));
+                if (active == 0) break;
+                if (pass >= kMaxPasses) {
+                    if ((r = launch_multi_flush(f, 1, pass, stream))) return r;
+                    break;
+                }
+                next_check = pass + 2;
+            }
+        }
+        npass = pass;
+        return YMRT_OK;
+    };
+    MultiVariant vadj = v;
+    vadj.adj = true;
+    FrameSlot rslot;
+    std::memset(&rslot, 0, sizeof(rslot));
     uint32_t passes_total = 0, chunks = 0;
     for (uint64_t base = 0; base < total_paths; base += chunk_paths_max) {
         prm.chunk_base = base;
@@ -293,46 +336,23 @@ static int run_frame(ymrt_scene *sc, RenderBuffers *rb, uint32_t chunk_paths_max
         slot.prm = prm;
         YM_HIP(hipMemcpyAsync(rb->slots, &slot, sizeof(slot), hipMemcpyHostToDevice, stream));
         YM_HIP(hipMemsetAsync(rb->counters, 0, sizeof(uint32_t) * kCountersPerSlot, stream));
+        if (adj) {
+            // the replay: the same chunk with its own counters and statistics
+            rslot = slot;
+            rslot.prm.counters = rb->adj_counters;
+            rslot.prm.stats = rb->adj_stats;
+            rslot.prm.adj = *adj;
+            YM_HIP(hipMemcpyAsync(rb->adj_slot, &rslot, sizeof(rslot), hipMemcpyHostToDevice, stream));
+            YM_HIP(hipMemsetAsync(rb->adj_counters, 0, sizeof(uint32_t) * kCountersPerSlot, stream));
+        }
         const uint32_t waves = persistent_waves(prm.chunk_paths, rs->persistent_waves, rb->ps.spill_threads);
         const uint32_t blocks = (prm.chunk_paths + kCompactSpan - 1) / kCompactSpan;
-        if ((rc = timed(0, [&] { return launch_multi_raygen(fs, 1, waves, stream); }))) return rc;
-        uint32_t pass = 0;
-        uint32_t next_check = (uint32_t)rs->ray_depth + 1;
-        for (;;) {
-            if ((rc = timed(1, [&] { return launch_multi_extend(fs, 1, v, waves, pass, stream); }))) return rc;
-            if (media && (rc = timed(5, [&] {
-                              int r = launch_multi_compact(fs, 1, blocks, pass, 0, stream);
-                              return r ? r : launch_multi_medium(fs, 1, v, waves, pass, stream);
-                          })))
-                return rc;
-            if ((rc = timed(2, [&] { return launch_multi_compact(fs, 1, blocks, pass, 1, stream); }))) return rc;
-            if ((rc = timed(2, [&] { return launch_multi_shade(fs, 1, v, false, waves, pass, stream); }))) return rc;
-            if (media && (rc = timed(2, [&] { return launch_multi_shade(fs, 1, v, true, waves, pass, stream); })))
-                return rc;
-            if ((rc = timed(3, [&] { return launch_multi_shadow(fs, 1, v, waves, pass, stream); }))) return rc;
-            if (media && (rc = timed(6, [&] {
-                              int r = launch_multi_compact(fs, 1, blocks, pass, 2, stream);
-                              return r ? r : launch_multi_transmit(fs, 1, v, waves, pass, stream);
-                          })))
-                return rc;
-            ++pass;
-            if (pass == next_check || pass >= kMaxPasses) {
-                uint32_t active = 0;
-                YM_HIP(hipMemcpyAsync(&active, rb->counters + pass * kCtrStride + kCtrActive, sizeof(uint32_t),
-                                      hipMemcpyDeviceToHost, stream));
-                YM_HIP(hipStreamSynchronize(stream));
-                if (active == 0) break;
-                if (pass >= kMaxPasses) {
-                    if ((rc = launch_multi_flush(fs, 1, pass, stream))) return rc;
-                    break;
-                }
-                next_check = pass + 2;
-            }
-        }
-        passes_total += pass;
+        uint32_t npass = 0;
+        if ((rc = run_passes(fs, rb->counters, v, false, waves, blocks, npass))) return rc;
+        passes_total += npass;
         const uint32_t pixel_blocks = (prm.chunk_paths / spp + 255) / 256;
         if ((rc = timed(4, [&] { return launch_multi_resolve(fs, 1, pixel_blocks, stream); }))) return rc;
-        if (prm.adj_mat && (rc = timed(7, [&] { return launch_adjoint(sc->dev, prm, rb->ps, stream); }))) return rc;
+        if (adj && (rc = run_passes(rb->adj_slot, rb->adj_counters, vadj, true, waves, blocks, npass))) return rc;
     }
     if (timing) YM_HIP(hipEventRecord(frame_b, stream));
     if (stats) {
@@ -386,13 +406,12 @@ static int run_frame(ymrt_scene *sc, RenderBuffers *rb, uint32_t chunk_paths_max
 }
 
 int render_frame(ymrt_scene *sc, const ymrt_render_settings *rs, int frame, float *noise, uint32_t *instance_ids,
-                 const float *adj_img, float *adj_grad, hipStream_t stream, int timing, ymrt_frame_stats *stats)
+                 const AdjParams *adj, hipStream_t stream, int timing, ymrt_frame_stats *stats)
 {
     YM_HIP(hipSetDevice(sc->ctx->device));
     const int rc = clear_stack_flag(sc, stream);
     if (rc) return rc;
-    return render_frame_with(sc, sc->render, rs, frame, noise, instance_ids, adj_img, adj_grad, stream, timing,
-                             stats);
+    return render_frame_with(sc, sc->render, rs, frame, noise, instance_ids, adj, stream, timing, stats);
 }
 
 }  // namespace ym

@@ -74,16 +74,6 @@ bool needs_nearby(bool has_boundaries, const std::vector<DevMaterial> &materials
     return r;
 }
 
-// Lambert-only DefaultMtl (the adjoint's differentiable materials): its one
-// lobe is the untextured albedo, so sampling and MIS do not depend on it.
-uint32_t lambert_only(const DevMaterial &m)
-{
-    return m.type == YMRT_MATERIAL_DEFAULT && m.specular_weight == 0.0f && m.metalness == 0.0f &&
-           m.transmission_weight == 0.0f && m.coat_weight == 0.0f && m.diffuse_albedo_tex == kEmpty &&
-           m.specular_weight_tex == kEmpty && m.metalness_tex == kEmpty && m.transmission_weight_tex == kEmpty &&
-           m.coat_weight_tex == kEmpty;
-}
-
 // A DefaultMtl whose coat, metal, specular and transmission weights are 0
 // (untextured), with no normal or bump map and a coat that cannot overflow
 // (albedo in [0, 1], finite thickness): InitBSDFSettings then leaves the
@@ -728,11 +718,11 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
     if ((rc = upload(sc, h.shade, &ds.shade))) return rc;
     if ((rc = upload(sc, h.materials, &ds.materials))) return rc;
     {
-        std::vector<uint32_t> lam(h.materials.size());
-        for (size_t i = 0; i < h.materials.size(); ++i) lam[i] = lambert_only(h.materials[i]);
-        const uint32_t *d = nullptr;
-        if ((rc = upload(sc, lam, &d))) return rc;
-        sc->adj_mat = d;
+        // the adjoint's per-material target slots, written by each adjoint call
+        std::vector<int32_t> slots(2 * h.materials.size(), -1);
+        const int32_t *d = nullptr;
+        if ((rc = upload(sc, slots, &d))) return rc;
+        sc->d_adj_slots = const_cast<int32_t *>(d);
         std::vector<uint32_t> dif(h.materials.size());
         for (size_t i = 0; i < h.materials.size(); ++i) dif[i] = shading_class(h.materials[i]);
         if ((rc = upload(sc, dif, &ds.mat_class))) return rc;
@@ -748,6 +738,8 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
     if ((rc = upload(sc, grids, &ds.grids))) return rc;
     if ((rc = upload(sc, voxels, &ds.voxels))) return rc;
     if ((rc = upload(sc, vols, &ds.volumes))) return rc;
+    sc->grids = grids;
+    sc->volumes = vols;
     if ((rc = upload(sc, lights, &ds.lights))) return rc;
     if ((rc = upload(sc, cdf, &ds.light_cdf))) return rc;
     if ((rc = upload(sc, table, &ds.halton))) return rc;
@@ -798,21 +790,32 @@ int update_materials(ymrt_scene *sc, uint32_t first, uint32_t count, const ymrt_
         YM_REQUIRE(material_textures_valid(dm, sc->texture_count), "material texture index out of range");
         next[first + i] = dm;
     }
-    std::vector<uint32_t> lam(count), dif(count);
-    for (uint32_t i = 0; i < count; ++i) {
-        lam[i] = lambert_only(next[first + i]);
-        dif[i] = shading_class(next[first + i]);
-    }
+    std::vector<uint32_t> dif(count);
+    for (uint32_t i = 0; i < count; ++i) dif[i] = shading_class(next[first + i]);
     YM_HIP(hipMemcpyAsync(const_cast<DevMaterial *>(sc->dev.materials) + first, next.data() + first,
                           sizeof(DevMaterial) * count, hipMemcpyHostToDevice, s));
-    YM_HIP(hipMemcpyAsync(const_cast<uint32_t *>(sc->adj_mat) + first, lam.data(), sizeof(uint32_t) * count,
-                          hipMemcpyHostToDevice, s));
     YM_HIP(hipMemcpyAsync(const_cast<uint32_t *>(sc->dev.mat_class) + first, dif.data(), sizeof(uint32_t) * count,
                           hipMemcpyHostToDevice, s));
     YM_HIP(hipStreamSynchronize(s));
     sc->materials.swap(next);
     sc->needs_nearby = needs_nearby(sc->has_boundaries, sc->materials);
     sc->dev.has_graph = uses_texture_graph(sc->procedural_textures, sc->materials) ? 1 : 0;
+    return YMRT_OK;
+}
+
+int update_grid(ymrt_scene *sc, uint32_t grid, const float *density, hipStream_t s)
+{
+    YM_REQUIRE(density != nullptr, "density is NULL");
+    YM_REQUIRE(grid < sc->grids.size(), "grid index out of range");
+    DevGrid &g = sc->grids[grid];
+    const size_t n = (size_t)g.nx * g.ny * g.nz;
+    float mx = 0.0f;
+    for (size_t k = 0; k < n; ++k) mx = density[k] > mx ? density[k] : mx;
+    g.max_density = mx;  // the majorant of the volumes that read the grid
+    YM_HIP(hipMemcpyAsync(const_cast<float *>(sc->dev.voxels) + g.offset, density, sizeof(float) * n,
+                          hipMemcpyHostToDevice, s));
+    YM_HIP(hipMemcpyAsync(const_cast<DevGrid *>(sc->dev.grids) + grid, &g, sizeof(DevGrid), hipMemcpyHostToDevice, s));
+    YM_HIP(hipStreamSynchronize(s));
     return YMRT_OK;
 }
 

@@ -41,7 +41,7 @@ int autotune(ymrt_scene *sc, const ymrt_render_settings *rs_in, int frame, hipSt
         ms = -1.0f;
         for (int k = 0; k < kTrialRepeats; ++k) {
             ymrt_frame_stats st{};
-            const int r = render_frame_with(sc, sc->render, &rs, frame, img, nullptr, nullptr, nullptr, stream, 1, &st);
+            const int r = render_frame_with(sc, sc->render, &rs, frame, img, nullptr, nullptr, stream, 1, &st);
             ++res.trials;
             if (r == YMRT_ERR_OUT_OF_MEMORY && cap != 0) return YMRT_OK;
             if (r) return r;
@@ -53,7 +53,7 @@ int autotune(ymrt_scene *sc, const ymrt_render_settings *rs_in, int frame, hipSt
         ymrt_render_settings warm = *rs_in;
         warm.path_capacity = 0;
         warm.persistent_waves = 0;
-        rc = render_frame_with(sc, sc->render, &warm, frame, img, nullptr, nullptr, nullptr, stream, 0, nullptr);
+        rc = render_frame_with(sc, sc->render, &warm, frame, img, nullptr, nullptr, stream, 0, nullptr);
         ++res.trials;
         std::vector<uint32_t> caps{0};
         for (uint64_t div : {2ull, 4ull}) {

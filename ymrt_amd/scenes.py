@@ -586,3 +586,142 @@ def batch_workload(count=64, seed=11):
         spp = int(rng.choice([4, 16, 64]))
         out.append((tris, res, spp, 6))
     return out
+
+
+# ------------------------------------------------------- inverse rendering
+def inverse_params(tex_size=256, grid_n=64, n_rough=8, seed=3):
+    """Parameters of the differentiable Cornell workload: an RGB float albedo
+    texture (tex_size^2, values in [0.15, 0.85]), n_rough GGX roughnesses in
+    [0.15, 0.6] and a grid_n^3 density grid in [0, 1] (a few smooth blobs)."""
+    rng = np.random.RandomState(seed)
+    lattice = max(2, min(16, tex_size // 4))
+    tex = np.stack([value_noise_2d(tex_size, 3, seed * 7 + c, lattice=lattice) for c in range(3)], -1)
+    tex = (0.15 + 0.7 * tex).astype(np.float32)
+    rough = rng.uniform(0.15, 0.6, n_rough).astype(np.float32)
+    c = (np.arange(grid_n) + 0.5) / grid_n
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    dens = np.zeros((grid_n, grid_n, grid_n), np.float64)
+    for _ in range(6):
+        p = rng.uniform(0.2, 0.8, 3)
+        r = rng.uniform(0.12, 0.3)
+        dens += np.exp(-((x - p[0]) ** 2 + (y - p[1]) ** 2 + (z - p[2]) ** 2) / (2 * r * r))
+    dens = (dens / dens.max()).astype(np.float32)
+    return dict(texture=tex, roughness=rough, density=dens)
+
+
+def perturb_params(params, frac=0.3, seed=5):
+    """Each parameter times (1 + u), u uniform in [-frac, frac]; albedo texels
+    clipped to [0, 1], roughness to [0.01, 0.99], density to >= 0."""
+    rng = np.random.RandomState(seed)
+    out = {}
+    for k, v in params.items():
+        out[k] = (v * (1.0 + rng.uniform(-frac, frac, v.shape))).astype(np.float32)
+    out["texture"] = np.clip(out["texture"], 0.0, 1.0).astype(np.float32)
+    out["roughness"] = np.clip(out["roughness"], 0.01, 0.99).astype(np.float32)
+    out["density"] = np.maximum(out["density"], 0.0).astype(np.float32)
+    return out
+
+
+def cornell_inverse(width=512, height=512, params=None, sigma_max=2.0, halton_seed=0):
+    """The Cornell box of the differentiable workload: the floor's albedo is a
+    float RGB texture, len(roughness) metal spheres of their own GGX roughness
+    stand on it in two rows, and a voxel-grid medium (gray sigma_t =
+    sigma_max times the density) fills the box's middle.  Returns the scene
+    and its targets: dict(albedo=[Lambert materials], roughness=[metal
+    materials], texture=index, volume=index, grid=index)."""
+    params = inverse_params() if params is None else params
+    sa = abi.SceneArrays()
+    sa.halton_seed = halton_seed
+    white = sa.add_material(abi.default_material(diffuse_albedo=(0.7, 0.7, 0.7), specular_weight=0.0))
+    red = sa.add_material(abi.default_material(diffuse_albedo=(0.7, 0.1, 0.1), specular_weight=0.0))
+    green = sa.add_material(abi.default_material(diffuse_albedo=(0.1, 0.7, 0.1), specular_weight=0.0))
+    tex = sa.add_texture(np.ascontiguousarray(params["texture"], np.float32))
+    floor = sa.add_material(abi.default_material(diffuse_albedo=(1.0, 1.0, 1.0), diffuse_albedo_tex=tex,
+                                                 specular_weight=0.0))
+    light = sa.add_material(abi.light_material((1.0, 1.0, 1.0), 15.0))
+    walls = [
+        (quad((0, 0, 0), (0, 0, 1), (1, 0, 1), (1, 0, 0)), floor),
+        (quad((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)), white),
+        (quad((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0)), white),
+        (quad((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1)), red),
+        (quad((1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0)), green),
+    ]
+    for geom, mat in walls:
+        sa.add_instance(_add(sa, geom), mat)
+    rough = [float(r) for r in params["roughness"]]
+    metals = []
+    per_row = (len(rough) + 1) // 2
+    sphere = sa.add_sphere(0.07)
+    for i, r in enumerate(rough):
+        m = sa.add_material(abi.default_material(diffuse_albedo=(0.95, 0.75, 0.45), specular_albedo=(1.0, 1.0, 1.0),
+                                                 roughness_x=r, roughness_y=r, metalness=1.0))
+        metals.append(m)
+        row, col = divmod(i, per_row)
+        x = 0.15 + 0.7 * (col + 0.5) / per_row
+        z = 0.35 + 0.3 * row
+        sa.add_instance(sphere, m, _translate(x, 0.07, z))
+    lq = quad((0.35, 0.999, 0.35), (0.65, 0.999, 0.35), (0.65, 0.999, 0.65), (0.35, 0.999, 0.65))
+    sa.add_instance(_add(sa, lq), light)
+    lo, hi = np.array((0.1, 0.3, 0.1)), np.array((0.9, 0.8, 0.9))
+    w2o = np.eye(4, dtype=np.float32)
+    w2o[:3, :3] = np.diag(1.0 / (hi - lo))
+    w2o[:3, 3] = -lo / (hi - lo)
+    grid = sa.add_grid(np.ascontiguousarray(params["density"], np.float32))
+    vol = sa.add_volume((sigma_max,) * 3, (0.85, 0.85, 0.85), 0.2, grid=grid, world_to_object=w2o)
+    bmat = sa.add_material(abi.default_material(ior_n=1.0))
+    sa.add_instance(_add(sa, box(tuple(lo), tuple(hi))), bmat, inner_volume=vol, boundary=True)
+    sa.set_camera((0.5, 0.5, 2.2), (0.5, 0.5, 0.0), 40.0, width / height)
+    return sa, dict(albedo=[white, red, green], roughness=metals, texture=tex, volume=vol, grid=grid)
+
+
+def absorbing_slab(width=48, height=16, d0=0.5, dmax=1.0, sigma=2.0, le=1.0):
+    """A camera looking along -z through a purely absorbing slab (z in
+    [-0.2, 0.2], albedo 0, gray sigma_t = sigma times a 5 x 1 x 1 voxel
+    grid along x) at an emissive quad (z = -1): the first four voxels hold d0
+    and cover every camera ray, the fifth holds dmax (the majorant) outside
+    the view.  A ray's radiance is le * exp(-sigma d0 l) for its chord l in
+    the slab.  Returns (scene, grid x bounds of the slab)."""
+    sa = abi.SceneArrays()
+    light = sa.add_material(abi.light_material((1.0, 1.0, 1.0), le))
+    sa.add_instance(_add(sa, quad((-2, -2, -1), (2, -2, -1), (2, 2, -1), (-2, 2, -1))), light)
+    lo, hi = np.array((-1.6, -1.0, -0.2)), np.array((2.4, 1.0, 0.2))
+    w2o = np.eye(4, dtype=np.float32)
+    w2o[:3, :3] = np.diag(1.0 / (hi - lo))
+    w2o[:3, 3] = -lo / (hi - lo)
+    dens = np.full((1, 1, 5), d0, np.float32)
+    dens[0, 0, 4] = dmax
+    g = sa.add_grid(dens)
+    vol = sa.add_volume((sigma,) * 3, (0.0, 0.0, 0.0), 0.0, grid=g, world_to_object=w2o)
+    sa.add_instance(_add(sa, box(tuple(lo), tuple(hi))), sa.add_material(abi.default_material(ior_n=1.0)),
+                    inner_volume=vol, boundary=True)
+    sa.set_camera((0.0, 0.0, 5.0), (0.0, 0.0, 0.0), 10.0, width / height)
+    return sa, (float(lo[0]), float(hi[0]))
+
+
+def lit_floor(width=32, height=32, material=None, slab=None, light_half=0.4, fov=70.0):
+    """A floor at z = -1 (facing +z) under a quad light at z = +1 facing it,
+    seen by a camera between them (z = -0.5, looking down): with ray depth 1
+    every image term is a next-event ray from the floor to the light (of
+    half-width light_half).  material: the floor's (default Lambert 0.6);
+    slab: a voxel-grid density
+    (nz, ny, nx) filling z in [-0.2, 0.2] that the shadow rays cross (gray
+    sigma_t 2, albedo 0.8)."""
+    sa = abi.SceneArrays()
+    m = material if material is not None else abi.default_material(diffuse_albedo=(0.6, 0.6, 0.6),
+                                                                   specular_weight=0.0)
+    floor = sa.add_material(m)
+    sa.add_instance(_add(sa, quad((-3, -3, -1), (3, -3, -1), (3, 3, -1), (-3, 3, -1))), floor)
+    light = sa.add_material(abi.light_material((1.0, 0.9, 0.8), 8.0))
+    h = float(light_half)
+    sa.add_instance(_add(sa, quad((-h, -h, 1), (-h, h, 1), (h, h, 1), (h, -h, 1))), light)
+    if slab is not None:
+        lo, hi = np.array((-2.0, -2.0, -0.2)), np.array((2.0, 2.0, 0.2))
+        w2o = np.eye(4, dtype=np.float32)
+        w2o[:3, :3] = np.diag(1.0 / (hi - lo))
+        w2o[:3, 3] = -lo / (hi - lo)
+        g = sa.add_grid(np.ascontiguousarray(slab, np.float32))
+        vol = sa.add_volume((2.0,) * 3, (0.8, 0.8, 0.8), 0.0, grid=g, world_to_object=w2o)
+        sa.add_instance(_add(sa, box(tuple(lo), tuple(hi))), sa.add_material(abi.default_material(ior_n=1.0)),
+                        inner_volume=vol, boundary=True)
+    sa.set_camera((0.0, 0.0, -0.5), (0.0, 0.0, -1.0), fov, width / height)
+    return sa
