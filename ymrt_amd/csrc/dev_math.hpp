@@ -29,6 +29,27 @@ YM_HD void static_for(F &&f)
 
 namespace ym {
 
+// Loads and stores through global-memory instructions.  The device arrays
+// reach the pass kernels as pointers read from the frame-slot table in
+// memory, so the compiler cannot tell them from LDS or scratch addresses and
+// emits flat instructions, which count against both the vector-memory and
+// the LDS wait counters: an LDS stack access must then also wait for every
+// node load in flight.  Casting the address through an integer to the
+// global address space keeps the global form (with a scalar base where the
+// index is per lane); the arrays are device allocations, so it is exact.
+template <typename T>
+YM_HD T gload(const T *p)
+{
+    T out;
+    __builtin_memcpy(&out, (const __attribute__((address_space(1))) T *)(uintptr_t)p, sizeof(T));
+    return out;
+}
+template <typename T>
+YM_HD void gstore(T *p, const T &v)
+{
+    __builtin_memcpy((__attribute__((address_space(1))) T *)(uintptr_t)p, &v, sizeof(T));
+}
+
 constexpr float kPi = 3.14159265358f;
 constexpr float kTwoPi = 6.28318530716f;
 constexpr float kInvPi = 0.318309886183f;

@@ -258,7 +258,7 @@ struct LaneScratch {
         } else {
             static_assert((kWindow & (kWindow - 1)) == 0, "window must be a power of two");
             const int w = (sp & (kWindow - 1)) * kBlock + lane;
-            if (sp >= kWindow) spill[(size_t)(sp - kWindow) * spill_stride + spill_index] = stack[w];
+            if (sp >= kWindow) gstore(spill + (size_t)(sp - kWindow) * spill_stride + spill_index, stack[w]);
             stack[w] = v;
         }
     }
@@ -270,7 +270,7 @@ struct LaneScratch {
         } else {
             const int w = (sp & (kWindow - 1)) * kBlock + lane;
             const uint32_t v = stack[w];
-            if (sp >= kWindow) stack[w] = spill[(size_t)(sp - kWindow) * spill_stride + spill_index];
+            if (sp >= kWindow) stack[w] = gload(spill + (size_t)(sp - kWindow) * spill_stride + spill_index);
             return v;
         }
     }
@@ -409,13 +409,13 @@ struct Traversal {
     {
         {
             const WNode *n = sc.nodes + node;
-            const float4 lx = *reinterpret_cast<const float4 *>(n->lo_x);
-            const float4 hx = *reinterpret_cast<const float4 *>(n->hi_x);
-            const float4 ly = *reinterpret_cast<const float4 *>(n->lo_y);
-            const float4 hy = *reinterpret_cast<const float4 *>(n->hi_y);
-            const float4 lz = *reinterpret_cast<const float4 *>(n->lo_z);
-            const float4 hz = *reinterpret_cast<const float4 *>(n->hi_z);
-            const uint4 ch = *reinterpret_cast<const uint4 *>(n->child);
+            const float4 lx = gload(reinterpret_cast<const float4 *>(n->lo_x));
+            const float4 hx = gload(reinterpret_cast<const float4 *>(n->hi_x));
+            const float4 ly = gload(reinterpret_cast<const float4 *>(n->lo_y));
+            const float4 hy = gload(reinterpret_cast<const float4 *>(n->hi_y));
+            const float4 lz = gload(reinterpret_cast<const float4 *>(n->lo_z));
+            const float4 hz = gload(reinterpret_cast<const float4 *>(n->hi_z));
+            const uint4 ch = gload(reinterpret_cast<const uint4 *>(n->child));
             if (kCount) cnt.nodes++;
             constexpr float kRobust = 1.0f + 2.0f * error_gamma(3);
             float k0, k1, k2, k3;
@@ -476,7 +476,7 @@ struct Traversal {
         } else if (!in_blas) {
             // top-level leaf: one instance
             const uint32_t ii = leaf_first(node);
-            const DevInstance &in = sc.instances[ii];
+            const DevInstance in = gload(sc.instances + ii);
             bool take = true;
             if (kMode == kModeSkipBoundary) take = !(in.flags & kInstBoundary);
             if (kMode == kModeBoundaryOnly) take = (in.flags & kInstBoundary) != 0;
@@ -523,7 +523,7 @@ struct Traversal {
             const uint32_t first = leaf_first(node), count = leaf_count(node);
             for (uint32_t k = 0; k < count; ++k) {
                 const float4 *tp = sc.tris + 3u * (first + k);
-                const float4 w0 = tp[0], w1 = tp[1], w2 = tp[2];
+                const float4 w0 = gload(tp), w1 = gload(tp + 1), w2 = gload(tp + 2);
                 const uint32_t tri = f2u(w0.w);
                 if (kCount) cnt.tris++;
                 if (skip_enabled && cur_inst == q.last_inst && tri == q.last_tri) continue;

@@ -513,9 +513,9 @@ __device__ __forceinline__ void extend_body(const DevScene &sc, const PtParams &
     for (;;) {
         uint32_t item = 0;
         if (rr.more && rr.refill(r == kLaneIdle, item)) {
-            p = qin[item];
-            const float4 o4 = ps.ray_o[p], d4 = ps.ray_d[p];
-            const uint2 last = ps.last[p];
+            p = gload(qin + item);
+            const float4 o4 = gload(ps.ray_o + p), d4 = gload(ps.ray_d + p);
+            const uint2 last = gload(ps.last + p);
             Query q;
             q.o = xyz(o4);
             q.d = xyz(d4);
@@ -540,7 +540,7 @@ __device__ __forceinline__ void extend_body(const DevScene &sc, const PtParams &
             if (r == kTravOverflow) atomicOr(prm.error, 1u);
             const bool hit = r == kTravHit;
             r = kLaneIdle;
-            uint32_t info = ps.info[p];
+            uint32_t info = gload(ps.info + p);
             if (info & kInfoPendingBunch) {
                 // the boundary bunch of the previous vertex reads that
                 // vertex's nearby list, which the writes below replace
@@ -555,15 +555,15 @@ __device__ __forceinline__ void extend_body(const DevScene &sc, const PtParams &
             cam_rays += (info_depth(info) == 1 && (info & kInfoNeverScatter)) ? 1u : 0u;
             const Hit &h = tv.hit;
             if (pass == 0 && prm.aov) ps.first_hit[p] = hit ? h.inst : kEmpty;
-            const bool in_medium = kMedia && prm.media && (ps.xhdr[p] & 0xFFu) != 0;
+            const bool in_medium = kMedia && prm.media && (gload(ps.xhdr + p) & 0xFFu) != 0;
             if (!hit && !in_medium) {
-                f3 Lv = xyz(ps.L[p]);
+                f3 Lv = xyz(gload(ps.L + p));
                 if (rs.enable_env_light)
-                    Lv = Lv + light_clamp(rs, xyz(ps.beta[p]) * background(tv.q.d), info_depth(info));
-                ps.out[p] = f4(Lv, 0.0f);
+                    Lv = Lv + light_clamp(rs, xyz(gload(ps.beta + p)) * background(tv.q.d), info_depth(info));
+                gstore(ps.out + p, f4(Lv, 0.0f));
             } else {
-                ps.flag[p] = (pass << kFlagBits) |
-                             (in_medium ? kFlagMedium : shade_flag(sc.mat_class[sc.instances[h.inst].material]));
+                const uint32_t mat = gload(&sc.instances[h.inst].material);
+                gstore(ps.flag + p, (pass << kFlagBits) | (in_medium ? kFlagMedium : shade_flag(gload(sc.mat_class + mat))));
                 uint32_t flags = h.back ? kHitBack : 0u;
                 bool listed = false;
                 const int nbc = tv.nb_count;
@@ -573,20 +573,20 @@ __device__ __forceinline__ void extend_body(const DevScene &sc, const PtParams &
                         flags |= (uint32_t)nbc << 4;
                         static_for<0, kNearbyCap>([&](auto K) {
                             if (K() < nbc)
-                                ps.nb[(size_t)K() * prm.cap + p] =
-                                    make_uint2(nbp[K() * kBlock + lane_id()] |
-                                                   ((uint32_t)(nbb[K() * kBlock + lane_id()] != 0) << 31),
-                                               __float_as_uint(nbt[K() * kBlock + lane_id()]));
+                                gstore(ps.nb + (size_t)K() * prm.cap + p,
+                                       make_uint2(nbp[K() * kBlock + lane_id()] |
+                                                      ((uint32_t)(nbb[K() * kBlock + lane_id()] != 0) << 31),
+                                                  __float_as_uint(nbt[K() * kBlock + lane_id()])));
                         });
                     }
                 }
                 if (!listed && hit) {
                     // without the list the closest instance is its own nearby hit
                     flags |= 1u << 4;
-                    ps.nb[p] = make_uint2(h.inst | ((uint32_t)(h.back != 0) << 31), __float_as_uint(h.t));
+                    gstore(ps.nb + p, make_uint2(h.inst | ((uint32_t)(h.back != 0) << 31), __float_as_uint(h.t)));
                 }
-                ps.hit_a[p] = make_uint4(__float_as_uint(hit ? h.t : kTMax), hit ? h.inst : kEmpty, h.tri, flags);
-                ps.hit_b[p] = make_float4(h.b0, h.b1, h.b2, 0.0f);
+                gstore(ps.hit_a + p, make_uint4(__float_as_uint(hit ? h.t : kTMax), hit ? h.inst : kEmpty, h.tri, flags));
+                gstore(ps.hit_b + p, make_float4(h.b0, h.b1, h.b2, 0.0f));
             }
         }
         if (!rr.more && __ballot(r != kLaneIdle) == 0) break;
@@ -1466,8 +1466,8 @@ __device__ __forceinline__ void shadow_body(const DevScene &sc, const PtParams &
     int k = 0;
     int r = kLaneIdle;
     auto start_ray = [&]() {
-        const float4 o4 = ps.sh_o[(size_t)k * cap + p];
-        const float4 d4 = ps.sh_d[(size_t)k * cap + p];
+        const float4 o4 = gload(ps.sh_o + (size_t)k * cap + p);
+        const float4 d4 = gload(ps.sh_d + (size_t)k * cap + p);
         Query q;
         q.o = xyz(o4);
         q.d = xyz(d4);
@@ -1480,8 +1480,8 @@ __device__ __forceinline__ void shadow_body(const DevScene &sc, const PtParams &
     for (;;) {
         uint32_t item = 0;
         if (rr.more && rr.refill(r == kLaneIdle, item)) {
-            p = ps.q_shadow[item];
-            fl = ps.sh_flags[p];
+            p = gload(ps.q_shadow + item);
+            fl = gload(ps.sh_flags + p);
             k = __builtin_ctz(fl & kShActiveMask);  // a queued path records at least one ray
             start_ray();
         }
