@@ -125,7 +125,7 @@ def batch_timing(ctx, count=64, seed=11):
         tris += sc.info().triangle_count
         jobs.append((sc, abi.render_settings(res, res, spp, depth), 2))
     gen_s = time.time() - t0
-    plain_ms, plain_rays, plain_imgs = 0.0, 0, []
+    plain_ms, plain_rays, plain_imgs, plain_each = 0.0, 0, [], []
     for sc, rs, frame in jobs:
         sc.render_frame(rs, 1)
         torch.cuda.synchronize()
@@ -135,6 +135,7 @@ def batch_timing(ctx, count=64, seed=11):
         b.record()
         torch.cuda.synchronize()
         plain_ms += a.elapsed_time(b)
+        plain_each.append(a.elapsed_time(b))
         plain_rays += st.extension_rays + st.shadow_rays + st.transmittance_rays
         plain_imgs.append(img)
         sc.release_buffers()
@@ -149,6 +150,19 @@ def batch_timing(ctx, count=64, seed=11):
     rays = sum(st.extension_rays + st.shadow_rays + st.transmittance_rays for _, st in out)
     same = all(torch.equal(img, ref) for (img, _), ref in zip(out, plain_imgs))
     samples = sum(rs.width * rs.height * rs.spp for _, rs, _ in jobs)
+    # the frames of at most 512^2 x 16 samples on their own: the share of the
+    # workload where one frame cannot fill the GPU
+    small = [j for j, (_, rs, _) in enumerate(jobs) if rs.width * rs.height * rs.spp <= 512 * 512 * 16]
+    sub = [jobs[j] for j in small]
+    ctx.render_batch(sub)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    ctx.render_batch(sub)
+    b.record()
+    torch.cuda.synchronize()
+    small_batch_ms = a.elapsed_time(b)
+    small_plain_ms = sum(plain_each[j] for j in small)
     for sc, _, _ in jobs:
         sc.close()
     return {"workload": "%d scenes, %d triangles in all (1k-500k each, log-uniform), 256^2/512^2/1024^2, 4/16/64 "
@@ -158,7 +172,11 @@ def batch_timing(ctx, count=64, seed=11):
             "plain_mrays_per_s": round(plain_rays / (plain_ms / 1e3) / 1e6, 1),
             "batch_mrays_per_s": round(rays / (batch_ms / 1e3) / 1e6, 1),
             "batch_samples_per_s": round(samples / (batch_ms / 1e3), 1),
-            "images_bit_identical": bool(same), "scene_gen_s": round(gen_s, 1)}
+            "images_bit_identical": bool(same), "scene_gen_s": round(gen_s, 1),
+            "small_frames": {"frames": len(small), "plain_ms": round(small_plain_ms, 2),
+                             "batch_ms": round(small_batch_ms, 2),
+                             "speedup": round(small_plain_ms / small_batch_ms, 3),
+                             "share_of_plain_time": round(small_plain_ms / plain_ms, 3)}}
 
 
 def traversal_timing(ctx, reps=3):
