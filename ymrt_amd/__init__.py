@@ -47,6 +47,10 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise YmrtError("libymrt_hip.so is not built (run __graft_entry__.build() or make -C %s)" % CSRC)
+        # torch ships its own HIP runtime under the same soname: loaded first,
+        # it is the one process-wide runtime the library binds to; loaded
+        # after the library's, torch would find no GPU
+        import torch  # noqa: F401
         L = C.CDLL(LIB_PATH)
         vp = C.c_void_p
         L.ymrt_last_error.restype = C.c_char_p
