@@ -404,28 +404,28 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
         total_tris += in.tri_count;
         h.instances.push_back(in);
         if (!sphere && in.tri_count == 0) continue;
-        // world bounds of the geometry's object-space box
-        Aabb ob;
+        // world bounds: of the transformed vertices for a mesh (a rotated
+        // mesh's transformed object box can be far larger, and every ray
+        // through the excess enters the instance for nothing), of the
+        // transformed object box for a sphere
+        Aabb wb;
         if (sphere) {
             const float r = std::fabs(in.radius);
-            const float lo[3] = {-r, -r, -r}, hi[3] = {r, r, r};
-            ob.grow(lo);
-            ob.grow(hi);
+            for (int c = 0; c < 8; ++c) {
+                f3 corner{(c & 1) ? r : -r, (c & 2) ? r : -r, (c & 4) ? r : -r};
+                f3 w = xf_point(in.o2w, corner);
+                const float wk[3] = {w.x, w.y, w.z};
+                wb.grow(wk);
+            }
         } else {
             const ymrt_mesh_desc &m = d->meshes[id.geometry];
             for (uint32_t k = 0; k < m.position_count; ++k) {
-                const float pk[3] = {m.positions[3 * k], m.positions[3 * k + 1], m.positions[3 * k + 2]};
-                ob.grow(pk);
+                f3 w = xf_point(in.o2w, f3{m.positions[3 * k], m.positions[3 * k + 1], m.positions[3 * k + 2]});
+                const float wk[3] = {w.x, w.y, w.z};
+                wb.grow(wk);
             }
         }
-        Aabb wb;
-        for (int c = 0; c < 8; ++c) {
-            f3 corner{(c & 1) ? ob.hi[0] : ob.lo[0], (c & 2) ? ob.hi[1] : ob.lo[1], (c & 4) ? ob.hi[2] : ob.lo[2]};
-            f3 w = xf_point(in.o2w, corner);
-            const float wk[3] = {w.x, w.y, w.z};
-            wb.grow(wk);
-        }
-        // pad by a relative margin so rounding of the corner transform never culls
+        // pad by a relative margin so rounding of the transform never culls
         for (int a = 0; a < 3; ++a) {
             float pad = 1e-5f * std::max({1.0f, std::fabs(wb.lo[a]), std::fabs(wb.hi[a])});
             wb.lo[a] -= pad;
