@@ -392,3 +392,17 @@ def test_float_texture_and_texel_update(gpu_ctx):
         assert same >= 0.999
         assert err <= 1e-3
     assert float(np.abs(g1 - g0).mean()) > 1e-3
+
+
+def test_open_medium_parity(gpu_ctx):
+    """Rays that enter a medium through its only boundary face and then hit
+    nothing while inside it (free flight to infinity, volume vertices lit
+    by a distant light and the environment), and rays that miss it."""
+    sa = scenes.open_medium(48, 32)
+    rs = abi.render_settings(48, 32, 8, 4, volume=1, env_light=1, distant_light=1)
+    g, o, stats = render_both(gpu_ctx, sa, rs)
+    assert np.isfinite(g).all()
+    same, err = close_frac(g, o), rel_rmse(g, o)
+    print("open medium: close pixels %.4f  rel rmse %.2e" % (same, err))
+    assert same >= 0.999
+    assert err <= 1e-4

@@ -562,7 +562,9 @@ __device__ __forceinline__ void extend_body(const DevScene &sc, const PtParams &
                     Lv = Lv + light_clamp(rs, xyz(gload(ps.beta + p)) * background(tv.q.d), info_depth(info));
                 gstore(ps.out + p, f4(Lv, 0.0f));
             } else {
-                const uint32_t mat = gload(&sc.instances[h.inst].material);
+                // a path in a medium may have no surface hit (h.inst empty);
+                // one outside a medium always has one here
+                const uint32_t mat = hit ? gload(&sc.instances[h.inst].material) : kEmpty;
                 gstore(ps.flag + p, (pass << kFlagBits) | (in_medium ? kFlagMedium : shade_flag(gload(sc.mat_class + mat))));
                 uint32_t flags = h.back ? kHitBack : 0u;
                 bool listed = false;

@@ -725,3 +725,20 @@ def lit_floor(width=32, height=32, material=None, slab=None, light_half=0.4, fov
                         inner_volume=vol, boundary=True)
     sa.set_camera((0.0, 0.0, -0.5), (0.0, 0.0, -1.0), fov, width / height)
     return sa
+
+
+def open_medium(width=48, height=32, sigma=0.3):
+    """A boundary quad (a medium's only face, facing the camera) with a
+    homogeneous medium behind it and nothing else: camera rays pass through
+    the quad into the medium and then leave the scene with no surface hit
+    while inside it, lit by the environment (enable env_light) and by a
+    distant light."""
+    sa = abi.SceneArrays()
+    vol = sa.add_volume((sigma,) * 3, (0.8, 0.7, 0.6), 0.3)
+    bmat = sa.add_material(abi.default_material(ior_n=1.0))
+    sa.add_instance(_add(sa, quad((-1, -1, 0), (1, -1, 0), (1, 1, 0), (-1, 1, 0))), bmat, inner_volume=vol,
+                    boundary=True)
+    rot = np.eye(4, dtype=np.float32)
+    sa.add_distant_light((1.0, 0.95, 0.9), 2.0, 5.0, rot)
+    sa.set_camera((0.3, 0.2, 3.0), (0.0, 0.0, 0.0), 30.0, width / height)
+    return sa
