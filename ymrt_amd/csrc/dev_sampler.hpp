@@ -8,9 +8,9 @@
 // Integer digit extraction is exact; the float part is
 // fma(float(radical_sum), base_multiply, postfix_zero), the single fused
 // operation the reference's nvcc build (-fmad=true) emits for
-// `radical_sum * base_multiply + postfix_zero`.  Indices below 2^32 take a
-// 32-bit division path (same quotients, far cheaper than 64-bit division on
-// gfx950).
+// `radical_sum * base_multiply + postfix_zero`.  Indices below 2^32 take an
+// exact double-precision digit loop (same quotients and sums, far cheaper
+// than 64-bit integer division on gfx950).
 #pragma once
 
 #include "dev_scene.hpp"
@@ -92,38 +92,41 @@ YM_HD float radical_inverse_plain(uint64_t index)
     return gmax(0.000001f, gmin((float)radical_sum * base_multiply, 0.999999f));
 }
 
-// Quotient of a 32-bit n by the prime whose magic is ceil(2^64 / p): the high
-// word of n * magic.  Exact because n * (magic - 2^64/p) / 2^64 < 2^-32 is
-// smaller than the gap 1/p between n/p and the next integer.
-YM_HD uint32_t div_magic(uint32_t n, uint64_t magic)
-{
-    const uint64_t lo = (uint64_t)n * (uint32_t)magic;
-    const uint64_t hi = (uint64_t)n * (uint32_t)(magic >> 32) + (lo >> 32);
-    return (uint32_t)(hi >> 32);
-}
-
 // Permuted radical inverse of dimension dim >= 2 (base = dim-th prime),
-// HaltonSampler::RadicalInversion (src/RandomSampler.cuh:442-472).  Indices
-// below 2^32 divide by the base with a multiply-high by ceil(2^64 / base)
-// instead of a 64-bit division.
+// HaltonSampler::RadicalInversion (src/RandomSampler.cuh:442-472).
+//
+// Indices below 2^32 run the digit loop in double precision, where every
+// value is an integer below 2^53 and so exact: the quotient q = n / base is
+// floor(fma(n, 1/base, 2^-20)) -- with n < 2^32 and 5 <= base < 2^13 the
+// fma lands within 2^-21 of n / base + 2^-20, which lies in (q, q + 1)
+// because the fractional part of n / base is 0 or between 1/base and
+// 1 - 1/base -- and the sum radical_sum * base + perm[digit] stays below
+// base * 2^32 < 2^45.  Converting the exact sum to float rounds it as the
+// reference's integer conversion does, so the value is the same; the
+// double operations replace a 64-bit multiply-high and two 64-bit
+// multiply-adds per digit.
 YM_HD float radical_inverse(const DevScene &sc, uint64_t index, uint32_t dim)
 {
-    const uint32_t base = sc.primes[dim];
-    const uint32_t *perm = sc.halton + sc.prime_sums[dim];
+    const HaltonDim hd = sc.halton_dims[dim];
+    const uint32_t base = hd.base;
+    const uint32_t *perm = sc.halton + hd.perm_off;
     const float inv_base = 1.0f / (float)base;
-    uint64_t radical_sum = 0;
     float base_multiply = 1.0f;
+    float sum;
     if (index <= 0xFFFFFFFFull) {
-        const uint64_t magic = sc.prime_magic[dim];
+        const double base_d = (double)base;
+        double radical_sum = 0.0;
         uint32_t idx = (uint32_t)index;
         while (idx) {
-            const uint32_t next = div_magic(idx, magic);
+            const uint32_t next = (uint32_t)__builtin_fma((double)idx, hd.inv_base, 0x1p-20);
             const uint32_t digit = idx - next * base;
-            radical_sum = radical_sum * base + perm[digit];
+            radical_sum = __builtin_fma(radical_sum, base_d, (double)perm[digit]);
             base_multiply *= inv_base;
             idx = next;
         }
+        sum = (float)radical_sum;
     } else {
+        uint64_t radical_sum = 0;
         while (index) {
             const uint64_t next = index / base;
             const uint64_t digit = index - next * base;
@@ -131,9 +134,10 @@ YM_HD float radical_inverse(const DevScene &sc, uint64_t index, uint32_t dim)
             base_multiply *= inv_base;
             index = next;
         }
+        sum = (float)radical_sum;
     }
     const float postfix_zero = (float)perm[0] * base_multiply * inv_base / (1.0f - inv_base);
-    return gmax(0.000001f, gmin(__builtin_fmaf((float)radical_sum, base_multiply, postfix_zero), 0.999999f));
+    return gmax(0.000001f, gmin(__builtin_fmaf(sum, base_multiply, postfix_zero), 0.999999f));
 }
 
 YM_HD void pcg4d(uint32_t v[4])

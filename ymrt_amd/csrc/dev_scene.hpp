@@ -56,6 +56,14 @@ constexpr uint32_t kInstBoundary = 1u;
 constexpr uint32_t kInstNoNormals = 2u;   // mesh without normals: shade with the face normal
 constexpr uint32_t kInstSphere = 4u;      // analytic sphere of DevInstance::radius, no bottom-level tree
 
+// Halton dimension d (base = the d-th prime, 0-based): its digit
+// permutation at halton[perm_off] and 1 / base rounded to double.  One 16-B
+// load gives the sampler all it needs before the digit loads.
+struct alignas(16) HaltonDim {
+    uint32_t base, perm_off;
+    double inv_base;
+};
+
 struct alignas(16) ShadeTri {      // 96 B, by original triangle index
     float p[9];                    // p0, p1, p2
     float n[9];                    // n0, n1, n2 (face normal repeated when the mesh has none)
@@ -165,9 +173,7 @@ struct DevScene {
     const float *light_cdf;
     const DevDistant *distant;
     const uint32_t *halton;        // permutation table
-    const uint32_t *primes;        // first 1000 primes
-    const uint32_t *prime_sums;    // offset of each prime's permutation
-    const uint64_t *prime_magic;   // ceil(2^64 / prime): exact division of 32-bit values
+    const HaltonDim *halton_dims;  // per dimension: prime, permutation offset, 1 / prime
     uint32_t tlas_root;            // all instances: node index, leaf code, or kSentinel
     uint32_t tlas_root_solid;      // instances that are not medium boundaries (shadow rays)
     uint32_t tlas_root_boundary;   // medium boundaries only (transmittance walk)

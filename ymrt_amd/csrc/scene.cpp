@@ -743,14 +743,11 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
     if ((rc = upload(sc, lights, &ds.lights))) return rc;
     if ((rc = upload(sc, cdf, &ds.light_cdf))) return rc;
     if ((rc = upload(sc, table, &ds.halton))) return rc;
-    if ((rc = upload(sc, primes, &ds.primes))) return rc;
-    if ((rc = upload(sc, prime_sums, &ds.prime_sums))) return rc;
     if ((rc = upload(sc, distant, &ds.distant))) return rc;
     ds.distant_count = (uint32_t)distant.size();
-    // ceil(2^64 / p): the high 64 bits of n * magic are n / p for every n < 2^32
-    std::vector<uint64_t> magic(1000);
-    for (int i = 0; i < 1000; ++i) magic[i] = ~0ull / primes[i] + 1ull;
-    if ((rc = upload(sc, magic, &ds.prime_magic))) return rc;
+    std::vector<HaltonDim> hdims(1000);
+    for (int i = 0; i < 1000; ++i) hdims[i] = HaltonDim{primes[i], prime_sums[i], 1.0 / (double)primes[i]};
+    if ((rc = upload(sc, hdims, &ds.halton_dims))) return rc;
     ds.texture_count = d->texture_count;
     ds.grid_count = d->grid_count;
     ds.has_tex_media = tex_media ? 1 : 0;
