@@ -255,6 +255,31 @@ def test_update_materials(gpu_ctx):
         sc.update_materials(len(sa.materials), [red])
 
 
+@pytest.mark.parametrize("kw", [dict(), dict(sphere_light=True)], ids=["quad_light", "sphere_light"])
+def test_update_light_material(gpu_ctx, kw):
+    """A light material's colour updated in place: next-event estimation and
+    emission on hit use the new radiance, so the frame equals that of a scene
+    built with it.  The light-selection CDF keeps its build-time values, so
+    the new colour keeps the light's power (sum of the channels times the
+    intensity, 3.0 in both) and the two scenes' CDFs are the same floats."""
+    sa = scenes.cornell_box(32, 32, **kw)
+    rs = abi.render_settings(32, 32, 4, 5)
+    used = {inst["material"] for inst in sa.instances}
+    lights = [i for i, m in enumerate(sa.materials) if m.type == abi.MATERIAL_LIGHT and i in used]
+    assert len(lights) == 1
+    intensity = sa.materials[lights[0]].intensity
+    assert list(sa.materials[lights[0]].light_color) == [1.0, 1.0, 1.0]
+    sc = ymrt_amd.Scene(gpu_ctx, sa)
+    before = sc.render_frame(rs, 1)[0].clone()
+    sc.update_materials(lights[0], [abi.light_material((1.5, 1.0, 0.5), intensity)])
+    after = sc.render_frame(rs, 1)[0]
+    sb = scenes.cornell_box(32, 32, **kw)
+    sb.materials[lights[0]] = abi.light_material((1.5, 1.0, 0.5), intensity)
+    built = ymrt_amd.Scene(gpu_ctx, sb).render_frame(rs, 1)[0]
+    assert torch.equal(after, built)
+    assert not torch.equal(after, before)
+
+
 def test_render_batch_equals_single_frames(gpu_ctx):
     """ymrt_render_batch: frames of several scenes (one scene twice, with two
     frame indices) in shared launches equal ymrt_render_frame's bit for bit,

@@ -128,10 +128,32 @@ struct DevVolume {
     float tex_bound;               // its majorant, Texture::GetUpperBound
 };
 
+// A shape light as the scene build finds it (LoadShapeLight): power for the
+// selection CDF, and the sampling area.
 struct DevLight {
     uint32_t instance, triangle;
     float power, area;
 };
+
+// A shape light with everything next-event estimation reads from it: the
+// emitting triangle in object space with its vertex normals (or the sphere's
+// radius), the instance transforms and the emitted radiance
+// light_color * intensity.  Sampling a light is then one dependent load
+// after the pick, where the scene's own records take three (light ->
+// instance -> triangle and material).
+struct alignas(16) DevLightRec {
+    Affine w2o, o2w;
+    float p[9], n[9];
+    float le[3];
+    float area;         // DevLight::area
+    float radius;       // sphere lights
+    uint32_t triangle;  // kEmpty: a sphere
+};
+static_assert(sizeof(DevLightRec) == 192, "DevLightRec size");
+
+// Light-selection CDFs of up to this many lights are also kept in the scene
+// record, so the pick reads them with scalar loads.
+constexpr uint32_t kLightCdfInline = 8;
 
 struct DevDistant {
     f3 color;                      // light_color
@@ -169,8 +191,9 @@ struct DevScene {
     const DevGrid *grids;
     const float *voxels;
     const DevVolume *volumes;
-    const DevLight *lights;
+    const DevLightRec *light_recs;  // per shape light
     const float *light_cdf;
+    float light_cdf_inline[kLightCdfInline];  // light_cdf when light_count <= kLightCdfInline
     const DevDistant *distant;
     const uint32_t *halton;        // permutation table
     const HaltonDim *halton_dims;  // per dimension: prime, permutation offset, 1 / prime

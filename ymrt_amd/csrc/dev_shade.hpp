@@ -1458,9 +1458,29 @@ __device__ __forceinline__ f3 eval_tr(const DevScene &sc, f3 ro_w, f3 rd_w, cons
 }
 
 // ---------------------------------------------------------------- lights
-// SampleLightPower (src/Light.h:398-416)
+// SampleLightPower (src/Light.h:398-416).  The binary search returns the
+// first index whose CDF value exceeds u (or the last light), which on a
+// non-decreasing CDF is the number of values <= u: for up to
+// kLightCdfInline lights that count is taken over the copy of the CDF in the
+// scene record, with no dependent loads.
 __device__ __forceinline__ int sample_light_power(const DevScene &sc, float u, float &pdf)
 {
+    if (sc.light_count <= kLightCdfInline) {
+        int idx = 0;
+        float c_idx = 0.0f, c_prev = 0.0f;
+#pragma unroll
+        for (uint32_t i = 0; i < kLightCdfInline; ++i) {
+            if (i < sc.light_count && sc.light_cdf_inline[i] <= u) idx = (int)i + 1;
+        }
+        if (idx > (int)sc.light_count - 1) idx = (int)sc.light_count - 1;
+#pragma unroll
+        for (uint32_t i = 0; i < kLightCdfInline; ++i) {
+            if ((int)i == idx) c_idx = sc.light_cdf_inline[i];
+            if ((int)i + 1 == idx) c_prev = sc.light_cdf_inline[i];
+        }
+        pdf = idx == 0 ? c_idx : c_idx - c_prev;
+        return idx;
+    }
     int left = 0, right = (int)sc.light_count;
     while (left < right) {
         int mid = (left + right) / 2;
@@ -1475,23 +1495,19 @@ __device__ __forceinline__ int sample_light_power(const DevScene &sc, float u, f
 
 struct LightTri {
     f3 p0, p1, p2, n0, n1, n2;
-    const DevInstance *in;
     f3 le;
 };
 
-__device__ __forceinline__ LightTri load_light(const DevScene &sc, const DevLight &l)
+__device__ __forceinline__ LightTri load_light(const DevLightRec &l)
 {
     LightTri lt;
-    lt.in = &sc.instances[l.instance];
-    const ShadeTri &st = sc.shade[lt.in->shade_base + l.triangle];
-    lt.p0 = f3{st.p[0], st.p[1], st.p[2]};
-    lt.p1 = f3{st.p[3], st.p[4], st.p[5]};
-    lt.p2 = f3{st.p[6], st.p[7], st.p[8]};
-    lt.n0 = f3{st.n[0], st.n[1], st.n[2]};
-    lt.n1 = f3{st.n[3], st.n[4], st.n[5]};
-    lt.n2 = f3{st.n[6], st.n[7], st.n[8]};
-    const DevMaterial &m = sc.materials[lt.in->material];
-    lt.le = f3{m.light_color[0], m.light_color[1], m.light_color[2]} * m.intensity;
+    lt.p0 = f3{l.p[0], l.p[1], l.p[2]};
+    lt.p1 = f3{l.p[3], l.p[4], l.p[5]};
+    lt.p2 = f3{l.p[6], l.p[7], l.p[8]};
+    lt.n0 = f3{l.n[0], l.n[1], l.n[2]};
+    lt.n1 = f3{l.n[3], l.n[4], l.n[5]};
+    lt.n2 = f3{l.n[6], l.n[7], l.n[8]};
+    lt.le = f3{l.le[0], l.le[1], l.le[2]};
     return lt;
 }
 
@@ -1525,24 +1541,17 @@ YM_HD f3 sample_cone(float u0, float u1, float cos_theta_max)
     return f3{cp * st, sp * st, ct};
 }
 
-YM_HD f3 light_emission(const DevScene &sc, const DevInstance &in)
-{
-    const DevMaterial &m = sc.materials[in.material];
-    return f3{m.light_color[0], m.light_color[1], m.light_color[2]} * m.intensity;
-}
 
 YM_HD f3 affine_translate(const Affine &m) { return f3{m.r[3], m.r[7], m.r[11]}; }
 
 // ShapeLight::SampleLi, sphere branch (src/Light.h:282-317): a direction in
 // the cone the sphere subtends, uniform in solid angle.
-__device__ __forceinline__ bool sphere_light_sample_li(const DevScene &sc, const DevLight &l, f3 pos, float u0,
-                                                       float u1, f3 &wi, f3 &lpos, f3 &lerr, f3 &lng, f3 &color,
-                                                       float &pdf)
+__device__ __forceinline__ bool sphere_light_sample_li(const DevLightRec &l, f3 pos, float u0, float u1, f3 &wi,
+                                                       f3 &lpos, f3 &lerr, f3 &lng, f3 &color, float &pdf)
 {
-    const DevInstance &in = sc.instances[l.instance];
-    const f3 opos = xf_position(in.w2o, pos, nullptr);
+    const f3 opos = xf_position(l.w2o, pos, nullptr);
     const float lpo = length(opos);
-    const float radius = in.radius;
+    const float radius = l.radius;
     if (lpo < radius) {
         pdf = 0.0f;
         return false;
@@ -1559,24 +1568,23 @@ __device__ __forceinline__ bool sphere_light_sample_li(const DevScene &sc, const
     const float lio = length(ip);
     ip = ip * (radius * safe_rcp(lio));
     f3 ie = error_gamma(5) * abs3(ip);
-    lpos = xf_position(in.o2w, ip, &ie);
+    lpos = xf_position(l.o2w, ip, &ie);
     lerr = ie;
-    lng = normalize(lpos - affine_translate(in.o2w));
-    wi = normalize(xf_vector(in.o2w, cs));
+    lng = normalize(lpos - affine_translate(l.o2w));
+    wi = normalize(xf_vector(l.o2w, cs));
     pdf = 1.0f / cone;
-    color = light_emission(sc, in) * cone;
+    color = f3{l.le[0], l.le[1], l.le[2]} * cone;
     return true;
 }
 
 // ShapeLight::EvalLi, sphere branch (src/Light.h:163-199)
-__device__ __forceinline__ f3 sphere_light_eval_li(const DevScene &sc, const DevLight &l, f3 pos, f3 wi, f3 &lpos,
-                                                   f3 &lerr, f3 &lng, float &pdf)
+__device__ __forceinline__ f3 sphere_light_eval_li(const DevLightRec &l, f3 pos, f3 wi, f3 &lpos, f3 &lerr, f3 &lng,
+                                                   float &pdf)
 {
-    const DevInstance &in = sc.instances[l.instance];
-    const f3 owi = xf_vector(in.w2o, wi);
-    const f3 opos = xf_position(in.w2o, pos, nullptr);
+    const f3 owi = xf_vector(l.w2o, wi);
+    const f3 opos = xf_position(l.w2o, pos, nullptr);
     const float lpo = length(opos);
-    const float radius = in.radius;
+    const float radius = l.radius;
     float t0 = 0.0f, t1 = 0.0f;
     if (lpo < radius || !ray_sphere_light(opos, owi, radius, t0, t1)) {
         pdf = 0.0f;
@@ -1593,19 +1601,19 @@ __device__ __forceinline__ f3 sphere_light_eval_li(const DevScene &sc, const Dev
     const float lio = length(ip);
     ip = ip * (radius * safe_rcp(lio));
     f3 ie = error_gamma(5) * abs3(ip);
-    lpos = xf_position(in.o2w, ip, &ie);
+    lpos = xf_position(l.o2w, ip, &ie);
     lerr = ie;
-    lng = normalize(lpos - affine_translate(in.o2w));
+    lng = normalize(lpos - affine_translate(l.o2w));
     pdf = 1.0f / cone;
-    return light_emission(sc, in);
+    return f3{l.le[0], l.le[1], l.le[2]};
 }
 
 // ShapeLight::SampleLi, mesh branch (src/Light.h:207-281)
-__device__ __forceinline__ bool light_sample_li(const DevScene &sc, const DevLight &l, f3 pos, float u0, float u1,
-                                                f3 &wi, f3 &lpos, f3 &lerr, f3 &lng, f3 &color, float &pdf)
+__device__ __forceinline__ bool light_sample_li(const DevLightRec &l, f3 pos, float u0, float u1, f3 &wi, f3 &lpos,
+                                                f3 &lerr, f3 &lng, f3 &color, float &pdf)
 {
-    if (l.triangle == kEmpty) return sphere_light_sample_li(sc, l, pos, u0, u1, wi, lpos, lerr, lng, color, pdf);
-    const LightTri lt = load_light(sc, l);
+    if (l.triangle == kEmpty) return sphere_light_sample_li(l, pos, u0, u1, wi, lpos, lerr, lng, color, pdf);
+    const LightTri lt = load_light(l);
     const f3 gn = cross(lt.p1 - lt.p0, lt.p2 - lt.p0);
     const float su = sqrtf(gmax(0.0f, u0));
     const float bu = 1.0f - su, bv = u1 * su;
@@ -1616,10 +1624,10 @@ __device__ __forceinline__ bool light_sample_li(const DevScene &sc, const DevLig
     const f3 sp = interp3(lt.p0, lt.p1, lt.p2, b);
     f3 se = error_gamma(7) * f3{xs, ys, zs};
     const f3 sn = interp3(lt.n0, lt.n1, lt.n2, b);
-    const f3 wn = normalize(xf_transpose_dir(lt.in->w2o, sn));
-    lpos = xf_position(lt.in->o2w, sp, &se);
+    const f3 wn = normalize(xf_transpose_dir(l.w2o, sn));
+    lpos = xf_position(l.o2w, sp, &se);
     lerr = se;
-    lng = normalize(xf_transpose_vector(lt.in->w2o, gn));
+    lng = normalize(xf_transpose_vector(l.w2o, gn));
     const float dist = length(lpos - pos);
     wi = (lpos - pos) * safe_rcp(dist);
     const float ct = -dot(wn, wi);
@@ -1670,14 +1678,14 @@ YM_HD bool ray_tri_light(f3 o, f3 d, f3 p0, f3 p1, f3 p2, f3 &bary)
 }
 
 // ShapeLight::EvalLi, mesh branch (src/Light.h:87-162)
-__device__ __forceinline__ f3 light_eval_li(const DevScene &sc, const DevLight &l, f3 pos, f3 wi, f3 &lpos, f3 &lerr,
-                                            f3 &lng, float &pdf)
+__device__ __forceinline__ f3 light_eval_li(const DevLightRec &l, f3 pos, f3 wi, f3 &lpos, f3 &lerr, f3 &lng,
+                                            float &pdf)
 {
-    if (l.triangle == kEmpty) return sphere_light_eval_li(sc, l, pos, wi, lpos, lerr, lng, pdf);
-    const LightTri lt = load_light(sc, l);
+    if (l.triangle == kEmpty) return sphere_light_eval_li(l, pos, wi, lpos, lerr, lng, pdf);
+    const LightTri lt = load_light(l);
     const f3 gn = cross(lt.p1 - lt.p0, lt.p2 - lt.p0);
-    const f3 owi = xf_vector(lt.in->w2o, wi);
-    const f3 opos = xf_position(lt.in->w2o, pos, nullptr);
+    const f3 owi = xf_vector(l.w2o, wi);
+    const f3 opos = xf_position(l.w2o, pos, nullptr);
     f3 b = s3(0.0f);
     if (!ray_tri_light(opos, owi, lt.p0, lt.p1, lt.p2, b)) {
         pdf = 0.0f;
@@ -1689,15 +1697,15 @@ __device__ __forceinline__ f3 light_eval_li(const DevScene &sc, const DevLight &
     const f3 sp = interp3(lt.p0, lt.p1, lt.p2, b);
     f3 se = error_gamma(7) * f3{xs, ys, zs};
     const f3 sn = interp3(lt.n0, lt.n1, lt.n2, b);
-    const f3 wn = normalize(xf_transpose_dir(lt.in->w2o, sn));
+    const f3 wn = normalize(xf_transpose_dir(l.w2o, sn));
     const float ct = -dot(wn, wi);
     if (ct <= 0.0f) {
         pdf = 0.0f;
         return s3(0.0f);
     }
-    lpos = xf_position(lt.in->o2w, sp, &se);
+    lpos = xf_position(l.o2w, sp, &se);
     lerr = se;
-    lng = normalize(xf_transpose_vector(lt.in->w2o, gn));
+    lng = normalize(xf_transpose_vector(l.w2o, gn));
     const float dist = length(lpos - pos);
     pdf = sqr(dist) * safe_rcp(l.area * ct);
     return lt.le;

@@ -61,6 +61,8 @@ struct ymrt_scene {
     std::vector<ym::DevTexture> textures;  // texture records (float texel updates)
     std::vector<ym::DevVolume> volumes;    // volume and grid records (adjoint targets)
     std::vector<ym::DevGrid> grids;
+    std::vector<ym::DevLightRec> light_recs;  // shape-light sampling records, whose emission
+    std::vector<uint32_t> light_material;     // follows updates of these materials
     int32_t *d_adj_slots = nullptr;        // adjoint: 2 x material_count target slots
     uint32_t texture_count = 0;
     bool has_boundaries = false;

@@ -1010,13 +1010,13 @@ __device__ __forceinline__ void shade_body(const DevScene &sc, const PtParams &p
                 }
                 if (rs.enable_shape_light && sc.light_count > 0 && nv > 0) {
                     const int li = sample_light_power(sc, sampler_next(sc, smp), sel_pdf);
-                    const DevLight light = sc.lights[li];
+                    const DevLightRec &light = sc.light_recs[li];
                     {
                         f3 ldir = s3(0.0f), lpos = s3(0.0f), lerr = s3(0.0f), lng = s3(0.0f), Li = s3(0.0f);
                         float lpdf = 0.0f;
                         const float u0 = sampler_next(sc, smp);
                         const float u1 = sampler_next(sc, smp);
-                        const bool valid = light_sample_li(sc, light, vpos, u0, u1, ldir, lpos, lerr, lng, Li, lpdf);
+                        const bool valid = light_sample_li(light, vpos, u0, u1, ldir, lpos, lerr, lng, Li, lpdf);
                         float wpdf = 0.0f;
                         const float ph = mixed_phase_eval(vw, gs, nv, -ray_d, ldir, wpdf);
                         if (ph > kMinColorEpsilon && valid)
@@ -1032,7 +1032,7 @@ __device__ __forceinline__ void shade_body(const DevScene &sc, const PtParams &p
                         if (mixed_phase_sample(vw, gs, nv, u0, u1, -ray_d, pw, pwi, ppdf)) {
                             f3 lpos = s3(0.0f), lerr = s3(0.0f), lng = s3(0.0f);
                             float lpdf = 0.0f;
-                            const f3 Li = light_eval_li(sc, light, vpos, pwi, lpos, lerr, lng, lpdf);
+                            const f3 Li = light_eval_li(light, vpos, pwi, lpos, lerr, lng, lpdf);
                             if (maxcomp(Li) > kMinColorEpsilon)
                                 so.emit(1, vpos, pwi, shadow_tmax(vpos, pwi, lpos, lerr, lng), false,
                                         s3(gmin(pw, kMaxColorClamp)), min3(Li, s3(kMaxColorClamp)),
@@ -1247,13 +1247,13 @@ __device__ __forceinline__ void shade_body(const DevScene &sc, const PtParams &p
                     }
                     if (rs.enable_shape_light && sc.light_count > 0) {
                         const int li = sample_light_power(sc, sampler_next(sc, smp), sel_pdf);
-                        const DevLight light = sc.lights[li];
+                        const DevLightRec &light = sc.light_recs[li];
                         {
                             f3 ldir = s3(0.0f), lpos = s3(0.0f), lerr = s3(0.0f), lng = s3(0.0f), Li = s3(0.0f);
                             float lpdf = 0.0f;
                             const float u0 = sampler_next(sc, smp);
                             const float u1 = sampler_next(sc, smp);
-                            const bool valid = light_sample_li(sc, light, pos, u0, u1, ldir, lpos, lerr, lng, Li, lpdf);
+                            const bool valid = light_sample_li(light, pos, u0, u1, ldir, lpos, lerr, lng, Li, lpdf);
                             const f3 lwi = bs.to_local(ldir);
                             float wpdf = 0.0f;
                             const f3 f = bsdf_eval<kLobes>(bs, wo, lwi, wpdf);
@@ -1278,7 +1278,7 @@ __device__ __forceinline__ void shade_body(const DevScene &sc, const PtParams &p
                                 const f3 ldir = bs.to_world(bwi);
                                 f3 lpos = s3(0.0f), lerr = s3(0.0f), lng = s3(0.0f);
                                 float lpdf = 0.0f;
-                                const f3 Li = light_eval_li(sc, light, pos, ldir, lpos, lerr, lng, lpdf);
+                                const f3 Li = light_eval_li(light, pos, ldir, lpos, lerr, lng, lpdf);
                                 if (maxcomp(Li) > kMinColorEpsilon) {
                                     const bool transmit = dot(-ray_d, ng) * dot(ldir, ng) < 0.0f;
                                     const bool invalid =

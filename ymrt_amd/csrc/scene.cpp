@@ -288,6 +288,12 @@ static void camera_media(const DevScene &hs, const HostScene &h, const ymrt_scen
     (void)d;
 }
 
+// light_color * intensity, as the shading code forms a light's radiance
+static void light_emission_of(const DevMaterial &m, float le[3])
+{
+    for (int c = 0; c < 3; ++c) le[c] = m.light_color[c] * m.intensity;
+}
+
 int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
 {
     auto t0 = std::chrono::high_resolution_clock::now();
@@ -660,6 +666,26 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
     }
     float trcp = safe_rcp(total);
     for (size_t i = 0; i < cdf.size(); ++i) cdf[i] = total == 0.0f ? 0.0f : cdf[i] * trcp;
+    std::vector<DevLightRec> light_recs(lights.size());
+    std::vector<uint32_t> light_material(lights.size());
+    for (size_t k = 0; k < lights.size(); ++k) {
+        const DevLight &l = lights[k];
+        const DevInstance &in = h.instances[l.instance];
+        DevLightRec &r = light_recs[k];
+        std::memset(&r, 0, sizeof(r));
+        r.w2o = in.w2o;
+        r.o2w = in.o2w;
+        if (l.triangle != kEmpty) {
+            const ShadeTri &st = h.shade[in.shade_base + l.triangle];
+            std::memcpy(r.p, st.p, sizeof(r.p));
+            std::memcpy(r.n, st.n, sizeof(r.n));
+        }
+        light_material[k] = in.material;
+        light_emission_of(h.materials[in.material], r.le);
+        r.area = l.area;
+        r.radius = in.radius;
+        r.triangle = l.triangle;
+    }
     // ------------------------------------------------------------ distant lights
     std::vector<DevDistant> distant;
     YM_REQUIRE(d->distant_light_count == 0 || d->distant_lights != nullptr, "distant_lights is NULL");
@@ -740,8 +766,11 @@ int build_scene(ymrt_scene *sc, const ymrt_scene_desc *d)
     if ((rc = upload(sc, vols, &ds.volumes))) return rc;
     sc->grids = grids;
     sc->volumes = vols;
-    if ((rc = upload(sc, lights, &ds.lights))) return rc;
+    if ((rc = upload(sc, light_recs, &ds.light_recs))) return rc;
     if ((rc = upload(sc, cdf, &ds.light_cdf))) return rc;
+    for (uint32_t i = 0; i < kLightCdfInline; ++i) ds.light_cdf_inline[i] = i < cdf.size() ? cdf[i] : 1.0f;
+    sc->light_recs = light_recs;
+    sc->light_material = light_material;
     if ((rc = upload(sc, table, &ds.halton))) return rc;
     if ((rc = upload(sc, distant, &ds.distant))) return rc;
     ds.distant_count = (uint32_t)distant.size();
@@ -793,6 +822,17 @@ int update_materials(ymrt_scene *sc, uint32_t first, uint32_t count, const ymrt_
                           sizeof(DevMaterial) * count, hipMemcpyHostToDevice, s));
     YM_HIP(hipMemcpyAsync(const_cast<uint32_t *>(sc->dev.mat_class) + first, dif.data(), sizeof(uint32_t) * count,
                           hipMemcpyHostToDevice, s));
+    // the shape lights of an updated light material emit its new radiance
+    bool lights_changed = false;
+    for (size_t k = 0; k < sc->light_recs.size(); ++k) {
+        const uint32_t lm = sc->light_material[k];
+        if (lm < first || lm >= first + count) continue;
+        light_emission_of(next[lm], sc->light_recs[k].le);
+        lights_changed = true;
+    }
+    if (lights_changed)
+        YM_HIP(hipMemcpyAsync(const_cast<DevLightRec *>(sc->dev.light_recs), sc->light_recs.data(),
+                              sizeof(DevLightRec) * sc->light_recs.size(), hipMemcpyHostToDevice, s));
     YM_HIP(hipStreamSynchronize(s));
     sc->materials.swap(next);
     sc->needs_nearby = needs_nearby(sc->has_boundaries, sc->materials);
