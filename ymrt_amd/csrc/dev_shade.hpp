@@ -165,16 +165,34 @@ __device__ __forceinline__ void boundary_bunch(const DevScene &sc, Transfer &t, 
 // ---------------------------------------------------------------- textures
 // EvalTexImage (src/TextureEval.cuh:162-237) over RGBA8 texels decoded through
 // the sRGB / linear table (LoadPng8, src/FileIO.cpp:17-45).
+// ((x % w) + w) % w, the repeat wrap, for w >= 1: one unsigned remainder
+// for x >= 0, none for x = -1 (the left or upper neighbour of texel 0, the
+// only negative coordinate a lookup makes), the signed formula otherwise.
+__device__ __forceinline__ int wrap_repeat(int x, int w)
+{
+    if (x >= 0) return (int)((uint32_t)x % (uint32_t)w);
+    if (x == -1) return w - 1;
+    return ((x % w) + w) % w;
+}
+
+// Texel columns (or rows) c and c + 1 after the texture's wrap: clamped to
+// [0, w - 1], or the repeat wrap of c and its successor ((c + 1) mod w
+// follows from c mod w without a second remainder).
+__device__ __forceinline__ void wrap_pair(const DevTexture &t, int c, int w, int &c0, int &c1)
+{
+    if (t.wrap == YMRT_WRAP_CLAMP) {
+        c0 = c < 0 ? 0 : (c > w - 1 ? w - 1 : c);
+        c1 = c + 1 < 0 ? 0 : (c + 1 > w - 1 ? w - 1 : c + 1);
+    } else {
+        c0 = wrap_repeat(c, w);
+        c1 = c0 + 1 == w ? 0 : c0 + 1;
+    }
+}
+
+// The texel at wrapped coordinates (x, y).
 __device__ __forceinline__ f3 texel(const DevScene &sc, const DevTexture &t, int x, int y)
 {
-    const int w = (int)t.width, h = (int)t.height;
-    if (t.wrap == YMRT_WRAP_CLAMP) {
-        x = x < 0 ? 0 : (x > w - 1 ? w - 1 : x);
-        y = y < 0 ? 0 : (y > h - 1 ? h - 1 : y);
-    } else {
-        x = ((x % w) + w) % w;
-        y = ((y % h) + h) % h;
-    }
+    const int w = (int)t.width;
     if (t.f32) {
         const float *p = sc.texels_f32 + t.offset + ((uint64_t)y * w + x) * t.channels;
         if (t.channels == 1) return s3(p[0]);
@@ -199,10 +217,13 @@ __device__ __forceinline__ f3 image_eval(const DevScene &sc, const DevTexture &t
     const float cx = tu * (float)t.width - 0.5f, cy = tv * (float)t.height - 0.5f;
     const int x0 = (int)floorf(cx), y0 = (int)floorf(cy);
     const float fx = cx - (float)x0, fy = cy - (float)y0;
-    f3 r = ((1.0f - fx) * (1.0f - fy)) * texel(sc, t, x0, y0);
-    r = r + (fx * (1.0f - fy)) * texel(sc, t, x0 + 1, y0);
-    r = r + ((1.0f - fx) * fy) * texel(sc, t, x0, y0 + 1);
-    r = r + (fx * fy) * texel(sc, t, x0 + 1, y0 + 1);
+    int xa, xb, ya, yb;
+    wrap_pair(t, x0, (int)t.width, xa, xb);
+    wrap_pair(t, y0, (int)t.height, ya, yb);
+    f3 r = ((1.0f - fx) * (1.0f - fy)) * texel(sc, t, xa, ya);
+    r = r + (fx * (1.0f - fy)) * texel(sc, t, xb, ya);
+    r = r + ((1.0f - fx) * fy) * texel(sc, t, xa, yb);
+    r = r + (fx * fy) * texel(sc, t, xb, yb);
     return r;
 }
 
@@ -222,19 +243,12 @@ __device__ __forceinline__ void image_taps(const DevTexture &t, f2 uv, uint32_t 
     w[1] = fx * (1.0f - fy);
     w[2] = (1.0f - fx) * fy;
     w[3] = fx * fy;
-    const int wd = (int)t.width, ht = (int)t.height;
+    int xs[2], ys[2];
+    wrap_pair(t, x0, (int)t.width, xs[0], xs[1]);
+    wrap_pair(t, y0, (int)t.height, ys[0], ys[1]);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        int x = x0 + (k & 1), y = y0 + (k >> 1);
-        if (t.wrap == YMRT_WRAP_CLAMP) {
-            x = x < 0 ? 0 : (x > wd - 1 ? wd - 1 : x);
-            y = y < 0 ? 0 : (y > ht - 1 ? ht - 1 : y);
-        } else {
-            x = ((x % wd) + wd) % wd;
-            y = ((y % ht) + ht) % ht;
-        }
-        off[k] = ((uint32_t)y * t.width + (uint32_t)x) * t.channels;
-    }
+    for (int k = 0; k < 4; ++k)
+        off[k] = ((uint32_t)ys[k >> 1] * t.width + (uint32_t)xs[k & 1]) * t.channels;
 }
 
 // Perlin noise (PerlinNoiseEval, src/TextureEval.cuh:55-131): Ken Perlin's
