@@ -138,6 +138,26 @@ def test_cornell_sphere_parity(gpu_ctx, kw):
     assert err <= 1e-4
 
 
+def test_many_lights_parity(gpu_ctx):
+    """More shape lights than the scene record's inline selection CDF holds
+    (8): the pick takes the binary search over the CDF in HBM.  Nine ceiling
+    tiles of their own intensities (18 triangles) besides the main light."""
+    sa = scenes.cornell_box(48, 48)
+    at = (0.05, 0.2, 0.75)  # 0.1-wide tiles clear of each other and of the main light
+    for k in range(9):
+        x0, z0 = at[k % 3], at[k // 3]
+        tile = scenes.quad((x0, 0.998, z0), (x0 + 0.1, 0.998, z0), (x0 + 0.1, 0.998, z0 + 0.1), (x0, 0.998, z0 + 0.1))
+        sa.add_instance(scenes._add(sa, tile), sa.add_material(abi.light_material((1.0, 0.9, 0.8), 4.0 + k)))
+    rs = abi.render_settings(48, 48, 8, 6)
+    g, o, stats = render_both(gpu_ctx, sa, rs)
+    assert ymrt_amd.Scene(gpu_ctx, sa).info().shape_light_count == 20
+    assert np.isfinite(g).all()
+    same, err = close_frac(g, o), rel_rmse(g, o)
+    print("cornell with 20 light triangles: close pixels %.4f  rel rmse %.5f" % (same, err))
+    assert same >= 0.999
+    assert err <= 1e-4
+
+
 @pytest.mark.parametrize("fog", [False, True], ids=["clear", "fog"])
 def test_outdoor_distant_light_parity(gpu_ctx, fog):
     """Distant light (EvalDistantLight / EvalVolumeDistantLight: cone sampling,
