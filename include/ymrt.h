@@ -314,8 +314,9 @@ int ymrt_scene_get_info(const ymrt_scene *scene, ymrt_scene_info *info);
  * (SceneModule::UpdateMaterial, src/core/SceneModule.cpp:654-663, which the
  * editor calls after changing a material).  Synchronous on stream.  As in
  * the reference, the shape-light CDF and the camera's medium records keep
- * the values the scene was built with; a material may not change its type
- * (YMRT_ERR_INVALID_ARGUMENT). */
+ * the values the scene was built with, while a light material's new colour
+ * and intensity are what its lights emit from the next frame on; a material
+ * may not change its type (YMRT_ERR_INVALID_ARGUMENT). */
 int ymrt_scene_update_materials(ymrt_scene *scene, uint32_t first, uint32_t count,
                                 const ymrt_material_desc *materials, void *stream);
 
