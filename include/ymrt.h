@@ -216,6 +216,9 @@ typedef struct ymrt_scene_desc {
 
 /* ---------------------------------------------------------------- ray queries */
 
+/* A query ray: hits count for t in [tmin, tmax] along d as given.  The
+ * reference's rays have no tmin (TraceRay starts at TMIN = 1e-7); pass 1e-7
+ * for its answers. */
 typedef struct ymrt_ray {          /* 32 B */
     float ox, oy, oz, tmin;
     float dx, dy, dz, tmax;
@@ -341,7 +344,7 @@ int ymrt_scene_update_texture(ymrt_scene *scene, uint32_t texture, const float *
  * density gradient writes its updated grid back this way. */
 int ymrt_scene_update_grid(ymrt_scene *scene, uint32_t grid, const float *density, void *stream);
 
-/* Closest hit (TraceRay) for n device-resident rays. */
+/* Closest hit (TraceRay) in [tmin, tmax] for n device-resident rays. */
 int ymrt_trace_closest(ymrt_scene *scene, const ymrt_ray *rays, uint32_t n,
                        ymrt_hit *hits, void *stream);
 /* Any hit (BVHTraverseShadow): occluded[i] = 1 when some non-boundary

@@ -156,3 +156,46 @@ def test_closest_spheres_and_meshes(gpu_ctx):
     assert sphere_hits.sum() > 3000
     occ = sc.trace_any(ymrt_amd.Scene.rays_to_device(shadow)).cpu().numpy()
     assert (occ == osc.trace_any(shadow)).mean() >= 0.9999
+
+
+def test_edge_inputs(gpu_ctx):
+    """Degenerate inputs: no rays; rays whose interval [tmin, tmax] is empty,
+    starts past the only surface on their line or ends before it; a scene
+    without instances, where every query misses and a frame is the
+    environment alone.  The answers are fixed by the geometry; the oracle,
+    which like TraceRay starts every ray at TMIN (the reference's rays have
+    no tmin), agrees on every ray whose answer does not depend on tmin."""
+    sa = scenes.cornell_box(16, 16)
+    sc = ymrt_amd.Scene(gpu_ctx, sa)
+    empty = torch.empty((0, 8), dtype=torch.float32, device="cuda")
+    assert tuple(sc.trace_closest(empty).shape) == (0, 4)
+    assert tuple(sc.trace_any(empty).shape) == (0,)
+    # straight down from the middle of the box: the floor at t = 0.5 is the
+    # only surface on the line (the boxes and the sphere are off it)
+    rays = np.zeros(4, abi.RAY_DTYPE)
+    rays["o"] = (0.5, 0.5, 0.5)
+    rays["d"] = (0.0, -1.0, 0.0)
+    rays["tmin"] = (1e-4, 0.6, 1e-4, 0.3)
+    rays["tmax"] = (1e30, 1e30, 0.4, 0.2)
+    gh = ymrt_amd.Scene.hits_to_numpy(sc.trace_closest(ymrt_amd.Scene.rays_to_device(rays)))
+    assert list(gh["instance"] != 0xFFFFFFFF) == [True, False, False, False]
+    assert abs(float(gh["t"][0]) - 0.5) < 1e-6
+    occ = sc.trace_any(ymrt_amd.Scene.rays_to_device(rays)).cpu().numpy()
+    assert list(occ) == [1, 0, 0, 0]
+    same = [0, 2, 3]  # ray 1 starts past the floor: only the GPU's queries honour tmin
+    oh = oracle.Scene(sa).trace_closest(rays)
+    assert np.array_equal(gh["instance"][same], oh["instance"][same])
+    assert np.array_equal(gh["triangle"][same], oh["triangle"][same])
+    assert gh["t"][0] == oh["t"][0]
+    assert list(oracle.Scene(sa).trace_any(rays)[same]) == [1, 0, 0]
+
+    bare = abi.SceneArrays()
+    bare.set_camera((0.0, 0.0, 1.0), (0.0, 0.0, 0.0), 40.0, 1.0)
+    sb = ymrt_amd.Scene(gpu_ctx, bare)
+    hb = ymrt_amd.Scene.hits_to_numpy(sb.trace_closest(ymrt_amd.Scene.rays_to_device(rays)))
+    assert np.all(hb["instance"] == 0xFFFFFFFF)
+    assert not sb.trace_any(ymrt_amd.Scene.rays_to_device(rays)).cpu().numpy().any()
+    rs = abi.render_settings(3, 2, 2, 4)
+    img = sb.render_frame(rs, 1)[0].cpu().numpy()
+    ref = oracle.Scene(bare).render(rs, 1)
+    assert np.array_equal(img[..., :3], ref[..., :3])
