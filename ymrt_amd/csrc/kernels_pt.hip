@@ -92,9 +92,10 @@ __device__ __forceinline__ void run_items(uint32_t *fetch, uint32_t n, uint32_t 
 // wave that waited for its slowest ray would run most of its steps with few
 // lanes active.  refill() hands the wave's idle lanes the next items of its
 // fetched range [cur, end), fetching kSub * 64 more as the range runs dry; it
-// returns true on the lanes that took item *item.  Refilling at 24 (extend)
-// and 16 (shadow) idle lanes measured fastest on the interior frame (8 and 32
-// were 1-3 % slower).
+// returns true on the lanes that took item *item.  Refilling at 32 (extend)
+// and 16 (shadow) idle lanes measured fastest on the interior frame (extend:
+// 16, 24, 28, 40 and 48 were 1-13 % slower; shadow: 8, 24 and 32 were
+// slower or the same).
 constexpr int kLaneIdle = 3;  // no ray (beside kTravContinue/Hit/Miss/Overflow)
 template <int kRefillLanes>
 struct RefillRange {
@@ -505,7 +506,7 @@ __device__ __forceinline__ void extend_body(const DevScene &sc, const PtParams &
     const ymrt_render_settings &rs = prm.rs;
     uint32_t rays = 0, cam_rays = 0;
     TraceCounters tc{0, 0};
-    RefillRange<24> rr{cnt + kCtrWorkExtend, cnt[kCtrActive], 0u, 0u, true};
+    RefillRange<32> rr{cnt + kCtrWorkExtend, cnt[kCtrActive], 0u, 0u, true};
 
     Traversal<kModeAll, false, kNearby, true, kSphere, kStack> tv;
     uint32_t p = 0;
