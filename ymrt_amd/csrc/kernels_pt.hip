@@ -31,13 +31,17 @@ __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 // global counter here is touched by all waves of the grid, and device-scope
 // atomics on one address serialise across the chip, so each wave fetches
 // kSub * 64 items at once and publishes each output queue's survivors of all
-// sub-batches with one atomic.
+// sub-batches with one atomic.  Stages with little work per item take more
+// per fetch (ray generation 32 sub-batches, free flight 8), where the one
+// counter's atomics would otherwise set the pace; the default of 4 measured
+// best for the others.
 constexpr int kSub = 4;
 
+template <int kS = kSub>
 __device__ __forceinline__ uint32_t wave_fetch(uint32_t *work)
 {
     uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(work, (uint32_t)(kBlock * kSub));
+    if (lane_id() == 0) base = atomicAdd(work, (uint32_t)(kBlock * kS));
     return __shfl(base, 0);
 }
 
@@ -45,19 +49,19 @@ __device__ __forceinline__ uint32_t wave_fetch(uint32_t *work)
 // item i < n, sets p (the path slot it appends) and out[q] for each output
 // queue q.  Survivors of a fetch's sub-batches are staged in LDS and then
 // published with one atomic per queue and a coalesced copy.
-template <int kQueues, typename Body>
+template <int kQueues, int kS = kSub, typename Body>
 __device__ __forceinline__ void run_items(uint32_t *fetch, uint32_t n, uint32_t *const *counters,
                                           uint32_t *const *queues, Body &&body)
 {
     constexpr int kQ = kQueues > 0 ? kQueues : 1;
-    __shared__ uint32_t stage[kQ * kSub * kBlock];
+    __shared__ uint32_t stage[kQ * kS * kBlock];
     const uint64_t below = (1ull << lane_id()) - 1ull;
     for (;;) {
-        const uint32_t base = wave_fetch(fetch);
+        const uint32_t base = wave_fetch<kS>(fetch);
         if (base >= n) break;
         uint32_t filled[kQ] = {};
 #pragma unroll 1
-        for (int k = 0; k < kSub; ++k) {
+        for (int k = 0; k < kS; ++k) {
             const uint32_t first = base + (uint32_t)k * kBlock;
             if (first >= n) break;
             const uint32_t i = first + lane_id();
@@ -66,7 +70,7 @@ __device__ __forceinline__ void run_items(uint32_t *fetch, uint32_t n, uint32_t 
             if (i < n) body(i, p, out);
             static_for<0, kQueues>([&](auto Q) {
                 const uint64_t m = __ballot(out[Q()]);
-                if (out[Q()]) stage[Q() * kSub * kBlock + filled[Q()] + (uint32_t)__popcll(m & below)] = p;
+                if (out[Q()]) stage[Q() * kS * kBlock + filled[Q()] + (uint32_t)__popcll(m & below)] = p;
                 filled[Q()] += (uint32_t)__popcll(m);
             });
         }
@@ -78,7 +82,7 @@ __device__ __forceinline__ void run_items(uint32_t *fetch, uint32_t n, uint32_t 
                     if (lane_id() == 0) dst = atomicAdd(counters[Q()], filled[Q()]);
                     dst = __shfl(dst, 0);
                     for (uint32_t j = lane_id(); j < filled[Q()]; j += kBlock)
-                        queues[Q()][dst + j] = stage[Q() * kSub * kBlock + j];
+                        queues[Q()][dst + j] = stage[Q() * kS * kBlock + j];
                 }
             });
             __syncthreads();
@@ -415,7 +419,7 @@ __device__ __forceinline__ void raygen_body(const DevScene &sc, const PtParams &
     uint32_t *cnt = prm.counters;  // pass 0 slots
     uint32_t *const ctrs[1] = {cnt + kCtrActive};
     uint32_t *const qs[1] = {ps.queue[0]};
-    run_items<1>(cnt + kCtrWorkRaygen, prm.chunk_paths, ctrs, qs, [&](uint32_t p, uint32_t &slot, bool(&out)[1]) {
+    run_items<1, 32>(cnt + kCtrWorkRaygen, prm.chunk_paths, ctrs, qs, [&](uint32_t p, uint32_t &slot, bool(&out)[1]) {
         slot = p;
         ps.flag[p] = 0;
         {
@@ -775,7 +779,7 @@ __device__ __forceinline__ void medium_body(const DevScene &sc, const PtParams &
     uint32_t *cnt = prm.counters + pass * kCtrStride;
     const uint32_t n = cnt[kCtrMedium];
     const ymrt_render_settings &rs = prm.rs;
-    run_items<0>(cnt + kCtrWorkMedium, n, nullptr, nullptr, [&](uint32_t i, uint32_t &p, bool(&)[1]) {
+    run_items<0, 8>(cnt + kCtrWorkMedium, n, nullptr, nullptr, [&](uint32_t i, uint32_t &p, bool(&)[1]) {
         {
             p = ps.q_medium[i];
             const uint4 ha = ps.hit_a[p];
