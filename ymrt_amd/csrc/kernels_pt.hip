@@ -706,19 +706,9 @@ __device__ __forceinline__ void compact_scatter_body(const PtParams &prm, const 
     const uint32_t tag = pass << kFlagBits;
     const uint32_t wave = threadIdx.x / kBlock;
     const uint64_t below = (1ull << lane_id()) - 1ull;
-    // this block's offsets: the counts of the blocks before it
-    static_for<0, kQ>([&](auto Q) {
-        uint32_t v = 0;
-        for (uint32_t j = threadIdx.x; j < b; j += kCompactThreads) v += ps.ccount[Q() * stride + j];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (lane_id() == 0) red[Q()][wave] = v;
-    });
-    __syncthreads();
-    if (threadIdx.x < (uint32_t)kQ) {
-        uint32_t v = 0;
-        for (uint32_t w = 0; w < kWaves; ++w) v += red[threadIdx.x][w];
-        base[threadIdx.x] = v;
-    }
+    // this block's offsets: the counts of the blocks before it, summed by
+    // k_compact_scan
+    if (threadIdx.x < (uint32_t)kQ) base[threadIdx.x] = ps.ccount[threadIdx.x * stride + b];
     __syncthreads();
     uint32_t off[kQ];
     static_for<0, kQ>([&](auto Q) { off[Q()] = base[Q()]; });
@@ -752,6 +742,46 @@ __device__ __forceinline__ void compact_scatter_body(const PtParams &prm, const 
     if (last == n && threadIdx.x < (uint32_t)kQ) cnt[ctr[threadIdx.x]] = off[threadIdx.x];
 }
 
+
+// The per-block counts of one slot (blockIdx.y; one block per queue) turned
+// into their exclusive prefix sums in place, so that each scatter block reads
+// its offsets instead of summing the counts of all blocks before it (a sum
+// whose total work grows with the square of the block count).
+constexpr uint32_t kScanThreads = 1024;
+template <int kMode>
+__global__ __launch_bounds__(kScanThreads) void k_compact_scan_m(const FrameSlot *__restrict__ fs)
+{
+    const FrameSlot &f = fs[blockIdx.y];
+    constexpr int kQ = Compact<kMode>::kQ;
+    constexpr uint32_t kWaves = kScanThreads / kBlock;
+    __shared__ uint32_t wsum[kWaves];
+    const uint32_t nb = (f.prm.chunk_paths + kCompactSpan - 1) / kCompactSpan;
+    const uint32_t stride = f.prm.ccount_stride;
+    const uint32_t per = (nb + kScanThreads - 1) / kScanThreads;
+    const uint32_t lo = min(threadIdx.x * per, nb), hi = min(lo + per, nb);
+    const uint32_t wave = threadIdx.x / kBlock;
+    {
+        const uint32_t q = blockIdx.x;  // one block per queue
+        if (q >= (uint32_t)kQ) return;
+        uint32_t *c = f.ps.ccount + (size_t)q * stride;
+        uint32_t own = 0;
+        for (uint32_t j = lo; j < hi; ++j) own += c[j];
+        uint32_t incl = own;
+        for (int off = 1; off < kBlock; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off);
+            if ((int)lane_id() >= off) incl += y;
+        }
+        if (lane_id() == kBlock - 1) wsum[wave] = incl;
+        __syncthreads();
+        uint32_t before = incl - own;
+        for (uint32_t w = 0; w < wave; ++w) before += wsum[w];
+        for (uint32_t j = lo; j < hi; ++j) {
+            const uint32_t v = c[j];
+            c[j] = before;
+            before += v;
+        }
+    }
+}
 
 // one row of the grid (blockIdx.y) per slot
 template <int kMode>
@@ -1875,15 +1905,18 @@ int launch_multi_medium(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v
 
 int launch_multi_compact(const FrameSlot *fs, uint32_t nfs, uint32_t blocks, uint32_t pass, int mode, hipStream_t s)
 {
-    const dim3 g(blocks ? blocks : 1, nfs), b(kCompactThreads);
+    const dim3 g(blocks ? blocks : 1, nfs), b(kCompactThreads), g1(4, nfs), b1(kScanThreads);
     if (mode == kCompactShade) {
         hipLaunchKernelGGL(k_compact_count_m<kCompactShade>, g, b, 0, s, fs, pass);
+        hipLaunchKernelGGL(k_compact_scan_m<kCompactShade>, g1, b1, 0, s, fs);
         hipLaunchKernelGGL(k_compact_scatter_m<kCompactShade>, g, b, 0, s, fs, pass);
     } else if (mode == kCompactMedium) {
         hipLaunchKernelGGL(k_compact_count_m<kCompactMedium>, g, b, 0, s, fs, pass);
+        hipLaunchKernelGGL(k_compact_scan_m<kCompactMedium>, g1, b1, 0, s, fs);
         hipLaunchKernelGGL(k_compact_scatter_m<kCompactMedium>, g, b, 0, s, fs, pass);
     } else {
         hipLaunchKernelGGL(k_compact_count_m<kCompactTransmit>, g, b, 0, s, fs, pass);
+        hipLaunchKernelGGL(k_compact_scan_m<kCompactTransmit>, g1, b1, 0, s, fs);
         hipLaunchKernelGGL(k_compact_scatter_m<kCompactTransmit>, g, b, 0, s, fs, pass);
     }
     YM_HIP(hipGetLastError());
