@@ -15,6 +15,9 @@
 // and are compacted per wave64 with a ballot + one atomic per wave.  Every
 // kernel fetches 64 items per wave from a device counter, so launches never
 // need the host to know queue lengths (no host round trip per stage).
+#include <mutex>
+#include <unordered_map>
+
 #include "dev_shade.hpp"
 #include "dev_traverse.hpp"
 #include "internal.hpp"
@@ -1852,6 +1855,34 @@ __global__ __launch_bounds__(kBlock) void k_active_m(const FrameSlot *__restrict
     if (threadIdx.x == 0) *out = v;
 }
 
+// A persistent pass kernel gets at most as many one-wave blocks as the GPU
+// holds at once (its occupancy times the CU count, queried once per
+// kernel): further waves could only start once the stage's work has run
+// out, and each would still take one fetch atomic on its counter.
+template <typename... Params, typename... Args>
+static void launch_persistent(void (*kernel)(Params...), uint32_t waves, hipStream_t s, Args... args)
+{
+    static std::mutex mu;
+    static std::unordered_map<const void *, uint32_t> resident;
+    uint32_t cap = 0;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = resident.find((const void *)kernel);
+        if (it == resident.end()) {
+            int blocks = 0, dev = 0, cus = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, kBlock, 0) == hipSuccess &&
+                hipGetDevice(&dev) == hipSuccess &&
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+                cap = (uint32_t)(blocks * cus);
+            resident.emplace((const void *)kernel, cap);
+        } else {
+            cap = it->second;
+        }
+    }
+    const uint32_t grid = cap > 0 && cap < waves ? cap : waves;
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, s, args...);
+}
+
 int launch_multi_active(const FrameSlot *fs, uint32_t nfs, uint32_t pass, uint32_t *out, hipStream_t s)
 {
     hipLaunchKernelGGL(k_active_m, dim3(1), dim3(kBlock), 0, s, fs, nfs, pass, out);
@@ -1861,7 +1892,7 @@ int launch_multi_active(const FrameSlot *fs, uint32_t nfs, uint32_t pass, uint32
 
 int launch_multi_raygen(const FrameSlot *fs, uint32_t nfs, uint32_t waves, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_raygen_m, dim3(waves), dim3(kBlock), 0, s, fs, nfs);
+    launch_persistent(k_raygen_m, waves, s, fs, nfs);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }
@@ -1869,8 +1900,7 @@ int launch_multi_raygen(const FrameSlot *fs, uint32_t nfs, uint32_t waves, hipSt
 template <bool kNearby, bool kMedia, bool kSphere>
 static void extend_m(const FrameSlot *fs, uint32_t nfs, uint32_t waves, uint32_t pass, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_extend_m<kNearby, kMedia, kSphere, kMaxStackDepth>), dim3(waves), dim3(kBlock), 0, s, fs,
-                       nfs, pass);
+    launch_persistent((k_extend_m<kNearby, kMedia, kSphere, kMaxStackDepth>), waves, s, fs, nfs, pass);
 }
 
 int launch_multi_extend(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v, uint32_t waves, uint32_t pass,
@@ -1894,11 +1924,11 @@ int launch_multi_extend(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v
 int launch_multi_medium(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v, uint32_t waves, uint32_t pass,
                         hipStream_t s)
 {
-    if (v.adj && v.het == 2) hipLaunchKernelGGL((k_medium_m<2, true>), dim3(waves), dim3(kBlock), 0, s, fs, nfs, pass);
-    else if (v.adj && v.het == 1) hipLaunchKernelGGL((k_medium_m<1, true>), dim3(waves), dim3(kBlock), 0, s, fs, nfs, pass);
-    else if (v.het == 2) hipLaunchKernelGGL(k_medium_m<2>, dim3(waves), dim3(kBlock), 0, s, fs, nfs, pass);
-    else if (v.het == 1) hipLaunchKernelGGL(k_medium_m<1>, dim3(waves), dim3(kBlock), 0, s, fs, nfs, pass);
-    else hipLaunchKernelGGL(k_medium_m<0>, dim3(waves), dim3(kBlock), 0, s, fs, nfs, pass);
+    if (v.adj && v.het == 2) launch_persistent((k_medium_m<2, true>), waves, s, fs, nfs, pass);
+    else if (v.adj && v.het == 1) launch_persistent((k_medium_m<1, true>), waves, s, fs, nfs, pass);
+    else if (v.het == 2) launch_persistent(k_medium_m<2>, waves, s, fs, nfs, pass);
+    else if (v.het == 1) launch_persistent(k_medium_m<1>, waves, s, fs, nfs, pass);
+    else launch_persistent(k_medium_m<0>, waves, s, fs, nfs, pass);
     YM_HIP(hipGetLastError());
     return YMRT_OK;
 }
@@ -1926,17 +1956,16 @@ int launch_multi_compact(const FrameSlot *fs, uint32_t nfs, uint32_t blocks, uin
 template <bool kGraph, bool kAdj>
 static void shade_m(const FrameSlot *fs, uint32_t nfs, uint32_t waves, uint32_t pass, hipStream_t s)
 {
-    const dim3 g(waves), b(kBlock);
-    hipLaunchKernelGGL((k_shade_m<false, kGraph, kMatGeneral, kAdj>), g, b, 0, s, fs, nfs, pass);
-    hipLaunchKernelGGL((k_shade_m<false, kGraph, kMatDiffuse, kAdj>), g, b, 0, s, fs, nfs, pass);
-    hipLaunchKernelGGL((k_shade_m<false, kGraph, kMatMetal, kAdj>), g, b, 0, s, fs, nfs, pass);
+    launch_persistent((k_shade_m<false, kGraph, kMatGeneral, kAdj>), waves, s, fs, nfs, pass);
+    launch_persistent((k_shade_m<false, kGraph, kMatDiffuse, kAdj>), waves, s, fs, nfs, pass);
+    launch_persistent((k_shade_m<false, kGraph, kMatMetal, kAdj>), waves, s, fs, nfs, pass);
 }
 
 int launch_multi_shade(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v, bool volume, uint32_t waves,
                        uint32_t pass, hipStream_t s)
 {
     // volume vertices carry no adjoint events of their own
-    if (volume) hipLaunchKernelGGL((k_shade_m<true, false>), dim3(waves), dim3(kBlock), 0, s, fs, nfs, pass);
+    if (volume) launch_persistent((k_shade_m<true, false>), waves, s, fs, nfs, pass);
     else if (v.graph) v.adj ? shade_m<true, true>(fs, nfs, waves, pass, s) : shade_m<true, false>(fs, nfs, waves, pass, s);
     else v.adj ? shade_m<false, true>(fs, nfs, waves, pass, s) : shade_m<false, false>(fs, nfs, waves, pass, s);
     YM_HIP(hipGetLastError());
@@ -1946,21 +1975,20 @@ int launch_multi_shade(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v,
 int launch_multi_shadow(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v, uint32_t waves, uint32_t pass,
                         hipStream_t s)
 {
-    const dim3 g(waves), b(kBlock);
     if (v.adj) {
         if (v.sphere) {
-            if (v.media) hipLaunchKernelGGL((k_shadow_m<true, true, kMaxStackDepth, true>), g, b, 0, s, fs, nfs, pass);
-            else hipLaunchKernelGGL((k_shadow_m<false, true, kMaxStackDepth, true>), g, b, 0, s, fs, nfs, pass);
+            if (v.media) launch_persistent((k_shadow_m<true, true, kMaxStackDepth, true>), waves, s, fs, nfs, pass);
+            else launch_persistent((k_shadow_m<false, true, kMaxStackDepth, true>), waves, s, fs, nfs, pass);
         } else {
-            if (v.media) hipLaunchKernelGGL((k_shadow_m<true, false, kMaxStackDepth, true>), g, b, 0, s, fs, nfs, pass);
-            else hipLaunchKernelGGL((k_shadow_m<false, false, kMaxStackDepth, true>), g, b, 0, s, fs, nfs, pass);
+            if (v.media) launch_persistent((k_shadow_m<true, false, kMaxStackDepth, true>), waves, s, fs, nfs, pass);
+            else launch_persistent((k_shadow_m<false, false, kMaxStackDepth, true>), waves, s, fs, nfs, pass);
         }
     } else if (v.sphere) {
-        if (v.media) hipLaunchKernelGGL((k_shadow_m<true, true, kMaxStackDepth>), g, b, 0, s, fs, nfs, pass);
-        else hipLaunchKernelGGL((k_shadow_m<false, true, kMaxStackDepth>), g, b, 0, s, fs, nfs, pass);
+        if (v.media) launch_persistent((k_shadow_m<true, true, kMaxStackDepth>), waves, s, fs, nfs, pass);
+        else launch_persistent((k_shadow_m<false, true, kMaxStackDepth>), waves, s, fs, nfs, pass);
     } else {
-        if (v.media) hipLaunchKernelGGL((k_shadow_m<true, false, kMaxStackDepth>), g, b, 0, s, fs, nfs, pass);
-        else hipLaunchKernelGGL((k_shadow_m<false, false, kMaxStackDepth>), g, b, 0, s, fs, nfs, pass);
+        if (v.media) launch_persistent((k_shadow_m<true, false, kMaxStackDepth>), waves, s, fs, nfs, pass);
+        else launch_persistent((k_shadow_m<false, false, kMaxStackDepth>), waves, s, fs, nfs, pass);
     }
     YM_HIP(hipGetLastError());
     return YMRT_OK;
@@ -1969,25 +1997,24 @@ int launch_multi_shadow(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v
 int launch_multi_transmit(const FrameSlot *fs, uint32_t nfs, const MultiVariant &v, uint32_t waves, uint32_t pass,
                           hipStream_t s)
 {
-    const dim3 g(waves), b(kBlock);
     if (v.adj) {
         if (v.sphere) {
-            if (v.het == 2) hipLaunchKernelGGL((k_transmit_m<true, kMaxStackDepth, 2, true>), g, b, 0, s, fs, nfs, pass);
-            else if (v.het == 1) hipLaunchKernelGGL((k_transmit_m<true, kMaxStackDepth, 1, true>), g, b, 0, s, fs, nfs, pass);
-            else hipLaunchKernelGGL((k_transmit_m<true, kMaxStackDepth, 0, true>), g, b, 0, s, fs, nfs, pass);
+            if (v.het == 2) launch_persistent((k_transmit_m<true, kMaxStackDepth, 2, true>), waves, s, fs, nfs, pass);
+            else if (v.het == 1) launch_persistent((k_transmit_m<true, kMaxStackDepth, 1, true>), waves, s, fs, nfs, pass);
+            else launch_persistent((k_transmit_m<true, kMaxStackDepth, 0, true>), waves, s, fs, nfs, pass);
         } else {
-            if (v.het == 2) hipLaunchKernelGGL((k_transmit_m<false, kMaxStackDepth, 2, true>), g, b, 0, s, fs, nfs, pass);
-            else if (v.het == 1) hipLaunchKernelGGL((k_transmit_m<false, kMaxStackDepth, 1, true>), g, b, 0, s, fs, nfs, pass);
-            else hipLaunchKernelGGL((k_transmit_m<false, kMaxStackDepth, 0, true>), g, b, 0, s, fs, nfs, pass);
+            if (v.het == 2) launch_persistent((k_transmit_m<false, kMaxStackDepth, 2, true>), waves, s, fs, nfs, pass);
+            else if (v.het == 1) launch_persistent((k_transmit_m<false, kMaxStackDepth, 1, true>), waves, s, fs, nfs, pass);
+            else launch_persistent((k_transmit_m<false, kMaxStackDepth, 0, true>), waves, s, fs, nfs, pass);
         }
     } else if (v.sphere) {
-        if (v.het == 2) hipLaunchKernelGGL((k_transmit_m<true, kMaxStackDepth, 2>), g, b, 0, s, fs, nfs, pass);
-        else if (v.het == 1) hipLaunchKernelGGL((k_transmit_m<true, kMaxStackDepth, 1>), g, b, 0, s, fs, nfs, pass);
-        else hipLaunchKernelGGL((k_transmit_m<true, kMaxStackDepth, 0>), g, b, 0, s, fs, nfs, pass);
+        if (v.het == 2) launch_persistent((k_transmit_m<true, kMaxStackDepth, 2>), waves, s, fs, nfs, pass);
+        else if (v.het == 1) launch_persistent((k_transmit_m<true, kMaxStackDepth, 1>), waves, s, fs, nfs, pass);
+        else launch_persistent((k_transmit_m<true, kMaxStackDepth, 0>), waves, s, fs, nfs, pass);
     } else {
-        if (v.het == 2) hipLaunchKernelGGL((k_transmit_m<false, kMaxStackDepth, 2>), g, b, 0, s, fs, nfs, pass);
-        else if (v.het == 1) hipLaunchKernelGGL((k_transmit_m<false, kMaxStackDepth, 1>), g, b, 0, s, fs, nfs, pass);
-        else hipLaunchKernelGGL((k_transmit_m<false, kMaxStackDepth, 0>), g, b, 0, s, fs, nfs, pass);
+        if (v.het == 2) launch_persistent((k_transmit_m<false, kMaxStackDepth, 2>), waves, s, fs, nfs, pass);
+        else if (v.het == 1) launch_persistent((k_transmit_m<false, kMaxStackDepth, 1>), waves, s, fs, nfs, pass);
+        else launch_persistent((k_transmit_m<false, kMaxStackDepth, 0>), waves, s, fs, nfs, pass);
     }
     YM_HIP(hipGetLastError());
     return YMRT_OK;
