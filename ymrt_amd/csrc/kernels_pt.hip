@@ -299,6 +299,10 @@ __device__ __forceinline__ void for_slots(const FrameSlot *__restrict__ fs, uint
     uint32_t total = 0;
     for (uint32_t s = 0; s < nfs; ++s) total += count(fs[s]);
     if (total == 0) return;
+    // waves beyond one per fetch of kSub * 64 items would find the stage
+    // dry: they leave without touching its work counter (whose atomics
+    // queue up behind each other), and the rest take every item
+    if ((uint64_t)blockIdx.x * (uint64_t)(kBlock * kSub) >= (uint64_t)total) return;
     const uint32_t target = (uint32_t)(((uint64_t)blockIdx.x * total) / gridDim.x);
     uint32_t home = 0;
     for (uint32_t s = 0, acc = 0; s < nfs; ++s) {
