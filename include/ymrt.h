@@ -257,7 +257,9 @@ typedef struct ymrt_render_settings {
     uint32_t persistent_waves;       /* waves of each persistent pass kernel,
                                         at most 5120 (the traversal spill
                                         area); 0: one per 256 paths of the
-                                        chunk, within [1024, 5120].  Like
+                                        chunk, within [1024, 5120].  Each
+                                        kernel launches at most the waves the
+                                        GPU holds of it at once.  Like
                                         path_capacity it changes no pixel.    */
 } ymrt_render_settings;
 
